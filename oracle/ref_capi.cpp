@@ -1,0 +1,115 @@
+// ref_capi.cpp -- extern "C" wrappers around the reference's own scalar path, compiled from the reference
+// sources where they lie (see oracle/Makefile).  TEST INFRASTRUCTURE ONLY: used by tests/ to pin the oracle
+// restatement and by bench.py's cpu_baseline leg.  Output goes to oracle/_ref/ only.
+#include "core/point_types.h"
+#include "core/rvv_common.h"
+#include "search/spatial_hashing.h"
+#include "search/radius_search.h"
+#include "filters/voxel_grid.h"
+#include "filters/statistical_outlier_removal.h"
+#include "features/normal_estimation.h"
+#include "segmentation/ransac_plane.h"
+
+#include <cstdint>
+#include <vector>
+
+using namespace rvpoint;
+
+static PointCloudSoA soa(const float *x, const float *y, const float *z, size_t n) {
+  PointCloudSoA c;
+  c.x = const_cast<float *>(x); c.y = const_cast<float *>(y); c.z = const_cast<float *>(z); c.n = n;
+  return c;
+}
+
+extern "C" {
+
+void ref_get_dist_sq(const float *x, const float *y, const float *z, float qx, float qy, float qz,
+                     float *out, size_t n) {
+  get_dist_sq_rvv(x, y, z, qx, qy, qz, out, n);
+}
+
+// SpatialHash::setInputCloud + build, then radiusSearch per query (spatial_hashing.cpp:76-223)
+void ref_hash_radius(const float *x, const float *y, const float *z, size_t n, float cell,
+                     const float *qx, const float *qy, const float *qz, size_t nq, float radius, int max_nn,
+                     size_t stride, int32_t *idx_out, float *d2_out, int32_t *counts) {
+  SpatialHash h;
+  h.setInputCloud(soa(x, y, z, n), cell);
+  h.build();
+  std::vector<int> idx;
+  std::vector<float> d;
+  for (size_t q = 0; q < nq; ++q) {
+    PointXYZ p{qx[q], qy[q], qz[q]};
+    size_t m = h.radiusSearch(p, radius, idx, d, max_nn);
+    counts[q] = (int32_t)m;
+    for (size_t j = 0; j < m && j < stride; ++j) {
+      if (idx_out) idx_out[q * stride + j] = idx[j];
+      if (d2_out) d2_out[q * stride + j] = d[j];
+    }
+  }
+}
+
+size_t ref_radius_search_sc(const float *aos, size_t n, float qx, float qy, float qz, float r,
+                            int32_t *idx, float *d, int max_nn) {
+  return radius_search_sc(reinterpret_cast<const PointXYZ *>(aos), n, PointXYZ{qx, qy, qz}, r, idx, d, max_nn);
+}
+
+size_t ref_radius_search_rvv(const float *x, const float *y, const float *z, size_t n, float qx, float qy,
+                             float qz, float r, int32_t *idx, float *d, int max_nn) {
+  return radius_search_rvv(soa(x, y, z, n), PointXYZ{qx, qy, qz}, r, idx, d, max_nn);
+}
+
+size_t ref_sor_sc(const float *aos, size_t n, float *out, int k, float alpha) {
+  return sor_sc(reinterpret_cast<const PointXYZ *>(aos), n, reinterpret_cast<PointXYZ *>(out), k, alpha);
+}
+
+size_t ref_sor_rvv(const float *x, const float *y, const float *z, size_t n, float *out, int k, float alpha) {
+  return sor_rvv(soa(x, y, z, n), reinterpret_cast<PointXYZ *>(out), k, alpha);
+}
+
+size_t ref_sor_spatial_hash(const float *x, const float *y, const float *z, size_t n, float cell, float *out,
+                            int k, float alpha, float search_radius) {
+  SpatialHash h;
+  PointCloudSoA c = soa(x, y, z, n);
+  h.setInputCloud(c, cell);
+  h.build();
+  return sor_spatial_hash(c, h, reinterpret_cast<PointXYZ *>(out), k, alpha, search_radius);
+}
+
+size_t ref_voxel_sc(const float *aos, size_t n, float *out, float leaf) {
+  return voxel_grid_downsamp_sc(reinterpret_cast<const PointXYZ *>(aos), n, reinterpret_cast<PointXYZ *>(out), leaf);
+}
+
+size_t ref_voxel_v2(const float *x, const float *y, const float *z, size_t n, float *out, float leaf) {
+  return voxel_grid_downsamp_rvv_v2(soa(x, y, z, n), reinterpret_cast<PointXYZ *>(out), leaf);
+}
+
+void ref_normal_sc(const float *aos, size_t n, float *nx, float *ny, float *nz, int k, float radius,
+                   float vpx, float vpy, float vpz, int iters) {
+  normal_estimation_sc(reinterpret_cast<const PointXYZ *>(aos), n, nx, ny, nz, k, radius, vpx, vpy, vpz, iters);
+}
+
+void ref_normal_hash(const float *x, const float *y, const float *z, size_t n, float cell, float *nx, float *ny,
+                     float *nz, int k, float radius, float vpx, float vpy, float vpz, int iters) {
+  SpatialHash h;
+  PointCloudSoA c = soa(x, y, z, n);
+  h.setInputCloud(c, cell);
+  h.build();
+  normal_estimation_rvv(c, h, nx, ny, nz, k, radius, vpx, vpy, vpz, iters);
+}
+
+int ref_ransac_sc(const float *aos, size_t n, float thr, int max_iters, float *model, float collinear, float prob) {
+  return ransac_plane_sc(reinterpret_cast<const PointXYZ *>(aos), n, thr, max_iters, model, collinear, prob);
+}
+
+int ref_ransac_rvv(const float *x, const float *y, const float *z, size_t n, float thr, int max_iters,
+                   float *model, float collinear, float prob) {
+  return ransac_plane_rvv(soa(x, y, z, n), thr, max_iters, model, collinear, prob);
+}
+
+void ref_extract(const float *x, const float *y, const float *z, size_t n, const float *model, float thr,
+                 float *inl, float *outl, size_t *ni, size_t *no) {
+  extract_plane_inliers_outliers_rvv(soa(x, y, z, n), model, thr, reinterpret_cast<PointXYZ *>(inl),
+                                     reinterpret_cast<PointXYZ *>(outl), *ni, *no);
+}
+
+}  // extern "C"
