@@ -1,0 +1,152 @@
+/*
+ * rvp.h -- C ABI of the MI355X (gfx950) neighbour-search path: librvp_hip.so.
+ *
+ * Every array argument is a DEVICE pointer (HBM) unless its comment says "host"; scalar results that
+ * the caller reads (counts, models) are host pointers and the call returns after they are valid.
+ * Calls are ordered on the given HIP stream (NULL = the null stream).  Return value: RVP_OK (0) or a
+ * negative RVP_E_* status; rvp_last_error() gives a message.  Nothing throws across this boundary.
+ *
+ * Points are SoA float x[n], y[n], z[n] (the reference's PointCloudSoA, src/core/point_types.h:24-29);
+ * AoS outputs are float[3*n] laid out as the reference's PointXYZ (src/core/point_types.h:12-14).
+ *
+ * Each entry point names the reference function (file:line) whose results it reproduces.
+ */
+#ifndef RVP_H
+#define RVP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RVP_OK = 0,
+  RVP_E_ARG = -1,        /* bad argument (NULL pointer, negative size, k out of range) */
+  RVP_E_OOM = -2,        /* device allocation failed */
+  RVP_E_DEVICE = -3,     /* HIP runtime / kernel error */
+  RVP_E_STATE = -4,      /* handle used before build */
+  RVP_E_UNSUPPORTED = -5 /* a size beyond what the kernels support */
+};
+
+typedef void *rvp_stream_t; /* hipStream_t */
+typedef struct rvp_hash rvp_hash;
+
+/* ---- runtime / memory helpers (so hosts need no HIP headers) ------------------------------------- */
+const char *rvp_last_error(void);
+const char *rvp_version(void);
+int rvp_device_count(int *count);
+int rvp_set_device(int device);
+int rvp_malloc(void **ptr, size_t bytes);
+int rvp_free(void *ptr);
+int rvp_memcpy_h2d(void *dst, const void *src, size_t bytes, rvp_stream_t s);
+int rvp_memcpy_d2h(void *dst, const void *src, size_t bytes, rvp_stream_t s);
+int rvp_memcpy_d2d(void *dst, const void *src, size_t bytes, rvp_stream_t s);
+int rvp_memset(void *dst, int value, size_t bytes, rvp_stream_t s);
+int rvp_stream_create(rvp_stream_t *s);
+int rvp_stream_destroy(rvp_stream_t s);
+int rvp_stream_sync(rvp_stream_t s);
+int rvp_device_sync(void);
+/* event timing on a stream: start/stop record, elapsed in ms */
+int rvp_event_create(void **ev);
+int rvp_event_destroy(void *ev);
+int rvp_event_record(void *ev, rvp_stream_t s);
+int rvp_event_elapsed(void *start, void *stop, float *ms);
+
+/* ---- H1/H2: SpatialHash::setInputCloud + build (src/search/spatial_hashing.cpp:76-163) ------------ */
+int rvp_hash_create(rvp_hash **h);
+int rvp_hash_destroy(rvp_hash *h);
+/* Keeps a view of x/y/z (caller keeps them alive, as the reference does, spatial_hashing.cpp:77) and
+ * builds the device grid: bbox with the 0.01*cell margin, cell keys, stable radix sort, cell table,
+ * cell-ordered SoA copy. */
+int rvp_hash_build(rvp_hash *h, const float *x, const float *y, const float *z, size_t n, float cell_size,
+                   rvp_stream_t s);
+/* host out: {minx,miny,minz,maxx,maxy,maxz} after the margin (spatial_hashing.cpp:141-146) */
+int rvp_hash_bbox(const rvp_hash *h, float *bbox6);
+/* host out: n points, occupied cells, grid dims (gx,gy,gz), internal cell size actually used */
+int rvp_hash_info(const rvp_hash *h, size_t *n, size_t *occupied_cells, int *dims3, float *grid_cell);
+/* device out: per point (original order) int32 cell coordinates ix,iy,iz (spatial_hashing.h:47-52) and the
+ * reference's int64 key ix + iy*73856093 + iz*19349663 (spatial_hashing.h:43-45); either may be NULL. */
+int rvp_hash_cell_coords(const rvp_hash *h, int32_t *ixyz, int64_t *keys, rvp_stream_t s);
+
+/* ---- H3: SpatialHash::radiusSearch, batched (spatial_hashing.cpp:165-223) -------------------------
+ * Query i's neighbours within `radius`, ascending (d2, index), truncated to max_nn.
+ * max_nn > 0: rows of stride max_nn: idx_out[i*max_nn + j], d2_out likewise (either may be NULL),
+ *             counts[i] = min(total_i, max_nn); total[i] (may be NULL) = untruncated count.
+ * max_nn <= 0 (unlimited, the reference's default): use rvp_radius_count then rvp_radius_fill. */
+int rvp_radius_search(const rvp_hash *h, const float *qx, const float *qy, const float *qz, size_t nq,
+                      float radius, int max_nn, int32_t *idx_out, float *d2_out, int32_t *counts,
+                      int32_t *total, rvp_stream_t s);
+/* counts[i] = number of neighbours of query i within radius (device out) */
+int rvp_radius_count(const rvp_hash *h, const float *qx, const float *qy, const float *qz, size_t nq,
+                     float radius, int32_t *counts, rvp_stream_t s);
+/* CSR fill: offsets[nq+1] (device, exclusive scan of counts, int64); writes each list sorted by (d2, idx). */
+int rvp_radius_fill(const rvp_hash *h, const float *qx, const float *qy, const float *qz, size_t nq,
+                    float radius, const int64_t *offsets, int32_t *idx_out, float *d2_out, rvp_stream_t s);
+/* exclusive scan helper: offsets[0..n] of counts[0..n) (device), host total */
+int rvp_exclusive_scan(const int32_t *counts, size_t n, int64_t *offsets, int64_t *total_host, rvp_stream_t s);
+
+/* ---- H4: radius_search_sc / radius_search_rvv, batched (src/search/radius_search.cpp:12-59) -------
+ * Brute force: the first max_nn hits in index order; counts[i] = min(total, max_nn). */
+int rvp_radius_bruteforce(const float *x, const float *y, const float *z, size_t n, const float *qx,
+                          const float *qy, const float *qz, size_t nq, float radius, int max_nn,
+                          int32_t *idx_out, float *d2_out, int32_t *counts, rvp_stream_t s);
+
+/* ---- kNN over the hash: k nearest by (d2, idx), rows of stride k; rows shorter than k when n < k are
+ * padded with idx -1, d2 +inf.  Query-minus-point order as sor_sc (statistical_outlier_removal.cpp:27-34). */
+int rvp_knn(const rvp_hash *h, const float *qx, const float *qy, const float *qz, size_t nq, int k,
+            int32_t *idx_out, float *d2_out, rvp_stream_t s);
+
+/* ---- S1: sor_sc / sor_rvv (src/filters/statistical_outlier_removal.cpp:19-63, 68-190) --------------
+ * Exact k-NN SOR.  out_aos (device, >= 3n floats) gets the kept points in input order; *n_out (host).
+ * keep (device uint8[n]) and means (device float[n]) may be NULL. */
+int rvp_sor_exact(const float *x, const float *y, const float *z, size_t n, int k, float alpha,
+                  float *out_aos, size_t *n_out, uint8_t *keep, float *means, rvp_stream_t s);
+
+/* ---- S2: sor_spatial_hash + filter_by_mean_dists (statistical_outlier_removal.cpp:246-270, 193-218) --
+ * h must be built on the same x/y/z (it is the caller's SpatialHash). */
+int rvp_sor_radius(const rvp_hash *h, int k, float alpha, float search_radius, float *out_aos,
+                   size_t *n_out, uint8_t *keep, float *means, rvp_stream_t s);
+
+/* ---- N1: normal_estimation_sc semantics (src/features/normal_estimation.cpp:72-112) ---------------
+ * neighbourhood = k+1 nearest incl. self by (d2, idx); FP32 compensated covariance re-centred at the
+ * query, closed-form FP32 eigen solve with FP64 fallback for near-degenerate spectra; flip toward vp
+ * (normal_estimation.cpp:55-67).  nx/ny/nz device float[n]. */
+int rvp_normals_knn(const float *x, const float *y, const float *z, size_t n, int k, float vpx, float vpy,
+                    float vpz, float *nx, float *ny, float *nz, rvp_stream_t s);
+
+/* ---- N2: normal_estimation_rvv(in, SpatialHash&, ...) (normal_estimation.cpp:189-214) --------------
+ * neighbourhood = H3 list within radius (max_nn > 0 caps it, 0 = unlimited); fewer than 3 -> zero normal.
+ * Queries are the hash's own points. */
+int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, float vpy, float vpz,
+                       float *nx, float *ny, float *nz, rvp_stream_t s);
+/* fraction of points of the last normals call on this thread that took the FP64 eigen fallback */
+int rvp_normals_fallback_count(size_t *count);
+
+/* ---- V1: voxel_grid_downsamp_sc (src/filters/voxel_grid_downsamp.cpp:183-213) ---------------------
+ * out_aos (device, >= 3n floats): centroids in lexicographic (vx,vy,vz) order; *n_out host. */
+int rvp_voxel(const float *x, const float *y, const float *z, size_t n, float leaf, float *out_aos,
+              size_t *n_out, rvp_stream_t s);
+
+/* ---- R1: ransac_plane_sc / ransac_plane_rvv (src/segmentation/ransac_plane.cpp:55-111) -------------
+ * Hypotheses from the glibc TYPE_3 rand() stream after srand(0) (3 draws per iteration), plane
+ * coefficients as compute_plane_coefficients (:18-50), inlier counts on the GPU, adaptive stop replayed
+ * on the host.  model (host float[4]); *inliers (host).  *iterations (host, may be NULL). */
+int rvp_ransac_plane(const float *x, const float *y, const float *z, size_t n, float dist_thresh,
+                     int max_iters, float collinear_thresh, float probability, float *model, int *inliers,
+                     int *iterations, rvp_stream_t s);
+/* Counting core: counts[h] = #{i : fabsf(((a x + b y) + c z) + d) <= thr} for H models (device float[4H]). */
+int rvp_plane_counts(const float *x, const float *y, const float *z, size_t n, const float *models, int H,
+                     float dist_thresh, int32_t *counts, rvp_stream_t s);
+
+/* ---- X1: extract_plane_inliers_outliers_rvv (ransac_plane.cpp:374-382, scalar order) -------------
+ * Stable split into inliers / outliers (AoS, device); counts host.  Either output may be NULL. */
+int rvp_plane_extract(const float *x, const float *y, const float *z, size_t n, const float *model_host,
+                      float dist_thresh, float *inliers_aos, size_t *n_in, float *outliers_aos,
+                      size_t *n_out, rvp_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,111 @@
+"""ctypes binding of librvp_hip.so (the C ABI declared in include/rvp.h).
+
+The library is loaded from rvpoint_amd/lib/ (built in-tree by rvpoint_amd.build).  There is no CPU
+fallback: when the library or a GPU is missing, every call raises RvpError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "librvp_hip.so")
+
+RVP_OK, RVP_E_ARG, RVP_E_OOM, RVP_E_DEVICE, RVP_E_STATE, RVP_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+
+
+class RvpError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"rvp status {status}: {msg}")
+        self.status = status
+
+
+vp = C.c_void_p
+sz = C.c_size_t
+fl = C.c_float
+i32 = C.c_int
+pvp = C.POINTER(C.c_void_p)
+psz = C.POINTER(C.c_size_t)
+pi32 = C.POINTER(C.c_int)
+pfl = C.POINTER(C.c_float)
+pi64 = C.POINTER(C.c_int64)
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+_SIGS = {
+    "rvp_last_error": [],
+    "rvp_version": [],
+    "rvp_device_count": [pi32],
+    "rvp_set_device": [i32],
+    "rvp_malloc": [pvp, sz],
+    "rvp_free": [vp],
+    "rvp_memcpy_h2d": [vp, vp, sz, vp],
+    "rvp_memcpy_d2h": [vp, vp, sz, vp],
+    "rvp_memcpy_d2d": [vp, vp, sz, vp],
+    "rvp_memset": [vp, i32, sz, vp],
+    "rvp_stream_create": [pvp],
+    "rvp_stream_destroy": [vp],
+    "rvp_stream_sync": [vp],
+    "rvp_device_sync": [],
+    "rvp_event_create": [pvp],
+    "rvp_event_destroy": [vp],
+    "rvp_event_record": [vp, vp],
+    "rvp_event_elapsed": [vp, vp, pfl],
+    "rvp_hash_create": [pvp],
+    "rvp_hash_destroy": [vp],
+    "rvp_hash_build": [vp, vp, vp, vp, sz, fl, vp],
+    "rvp_hash_bbox": [vp, pfl],
+    "rvp_hash_info": [vp, psz, psz, pi32, pfl],
+    "rvp_hash_cell_coords": [vp, vp, vp, vp],
+    "rvp_radius_search": [vp, vp, vp, vp, sz, fl, i32, vp, vp, vp, vp, vp],
+    "rvp_radius_count": [vp, vp, vp, vp, sz, fl, vp, vp],
+    "rvp_radius_fill": [vp, vp, vp, vp, sz, fl, vp, vp, vp, vp],
+    "rvp_exclusive_scan": [vp, sz, vp, pi64, vp],
+    "rvp_radius_bruteforce": [vp, vp, vp, sz, vp, vp, vp, sz, fl, i32, vp, vp, vp, vp],
+    "rvp_knn": [vp, vp, vp, vp, sz, i32, vp, vp, vp],
+    "rvp_sor_exact": [vp, vp, vp, sz, i32, fl, vp, psz, vp, vp, vp],
+    "rvp_sor_radius": [vp, i32, fl, fl, vp, psz, vp, vp, vp],
+    "rvp_normals_knn": [vp, vp, vp, sz, i32, fl, fl, fl, vp, vp, vp, vp],
+    "rvp_normals_radius": [vp, fl, i32, fl, fl, fl, vp, vp, vp, vp],
+    "rvp_normals_fallback_count": [psz],
+    "rvp_voxel": [vp, vp, vp, sz, fl, vp, psz, vp],
+    "rvp_ransac_plane": [vp, vp, vp, sz, fl, i32, fl, fl, pfl, pi32, pi32, vp],
+    "rvp_plane_counts": [vp, vp, vp, sz, vp, i32, fl, vp, vp],
+    "rvp_plane_extract": [vp, vp, vp, sz, pfl, fl, vp, psz, vp, psz, vp],
+}
+_RESTYPE = {"rvp_last_error": C.c_char_p, "rvp_version": C.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """Load librvp_hip.so once and declare the C ABI."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RvpError(RVP_E_STATE, f"{LIB_PATH} not built (python -m rvpoint_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def call(name, *args):
+    st = getattr(lib(), name)(*args)
+    if st != RVP_OK:
+        raise RvpError(st, lib().rvp_last_error().decode())
+    return st
+
+
+def gpu_available() -> bool:
+    try:
+        n = C.c_int(0)
+        return lib().rvp_device_count(C.byref(n)) == RVP_OK and n.value > 0
+    except (RvpError, OSError):
+        return False

@@ -1,0 +1,356 @@
+"""Host-side mirror of the reference's point-cloud API (namespace rvpoint, the "rvv_pcl.h" signatures),
+backed by the MI355X kernels through the C ABI.
+
+Names, argument meaning and result conventions follow the reference:
+  radius_search_sc / radius_search_rvv        src/search/radius_search.h:11-20
+  SpatialHash                                  src/search/spatial_hashing.h:12-53
+  sor_sc / sor_rvv / sor_spatial_hash          src/filters/statistical_outlier_removal.h:18-42
+  voxel_grid_downsamp_sc / _rvv / _rvv_v2      src/filters/voxel_grid.h:14-34
+  normal_estimation_sc / normal_estimation_rvv src/features/normal_estimation.h:15-49
+  ransac_plane_sc / ransac_plane_rvv, extract_plane_*   src/segmentation/ransac_plane.h:11-45
+Host arrays are numpy; AoS clouds are float32 (n, 3) arrays (PointXYZ), SoA clouds are PointCloudSoA.
+Every call stages its inputs into HBM, runs on the GPU and copies results back; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import call
+from .device import DeviceArray, DeviceCloud
+
+
+class PointCloudSoA:
+    """Non-owning SoA view in the reference (src/core/point_types.h:24-29); here it holds numpy arrays."""
+
+    def __init__(self, x, y, z):
+        self.x = np.ascontiguousarray(x, np.float32)
+        self.y = np.ascontiguousarray(y, np.float32)
+        self.z = np.ascontiguousarray(z, np.float32)
+        self.n = len(self.x)
+
+    @classmethod
+    def from_aos(cls, aos):
+        aos = np.asarray(aos, np.float32).reshape(-1, 3)
+        return cls(aos[:, 0], aos[:, 1], aos[:, 2])
+
+    def aos(self):
+        return np.stack([self.x, self.y, self.z], 1)
+
+
+def _soa(cloud):
+    if isinstance(cloud, PointCloudSoA):
+        return cloud
+    return PointCloudSoA.from_aos(cloud)
+
+
+def _q(query):
+    q = np.asarray(query, np.float32).reshape(-1, 3)
+    return [np.ascontiguousarray(q[:, i]) for i in range(3)]
+
+
+# ----------------------------------------------------------------------------------------------------
+# H4: brute-force radius search
+# ----------------------------------------------------------------------------------------------------
+def radius_search_batch_bruteforce(cloud, queries, radius, max_nn):
+    """First max_nn hits of every query in index order; returns (idx [nq, max_nn], d2, counts)."""
+    c = _soa(cloud)
+    qx, qy, qz = _q(queries)
+    nq = len(qx)
+    dc = DeviceCloud.from_soa(c.x, c.y, c.z)
+    dq = DeviceCloud.from_soa(qx, qy, qz)
+    w = max(max_nn, 1)
+    idx = DeviceArray((nq, w), np.int32)
+    d2 = DeviceArray((nq, w), np.float32)
+    cnt = DeviceArray(nq, np.int32)
+    call("rvp_radius_bruteforce", *dc.ptrs(), c.n, *dq.ptrs(), nq, float(radius), int(max_nn), idx.ptr, d2.ptr,
+         cnt.ptr, None)
+    return idx.numpy(), d2.numpy(), cnt.numpy()
+
+
+def radius_search_sc(cloud_aos, query, radius, max_nn):
+    """radius_search_sc (src/search/radius_search.cpp:12-33): (indices, dists) of the first max_nn hits."""
+    idx, d2, cnt = radius_search_batch_bruteforce(cloud_aos, [query], radius, max_nn)
+    m = int(cnt[0])
+    return idx[0, :m].copy(), d2[0, :m].copy()
+
+
+def radius_search_rvv(cloud_soa, query, radius, max_nn):
+    """radius_search_rvv (src/search/radius_search.cpp:38-59); same result as radius_search_sc."""
+    if _soa(cloud_soa).n == 0:
+        return np.zeros(0, np.int32), np.zeros(0, np.float32)
+    return radius_search_sc(cloud_soa, query, radius, max_nn)
+
+
+# ----------------------------------------------------------------------------------------------------
+# H1-H3: SpatialHash
+# ----------------------------------------------------------------------------------------------------
+class SpatialHash:
+    """SpatialHash (src/search/spatial_hashing.h:12-53) with a device-resident grid.
+
+    setInputCloud + build upload the cloud once; radiusSearch answers one query as the reference does,
+    radius_search_batch answers many in one launch.  setHashPrimes is accepted for API compatibility; the
+    device grid does not hash, so results never depended on the primes."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        call("rvp_hash_create", C.byref(h))
+        self.handle = h.value
+        self.cloud = None
+        self.dev = None
+        self.cell_size = 0.0
+        self.primes = (73856093, 19349663)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                from ._lib import lib
+                lib().rvp_hash_destroy(self.handle)
+        except Exception:
+            pass
+
+    def setInputCloud(self, cloud, cell_size):
+        self.cloud = _soa(cloud)
+        self.cell_size = float(cell_size)
+
+    def setHashPrimes(self, p1, p2):
+        self.primes = (int(p1), int(p2))
+
+    def build(self):
+        c = self.cloud
+        self.dev = DeviceCloud.from_soa(c.x, c.y, c.z)
+        call("rvp_hash_build", self.handle, *self.dev.ptrs(), c.n, self.cell_size, None)
+
+    @classmethod
+    def from_device(cls, dev: DeviceCloud, cell_size, stream=None):
+        """Build on a cloud already resident in HBM."""
+        h = cls()
+        h.dev = dev
+        h.cell_size = float(cell_size)
+        call("rvp_hash_build", h.handle, *dev.ptrs(), dev.n, h.cell_size, stream)
+        return h
+
+    # -- introspection
+    def bbox(self):
+        out = (C.c_float * 6)()
+        call("rvp_hash_bbox", self.handle, out)
+        return np.array(out, np.float32)
+
+    def info(self):
+        n, occ = C.c_size_t(), C.c_size_t()
+        dims = (C.c_int * 3)()
+        cell = C.c_float()
+        call("rvp_hash_info", self.handle, C.byref(n), C.byref(occ), dims, C.byref(cell))
+        return {"n": n.value, "occupied": occ.value, "dims": tuple(dims), "grid_cell": cell.value}
+
+    def cell_coords(self):
+        n = self.dev.n
+        ixyz = DeviceArray((n, 3), np.int32)
+        keys = DeviceArray(n, np.int64)
+        call("rvp_hash_cell_coords", self.handle, ixyz.ptr, keys.ptr, None)
+        return ixyz.numpy(), keys.numpy()
+
+    # -- queries
+    def radius_search_batch(self, queries, radius, max_nn=0, stream=None):
+        """Batched radiusSearch: list of (indices, dists) per query, each sorted by (d2, idx), truncated to
+        max_nn when max_nn > 0.  queries=None queries the cloud's own points."""
+        if queries is None:
+            nq, qp = self.dev.n, (None, None, None)
+            dq = None
+        else:
+            qx, qy, qz = _q(queries)
+            nq = len(qx)
+            dq = DeviceCloud.from_soa(qx, qy, qz)
+            qp = dq.ptrs()
+        if nq == 0:
+            return []
+        if max_nn > 0:
+            idx = DeviceArray((nq, max_nn), np.int32)
+            d2 = DeviceArray((nq, max_nn), np.float32)
+            cnt = DeviceArray(nq, np.int32)
+            call("rvp_radius_search", self.handle, *qp, nq, float(radius), int(max_nn), idx.ptr, d2.ptr, cnt.ptr,
+                 None, stream)
+            I, D, N = idx.numpy(), d2.numpy(), cnt.numpy()
+            return [(I[i, :N[i]].copy(), D[i, :N[i]].copy()) for i in range(nq)]
+        cnt = DeviceArray(nq, np.int32)
+        call("rvp_radius_count", self.handle, *qp, nq, float(radius), cnt.ptr, stream)
+        off = DeviceArray(nq + 1, np.int64)
+        tot = C.c_int64()
+        call("rvp_exclusive_scan", cnt.ptr, nq, off.ptr, C.byref(tot), stream)
+        idx = DeviceArray(max(tot.value, 1), np.int32)
+        d2 = DeviceArray(max(tot.value, 1), np.float32)
+        call("rvp_radius_fill", self.handle, *qp, nq, float(radius), off.ptr, idx.ptr, d2.ptr, stream)
+        O = off.numpy()
+        I, D = idx.numpy(), d2.numpy()
+        return [(I[O[i]:O[i + 1]].copy(), D[O[i]:O[i + 1]].copy()) for i in range(nq)]
+
+    def radiusSearch(self, query, radius, max_nn=0):
+        """SpatialHash::radiusSearch (spatial_hashing.cpp:165-223) for one query: (indices, dists)."""
+        return self.radius_search_batch([query], radius, max_nn)[0]
+
+    def knn(self, queries, k):
+        """k nearest by (d2, idx) of each query (rows padded with -1 / inf when the cloud is smaller)."""
+        if queries is None:
+            nq, qp = self.dev.n, (None, None, None)
+        else:
+            qx, qy, qz = _q(queries)
+            nq = len(qx)
+            dq = DeviceCloud.from_soa(qx, qy, qz)
+            qp = dq.ptrs()
+        idx = DeviceArray((nq, k), np.int32)
+        d2 = DeviceArray((nq, k), np.float32)
+        call("rvp_knn", self.handle, *qp, nq, int(k), idx.ptr, d2.ptr, None)
+        return idx.numpy(), d2.numpy()
+
+
+# ----------------------------------------------------------------------------------------------------
+# SOR
+# ----------------------------------------------------------------------------------------------------
+def _sor_exact(c: PointCloudSoA, k, alpha, with_means=False):
+    n = c.n
+    if n == 0:
+        return (np.zeros((0, 3), np.float32), np.zeros(0, np.float32)) if with_means else np.zeros((0, 3), np.float32)
+    dc = DeviceCloud.from_soa(c.x, c.y, c.z)
+    out = DeviceArray((n, 3), np.float32)
+    means = DeviceArray(n, np.float32)
+    m = C.c_size_t()
+    call("rvp_sor_exact", *dc.ptrs(), n, int(k), float(alpha), out.ptr, C.byref(m), None, means.ptr, None)
+    res = out.numpy()[:m.value]
+    return (res, means.numpy()) if with_means else res
+
+
+def sor_sc(cloud_aos, k, alpha, with_means=False):
+    """sor_sc (statistical_outlier_removal.cpp:19-63): kept points (AoS) in input order."""
+    return _sor_exact(_soa(cloud_aos), k, alpha, with_means)
+
+
+def sor_rvv(cloud_soa, k, alpha, with_means=False):
+    """sor_rvv (statistical_outlier_removal.cpp:68-190); same result as sor_sc."""
+    return _sor_exact(_soa(cloud_soa), k, alpha, with_means)
+
+
+def sor_spatial_hash(cloud_soa, hash_: SpatialHash, k, alpha, search_radius=0.5, with_means=False):
+    """sor_spatial_hash (statistical_outlier_removal.cpp:246-270); `hash_` is built on the same cloud."""
+    n = hash_.dev.n
+    out = DeviceArray((max(n, 1), 3), np.float32)
+    means = DeviceArray(max(n, 1), np.float32)
+    m = C.c_size_t()
+    call("rvp_sor_radius", hash_.handle, int(k), float(alpha), float(search_radius), out.ptr, C.byref(m), None,
+         means.ptr, None)
+    res = out.numpy()[:m.value]
+    return (res, means.numpy()[:n]) if with_means else res
+
+
+# ----------------------------------------------------------------------------------------------------
+# voxel grid
+# ----------------------------------------------------------------------------------------------------
+def _voxel(c: PointCloudSoA, leaf):
+    if c.n == 0:
+        return np.zeros((0, 3), np.float32)
+    dc = DeviceCloud.from_soa(c.x, c.y, c.z)
+    out = DeviceArray((c.n, 3), np.float32)
+    m = C.c_size_t()
+    call("rvp_voxel", *dc.ptrs(), c.n, float(leaf), out.ptr, C.byref(m), None)
+    return out.numpy()[:m.value]
+
+
+def voxel_grid_downsamp_sc(cloud_aos, leaf_size):
+    """voxel_grid_downsamp_sc (voxel_grid_downsamp.cpp:183-213): centroids in (vx, vy, vz) order."""
+    return _voxel(_soa(cloud_aos), leaf_size)
+
+
+def voxel_grid_downsamp_rvv(cloud_soa, leaf_size):
+    return _voxel(_soa(cloud_soa), leaf_size)
+
+
+def voxel_grid_downsamp_rvv_v2(cloud_soa, leaf_size):
+    return _voxel(_soa(cloud_soa), leaf_size)
+
+
+# ----------------------------------------------------------------------------------------------------
+# normals
+# ----------------------------------------------------------------------------------------------------
+def _normals_out(n):
+    return [DeviceArray(max(n, 1), np.float32) for _ in range(3)]
+
+
+def normal_estimation_sc(cloud_aos, k, radius, vp_x=0.0, vp_y=0.0, vp_z=0.0, eigen_iters=4):
+    """normal_estimation_sc (normal_estimation.cpp:72-112): (nx, ny, nz).  `radius` and `eigen_iters` are
+    accepted for signature compatibility (the reference ignores radius here; eigen solves converge)."""
+    c = _soa(cloud_aos)
+    if c.n == 0:
+        return tuple(np.zeros(0, np.float32) for _ in range(3))
+    dc = DeviceCloud.from_soa(c.x, c.y, c.z)
+    nx, ny, nz = _normals_out(c.n)
+    call("rvp_normals_knn", *dc.ptrs(), c.n, int(k), float(vp_x), float(vp_y), float(vp_z), nx.ptr, ny.ptr, nz.ptr,
+         None)
+    return nx.numpy()[:c.n], ny.numpy()[:c.n], nz.numpy()[:c.n]
+
+
+def normal_estimation_rvv(cloud_soa, hash_: SpatialHash, k, radius, vp_x=0.0, vp_y=0.0, vp_z=0.0, eigen_iters=4,
+                          max_nn=0):
+    """normal_estimation_rvv(in, SpatialHash&, ...) (normal_estimation.cpp:189-214): all neighbours within
+    radius (max_nn > 0 caps the (d2, idx)-sorted list); fewer than 3 gives a zero normal."""
+    n = hash_.dev.n
+    if n == 0:
+        return tuple(np.zeros(0, np.float32) for _ in range(3))
+    nx, ny, nz = _normals_out(n)
+    call("rvp_normals_radius", hash_.handle, float(radius), int(max_nn), float(vp_x), float(vp_y), float(vp_z),
+         nx.ptr, ny.ptr, nz.ptr, None)
+    return nx.numpy()[:n], ny.numpy()[:n], nz.numpy()[:n]
+
+
+def normals_fallback_count():
+    c = C.c_size_t()
+    call("rvp_normals_fallback_count", C.byref(c))
+    return c.value
+
+
+# ----------------------------------------------------------------------------------------------------
+# RANSAC + extraction
+# ----------------------------------------------------------------------------------------------------
+def _ransac(c: PointCloudSoA, dist_thresh, max_iters, collinear_thresh, probability):
+    model = (C.c_float * 4)()
+    best = C.c_int()
+    iters = C.c_int()
+    if c.n < 3:
+        return 0, np.zeros(4, np.float32), 0
+    dc = DeviceCloud.from_soa(c.x, c.y, c.z)
+    call("rvp_ransac_plane", *dc.ptrs(), c.n, float(dist_thresh), int(max_iters), float(collinear_thresh),
+         float(probability), model, C.byref(best), C.byref(iters), None)
+    return best.value, np.array(model, np.float32), iters.value
+
+
+def ransac_plane_sc(cloud_aos, dist_thresh, max_iters, collinear_thresh=1e-6, probability=0.99):
+    """ransac_plane_sc (ransac_plane.cpp:55-111): (inliers, model[4])."""
+    best, model, _ = _ransac(_soa(cloud_aos), dist_thresh, max_iters, collinear_thresh, probability)
+    return best, model
+
+
+def ransac_plane_rvv(cloud_soa, dist_thresh, max_iters, collinear_thresh=1e-6, probability=0.99):
+    best, model, _ = _ransac(_soa(cloud_soa), dist_thresh, max_iters, collinear_thresh, probability)
+    return best, model
+
+
+def extract_plane_inliers_outliers_rvv(cloud_soa, model, dist_thresh):
+    """extract_plane_inliers_outliers_rvv (ransac_plane.cpp:374-382): (inliers AoS, outliers AoS)."""
+    c = _soa(cloud_soa)
+    if c.n == 0:
+        z = np.zeros((0, 3), np.float32)
+        return z, z
+    dc = DeviceCloud.from_soa(c.x, c.y, c.z)
+    a = DeviceArray((c.n, 3), np.float32)
+    b = DeviceArray((c.n, 3), np.float32)
+    m = (C.c_float * 4)(*[float(v) for v in np.asarray(model, np.float32)])
+    ni, no = C.c_size_t(), C.c_size_t()
+    call("rvp_plane_extract", *dc.ptrs(), c.n, m, float(dist_thresh), a.ptr, C.byref(ni), b.ptr, C.byref(no), None)
+    return a.numpy()[:ni.value], b.numpy()[:no.value]
+
+
+def extract_plane_inliers_rvv(cloud_soa, model, dist_thresh):
+    return extract_plane_inliers_outliers_rvv(cloud_soa, model, dist_thresh)[0]
+
+
+def extract_plane_outliers_rvv(cloud_soa, model, dist_thresh):
+    return extract_plane_inliers_outliers_rvv(cloud_soa, model, dist_thresh)[1]

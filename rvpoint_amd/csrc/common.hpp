@@ -1,0 +1,177 @@
+// common.hpp -- shared host/device helpers for librvp_hip.so (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <utility>
+
+#include "../../include/rvp.h"
+
+namespace rvp {
+
+// ------------------------------------------------------------------------------------------------
+// error plumbing: every C entry point returns an int status; the message is kept per host thread.
+// ------------------------------------------------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define RVP_HIP(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return ::rvp::fail(RVP_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+#define RVP_CHECK(cond, code, msg)                                                                 \
+  do {                                                                                             \
+    if (!(cond)) return ::rvp::fail((code), (msg));                                               \
+  } while (0)
+
+#define RVP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    int s_ = (expr);                                                                               \
+    if (s_ != RVP_OK) return s_;                                                                   \
+  } while (0)
+
+// kernel launch check (catches bad launch configs; faults surface at the next sync)
+#define RVP_LAUNCHED() RVP_HIP(hipGetLastError())
+
+// ------------------------------------------------------------------------------------------------
+// stream-ordered temporary device buffer (hipMallocAsync pool, cached across calls)
+// ------------------------------------------------------------------------------------------------
+int pool_init();
+
+// A temporary lives in the stream-ordered pool and is freed on its stream; a persistent buffer (owned by
+// a handle that may outlive the stream it was built on) uses hipMalloc/hipFree.  Buffers are reused when
+// a later request of the same kind fits.
+struct Tmp {
+  void *p = nullptr;
+  size_t cap = 0;
+  bool persistent = false;
+  hipStream_t s = nullptr;
+  Tmp() = default;
+  Tmp(const Tmp &) = delete;
+  Tmp &operator=(const Tmp &) = delete;
+  ~Tmp() { release(); }
+  void release() {
+    if (p) {
+      if (persistent) (void)hipFree(p);
+      else (void)hipFreeAsync(p, s);
+    }
+    p = nullptr;
+    cap = 0;
+  }
+  int alloc(size_t bytes, hipStream_t stream, bool keep = false) {
+    if (bytes == 0) bytes = 16;
+    if (p && keep == persistent && bytes <= cap && (keep || stream == s)) return RVP_OK;
+    release();
+    s = stream;
+    persistent = keep;
+    hipError_t e = keep ? hipMalloc(&p, bytes) : hipMallocAsync(&p, bytes, stream);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(RVP_E_OOM, std::string(keep ? "hipMalloc(" : "hipMallocAsync(") + std::to_string(bytes) +
+                                 "): " + hipGetErrorString(e));
+    }
+    cap = bytes;
+    return RVP_OK;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+  void swap(Tmp &o) {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+    std::swap(persistent, o.persistent);
+    std::swap(s, o.s);
+  }
+};
+
+inline hipStream_t S(rvp_stream_t s) { return static_cast<hipStream_t>(s); }
+
+constexpr int WAVE = 64;
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------------------------------------
+// device helpers
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int lane() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
+
+__device__ __forceinline__ int popc(uint64_t m) { return __popcll(m); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() { return (uint64_t)__lanemask_lt(); }
+
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float bitsf(uint32_t u) { return __uint_as_float(u); }
+
+// (d2, idx) -> 64-bit key whose unsigned order is the reference's std::pair<float,int> order
+// (spatial_hashing.cpp:203-209); valid because d2 is a sum of squares, never -0 or negative.
+__device__ __forceinline__ uint64_t pack_key(float d2, uint32_t idx) {
+  return ((uint64_t)fbits(d2) << 32) | (uint64_t)idx;
+}
+__device__ __forceinline__ float key_d2(uint64_t k) { return bitsf((uint32_t)(k >> 32)); }
+__device__ __forceinline__ uint32_t key_idx(uint64_t k) { return (uint32_t)(k & 0xffffffffu); }
+
+// D1 (rvv_common.cpp:36-43): ((dx*dx + dy*dy) + dz*dz), each op rounded once.  The library is built
+// with -ffp-contract=off; the explicit _rn intrinsics make the order independent of that flag.
+__device__ __forceinline__ float dist2(float px, float py, float pz, float qx, float qy, float qz) {
+  float dx = __fsub_rn(px, qx);
+  float dy = __fsub_rn(py, qy);
+  float dz = __fsub_rn(pz, qz);
+  return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = __shfl_xor(lo, m);
+  hi = __shfl_xor(hi, m);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo = __shfl(lo, src);
+  hi = __shfl(hi, src);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ float readlanef(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ uint32_t readlaneu(uint32_t v, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
+template <class T> __device__ __forceinline__ T wave_incl_scan(T v) {
+  const int l = lane();
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    T t = __shfl_up(v, d);
+    if (l >= d) v += t;
+  }
+  return v;
+}
+
+template <class T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int d = WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+// next representable float toward +inf / -inf (finite inputs)
+__device__ __forceinline__ float next_up(float f) {
+  uint32_t u = fbits(f);
+  if (f == 0.0f) return bitsf(1u);
+  return bitsf((int32_t)u >= 0 ? u + 1 : u - 1);
+}
+__device__ __forceinline__ float next_down(float f) { return -next_up(-f); }
+
+}  // namespace rvp

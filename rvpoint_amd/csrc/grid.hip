@@ -1,0 +1,406 @@
+// grid.hip -- SpatialHash build on the GPU: bbox, cell keys, stable radix sort, cell-ordered SoA, cell table.
+//
+// Reference: SpatialHash::setInputCloud / build (src/search/spatial_hashing.cpp:76-163).  The reference
+// buckets point indices in an unordered_map keyed by ix + iy*73856093 + iz*19349663; here the cells of a
+// dense grid are sorted by a linear id so that every run of x-adjacent cells is one contiguous slice of
+// the cell-ordered point arrays.  Query results do not depend on the cell layout (see query.hip).
+#include <algorithm>
+#include <vector>
+
+#include "grid.hpp"
+#include "prims.hpp"
+
+namespace rvp {
+
+// ------------------------------------------------------------------------------------------------
+// bbox
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bbox_partial_k(const float *__restrict__ x, const float *__restrict__ y,
+                                                       const float *__restrict__ z, size_t n,
+                                                       float *__restrict__ part) {
+  float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    float a = x[i], b = y[i], c = z[i];
+    // strict comparisons: NaN never updates, as the reference's scalar loop (spatial_hashing.cpp:131-138)
+    if (a < v[0]) v[0] = a;
+    if (b < v[1]) v[1] = b;
+    if (c < v[2]) v[2] = c;
+    if (a > v[3]) v[3] = a;
+    if (b > v[4]) v[4] = b;
+    if (c > v[5]) v[5] = c;
+  }
+  __shared__ float sm[4][6];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float o = __shfl_xor(v[j], d);
+      if (o < v[j]) v[j] = o;
+      float p = __shfl_xor(v[j + 3], d);
+      if (p > v[j + 3]) v[j + 3] = p;
+    }
+  }
+  if (lane() == 0)
+    for (int j = 0; j < 6; ++j) sm[threadIdx.x >> 6][j] = v[j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w)
+      for (int j = 0; j < 3; ++j) {
+        if (sm[w][j] < v[j]) v[j] = sm[w][j];
+        if (sm[w][j + 3] > v[j + 3]) v[j + 3] = sm[w][j + 3];
+      }
+    for (int j = 0; j < 6; ++j) part[blockIdx.x * 6 + j] = v[j];
+  }
+}
+
+__global__ void bbox_final_k(const float *__restrict__ part, int nb, float *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int b = 0; b < nb; ++b)
+    for (int j = 0; j < 3; ++j) {
+      if (part[b * 6 + j] < v[j]) v[j] = part[b * 6 + j];
+      if (part[b * 6 + j + 3] > v[j + 3]) v[j + 3] = part[b * 6 + j + 3];
+    }
+  for (int j = 0; j < 6; ++j) out[j] = v[j];
+}
+
+int bbox_minmax(const float *x, const float *y, const float *z, size_t n, float *out6, hipStream_t s) {
+  const int nb = (int)std::min<size_t>(1024, std::max<size_t>(1, ceil_div(n, 256 * 8)));
+  Tmp part;
+  RVP_TRY(part.alloc(nb * 6 * sizeof(float), s));
+  bbox_partial_k<<<nb, 256, 0, s>>>(x, y, z, n, part.as<float>());
+  RVP_LAUNCHED();
+  bbox_final_k<<<1, 64, 0, s>>>(part.as<float>(), nb, out6);
+  RVP_LAUNCHED();
+  return RVP_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// header: reference bbox (margin 0.01*cell, spatial_hashing.cpp:141-146) and grid dims for cell*2^j
+// ------------------------------------------------------------------------------------------------
+constexpr int NCAND = 40;
+
+struct Header {
+  float bb[6];           // min/max after margin
+  double dims[NCAND][3]; // cells per axis for cell * 2^j
+};
+
+__global__ void header_k(const float *__restrict__ mm, float cell, Header *__restrict__ h) {
+  if (threadIdx.x != 0) return;
+  const float eps_scale = 0.01f;
+  const float m = __fmul_rn(cell, eps_scale);
+  float mn[3], mx[3];
+  for (int j = 0; j < 3; ++j) {
+    mn[j] = __fsub_rn(mm[j], m);
+    mx[j] = __fadd_rn(mm[j + 3], m);
+    h->bb[j] = mn[j];
+    h->bb[j + 3] = mx[j];
+  }
+  float c = cell;
+  for (int k = 0; k < NCAND; ++k) {
+    for (int j = 0; j < 3; ++j) {
+      // largest cell index of any point: floor((data_max - mn) / c) (monotone cell function)
+      float f = floorf(__fdiv_rn(__fsub_rn(mm[j + 3], mn[j]), c));
+      h->dims[k][j] = (double)f + 1.0;
+    }
+    c = c * 2.0f;
+  }
+}
+
+__global__ void keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+                       size_t n, float mnx, float mny, float mnz, float c, int gx, int gy, int gz,
+                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int ix = cell_clamped(x[i], mnx, c, gx);
+  int iy = cell_clamped(y[i], mny, c, gy);
+  int iz = cell_clamped(z[i], mnz, c, gz);
+  keys[i] = (uint32_t)ix + (uint32_t)gx * ((uint32_t)iy + (uint32_t)gy * (uint32_t)iz);
+  vals[i] = (uint32_t)i;
+}
+
+__global__ void gather_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+                         const uint32_t *__restrict__ perm, size_t n, float *__restrict__ xs,
+                         float *__restrict__ ys, float *__restrict__ zs) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t p = perm[i];
+  xs[i] = x[p];
+  ys[i] = y[p];
+  zs[i] = z[p];
+}
+
+// cell_start[c] = first sorted position whose key >= c, for c in [0, ncells]; short gaps are filled by the
+// point that ends them, long ones are queued for fill_gaps_k.
+struct Gap {
+  uint32_t lo, hi, val;  // cells [lo, hi] get val
+};
+constexpr uint32_t SHORT_GAP = 32;
+
+__global__ void cell_start_k(const uint32_t *__restrict__ keys, size_t n, uint32_t ncells,
+                             uint32_t *__restrict__ cs, uint32_t *__restrict__ occupied, Gap *__restrict__ gaps,
+                             uint32_t *__restrict__ ngaps, uint32_t gap_cap) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  uint32_t lo;
+  bool start;
+  if (i == 0) {
+    lo = 0;
+    start = true;
+  } else {
+    const uint32_t kp = keys[i - 1];
+    start = k != kp;
+    lo = kp + 1;
+  }
+  if (start) {
+    atomicAdd(occupied, 1u);
+    if (k - lo + 1 <= SHORT_GAP) {
+      for (uint32_t c = lo; c <= k; ++c) cs[c] = (uint32_t)i;
+    } else {
+      uint32_t g = atomicAdd(ngaps, 1u);
+      if (g < gap_cap) gaps[g] = Gap{lo, k, (uint32_t)i};
+    }
+  }
+  if (i == n - 1) {
+    // tail: cells (k, ncells] start at n
+    if (ncells - k <= SHORT_GAP) {
+      for (uint32_t c = k + 1; c <= ncells; ++c) cs[c] = (uint32_t)n;
+    } else {
+      uint32_t g = atomicAdd(ngaps, 1u);
+      if (g < gap_cap) gaps[g] = Gap{k + 1, ncells, (uint32_t)n};
+    }
+  }
+}
+
+__global__ void fill_gaps_k(const Gap *__restrict__ gaps, const uint32_t *__restrict__ ngaps, uint32_t gap_cap,
+                            uint32_t *__restrict__ cs) {
+  uint32_t ng = *ngaps;
+  if (ng > gap_cap) ng = gap_cap;
+  for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x) {
+    Gap gp = gaps[g];
+    for (uint64_t c = (uint64_t)gp.lo + threadIdx.x; c <= gp.hi; c += blockDim.x) cs[c] = gp.val;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Grid::build
+// ------------------------------------------------------------------------------------------------
+int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, float cell_, hipStream_t s) {
+  RVP_CHECK(cell_ > 0.0f && cell_ < FLT_MAX, RVP_E_ARG, "cell size must be positive and finite");
+  RVP_CHECK(n_ < 0xfffffff0ull, RVP_E_UNSUPPORTED, "more than 2^32-16 points");
+  x = x_; y = y_; z = z_; n = n_; req_cell = cell_;
+  built = false;
+  stream = s;
+  occupied = 0;
+  if (n == 0) {
+    std::fill(bb, bb + 6, 0.f);
+    cell = cell_;
+    dims[0] = dims[1] = dims[2] = 1;
+    ncells = 1;
+    RVP_TRY(cstart.alloc(2 * sizeof(uint32_t), s, persistent));
+    RVP_HIP(hipMemsetAsync(cstart.p, 0, 2 * sizeof(uint32_t), s));
+    RVP_TRY(xs.alloc(4, s, persistent));
+    RVP_TRY(ys.alloc(4, s, persistent));
+    RVP_TRY(zs.alloc(4, s, persistent));
+    RVP_TRY(perm.alloc(4, s, persistent));
+    built = true;
+    return RVP_OK;
+  }
+  RVP_CHECK(x && y && z, RVP_E_ARG, "null coordinate pointer");
+  Tmp mm, hd;
+  RVP_TRY(mm.alloc(6 * sizeof(float), s));
+  RVP_TRY(hd.alloc(sizeof(Header), s));
+  RVP_TRY(bbox_minmax(x, y, z, n, mm.as<float>(), s));
+  header_k<<<1, 64, 0, s>>>(mm.as<float>(), cell_, hd.as<Header>());
+  RVP_LAUNCHED();
+  Header h;
+  RVP_HIP(hipMemcpyAsync(&h, hd.p, sizeof(Header), hipMemcpyDeviceToHost, s));
+  RVP_HIP(hipStreamSynchronize(s));
+  std::copy(h.bb, h.bb + 6, bb);
+  // all-NaN cloud: min > max; every point then lands in cell 0 and never matches a query
+  const bool degenerate = !(h.bb[0] <= h.bb[3] && h.bb[1] <= h.bb[4] && h.bb[2] <= h.bb[5]);
+  const double limit = std::max<double>(16777216.0, 4.0 * (double)n);
+  int pick = -1;
+  for (int k = 0; k < NCAND && !degenerate; ++k) {
+    double d0 = h.dims[k][0], d1 = h.dims[k][1], d2 = h.dims[k][2];
+    if (!(d0 >= 1 && d1 >= 1 && d2 >= 1)) continue;
+    if (d0 > 1e9 || d1 > 1e9 || d2 > 1e9) continue;
+    if (d0 * d1 * d2 <= limit) { pick = k; break; }
+  }
+  if (pick < 0) {
+    RVP_CHECK(degenerate, RVP_E_UNSUPPORTED, "grid: coordinate extent too large for a dense cell table");
+    cell = cell_;
+    dims[0] = dims[1] = dims[2] = 1;
+  } else {
+    cell = cell_;
+    for (int k = 0; k < pick; ++k) cell = cell * 2.0f;
+    dims[0] = (int)h.dims[pick][0];
+    dims[1] = (int)h.dims[pick][1];
+    dims[2] = (int)h.dims[pick][2];
+  }
+  ncells = (uint64_t)dims[0] * dims[1] * dims[2];
+
+  Tmp kt, vt;
+  RVP_TRY(keys.alloc(n * sizeof(uint32_t), s, persistent));
+  RVP_TRY(perm.alloc(n * sizeof(uint32_t), s, persistent));
+  RVP_TRY(kt.alloc(n * sizeof(uint32_t), s));
+  RVP_TRY(vt.alloc(n * sizeof(uint32_t), s));
+  const unsigned nb = ceil_div(n, 256);
+  keys_k<<<nb, 256, 0, s>>>(x, y, z, n, bb[0], bb[1], bb[2], cell, dims[0], dims[1], dims[2], keys.as<uint32_t>(),
+                            perm.as<uint32_t>());
+  RVP_LAUNCHED();
+  uint32_t *kp = keys.as<uint32_t>(), *vp = perm.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
+  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n, bits_for(ncells - 1), s));
+  if (kp != keys.as<uint32_t>()) {  // odd pass count: the sorted data sits in the temporaries
+    RVP_HIP(hipMemcpyAsync(keys.p, kp, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    RVP_HIP(hipMemcpyAsync(perm.p, vp, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  }
+  RVP_TRY(xs.alloc(n * sizeof(float), s, persistent));
+  RVP_TRY(ys.alloc(n * sizeof(float), s, persistent));
+  RVP_TRY(zs.alloc(n * sizeof(float), s, persistent));
+  gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), n, xs.as<float>(), ys.as<float>(), zs.as<float>());
+  RVP_LAUNCHED();
+
+  RVP_TRY(cstart.alloc((ncells + 1) * sizeof(uint32_t), s, persistent));
+  const uint32_t gap_cap = (uint32_t)std::min<size_t>(n + 2, 1u << 22);
+  Tmp gaps, ctr;
+  RVP_TRY(gaps.alloc(gap_cap * sizeof(Gap), s));
+  RVP_TRY(ctr.alloc(2 * sizeof(uint32_t), s));
+  RVP_HIP(hipMemsetAsync(ctr.p, 0, 2 * sizeof(uint32_t), s));
+  cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), n, (uint32_t)ncells, cstart.as<uint32_t>(),
+                                  ctr.as<uint32_t>(), gaps.as<Gap>(), ctr.as<uint32_t>() + 1, gap_cap);
+  RVP_LAUNCHED();
+  fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<Gap>(), ctr.as<uint32_t>() + 1, gap_cap, cstart.as<uint32_t>());
+  RVP_LAUNCHED();
+  uint32_t hc[2];
+  RVP_HIP(hipMemcpyAsync(hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost, s));
+  RVP_HIP(hipStreamSynchronize(s));
+  RVP_CHECK(hc[1] <= gap_cap, RVP_E_DEVICE, "cell table: gap list overflow");
+  occupied = hc[0];
+  built = true;
+  return RVP_OK;
+}
+
+int Grid::build_for_knn(const float *x_, const float *y_, const float *z_, size_t n_, int k, hipStream_t s) {
+  if (n_ == 0) return build(x_, y_, z_, n_, 1.0f, s);
+  // first guess from the bbox volume, then one correction from the measured occupancy: about 2(k+1)
+  // candidates in a 3x3x3 window, assuming surface-like data (9 of 27 cells occupied)
+  Tmp mm;
+  RVP_TRY(mm.alloc(6 * sizeof(float), s));
+  RVP_TRY(bbox_minmax(x_, y_, z_, n_, mm.as<float>(), s));
+  float h[6];
+  RVP_HIP(hipMemcpyAsync(h, mm.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  RVP_HIP(hipStreamSynchronize(s));
+  double ext[3], vol = 1.0, emax = 0.0;
+  for (int j = 0; j < 3; ++j) {
+    ext[j] = (h[j + 3] >= h[j]) ? (double)h[j + 3] - (double)h[j] : 0.0;
+    emax = std::max(emax, ext[j]);
+  }
+  if (!(emax > 0.0) || !(emax < 1e30)) return build(x_, y_, z_, n_, 1.0f, s);
+  for (int j = 0; j < 3; ++j) vol *= std::max(ext[j], emax * 1e-3);
+  double c = std::cbrt(vol / (double)n_ * (double)(k + 1));
+  c = std::max(c, emax * 1e-6);
+  RVP_TRY(build(x_, y_, z_, n_, (float)c, s));
+  const double per_cell = (double)n_ / (double)std::max<size_t>(occupied, 1);
+  const double want = 2.0 * (k + 1) / 9.0;
+  double f = std::sqrt(want / per_cell);
+  if (f < 0.7 || f > 1.4) {
+    f = std::min(std::max(f, 0.1), 4.0);
+    RVP_TRY(build(x_, y_, z_, n_, (float)(c * f), s));
+  }
+  return RVP_OK;
+}
+
+GridView Grid::view() const {
+  GridView v;
+  v.xs = xs.as<float>();
+  v.ys = ys.as<float>();
+  v.zs = zs.as<float>();
+  v.perm = perm.as<uint32_t>();
+  v.cell_start = cstart.as<uint32_t>();
+  v.mnx = bb[0];
+  v.mny = bb[1];
+  v.mnz = bb[2];
+  v.cell = cell;
+  v.gx = dims[0];
+  v.gy = dims[1];
+  v.gz = dims[2];
+  v.n = (uint32_t)n;
+  return v;
+}
+
+// reference cell coordinates and hash keys (spatial_hashing.h:43-52) with the caller's cell size
+__global__ void ref_cells_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+                            size_t n, float mnx, float mny, float mnz, float c, int32_t *__restrict__ ixyz,
+                            int64_t *__restrict__ keys) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int ix = (int)floorf(__fdiv_rn(__fsub_rn(x[i], mnx), c));
+  int iy = (int)floorf(__fdiv_rn(__fsub_rn(y[i], mny), c));
+  int iz = (int)floorf(__fdiv_rn(__fsub_rn(z[i], mnz), c));
+  if (ixyz) {
+    ixyz[3 * i + 0] = ix;
+    ixyz[3 * i + 1] = iy;
+    ixyz[3 * i + 2] = iz;
+  }
+  if (keys) keys[i] = (int64_t)ix + (int64_t)iy * 73856093LL + (int64_t)iz * 19349663LL;
+}
+
+}  // namespace rvp
+
+using namespace rvp;
+
+extern "C" {
+
+int rvp_hash_create(rvp_hash **h) {
+  RVP_CHECK(h, RVP_E_ARG, "null handle pointer");
+  RVP_TRY(pool_init());
+  *h = new (std::nothrow) rvp_hash();
+  RVP_CHECK(*h, RVP_E_OOM, "host allocation failed");
+  (*h)->g.persistent = true;
+  return RVP_OK;
+}
+
+int rvp_hash_destroy(rvp_hash *h) {
+  if (!h) return RVP_OK;
+  delete h;
+  return RVP_OK;
+}
+
+int rvp_hash_build(rvp_hash *h, const float *x, const float *y, const float *z, size_t n, float cell,
+                   rvp_stream_t s) {
+  RVP_CHECK(h, RVP_E_ARG, "null handle");
+  return h->g.build(x, y, z, n, cell, S(s));
+}
+
+int rvp_hash_bbox(const rvp_hash *h, float *bbox6) {
+  RVP_CHECK(h && bbox6, RVP_E_ARG, "null argument");
+  RVP_CHECK(h->g.built, RVP_E_STATE, "hash not built");
+  std::copy(h->g.bb, h->g.bb + 6, bbox6);
+  return RVP_OK;
+}
+
+int rvp_hash_info(const rvp_hash *h, size_t *n, size_t *occ, int *dims3, float *grid_cell) {
+  RVP_CHECK(h, RVP_E_ARG, "null handle");
+  RVP_CHECK(h->g.built, RVP_E_STATE, "hash not built");
+  if (n) *n = h->g.n;
+  if (occ) *occ = h->g.occupied;
+  if (dims3) std::copy(h->g.dims, h->g.dims + 3, dims3);
+  if (grid_cell) *grid_cell = h->g.cell;
+  return RVP_OK;
+}
+
+int rvp_hash_cell_coords(const rvp_hash *h, int32_t *ixyz, int64_t *keys, rvp_stream_t s) {
+  RVP_CHECK(h, RVP_E_ARG, "null handle");
+  RVP_CHECK(h->g.built, RVP_E_STATE, "hash not built");
+  const Grid &g = h->g;
+  if (g.n == 0) return RVP_OK;
+  ref_cells_k<<<ceil_div(g.n, 256), 256, 0, S(s)>>>(g.x, g.y, g.z, g.n, g.bb[0], g.bb[1], g.bb[2], g.req_cell,
+                                                   ixyz, keys);
+  RVP_LAUNCHED();
+  RVP_HIP(hipStreamSynchronize(S(s)));
+  return RVP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,366 @@
+// normals.hip -- N1 / N2 normal estimation: neighbourhood (exact kNN or radius list), FP32 compensated
+// covariance re-centred at the query point, closed-form FP32 3x3 eigen solve with an FP64 Jacobi
+// fallback for near-degenerate spectra, flip toward the viewpoint.
+//
+// Reference: normal_estimation_sc (src/features/normal_estimation.cpp:72-112), normal_estimation_rvv with
+// a SpatialHash (:189-214), flipNormalTowardsViewpoint (:55-67).  The reference's own eigen solver
+// (simple_eigen3x3_smallest, :16-52: four non-converged Jacobi steps with a sign error) is deliberately not
+// reproduced; results are graded against an FP64 eigen solve of the same neighbourhood.
+#include <atomic>
+
+#include "prims.hpp"
+#include "query.hpp"
+
+namespace rvp {
+
+static std::atomic<unsigned long long> g_fallbacks{0};
+
+struct NormLds {
+  float c[6][WAVE];  // covariance of query j
+  float q[3][WAVE];  // query point
+  uint32_t row[WAVE];
+  int m[WAVE];
+};
+
+// cyclic Jacobi in double precision (symmetric 3x3), eigenvector of the smallest eigenvalue
+__device__ void jacobi_smallest_f64(double a00, double a01, double a02, double a11, double a12, double a22,
+                                    double v[3]) {
+  double A[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
+  double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    const double off = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
+    const double dia = fabs(A[0][0]) + fabs(A[1][1]) + fabs(A[2][2]);
+    if (off == 0.0 || off < 1e-18 * dia) break;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        const double apq = A[p][q];
+        if (apq == 0.0) continue;
+        const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 3; ++k) {
+          const double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < 3; ++k) {
+          const double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - s * vkq;
+          V[k][q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  int mi = 0;
+  if (A[1][1] < A[mi][mi]) mi = 1;
+  if (A[2][2] < A[mi][mi]) mi = 2;
+  const double nn = sqrt(V[0][mi] * V[0][mi] + V[1][mi] * V[1][mi] + V[2][mi] * V[2][mi]);
+  v[0] = V[0][mi] / nn;
+  v[1] = V[1][mi] / nn;
+  v[2] = V[2][mi] / nn;
+}
+
+// Smallest-eigenvalue eigenvector of a symmetric 3x3 (Smith's trigonometric eigenvalues, eigenvector from
+// the largest cross product of two rows of A - l3 I).  Returns true when it fell back to FP64 Jacobi
+// (relative gap (l2 - l3) / l1 < 1e-4, or an isotropic spectrum).
+__device__ bool smallest_eigvec(const float c[6], float n[3]) {
+  float a00 = c[0], a01 = c[1], a02 = c[2], a11 = c[3], a12 = c[4], a22 = c[5];
+  float scale = fmaxf(fmaxf(fmaxf(fabsf(a00), fabsf(a01)), fmaxf(fabsf(a02), fabsf(a11))),
+                      fmaxf(fabsf(a12), fabsf(a22)));
+  if (!(scale > 0.0f) || !(scale < FLT_MAX)) {
+    n[0] = 1.0f; n[1] = 0.0f; n[2] = 0.0f;
+    return false;
+  }
+  const float is = 1.0f / scale;
+  a00 *= is; a01 *= is; a02 *= is; a11 *= is; a12 *= is; a22 *= is;
+  const float q = (a00 + a11 + a22) * (1.0f / 3.0f);
+  const float b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
+  const float p2 = (b00 * b00 + b11 * b11 + b22 * b22 + 2.0f * (a01 * a01 + a02 * a02 + a12 * a12)) * (1.0f / 6.0f);
+  bool fallback = !(p2 > 1e-30f);
+  if (!fallback) {
+    const float p = sqrtf(p2), ip = 1.0f / p;
+    const float B00 = b00 * ip, B11 = b11 * ip, B22 = b22 * ip, B01 = a01 * ip, B02 = a02 * ip, B12 = a12 * ip;
+    const float det = B00 * (B11 * B22 - B12 * B12) - B01 * (B01 * B22 - B12 * B02) + B02 * (B01 * B12 - B11 * B02);
+    const float r = fminf(fmaxf(0.5f * det, -1.0f), 1.0f);
+    const float phi = acosf(r) * (1.0f / 3.0f);
+    const float l1 = q + 2.0f * p * cosf(phi);
+    const float l3 = q + 2.0f * p * cosf(phi + 2.0943951023931953f);
+    const float l2 = 3.0f * q - l1 - l3;
+    if (!((l2 - l3) >= 1e-4f * fmaxf(fabsf(l1), 1e-30f))) {
+      fallback = true;
+    } else {
+      const float r0[3] = {a00 - l3, a01, a02}, r1[3] = {a01, a11 - l3, a12}, r2[3] = {a02, a12, a22 - l3};
+      float v[3][3];
+      v[0][0] = r0[1] * r1[2] - r0[2] * r1[1]; v[0][1] = r0[2] * r1[0] - r0[0] * r1[2]; v[0][2] = r0[0] * r1[1] - r0[1] * r1[0];
+      v[1][0] = r0[1] * r2[2] - r0[2] * r2[1]; v[1][1] = r0[2] * r2[0] - r0[0] * r2[2]; v[1][2] = r0[0] * r2[1] - r0[1] * r2[0];
+      v[2][0] = r1[1] * r2[2] - r1[2] * r2[1]; v[2][1] = r1[2] * r2[0] - r1[0] * r2[2]; v[2][2] = r1[0] * r2[1] - r1[1] * r2[0];
+      float best = -1.0f;
+      int bi = 0;
+      for (int i = 0; i < 3; ++i) {
+        const float nn = v[i][0] * v[i][0] + v[i][1] * v[i][1] + v[i][2] * v[i][2];
+        if (nn > best) { best = nn; bi = i; }
+      }
+      if (!(best > 0.0f)) {
+        fallback = true;
+      } else {
+        const float inv = 1.0f / sqrtf(best);
+        n[0] = v[bi][0] * inv; n[1] = v[bi][1] * inv; n[2] = v[bi][2] * inv;
+      }
+    }
+  }
+  if (fallback) {
+    double v[3];
+    jacobi_smallest_f64(c[0], c[1], c[2], c[3], c[4], c[5], v);
+    n[0] = (float)v[0]; n[1] = (float)v[1]; n[2] = (float)v[2];
+  }
+  return fallback;
+}
+
+// flipNormalTowardsViewpoint (normal_estimation.cpp:55-67), fp32 in the reference's order
+__device__ __forceinline__ void flip_toward(float px, float py, float pz, float vpx, float vpy, float vpz, float n[3]) {
+  const float vx = __fsub_rn(vpx, px), vy = __fsub_rn(vpy, py), vz = __fsub_rn(vpz, pz);
+  const float dot = __fadd_rn(__fadd_rn(__fmul_rn(vx, n[0]), __fmul_rn(vy, n[1])), __fmul_rn(vz, n[2]));
+  if (dot < 0.0f) {
+    n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
+  }
+}
+
+// covariance of the m <= 64 neighbours held one per lane (payload = cell-order position), coordinates
+// re-centred at the query point, compensated butterfly sums; result uniform across the wave
+__device__ __forceinline__ void wave_cov(const GridView &g, bool valid, uint32_t pos, int m, float qx, float qy,
+                                         float qz, float c[6]) {
+  float px = 0.f, py = 0.f, pz = 0.f;
+  if (valid) {
+    px = __fsub_rn(g.xs[pos], qx);
+    py = __fsub_rn(g.ys[pos], qy);
+    pz = __fsub_rn(g.zs[pos], qz);
+  }
+  const float fm = __int2float_rn(m);
+  const float mx = __fdiv_rn(wave_sum_comp(px), fm);
+  const float my = __fdiv_rn(wave_sum_comp(py), fm);
+  const float mz = __fdiv_rn(wave_sum_comp(pz), fm);
+  const float dx = valid ? __fsub_rn(px, mx) : 0.f;
+  const float dy = valid ? __fsub_rn(py, my) : 0.f;
+  const float dz = valid ? __fsub_rn(pz, mz) : 0.f;
+  c[0] = wave_sum_comp(__fmul_rn(dx, dx));
+  c[1] = wave_sum_comp(__fmul_rn(dx, dy));
+  c[2] = wave_sum_comp(__fmul_rn(dx, dz));
+  c[3] = wave_sum_comp(__fmul_rn(dy, dy));
+  c[4] = wave_sum_comp(__fmul_rn(dy, dz));
+  c[5] = wave_sum_comp(__fmul_rn(dz, dz));
+}
+
+// lane j solves query j of the wave's batch
+__device__ __forceinline__ void solve_batch(NormLds &N, int nbatch, float vpx, float vpy, float vpz,
+                                            float *__restrict__ nx, float *__restrict__ ny, float *__restrict__ nz,
+                                            unsigned long long *fallbacks) {
+  wave_lds_sync();
+  const int l = lane();
+  bool fb = false;
+  if (l < nbatch) {
+    float n[3] = {0.f, 0.f, 0.f};
+    if (N.m[l] >= 3) {
+      const float c[6] = {N.c[0][l], N.c[1][l], N.c[2][l], N.c[3][l], N.c[4][l], N.c[5][l]};
+      fb = smallest_eigvec(c, n);
+      flip_toward(N.q[0][l], N.q[1][l], N.q[2][l], vpx, vpy, vpz, n);
+    }
+    const uint32_t r = N.row[l];
+    nx[r] = n[0];
+    ny[r] = n[1];
+    nz[r] = n[2];
+  }
+  const int nf = popc(ballot(fb));
+  if (l == 0 && nf) atomicAdd(fallbacks, (unsigned long long)nf);
+  wave_lds_sync();
+}
+
+__device__ __forceinline__ void stash(NormLds &N, int j, const float c[6], float x, float y, float z, uint32_t row,
+                                      int m) {
+  if (lane() == 0) {
+    for (int e = 0; e < 6; ++e) N.c[e][j] = c[e];
+    N.q[0][j] = x;
+    N.q[1][j] = y;
+    N.q[2][j] = z;
+    N.row[j] = row;
+    N.m[j] = m;
+  }
+}
+
+// MODE 0: k nearest (N1, K = k+1 incl. self); MODE 1: radius list capped at K (N2)
+template <int MODE>
+__global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, float vpx, float vpy, float vpz,
+                                                      float *__restrict__ nx, float *__restrict__ ny,
+                                                      float *__restrict__ nz, unsigned long long *fallbacks) {
+  __shared__ WaveLds lds[QW];
+  __shared__ NormLds nl[QW];
+  const int w = threadIdx.x >> 6;
+  const uint32_t base = (blockIdx.x * QW + w) * WAVE;
+  if (base >= g.n) return;
+  WaveLds &L = lds[w];
+  NormLds &N = nl[w];
+  const int nbatch = (int)min<uint32_t>(WAVE, g.n - base);
+  const float r2 = __fmul_rn(radius, radius);
+  for (int j = 0; j < nbatch; ++j) {
+    const uint32_t q = base + j;
+    const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
+    TopK t;
+    t.init(K);
+    if (MODE == 0) {
+      knn_search<true>(g, x, y, z, t, L);
+    } else {
+      int lo[3], hi[3];
+      window3(g, x, y, z, radius, lo, hi);
+      TopKSink<true> sink{t, L};
+      scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
+      topk_flush<true>(t, L);
+    }
+    const int m = min(t.cnt, K);
+    float c[6];
+    wave_cov(g, lane() < m, t.pay, m, x, y, z, c);
+    stash(N, j, c, x, y, z, g.perm[q], m);
+  }
+  solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
+}
+
+// N2 without a cap: neighbour positions in CSR rows (row = original index), covariance in chunks of 64
+__global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *__restrict__ offsets, int cap,
+                                                     const uint32_t *__restrict__ pos, float vpx, float vpy, float vpz,
+                                                     float *__restrict__ nx, float *__restrict__ ny,
+                                                     float *__restrict__ nz, unsigned long long *fallbacks) {
+  __shared__ NormLds nl[QW];
+  const int w = threadIdx.x >> 6;
+  const uint32_t base = (blockIdx.x * QW + w) * WAVE;
+  if (base >= g.n) return;
+  NormLds &N = nl[w];
+  const int l = lane();
+  const int nbatch = (int)min<uint32_t>(WAVE, g.n - base);
+  for (int j = 0; j < nbatch; ++j) {
+    const uint32_t q = base + j;
+    const uint32_t row = g.perm[q];
+    const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
+    const int64_t b = offsets[row];
+    const int m = (int)min<int64_t>(offsets[row + 1] - b, (int64_t)cap);
+    float c[6] = {0, 0, 0, 0, 0, 0};
+    if (m <= WAVE) {
+      wave_cov(g, l < m, l < m ? pos[b + l] : 0u, m, x, y, z, c);
+    } else {
+      // two passes over the row; each lane keeps a compensated running sum across chunks
+      float s[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+      for (int c0 = 0; c0 < m; c0 += WAVE) {
+        if (c0 + l < m) {
+          const uint32_t p = pos[b + c0 + l];
+          const float v[3] = {__fsub_rn(g.xs[p], x), __fsub_rn(g.ys[p], y), __fsub_rn(g.zs[p], z)};
+          for (int a = 0; a < 3; ++a) {
+            float t, err;
+            two_sum(s[a], v[a], t, err);
+            s[a] = t;
+            e[a] = __fadd_rn(e[a], err);
+          }
+        }
+      }
+      const float fm = __int2float_rn(m);
+      float mu[3];
+      for (int a = 0; a < 3; ++a) mu[a] = __fdiv_rn(wave_sum_comp(s[a], e[a]), fm);
+      float cs[6] = {0, 0, 0, 0, 0, 0}, ce[6] = {0, 0, 0, 0, 0, 0};
+      for (int c0 = 0; c0 < m; c0 += WAVE) {
+        if (c0 + l < m) {
+          const uint32_t p = pos[b + c0 + l];
+          const float d[3] = {__fsub_rn(__fsub_rn(g.xs[p], x), mu[0]), __fsub_rn(__fsub_rn(g.ys[p], y), mu[1]),
+                              __fsub_rn(__fsub_rn(g.zs[p], z), mu[2])};
+          const float pr[6] = {__fmul_rn(d[0], d[0]), __fmul_rn(d[0], d[1]), __fmul_rn(d[0], d[2]),
+                               __fmul_rn(d[1], d[1]), __fmul_rn(d[1], d[2]), __fmul_rn(d[2], d[2])};
+          for (int a = 0; a < 6; ++a) {
+            float t, err;
+            two_sum(cs[a], pr[a], t, err);
+            cs[a] = t;
+            ce[a] = __fadd_rn(ce[a], err);
+          }
+        }
+      }
+      for (int a = 0; a < 6; ++a) c[a] = wave_sum_comp(cs[a], ce[a]);
+    }
+    stash(N, j, c, x, y, z, row, m);
+  }
+  solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
+}
+
+}  // namespace rvp
+
+using namespace rvp;
+
+static int finish_fallbacks(unsigned long long *dev, hipStream_t s) {
+  unsigned long long h = 0;
+  RVP_HIP(hipMemcpyAsync(&h, dev, sizeof(h), hipMemcpyDeviceToHost, s));
+  RVP_HIP(hipStreamSynchronize(s));
+  g_fallbacks.store(h);
+  return RVP_OK;
+}
+
+extern "C" {
+
+int rvp_normals_knn(const float *x, const float *y, const float *z, size_t n, int k, float vpx, float vpy, float vpz,
+                    float *nx, float *ny, float *nz, rvp_stream_t st) {
+  RVP_CHECK(k >= 1 && k + 1 <= MAX_TOPK, RVP_E_UNSUPPORTED, "k must be in [1, 63]");
+  RVP_CHECK(nx && ny && nz, RVP_E_ARG, "null output");
+  if (n == 0) return RVP_OK;
+  RVP_TRY(pool_init());
+  hipStream_t s = S(st);
+  Grid G;
+  RVP_TRY(G.build_for_knn(x, y, z, n, k + 1, s));
+  Tmp fb;
+  RVP_TRY(fb.alloc(sizeof(unsigned long long), s));
+  RVP_HIP(hipMemsetAsync(fb.p, 0, sizeof(unsigned long long), s));
+  normals_topk_k<0><<<ceil_div(n, QW * WAVE), QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz, nx, ny, nz,
+                                                          fb.as<unsigned long long>());
+  RVP_LAUNCHED();
+  return finish_fallbacks(fb.as<unsigned long long>(), s);
+}
+
+int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, float vpy, float vpz, float *nx,
+                       float *ny, float *nz, rvp_stream_t st) {
+  RVP_CHECK(h && h->g.built, RVP_E_STATE, "hash not built");
+  RVP_CHECK(nx && ny && nz, RVP_E_ARG, "null output");
+  RVP_CHECK(max_nn >= 0, RVP_E_ARG, "max_nn must be >= 0");
+  const size_t n = h->g.n;
+  if (n == 0) return RVP_OK;
+  hipStream_t s = S(st);
+  Tmp fb;
+  RVP_TRY(fb.alloc(sizeof(unsigned long long), s));
+  RVP_HIP(hipMemsetAsync(fb.p, 0, sizeof(unsigned long long), s));
+  const GridView g = h->g.view();
+  if (max_nn > 0 && max_nn <= MAX_TOPK) {
+    normals_topk_k<1><<<ceil_div(n, QW * WAVE), QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx, ny, nz,
+                                                            fb.as<unsigned long long>());
+    RVP_LAUNCHED();
+  } else {
+    // unlimited (or a cap above 64): full lists, sorted by (d2, idx), truncated by the row offsets
+    Tmp cnt, off, keys, pos;
+    RVP_TRY(cnt.alloc(n * sizeof(int32_t), s));
+    RVP_TRY(off.alloc((n + 1) * sizeof(int64_t), s));
+    RVP_TRY(rvp_radius_count(h, nullptr, nullptr, nullptr, n, radius, cnt.as<int32_t>(), st));
+    int64_t tot = 0;
+    RVP_TRY(rvp_exclusive_scan(cnt.as<int32_t>(), n, off.as<int64_t>(), &tot, st));
+    RVP_TRY(keys.alloc((size_t)tot * sizeof(uint64_t), s));
+    RVP_TRY(pos.alloc((size_t)tot * sizeof(uint32_t), s));
+    RVP_TRY(radius_csr(h->g, nullptr, nullptr, nullptr, n, radius, off.as<int64_t>(), keys.as<uint64_t>(),
+                       pos.as<uint32_t>(), s));
+    // a cap keeps the first min(len, max_nn) entries of each (d2, idx)-sorted row
+    const int cap = max_nn > 0 ? max_nn : 0x7fffffff;
+    normals_csr_k<<<ceil_div(n, QW * WAVE), QT, 0, s>>>(g, off.as<int64_t>(), cap, pos.as<uint32_t>(), vpx, vpy,
+                                                        vpz, nx, ny, nz, fb.as<unsigned long long>());
+    RVP_LAUNCHED();
+  }
+  return finish_fallbacks(fb.as<unsigned long long>(), s);
+}
+
+int rvp_normals_fallback_count(size_t *count) {
+  RVP_CHECK(count, RVP_E_ARG, "null argument");
+  *count = (size_t)g_fallbacks.load();
+  return RVP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// prims.hpp -- device-wide primitives: exclusive scan, stable LSD radix sort, stream compaction.
+#pragma once
+
+#include "common.hpp"
+
+namespace rvp {
+
+constexpr int PRIM_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;                       // items per thread per scan tile
+constexpr int SCAN_TILE = PRIM_THREADS * SCAN_ITEMS; // 2048
+
+// Exclusive scan out[i] = sum(in[0..i)), i < n; if total_dev != nullptr it receives sum(in[0..n)).
+// In/Out may alias.  Out is uint32_t or uint64_t (int64_t via reinterpretation).
+int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev, hipStream_t s);
+int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *out, size_t n, uint64_t *total_dev, hipStream_t s);
+
+// Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the key (8-bit digits).
+// On return the sorted data is in (*keys, *vals): the function swaps the pointers with the temporaries
+// when an odd number of passes leaves it there.
+int radix_sort_pairs_u32(uint32_t **keys, uint32_t **vals, uint32_t **keys_tmp, uint32_t **vals_tmp,
+                         size_t n, int bits, hipStream_t s);
+int radix_sort_pairs_u64(uint64_t **keys, uint32_t **vals, uint64_t **keys_tmp, uint32_t **vals_tmp,
+                         size_t n, int bits, hipStream_t s);
+
+// Stable compaction of points whose flag equals `want` into AoS (float[3*m]); *count_dev = m (device).
+int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want,
+                    size_t n, float *out_aos, uint32_t *count_dev, hipStream_t s);
+
+inline int bits_for(uint64_t max_value) {
+  int b = 0;
+  while (b < 64 && (max_value >> b) != 0) ++b;
+  return b;
+}
+
+// block-wide exclusive scan for 256-thread blocks; smem must hold 4 T
+template <class T> __device__ __forceinline__ T block_excl_scan_256(T v, T &block_total, T *smem) {
+  T inc = wave_incl_scan(v);
+  const int w = threadIdx.x >> 6;
+  if (lane() == WAVE - 1) smem[w] = inc;
+  __syncthreads();
+  T wofs = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < PRIM_THREADS / WAVE; ++i) {
+    T t = smem[i];
+    if (i < w) wofs += t;
+    tot += t;
+  }
+  __syncthreads();
+  block_total = tot;
+  return wofs + inc - v;
+}
+
+}  // namespace rvp

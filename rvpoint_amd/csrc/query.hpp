@@ -1,0 +1,286 @@
+// query.hpp -- wave-level neighbour machinery shared by the radius, kNN, SOR and normals kernels.
+//
+// One wavefront (64 lanes) serves one query.  Candidates are the points of a box of cells; because the
+// linear cell id is x-fastest, each (y, z) row of the box is ONE contiguous slice of the cell-ordered
+// arrays, so a 3x3x3 box is 9 coalesced slices.  The 64 lanes walk the concatenated slices 64 at a time,
+// compute D1 exactly as the reference (dist2), and offer hits to a per-wave top-K: hits are compacted
+// with a ballot prefix into an LDS staging buffer and, when it fills, bitonic-sorted across the lanes
+// and merged into the sorted best list held one entry per lane.  Keys are (d2 bits << 32 | index), so
+// the order is the reference's (d2, idx) pair order (spatial_hashing.cpp:203-209).
+#pragma once
+
+#include "grid.hpp"
+
+namespace rvp {
+
+constexpr int QT = 256;         // threads per query block
+constexpr int QW = QT / WAVE;   // waves (queries in flight) per block
+constexpr int MAX_TOPK = 64;    // entries one wave keeps in registers
+
+struct WaveLds {
+  uint64_t bk[WAVE];   // staged hit keys
+  uint32_t bp[WAVE];   // staged payloads (cell-order position)
+  uint32_t excl[WAVE]; // row-slice prefix offsets
+  uint32_t s0[WAVE], l0[WAVE], s1[WAVE];
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct TopK {
+  uint64_t key;  // this lane's entry; lanes [0, cnt) sorted ascending, the rest ~0
+  uint32_t pay;
+  int cnt;       // valid entries (<= 64)
+  int K;
+  uint64_t kth;  // key at K-1 once cnt >= K, else ~0 (hits must beat it to be staged)
+  int nb;        // staged entries
+  uint32_t hits; // all hits offered (untruncated count)
+  __device__ __forceinline__ void init(int k) {
+    key = ~0ull;
+    pay = 0;
+    cnt = 0;
+    K = k;
+    kth = ~0ull;
+    nb = 0;
+    hits = 0;
+  }
+};
+
+// all compare-exchanges keep the minimum at the lower lane ("mirror" bitonic formulation)
+template <bool PAY>
+__device__ __forceinline__ void cx(uint64_t &k, uint32_t &p, int partner_mask, bool lower) {
+  uint64_t ok = shfl_xor64(k, partner_mask);
+  uint32_t op = PAY ? (uint32_t)__shfl_xor((int)p, partner_mask) : 0u;
+  bool take = lower ? (ok < k) : (ok > k);
+  if (take) {
+    k = ok;
+    if (PAY) p = op;
+  }
+}
+
+template <bool PAY> __device__ __forceinline__ void bitonic_sort64(uint64_t &k, uint32_t &p) {
+  const int l = lane();
+#pragma unroll
+  for (int s = 2; s <= WAVE; s <<= 1) {
+    cx<PAY>(k, p, s - 1, (l & (s >> 1)) == 0);
+#pragma unroll
+    for (int j = s >> 2; j > 0; j >>= 1) cx<PAY>(k, p, j, (l & j) == 0);
+  }
+}
+
+template <bool PAY> __device__ __forceinline__ void bitonic_clean64(uint64_t &k, uint32_t &p) {
+  const int l = lane();
+#pragma unroll
+  for (int j = WAVE / 2; j > 0; j >>= 1) cx<PAY>(k, p, j, (l & j) == 0);
+}
+
+template <bool PAY> __device__ __forceinline__ void topk_flush(TopK &t, WaveLds &L) {
+  if (t.nb == 0) return;
+  const int l = lane();
+  wave_lds_sync();
+  uint64_t nk = l < t.nb ? L.bk[l] : ~0ull;
+  uint32_t np = (PAY && l < t.nb) ? L.bp[l] : 0u;
+  wave_lds_sync();
+  bitonic_sort64<PAY>(nk, np);
+  if (t.cnt == 0) {
+    t.key = nk;
+    t.pay = np;
+  } else {
+    // min(best[l], new[63-l]) holds the 64 smallest of the union as a bitonic sequence
+    uint64_t rk = shfl64(nk, WAVE - 1 - l);
+    uint32_t rp = PAY ? (uint32_t)__shfl((int)np, WAVE - 1 - l) : 0u;
+    if (rk < t.key) {
+      t.key = rk;
+      t.pay = rp;
+    }
+    bitonic_clean64<PAY>(t.key, t.pay);
+  }
+  t.cnt = min(WAVE, t.cnt + t.nb);
+  t.nb = 0;
+  t.kth = (t.cnt >= t.K) ? readlane64(t.key, t.K - 1) : ~0ull;
+}
+
+template <bool PAY>
+__device__ __forceinline__ void topk_offer(TopK &t, WaveLds &L, bool hit, uint64_t key, uint32_t pay) {
+  t.hits += (uint32_t)popc(ballot(hit));
+  bool pass = hit && key < t.kth;
+  uint64_t m = ballot(pass);
+  int np = popc(m);
+  if (np == 0) return;
+  if (t.nb + np > WAVE) {
+    topk_flush<PAY>(t, L);
+    pass = hit && key < t.kth;
+    m = ballot(pass);
+    np = popc(m);
+    if (np == 0) return;
+  }
+  if (pass) {
+    const int pos = t.nb + popc(m & lanemask_lt());
+    L.bk[pos] = key;
+    if (PAY) L.bp[pos] = pay;
+  }
+  t.nb += np;
+}
+
+// Walk the cells of box [lo, hi] (minus box [ilo, ihi] when `inner`), calling sink(valid, hit, key, pos)
+// uniformly on every lane for each group of 64 candidates.  radius_mode: hit = d2 <= r2; otherwise
+// hit = d2 is not NaN (k-NN).
+template <class Sink>
+__device__ __forceinline__ void scan_box(const GridView &g, float qx, float qy, float qz, const int lo[3],
+                                         const int hi[3], bool inner, const int ilo[3], const int ihi[3],
+                                         float r2, bool radius_mode, WaveLds &L, Sink &sink) {
+  const int l = lane();
+  const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+  if (ny <= 0 || nz <= 0 || hi[0] < lo[0]) return;
+  const int nrows = ny * nz;
+  for (int r0 = 0; r0 < nrows; r0 += WAVE) {
+    const int row = r0 + l;
+    uint32_t s0 = 0, l0 = 0, s1 = 0, l1 = 0;
+    if (row < nrows) {
+      const int iy = lo[1] + row % ny, iz = lo[2] + row / ny;
+      const size_t base = ((size_t)iz * (size_t)g.gy + (size_t)iy) * (size_t)g.gx;
+      if (inner && iy >= ilo[1] && iy <= ihi[1] && iz >= ilo[2] && iz <= ihi[2]) {
+        if (ilo[0] > lo[0]) {
+          s0 = g.cell_start[base + lo[0]];
+          l0 = g.cell_start[base + ilo[0]] - s0;
+        }
+        if (ihi[0] < hi[0]) {
+          s1 = g.cell_start[base + ihi[0] + 1];
+          l1 = g.cell_start[base + hi[0] + 1] - s1;
+        }
+      } else {
+        s0 = g.cell_start[base + lo[0]];
+        l0 = g.cell_start[base + hi[0] + 1] - s0;
+      }
+    }
+    const uint32_t len = l0 + l1;
+    const uint32_t inc = wave_incl_scan(len);
+    const uint32_t total = readlaneu(inc, WAVE - 1);
+    if (total == 0) continue;
+    wave_lds_sync();
+    L.excl[l] = inc - len;
+    L.s0[l] = s0;
+    L.l0[l] = l0;
+    L.s1[l] = s1;
+    wave_lds_sync();
+    for (uint32_t c0 = 0; c0 < total; c0 += WAVE) {
+      const uint32_t v = c0 + (uint32_t)l;
+      const bool valid = v < total;
+      int o = 0;
+#pragma unroll
+      for (int st = WAVE / 2; st > 0; st >>= 1)
+        if (L.excl[o + st] <= v) o += st;
+      const uint32_t off = v - L.excl[o];
+      const uint32_t pos = off < L.l0[o] ? L.s0[o] + off : L.s1[o] + (off - L.l0[o]);
+      float d2 = 0.0f;
+      uint32_t idx = 0;
+      bool hit = false;
+      if (valid) {
+        d2 = dist2(g.xs[pos], g.ys[pos], g.zs[pos], qx, qy, qz);
+        idx = g.perm[pos];
+        hit = radius_mode ? (d2 <= r2) : (d2 == d2);
+      }
+      sink(valid, hit, pack_key(d2, idx), pos);
+    }
+    wave_lds_sync();
+  }
+}
+
+template <bool PAY> struct TopKSink {
+  TopK &t;
+  WaveLds &L;
+  __device__ __forceinline__ void operator()(bool, bool hit, uint64_t key, uint32_t pos) {
+    topk_offer<PAY>(t, L, hit, key, pos);
+  }
+};
+
+// Exact k nearest by (d2, idx): 3x3x3 cells around the query, doubling growth until k candidates are
+// known, then the box certified by window3(q, R) with fl(R*R) > d2_k: every point outside it has a
+// larger d2, so the k best are final.
+template <bool PAY>
+__device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy, float qz, TopK &t, WaveLds &L) {
+  const int c[3] = {cell_clamped(qx, g.mnx, g.cell, g.gx), cell_clamped(qy, g.mny, g.cell, g.gy),
+                    cell_clamped(qz, g.mnz, g.cell, g.gz)};
+  const int gd[3] = {g.gx, g.gy, g.gz};
+  int lo[3], hi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = max(c[a] - 1, 0);
+    hi[a] = min(c[a] + 1, gd[a] - 1);
+  }
+  TopKSink<PAY> sink{t, L};
+  scan_box(g, qx, qy, qz, lo, hi, false, lo, hi, 0.0f, false, L, sink);
+  int step = 1;
+  for (;;) {
+    topk_flush<PAY>(t, L);
+    if (lo[0] == 0 && lo[1] == 0 && lo[2] == 0 && hi[0] == gd[0] - 1 && hi[1] == gd[1] - 1 && hi[2] == gd[2] - 1)
+      break;
+    int nlo[3], nhi[3];
+    if (t.cnt >= t.K) {
+      const float dk = key_d2(t.kth);
+      const float R = next_up(__fmul_rn(__fsqrt_rn(dk), 1.0f + 0x1p-22f));
+      if (!(R < FLT_MAX)) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          nlo[a] = 0;
+          nhi[a] = gd[a] - 1;
+        }
+      } else {
+        int wlo[3], whi[3];
+        window3(g, qx, qy, qz, R, wlo, whi);
+        if (wlo[0] >= lo[0] && wlo[1] >= lo[1] && wlo[2] >= lo[2] && whi[0] <= hi[0] && whi[1] <= hi[1] &&
+            whi[2] <= hi[2])
+          break;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          nlo[a] = min(lo[a], wlo[a]);
+          nhi[a] = max(hi[a], whi[a]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        nlo[a] = max(lo[a] - step, 0);
+        nhi[a] = min(hi[a] + step, gd[a] - 1);
+      }
+      step = step < (1 << 20) ? step * 2 : step;
+    }
+    scan_box(g, qx, qy, qz, nlo, nhi, true, lo, hi, 0.0f, false, L, sink);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = nlo[a];
+      hi[a] = nhi[a];
+    }
+  }
+}
+
+// error-free transformation a + b = s + e
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+  s = __fadd_rn(a, b);
+  const float bb = __fsub_rn(s, a);
+  e = __fadd_rn(__fsub_rn(a, __fsub_rn(s, bb)), __fsub_rn(b, bb));
+}
+
+// compensated butterfly sum across the wave (identical result on every lane)
+__device__ __forceinline__ float wave_sum_comp(float v, float e_in = 0.0f) {
+  float s = v, e = e_in;
+#pragma unroll
+  for (int d = WAVE / 2; d > 0; d >>= 1) {
+    const float so = __shfl_xor(s, d), eo = __shfl_xor(e, d);
+    float t, err;
+    two_sum(s, so, t, err);
+    e = __fadd_rn(__fadd_rn(e, eo), err);
+    s = t;
+  }
+  return __fadd_rn(s, e);
+}
+
+// CSR fill of radius lists (rows = original query index; qx == nullptr: the grid's own points), each row
+// sorted by (d2, idx); pos (may be null) receives the cell-order position of each neighbour.
+int radius_csr(const Grid &G, const float *qx, const float *qy, const float *qz, size_t nq, float r,
+               const int64_t *offsets, uint64_t *keys, uint32_t *pos, hipStream_t s);
+
+}  // namespace rvp

@@ -1,0 +1,55 @@
+"""CPU-side checks of the C ABI library: it loads, exports exactly what include/rvp.h declares, and
+reports errors (not crashes) without a GPU.  No compute calls."""
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SO = os.path.join(ROOT, "rvpoint_amd", "lib", "librvp_hip.so")
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "rvp.h")).read()
+    return set(re.findall(r"\b(rvp_[a-z0-9_]+)\s*\(", txt))
+
+
+@pytest.fixture(scope="module")
+def built():
+    if not os.path.exists(SO):
+        from rvpoint_amd import build
+        build.build()
+    return SO
+
+
+def test_exports_match_header(built):
+    out = subprocess.run(["nm", "-D", "--defined-only", built], capture_output=True, text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T rvp_" in l}
+    assert exported == declared()
+
+
+def test_python_binding_declares_every_symbol(built):
+    from rvpoint_amd import _lib
+    assert set(_lib.exported_symbols()) == declared()
+    L = _lib.lib()
+    for name in declared():
+        assert hasattr(L, name)
+
+
+def test_library_is_gfx950(built):
+    blob = open(built, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_errors_without_device(built):
+    import ctypes as C
+
+    from rvpoint_amd import _lib
+    L = _lib.lib()
+    assert L.rvp_version().decode().startswith("rvpoint-amd")
+    # bad arguments are rejected before any device work
+    assert L.rvp_hash_create(None) == _lib.RVP_E_ARG
+    assert L.rvp_radius_search(None, None, None, None, 0, C.c_float(1.0), 4, None, None, None, None, None) \
+        == _lib.RVP_E_ARG
+    assert b"null" in L.rvp_last_error()

@@ -1,0 +1,283 @@
+"""GPU parity: every HIP entry point against the CPU oracle on the same seeded inputs.
+
+Bit-exact: neighbour lists (indices, d2, order, counts), voxel centroids, SOR keep-sets and means, RANSAC
+count/model/iterations, plane extraction.  Normals: angular error against the FP64 oracle on the same
+neighbourhoods, median <= 0.01 deg (sign-agnostic), p99 <= 1 deg."""
+import numpy as np
+import pytest
+
+import oracle as O
+from rvpoint_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+R = pytest.importorskip("rvpoint_amd")
+
+
+def soa(p):
+    return [np.ascontiguousarray(p[:, i]) for i in range(3)]
+
+
+def gauss(n, seed, scale=1.0):
+    return np.random.default_rng(seed).normal(0, scale, (n, 3)).astype(np.float32)
+
+
+def c0_small(n=3000, seed=7):
+    return scenes.plane_sphere_cloud(n, seed=seed)
+
+
+# ------------------------------------------------------------------------------------------------ H1/H2
+@pytest.mark.parametrize("cell", [0.02, 0.1, 0.37])
+def test_hash_bbox_and_cells(cell):
+    p = c0_small()
+    x, y, z = soa(p)
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(x, y, z), cell)
+    h.build()
+    assert (h.bbox() == O.bbox(x, y, z, cell)).all()
+    ixyz, keys = h.cell_coords()
+    oi, ok = O.cell_coords(x, y, z, cell)
+    assert (ixyz == oi).all() and (keys == ok).all()
+
+
+# ------------------------------------------------------------------------------------------------ H3
+def _check_lists(got, idx, d2, cnt):
+    for i, (gi, gd) in enumerate(got):
+        m = cnt[i]
+        assert len(gi) == m, i
+        assert (gi == idx[i, :m]).all(), i
+        assert (gd == d2[i, :m]).all(), i
+
+
+@pytest.mark.parametrize("cell,r,max_nn", [(0.02, 0.02, 64), (0.02, 0.05, 16), (0.05, 0.02, 1), (0.01, 0.03, 64),
+                                           (0.02, 0.02, 100)])
+def test_radius_capped_external_queries(cell, r, max_nn):
+    p = c0_small(4000, 11)
+    x, y, z = soa(p)
+    rng = np.random.default_rng(3)
+    q = np.concatenate([p[:300], rng.uniform(-0.1, 1.1, (100, 3)).astype(np.float32)])
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(x, y, z), cell)
+    h.build()
+    got = h.radius_search_batch(q, r, max_nn)
+    idx, d2, cnt, _ = O.hash_radius(x, y, z, cell, *soa(q), r, max_nn)
+    _check_lists(got, idx, d2, cnt)
+
+
+@pytest.mark.parametrize("cell,r", [(0.02, 0.02), (0.05, 0.08)])
+def test_radius_unlimited_self(cell, r):
+    p = c0_small(3000, 12)
+    x, y, z = soa(p)
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(x, y, z), cell)
+    h.build()
+    got = h.radius_search_batch(None, r, 0)
+    idx, d2, cnt, _ = O.hash_radius(x, y, z, cell, x, y, z, r, 0)
+    _check_lists(got, idx, d2, cnt)
+
+
+def test_radius_long_lists_lds_and_global_sort():
+    """rows of 65..4096 (LDS sort) and > 4096 (global-memory sort)."""
+    p = gauss(9000, 13, 0.3)
+    x, y, z = soa(p)
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(x, y, z), 0.2)
+    h.build()
+    q = np.array([[0, 0, 0], [0.2, 0.1, 0.0], [1.0, 1.0, 1.0]], np.float32)
+    for r in (0.15, 0.5, 2.0):
+        got = h.radius_search_batch(q, r, 0)
+        idx, d2, cnt, _ = O.radius_all(x, y, z, *soa(q), r, 0)
+        _check_lists(got, idx, d2, cnt)
+    assert max(len(g[0]) for g in got) > 4096
+
+
+def test_radius_duplicates_tie_break_by_index():
+    base = gauss(50, 14, 0.01)
+    p = np.repeat(base, 30, axis=0)          # exact duplicates: equal d2, order by index
+    x, y, z = soa(p)
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(x, y, z), 0.01)
+    h.build()
+    for mnn in (0, 20, 64):
+        got = h.radius_search_batch(p[:40], 0.01, mnn)
+        idx, d2, cnt, _ = O.hash_radius(x, y, z, 0.01, *soa(p[:40]), 0.01, mnn)
+        _check_lists(got, idx, d2, cnt)
+
+
+def test_radius_empty_and_tiny():
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(np.zeros(0), np.zeros(0), np.zeros(0)), 0.1)
+    h.build()
+    assert [len(g[0]) for g in h.radius_search_batch([[0, 0, 0]], 1.0, 5)] == [0]
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA([1.0], [2.0], [3.0]), 0.1)
+    h.build()
+    i, d = h.radiusSearch([1.0, 2.0, 3.0], 0.5)
+    assert list(i) == [0] and list(d) == [0.0]
+
+
+def test_radius_reference_unit_test_shape():
+    """tests/unit/test_radius.cpp input shape: 1000 N(0,2) points, q=0, r=2, max_nn=100: brute force sc/rvv."""
+    p = gauss(1000, 42, 2.0)
+    for mnn in (100, 5, 0):
+        gi, gd = R.radius_search_sc(p, (0, 0, 0), 2.0, mnn)
+        oi, od = O.radius_bruteforce(*soa(p), (0.0, 0.0, 0.0), 2.0, mnn)
+        assert (gi == oi).all() and (gd == od).all()
+        gi2, gd2 = R.radius_search_rvv(R.PointCloudSoA(*soa(p)), (0, 0, 0), 2.0, mnn)
+        assert (gi2 == oi).all() and (gd2 == od).all()
+
+
+# ------------------------------------------------------------------------------------------------ kNN
+@pytest.mark.parametrize("k", [1, 11, 33, 64])
+def test_knn(k):
+    p = c0_small(2500, 15)
+    x, y, z = soa(p)
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(x, y, z), 0.03)
+    h.build()
+    q = np.concatenate([p[:200], np.random.default_rng(4).uniform(-0.5, 1.5, (50, 3)).astype(np.float32)])
+    gi, gd = h.knn(q, k)
+    oi, od = O.knn(x, y, z, *soa(q), k)
+    assert (gi == oi).all() and (gd == od).all()
+
+
+def test_knn_fewer_points_than_k():
+    p = gauss(20, 16)
+    x, y, z = soa(p)
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(x, y, z), 0.5)
+    h.build()
+    gi, gd = h.knn(p, 32)
+    oi, od = O.knn(x, y, z, x, y, z, 32)
+    assert (gi == oi).all() and (gd[:, :20] == od[:, :20]).all() and np.isinf(gd[:, 20:]).all()
+
+
+# ------------------------------------------------------------------------------------------------ SOR
+@pytest.mark.parametrize("k,alpha", [(10, 1.0), (32, 1.0), (50, 0.5), (20, 2.0)])
+def test_sor_exact(k, alpha):
+    p = c0_small(3000, 17)
+    out, means = R.sor_sc(p, k, alpha, with_means=True)
+    oout, omeans = O.sor_sc(p, k, alpha)
+    assert (means == omeans).all()
+    assert out.shape == oout.shape and (out == oout).all()
+    out2 = R.sor_rvv(R.PointCloudSoA(*soa(p)), k, alpha)
+    assert (out2 == oout).all()
+
+
+def test_sor_reference_unit_test_shape():
+    """tests/unit/test_sor.cpp input shape: 900 N(0,1) + 100 U(-20,20), k=10, alpha=1."""
+    rng = np.random.default_rng(42)
+    p = np.concatenate([rng.normal(0, 1, (900, 3)), rng.uniform(-20, 20, (100, 3))]).astype(np.float32)
+    out = R.sor_sc(p, 10, 1.0)
+    oout, _ = O.sor_sc(p, 10, 1.0)
+    assert (out == oout).all()
+
+
+@pytest.mark.parametrize("cell,r,k", [(0.02, 0.02, 10), (0.05, 0.05, 32), (0.5, 0.5, 20)])
+def test_sor_radius(cell, r, k):
+    p = c0_small(4000, 18)
+    x, y, z = soa(p)
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(x, y, z), cell)
+    h.build()
+    out, means = R.sor_spatial_hash(R.PointCloudSoA(x, y, z), h, k, 1.0, r, with_means=True)
+    oout, omeans = O.sor_radius(x, y, z, cell, k, 1.0, r)
+    assert (means == omeans).all()
+    assert (out == oout).all()
+
+
+# ------------------------------------------------------------------------------------------------ V1
+@pytest.mark.parametrize("leaf", [0.01, 0.05, 0.3])
+def test_voxel(leaf):
+    p = c0_small(20000, 19)
+    v = R.voxel_grid_downsamp_sc(p, leaf)
+    ov = O.voxel_sc(p, leaf)
+    assert v.shape == ov.shape and (v == ov).all()
+
+
+def test_voxel_negative_and_wide_keys():
+    rng = np.random.default_rng(20)
+    p = np.concatenate([rng.uniform(-500, 500, (3000, 3)), rng.normal(0, 1, (3000, 3))]).astype(np.float32)
+    for leaf in (0.001, 0.5):             # 0.001 needs 64-bit keys
+        v = R.voxel_grid_downsamp_rvv_v2(R.PointCloudSoA(*soa(p)), leaf)
+        assert (v == O.voxel_sc(p, leaf)).all()
+
+
+def test_voxel_kat_kitti():
+    import os
+    xyz = np.load(os.path.join(os.path.dirname(__file__), "golden", "kitti_0000000000.npz"))["xyz"]
+    v = R.voxel_grid_downsamp_sc(xyz, 0.2)
+    assert len(v) == 18542
+    assert (v == O.voxel_sc(xyz, 0.2)).all()
+
+
+# ------------------------------------------------------------------------------------------------ R1 / X1
+@pytest.mark.parametrize("seed", [42, 1, 2])
+def test_ransac(seed):
+    rng = np.random.default_rng(seed)
+    p = np.concatenate([np.c_[rng.uniform(-10, 10, (500, 2)), rng.normal(10, 0.01, 500)],
+                        rng.uniform(-20, 20, (500, 3))]).astype(np.float32)
+    best, model = R.ransac_plane_sc(p, 0.1, 1000)
+    ob, om, _ = O.ransac_plane(*soa(p), 0.1, 1000)
+    assert best == ob and (model == om).all()
+    assert best >= 450 and abs(model[2]) > 0.9 and abs(model[3]) >= 9.0
+
+
+def test_ransac_large_max_iters_and_low_inlier_ratio():
+    p = c0_small(20000, 21)
+    for thr, iters in [(0.005, 500), (0.001, 3000)]:
+        best, model = R.ransac_plane_rvv(R.PointCloudSoA(*soa(p)), thr, iters)
+        ob, om, _ = O.ransac_plane(*soa(p), thr, iters)
+        assert best == ob and (model == om).all()
+
+
+def test_plane_extract():
+    p = c0_small(5000, 22)
+    m = np.array([0.0, 0.0, 1.0, -0.2], np.float32)
+    a, b = R.extract_plane_inliers_outliers_rvv(R.PointCloudSoA(*soa(p)), m, 0.005)
+    oa, ob = O.plane_extract(*soa(p), m, 0.005)
+    assert (a == oa).all() and (b == ob).all()
+
+
+# ------------------------------------------------------------------------------------------------ normals
+def _angles_deg(n, ref):
+    n = np.asarray(n, np.float64)
+    ref = np.asarray(ref, np.float64)
+    c = np.abs((n * ref).sum(1)) / np.maximum(np.linalg.norm(n, axis=1) * np.linalg.norm(ref, axis=1), 1e-300)
+    return np.degrees(np.arccos(np.clip(c, -1, 1)))
+
+
+@pytest.mark.parametrize("k", [10, 24, 30])
+def test_normals_knn_vs_fp64(k):
+    p = c0_small(3000, 23)
+    nx, ny, nz = R.normal_estimation_sc(p, k, 0.0)
+    ref = O.normals_knn_f64(*soa(p), k)
+    err = _angles_deg(np.c_[nx, ny, nz], ref)
+    assert np.median(err) <= 0.01 and np.percentile(err, 99) <= 1.0, (np.median(err), np.percentile(err, 99))
+    # flip rule toward the viewpoint (origin), evaluated in fp64
+    dots = (-p.astype(np.float64) * np.c_[nx, ny, nz]).sum(1)
+    assert (dots >= -1e-5).all()
+
+
+@pytest.mark.parametrize("r,max_nn", [(0.03, 64), (0.03, 0), (0.05, 16), (0.02, 100)])
+def test_normals_radius_vs_fp64(r, max_nn):
+    p = c0_small(3000, 24)
+    x, y, z = soa(p)
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(x, y, z), r)
+    h.build()
+    nx, ny, nz = R.normal_estimation_rvv(R.PointCloudSoA(x, y, z), h, 10, r, max_nn=max_nn)
+    ref = O.normals_radius_f64(x, y, z, r, r, max_nn)
+    zero_ref = np.linalg.norm(ref, axis=1) == 0
+    got = np.c_[nx, ny, nz]
+    assert ((np.linalg.norm(got, axis=1) == 0) == zero_ref).all()
+    err = _angles_deg(got[~zero_ref], ref[~zero_ref])
+    assert np.median(err) <= 0.01 and np.percentile(err, 99) <= 1.0, (np.median(err), np.percentile(err, 99))
+
+
+def test_normals_reference_unit_test_shape():
+    """tests/unit/test_normal.cpp: plane z ~ N(0, 0.01) over [-10,10]^2, k=10: >= 95% with |nz| > 0.9."""
+    rng = np.random.default_rng(42)
+    p = np.c_[rng.uniform(-10, 10, (1000, 2)), rng.normal(0, 0.01, 1000)].astype(np.float32)
+    nx, ny, nz = R.normal_estimation_sc(p, 10, 2.0)
+    assert (np.abs(nz) > 0.9).mean() >= 0.95
