@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Single LiDAR frame through the neighbour-search pipeline on one MI355X.
+
+Workload (one "step"): a 64-ring x 18,750-azimuth LiDAR frame (1.2M rays, ~1.04M returns, resident in HBM)
+  voxel grid 0.1 m -> exact-kNN SOR k=50 alpha=1 -> normals, radius 0.5 m capped at 64 neighbours
+  -> RANSAC plane, 1000 hypotheses, threshold 0.05 m
+Metric: input points per second of the whole pipeline (Mpts/s), with p50/p99 ms per frame.
+Also reported: the dominant kernel's HBM-roofline fraction (HIP events around its launches on its own
+stream) and the reference's scalar path timed on this host (oracle/_ref, or the oracle port).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per unit of work for each stage (compulsory traffic: inputs read once, outputs
+# written once).  unit = input point of the stage unless noted.
+STAGE_BYTES = {
+    "voxel": (20, 16),          # 12 read + 8 key per point, + 16 per voxel (centroid + count)
+    "sor": (41, 0),             # 12 query + 12 candidate + 4 mean + 1 mask + 12 * keep (upper bound)
+    "hash": (28, 16),           # 12 read + 12 sorted write + 4 index, + 16 per occupied cell
+    "normals": (36, 0),         # 12 query + 12 candidate + 12 normal
+    "ransac": (12, 0),          # 12 per point per launch evaluating all its hypotheses
+}
+KERNEL_STAGE = {
+    "voxel_keys_k": "voxel", "voxel_centroid_k": "voxel", "voxel_minmax_k": "voxel",
+    "sor_knn_means_k": "sor", "sor_stats_k": "sor", "sor_keep_k": "sor",
+    "keys_k": "hash", "gather_k": "hash", "cell_start_k": "hash",
+    "normals_topk_k": "normals", "plane_count_k": "ransac",
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sor-sample", type=int, default=30000, help="points of the O(N^2) SOR CPU sample")
+    ap.add_argument("--cpu-normals-sample", type=int, default=100000)
+    return ap.parse_args()
+
+
+class FramePipeline:
+    """voxel -> SOR -> normals -> RANSAC on device buffers, one frame per step."""
+
+    def __init__(self, frame_xyz, stream=None):
+        import rvpoint_amd as R
+        from rvpoint_amd._lib import call
+        self.R, self.call, self.s = R, call, stream
+        self.n = len(frame_xyz)
+        n = self.n
+        self.frame = R.DeviceCloud(frame_xyz)
+        D = R.DeviceArray
+        self.vox = D((n, 3)); self.vx, self.vy, self.vz = D(n), D(n), D(n)
+        self.sor = D((n, 3)); self.sx, self.sy, self.sz = D(n), D(n), D(n)
+        self.nx, self.ny, self.nz = D(n), D(n), D(n)
+        h = C.c_void_p()
+        call("rvp_hash_create", C.byref(h))
+        self.hash = h.value
+        self.leaf, self.k, self.alpha = 0.1, 50, 1.0
+        self.radius, self.max_nn, self.vp = 0.5, 64, (0.0, 0.0, 1.7)
+        self.thr, self.iters = 0.05, 1000
+
+    def step(self):
+        call, s = self.call, self.s
+        nv, ns = C.c_size_t(), C.c_size_t()
+        call("rvp_voxel", *self.frame.ptrs(), self.n, self.leaf, self.vox.ptr, C.byref(nv), s)
+        call("rvp_aos_to_soa", self.vox.ptr, nv.value, self.vx.ptr, self.vy.ptr, self.vz.ptr, s)
+        call("rvp_sor_exact", self.vx.ptr, self.vy.ptr, self.vz.ptr, nv.value, self.k, self.alpha, self.sor.ptr,
+             C.byref(ns), None, None, s)
+        call("rvp_aos_to_soa", self.sor.ptr, ns.value, self.sx.ptr, self.sy.ptr, self.sz.ptr, s)
+        call("rvp_hash_build", self.hash, self.sx.ptr, self.sy.ptr, self.sz.ptr, ns.value, self.radius, s)
+        call("rvp_normals_radius", self.hash, self.radius, self.max_nn, *self.vp, self.nx.ptr, self.ny.ptr,
+             self.nz.ptr, s)
+        model = (C.c_float * 4)()
+        inl, its = C.c_int(), C.c_int()
+        call("rvp_ransac_plane", self.sx.ptr, self.sy.ptr, self.sz.ptr, ns.value, self.thr, self.iters, 1e-6, 0.99,
+             model, C.byref(inl), C.byref(its), s)
+        return {"n_in": self.n, "n_voxel": nv.value, "n_sor": ns.value, "inliers": inl.value,
+                "ransac_iters": its.value, "model": [float(v) for v in model]}
+
+
+def cpu_baseline(frame, info, args):
+    """The reference's scalar path (oracle/_ref when present, else the oracle port) on a bounded sample of the
+    same workload, scaled to the full frame: voxel and RANSAC at full size, exact-kNN SOR on a subset scaled by
+    N^2, SpatialHash radius normals on a subset scaled by N."""
+    import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    pts = np.ascontiguousarray(frame, np.float32)
+    n = len(pts)
+    t = {}
+    if kind == "reference":
+        R = O.ref()
+        out = np.zeros_like(pts)
+        t0 = time.perf_counter()
+        nv = R.ref_voxel_sc(pts.reshape(-1), n, out.reshape(-1), 0.1)
+        t["voxel"] = time.perf_counter() - t0
+        vox = out[:nv].copy()
+    else:
+        t0 = time.perf_counter()
+        vox = O.voxel_sc(pts, 0.1)
+        t["voxel"] = time.perf_counter() - t0
+        nv = len(vox)
+    m = min(args.cpu_sor_sample, nv)
+    sub = np.ascontiguousarray(vox[np.random.default_rng(0).choice(nv, m, replace=False)])
+    t0 = time.perf_counter()
+    if kind == "reference":
+        o = np.zeros_like(sub)
+        O.ref().ref_sor_sc(sub.reshape(-1), m, o.reshape(-1), 50, 1.0)
+    else:
+        O.sor_sc(sub, 50, 1.0)
+    t["sor_sample"] = time.perf_counter() - t0
+    # sample density is lower than the full cloud's, so an N^2 scale is the honest upper-bound extrapolation
+    t["sor"] = t["sor_sample"] * (nv / m) ** 2
+    ns = info["n_sor"]
+    m2 = min(args.cpu_normals_sample, nv)
+    sx, sy, sz = (np.ascontiguousarray(vox[:m2, i]) for i in range(3))
+    t0 = time.perf_counter()
+    if kind == "reference":
+        nx, ny, nz = (np.zeros(m2, np.float32) for _ in range(3))
+        O.ref().ref_normal_hash(sx, sy, sz, m2, 0.5, nx, ny, nz, 10, 0.5, 0.0, 0.0, 1.7, 4)
+    else:
+        O.normals_radius_f64(sx, sy, sz, 0.5, 0.5, 64, (0.0, 0.0, 1.7))
+    t["normals_sample"] = time.perf_counter() - t0
+    t["normals"] = t["normals_sample"] * ns / m2
+    x, y, z = (np.ascontiguousarray(vox[:ns, i]) for i in range(3))
+    t0 = time.perf_counter()
+    if kind == "reference":
+        model = np.zeros(4, np.float32)
+        O.ref().ref_ransac_rvv(x, y, z, ns, 0.05, 1000, model, 1e-6, 0.99)
+    else:
+        O.ransac_plane(x, y, z, 0.05, 1000)
+    t["ransac"] = time.perf_counter() - t0
+    total = t["voxel"] + t["sor"] + t["normals"] + t["ransac"]
+    sampled = t["voxel"] + t["sor_sample"] + t["normals_sample"] + t["ransac"]
+    return {"value": round(n / total / 1e6, 6), "unit": "Mpts/s", "cores": 1, "kind": kind,
+            "sample": (f"voxel 0.1 full frame ({n} pts); sor_sc k=50 on {m} random voxels scaled by (n/m)^2 to "
+                       f"{nv}; SpatialHash radius-0.5 normals on {m2} pts scaled linearly to {ns}; ransac 1000 "
+                       f"hyps full; {sampled:.1f} s measured, {total:.1f} s extrapolated frame time"),
+            "stage_seconds": {k: round(v, 4) for k, v in t.items()}}
+
+
+def roofline(prof, info, steps):
+    """Dominant kernel by total time; achieved = algorithmic bytes per launch / mean launch duration."""
+    if not prof:
+        return None, {}
+    name, (launches, total_ms) = max(prof.items(), key=lambda kv: kv[1][1])
+    stage = KERNEL_STAGE.get(name)
+    units = {"voxel": (info["n_in"], info["n_voxel"]), "sor": (info["n_voxel"], 0),
+             "hash": (info["n_sor"], 0), "normals": (info["n_sor"], 0), "ransac": (info["n_sor"], 0)}
+    if stage is None:
+        return {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None, "traffic": None}, prof
+    per_pt, per_unit2 = STAGE_BYTES[stage]
+    u1, u2 = units[stage]
+    per_step_bytes = per_pt * u1 + per_unit2 * u2
+    launches_per_step = launches / steps
+    bytes_per_launch = per_step_bytes / launches_per_step
+    avg_ms = total_ms / launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+            "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_ms, 5),
+            "stage": stage, "launches": launches}, prof
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU, and "
+              "the multi-GPU split of this workload is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import rvpoint_amd as R
+    from rvpoint_amd import _lib, scenes
+    if not R.gpu_available():
+        print("no GPU visible to librvp_hip.so", file=sys.stderr)
+        sys.exit(1)
+    print("bench: generating LiDAR frame (64 rings x 18,750 azimuths)", file=sys.stderr, flush=True)
+    frame = scenes.lidar_frame()
+    pipe = FramePipeline(frame)
+    for _ in range(args.warmup):
+        info = pipe.step()
+    R.synchronize()
+    print(f"bench: warmup done ({args.warmup} steps); timing {args.steps}", file=sys.stderr, flush=True)
+    _lib.profile_enable(True)
+    times = []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        info = pipe.step()
+        times.append((time.perf_counter() - t0) * 1e3)
+    R.synchronize()
+    wall = time.perf_counter() - t_all
+    _lib.profile_enable(False)
+    prof = _lib.profile_collect()
+    ms = wall * 1e3 / args.steps
+    rl, prof = roofline(prof, info, args.steps)
+    line = {
+        "metric": "Mpts/s hash-grid radius/kNN+SOR+normals; batched vs per-cloud; p99 ms/frame",
+        "value": round(info["n_in"] / (ms * 1e-3) / 1e6, 4),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "p50_ms": round(float(np.percentile(times, 50)), 4),
+        "p99_ms": round(float(np.percentile(times, 99)), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "lidar_frame: 64 rings x 18750 azimuths (1.2M rays), voxel 0.1 -> SOR k=50 "
+                               "-> normals r=0.5 cap 64 -> RANSAC 1000 thr 0.05",
+                   "points": info["n_in"], "n_voxel": info["n_voxel"], "n_sor": info["n_sor"],
+                   "inliers": info["inliers"], "parallelism": "single GPU"},
+        "roofline": rl,
+        "kernels_ms_per_step": {k: round(v[1] / args.steps, 4) for k, v in sorted(prof.items(),
+                                                                                  key=lambda kv: -kv[1][1])[:12]},
+    }
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline (reference scalar path, 1 thread)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(frame, info, args)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -54,6 +54,24 @@ int rvp_event_destroy(void *ev);
 int rvp_event_record(void *ev, rvp_stream_t s);
 int rvp_event_elapsed(void *start, void *stop, float *ms);
 
+/* AoS float[3n] -> SoA x/y/z (the copy the reference's drivers make between stages,
+ * src/tools/pipeline_export.cpp:336-342) */
+int rvp_aos_to_soa(const float *aos, size_t n, float *x, float *y, float *z, rvp_stream_t s);
+/* Per-kernel timing: while enabled, every kernel launch of the library is bracketed by a HIP event pair on
+ * the stream it runs on.  collect() synchronises, writes "kernel launches total_ms" lines into buf
+ * (NUL-terminated, *needed = size required) and clears the records. */
+int rvp_profile_enable(int on);
+int rvp_profile_collect(char *buf, size_t buflen, size_t *needed);
+
+/* ---- D1 / D2 (src/core/rvv_common.cpp:9-107) ---------------------------------------------------------
+ * D1 get_dist_sq_rvv: out[i] = ((x-qx)^2 + (y-qy)^2) + (z-qz)^2, each op rounded once.
+ * D2 get_inds_in_radius_rvv: (subset[i], d2) for subset points with d2 <= r2, in subset order; *n_out host. */
+int rvp_dist_sq(const float *x, const float *y, const float *z, size_t n, float qx, float qy, float qz,
+                float *out, rvp_stream_t s);
+int rvp_inds_in_radius(const float *x, const float *y, const float *z, const int32_t *subset, size_t m, float qx,
+                       float qy, float qz, float r2, int32_t *out_idx, float *out_d2, size_t *n_out,
+                       rvp_stream_t s);
+
 /* ---- H1/H2: SpatialHash::setInputCloud + build (src/search/spatial_hashing.cpp:76-163) ------------ */
 int rvp_hash_create(rvp_hash **h);
 int rvp_hash_destroy(rvp_hash *h);

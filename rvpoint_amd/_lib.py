@@ -50,6 +50,11 @@ _SIGS = {
     "rvp_event_destroy": [vp],
     "rvp_event_record": [vp, vp],
     "rvp_event_elapsed": [vp, vp, pfl],
+    "rvp_aos_to_soa": [vp, sz, vp, vp, vp, vp],
+    "rvp_profile_enable": [i32],
+    "rvp_profile_collect": [C.c_char_p, sz, psz],
+    "rvp_dist_sq": [vp, vp, vp, sz, fl, fl, fl, vp, vp],
+    "rvp_inds_in_radius": [vp, vp, vp, vp, sz, fl, fl, fl, fl, vp, vp, psz, vp],
     "rvp_hash_create": [pvp],
     "rvp_hash_destroy": [vp],
     "rvp_hash_build": [vp, vp, vp, vp, sz, fl, vp],
@@ -101,6 +106,23 @@ def call(name, *args):
     if st != RVP_OK:
         raise RvpError(st, lib().rvp_last_error().decode())
     return st
+
+
+def profile_enable(on=True):
+    call("rvp_profile_enable", 1 if on else 0)
+
+
+def profile_collect():
+    """{kernel: (launches, total_ms)} of the launches recorded since the last collect."""
+    need = sz()
+    call("rvp_profile_collect", None, 0, C.byref(need))
+    buf = C.create_string_buffer(max(need.value, 1))
+    call("rvp_profile_collect", buf, len(buf), C.byref(need))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, n, ms = line.split()
+        out[name] = (int(n), float(ms))
+    return out
 
 
 def gpu_available() -> bool:

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib")
 OBJ = os.path.join(LIB, "obj")
 
-HIP_SOURCES = ["runtime.hip", "prims.hip", "grid.hip", "query.hip", "normals.hip", "voxel_ransac.hip"]
+HIP_SOURCES = ["runtime.hip", "prims.hip", "grid.hip", "query.hip", "normals.hip", "voxel_ransac.hip", "distance.hip"]
 CXX_SOURCES = ["rvpoint_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -56,6 +56,16 @@ def build(verbose: bool = False) -> str:
         _run(["g++", *CXX_FLAGS, "-shared", *srcs, "-o", tmp, "-L", LIB, "-lrvp_hip",
               "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, api_so)
+    # C++ programs that exercise the drop-in API the way the reference's own unit tests do
+    test_dir = os.path.join(ROOT, "tests", "cpp")
+    if os.path.isdir(test_dir) and os.path.exists(api_so):
+        out_dir = os.path.join(test_dir, "_build")
+        os.makedirs(out_dir, exist_ok=True)
+        for src in sorted(os.listdir(test_dir)):
+            if src.endswith(".cpp"):
+                exe = os.path.join(out_dir, src[:-4])
+                _run(["g++", *CXX_FLAGS, os.path.join(test_dir, src), "-o", exe, "-L", LIB, "-lrvpoint_amd",
+                      "-lrvp_hip", "-Wl,-rpath,$ORIGIN/../../../rvpoint_amd/lib"])
     if verbose:
         print("built", hip_so, api_so)
     return hip_so

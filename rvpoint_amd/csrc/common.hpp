@@ -91,6 +91,22 @@ struct Tmp {
 
 inline hipStream_t S(rvp_stream_t s) { return static_cast<hipStream_t>(s); }
 
+// ------------------------------------------------------------------------------------------------
+// optional per-kernel timing: when enabled (rvp_profile_enable), every RVP_LAUNCH records a HIP event
+// pair around the launch on its own stream; rvp_profile_collect sums the elapsed times per kernel.
+// ------------------------------------------------------------------------------------------------
+bool prof_enabled();
+hipEvent_t prof_start(hipStream_t s);
+void prof_stop(const char *name, hipStream_t s, hipEvent_t start);
+
+#define RVP_LAUNCH(name, stream, ...)                                                              \
+  do {                                                                                             \
+    hipEvent_t ev_ = ::rvp::prof_enabled() ? ::rvp::prof_start(stream) : nullptr;                  \
+    __VA_ARGS__;                                                                                   \
+    if (ev_) ::rvp::prof_stop(name, stream, ev_);                                                  \
+    RVP_LAUNCHED();                                                                                \
+  } while (0)
+
 constexpr int WAVE = 64;
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

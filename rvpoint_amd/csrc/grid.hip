@@ -68,10 +68,8 @@ int bbox_minmax(const float *x, const float *y, const float *z, size_t n, float 
   const int nb = (int)std::min<size_t>(1024, std::max<size_t>(1, ceil_div(n, 256 * 8)));
   Tmp part;
   RVP_TRY(part.alloc(nb * 6 * sizeof(float), s));
-  bbox_partial_k<<<nb, 256, 0, s>>>(x, y, z, n, part.as<float>());
-  RVP_LAUNCHED();
-  bbox_final_k<<<1, 64, 0, s>>>(part.as<float>(), nb, out6);
-  RVP_LAUNCHED();
+  RVP_LAUNCH("bbox_partial_k", s, bbox_partial_k<<<nb, 256, 0, s>>>(x, y, z, n, part.as<float>()));
+  RVP_LAUNCH("bbox_final_k", s, bbox_final_k<<<1, 64, 0, s>>>(part.as<float>(), nb, out6));
   return RVP_OK;
 }
 
@@ -212,8 +210,7 @@ int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, fl
   RVP_TRY(mm.alloc(6 * sizeof(float), s));
   RVP_TRY(hd.alloc(sizeof(Header), s));
   RVP_TRY(bbox_minmax(x, y, z, n, mm.as<float>(), s));
-  header_k<<<1, 64, 0, s>>>(mm.as<float>(), cell_, hd.as<Header>());
-  RVP_LAUNCHED();
+  RVP_LAUNCH("header_k", s, header_k<<<1, 64, 0, s>>>(mm.as<float>(), cell_, hd.as<Header>()));
   Header h;
   RVP_HIP(hipMemcpyAsync(&h, hd.p, sizeof(Header), hipMemcpyDeviceToHost, s));
   RVP_HIP(hipStreamSynchronize(s));
@@ -247,8 +244,8 @@ int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, fl
   RVP_TRY(kt.alloc(n * sizeof(uint32_t), s));
   RVP_TRY(vt.alloc(n * sizeof(uint32_t), s));
   const unsigned nb = ceil_div(n, 256);
-  keys_k<<<nb, 256, 0, s>>>(x, y, z, n, bb[0], bb[1], bb[2], cell, dims[0], dims[1], dims[2], keys.as<uint32_t>(),
-                            perm.as<uint32_t>());
+  RVP_LAUNCH("keys_k", s, keys_k<<<nb, 256, 0, s>>>(x, y, z, n, bb[0], bb[1], bb[2], cell, dims[0], dims[1], dims[2], keys.as<uint32_t>(),
+                            perm.as<uint32_t>()));
   RVP_LAUNCHED();
   uint32_t *kp = keys.as<uint32_t>(), *vp = perm.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
   RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n, bits_for(ncells - 1), s));
@@ -259,8 +256,7 @@ int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, fl
   RVP_TRY(xs.alloc(n * sizeof(float), s, persistent));
   RVP_TRY(ys.alloc(n * sizeof(float), s, persistent));
   RVP_TRY(zs.alloc(n * sizeof(float), s, persistent));
-  gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), n, xs.as<float>(), ys.as<float>(), zs.as<float>());
-  RVP_LAUNCHED();
+  RVP_LAUNCH("gather_k", s, gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), n, xs.as<float>(), ys.as<float>(), zs.as<float>()));
 
   RVP_TRY(cstart.alloc((ncells + 1) * sizeof(uint32_t), s, persistent));
   const uint32_t gap_cap = (uint32_t)std::min<size_t>(n + 2, 1u << 22);
@@ -268,11 +264,10 @@ int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, fl
   RVP_TRY(gaps.alloc(gap_cap * sizeof(Gap), s));
   RVP_TRY(ctr.alloc(2 * sizeof(uint32_t), s));
   RVP_HIP(hipMemsetAsync(ctr.p, 0, 2 * sizeof(uint32_t), s));
-  cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), n, (uint32_t)ncells, cstart.as<uint32_t>(),
-                                  ctr.as<uint32_t>(), gaps.as<Gap>(), ctr.as<uint32_t>() + 1, gap_cap);
+  RVP_LAUNCH("cell_start_k", s, cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), n, (uint32_t)ncells, cstart.as<uint32_t>(),
+                                  ctr.as<uint32_t>(), gaps.as<Gap>(), ctr.as<uint32_t>() + 1, gap_cap));
   RVP_LAUNCHED();
-  fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<Gap>(), ctr.as<uint32_t>() + 1, gap_cap, cstart.as<uint32_t>());
-  RVP_LAUNCHED();
+  RVP_LAUNCH("fill_gaps_k", s, fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<Gap>(), ctr.as<uint32_t>() + 1, gap_cap, cstart.as<uint32_t>()));
   uint32_t hc[2];
   RVP_HIP(hipMemcpyAsync(hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost, s));
   RVP_HIP(hipStreamSynchronize(s));
@@ -396,8 +391,8 @@ int rvp_hash_cell_coords(const rvp_hash *h, int32_t *ixyz, int64_t *keys, rvp_st
   RVP_CHECK(h->g.built, RVP_E_STATE, "hash not built");
   const Grid &g = h->g;
   if (g.n == 0) return RVP_OK;
-  ref_cells_k<<<ceil_div(g.n, 256), 256, 0, S(s)>>>(g.x, g.y, g.z, g.n, g.bb[0], g.bb[1], g.bb[2], g.req_cell,
-                                                   ixyz, keys);
+  RVP_LAUNCH("ref_cells_k", S(s), ref_cells_k<<<ceil_div(g.n, 256), 256, 0, S(s)>>>(g.x, g.y, g.z, g.n, g.bb[0], g.bb[1], g.bb[2], g.req_cell,
+                                                   ixyz, keys));
   RVP_LAUNCHED();
   RVP_HIP(hipStreamSynchronize(S(s)));
   return RVP_OK;

@@ -314,8 +314,8 @@ int rvp_normals_knn(const float *x, const float *y, const float *z, size_t n, in
   Tmp fb;
   RVP_TRY(fb.alloc(sizeof(unsigned long long), s));
   RVP_HIP(hipMemsetAsync(fb.p, 0, sizeof(unsigned long long), s));
-  normals_topk_k<0><<<ceil_div(n, QW * WAVE), QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz, nx, ny, nz,
-                                                          fb.as<unsigned long long>());
+  RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0><<<ceil_div(n, QW * WAVE), QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz, nx, ny, nz,
+                                                          fb.as<unsigned long long>()));
   RVP_LAUNCHED();
   return finish_fallbacks(fb.as<unsigned long long>(), s);
 }
@@ -333,8 +333,8 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
   RVP_HIP(hipMemsetAsync(fb.p, 0, sizeof(unsigned long long), s));
   const GridView g = h->g.view();
   if (max_nn > 0 && max_nn <= MAX_TOPK) {
-    normals_topk_k<1><<<ceil_div(n, QW * WAVE), QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx, ny, nz,
-                                                            fb.as<unsigned long long>());
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1><<<ceil_div(n, QW * WAVE), QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx, ny, nz,
+                                                            fb.as<unsigned long long>()));
     RVP_LAUNCHED();
   } else {
     // unlimited (or a cap above 64): full lists, sorted by (d2, idx), truncated by the row offsets
@@ -350,8 +350,8 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
                        pos.as<uint32_t>(), s));
     // a cap keeps the first min(len, max_nn) entries of each (d2, idx)-sorted row
     const int cap = max_nn > 0 ? max_nn : 0x7fffffff;
-    normals_csr_k<<<ceil_div(n, QW * WAVE), QT, 0, s>>>(g, off.as<int64_t>(), cap, pos.as<uint32_t>(), vpx, vpy,
-                                                        vpz, nx, ny, nz, fb.as<unsigned long long>());
+    RVP_LAUNCH("normals_csr_k", s, normals_csr_k<<<ceil_div(n, QW * WAVE), QT, 0, s>>>(g, off.as<int64_t>(), cap, pos.as<uint32_t>(), vpx, vpy,
+                                                        vpz, nx, ny, nz, fb.as<unsigned long long>()));
     RVP_LAUNCHED();
   }
   return finish_fallbacks(fb.as<unsigned long long>(), s);

@@ -72,12 +72,9 @@ static int exclusive_scan_impl(const uint32_t *in, Out *out, size_t n, Out *tota
   const size_t nb = ceil_div(n, SCAN_TILE);
   Tmp part;
   RVP_TRY(part.alloc(nb * sizeof(Out), s));
-  scan_reduce_k<Out><<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, n, part.as<Out>());
-  RVP_LAUNCHED();
-  scan_partials_k<Out><<<1, PRIM_THREADS, 0, s>>>(part.as<Out>(), nb, total_dev);
-  RVP_LAUNCHED();
-  scan_down_k<Out><<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, out, n, part.as<Out>());
-  RVP_LAUNCHED();
+  RVP_LAUNCH("scan_reduce_k", s, scan_reduce_k<Out><<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, n, part.as<Out>()));
+  RVP_LAUNCH("scan_partials_k", s, scan_partials_k<Out><<<1, PRIM_THREADS, 0, s>>>(part.as<Out>(), nb, total_dev));
+  RVP_LAUNCH("scan_down_k", s, scan_down_k<Out><<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, out, n, part.as<Out>()));
   return RVP_OK;
 }
 
@@ -170,11 +167,9 @@ static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, siz
   Tmp hist;
   RVP_TRY(hist.alloc((size_t)256 * nb * sizeof(uint32_t), s));
   for (int shift = 0; shift < bits; shift += 8) {
-    radix_hist_k<K><<<nb, PRIM_THREADS, 0, s>>>(*keys, n, shift, hist.as<uint32_t>(), nb);
-    RVP_LAUNCHED();
+    RVP_LAUNCH("radix_hist_k", s, radix_hist_k<K><<<nb, PRIM_THREADS, 0, s>>>(*keys, n, shift, hist.as<uint32_t>(), nb));
     RVP_TRY(exclusive_scan_u32(hist.as<uint32_t>(), hist.as<uint32_t>(), (size_t)256 * nb, nullptr, s));
-    radix_scatter_k<K><<<nb, PRIM_THREADS, 0, s>>>(*keys, *vals, *kt, *vt, n, shift, hist.as<uint32_t>(), nb);
-    RVP_LAUNCHED();
+    RVP_LAUNCH("radix_scatter_k", s, radix_scatter_k<K><<<nb, PRIM_THREADS, 0, s>>>(*keys, *vals, *kt, *vt, n, shift, hist.as<uint32_t>(), nb));
     std::swap(*keys, *kt);
     std::swap(*vals, *vt);
   }
@@ -219,12 +214,10 @@ int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_
   Tmp f;
   RVP_TRY(f.alloc(n * sizeof(uint32_t), s));
   const unsigned nb = ceil_div(n, 256);
-  flag_eq_k<<<nb, 256, 0, s>>>(flags, want, n, f.as<uint32_t>());
-  RVP_LAUNCHED();
+  RVP_LAUNCH("flag_eq_k", s, flag_eq_k<<<nb, 256, 0, s>>>(flags, want, n, f.as<uint32_t>()));
   RVP_TRY(exclusive_scan_u32(f.as<uint32_t>(), f.as<uint32_t>(), n, count_dev, s));
   if (out_aos) {
-    scatter_aos_k<<<nb, 256, 0, s>>>(x, y, z, flags, want, n, f.as<uint32_t>(), out_aos);
-    RVP_LAUNCHED();
+    RVP_LAUNCH("scatter_aos_k", s, scatter_aos_k<<<nb, 256, 0, s>>>(x, y, z, flags, want, n, f.as<uint32_t>(), out_aos));
   }
   return RVP_OK;
 }

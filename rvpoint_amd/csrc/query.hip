@@ -400,15 +400,13 @@ static int sor_tail(const Grid &grid, const float *x, const float *y, const floa
                     float *means, float *out_aos, size_t *n_out, uint8_t *keep, hipStream_t s) {
   Tmp stats, keep_t, cnt;
   RVP_TRY(stats.alloc(3 * sizeof(float), s));
-  sor_stats_k<<<1, 64, 0, s>>>(means, (uint32_t)n, alpha, stats.as<float>());
-  RVP_LAUNCHED();
+  RVP_LAUNCH("sor_stats_k", s, sor_stats_k<<<1, 64, 0, s>>>(means, (uint32_t)n, alpha, stats.as<float>()));
   uint8_t *kp = keep;
   if (!kp) {
     RVP_TRY(keep_t.alloc(n, s));
     kp = keep_t.as<uint8_t>();
   }
-  sor_keep_k<<<ceil_div(n, 256), 256, 0, s>>>(means, (uint32_t)n, stats.as<float>(), kp);
-  RVP_LAUNCHED();
+  RVP_LAUNCH("sor_keep_k", s, sor_keep_k<<<ceil_div(n, 256), 256, 0, s>>>(means, (uint32_t)n, stats.as<float>(), kp));
   RVP_TRY(cnt.alloc(sizeof(uint32_t), s));
   RVP_TRY(compact_xyz_aos(x, y, z, kp, 1, n, out_aos, cnt.as<uint32_t>(), s));
   uint32_t m = 0;
@@ -433,13 +431,10 @@ int radius_csr(const Grid &G, const float *qx, const float *qy, const float *qz,
   const GridView g = G.view();
   const bool self = qx == nullptr;
   const unsigned nb = ceil_div(nq, QW);
-  if (self) radius_fill_k<true><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, r, offsets, keys, pos);
-  else radius_fill_k<false><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, r, offsets, keys, pos);
-  RVP_LAUNCHED();
-  segsort_small_k<<<nb, QT, 0, s>>>(offsets, (uint32_t)nq, keys, pos);
-  RVP_LAUNCHED();
-  segsort_large_k<<<std::min<unsigned>(ceil_div(nq, 1), 4096), 256, 0, s>>>(offsets, (uint32_t)nq, keys, pos);
-  RVP_LAUNCHED();
+  if (self) RVP_LAUNCH("radius_fill_k", s, radius_fill_k<true><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, r, offsets, keys, pos));
+  else RVP_LAUNCH("radius_fill_k", s, radius_fill_k<false><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, r, offsets, keys, pos));
+  RVP_LAUNCH("segsort_small_k", s, segsort_small_k<<<nb, QT, 0, s>>>(offsets, (uint32_t)nq, keys, pos));
+  RVP_LAUNCH("segsort_large_k", s, segsort_large_k<<<std::min<unsigned>(ceil_div(nq, 1), 4096), 256, 0, s>>>(offsets, (uint32_t)nq, keys, pos));
   return RVP_OK;
 }
 
@@ -447,9 +442,8 @@ static int radius_counts(const Grid &G, const float *qx, const float *qy, const 
                          int32_t *counts, hipStream_t s) {
   const GridView g = G.view();
   const unsigned nb = ceil_div(nq, QW);
-  if (!qx) radius_count_k<true><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, r, counts);
-  else radius_count_k<false><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, r, counts);
-  RVP_LAUNCHED();
+  if (!qx) RVP_LAUNCH("radius_count_k", s, radius_count_k<true><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, r, counts));
+  else RVP_LAUNCH("radius_count_k", s, radius_count_k<false><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, r, counts));
   return RVP_OK;
 }
 
@@ -482,10 +476,10 @@ int rvp_radius_search(const rvp_hash *h, const float *qx, const float *qy, const
   const GridView g = h->g.view();
   if (max_nn <= MAX_TOPK) {
     const unsigned nb = ceil_div(nq, QW);
-    if (!qx) radius_topk_k<true><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, radius, max_nn, idx_out, d2_out,
-                                                   counts, total);
-    else radius_topk_k<false><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, radius, max_nn, idx_out, d2_out,
-                                                counts, total);
+    if (!qx) RVP_LAUNCH("radius_topk_k", s, radius_topk_k<true><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, radius, max_nn, idx_out, d2_out,
+                                                   counts, total));
+    else RVP_LAUNCH("radius_topk_k", s, radius_topk_k<false><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, radius, max_nn, idx_out, d2_out,
+                                                counts, total));
     RVP_LAUNCHED();
   } else {
     Tmp cnt, off, keys;
@@ -497,8 +491,8 @@ int rvp_radius_search(const rvp_hash *h, const float *qx, const float *qy, const
     RVP_TRY(keys.alloc((size_t)tot * sizeof(uint64_t), s));
     RVP_TRY(radius_csr(h->g, qx, qy, qz, nq, radius, off.as<int64_t>(), keys.as<uint64_t>(), nullptr, s));
     // CSR rows are indexed by output row already (self mode writes at the original index)
-    truncate_rows_k<<<(unsigned)nq, 64, 0, s>>>(keys.as<uint64_t>(), off.as<int64_t>(), (uint32_t)nq, max_nn,
-                                                idx_out, d2_out, counts, total);
+    RVP_LAUNCH("truncate_rows_k", s, truncate_rows_k<<<(unsigned)nq, 64, 0, s>>>(keys.as<uint64_t>(), off.as<int64_t>(), (uint32_t)nq, max_nn,
+                                                idx_out, d2_out, counts, total));
     RVP_LAUNCHED();
   }
   RVP_HIP(hipStreamSynchronize(s));
@@ -528,8 +522,7 @@ int rvp_radius_fill(const rvp_hash *h, const float *qx, const float *qy, const f
   RVP_TRY(keys.alloc((size_t)tot * sizeof(uint64_t), s));
   RVP_TRY(radius_csr(h->g, qx, qy, qz, nq, radius, offsets, keys.as<uint64_t>(), nullptr, s));
   if (tot > 0) {
-    unpack_rows_k<<<ceil_div((size_t)tot, 256), 256, 0, s>>>(keys.as<uint64_t>(), (size_t)tot, idx_out, d2_out);
-    RVP_LAUNCHED();
+    RVP_LAUNCH("unpack_rows_k", s, unpack_rows_k<<<ceil_div((size_t)tot, 256), 256, 0, s>>>(keys.as<uint64_t>(), (size_t)tot, idx_out, d2_out));
   }
   RVP_HIP(hipStreamSynchronize(s));
   return RVP_OK;
@@ -543,8 +536,8 @@ int rvp_radius_bruteforce(const float *x, const float *y, const float *z, size_t
   RVP_CHECK(n < 0xffffffffull && nq < 0x7fffffffull, RVP_E_UNSUPPORTED, "size");
   if (nq == 0) return RVP_OK;
   RVP_TRY(pool_init());
-  bruteforce_k<<<ceil_div(nq, QW), QT, 0, S(st)>>>(x, y, z, (uint32_t)n, qx, qy, qz, (uint32_t)nq, radius, max_nn,
-                                                   idx_out, d2_out, counts);
+  RVP_LAUNCH("bruteforce_k", S(st), bruteforce_k<<<ceil_div(nq, QW), QT, 0, S(st)>>>(x, y, z, (uint32_t)n, qx, qy, qz, (uint32_t)nq, radius, max_nn,
+                                                   idx_out, d2_out, counts));
   RVP_LAUNCHED();
   RVP_HIP(hipStreamSynchronize(S(st)));
   return RVP_OK;
@@ -557,9 +550,8 @@ int rvp_knn(const rvp_hash *h, const float *qx, const float *qy, const float *qz
   if (nq == 0) return RVP_OK;
   const GridView g = h->g.view();
   const unsigned nb = ceil_div(nq, QW);
-  if (!qx) knn_k<true><<<nb, QT, 0, S(st)>>>(g, qx, qy, qz, (uint32_t)nq, k, idx_out, d2_out);
-  else knn_k<false><<<nb, QT, 0, S(st)>>>(g, qx, qy, qz, (uint32_t)nq, k, idx_out, d2_out);
-  RVP_LAUNCHED();
+  if (!qx) RVP_LAUNCH("knn_k", S(st), knn_k<true><<<nb, QT, 0, S(st)>>>(g, qx, qy, qz, (uint32_t)nq, k, idx_out, d2_out));
+  else RVP_LAUNCH("knn_k", S(st), knn_k<false><<<nb, QT, 0, S(st)>>>(g, qx, qy, qz, (uint32_t)nq, k, idx_out, d2_out));
   RVP_HIP(hipStreamSynchronize(S(st)));
   return RVP_OK;
 }
@@ -580,8 +572,7 @@ int rvp_sor_exact(const float *x, const float *y, const float *z, size_t n, int 
     RVP_TRY(mt.alloc(n * sizeof(float), s));
     mp = mt.as<float>();
   }
-  sor_knn_means_k<<<ceil_div(n, QW), QT, 0, s>>>(G.view(), k, mp);
-  RVP_LAUNCHED();
+  RVP_LAUNCH("sor_knn_means_k", s, sor_knn_means_k<<<ceil_div(n, QW), QT, 0, s>>>(G.view(), k, mp));
   return sor_tail(G, x, y, z, n, alpha, mp, out_aos, n_out, keep, s);
 }
 
@@ -600,8 +591,7 @@ int rvp_sor_radius(const rvp_hash *h, int k, float alpha, float search_radius, f
     RVP_TRY(mt.alloc(n * sizeof(float), s));
     mp = mt.as<float>();
   }
-  sor_radius_means_k<<<ceil_div(n, QW), QT, 0, s>>>(h->g.view(), k, search_radius, mp);
-  RVP_LAUNCHED();
+  RVP_LAUNCH("sor_radius_means_k", s, sor_radius_means_k<<<ceil_div(n, QW), QT, 0, s>>>(h->g.view(), k, search_radius, mp));
   return sor_tail(h->g, h->g.x, h->g.y, h->g.z, n, alpha, mp, out_aos, n_out, keep, s);
 }
 

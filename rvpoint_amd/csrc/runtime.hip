@@ -1,5 +1,10 @@
 // runtime.hip -- error state, memory pool set-up and the thin runtime helpers of the C ABI.
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <map>
 #include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "prims.hpp"
@@ -31,6 +36,38 @@ int pool_init() {
   RVP_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
   done[dev] = true;
   return RVP_OK;
+}
+
+// ---- kernel timer ---------------------------------------------------------------------------------
+struct ProfRec {
+  const char *name;
+  hipEvent_t a, b;
+};
+static std::mutex g_prof_mu;
+static std::vector<ProfRec> g_prof;
+static std::atomic<bool> g_prof_on{false};
+
+bool prof_enabled() { return g_prof_on.load(std::memory_order_relaxed); }
+
+hipEvent_t prof_start(hipStream_t s) {
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  if (hipEventRecord(e, s) != hipSuccess) {
+    (void)hipEventDestroy(e);
+    return nullptr;
+  }
+  return e;
+}
+
+void prof_stop(const char *name, hipStream_t s, hipEvent_t a) {
+  hipEvent_t b = nullptr;
+  if (hipEventCreate(&b) != hipSuccess || hipEventRecord(b, s) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof.push_back(ProfRec{name, a, b});
 }
 
 }  // namespace rvp
@@ -154,4 +191,63 @@ int rvp_exclusive_scan(const int32_t *counts, size_t n, int64_t *offsets, int64_
   return RVP_OK;
 }
 
+int rvp_profile_enable(int on) {
+  g_prof_on.store(on != 0);
+  return RVP_OK;
+}
+
+int rvp_profile_collect(char *buf, size_t buflen, size_t *needed) {
+  std::vector<ProfRec> recs;
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    recs.swap(g_prof);
+  }
+  std::map<std::string, std::pair<long, double>> acc;
+  int status = RVP_OK;
+  for (auto &r : recs) {
+    float ms = 0.f;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      auto &e = acc[r.name];
+      e.first += 1;
+      e.second += ms;
+    } else {
+      status = fail(RVP_E_DEVICE, "profile: event query failed");
+    }
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  std::string out;
+  for (auto &kv : acc) {
+    char line[256];
+    snprintf(line, sizeof(line), "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+    out += line;
+  }
+  if (needed) *needed = out.size() + 1;
+  if (buf && buflen) {
+    const size_t m = std::min(buflen - 1, out.size());
+    memcpy(buf, out.data(), m);
+    buf[m] = 0;
+  }
+  return status;
+}
+
+// AoS (PointXYZ[n]) -> SoA, the copy the reference's drivers make between stages (pipeline_export.cpp:336-342)
 }  // extern "C"
+
+namespace rvp {
+__global__ void aos_to_soa_k(const float *__restrict__ aos, size_t n, float *__restrict__ x, float *__restrict__ y,
+                             float *__restrict__ z) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = aos[3 * i];
+  y[i] = aos[3 * i + 1];
+  z[i] = aos[3 * i + 2];
+}
+}  // namespace rvp
+
+extern "C" int rvp_aos_to_soa(const float *aos, size_t n, float *x, float *y, float *z, rvp_stream_t st) {
+  if (n == 0) return RVP_OK;
+  RVP_CHECK(aos && x && y && z, RVP_E_ARG, "null pointer");
+  RVP_LAUNCH("aos_to_soa", S(st), aos_to_soa_k<<<ceil_div(n, 256), 256, 0, S(st)>>>(aos, n, x, y, z));
+  return RVP_OK;
+}

@@ -104,8 +104,8 @@ static int voxel_sorted(const float *x, const float *y, const float *z, size_t n
   RVP_TRY(kt.alloc(n * sizeof(K), s));
   RVP_TRY(vt.alloc(n * sizeof(uint32_t), s));
   const unsigned nb = ceil_div(n, 256);
-  voxel_keys_k<K><<<nb, 256, 0, s>>>(x, y, z, n, inv, mm[0], mm[1], mm[2], (K)ry, (K)rz, keys.as<K>(),
-                                     vals.as<uint32_t>());
+  RVP_LAUNCH("voxel_keys_k", s, voxel_keys_k<K><<<nb, 256, 0, s>>>(x, y, z, n, inv, mm[0], mm[1], mm[2], (K)ry, (K)rz, keys.as<K>(),
+                                     vals.as<uint32_t>()));
   RVP_LAUNCHED();
   K *kp = keys.as<K>(), *ktp = kt.as<K>();
   uint32_t *vp = vals.as<uint32_t>(), *vtp = vt.as<uint32_t>();
@@ -113,14 +113,11 @@ static int voxel_sorted(const float *x, const float *y, const float *z, size_t n
   else RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n, bits, s));
   RVP_TRY(vid.alloc(n * sizeof(uint32_t), s));
   RVP_TRY(nv.alloc(sizeof(uint32_t), s));
-  run_flags_k<K><<<nb, 256, 0, s>>>(kp, n, vid.as<uint32_t>());
-  RVP_LAUNCHED();
+  RVP_LAUNCH("run_flags_k", s, run_flags_k<K><<<nb, 256, 0, s>>>(kp, n, vid.as<uint32_t>()));
   RVP_TRY(exclusive_scan_u32(vid.as<uint32_t>(), vid.as<uint32_t>(), n, nv.as<uint32_t>(), s));
   RVP_TRY(start.alloc(n * sizeof(uint32_t), s));
-  run_starts_k<K><<<nb, 256, 0, s>>>(kp, n, vid.as<uint32_t>(), start.as<uint32_t>());
-  RVP_LAUNCHED();
-  voxel_centroid_k<<<nb, 256, 0, s>>>(x, y, z, vp, start.as<uint32_t>(), nv.as<uint32_t>(), (uint32_t)n, out);
-  RVP_LAUNCHED();
+  RVP_LAUNCH("run_starts_k", s, run_starts_k<K><<<nb, 256, 0, s>>>(kp, n, vid.as<uint32_t>(), start.as<uint32_t>()));
+  RVP_LAUNCH("voxel_centroid_k", s, voxel_centroid_k<<<nb, 256, 0, s>>>(x, y, z, vp, start.as<uint32_t>(), nv.as<uint32_t>(), (uint32_t)n, out));
   uint32_t V = 0;
   RVP_HIP(hipMemcpyAsync(&V, nv.p, sizeof(V), hipMemcpyDeviceToHost, s));
   RVP_HIP(hipStreamSynchronize(s));
@@ -236,8 +233,7 @@ static int plane_counts_dev(const float *x, const float *y, const float *z, size
   RVP_HIP(hipMemsetAsync(counts, 0, (size_t)H * sizeof(int32_t), s));
   if (n == 0 || H == 0) return RVP_OK;
   dim3 grid(ceil_div(n, RC_TILE), ceil_div((size_t)H, RC_HYP));
-  plane_count_k<<<grid, 256, 0, s>>>(x, y, z, (uint32_t)n, models, valid, H, thr, counts);
-  RVP_LAUNCHED();
+  RVP_LAUNCH("plane_count_k", s, plane_count_k<<<grid, 256, 0, s>>>(x, y, z, (uint32_t)n, models, valid, H, thr, counts));
   return RVP_OK;
 }
 
@@ -270,11 +266,9 @@ int rvp_voxel(const float *x, const float *y, const float *z, size_t n, float le
   const float inv = 1.0f / leaf;  // voxel_grid_downsamp.cpp:189 (IEEE division on the host)
   Tmp mmd;
   RVP_TRY(mmd.alloc(6 * sizeof(int), s));
-  voxel_minmax_init_k<<<1, 64, 0, s>>>(mmd.as<int>());
-  RVP_LAUNCHED();
+  RVP_LAUNCH("voxel_minmax_init_k", s, voxel_minmax_init_k<<<1, 64, 0, s>>>(mmd.as<int>()));
   const int nb = (int)std::min<size_t>(1024, ceil_div(n, 256 * 8));
-  voxel_minmax_k<<<nb, 256, 0, s>>>(x, y, z, n, inv, mmd.as<int>());
-  RVP_LAUNCHED();
+  RVP_LAUNCH("voxel_minmax_k", s, voxel_minmax_k<<<nb, 256, 0, s>>>(x, y, z, n, inv, mmd.as<int>()));
   int mm[6];
   RVP_HIP(hipMemcpyAsync(mm, mmd.p, sizeof(mm), hipMemcpyDeviceToHost, s));
   RVP_HIP(hipStreamSynchronize(s));
@@ -345,8 +339,8 @@ int rvp_ransac_plane(const float *x, const float *y, const float *z, size_t n, f
     RVP_TRY(valid_d.alloc((size_t)H * sizeof(int32_t), s));
     RVP_TRY(counts_d.alloc((size_t)H * sizeof(int32_t), s));
     RVP_HIP(hipMemcpyAsync(idx_d.p, idx3.data(), idx3.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-    plane_models_k<<<ceil_div(H, 64), 64, 0, s>>>(x, y, z, idx_d.as<int32_t>(), H, collinear_thresh,
-                                                  models_d.as<float>(), valid_d.as<int32_t>());
+    RVP_LAUNCH("plane_models_k", s, plane_models_k<<<ceil_div(H, 64), 64, 0, s>>>(x, y, z, idx_d.as<int32_t>(), H, collinear_thresh,
+                                                  models_d.as<float>(), valid_d.as<int32_t>()));
     RVP_LAUNCHED();
     RVP_TRY(plane_counts_dev(x, y, z, n, models_d.as<float>(), valid_d.as<int32_t>(), H, dist_thresh,
                              counts_d.as<int32_t>(), s));
@@ -397,8 +391,7 @@ int rvp_plane_extract(const float *x, const float *y, const float *z, size_t n, 
   Tmp f, c;
   RVP_TRY(f.alloc(n, s));
   RVP_TRY(c.alloc(2 * sizeof(uint32_t), s));
-  plane_flags_k<<<ceil_div(n, 256), 256, 0, s>>>(x, y, z, n, m[0], m[1], m[2], m[3], t, f.as<uint8_t>());
-  RVP_LAUNCHED();
+  RVP_LAUNCH("plane_flags_k", s, plane_flags_k<<<ceil_div(n, 256), 256, 0, s>>>(x, y, z, n, m[0], m[1], m[2], m[3], t, f.as<uint8_t>()));
   RVP_TRY(compact_xyz_aos(x, y, z, f.as<uint8_t>(), 1, n, in_aos, c.as<uint32_t>(), s));
   RVP_TRY(compact_xyz_aos(x, y, z, f.as<uint8_t>(), 0, n, out_aos, c.as<uint32_t>() + 1, s));
   uint32_t h[2];
