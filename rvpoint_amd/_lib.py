@@ -121,7 +121,8 @@ def profile_collect():
     out = {}
     for line in buf.value.decode().splitlines():
         name, n, ms = line.split()
-        out[name] = (int(n), float(ms))
+        c, t = out.get(name, (0, 0.0))
+        out[name] = (c + int(n), t + float(ms))
     return out
 
 

@@ -21,7 +21,7 @@ ARCH = os.environ.get("RVP_OFFLOAD_ARCH", "gfx950")
 # -ffp-contract=off: every fp32 op of the reference path rounds once (no FMA contraction).
 # Packed fp32 VALU forms are disabled for the whole library.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result",
+             "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result",
              "-I", os.path.join(ROOT, "include")]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", os.path.join(ROOT, "include")]
 

@@ -11,6 +11,11 @@
 
 #include "../../include/rvp.h"
 
+// Every fp32 operation of the reference path rounds once: no multiply-add contraction anywhere in the
+// library (also passed as -ffp-contract=off).  Division and sqrtf are the correctly rounded forms
+// (-fhip-fp32-correctly-rounded-divide-sqrt); HIP's __fsqrt_rn is an approximation and is not used.
+#pragma clang fp contract(off)
+
 namespace rvp {
 
 // ------------------------------------------------------------------------------------------------
@@ -133,8 +138,7 @@ __device__ __forceinline__ uint64_t pack_key(float d2, uint32_t idx) {
 __device__ __forceinline__ float key_d2(uint64_t k) { return bitsf((uint32_t)(k >> 32)); }
 __device__ __forceinline__ uint32_t key_idx(uint64_t k) { return (uint32_t)(k & 0xffffffffu); }
 
-// D1 (rvv_common.cpp:36-43): ((dx*dx + dy*dy) + dz*dz), each op rounded once.  The library is built
-// with -ffp-contract=off; the explicit _rn intrinsics make the order independent of that flag.
+// D1 (rvv_common.cpp:36-43): ((dx*dx + dy*dy) + dz*dz), each op rounded once, in this order.
 __device__ __forceinline__ float dist2(float px, float py, float pz, float qx, float qy, float qz) {
   float dx = __fsub_rn(px, qx);
   float dy = __fsub_rn(py, qy);

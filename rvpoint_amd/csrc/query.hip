@@ -316,7 +316,7 @@ __global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, float *
   knn_search<false>(g, g.xs[q], g.ys[q], g.zs[q], t, lds[w]);
   const int l = lane();
   const int m = min(t.cnt, k + 1);
-  const float sq = (l >= 1 && l <= k && l < m) ? __fsqrt_rn(key_d2(t.key)) : 0.0f;
+  const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(key_d2(t.key)) : 0.0f;
   float s = 0.0f;
   for (int j = 1; j <= k; ++j) s = __fadd_rn(s, readlanef(sq, j));  // +0 beyond m leaves s unchanged
   if (l == 0) means[g.perm[q]] = __fdiv_rn(s, __int2float_rn(k));
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(QT) void sor_radius_means_k(GridView g, int k, floa
   float mean = radius;
   if (m > 1) {
     const int valid_k = min(k, m - 1);
-    const float sq = (l >= 1 && l <= valid_k) ? __fsqrt_rn(key_d2(t.key)) : 0.0f;
+    const float sq = (l >= 1 && l <= valid_k) ? sqrtf(key_d2(t.key)) : 0.0f;
     float s = 0.0f;
     for (int j = 1; j <= valid_k; ++j) s = __fadd_rn(s, readlanef(sq, j));
     mean = __fdiv_rn(s, __int2float_rn(valid_k));
@@ -380,7 +380,7 @@ __global__ __launch_bounds__(64) void sor_stats_k(const float *__restrict__ mean
     for (int j = 0; j < WAVE; ++j) vs = __fadd_rn(vs, readlanef(sq, j));
     sq = sqn;
   }
-  const float gstd = __fsqrt_rn(__fdiv_rn(vs, nf));
+  const float gstd = sqrtf(__fdiv_rn(vs, nf));
   const float thr = __fadd_rn(gmean, __fmul_rn(alpha, gstd));
   if (l == 0) {
     out[0] = gmean;

@@ -221,7 +221,7 @@ __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy
     int nlo[3], nhi[3];
     if (t.cnt >= t.K) {
       const float dk = key_d2(t.kth);
-      const float R = next_up(__fmul_rn(__fsqrt_rn(dk), 1.0f + 0x1p-22f));
+      const float R = next_up(__fmul_rn(sqrtf(dk), 1.0f + 0x1p-22f));
       if (!(R < FLT_MAX)) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {

@@ -196,6 +196,9 @@ int rvp_profile_enable(int on) {
   return RVP_OK;
 }
 
+// text of records collected but not yet handed out (a size query leaves it here for the next call)
+static std::string g_prof_text;
+
 int rvp_profile_collect(char *buf, size_t buflen, size_t *needed) {
   std::vector<ProfRec> recs;
   {
@@ -216,17 +219,17 @@ int rvp_profile_collect(char *buf, size_t buflen, size_t *needed) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
-  std::string out;
+  std::string &out = g_prof_text;
   for (auto &kv : acc) {
     char line[256];
     snprintf(line, sizeof(line), "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
     out += line;
   }
   if (needed) *needed = out.size() + 1;
-  if (buf && buflen) {
-    const size_t m = std::min(buflen - 1, out.size());
-    memcpy(buf, out.data(), m);
-    buf[m] = 0;
+  if (buf && buflen > out.size()) {
+    memcpy(buf, out.data(), out.size());
+    buf[out.size()] = 0;
+    out.clear();
   }
   return status;
 }

@@ -170,7 +170,7 @@ __global__ void plane_models_k(const float *__restrict__ x, const float *__restr
   float a = __fsub_rn(__fmul_rn(v1y, v2z), __fmul_rn(v1z, v2y));
   float b = __fsub_rn(__fmul_rn(v1z, v2x), __fmul_rn(v1x, v2z));
   float c = __fsub_rn(__fmul_rn(v1x, v2y), __fmul_rn(v1y, v2x));
-  const float norm = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(c, c)));
+  const float norm = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(c, c)));
   if (norm < collinear) {
     valid[h] = 0;
     return;
