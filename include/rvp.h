@@ -127,6 +127,13 @@ int rvp_sor_exact(const float *x, const float *y, const float *z, size_t n, int 
 int rvp_sor_radius(const rvp_hash *h, int k, float alpha, float search_radius, float *out_aos,
                    size_t *n_out, uint8_t *keep, float *means, rvp_stream_t s);
 
+/* The sequential fp32 sum the SOR statistics use (statistical_outlier_removal.cpp:43-52, 193-205), exact:
+ * *out_dev = fl(...fl(fl(0 + x_0) + x_1)... + x_{n-1}) for non-negative terms x_i = v[i], or, when
+ * squared_deviation != 0, x_i = fl(fl(v[i] - *mean_dev)^2).  Computed in parallel (binade-wise integer
+ * composition, verified per chunk); out_dev and mean_dev are device pointers. */
+int rvp_seq_sum(const float *v, size_t n, int squared_deviation, const float *mean_dev, float *out_dev,
+                rvp_stream_t s);
+
 /* ---- N1: normal_estimation_sc semantics (src/features/normal_estimation.cpp:72-112) ---------------
  * neighbourhood = k+1 nearest incl. self by (d2, idx); FP32 compensated covariance re-centred at the
  * query, closed-form FP32 eigen solve with FP64 fallback for near-degenerate spectra; flip toward vp

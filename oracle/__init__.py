@@ -59,6 +59,8 @@ def lib():
         L.orc_sor_radius.restype = _sz
         L.orc_sor_filter.argtypes = [_f32p, _f32p, _f32p, _sz, _f32p, C.c_float, _f32p, _u8p, _f32p]
         L.orc_sor_filter.restype = _sz
+        L.orc_seq_sum.argtypes = [_f32p, _sz, C.c_int, C.c_float]
+        L.orc_seq_sum.restype = C.c_float
         L.orc_voxel_sc.argtypes = [_f32p, _sz, C.c_float, _f32p]
         L.orc_voxel_sc.restype = _sz
         L.orc_normals_knn_f64.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_int, _f32p, _f64p, _f64p, _f64p]
@@ -210,6 +212,11 @@ def sor_filter(x, y, z, means, alpha):
     stats = np.zeros(3, np.float32)
     m = lib().orc_sor_filter(_c(x), _c(y), _c(z), n, _c(means), alpha, out.reshape(-1), keep, stats)
     return out[:m], keep[:n].astype(bool), stats
+
+
+def seq_sum(v, squared_dev=False, mean=0.0):
+    v = _c(v)
+    return np.float32(lib().orc_seq_sum(v, len(v), 1 if squared_dev else 0, mean))
 
 
 def voxel_sc(aos, leaf):

@@ -305,6 +305,19 @@ size_t orc_sor_filter(const float *x, const float *y, const float *z, size_t n,
     return count;
 }
 
+float orc_seq_sum(const float *v, size_t n, int squared_dev, float mean) {
+    float s = 0.0f;
+    for (size_t i = 0; i < n; ++i) {
+        float x = v[i];
+        if (squared_dev) {
+            float d = x - mean;
+            x = d * d;
+        }
+        s += x;
+    }
+    return s;
+}
+
 /* sor_sc (statistical_outlier_removal.cpp:19-63): exact k+1 smallest of all d2 (incl. self), mean of sqrt of
  * entries 1..k summed in ascending order, divided by k. */
 size_t orc_sor_sc(const float *aos, size_t n, int k, float alpha, float *out, float *means) {

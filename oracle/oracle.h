@@ -78,6 +78,10 @@ size_t orc_sor_radius(const float *x, const float *y, const float *z, size_t n, 
 size_t orc_sor_filter(const float *x, const float *y, const float *z, size_t n,
                       const float *means, float alpha, float *out, uint8_t *keep, float stats[3]);
 
+/* Sequential fp32 sums of sor_sc / filter_by_mean_dists (statistical_outlier_removal.cpp:43-52, 193-205):
+ * sum of v (squared_dev == 0) or of fl(fl(v - mean)^2), left to right from +0. */
+float orc_seq_sum(const float *v, size_t n, int squared_dev, float mean);
+
 /* V1: voxel_grid_downsamp_sc (voxel_grid_downsamp.cpp:183-213). aos = n*3. Output AoS in (vx,vy,vz) order. */
 size_t orc_voxel_sc(const float *aos, size_t n, float leaf, float *out);
 

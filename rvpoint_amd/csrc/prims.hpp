@@ -26,6 +26,10 @@ int radix_sort_pairs_u64(uint64_t **keys, uint32_t **vals, uint64_t **keys_tmp, 
 int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want,
                     size_t n, float *out_aos, uint32_t *count_dev, hipStream_t s);
 
+// The reference's sequential fp32 sum s = fl(s + x_i) over i = 0..n-1 of non-negative terms, exact, in
+// parallel (seqsum.hip).  mode 0: x_i = v[i]; mode 1: x_i = fl(fl(v[i] - *mean_dev)^2).  *out_dev = s.
+int exact_seq_sum(const float *v, size_t n, int mode, const float *mean_dev, float *out_dev, hipStream_t s);
+
 inline int bits_for(uint64_t max_value) {
   int b = 0;
   while (b < 64 && (max_value >> b) != 0) ++b;

@@ -353,40 +353,17 @@ __global__ __launch_bounds__(QT) void sor_radius_means_k(GridView g, int k, floa
 }
 
 // Global statistics exactly as filter_by_mean_dists / sor_sc (statistical_outlier_removal.cpp:43-55,
-// 193-206): sequential fp32 sums in index order.  One wave; the dependent add chain runs through SGPR
-// operands (readlane) while the next 64 values load.  out = {mean, std, threshold}.
-__global__ __launch_bounds__(64) void sor_stats_k(const float *__restrict__ means, uint32_t n, float alpha,
-                                                   float *__restrict__ out) {
-  const int l = lane();
-  float s = 0.0f;
-  float v = (uint32_t)l < n ? means[l] : 0.0f;
-  for (uint32_t base = 0; base < n; base += WAVE) {
-    const uint32_t nx = base + WAVE + (uint32_t)l;
-    const float vn = nx < n ? means[nx] : 0.0f;
-#pragma unroll
-    for (int j = 0; j < WAVE; ++j) s = __fadd_rn(s, readlanef(v, j));
-    v = vn;
-  }
-  const float nf = __uint2float_rn(n);
-  const float gmean = __fdiv_rn(s, nf);
-  float vs = 0.0f;
-  float d = (uint32_t)l < n ? __fsub_rn(means[l], gmean) : 0.0f;
-  float sq = __fmul_rn(d, d);
-  for (uint32_t base = 0; base < n; base += WAVE) {
-    const uint32_t nx = base + WAVE + (uint32_t)l;
-    const float dn = nx < n ? __fsub_rn(means[nx], gmean) : 0.0f;
-    const float sqn = __fmul_rn(dn, dn);
-#pragma unroll
-    for (int j = 0; j < WAVE; ++j) vs = __fadd_rn(vs, readlanef(sq, j));
-    sq = sqn;
-  }
-  const float gstd = sqrtf(__fdiv_rn(vs, nf));
-  const float thr = __fadd_rn(gmean, __fmul_rn(alpha, gstd));
-  if (l == 0) {
-    out[0] = gmean;
-    out[1] = gstd;
-    out[2] = thr;
-  }
+// 193-206): sequential fp32 sums in index order (exact_seq_sum), then mean = sum / n,
+// std = sqrt(varsum / n), threshold = mean + alpha * std.  stats = {mean, std, threshold, sum, varsum}.
+__global__ void sor_mean_k(uint32_t n, float *__restrict__ stats) {
+  if (threadIdx.x == 0) stats[0] = stats[3] / __uint2float_rn(n);
+}
+
+__global__ void sor_threshold_k(uint32_t n, float alpha, float *__restrict__ stats) {
+  if (threadIdx.x != 0) return;
+  const float gstd = sqrtf(stats[4] / __uint2float_rn(n));
+  stats[1] = gstd;
+  stats[2] = stats[0] + alpha * gstd;
 }
 
 __global__ void sor_keep_k(const float *__restrict__ means, uint32_t n, const float *__restrict__ stats,
@@ -399,8 +376,12 @@ __global__ void sor_keep_k(const float *__restrict__ means, uint32_t n, const fl
 static int sor_tail(const Grid &grid, const float *x, const float *y, const float *z, size_t n, float alpha,
                     float *means, float *out_aos, size_t *n_out, uint8_t *keep, hipStream_t s) {
   Tmp stats, keep_t, cnt;
-  RVP_TRY(stats.alloc(3 * sizeof(float), s));
-  RVP_LAUNCH("sor_stats_k", s, sor_stats_k<<<1, 64, 0, s>>>(means, (uint32_t)n, alpha, stats.as<float>()));
+  RVP_TRY(stats.alloc(5 * sizeof(float), s));
+  float *st = stats.as<float>();
+  RVP_TRY(exact_seq_sum(means, n, 0, nullptr, st + 3, s));
+  RVP_LAUNCH("sor_mean_k", s, sor_mean_k<<<1, 64, 0, s>>>((uint32_t)n, st));
+  RVP_TRY(exact_seq_sum(means, n, 1, st, st + 4, s));
+  RVP_LAUNCH("sor_threshold_k", s, sor_threshold_k<<<1, 64, 0, s>>>((uint32_t)n, alpha, st));
   uint8_t *kp = keep;
   if (!kp) {
     RVP_TRY(keep_t.alloc(n, s));

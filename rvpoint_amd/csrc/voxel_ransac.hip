@@ -39,11 +39,15 @@ __global__ __launch_bounds__(256) void voxel_minmax_k(const float *__restrict__ 
       v[j] = min(v[j], __shfl_xor(v[j], d));
       v[j + 3] = max(v[j + 3], __shfl_xor(v[j + 3], d));
     }
+  __shared__ int sm[4][6];
   if (lane() == 0)
-    for (int j = 0; j < 3; ++j) {
-      atomicMin(&mm[j], v[j]);
-      atomicMax(&mm[j + 3], v[j + 3]);
-    }
+    for (int j = 0; j < 6; ++j) sm[threadIdx.x >> 6][j] = v[j];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int j = threadIdx.x;
+    atomicMin(&mm[j], min(min(sm[0][j], sm[1][j]), min(sm[2][j], sm[3][j])));
+    atomicMax(&mm[j + 3], max(max(sm[0][j + 3], sm[1][j + 3]), max(sm[2][j + 3], sm[3][j + 3])));
+  }
 }
 
 template <class K>
@@ -267,7 +271,7 @@ int rvp_voxel(const float *x, const float *y, const float *z, size_t n, float le
   Tmp mmd;
   RVP_TRY(mmd.alloc(6 * sizeof(int), s));
   RVP_LAUNCH("voxel_minmax_init_k", s, voxel_minmax_init_k<<<1, 64, 0, s>>>(mmd.as<int>()));
-  const int nb = (int)std::min<size_t>(1024, ceil_div(n, 256 * 8));
+  const int nb = (int)std::min<size_t>(256, ceil_div(n, 256 * 8));
   RVP_LAUNCH("voxel_minmax_k", s, voxel_minmax_k<<<nb, 256, 0, s>>>(x, y, z, n, inv, mmd.as<int>()));
   int mm[6];
   RVP_HIP(hipMemcpyAsync(mm, mmd.p, sizeof(mm), hipMemcpyDeviceToHost, s));

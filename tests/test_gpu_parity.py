@@ -281,3 +281,44 @@ def test_normals_reference_unit_test_shape():
     p = np.c_[rng.uniform(-10, 10, (1000, 2)), rng.normal(0, 0.01, 1000)].astype(np.float32)
     nx, ny, nz = R.normal_estimation_sc(p, 10, 2.0)
     assert (np.abs(nz) > 0.9).mean() >= 0.95
+
+
+# ------------------------------------------------------------------------------------------------ exact sums
+def _gpu_seq_sum(v, squared=False, mean=0.0):
+    import ctypes as C
+    from rvpoint_amd._lib import call
+    dv = R.DeviceArray.from_numpy(np.ascontiguousarray(v, np.float32))
+    dm = R.DeviceArray.from_numpy(np.array([mean], np.float32))
+    out = R.DeviceArray(1, np.float32)
+    call("rvp_seq_sum", dv.ptr, len(v), 1 if squared else 0, dm.ptr, out.ptr, None)
+    return out.numpy()[0]
+
+
+SUM_KINDS = ["uniform", "ties", "wide", "zeros_first", "tiny", "large", "nan"]
+
+
+@pytest.mark.parametrize("kind", SUM_KINDS)
+def test_exact_sequential_sum(kind):
+    rng = np.random.default_rng(SUM_KINDS.index(kind))
+    if kind == "uniform":
+        v = rng.random(200_003)
+    elif kind == "ties":      # halfway cases at every binade: exercises round-to-even on the running sum
+        v = rng.integers(0, 1000, 300_000) * 2.0 ** -14
+    elif kind == "wide":
+        v = np.exp(rng.normal(0, 4, 100_000))
+    elif kind == "zeros_first":
+        v = np.concatenate([np.zeros(5000), rng.random(50_000) * 1e-30, rng.random(50_000)])
+    elif kind == "tiny":
+        v = rng.random(37)
+    elif kind == "large":
+        v = rng.random(3_000_000) * 0.1
+    else:
+        v = rng.random(10_000)
+        v[7777] = np.nan
+    v = v.astype(np.float32)
+    got = _gpu_seq_sum(v)
+    exp = O.seq_sum(v)
+    assert (np.isnan(got) and np.isnan(exp)) or got == exp
+    if kind != "nan":
+        mean = np.float32(exp / np.float32(len(v)))
+        assert _gpu_seq_sum(v, True, mean) == O.seq_sum(v, True, mean)
