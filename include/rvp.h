@@ -61,6 +61,8 @@ int rvp_aos_to_soa(const float *aos, size_t n, float *x, float *y, float *z, rvp
  * the stream it runs on.  collect() synchronises, writes "kernel launches total_ms" lines into buf
  * (NUL-terminated, *needed = size required) and clears the records. */
 int rvp_profile_enable(int on);
+/* restrict the timing to launches of one kernel (NULL or "" = all kernels) */
+int rvp_profile_filter(const char *kernel);
 int rvp_profile_collect(char *buf, size_t buflen, size_t *needed);
 
 /* ---- D1 / D2 (src/core/rvv_common.cpp:9-107) ---------------------------------------------------------
@@ -146,7 +148,7 @@ int rvp_normals_knn(const float *x, const float *y, const float *z, size_t n, in
  * Queries are the hash's own points. */
 int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, float vpy, float vpz,
                        float *nx, float *ny, float *nz, rvp_stream_t s);
-/* fraction of points of the last normals call on this thread that took the FP64 eigen fallback */
+/* number of points of the last normals call that took the FP64 eigen fallback (synchronises the device) */
 int rvp_normals_fallback_count(size_t *count);
 
 /* ---- V1: voxel_grid_downsamp_sc (src/filters/voxel_grid_downsamp.cpp:183-213) ---------------------

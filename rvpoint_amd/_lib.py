@@ -53,6 +53,7 @@ _SIGS = {
     "rvp_aos_to_soa": [vp, sz, vp, vp, vp, vp],
     "rvp_profile_enable": [i32],
     "rvp_profile_collect": [C.c_char_p, sz, psz],
+    "rvp_profile_filter": [C.c_char_p],
     "rvp_dist_sq": [vp, vp, vp, sz, fl, fl, fl, vp, vp],
     "rvp_inds_in_radius": [vp, vp, vp, vp, sz, fl, fl, fl, fl, vp, vp, psz, vp],
     "rvp_hash_create": [pvp],
@@ -109,7 +110,9 @@ def call(name, *args):
     return st
 
 
-def profile_enable(on=True):
+def profile_enable(on=True, kernel=None):
+    """Time kernel launches with HIP events on their own streams (optionally only `kernel`)."""
+    call("rvp_profile_filter", kernel.encode() if kernel else None)
     call("rvp_profile_enable", 1 if on else 0)
 
 

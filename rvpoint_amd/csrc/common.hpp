@@ -101,12 +101,14 @@ inline hipStream_t S(rvp_stream_t s) { return static_cast<hipStream_t>(s); }
 // pair around the launch on its own stream; rvp_profile_collect sums the elapsed times per kernel.
 // ------------------------------------------------------------------------------------------------
 bool prof_enabled();
+bool prof_want(const char *name);
 hipEvent_t prof_start(hipStream_t s);
 void prof_stop(const char *name, hipStream_t s, hipEvent_t start);
 
 #define RVP_LAUNCH(name, stream, ...)                                                              \
   do {                                                                                             \
-    hipEvent_t ev_ = ::rvp::prof_enabled() ? ::rvp::prof_start(stream) : nullptr;                  \
+    hipEvent_t ev_ = (::rvp::prof_enabled() && ::rvp::prof_want(name)) ? ::rvp::prof_start(stream) \
+                                                                       : nullptr;                  \
     __VA_ARGS__;                                                                                   \
     if (ev_) ::rvp::prof_stop(name, stream, ev_);                                                  \
     RVP_LAUNCHED();                                                                                \

@@ -74,35 +74,33 @@ int bbox_minmax(const float *x, const float *y, const float *z, size_t n, float 
 }
 
 // ------------------------------------------------------------------------------------------------
-// header: reference bbox (margin 0.01*cell, spatial_hashing.cpp:141-146) and grid dims for cell*2^j
+// grid geometry (host): the reference's bbox margin and the cells per axis for an edge
 // ------------------------------------------------------------------------------------------------
-constexpr int NCAND = 40;
-
-struct Header {
-  float bb[6];           // min/max after margin
-  double dims[NCAND][3]; // cells per axis for cell * 2^j
-};
-
-__global__ void header_k(const float *__restrict__ mm, float cell, Header *__restrict__ h) {
-  if (threadIdx.x != 0) return;
-  const float eps_scale = 0.01f;
-  const float m = __fmul_rn(cell, eps_scale);
-  float mn[3], mx[3];
+// spatial_hashing.cpp:141-146: min -= cell * 0.01f, max += cell * 0.01f (fp32, as on the host there)
+static void margin_bbox(const float mm[6], float cell, float bb[6]) {
+  const float m = cell * 0.01f;
   for (int j = 0; j < 3; ++j) {
-    mn[j] = __fsub_rn(mm[j], m);
-    mx[j] = __fadd_rn(mm[j + 3], m);
-    h->bb[j] = mn[j];
-    h->bb[j + 3] = mx[j];
+    bb[j] = mm[j] - m;
+    bb[j + 3] = mm[j + 3] + m;
   }
-  float c = cell;
-  for (int k = 0; k < NCAND; ++k) {
-    for (int j = 0; j < 3; ++j) {
-      // largest cell index of any point: floor((data_max - mn) / c) (monotone cell function)
-      float f = floorf(__fdiv_rn(__fsub_rn(mm[j + 3], mn[j]), c));
-      h->dims[k][j] = (double)f + 1.0;
-    }
-    c = c * 2.0f;
-  }
+}
+
+// cells per axis = floor((data_max - origin) / edge) + 1, the same fp32 expression the device cell
+// function evaluates; points beyond the last cell would be clamped into it, never dropped
+static double cells_on_axis(float data_max, float origin, float edge) {
+  const float t = (data_max - origin) / edge;
+  return std::floor((double)t) + 1.0;
+}
+
+__global__ void occupancy_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+                            size_t n, float mnx, float mny, float mnz, float c, int gx, int gy, int gz,
+                            uint32_t *__restrict__ flags, uint32_t *__restrict__ count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = (uint32_t)cell_clamped(x[i], mnx, c, gx) +
+                     (uint32_t)gx * ((uint32_t)cell_clamped(y[i], mny, c, gy) +
+                                     (uint32_t)gy * (uint32_t)cell_clamped(z[i], mnz, c, gz));
+  if (atomicExch(&flags[k], 1u) == 0u) atomicAdd(count, 1u);
 }
 
 __global__ void keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
@@ -190,7 +188,6 @@ int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, fl
   x = x_; y = y_; z = z_; n = n_; req_cell = cell_;
   built = false;
   stream = s;
-  occupied = 0;
   if (n == 0) {
     std::fill(bb, bb + 6, 0.f);
     cell = cell_;
@@ -202,39 +199,49 @@ int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, fl
     RVP_TRY(ys.alloc(4, s, persistent));
     RVP_TRY(zs.alloc(4, s, persistent));
     RVP_TRY(perm.alloc(4, s, persistent));
+    RVP_TRY(occ.alloc(sizeof(uint32_t), s, persistent));
+    RVP_HIP(hipMemsetAsync(occ.p, 0, sizeof(uint32_t), s));
     built = true;
     return RVP_OK;
   }
   RVP_CHECK(x && y && z, RVP_E_ARG, "null coordinate pointer");
-  Tmp mm, hd;
-  RVP_TRY(mm.alloc(6 * sizeof(float), s));
-  RVP_TRY(hd.alloc(sizeof(Header), s));
-  RVP_TRY(bbox_minmax(x, y, z, n, mm.as<float>(), s));
-  RVP_LAUNCH("header_k", s, header_k<<<1, 64, 0, s>>>(mm.as<float>(), cell_, hd.as<Header>()));
-  Header h;
-  RVP_HIP(hipMemcpyAsync(&h, hd.p, sizeof(Header), hipMemcpyDeviceToHost, s));
+  float mm[6];
+  RVP_TRY(minmax_host(mm, s));
+  return build_from(mm, cell_, s);
+}
+
+int Grid::minmax_host(float mm[6], hipStream_t s) {
+  Tmp d;
+  RVP_TRY(d.alloc(6 * sizeof(float), s));
+  RVP_TRY(bbox_minmax(x, y, z, n, d.as<float>(), s));
+  RVP_HIP(hipMemcpyAsync(mm, d.p, 6 * sizeof(float), hipMemcpyDeviceToHost, s));
   RVP_HIP(hipStreamSynchronize(s));
-  std::copy(h.bb, h.bb + 6, bb);
+  return RVP_OK;
+}
+
+int Grid::build_from(const float mm[6], float cell_, hipStream_t s) {
+  req_cell = cell_;
+  margin_bbox(mm, cell_, bb);
   // all-NaN cloud: min > max; every point then lands in cell 0 and never matches a query
-  const bool degenerate = !(h.bb[0] <= h.bb[3] && h.bb[1] <= h.bb[4] && h.bb[2] <= h.bb[5]);
+  const bool degenerate = !(bb[0] <= bb[3] && bb[1] <= bb[4] && bb[2] <= bb[5]);
   const double limit = std::max<double>(16777216.0, 4.0 * (double)n);
-  int pick = -1;
-  for (int k = 0; k < NCAND && !degenerate; ++k) {
-    double d0 = h.dims[k][0], d1 = h.dims[k][1], d2 = h.dims[k][2];
-    if (!(d0 >= 1 && d1 >= 1 && d2 >= 1)) continue;
-    if (d0 > 1e9 || d1 > 1e9 || d2 > 1e9) continue;
-    if (d0 * d1 * d2 <= limit) { pick = k; break; }
-  }
-  if (pick < 0) {
-    RVP_CHECK(degenerate, RVP_E_UNSUPPORTED, "grid: coordinate extent too large for a dense cell table");
-    cell = cell_;
-    dims[0] = dims[1] = dims[2] = 1;
-  } else {
-    cell = cell_;
-    for (int k = 0; k < pick; ++k) cell = cell * 2.0f;
-    dims[0] = (int)h.dims[pick][0];
-    dims[1] = (int)h.dims[pick][1];
-    dims[2] = (int)h.dims[pick][2];
+  cell = cell_;
+  dims[0] = dims[1] = dims[2] = 1;
+  if (!degenerate) {
+    bool fits = false;
+    for (int k = 0; k < 64 && cell < FLT_MAX; ++k) {
+      const double d0 = cells_on_axis(mm[3], bb[0], cell), d1 = cells_on_axis(mm[4], bb[1], cell),
+                   d2 = cells_on_axis(mm[5], bb[2], cell);
+      if (d0 >= 1 && d1 >= 1 && d2 >= 1 && d0 <= 1e9 && d1 <= 1e9 && d2 <= 1e9 && d0 * d1 * d2 <= limit) {
+        dims[0] = (int)d0;
+        dims[1] = (int)d1;
+        dims[2] = (int)d2;
+        fits = true;
+        break;
+      }
+      cell = cell * 2.0f;
+    }
+    RVP_CHECK(fits, RVP_E_UNSUPPORTED, "grid: coordinate extent too large for a dense cell table");
   }
   ncells = (uint64_t)dims[0] * dims[1] * dims[2];
 
@@ -244,9 +251,8 @@ int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, fl
   RVP_TRY(kt.alloc(n * sizeof(uint32_t), s));
   RVP_TRY(vt.alloc(n * sizeof(uint32_t), s));
   const unsigned nb = ceil_div(n, 256);
-  RVP_LAUNCH("keys_k", s, keys_k<<<nb, 256, 0, s>>>(x, y, z, n, bb[0], bb[1], bb[2], cell, dims[0], dims[1], dims[2], keys.as<uint32_t>(),
-                            perm.as<uint32_t>()));
-  RVP_LAUNCHED();
+  RVP_LAUNCH("keys_k", s, keys_k<<<nb, 256, 0, s>>>(x, y, z, n, bb[0], bb[1], bb[2], cell, dims[0], dims[1],
+                                                    dims[2], keys.as<uint32_t>(), perm.as<uint32_t>()));
   uint32_t *kp = keys.as<uint32_t>(), *vp = perm.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
   RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n, bits_for(ncells - 1), s));
   if (kp != keys.as<uint32_t>()) {  // odd pass count: the sorted data sits in the temporaries
@@ -256,55 +262,75 @@ int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, fl
   RVP_TRY(xs.alloc(n * sizeof(float), s, persistent));
   RVP_TRY(ys.alloc(n * sizeof(float), s, persistent));
   RVP_TRY(zs.alloc(n * sizeof(float), s, persistent));
-  RVP_LAUNCH("gather_k", s, gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), n, xs.as<float>(), ys.as<float>(), zs.as<float>()));
+  RVP_LAUNCH("gather_k", s, gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), n, xs.as<float>(),
+                                                        ys.as<float>(), zs.as<float>()));
 
   RVP_TRY(cstart.alloc((ncells + 1) * sizeof(uint32_t), s, persistent));
-  const uint32_t gap_cap = (uint32_t)std::min<size_t>(n + 2, 1u << 22);
-  Tmp gaps, ctr;
+  // long gaps are at most one per run start plus the tail: n + 1 entries always suffice
+  const uint32_t gap_cap = (uint32_t)(n + 2);
+  Tmp gaps;
   RVP_TRY(gaps.alloc(gap_cap * sizeof(Gap), s));
-  RVP_TRY(ctr.alloc(2 * sizeof(uint32_t), s));
-  RVP_HIP(hipMemsetAsync(ctr.p, 0, 2 * sizeof(uint32_t), s));
-  RVP_LAUNCH("cell_start_k", s, cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), n, (uint32_t)ncells, cstart.as<uint32_t>(),
-                                  ctr.as<uint32_t>(), gaps.as<Gap>(), ctr.as<uint32_t>() + 1, gap_cap));
-  RVP_LAUNCHED();
-  RVP_LAUNCH("fill_gaps_k", s, fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<Gap>(), ctr.as<uint32_t>() + 1, gap_cap, cstart.as<uint32_t>()));
-  uint32_t hc[2];
-  RVP_HIP(hipMemcpyAsync(hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost, s));
-  RVP_HIP(hipStreamSynchronize(s));
-  RVP_CHECK(hc[1] <= gap_cap, RVP_E_DEVICE, "cell table: gap list overflow");
-  occupied = hc[0];
+  RVP_TRY(occ.alloc(2 * sizeof(uint32_t), s, persistent));
+  RVP_HIP(hipMemsetAsync(occ.p, 0, 2 * sizeof(uint32_t), s));
+  RVP_LAUNCH("cell_start_k", s, cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), n, (uint32_t)ncells,
+                                                                cstart.as<uint32_t>(), occ.as<uint32_t>(), gaps.as<Gap>(),
+                                                                occ.as<uint32_t>() + 1, gap_cap));
+  RVP_LAUNCH("fill_gaps_k", s, fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<Gap>(), occ.as<uint32_t>() + 1, gap_cap,
+                                                                cstart.as<uint32_t>()));
   built = true;
   return RVP_OK;
 }
 
+size_t Grid::occupied_cells() const {
+  if (!built || !occ.p) return 0;
+  uint32_t v = 0;
+  if (hipMemcpyAsync(&v, occ.p, sizeof(v), hipMemcpyDeviceToHost, stream) != hipSuccess) return 0;
+  if (hipStreamSynchronize(stream) != hipSuccess) return 0;
+  return v;
+}
+
+// Cell edge for k-nearest queries.  A first edge c0 from the bbox volume (k points per cell if the cloud
+// filled it) gives an occupancy count; with m points per occupied cell the surface density is about
+// m / c0^2, so the k-th neighbour distance is about R = c0 sqrt(k / (pi m)), and the edge is R / 2.8:
+// the certified box then spans about 6 cells per axis, which keeps the candidates near 2-3x the k
+// neighbours for surfaces and volumes alike.
 int Grid::build_for_knn(const float *x_, const float *y_, const float *z_, size_t n_, int k, hipStream_t s) {
+  x = x_; y = y_; z = z_; n = n_;
   if (n_ == 0) return build(x_, y_, z_, n_, 1.0f, s);
-  // first guess from the bbox volume, then one correction from the measured occupancy: about 2(k+1)
-  // candidates in a 3x3x3 window, assuming surface-like data (9 of 27 cells occupied)
-  Tmp mm;
-  RVP_TRY(mm.alloc(6 * sizeof(float), s));
-  RVP_TRY(bbox_minmax(x_, y_, z_, n_, mm.as<float>(), s));
-  float h[6];
-  RVP_HIP(hipMemcpyAsync(h, mm.p, sizeof(h), hipMemcpyDeviceToHost, s));
-  RVP_HIP(hipStreamSynchronize(s));
+  stream = s;
+  float mm[6];
+  RVP_TRY(minmax_host(mm, s));
   double ext[3], vol = 1.0, emax = 0.0;
   for (int j = 0; j < 3; ++j) {
-    ext[j] = (h[j + 3] >= h[j]) ? (double)h[j + 3] - (double)h[j] : 0.0;
+    ext[j] = (mm[j + 3] >= mm[j]) ? (double)mm[j + 3] - (double)mm[j] : 0.0;
     emax = std::max(emax, ext[j]);
   }
-  if (!(emax > 0.0) || !(emax < 1e30)) return build(x_, y_, z_, n_, 1.0f, s);
+  if (!(emax > 0.0) || !(emax < 1e30)) return build_from(mm, 1.0f, s);
   for (int j = 0; j < 3; ++j) vol *= std::max(ext[j], emax * 1e-3);
-  double c = std::cbrt(vol / (double)n_ * (double)(k + 1));
-  c = std::max(c, emax * 1e-6);
-  RVP_TRY(build(x_, y_, z_, n_, (float)c, s));
-  const double per_cell = (double)n_ / (double)std::max<size_t>(occupied, 1);
-  const double want = 2.0 * (k + 1) / 9.0;
-  double f = std::sqrt(want / per_cell);
-  if (f < 0.7 || f > 1.4) {
-    f = std::min(std::max(f, 0.1), 4.0);
-    RVP_TRY(build(x_, y_, z_, n_, (float)(c * f), s));
+  double c0 = std::max(std::cbrt(vol / (double)n_ * (double)k), emax * 1e-6);
+  double c = c0;
+  float b0[6];
+  margin_bbox(mm, (float)c0, b0);
+  const double g0 = cells_on_axis(mm[3], b0[0], (float)c0), g1 = cells_on_axis(mm[4], b0[1], (float)c0),
+               g2 = cells_on_axis(mm[5], b0[2], (float)c0);
+  if (g0 * g1 * g2 <= 4194304.0) {
+    const size_t nc = (size_t)(g0 * g1 * g2);
+    Tmp flags, cnt;
+    RVP_TRY(flags.alloc(nc * sizeof(uint32_t), s));
+    RVP_TRY(cnt.alloc(sizeof(uint32_t), s));
+    RVP_HIP(hipMemsetAsync(flags.p, 0, nc * sizeof(uint32_t), s));
+    RVP_HIP(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
+    RVP_LAUNCH("occupancy_k", s, occupancy_k<<<ceil_div(n_, 256), 256, 0, s>>>(
+                                     x_, y_, z_, n_, b0[0], b0[1], b0[2], (float)c0, (int)g0, (int)g1, (int)g2,
+                                     flags.as<uint32_t>(), cnt.as<uint32_t>()));
+    uint32_t occupied0 = 0;
+    RVP_HIP(hipMemcpyAsync(&occupied0, cnt.p, sizeof(occupied0), hipMemcpyDeviceToHost, s));
+    RVP_HIP(hipStreamSynchronize(s));
+    const double m = (double)n_ / (double)std::max<uint32_t>(occupied0, 1);
+    const double R = c0 * std::sqrt((double)k / (3.14159265358979 * m));
+    c = std::min(std::max(R / 2.8, c0 / 64.0), c0 * 2.0);
   }
-  return RVP_OK;
+  return build_from(mm, (float)c, s);
 }
 
 GridView Grid::view() const {
@@ -380,7 +406,7 @@ int rvp_hash_info(const rvp_hash *h, size_t *n, size_t *occ, int *dims3, float *
   RVP_CHECK(h, RVP_E_ARG, "null handle");
   RVP_CHECK(h->g.built, RVP_E_STATE, "hash not built");
   if (n) *n = h->g.n;
-  if (occ) *occ = h->g.occupied;
+  if (occ) *occ = h->g.occupied_cells();
   if (dims3) std::copy(h->g.dims, h->g.dims + 3, dims3);
   if (grid_cell) *grid_cell = h->g.cell;
   return RVP_OK;

@@ -58,17 +58,20 @@ struct Grid {
   float cell = 0.f;         // cell size of the device grid
   int dims[3] = {0, 0, 0};
   uint64_t ncells = 0;
-  size_t occupied = 0;
   bool built = false;
   bool persistent = false;  // owned by an rvp_hash: hipMalloc'd buffers that outlive the build stream
-  Tmp xs, ys, zs, perm, keys, cstart;
+  Tmp xs, ys, zs, perm, keys, cstart, occ;
   hipStream_t stream = nullptr;
 
   // Build from device SoA.  The device grid uses the requested cell unless its dense table would exceed
   // max(2^24, 4n) cells; then the edge doubles until it fits (results do not depend on the edge).
   int build(const float *x, const float *y, const float *z, size_t n, float cell, hipStream_t s);
-  // Build with a cell edge chosen for k-nearest-neighbour queries (about 2k points per 3x3x3 window).
+  // Build with a cell edge chosen for k-nearest-neighbour queries.
   int build_for_knn(const float *x, const float *y, const float *z, size_t n, int k, hipStream_t s);
+  // Build from the data min/max (device bbox already read back): one host sync fewer.
+  int build_from(const float mm[6], float cell, hipStream_t s);
+  int minmax_host(float mm[6], hipStream_t s);
+  size_t occupied_cells() const;  // synchronises
   GridView view() const;
 };
 

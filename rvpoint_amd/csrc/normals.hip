@@ -13,13 +13,30 @@
 
 namespace rvp {
 
-static std::atomic<unsigned long long> g_fallbacks{0};
+// FP64-fallback counter of the last normals call, per device (device memory, read on request)
+static unsigned long long *g_fallback_dev[64] = {nullptr};
+static int g_fallback_last_dev = -1;
+
+static int fallback_counter(hipStream_t s, unsigned long long **out) {
+  int dev = 0;
+  RVP_HIP(hipGetDevice(&dev));
+  RVP_CHECK(dev >= 0 && dev < 64, RVP_E_UNSUPPORTED, "device index");
+  if (!g_fallback_dev[dev]) RVP_HIP(hipMalloc(&g_fallback_dev[dev], sizeof(unsigned long long)));
+  RVP_HIP(hipMemsetAsync(g_fallback_dev[dev], 0, sizeof(unsigned long long), s));
+  g_fallback_last_dev = dev;
+  *out = g_fallback_dev[dev];
+  return RVP_OK;
+}
+
+// Each wave handles NB consecutive queries (cell order) and then solves their NB eigen problems in NB
+// lanes at once; NB is small so that a launch has enough waves to hide memory latency.
+constexpr int NB = 8;
 
 struct NormLds {
-  float c[6][WAVE];  // covariance of query j
-  float q[3][WAVE];  // query point
-  uint32_t row[WAVE];
-  int m[WAVE];
+  float c[6][NB];  // covariance of query j
+  float q[3][NB];  // query point
+  uint32_t row[NB];
+  int m[NB];
 };
 
 // cyclic Jacobi in double precision (symmetric 3x3), eigenvector of the smallest eigenvalue
@@ -198,11 +215,11 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
   __shared__ WaveLds lds[QW];
   __shared__ NormLds nl[QW];
   const int w = threadIdx.x >> 6;
-  const uint32_t base = (blockIdx.x * QW + w) * WAVE;
+  const uint32_t base = (blockIdx.x * QW + w) * NB;
   if (base >= g.n) return;
   WaveLds &L = lds[w];
   NormLds &N = nl[w];
-  const int nbatch = (int)min<uint32_t>(WAVE, g.n - base);
+  const int nbatch = (int)min<uint32_t>(NB, g.n - base);
   const float r2 = __fmul_rn(radius, radius);
   for (int j = 0; j < nbatch; ++j) {
     const uint32_t q = base + j;
@@ -233,11 +250,11 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
                                                      float *__restrict__ nz, unsigned long long *fallbacks) {
   __shared__ NormLds nl[QW];
   const int w = threadIdx.x >> 6;
-  const uint32_t base = (blockIdx.x * QW + w) * WAVE;
+  const uint32_t base = (blockIdx.x * QW + w) * NB;
   if (base >= g.n) return;
   NormLds &N = nl[w];
   const int l = lane();
-  const int nbatch = (int)min<uint32_t>(WAVE, g.n - base);
+  const int nbatch = (int)min<uint32_t>(NB, g.n - base);
   for (int j = 0; j < nbatch; ++j) {
     const uint32_t q = base + j;
     const uint32_t row = g.perm[q];
@@ -292,13 +309,6 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
 
 using namespace rvp;
 
-static int finish_fallbacks(unsigned long long *dev, hipStream_t s) {
-  unsigned long long h = 0;
-  RVP_HIP(hipMemcpyAsync(&h, dev, sizeof(h), hipMemcpyDeviceToHost, s));
-  RVP_HIP(hipStreamSynchronize(s));
-  g_fallbacks.store(h);
-  return RVP_OK;
-}
 
 extern "C" {
 
@@ -311,13 +321,12 @@ int rvp_normals_knn(const float *x, const float *y, const float *z, size_t n, in
   hipStream_t s = S(st);
   Grid G;
   RVP_TRY(G.build_for_knn(x, y, z, n, k + 1, s));
-  Tmp fb;
-  RVP_TRY(fb.alloc(sizeof(unsigned long long), s));
-  RVP_HIP(hipMemsetAsync(fb.p, 0, sizeof(unsigned long long), s));
-  RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0><<<ceil_div(n, QW * WAVE), QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz, nx, ny, nz,
-                                                          fb.as<unsigned long long>()));
+  unsigned long long *fb = nullptr;
+  RVP_TRY(fallback_counter(s, &fb));
+  RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0><<<ceil_div(n, QW * NB), QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz, nx, ny, nz,
+                                                          fb));
   RVP_LAUNCHED();
-  return finish_fallbacks(fb.as<unsigned long long>(), s);
+  return RVP_OK;
 }
 
 int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, float vpy, float vpz, float *nx,
@@ -328,13 +337,12 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
   const size_t n = h->g.n;
   if (n == 0) return RVP_OK;
   hipStream_t s = S(st);
-  Tmp fb;
-  RVP_TRY(fb.alloc(sizeof(unsigned long long), s));
-  RVP_HIP(hipMemsetAsync(fb.p, 0, sizeof(unsigned long long), s));
+  unsigned long long *fb = nullptr;
+  RVP_TRY(fallback_counter(s, &fb));
   const GridView g = h->g.view();
   if (max_nn > 0 && max_nn <= MAX_TOPK) {
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1><<<ceil_div(n, QW * WAVE), QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx, ny, nz,
-                                                            fb.as<unsigned long long>()));
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1><<<ceil_div(n, QW * NB), QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx, ny, nz,
+                                                            fb));
     RVP_LAUNCHED();
   } else {
     // unlimited (or a cap above 64): full lists, sorted by (d2, idx), truncated by the row offsets
@@ -350,16 +358,22 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
                        pos.as<uint32_t>(), s));
     // a cap keeps the first min(len, max_nn) entries of each (d2, idx)-sorted row
     const int cap = max_nn > 0 ? max_nn : 0x7fffffff;
-    RVP_LAUNCH("normals_csr_k", s, normals_csr_k<<<ceil_div(n, QW * WAVE), QT, 0, s>>>(g, off.as<int64_t>(), cap, pos.as<uint32_t>(), vpx, vpy,
-                                                        vpz, nx, ny, nz, fb.as<unsigned long long>()));
+    RVP_LAUNCH("normals_csr_k", s, normals_csr_k<<<ceil_div(n, QW * NB), QT, 0, s>>>(g, off.as<int64_t>(), cap, pos.as<uint32_t>(), vpx, vpy,
+                                                        vpz, nx, ny, nz, fb));
     RVP_LAUNCHED();
   }
-  return finish_fallbacks(fb.as<unsigned long long>(), s);
+  return RVP_OK;
 }
 
 int rvp_normals_fallback_count(size_t *count) {
   RVP_CHECK(count, RVP_E_ARG, "null argument");
-  *count = (size_t)g_fallbacks.load();
+  *count = 0;
+  if (g_fallback_last_dev < 0) return RVP_OK;
+  RVP_HIP(hipSetDevice(g_fallback_last_dev));
+  RVP_HIP(hipDeviceSynchronize());
+  unsigned long long h = 0;
+  RVP_HIP(hipMemcpy(&h, g_fallback_dev[g_fallback_last_dev], sizeof(h), hipMemcpyDeviceToHost));
+  *count = (size_t)h;
   return RVP_OK;
 }
 

@@ -45,25 +45,51 @@ struct ProfRec {
 };
 static std::mutex g_prof_mu;
 static std::vector<ProfRec> g_prof;
+static std::vector<hipEvent_t> g_ev_pool;  // events reused across collects
 static std::atomic<bool> g_prof_on{false};
+static std::string g_prof_filter;          // empty: every kernel
 
 bool prof_enabled() { return g_prof_on.load(std::memory_order_relaxed); }
 
-hipEvent_t prof_start(hipStream_t s) {
+bool prof_want(const char *name) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  return g_prof_filter.empty() || g_prof_filter == name;
+}
+
+static hipEvent_t pool_event() {
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_ev_pool.empty()) {
+      hipEvent_t e = g_ev_pool.back();
+      g_ev_pool.pop_back();
+      return e;
+    }
+  }
   hipEvent_t e = nullptr;
   if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+static void release_event(hipEvent_t e) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_ev_pool.push_back(e);
+}
+
+hipEvent_t prof_start(hipStream_t s) {
+  hipEvent_t e = pool_event();
+  if (!e) return nullptr;
   if (hipEventRecord(e, s) != hipSuccess) {
-    (void)hipEventDestroy(e);
+    release_event(e);
     return nullptr;
   }
   return e;
 }
 
 void prof_stop(const char *name, hipStream_t s, hipEvent_t a) {
-  hipEvent_t b = nullptr;
-  if (hipEventCreate(&b) != hipSuccess || hipEventRecord(b, s) != hipSuccess) {
-    (void)hipEventDestroy(a);
-    if (b) (void)hipEventDestroy(b);
+  hipEvent_t b = pool_event();
+  if (!b || hipEventRecord(b, s) != hipSuccess) {
+    release_event(a);
+    if (b) release_event(b);
     return;
   }
   std::lock_guard<std::mutex> lk(g_prof_mu);
@@ -196,6 +222,12 @@ int rvp_profile_enable(int on) {
   return RVP_OK;
 }
 
+int rvp_profile_filter(const char *kernel) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_filter = kernel ? kernel : "";
+  return RVP_OK;
+}
+
 // text of records collected but not yet handed out (a size query leaves it here for the next call)
 static std::string g_prof_text;
 
@@ -216,8 +248,8 @@ int rvp_profile_collect(char *buf, size_t buflen, size_t *needed) {
     } else {
       status = fail(RVP_E_DEVICE, "profile: event query failed");
     }
-    (void)hipEventDestroy(r.a);
-    (void)hipEventDestroy(r.b);
+    release_event(r.a);
+    release_event(r.b);
   }
   std::string &out = g_prof_text;
   for (auto &kv : acc) {
