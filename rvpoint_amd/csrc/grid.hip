@@ -100,7 +100,12 @@ __global__ void occupancy_k(const float *__restrict__ x, const float *__restrict
   const uint32_t k = (uint32_t)cell_clamped(x[i], mnx, c, gx) +
                      (uint32_t)gx * ((uint32_t)cell_clamped(y[i], mny, c, gy) +
                                      (uint32_t)gy * (uint32_t)cell_clamped(z[i], mnz, c, gz));
-  if (atomicExch(&flags[k], 1u) == 0u) atomicAdd(count, 1u);
+  // most points find their cell already marked: read before the atomic, and count one mark per wave
+  bool first = false;
+  if (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+    first = atomicExch(&flags[k], 1u) == 0u;
+  const int c = popc(ballot(first));
+  if (lane() == 0 && c) atomicAdd(count, (uint32_t)c);
 }
 
 __global__ void keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
