@@ -104,8 +104,8 @@ __global__ void occupancy_k(const float *__restrict__ x, const float *__restrict
   bool first = false;
   if (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
     first = atomicExch(&flags[k], 1u) == 0u;
-  const int c = popc(ballot(first));
-  if (lane() == 0 && c) atomicAdd(count, (uint32_t)c);
+  const int marked = popc(ballot(first));
+  if (lane() == 0 && marked) atomicAdd(count, (uint32_t)marked);
 }
 
 __global__ void keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
