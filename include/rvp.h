@@ -173,6 +173,28 @@ int rvp_plane_extract(const float *x, const float *y, const float *z, size_t n, 
                       float dist_thresh, float *inliers_aos, size_t *n_in, float *outliers_aos,
                       size_t *n_out, rvp_stream_t s);
 
+/* ---- batches: many clouds, one launch per stage ---------------------------------------------------
+ * The clouds lie back to back in x/y/z; offsets[nclouds+1] (device, uint32) delimit them; n_bound >= the
+ * total number of points (buffers are sized by it; the true total is read from offsets on the device).
+ * Every cloud's result equals what the single-cloud entry point returns for it.  These calls do not wait
+ * on the host: outputs (including out_offsets) are device memory, ordered on the stream. */
+
+/* S1 per cloud (sor_sc, statistical_outlier_removal.cpp:19-63): kept points of cloud c at
+ * out_aos[3*out_offsets[c] ..], in input order; keep / means (device, n_bound) may be NULL. */
+int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                        size_t n_bound, int k, float alpha, float *out_aos, uint32_t *out_offsets, uint8_t *keep,
+                        float *means, rvp_stream_t s);
+/* H3 per cloud (spatial_hashing.cpp:165-223, max_nn in [1, 64]): every point queries its own cloud with
+ * radius[c] (device, per cloud) on a grid of edge cell[c] (NULL: radius); rows of stride max_nn indexed by
+ * the point's batch position, neighbour indices local to the cloud. */
+int rvp_batch_radius_search(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                            size_t n_bound, const float *radius, const float *cell, int max_nn, int32_t *idx_out,
+                            float *d2_out, int32_t *counts, rvp_stream_t s);
+/* N1 per cloud (normal_estimation.cpp:72-112 semantics, as rvp_normals_knn): outputs by batch position. */
+int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                          size_t n_bound, int k, float vpx, float vpy, float vpz, float *nx, float *ny, float *nz,
+                          rvp_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

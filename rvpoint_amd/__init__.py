@@ -2,6 +2,10 @@
 queries, SOR, normals, voxel grid and RANSAC plane counting, behind the reference's API names."""
 from ._lib import RvpError, gpu_available, lib  # noqa: F401
 from .api import (  # noqa: F401
+    Batch,
+    batch_normals_knn,
+    batch_radius_search,
+    batch_sor_exact,
     PointCloudSoA,
     SpatialHash,
     extract_plane_inliers_outliers_rvv,

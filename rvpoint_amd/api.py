@@ -354,3 +354,60 @@ def extract_plane_inliers_rvv(cloud_soa, model, dist_thresh):
 
 def extract_plane_outliers_rvv(cloud_soa, model, dist_thresh):
     return extract_plane_inliers_outliers_rvv(cloud_soa, model, dist_thresh)[1]
+
+
+# ----------------------------------------------------------------------------------------------------
+# batches: many clouds through one launch per stage
+# ----------------------------------------------------------------------------------------------------
+class Batch:
+    """Clouds back to back in HBM with device offsets (the batched workload: one launch per stage)."""
+
+    def __init__(self, clouds, stream=None):
+        clouds = [np.asarray(c, np.float32).reshape(-1, 3) for c in clouds]
+        self.sizes = np.array([len(c) for c in clouds], np.int64)
+        self.offsets_host = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.uint32)
+        allp = np.concatenate(clouds) if clouds else np.zeros((0, 3), np.float32)
+        self.n = len(allp)
+        self.ncl = len(clouds)
+        self.dev = DeviceCloud(allp if self.n else np.zeros((1, 3), np.float32), stream)
+        self.offsets = DeviceArray.from_numpy(self.offsets_host, stream)
+
+    def split(self, arr):
+        return [arr[self.offsets_host[c]:self.offsets_host[c + 1]] for c in range(self.ncl)]
+
+
+def batch_sor_exact(batch: Batch, k, alpha, stream=None):
+    """S1 per cloud in one pass: (list of kept AoS arrays, means per point)."""
+    nb = max(batch.n, 1)
+    out = DeviceArray((nb, 3), np.float32)
+    out_off = DeviceArray(batch.ncl + 1, np.uint32)
+    means = DeviceArray(nb, np.float32)
+    call("rvp_batch_sor_exact", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, int(k), float(alpha), out.ptr,
+         out_off.ptr, None, means.ptr, stream)
+    o = out_off.numpy()
+    res = out.numpy()
+    return [res[o[c]:o[c + 1]].copy() for c in range(batch.ncl)], means.numpy()[:batch.n]
+
+
+def batch_radius_search(batch: Batch, radius, max_nn, cell=None, stream=None):
+    """H3 per cloud: (idx rows [n, max_nn] with cloud-local indices, d2 rows, counts)."""
+    nb = max(batch.n, 1)
+    r = np.broadcast_to(np.asarray(radius, np.float32), (batch.ncl,)).copy()
+    dr = DeviceArray.from_numpy(r, stream)
+    dc = DeviceArray.from_numpy(np.broadcast_to(np.asarray(cell, np.float32), (batch.ncl,)).copy(), stream) \
+        if cell is not None else None
+    idx = DeviceArray((nb, max_nn), np.int32)
+    d2 = DeviceArray((nb, max_nn), np.float32)
+    cnt = DeviceArray(nb, np.int32)
+    call("rvp_batch_radius_search", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, dr.ptr,
+         dc.ptr if dc is not None else None, int(max_nn), idx.ptr, d2.ptr, cnt.ptr, stream)
+    return idx.numpy()[:batch.n], d2.numpy()[:batch.n], cnt.numpy()[:batch.n]
+
+
+def batch_normals_knn(batch: Batch, k, vp=(0.0, 0.0, 0.0), stream=None):
+    """N1 per cloud: (n, 3) normals by batch position."""
+    nb = max(batch.n, 1)
+    nx, ny, nz = (DeviceArray(nb, np.float32) for _ in range(3))
+    call("rvp_batch_normals_knn", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, int(k), float(vp[0]),
+         float(vp[1]), float(vp[2]), nx.ptr, ny.ptr, nz.ptr, stream)
+    return np.stack([nx.numpy()[:batch.n], ny.numpy()[:batch.n], nz.numpy()[:batch.n]], 1)

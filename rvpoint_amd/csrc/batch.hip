@@ -1,0 +1,473 @@
+// batch.hip -- many clouds through one launch per stage (the batched workload of the north star).
+//
+// A batch is one SoA array with the clouds back to back and device offsets off[ncl+1].  Each cloud gets its
+// own grid (origin, edge, dims), all grids share one cell table: cloud c's cells occupy
+// [toff[c], toff[c] + ncells_c), so the sort key of a point is toff[c] + its local cell id, one stable
+// radix sort orders the whole batch cloud-major and cell-major, and a query builds its cloud's GridView from
+// the descriptor table and runs the same wave kernels as a single cloud (query.hpp).  Results per cloud are
+// identical to processing that cloud alone.  Counts stay in device memory: every kernel is launched for an
+// upper bound of points and reads the true totals from the offsets, so a stage never waits on the host.
+#include <algorithm>
+#include <cmath>
+
+#include "batch.hpp"
+#include "prims.hpp"
+
+namespace rvp {
+
+// ------------------------------------------------------------------------------------------------
+// segmented grid build
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void seg_minmax_k(const float *__restrict__ x, const float *__restrict__ y,
+                                                     const float *__restrict__ z, const uint32_t *__restrict__ off,
+                                                     float *__restrict__ mm) {
+  const int c = blockIdx.x;
+  float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (uint32_t i = off[c] + threadIdx.x; i < off[c + 1]; i += blockDim.x) {
+    const float a = x[i], b = y[i], cz = z[i];
+    if (a < v[0]) v[0] = a;
+    if (b < v[1]) v[1] = b;
+    if (cz < v[2]) v[2] = cz;
+    if (a > v[3]) v[3] = a;
+    if (b > v[4]) v[4] = b;
+    if (cz > v[5]) v[5] = cz;
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1)
+    for (int j = 0; j < 3; ++j) {
+      const float o = __shfl_xor(v[j], d);
+      if (o < v[j]) v[j] = o;
+      const float p = __shfl_xor(v[j + 3], d);
+      if (p > v[j + 3]) v[j + 3] = p;
+    }
+  __shared__ float sm[4][6];
+  if (lane() == 0)
+    for (int j = 0; j < 6; ++j) sm[threadIdx.x >> 6][j] = v[j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w)
+      for (int j = 0; j < 3; ++j) {
+        if (sm[w][j] < v[j]) v[j] = sm[w][j];
+        if (sm[w][j + 3] > v[j + 3]) v[j + 3] = sm[w][j + 3];
+      }
+    for (int j = 0; j < 6; ++j) mm[6 * c + j] = v[j];
+  }
+}
+
+// cells per axis for an edge: floor((max - origin) / edge) + 1 (the device cell function's arithmetic)
+__device__ __forceinline__ double axis_cells(float mx, float origin, float edge) {
+  return (double)floorf((mx - origin) / edge) + 1.0;
+}
+
+constexpr int OCC_BITS = 1 << 16;  // occupancy bitmap cells per cloud for the k-NN edge estimate
+
+// k-NN edge, step 1: volume-based edge c0 (coarsened until its grid fits the occupancy bitmap)
+__global__ void seg_knn_c0_k(const float *__restrict__ mm, const uint32_t *__restrict__ off, int ncl, int K,
+                             float *__restrict__ c0, SegDesc *__restrict__ d0) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  const float *m = mm + 6 * c;
+  const double n = (double)(off[c + 1] - off[c]);
+  double ext[3], emax = 0.0, vol = 1.0;
+  for (int j = 0; j < 3; ++j) {
+    ext[j] = m[j + 3] >= m[j] ? (double)m[j + 3] - (double)m[j] : 0.0;
+    emax = fmax(emax, ext[j]);
+  }
+  SegDesc d{};
+  if (!(n > 0.0) || !(emax > 0.0) || !(emax < 1e30)) {
+    c0[c] = 1.0f;
+    d.cell = 1.0f;
+    d.g[0] = d.g[1] = d.g[2] = 1;
+    d0[c] = d;
+    return;
+  }
+  for (int j = 0; j < 3; ++j) vol *= fmax(ext[j], emax * 1e-3);
+  float e = (float)fmax(cbrt(vol / n * (double)K), emax * 1e-6);
+  for (int it = 0; it < 64; ++it) {
+    const float mg = e * 0.01f;
+    double g0 = axis_cells(m[3], m[0] - mg, e), g1 = axis_cells(m[4], m[1] - mg, e), g2 = axis_cells(m[5], m[2] - mg, e);
+    if (g0 * g1 * g2 <= (double)OCC_BITS) {
+      d.mn[0] = m[0] - mg;
+      d.mn[1] = m[1] - mg;
+      d.mn[2] = m[2] - mg;
+      d.cell = e;
+      d.g[0] = (int)g0;
+      d.g[1] = (int)g1;
+      d.g[2] = (int)g2;
+      break;
+    }
+    e *= 2.0f;
+  }
+  c0[c] = e;
+  d0[c] = d;
+}
+
+// k-NN edge, step 2: occupied cells of the c0 grid (one bit per cell)
+__global__ void seg_occ_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+                          const uint32_t *__restrict__ off, int ncl, const SegDesc *__restrict__ d0,
+                          uint32_t *__restrict__ bits, uint32_t *__restrict__ occ) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool fresh = false;
+  int c = 0;
+  if (i < off[ncl]) {
+    c = cloud_of(off, ncl, i);
+    const SegDesc d = d0[c];
+    const uint32_t k = (uint32_t)cell_clamped(x[i], d.mn[0], d.cell, d.g[0]) +
+                       (uint32_t)d.g[0] * ((uint32_t)cell_clamped(y[i], d.mn[1], d.cell, d.g[1]) +
+                                           (uint32_t)d.g[1] * (uint32_t)cell_clamped(z[i], d.mn[2], d.cell, d.g[2]));
+    uint32_t *w = bits + (size_t)c * (OCC_BITS / 32) + (k >> 5);
+    const uint32_t b = 1u << (k & 31);
+    if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & b)) fresh = !(atomicOr(w, b) & b);
+  }
+  if (fresh) atomicAdd(&occ[c], 1u);
+}
+
+// k-NN edge, step 3: surface density m / c0^2 from m points per occupied cell gives the k-th neighbour
+// distance R = c0 sqrt(K / (pi m)); the edge is R / 2.8
+__global__ void seg_knn_cell_k(const uint32_t *__restrict__ off, int ncl, int K, const float *__restrict__ c0,
+                               const uint32_t *__restrict__ occ, float *__restrict__ cell) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  const double n = (double)(off[c + 1] - off[c]);
+  const double m = n / (double)max(occ[c], 1u);
+  const double e0 = (double)c0[c];
+  const double R = e0 * sqrt((double)K / (3.14159265358979 * fmax(m, 1e-30)));
+  cell[c] = (float)fmin(fmax(R / 2.8, e0 / 64.0), e0 * 2.0);
+}
+
+// per cloud: origin (reference margin), dims, coarsened until the cloud's dense table fits its budget
+__global__ void seg_geom_k(const float *__restrict__ mm, const uint32_t *__restrict__ off, int ncl,
+                           const float *__restrict__ cell, SegDesc *__restrict__ desc, uint32_t *__restrict__ ncells) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  const float *m = mm + 6 * c;
+  const uint32_t n = off[c + 1] - off[c];
+  const double budget = fmax(16384.0, 4.0 * (double)n);
+  SegDesc d{};
+  const float req = cell[c];
+  const float mg = req * 0.01f;
+  d.mn[0] = m[0] - mg;
+  d.mn[1] = m[1] - mg;
+  d.mn[2] = m[2] - mg;
+  d.cell = req;
+  d.g[0] = d.g[1] = d.g[2] = 1;
+  const bool ok = n > 0 && m[0] <= m[3] && m[1] <= m[4] && m[2] <= m[5] && req > 0.0f;
+  if (ok) {
+    float e = req;
+    for (int it = 0; it < 64 && e < FLT_MAX; ++it) {
+      const double g0 = axis_cells(m[3], d.mn[0], e), g1 = axis_cells(m[4], d.mn[1], e), g2 = axis_cells(m[5], d.mn[2], e);
+      if (g0 >= 1 && g1 >= 1 && g2 >= 1 && g0 * g1 * g2 <= budget) {
+        d.cell = e;
+        d.g[0] = (int)g0;
+        d.g[1] = (int)g1;
+        d.g[2] = (int)g2;
+        break;
+      }
+      e *= 2.0f;
+    }
+  }
+  desc[c] = d;
+  ncells[c] = (uint32_t)d.g[0] * (uint32_t)d.g[1] * (uint32_t)d.g[2];
+}
+
+__global__ void seg_toff_k(const uint32_t *__restrict__ toff, int ncl, SegDesc *__restrict__ desc) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncl) desc[c].toff = toff[c];
+}
+
+__global__ void seg_keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+                           const uint32_t *__restrict__ off, int ncl, const SegDesc *__restrict__ desc, uint32_t n_bound,
+                           uint32_t pad_key, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_bound) return;
+  vals[i] = i;
+  if (i >= off[ncl]) {  // slots past the batch sort to the end
+    keys[i] = pad_key;
+    return;
+  }
+  const int c = cloud_of(off, ncl, i);
+  const SegDesc d = desc[c];
+  keys[i] = d.toff + (uint32_t)cell_clamped(x[i], d.mn[0], d.cell, d.g[0]) +
+            (uint32_t)d.g[0] * ((uint32_t)cell_clamped(y[i], d.mn[1], d.cell, d.g[1]) +
+                                (uint32_t)d.g[1] * (uint32_t)cell_clamped(z[i], d.mn[2], d.cell, d.g[2]));
+}
+
+__global__ void seg_gather_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+                             const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off, int ncl,
+                             float *__restrict__ xs, float *__restrict__ ys, float *__restrict__ zs) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= off[ncl]) return;
+  const uint32_t p = perm[i];
+  xs[i] = x[p];
+  ys[i] = y[p];
+  zs[i] = z[p];
+}
+
+struct SegGap {
+  uint32_t lo, hi, val;
+};
+constexpr uint32_t SEG_SHORT_GAP = 32;
+
+// cstart[g] = first cell-order position whose key >= g, for g in [0, table_size]
+__global__ void seg_cell_start_k(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ off, int ncl,
+                                 uint32_t table_size, uint32_t *__restrict__ cs, SegGap *__restrict__ gaps,
+                                 uint32_t *__restrict__ ngaps) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = off[ncl];
+  if (n == 0) {
+    if (i == 0) {
+      const uint32_t g = atomicAdd(ngaps, 1u);
+      gaps[g] = SegGap{0, table_size, 0};
+    }
+    return;
+  }
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  const uint32_t lo = i == 0 ? 0u : keys[i - 1] + 1;
+  if (i == 0 || k != keys[i - 1]) {
+    if (k - lo + 1 <= SEG_SHORT_GAP) {
+      for (uint32_t c = lo; c <= k; ++c) cs[c] = i;
+    } else {
+      const uint32_t g = atomicAdd(ngaps, 1u);
+      gaps[g] = SegGap{lo, k, i};
+    }
+  }
+  if (i == n - 1) {
+    const uint32_t g = atomicAdd(ngaps, 1u);
+    gaps[g] = SegGap{k + 1, table_size, n};
+  }
+}
+
+__global__ void seg_fill_gaps_k(const SegGap *__restrict__ gaps, const uint32_t *__restrict__ ngaps,
+                                uint32_t *__restrict__ cs) {
+  const uint32_t ng = *ngaps;
+  for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x) {
+    const SegGap gp = gaps[g];
+    for (uint64_t c = (uint64_t)gp.lo + threadIdx.x; c <= gp.hi; c += blockDim.x) cs[c] = gp.val;
+  }
+}
+
+int SegGrid::build(const float *x, const float *y, const float *z, const uint32_t *off_, int ncl_, size_t n_bound_,
+                 bool mode_knn, int K, const float *cell_dev, hipStream_t s) {
+  ncl = ncl_;
+  off = off_;
+  n_bound = n_bound_;
+  RVP_CHECK(ncl > 0, RVP_E_ARG, "no clouds");
+  RVP_CHECK(n_bound < 0xfffffff0ull / 5, RVP_E_UNSUPPORTED, "batch too large");
+  const unsigned cb = ceil_div(ncl, 64);
+  RVP_TRY(mm.alloc(ncl * 6 * sizeof(float), s));
+  RVP_TRY(desc.alloc(ncl * sizeof(SegDesc), s));
+  RVP_TRY(cellv.alloc(ncl * sizeof(float), s));
+  RVP_TRY(ncells.alloc((ncl + 1) * sizeof(uint32_t), s));
+  RVP_TRY(toff.alloc((ncl + 1) * sizeof(uint32_t), s));
+  RVP_LAUNCH("seg_minmax_k", s, seg_minmax_k<<<ncl, 256, 0, s>>>(x, y, z, off, mm.as<float>()));
+  const float *cells = cell_dev;
+  if (mode_knn) {
+    Tmp c0, d0, bits, occ;
+    RVP_TRY(c0.alloc(ncl * sizeof(float), s));
+    RVP_TRY(d0.alloc(ncl * sizeof(SegDesc), s));
+    RVP_TRY(bits.alloc((size_t)ncl * (OCC_BITS / 8), s));
+    RVP_TRY(occ.alloc(ncl * sizeof(uint32_t), s));
+    RVP_HIP(hipMemsetAsync(bits.p, 0, (size_t)ncl * (OCC_BITS / 8), s));
+    RVP_HIP(hipMemsetAsync(occ.p, 0, ncl * sizeof(uint32_t), s));
+    RVP_LAUNCH("seg_knn_c0_k", s, seg_knn_c0_k<<<cb, 64, 0, s>>>(mm.as<float>(), off, ncl, K, c0.as<float>(),
+                                                                 d0.as<SegDesc>()));
+    RVP_LAUNCH("seg_occ_k", s, seg_occ_k<<<ceil_div(n_bound, 256), 256, 0, s>>>(
+                                   x, y, z, off, ncl, d0.as<SegDesc>(), bits.as<uint32_t>(), occ.as<uint32_t>()));
+    RVP_LAUNCH("seg_knn_cell_k", s, seg_knn_cell_k<<<cb, 64, 0, s>>>(off, ncl, K, c0.as<float>(),
+                                                                     occ.as<uint32_t>(), cellv.as<float>()));
+    cells = cellv.as<float>();
+  }
+  RVP_LAUNCH("seg_geom_k", s, seg_geom_k<<<cb, 64, 0, s>>>(mm.as<float>(), off, ncl, cells, desc.as<SegDesc>(),
+                                                           ncells.as<uint32_t>()));
+  RVP_TRY(exclusive_scan_u32(ncells.as<uint32_t>(), toff.as<uint32_t>(), ncl, toff.as<uint32_t>() + ncl, s));
+  RVP_LAUNCH("seg_toff_k", s, seg_toff_k<<<cb, 64, 0, s>>>(toff.as<uint32_t>(), ncl, desc.as<SegDesc>()));
+  // the table holds at most sum(max(2^14, 4 n_c)) cells
+  const uint64_t table = 4ull * n_bound + 16384ull * (uint64_t)ncl;
+  RVP_CHECK(table < 0xfffffff0ull, RVP_E_UNSUPPORTED, "batch cell table too large");
+  const uint32_t n32 = (uint32_t)n_bound;
+  Tmp kt, vt;
+  RVP_TRY(keys.alloc(n_bound * sizeof(uint32_t), s));
+  RVP_TRY(perm.alloc(n_bound * sizeof(uint32_t), s));
+  RVP_TRY(kt.alloc(n_bound * sizeof(uint32_t), s));
+  RVP_TRY(vt.alloc(n_bound * sizeof(uint32_t), s));
+  const unsigned nb = ceil_div(n_bound, 256);
+  RVP_LAUNCH("seg_keys_k", s, seg_keys_k<<<nb, 256, 0, s>>>(x, y, z, off, ncl, desc.as<SegDesc>(), n32,
+                                                            (uint32_t)table, keys.as<uint32_t>(), perm.as<uint32_t>()));
+  uint32_t *kp = keys.as<uint32_t>(), *vp = perm.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
+  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n_bound, bits_for(table), s));
+  if (kp != keys.as<uint32_t>()) {
+    keys.swap(kt);
+    perm.swap(vt);
+  }
+  RVP_TRY(xs.alloc(n_bound * sizeof(float), s));
+  RVP_TRY(ys.alloc(n_bound * sizeof(float), s));
+  RVP_TRY(zs.alloc(n_bound * sizeof(float), s));
+  RVP_LAUNCH("seg_gather_k", s, seg_gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), off, ncl,
+                                                                xs.as<float>(), ys.as<float>(), zs.as<float>()));
+  RVP_TRY(cstart.alloc((table + 1) * sizeof(uint32_t), s));
+  Tmp gaps, ng;
+  RVP_TRY(gaps.alloc((n_bound + 2) * sizeof(SegGap), s));
+  RVP_TRY(ng.alloc(sizeof(uint32_t), s));
+  RVP_HIP(hipMemsetAsync(ng.p, 0, sizeof(uint32_t), s));
+  RVP_LAUNCH("seg_cell_start_k", s, seg_cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), off, ncl,
+                                                                        (uint32_t)table, cstart.as<uint32_t>(),
+                                                                        gaps.as<SegGap>(), ng.as<uint32_t>()));
+  RVP_LAUNCH("seg_fill_gaps_k", s, seg_fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<SegGap>(), ng.as<uint32_t>(),
+                                                                        cstart.as<uint32_t>()));
+  return RVP_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// batched stages
+// ------------------------------------------------------------------------------------------------
+// S1 per cloud: k+1 nearest within the query's cloud, mean of sqrt of entries 1..k (sequential)
+__global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, float *__restrict__ means) {
+  __shared__ WaveLds lds[QW];
+  const int w = threadIdx.x >> 6;
+  const uint32_t q = blockIdx.x * QW + w;
+  if (q >= A.off[A.ncl]) return;
+  const int c = cloud_of(A.off, A.ncl, q);
+  const GridView g = seg_view(A, c);
+  TopK t;
+  t.init(k + 1);
+  knn_search<false>(g, g.xs[q], g.ys[q], g.zs[q], t, lds[w]);
+  const int l = lane();
+  const int m = min(t.cnt, k + 1);
+  const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(key_d2(t.key)) : 0.0f;
+  float s = 0.0f;
+  for (int j = 1; j <= k; ++j) s = s + readlanef(sq, j);
+  if (l == 0) means[g.perm[q]] = s / __int2float_rn(k);
+}
+
+__global__ void seg_mean_k(const uint32_t *__restrict__ off, int ncl, const float *__restrict__ sums,
+                           float *__restrict__ mean) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncl) mean[c] = sums[c] / __uint2float_rn(off[c + 1] - off[c]);
+}
+
+__global__ void seg_threshold_k(const uint32_t *__restrict__ off, int ncl, float alpha, const float *__restrict__ mean,
+                                const float *__restrict__ vsum, float *__restrict__ thr) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  const float sd = sqrtf(vsum[c] / __uint2float_rn(off[c + 1] - off[c]));
+  thr[c] = mean[c] + alpha * sd;
+}
+
+__global__ void seg_keep_k(const float *__restrict__ means, const uint32_t *__restrict__ off, int ncl,
+                           const float *__restrict__ thr, uint32_t n_bound, uint8_t *__restrict__ keep) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_bound) return;
+  if (i >= off[ncl]) {
+    keep[i] = 0;
+    return;
+  }
+  keep[i] = means[i] <= thr[cloud_of(off, ncl, i)] ? 1 : 0;
+}
+
+__global__ void seg_out_offsets_k(const uint32_t *__restrict__ off, int ncl, uint32_t n_bound,
+                                  const uint32_t *__restrict__ pos, const uint32_t *__restrict__ kept_total,
+                                  uint32_t *__restrict__ out_off) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncl) return;
+  const uint32_t o = off[c];
+  out_off[c] = (o < off[ncl] && o < n_bound) ? pos[o] : *kept_total;
+}
+
+// H3 per cloud: capped radius lists (cloud-local indices), radius per cloud
+__global__ __launch_bounds__(QT) void seg_radius_topk_k(SegArrays A, const float *__restrict__ radius, int K,
+                                                         int32_t *__restrict__ idx_out, float *__restrict__ d2_out,
+                                                         int32_t *__restrict__ counts) {
+  __shared__ WaveLds lds[QW];
+  const int w = threadIdx.x >> 6;
+  const uint32_t q = blockIdx.x * QW + w;
+  if (q >= A.off[A.ncl]) return;
+  const int c = cloud_of(A.off, A.ncl, q);
+  const GridView g = seg_view(A, c);
+  const float r = radius[c];
+  const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
+  int lo[3], hi[3];
+  window3(g, x, y, z, r, lo, hi);
+  TopK t;
+  t.init(K);
+  TopKSink<false> sink{t, lds[w]};
+  scan_box(g, x, y, z, lo, hi, false, lo, hi, r * r, true, lds[w], sink);
+  topk_flush<false>(t, lds[w]);
+  const int m = min(t.cnt, K);
+  const uint32_t row = g.perm[q];
+  const uint32_t base = A.off[c];
+  const int l = lane();
+  if (l < m) {
+    if (idx_out) idx_out[(size_t)row * K + l] = (int32_t)(key_idx(t.key) - base);
+    if (d2_out) d2_out[(size_t)row * K + l] = key_d2(t.key);
+  }
+  if (l == 0) counts[row] = m;
+}
+
+}  // namespace rvp
+
+using namespace rvp;
+
+extern "C" {
+
+int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                        size_t n_bound, int k, float alpha, float *out_aos, uint32_t *out_offsets, uint8_t *keep,
+                        float *means, rvp_stream_t st) {
+  RVP_CHECK(x && y && z && offsets && out_aos && out_offsets, RVP_E_ARG, "null pointer");
+  RVP_CHECK(nclouds > 0, RVP_E_ARG, "nclouds must be positive");
+  RVP_CHECK(k >= 1 && k + 1 <= MAX_TOPK, RVP_E_UNSUPPORTED, "k must be in [1, 63]");
+  RVP_TRY(pool_init());
+  hipStream_t s = S(st);
+  SegGrid G;
+  RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, true, k + 1, nullptr, s));
+  Tmp mt, kt, sums, mean, vsum, thr, pos, kept;
+  float *mp = means;
+  if (!mp) {
+    RVP_TRY(mt.alloc(n_bound * sizeof(float), s));
+    mp = mt.as<float>();
+  }
+  RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<<<ceil_div(n_bound, QW), QT, 0, s>>>(G.arrays(), k, mp));
+  RVP_TRY(sums.alloc(nclouds * sizeof(float), s));
+  RVP_TRY(mean.alloc(nclouds * sizeof(float), s));
+  RVP_TRY(vsum.alloc(nclouds * sizeof(float), s));
+  RVP_TRY(thr.alloc(nclouds * sizeof(float), s));
+  const unsigned cb = ceil_div(nclouds, 64);
+  RVP_TRY(exact_seq_sum_seg(mp, offsets, nclouds, n_bound, 0, nullptr, sums.as<float>(), s));
+  RVP_LAUNCH("seg_mean_k", s, seg_mean_k<<<cb, 64, 0, s>>>(offsets, nclouds, sums.as<float>(), mean.as<float>()));
+  RVP_TRY(exact_seq_sum_seg(mp, offsets, nclouds, n_bound, 1, mean.as<float>(), vsum.as<float>(), s));
+  RVP_LAUNCH("seg_threshold_k", s, seg_threshold_k<<<cb, 64, 0, s>>>(offsets, nclouds, alpha, mean.as<float>(),
+                                                                     vsum.as<float>(), thr.as<float>()));
+  uint8_t *kp = keep;
+  if (!kp) {
+    RVP_TRY(kt.alloc(n_bound, s));
+    kp = kt.as<uint8_t>();
+  }
+  const unsigned nb = ceil_div(n_bound, 256);
+  RVP_LAUNCH("seg_keep_k", s, seg_keep_k<<<nb, 256, 0, s>>>(mp, offsets, nclouds, thr.as<float>(), (uint32_t)n_bound, kp));
+  // stable compaction over the whole batch keeps the clouds in order; a cloud's output starts where the
+  // compaction positions its first point
+  RVP_TRY(pos.alloc(n_bound * sizeof(uint32_t), s));
+  RVP_TRY(kept.alloc(sizeof(uint32_t), s));
+  RVP_TRY(compact_xyz_aos_pos(x, y, z, kp, 1, n_bound, out_aos, pos.as<uint32_t>(), kept.as<uint32_t>(), s));
+  RVP_LAUNCH("seg_out_offsets_k", s, seg_out_offsets_k<<<ceil_div(nclouds + 1, 64), 64, 0, s>>>(
+                                         offsets, nclouds, (uint32_t)n_bound, pos.as<uint32_t>(), kept.as<uint32_t>(),
+                                         out_offsets));
+  return RVP_OK;
+}
+
+int rvp_batch_radius_search(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                            size_t n_bound, const float *radius, const float *cell, int max_nn, int32_t *idx_out,
+                            float *d2_out, int32_t *counts, rvp_stream_t st) {
+  RVP_CHECK(x && y && z && offsets && radius && counts, RVP_E_ARG, "null pointer");
+  RVP_CHECK(nclouds > 0, RVP_E_ARG, "nclouds must be positive");
+  RVP_CHECK(max_nn >= 1 && max_nn <= MAX_TOPK, RVP_E_UNSUPPORTED, "max_nn must be in [1, 64]");
+  RVP_TRY(pool_init());
+  hipStream_t s = S(st);
+  SegGrid G;
+  RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, false, 0, cell ? cell : radius, s));
+  RVP_LAUNCH("seg_radius_topk_k", s, seg_radius_topk_k<<<ceil_div(n_bound, QW), QT, 0, s>>>(
+                                         G.arrays(), radius, max_nn, idx_out, d2_out, counts));
+  return RVP_OK;
+}
+
+}  // extern "C"

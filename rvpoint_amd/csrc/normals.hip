@@ -8,6 +8,7 @@
 // reproduced; results are graded against an FP64 eigen solve of the same neighbourhood.
 #include <atomic>
 
+#include "batch.hpp"
 #include "prims.hpp"
 #include "query.hpp"
 
@@ -305,6 +306,33 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
   solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
 }
 
+// N1 over a batch: every query's k+1 nearest within its own cloud
+__global__ __launch_bounds__(QT) void seg_normals_knn_k(SegArrays A, int K, float vpx, float vpy, float vpz,
+                                                         float *__restrict__ nx, float *__restrict__ ny,
+                                                         float *__restrict__ nz, unsigned long long *fallbacks) {
+  __shared__ WaveLds lds[QW];
+  __shared__ NormLds nl[QW];
+  const int w = threadIdx.x >> 6;
+  const uint32_t total = A.off[A.ncl];
+  const uint32_t base = (blockIdx.x * QW + w) * NB;
+  if (base >= total) return;
+  NormLds &N = nl[w];
+  const int nbatch = (int)min<uint32_t>(NB, total - base);
+  for (int j = 0; j < nbatch; ++j) {
+    const uint32_t q = base + j;
+    const GridView g = seg_view(A, cloud_of(A.off, A.ncl, q));
+    const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
+    TopK t;
+    t.init(K);
+    knn_search<true>(g, x, y, z, t, lds[w]);
+    const int m = min(t.cnt, K);
+    float c[6];
+    wave_cov(g, lane() < m, t.pay, m, x, y, z, c);
+    stash(N, j, c, x, y, z, g.perm[q], m);
+  }
+  solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
+}
+
 }  // namespace rvp
 
 using namespace rvp;
@@ -362,6 +390,23 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
                                                         vpz, nx, ny, nz, fb));
     RVP_LAUNCHED();
   }
+  return RVP_OK;
+}
+
+int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                          size_t n_bound, int k, float vpx, float vpy, float vpz, float *nx, float *ny, float *nz,
+                          rvp_stream_t st) {
+  RVP_CHECK(x && y && z && offsets && nx && ny && nz, RVP_E_ARG, "null pointer");
+  RVP_CHECK(nclouds > 0, RVP_E_ARG, "nclouds must be positive");
+  RVP_CHECK(k >= 1 && k + 1 <= MAX_TOPK, RVP_E_UNSUPPORTED, "k must be in [1, 63]");
+  RVP_TRY(pool_init());
+  hipStream_t s = S(st);
+  SegGrid G;
+  RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, true, k + 1, nullptr, s));
+  unsigned long long *fb = nullptr;
+  RVP_TRY(fallback_counter(s, &fb));
+  RVP_LAUNCH("seg_normals_knn_k", s, seg_normals_knn_k<<<ceil_div(n_bound, QW * NB), QT, 0, s>>>(
+                                         G.arrays(), k + 1, vpx, vpy, vpz, nx, ny, nz, fb));
   return RVP_OK;
 }
 

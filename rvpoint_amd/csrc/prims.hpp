@@ -25,10 +25,17 @@ int radix_sort_pairs_u64(uint64_t **keys, uint32_t **vals, uint64_t **keys_tmp, 
 // Stable compaction of points whose flag equals `want` into AoS (float[3*m]); *count_dev = m (device).
 int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want,
                     size_t n, float *out_aos, uint32_t *count_dev, hipStream_t s);
+// the same, keeping the exclusive-scan positions (device uint32[n])
+int compact_xyz_aos_pos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want,
+                        size_t n, float *out_aos, uint32_t *pos, uint32_t *count_dev, hipStream_t s);
 
 // The reference's sequential fp32 sum s = fl(s + x_i) over i = 0..n-1 of non-negative terms, exact, in
 // parallel (seqsum.hip).  mode 0: x_i = v[i]; mode 1: x_i = fl(fl(v[i] - *mean_dev)^2).  *out_dev = s.
 int exact_seq_sum(const float *v, size_t n, int mode, const float *mean_dev, float *out_dev, hipStream_t s);
+// The same for nseg independent segments [seg_off[s], seg_off[s+1]) (device offsets, n_bound >= seg_off[nseg]);
+// mode 1 reads one mean per segment from mean_dev[s]; out[s] receives segment s's sum.  No host sync.
+int exact_seq_sum_seg(const float *v, const uint32_t *seg_off, int nseg, size_t n_bound, int mode,
+                      const float *mean_dev, float *out, hipStream_t s);
 
 inline int bits_for(uint64_t max_value) {
   int b = 0;

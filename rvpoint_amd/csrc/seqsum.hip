@@ -1,6 +1,6 @@
 // seqsum.hip -- the reference's sequential fp32 sum  s = fl(s + x_i), i = 0..n-1, x_i >= 0, computed in
 // parallel and bit-exactly (used for SOR's global mean and variance, statistical_outlier_removal.cpp:43-52,
-// 193-205).
+// 193-205), for one array or for many independent segments (one per cloud of a batch) at once.
 //
 // While the running sum stays in one binade [2^e, 2^(e+1)) it is an integer multiple M of u = 2^(e-23),
 // and each step is exact integer arithmetic:  M <- M + round(x_i / u), round-to-nearest-even, where only
@@ -8,12 +8,12 @@
 //   M -> M + (M even ? ie : io),
 // and such maps compose associatively:  (ie, io) then (je, jo) = (ie + (ie odd ? jo : je),
 //                                                               io + (io odd ? je : jo)).
-// The array is cut into chunks of 256.  Per chunk, the binade of the running sum is predicted from fp64
+// Each segment is cut into chunks of 256.  Per chunk, the binade of the running sum is predicted from fp64
 // prefix sums; where the prediction is one binade (with a 1e-3 relative margin), the chunk's map is built
-// by an ordered tree reduction over its elements.  One wave then walks the chunks in order: a chunk whose
-// map applies (the running sum is in the predicted binade and M stays below 2^24) costs one integer
-// step; any other chunk (the first one, binade crossings, zeros, subnormals, non-finite values) is
-// replayed element by element with the fp32 adder.  Every applied map is checked, so the result equals
+// by an ordered tree reduction over its elements.  One wave per segment then walks its chunks in order: a
+// chunk whose map applies (the running sum is in the predicted binade and M stays below 2^24) costs one
+// integer step; any other chunk (the first one, binade crossings, zeros, subnormals, non-finite values)
+// is replayed element by element with the fp32 adder.  Every applied map is checked, so the result equals
 // the sequential sum exactly; the prediction only decides how many chunks take the slow path.
 #include "prims.hpp"
 
@@ -48,32 +48,69 @@ __device__ __forceinline__ void ss_compose(long long &fe, long long &fo, long lo
   fo = no;
 }
 
-__global__ __launch_bounds__(64) void ss_chunksum_k(const float *__restrict__ v, size_t n, int mode,
-                                                     const float *__restrict__ mean_dev, double *__restrict__ csum) {
-  const size_t c = blockIdx.x;
-  const float mean = mode ? *mean_dev : 0.0f;
-  double acc = 0.0;
-#pragma unroll
-  for (int r = 0; r < SS_LANE; ++r) {
-    const size_t i = c * SS_C + (size_t)lane() * SS_LANE + r;
-    if (i < n) acc += (double)ss_term(v[i], mode, mean);
+// largest s in [0, nseg) with off[s] <= v (off non-decreasing, off[0] == 0)
+__device__ __forceinline__ int seg_of(const uint32_t *__restrict__ off, int nseg, uint32_t v) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= v) lo = mid;
+    else hi = mid - 1;
   }
-  acc = wave_sum_f64(acc);
-  if (lane() == 0) csum[c] = acc;
+  return lo;
 }
 
-__global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, size_t n, int mode,
+__global__ void ss_nchunks_k(const uint32_t *__restrict__ seg, int nseg, uint32_t *__restrict__ nch) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nseg) nch[s] = (seg[s + 1] - seg[s] + SS_C - 1) / SS_C;
+}
+
+struct ChunkRef {
+  int s;          // segment
+  uint32_t a, b;  // element range
+  uint32_t c0;    // first chunk of the segment
+};
+
+__device__ __forceinline__ bool chunk_ref(const uint32_t *__restrict__ seg, const uint32_t *__restrict__ coff,
+                                          int nseg, uint32_t c, ChunkRef &r) {
+  if (c >= coff[nseg]) return false;
+  r.s = seg_of(coff, nseg, c);
+  r.c0 = coff[r.s];
+  r.a = seg[r.s] + (c - r.c0) * SS_C;
+  r.b = min(r.a + (uint32_t)SS_C, seg[r.s + 1]);
+  return true;
+}
+
+__global__ __launch_bounds__(64) void ss_chunksum_k(const float *__restrict__ v, const uint32_t *__restrict__ seg,
+                                                     const uint32_t *__restrict__ coff, int nseg, int mode,
+                                                     const float *__restrict__ mean_dev, double *__restrict__ csum) {
+  ChunkRef r;
+  if (!chunk_ref(seg, coff, nseg, blockIdx.x, r)) return;
+  const float mean = mode ? mean_dev[r.s] : 0.0f;
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < SS_LANE; ++q) {
+    const uint32_t i = r.a + (uint32_t)lane() * SS_LANE + q;
+    if (i < r.b) acc += (double)ss_term(v[i], mode, mean);
+  }
+  acc = wave_sum_f64(acc);
+  if (lane() == 0) csum[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, const uint32_t *__restrict__ seg,
+                                                    const uint32_t *__restrict__ coff, int nseg, int mode,
                                                     const float *__restrict__ mean_dev,
                                                     const double *__restrict__ csum, SSChunk *__restrict__ out) {
-  const size_t c = blockIdx.x;
+  const uint32_t c = blockIdx.x;
+  ChunkRef r;
+  if (!chunk_ref(seg, coff, nseg, c, r)) return;
   const int l = lane();
   double p = 0.0;
-  for (size_t j = l; j < c; j += WAVE) p += csum[j];
+  for (uint32_t j = r.c0 + l; j < c; j += WAVE) p += csum[j];
   p = wave_sum_f64(p);
   const double lo = p, hi = p + csum[c];
   int e = 0;
   bool clean = false;
-  if (c > 0 && lo > 0.0 && hi < 1e38) {
+  if (c > r.c0 && lo > 0.0 && hi < 1e38) {
     const int elo = ilogb(lo * (1.0 - 1e-3)), ehi = ilogb(hi * (1.0 + 1e-3));
     if (elo == ehi && elo >= -125 && elo <= 126) {
       clean = true;
@@ -84,13 +121,13 @@ __global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, 
     if (l == 0) out[c] = SSChunk{0, 0, 0, 0};
     return;
   }
-  const float mean = mode ? *mean_dev : 0.0f;
+  const float mean = mode ? mean_dev[r.s] : 0.0f;
   const double scale = ldexp(1.0, 23 - e);
   long long fe = 0, fo = 0;
 #pragma unroll
-  for (int r = 0; r < SS_LANE; ++r) {
-    const size_t i = c * SS_C + (size_t)l * SS_LANE + r;
-    if (i < n) {
+  for (int q = 0; q < SS_LANE; ++q) {
+    const uint32_t i = r.a + (uint32_t)l * SS_LANE + q;
+    if (i < r.b) {
       double t = (double)ss_term(v[i], mode, mean) * scale;  // exact: power-of-two scaling
       if (!(t < 1099511627776.0)) t = 1099511627776.0;       // 2^40: saturate (the chunk then fails its check)
       const double a = floor(t), fr = t - a;
@@ -116,65 +153,88 @@ __global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, 
   if (l == 0) out[c] = SSChunk{fe, fo, e, 1};
 }
 
-// single wave: walk the chunks in order
-__global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, size_t n, int mode,
+// one wave per segment walks its chunks in order
+__global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, const uint32_t *__restrict__ seg,
+                                                  const uint32_t *__restrict__ coff, int nseg, int mode,
                                                   const float *__restrict__ mean_dev,
-                                                  const SSChunk *__restrict__ ch, size_t nchunks,
-                                                  float *__restrict__ out) {
+                                                  const SSChunk *__restrict__ ch, float *__restrict__ out) {
+  const int s = blockIdx.x;
+  if (s >= nseg) return;
   const int l = lane();
-  const float mean = mode ? *mean_dev : 0.0f;
-  float s = 0.0f;
-  for (size_t c0 = 0; c0 < nchunks; c0 += WAVE) {
-    SSChunk mine = (c0 + l < nchunks) ? ch[c0 + l] : SSChunk{0, 0, 0, 0};
-    const int cnt = (int)min<size_t>(WAVE, nchunks - c0);
+  const float mean = mode ? mean_dev[s] : 0.0f;
+  const uint32_t c_begin = coff[s], c_end = coff[s + 1], e_end = seg[s + 1];
+  float acc = 0.0f;
+  for (uint32_t c0 = c_begin; c0 < c_end; c0 += WAVE) {
+    const SSChunk mine = (c0 + l < c_end) ? ch[c0 + l] : SSChunk{0, 0, 0, 0};
+    const int cnt = (int)min<uint32_t>(WAVE, c_end - c0);
     for (int j = 0; j < cnt; ++j) {
-      const size_t c = c0 + j;
+      const uint32_t c = c0 + j;
       const int clean = __builtin_amdgcn_readlane(mine.clean, j);
       bool ok = false;
       if (clean) {
         const int e = __builtin_amdgcn_readlane(mine.e, j);
-        if (s >= ldexpf(1.0f, e) && s < ldexpf(1.0f, e + 1)) {
+        if (acc >= ldexpf(1.0f, e) && acc < ldexpf(1.0f, e + 1)) {
           const long long ie = (long long)readlane64((uint64_t)mine.ie, j);
           const long long io = (long long)readlane64((uint64_t)mine.io, j);
-          const long long M = (long long)ldexpf(s, 23 - e);
+          const long long M = (long long)ldexpf(acc, 23 - e);
           const long long Mn = M + ((M & 1) ? io : ie);
           if (Mn < (1ll << 24)) {
-            s = ldexpf((float)Mn, e - 23);
+            acc = ldexpf((float)Mn, e - 23);
             ok = true;
           }
         }
       }
       if (!ok) {
+        const uint32_t a = seg[s] + (c - c_begin) * SS_C;
         float x[SS_LANE];
 #pragma unroll
-        for (int r = 0; r < SS_LANE; ++r) {
-          const size_t i = c * SS_C + (size_t)l * SS_LANE + r;
-          x[r] = i < n ? ss_term(v[i], mode, mean) : 0.0f;  // + 0 leaves s unchanged
+        for (int q = 0; q < SS_LANE; ++q) {
+          const uint32_t i = a + (uint32_t)l * SS_LANE + q;
+          x[q] = i < e_end ? ss_term(v[i], mode, mean) : 0.0f;  // + 0 leaves the sum unchanged
         }
         for (int q = 0; q < WAVE; ++q) {
 #pragma unroll
-          for (int r = 0; r < SS_LANE; ++r) s = s + readlanef(x[r], q);
+          for (int r = 0; r < SS_LANE; ++r) acc = acc + readlanef(x[r], q);
         }
       }
     }
   }
-  if (l == 0) *out = s;
+  if (l == 0) out[s] = acc;
+}
+
+int exact_seq_sum_seg(const float *v, const uint32_t *seg_off, int nseg, size_t n_bound, int mode,
+                      const float *mean_dev, float *out, hipStream_t s) {
+  if (nseg <= 0) return RVP_OK;
+  const size_t bound = n_bound / SS_C + (size_t)nseg + 1;  // chunks: sum of ceil(len / 256)
+  Tmp coff, cs, ch;
+  RVP_TRY(coff.alloc((nseg + 1) * sizeof(uint32_t), s));
+  RVP_TRY(cs.alloc(bound * sizeof(double), s));
+  RVP_TRY(ch.alloc(bound * sizeof(SSChunk), s));
+  RVP_LAUNCH("ss_nchunks_k", s, ss_nchunks_k<<<ceil_div(nseg, 256), 256, 0, s>>>(seg_off, nseg, coff.as<uint32_t>()));
+  RVP_TRY(exclusive_scan_u32(coff.as<uint32_t>(), coff.as<uint32_t>(), nseg, coff.as<uint32_t>() + nseg, s));
+  RVP_LAUNCH("ss_chunksum_k", s, ss_chunksum_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff.as<uint32_t>(), nseg,
+                                                                              mode, mean_dev, cs.as<double>()));
+  RVP_LAUNCH("ss_compose_k", s, ss_compose_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff.as<uint32_t>(), nseg,
+                                                                            mode, mean_dev, cs.as<double>(),
+                                                                            ch.as<SSChunk>()));
+  RVP_LAUNCH("ss_drive_k", s, ss_drive_k<<<nseg, 64, 0, s>>>(v, seg_off, coff.as<uint32_t>(), nseg, mode, mean_dev,
+                                                             ch.as<SSChunk>(), out));
+  return RVP_OK;
+}
+
+__global__ void one_segment_k(uint32_t n, uint32_t *__restrict__ seg) {
+  if (threadIdx.x == 0) {
+    seg[0] = 0;
+    seg[1] = n;
+  }
 }
 
 int exact_seq_sum(const float *v, size_t n, int mode, const float *mean_dev, float *out_dev, hipStream_t s) {
-  if (n == 0) {
-    RVP_HIP(hipMemsetAsync(out_dev, 0, sizeof(float), s));
-    return RVP_OK;
-  }
-  const size_t nch = (n + SS_C - 1) / SS_C;
-  Tmp cs, ch;
-  RVP_TRY(cs.alloc(nch * sizeof(double), s));
-  RVP_TRY(ch.alloc(nch * sizeof(SSChunk), s));
-  RVP_LAUNCH("ss_chunksum_k", s, ss_chunksum_k<<<(unsigned)nch, 64, 0, s>>>(v, n, mode, mean_dev, cs.as<double>()));
-  RVP_LAUNCH("ss_compose_k", s, ss_compose_k<<<(unsigned)nch, 64, 0, s>>>(v, n, mode, mean_dev, cs.as<double>(),
-                                                                          ch.as<SSChunk>()));
-  RVP_LAUNCH("ss_drive_k", s, ss_drive_k<<<1, 64, 0, s>>>(v, n, mode, mean_dev, ch.as<SSChunk>(), nch, out_dev));
-  return RVP_OK;
+  RVP_CHECK(n < 0xffffff00ull, RVP_E_UNSUPPORTED, "sequential sum: more than 2^32 - 256 terms");
+  Tmp seg;
+  RVP_TRY(seg.alloc(2 * sizeof(uint32_t), s));
+  RVP_LAUNCH("one_segment_k", s, one_segment_k<<<1, 64, 0, s>>>((uint32_t)n, seg.as<uint32_t>()));
+  return exact_seq_sum_seg(v, seg.as<uint32_t>(), 1, n, mode, mean_dev, out_dev, s);
 }
 
 }  // namespace rvp
@@ -182,5 +242,6 @@ int exact_seq_sum(const float *v, size_t n, int mode, const float *mean_dev, flo
 extern "C" int rvp_seq_sum(const float *v, size_t n, int squared_deviation, const float *mean_dev, float *out_dev,
                            rvp_stream_t st) {
   RVP_CHECK(out_dev && (v || n == 0) && (!squared_deviation || mean_dev), RVP_E_ARG, "null");
+  RVP_TRY(rvp::pool_init());
   return rvp::exact_seq_sum(v, n, squared_deviation ? 1 : 0, mean_dev, out_dev, rvp::S(st));
 }
