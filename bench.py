@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sor-sample", type=int, default=30000, help="points of the O(N^2) SOR CPU sample")
     ap.add_argument("--cpu-normals-sample", type=int, default=100000)
+    ap.add_argument("--batch-steps", type=int, default=3, help="timed passes of the 256-cloud batch leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched vs per-cloud leg")
     return ap.parse_args()
 
 
@@ -92,6 +94,116 @@ class FramePipeline:
              model, C.byref(inl), C.byref(its), s)
         return {"n_in": self.n, "n_voxel": nv.value, "n_sor": ns.value, "inliers": inl.value,
                 "ransac_iters": its.value, "model": [float(v) for v in model]}
+
+
+class BatchPipeline:
+    """256 plane+sphere clouds back to back in HBM, one launch per stage and no host sync between stages:
+    exact-kNN SOR k=32 -> radius lists r=0.02 s_i capped at 64 on the kept points -> kNN normals k=24."""
+
+    K_SOR, K_NRM, CAP = 32, 24, 64
+
+    def __init__(self, clouds, scales, stream=None):
+        import rvpoint_amd as R
+        from rvpoint_amd._lib import call
+        self.call, self.s = call, stream
+        self.sizes = np.array([len(c) for c in clouds], np.int64)
+        self.ncl = len(clouds)
+        self.off_host = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.uint32)
+        self.n = int(self.off_host[-1])
+        n, D = self.n, R.DeviceArray
+        self.cloud = R.DeviceCloud(np.concatenate(clouds))
+        self.off = D.from_numpy(self.off_host)
+        self.radius_host = (0.02 * np.asarray(scales)).astype(np.float32)
+        self.radius = D.from_numpy(self.radius_host)
+        self.sor, self.sor_off = D((n, 3)), D(self.ncl + 1, np.uint32)
+        self.sx, self.sy, self.sz = D(n), D(n), D(n)
+        self.idx, self.d2, self.cnt = D((n, self.CAP), np.int32), D((n, self.CAP)), D(n, np.int32)
+        self.nx, self.ny, self.nz = D(n), D(n), D(n)
+
+    def step(self):
+        call, s, n = self.call, self.s, self.n
+        call("rvp_batch_sor_exact", *self.cloud.ptrs(), self.off.ptr, self.ncl, n, self.K_SOR, 1.0, self.sor.ptr,
+             self.sor_off.ptr, None, None, s)
+        call("rvp_aos_to_soa", self.sor.ptr, n, self.sx.ptr, self.sy.ptr, self.sz.ptr, s)
+        kept = (self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, self.ncl, n)
+        call("rvp_batch_radius_search", *kept, self.radius.ptr, None, self.CAP, self.idx.ptr, self.d2.ptr,
+             self.cnt.ptr, s)
+        call("rvp_batch_normals_knn", *kept, self.K_NRM, 0.0, 0.0, 0.0, self.nx.ptr, self.ny.ptr, self.nz.ptr, s)
+
+    def kept_total(self):
+        return int(self.sor_off.numpy()[-1])
+
+
+class PerCloudPipeline:
+    """The same three stages one cloud per launch (the plain GPU path): SOR, host read of the kept count,
+    hash build and radius lists, kNN normals, cloud after cloud on one stream."""
+
+    def __init__(self, bp: BatchPipeline):
+        from rvpoint_amd._lib import call
+        self.bp, self.call = bp, call
+        h = C.c_void_p()
+        call("rvp_hash_create", C.byref(h))
+        self.hash = h.value
+
+    def run_cloud(self, c):
+        bp, call, s = self.bp, self.call, self.bp.s
+        o, m = int(bp.off_host[c]), int(bp.sizes[c])
+        at = lambda arr, width=1, esz=4: arr.ptr + o * width * esz  # noqa: E731
+        ns = C.c_size_t()
+        call("rvp_sor_exact", *(p + 4 * o for p in bp.cloud.ptrs()), m, bp.K_SOR, 1.0, at(bp.sor, 3), C.byref(ns),
+             None, None, s)
+        k = ns.value
+        call("rvp_aos_to_soa", at(bp.sor, 3), k, at(bp.sx), at(bp.sy), at(bp.sz), s)
+        r = float(bp.radius_host[c])
+        call("rvp_hash_build", self.hash, at(bp.sx), at(bp.sy), at(bp.sz), k, r, s)
+        call("rvp_radius_search", self.hash, None, None, None, k, r, bp.CAP, at(bp.idx, bp.CAP), at(bp.d2, bp.CAP),
+             at(bp.cnt), None, s)
+        call("rvp_normals_knn", at(bp.sx), at(bp.sy), at(bp.sz), k, bp.K_NRM, 0.0, 0.0, 0.0, at(bp.nx), at(bp.ny),
+             at(bp.nz), s)
+
+
+def batch_leg(R, args):
+    """Batched (one launch per stage) vs per-cloud on the 256-cloud workload: Mpts/s of input points."""
+    from rvpoint_amd import scenes
+    print("bench: generating 256-cloud batch", file=sys.stderr, flush=True)
+    clouds, scales = scenes.batch_clouds(with_scales=True)
+    bp = BatchPipeline(clouds, scales)
+    del clouds
+    for _ in range(max(args.warmup, 1)):
+        bp.step()
+    R.synchronize()
+    bt = []
+    for _ in range(args.batch_steps):
+        t0 = time.perf_counter()
+        bp.step()
+        R.synchronize()
+        bt.append((time.perf_counter() - t0) * 1e3)
+    kept = bp.kept_total()
+    print(f"bench: batched {np.median(bt):.2f} ms/batch; per-cloud leg", file=sys.stderr, flush=True)
+    pc = PerCloudPipeline(bp)
+    for c in range(bp.ncl):  # warm every cloud size once
+        pc.run_cloud(c)
+    R.synchronize()
+    pt, per_cloud_ms = [], []
+    for _ in range(args.batch_steps):
+        t0 = time.perf_counter()
+        for c in range(bp.ncl):
+            t1 = time.perf_counter()
+            pc.run_cloud(c)
+            R.synchronize()
+            per_cloud_ms.append((time.perf_counter() - t1) * 1e3)
+        pt.append((time.perf_counter() - t0) * 1e3)
+    b_ms, p_ms = float(np.median(bt)), float(np.median(pt))
+    return {
+        "workload": "256 plane+sphere clouds, log-uniform 2e3..5e5 points, scale U[1,20]; SOR k=32 -> "
+                    "radius r=0.02*scale cap 64 -> normals k=24",
+        "points": bp.n, "kept_after_sor": kept, "clouds": bp.ncl, "steps": args.batch_steps,
+        "batched_ms": round(b_ms, 3), "batched_p99_ms": round(float(np.percentile(bt, 99)), 3),
+        "batched_mpts_s": round(bp.n / (b_ms * 1e-3) / 1e6, 3),
+        "per_cloud_ms": round(p_ms, 3), "per_cloud_mpts_s": round(bp.n / (p_ms * 1e-3) / 1e6, 3),
+        "per_cloud_p99_ms_per_cloud": round(float(np.percentile(per_cloud_ms, 99)), 3),
+        "speedup": round(p_ms / b_ms, 3),
+    }
 
 
 def cpu_baseline(frame, info, args):
@@ -242,6 +354,7 @@ def main():
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(per_kernel.items(),
                                                                      key=lambda kv: -kv[1][1])[:12]},
     }
+    line["batched"] = None if args.no_batch else batch_leg(R, args)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (reference scalar path, 1 thread)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(frame, info, args)

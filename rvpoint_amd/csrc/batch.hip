@@ -18,6 +18,25 @@ namespace rvp {
 // ------------------------------------------------------------------------------------------------
 // segmented grid build
 // ------------------------------------------------------------------------------------------------
+// out[c] = running max of min(in[c], n_bound), out[0] = 0: one wave, 64 offsets per step
+__global__ __launch_bounds__(64) void seg_offsets_k(const uint32_t *__restrict__ in, int ncl, uint32_t n_bound,
+                                                    uint32_t *__restrict__ out) {
+  const int l = lane();
+  uint32_t carry = 0;
+  for (int b = 0; b <= ncl; b += 64) {
+    const int c = b + l;
+    uint32_t v = c == 0 ? 0u : (c <= ncl ? min(in[c], n_bound) : n_bound);
+    v = max(v, carry);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(v, d);
+      if (l >= d) v = max(v, o);
+    }
+    if (c <= ncl) out[c] = v;
+    carry = __shfl(v, 63);
+  }
+}
+
 __global__ __launch_bounds__(256) void seg_minmax_k(const float *__restrict__ x, const float *__restrict__ y,
                                                      const float *__restrict__ z, const uint32_t *__restrict__ off,
                                                      float *__restrict__ mm) {
@@ -73,11 +92,11 @@ __global__ void seg_knn_c0_k(const float *__restrict__ mm, const uint32_t *__res
     ext[j] = m[j + 3] >= m[j] ? (double)m[j + 3] - (double)m[j] : 0.0;
     emax = fmax(emax, ext[j]);
   }
-  SegDesc d{};
+  SegDesc d{};  // one cell unless a grid below fits: every point then maps to cell 0
+  d.cell = 1.0f;
+  d.g[0] = d.g[1] = d.g[2] = 1;
   if (!(n > 0.0) || !(emax > 0.0) || !(emax < 1e30)) {
     c0[c] = 1.0f;
-    d.cell = 1.0f;
-    d.g[0] = d.g[1] = d.g[2] = 1;
     d0[c] = d;
     return;
   }
@@ -250,10 +269,13 @@ __global__ void seg_fill_gaps_k(const SegGap *__restrict__ gaps, const uint32_t 
 int SegGrid::build(const float *x, const float *y, const float *z, const uint32_t *off_, int ncl_, size_t n_bound_,
                  bool mode_knn, int K, const float *cell_dev, hipStream_t s) {
   ncl = ncl_;
-  off = off_;
   n_bound = n_bound_;
   RVP_CHECK(ncl > 0, RVP_E_ARG, "no clouds");
+  RVP_CHECK(n_bound > 0, RVP_E_ARG, "n_bound must be positive");
   RVP_CHECK(n_bound < 0xfffffff0ull / 5, RVP_E_UNSUPPORTED, "batch too large");
+  RVP_TRY(offs.alloc((ncl + 1) * sizeof(uint32_t), s));
+  RVP_LAUNCH("seg_offsets_k", s, seg_offsets_k<<<1, 64, 0, s>>>(off_, ncl, (uint32_t)n_bound, offs.as<uint32_t>()));
+  off = offs.as<uint32_t>();
   const unsigned cb = ceil_div(ncl, 64);
   RVP_TRY(mm.alloc(ncl * 6 * sizeof(float), s));
   RVP_TRY(desc.alloc(ncl * sizeof(SegDesc), s));
@@ -432,10 +454,10 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
   RVP_TRY(vsum.alloc(nclouds * sizeof(float), s));
   RVP_TRY(thr.alloc(nclouds * sizeof(float), s));
   const unsigned cb = ceil_div(nclouds, 64);
-  RVP_TRY(exact_seq_sum_seg(mp, offsets, nclouds, n_bound, 0, nullptr, sums.as<float>(), s));
-  RVP_LAUNCH("seg_mean_k", s, seg_mean_k<<<cb, 64, 0, s>>>(offsets, nclouds, sums.as<float>(), mean.as<float>()));
-  RVP_TRY(exact_seq_sum_seg(mp, offsets, nclouds, n_bound, 1, mean.as<float>(), vsum.as<float>(), s));
-  RVP_LAUNCH("seg_threshold_k", s, seg_threshold_k<<<cb, 64, 0, s>>>(offsets, nclouds, alpha, mean.as<float>(),
+  RVP_TRY(exact_seq_sum_seg(mp, G.off, nclouds, n_bound, 0, nullptr, sums.as<float>(), s));
+  RVP_LAUNCH("seg_mean_k", s, seg_mean_k<<<cb, 64, 0, s>>>(G.off, nclouds, sums.as<float>(), mean.as<float>()));
+  RVP_TRY(exact_seq_sum_seg(mp, G.off, nclouds, n_bound, 1, mean.as<float>(), vsum.as<float>(), s));
+  RVP_LAUNCH("seg_threshold_k", s, seg_threshold_k<<<cb, 64, 0, s>>>(G.off, nclouds, alpha, mean.as<float>(),
                                                                      vsum.as<float>(), thr.as<float>()));
   uint8_t *kp = keep;
   if (!kp) {
@@ -443,14 +465,14 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
     kp = kt.as<uint8_t>();
   }
   const unsigned nb = ceil_div(n_bound, 256);
-  RVP_LAUNCH("seg_keep_k", s, seg_keep_k<<<nb, 256, 0, s>>>(mp, offsets, nclouds, thr.as<float>(), (uint32_t)n_bound, kp));
+  RVP_LAUNCH("seg_keep_k", s, seg_keep_k<<<nb, 256, 0, s>>>(mp, G.off, nclouds, thr.as<float>(), (uint32_t)n_bound, kp));
   // stable compaction over the whole batch keeps the clouds in order; a cloud's output starts where the
   // compaction positions its first point
   RVP_TRY(pos.alloc(n_bound * sizeof(uint32_t), s));
   RVP_TRY(kept.alloc(sizeof(uint32_t), s));
   RVP_TRY(compact_xyz_aos_pos(x, y, z, kp, 1, n_bound, out_aos, pos.as<uint32_t>(), kept.as<uint32_t>(), s));
   RVP_LAUNCH("seg_out_offsets_k", s, seg_out_offsets_k<<<ceil_div(nclouds + 1, 64), 64, 0, s>>>(
-                                         offsets, nclouds, (uint32_t)n_bound, pos.as<uint32_t>(), kept.as<uint32_t>(),
+                                         G.off, nclouds, (uint32_t)n_bound, pos.as<uint32_t>(), kept.as<uint32_t>(),
                                          out_offsets));
   return RVP_OK;
 }

@@ -52,9 +52,9 @@ __device__ __forceinline__ GridView seg_view(const SegArrays &A, int c) {
 
 // Host owner of a segmented grid (stream-ordered temporaries).
 struct SegGrid {
-  Tmp desc, mm, cellv, ncells, toff, xs, ys, zs, perm, keys, cstart;
+  Tmp desc, mm, cellv, ncells, toff, xs, ys, zs, perm, keys, cstart, offs;
   int ncl = 0;
-  const uint32_t *off = nullptr;
+  const uint32_t *off = nullptr;  // the caller's offsets made non-decreasing, off[0] = 0, off[ncl] <= n_bound
   size_t n_bound = 0;
 
   SegArrays arrays() const {
@@ -62,7 +62,9 @@ struct SegGrid {
                      desc.as<SegDesc>(), off, ncl};
   }
 
-  // Build the grids of the ncl clouds [off[c], off[c+1]) of x/y/z (n_bound >= off[ncl], device offsets).
+  // Build the grids of the ncl clouds [off[c], off[c+1]) of x/y/z (device offsets, n_bound >= off[ncl]).
+  // The offsets are first clamped into a valid partition of [0, min(off[ncl], n_bound)); every later
+  // kernel of the batch reads that copy (member off), so no index leaves the n_bound arrays.
   // mode_knn: edge from the k-NN estimate for K neighbours incl. self; otherwise edge = cell_dev[c].
   int build(const float *x, const float *y, const float *z, const uint32_t *off, int ncl, size_t n_bound,
             bool mode_knn, int K, const float *cell_dev, hipStream_t s);

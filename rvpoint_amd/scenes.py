@@ -152,14 +152,17 @@ def batch_sizes(count=256, lo=2_000, hi=500_000, seed=3000):
     return np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
 
 
-def batch_clouds(count=256, seed=3000):
+def batch_clouds(count=256, seed=3000, with_scales=False):
+    """`count` plane+sphere clouds with log-uniform sizes in [2e3, 5e5], each scaled by s ~ U[1, 20] (all
+    lengths, noise included); with_scales also returns the scales (the radius of cloud i is 0.02 s_i)."""
     sizes = batch_sizes(count, seed=seed)
-    out = []
+    out, scales = [], []
     for i, n in enumerate(sizes):
         rng = _rng(seed + 1 + i)
         s = rng.uniform(1.0, 20.0)
+        scales.append(s)
         out.append(plane_sphere_cloud(int(n), seed=seed + 1 + i, noise=0.002, scale=s))
-    return out
+    return (out, np.array(scales)) if with_scales else out
 
 
 def lidar_stream(frames=32, seed=4004):

@@ -53,3 +53,11 @@ def test_errors_without_device(built):
     assert L.rvp_radius_search(None, None, None, None, 0, C.c_float(1.0), 4, None, None, None, None, None) \
         == _lib.RVP_E_ARG
     assert b"null" in L.rvp_last_error()
+    # batched entry points: empty batch, k out of range, missing buffers
+    p = C.c_void_p(16)
+    assert L.rvp_batch_sor_exact(p, p, p, p, 0, 8, 4, C.c_float(1.0), p, p, None, None, None) == _lib.RVP_E_ARG
+    assert L.rvp_batch_sor_exact(p, p, p, p, 1, 8, 64, C.c_float(1.0), p, p, None, None, None) \
+        == _lib.RVP_E_UNSUPPORTED
+    assert L.rvp_batch_radius_search(p, p, p, p, 1, 8, None, None, 8, None, None, p, None) == _lib.RVP_E_ARG
+    assert L.rvp_batch_normals_knn(p, p, p, p, 1, 8, 0, C.c_float(0), C.c_float(0), C.c_float(0), p, p, p,
+                                   None) == _lib.RVP_E_UNSUPPORTED
