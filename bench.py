@@ -4,9 +4,12 @@
 Workload (one "step"): a 64-ring x 18,750-azimuth LiDAR frame (1.2M rays, ~1.04M returns, resident in HBM)
   voxel grid 0.1 m -> exact-kNN SOR k=50 alpha=1 -> normals, radius 0.5 m capped at 64 neighbours
   -> RANSAC plane, 1000 hypotheses, threshold 0.05 m
-Metric: input points per second of the whole pipeline (Mpts/s), with p50/p99 ms per frame.
+Metric: input points per second of the whole pipeline (Mpts/s), with p50/p99 ms per frame, for the frame run
+as a batch of one (stages chained on device-resident counts; the value) and through the single-cloud entry
+points with a host round trip per stage (plain_path); both must give the same voxel/SOR counts and plane.
 Also reported: the dominant kernel's HBM-roofline fraction (HIP events around its launches on its own
-stream) and the reference's scalar path timed on this host (oracle/_ref, or the oracle port).
+stream), the 256-cloud batched vs per-cloud leg, and the reference's scalar path timed on this host
+(oracle/_ref, or the oracle port).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -36,10 +39,12 @@ STAGE_BYTES = {
     "ransac": (12, 0),          # 12 per point per launch evaluating all its hypotheses
 }
 KERNEL_STAGE = {
-    "voxel_keys_k": "voxel", "voxel_centroid_k": "voxel", "voxel_minmax_k": "voxel",
-    "sor_knn_means_k": "sor", "sor_stats_k": "sor", "sor_keep_k": "sor",
-    "keys_k": "hash", "gather_k": "hash", "cell_start_k": "hash",
-    "normals_topk_k": "normals", "plane_count_k": "ransac",
+    "voxel_keys_k": "voxel", "voxel_centroid_k": "voxel", "voxel_minmax_k": "voxel", "seg_vox_keys_k": "voxel",
+    "seg_vox_minmax_k": "voxel",
+    "sor_knn_means_k": "sor", "sor_stats_k": "sor", "sor_keep_k": "sor", "seg_sor_means_k": "sor",
+    "keys_k": "hash", "gather_k": "hash", "cell_start_k": "hash", "seg_keys_k": "hash", "seg_gather_k": "hash",
+    "seg_cell_start_k": "hash", "seg_fill_gaps_k": "hash",
+    "normals_topk_k": "normals", "seg_normals_k": "normals", "plane_count_k": "ransac", "seg_plane_count_k": "ransac",
 }
 
 
@@ -53,11 +58,14 @@ def parse():
     ap.add_argument("--cpu-normals-sample", type=int, default=100000)
     ap.add_argument("--batch-steps", type=int, default=3, help="timed passes of the 256-cloud batch leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs per-cloud leg")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="one device-to-device copy of this many MiB before warmup (HBM counter calibration)")
     return ap.parse_args()
 
 
 class FramePipeline:
-    """voxel -> SOR -> normals -> RANSAC on device buffers, one frame per step."""
+    """The plain GPU path: voxel -> SOR -> normals -> RANSAC through the single-cloud entry points, one frame
+    per step; every stage hands its point count back to the host before the next one starts."""
 
     def __init__(self, frame_xyz, stream=None):
         import rvpoint_amd as R
@@ -206,6 +214,64 @@ def batch_leg(R, args):
     }
 
 
+class DeviceFramePipeline:
+    """The same frame as a batch of one: every stage reads its point count from the previous stage's device
+    offsets, so nothing waits on the host until RANSAC's adaptive stop is replayed at the end."""
+
+    def __init__(self, frame_xyz, stream=None):
+        import rvpoint_amd as R
+        from rvpoint_amd._lib import call
+        self.call, self.s = call, stream
+        self.n = n = len(frame_xyz)
+        D = R.DeviceArray
+        self.frame = R.DeviceCloud(frame_xyz)
+        self.off = D.from_numpy(np.array([0, n], np.uint32))
+        self.leaf = D.from_numpy(np.array([0.1], np.float32))
+        self.radius = D.from_numpy(np.array([0.5], np.float32))
+        self.vox, self.vox_off, self.status = D((n, 3)), D(2, np.uint32), D(1, np.uint32)
+        self.vx, self.vy, self.vz = D(n), D(n), D(n)
+        self.sor, self.sor_off = D((n, 3)), D(2, np.uint32)
+        self.sx, self.sy, self.sz = D(n), D(n), D(n)
+        self.nx, self.ny, self.nz = D(n), D(n), D(n)
+        self.k, self.alpha, self.max_nn, self.vp = 50, 1.0, 64, (0.0, 0.0, 1.7)
+        self.thr, self.iters = 0.05, 1000
+        self.model = np.zeros(4, np.float32)
+        self.inl = np.zeros(1, np.int32)
+        self.its = np.zeros(1, np.int32)
+
+    def step(self):
+        call, s, n = self.call, self.s, self.n
+        call("rvp_batch_voxel", *self.frame.ptrs(), self.off.ptr, 1, n, self.leaf.ptr, self.vox.ptr,
+             self.vox_off.ptr, self.status.ptr, s)
+        call("rvp_aos_to_soa", self.vox.ptr, n, self.vx.ptr, self.vy.ptr, self.vz.ptr, s)
+        call("rvp_batch_sor_exact", self.vx.ptr, self.vy.ptr, self.vz.ptr, self.vox_off.ptr, 1, n, self.k,
+             self.alpha, self.sor.ptr, self.sor_off.ptr, None, None, s)
+        call("rvp_aos_to_soa", self.sor.ptr, n, self.sx.ptr, self.sy.ptr, self.sz.ptr, s)
+        kept = (self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, 1, n)
+        call("rvp_batch_normals_radius", *kept, self.radius.ptr, None, self.max_nn, *self.vp, self.nx.ptr,
+             self.ny.ptr, self.nz.ptr, s)
+        P = C.POINTER
+        call("rvp_batch_ransac_plane", *kept, self.thr, self.iters, 1e-6, 0.99,
+             self.model.ctypes.data_as(P(C.c_float)), self.inl.ctypes.data_as(P(C.c_int)),
+             self.its.ctypes.data_as(P(C.c_int)), s)
+
+    def info(self):
+        return {"n_in": self.n, "n_voxel": int(self.vox_off.numpy()[1]), "n_sor": int(self.sor_off.numpy()[1]),
+                "inliers": int(self.inl[0]), "ransac_iters": int(self.its[0]),
+                "model": [float(v) for v in self.model]}
+
+
+def time_steps(R, fn, steps):
+    times = []
+    t_all = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        fn()
+        times.append((time.perf_counter() - t0) * 1e3)
+    R.synchronize()
+    return (time.perf_counter() - t_all) * 1e3 / steps, times
+
+
 def cpu_baseline(frame, info, args):
     """The reference's scalar path (oracle/_ref when present, else the oracle port) on a bounded sample of the
     same workload, scaled to the full frame: voxel and RANSAC at full size, exact-kNN SOR on a subset scaled by
@@ -303,9 +369,24 @@ def main():
         sys.exit(1)
     print("bench: generating LiDAR frame (64 rings x 18,750 azimuths)", file=sys.stderr, flush=True)
     frame = scenes.lidar_frame()
-    pipe = FramePipeline(frame)
+    if args.copy_mib > 0:
+        nb = args.copy_mib << 20
+        a, b = R.DeviceArray(nb // 4), R.DeviceArray(nb // 4)
+        _lib.call("rvp_memcpy_d2d", b.ptr, a.ptr, nb, None)
+        R.synchronize()
+        del a, b
+    # the plain path: single-cloud entry points, a host round trip per stage
+    plain = FramePipeline(frame)
     for _ in range(args.warmup):
-        info = pipe.step()
+        info_plain = plain.step()
+    R.synchronize()
+    plain_ms, plain_times = time_steps(R, plain.step, args.steps)
+    info_plain = plain.step()
+    R.synchronize()
+    # the measured path: the frame as a batch of one, stages chained on device offsets
+    pipe = DeviceFramePipeline(frame)
+    for _ in range(args.warmup):
+        pipe.step()
     R.synchronize()
     # which kernel dominates: two profiled steps outside the timed region
     _lib.profile_enable(True)
@@ -319,17 +400,11 @@ def main():
           file=sys.stderr, flush=True)
     # timed region: HIP events only around the dominant kernel's launches, on the stream it runs on
     _lib.profile_enable(True, dominant)
-    times = []
-    t_all = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        info = pipe.step()
-        times.append((time.perf_counter() - t0) * 1e3)
-    R.synchronize()
-    wall = time.perf_counter() - t_all
+    ms, times = time_steps(R, pipe.step, args.steps)
     _lib.profile_enable(False)
     prof = _lib.profile_collect()
-    ms = wall * 1e3 / args.steps
+    info = pipe.info()
+    same = all(info[k] == info_plain[k] for k in ("n_voxel", "n_sor", "inliers", "model"))
     rl, prof = roofline(prof, info, args.steps)
     line = {
         "metric": "Mpts/s hash-grid radius/kNN+SOR+normals; batched vs per-cloud; p99 ms/frame",
@@ -349,7 +424,12 @@ def main():
         "config": {"workload": "lidar_frame: 64 rings x 18750 azimuths (1.2M rays), voxel 0.1 -> SOR k=50 "
                                "-> normals r=0.5 cap 64 -> RANSAC 1000 thr 0.05",
                    "points": info["n_in"], "n_voxel": info["n_voxel"], "n_sor": info["n_sor"],
-                   "inliers": info["inliers"], "parallelism": "single GPU"},
+                   "inliers": info["inliers"], "parallelism": "single GPU",
+                   "path": "frame as a batch of one: stages chained on device offsets, one host wait (RANSAC)"},
+        "plain_path": {"ms_per_step": round(plain_ms, 4), "p50_ms": round(float(np.percentile(plain_times, 50)), 4),
+                       "p99_ms": round(float(np.percentile(plain_times, 99)), 4),
+                       "mpts_s": round(info["n_in"] / (plain_ms * 1e-3) / 1e6, 4),
+                       "same_results": bool(same)},
         "roofline": rl,
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(per_kernel.items(),
                                                                      key=lambda kv: -kv[1][1])[:12]},

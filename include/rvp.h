@@ -46,6 +46,10 @@ int rvp_memcpy_d2d(void *dst, const void *src, size_t bytes, rvp_stream_t s);
 int rvp_memset(void *dst, int value, size_t bytes, rvp_stream_t s);
 int rvp_stream_create(rvp_stream_t *s);
 int rvp_stream_destroy(rvp_stream_t s);
+/* Temporaries come from a cache keyed by (device, stream) and stay cached between calls; trim() waits for
+ * the device and returns every cached block of the current device to the driver (destroying a stream
+ * returns that stream's blocks). */
+int rvp_cache_trim(void);
 int rvp_stream_sync(rvp_stream_t s);
 int rvp_device_sync(void);
 /* event timing on a stream: start/stop record, elapsed in ms */
@@ -64,6 +68,13 @@ int rvp_profile_enable(int on);
 /* restrict the timing to launches of one kernel (NULL or "" = all kernels) */
 int rvp_profile_filter(const char *kernel);
 int rvp_profile_collect(char *buf, size_t buflen, size_t *needed);
+/* Process-wide tuning knobs (no counterpart in the reference; results never depend on them, only speed):
+ *   "knn_cell_div"         k-NN grid edge = estimated k-th neighbour distance / this (default 2.8, [0.25, 16])
+ *   "seg_cells_per_point"  dense cells a batched grid may give each point of a cloud before coarsening
+ *                          (default 32, [1, 64]; at least 2^17 cells per cloud)
+ * set() returns RVP_E_ARG for an unknown name or a value out of range. */
+int rvp_tune_set(const char *name, double value);
+int rvp_tune_get(const char *name, double *value);
 
 /* ---- D1 / D2 (src/core/rvv_common.cpp:9-107) ---------------------------------------------------------
  * D1 get_dist_sq_rvv: out[i] = ((x-qx)^2 + (y-qy)^2) + (z-qz)^2, each op rounded once.
@@ -159,7 +170,8 @@ int rvp_voxel(const float *x, const float *y, const float *z, size_t n, float le
 /* ---- R1: ransac_plane_sc / ransac_plane_rvv (src/segmentation/ransac_plane.cpp:55-111) -------------
  * Hypotheses from the glibc TYPE_3 rand() stream after srand(0) (3 draws per iteration), plane
  * coefficients as compute_plane_coefficients (:18-50), inlier counts on the GPU, adaptive stop replayed
- * on the host.  model (host float[4]); *inliers (host).  *iterations (host, may be NULL). */
+ * on the host.  model (host float[4]); *inliers (host).  *iterations (host, may be NULL): iterations the
+ * reference's loop runs (:68), which can exceed the final adaptive bound when it drops below the count. */
 int rvp_ransac_plane(const float *x, const float *y, const float *z, size_t n, float dist_thresh,
                      int max_iters, float collinear_thresh, float probability, float *model, int *inliers,
                      int *iterations, rvp_stream_t s);
@@ -177,7 +189,9 @@ int rvp_plane_extract(const float *x, const float *y, const float *z, size_t n, 
  * The clouds lie back to back in x/y/z; offsets[nclouds+1] (device, uint32) delimit them; n_bound >= the
  * total number of points (buffers are sized by it; the true total is read from offsets on the device).
  * Every cloud's result equals what the single-cloud entry point returns for it.  These calls do not wait
- * on the host: outputs (including out_offsets) are device memory, ordered on the stream. */
+ * on the host (rvp_batch_ransac_plane excepted): outputs (including out_offsets) are device memory, ordered
+ * on the stream, so stages chain without a host round trip (the next stage takes out_offsets as its
+ * offsets and the same n_bound). */
 
 /* S1 per cloud (sor_sc, statistical_outlier_removal.cpp:19-63): kept points of cloud c at
  * out_aos[3*out_offsets[c] ..], in input order; keep / means (device, n_bound) may be NULL. */
@@ -194,6 +208,25 @@ int rvp_batch_radius_search(const float *x, const float *y, const float *z, cons
 int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
                           size_t n_bound, int k, float vpx, float vpy, float vpz, float *nx, float *ny, float *nz,
                           rvp_stream_t s);
+/* N2 per cloud with a cap (normal_estimation.cpp:189-214 neighbourhood, as rvp_normals_radius with
+ * max_nn in [1, 64]): radius[c] / cell[c] device arrays (cell NULL: radius). */
+int rvp_batch_normals_radius(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                             size_t n_bound, const float *radius, const float *cell, int max_nn, float vpx, float vpy,
+                             float vpz, float *nx, float *ny, float *nz, rvp_stream_t s);
+/* V1 per cloud (voxel_grid_downsamp.cpp:183-213, as rvp_voxel): leaf[c] device array; voxels of cloud c at
+ * out_aos[3*out_offsets[c] ..] in lexicographic (vx, vy, vz) order.  status[c] (device) = 1 when cloud c's
+ * voxel key range exceeds 2^48 (its output is then one centroid of the whole cloud), else 0. */
+int rvp_batch_voxel(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                    size_t n_bound, const float *leaf, float *out_aos, uint32_t *out_offsets, uint32_t *status,
+                    rvp_stream_t s);
+/* R1 per cloud (ransac_plane.cpp:55-111, as rvp_ransac_plane): every hypothesis of every cloud is counted
+ * in one launch, then the adaptive stop is replayed per cloud on the host.  Waits for the stream; outputs
+ * are host arrays: models_out[4*nclouds], inliers_out[nclouds], iterations_out[nclouds] (may be NULL).
+ * Unlike ransac_plane_sc it leaves the process rand() state alone. */
+int rvp_batch_ransac_plane(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                           size_t n_bound, float dist_thresh, int max_iters, float collinear_thresh,
+                           float probability, float *models_out, int *inliers_out, int *iterations_out,
+                           rvp_stream_t s);
 
 #ifdef __cplusplus
 }

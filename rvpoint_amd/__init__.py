@@ -4,8 +4,11 @@ from ._lib import RvpError, gpu_available, lib  # noqa: F401
 from .api import (  # noqa: F401
     Batch,
     batch_normals_knn,
+    batch_normals_radius,
     batch_radius_search,
+    batch_ransac_plane,
     batch_sor_exact,
+    batch_voxel,
     PointCloudSoA,
     SpatialHash,
     extract_plane_inliers_outliers_rvv,

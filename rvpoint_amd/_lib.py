@@ -54,6 +54,9 @@ _SIGS = {
     "rvp_profile_enable": [i32],
     "rvp_profile_collect": [C.c_char_p, sz, psz],
     "rvp_profile_filter": [C.c_char_p],
+    "rvp_cache_trim": [],
+    "rvp_tune_set": [C.c_char_p, C.c_double],
+    "rvp_tune_get": [C.c_char_p, C.POINTER(C.c_double)],
     "rvp_dist_sq": [vp, vp, vp, sz, fl, fl, fl, vp, vp],
     "rvp_inds_in_radius": [vp, vp, vp, vp, sz, fl, fl, fl, fl, vp, vp, psz, vp],
     "rvp_hash_create": [pvp],
@@ -81,6 +84,9 @@ _SIGS = {
     "rvp_batch_sor_exact": [vp, vp, vp, vp, i32, sz, i32, fl, vp, vp, vp, vp, vp],
     "rvp_batch_radius_search": [vp, vp, vp, vp, i32, sz, vp, vp, i32, vp, vp, vp, vp],
     "rvp_batch_normals_knn": [vp, vp, vp, vp, i32, sz, i32, fl, fl, fl, vp, vp, vp, vp],
+    "rvp_batch_normals_radius": [vp, vp, vp, vp, i32, sz, vp, vp, i32, fl, fl, fl, vp, vp, vp, vp],
+    "rvp_batch_voxel": [vp, vp, vp, vp, i32, sz, vp, vp, vp, vp, vp],
+    "rvp_batch_ransac_plane": [vp, vp, vp, vp, i32, sz, fl, i32, fl, fl, pfl, pi32, pi32, vp],
 }
 _RESTYPE = {"rvp_last_error": C.c_char_p, "rvp_version": C.c_char_p}
 
@@ -111,6 +117,15 @@ def call(name, *args):
     if st != RVP_OK:
         raise RvpError(st, lib().rvp_last_error().decode())
     return st
+
+
+def tune(name, value=None):
+    """Read (value None) or set a process-wide tuning knob; returns the value in effect."""
+    if value is not None:
+        call("rvp_tune_set", name.encode(), float(value))
+    v = C.c_double()
+    call("rvp_tune_get", name.encode(), C.byref(v))
+    return v.value
 
 
 def profile_enable(on=True, kernel=None):

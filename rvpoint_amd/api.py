@@ -411,3 +411,39 @@ def batch_normals_knn(batch: Batch, k, vp=(0.0, 0.0, 0.0), stream=None):
     call("rvp_batch_normals_knn", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, int(k), float(vp[0]),
          float(vp[1]), float(vp[2]), nx.ptr, ny.ptr, nz.ptr, stream)
     return np.stack([nx.numpy()[:batch.n], ny.numpy()[:batch.n], nz.numpy()[:batch.n]], 1)
+
+
+def batch_normals_radius(batch: Batch, radius, max_nn, vp=(0.0, 0.0, 0.0), stream=None):
+    """N2 per cloud with a cap (radius per cloud or scalar): (n, 3) normals by batch position."""
+    nb = max(batch.n, 1)
+    r = DeviceArray.from_numpy(np.broadcast_to(np.asarray(radius, np.float32), (batch.ncl,)).copy(), stream)
+    nx, ny, nz = (DeviceArray(nb, np.float32) for _ in range(3))
+    call("rvp_batch_normals_radius", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, r.ptr, None, int(max_nn),
+         float(vp[0]), float(vp[1]), float(vp[2]), nx.ptr, ny.ptr, nz.ptr, stream)
+    return np.stack([nx.numpy()[:batch.n], ny.numpy()[:batch.n], nz.numpy()[:batch.n]], 1)
+
+
+def batch_voxel(batch: Batch, leaf, stream=None):
+    """V1 per cloud (leaf per cloud or scalar): (list of AoS voxel centroids, status per cloud)."""
+    nb = max(batch.n, 1)
+    lf = DeviceArray.from_numpy(np.broadcast_to(np.asarray(leaf, np.float32), (batch.ncl,)).copy(), stream)
+    out = DeviceArray((nb, 3), np.float32)
+    out_off = DeviceArray(batch.ncl + 1, np.uint32)
+    status = DeviceArray(batch.ncl, np.uint32)
+    call("rvp_batch_voxel", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, lf.ptr, out.ptr, out_off.ptr,
+         status.ptr, stream)
+    o = out_off.numpy()
+    res = out.numpy()
+    return [res[o[c]:o[c + 1]].copy() for c in range(batch.ncl)], status.numpy()
+
+
+def batch_ransac_plane(batch: Batch, dist_thresh, max_iters, collinear_thresh=1e-6, probability=0.99, stream=None):
+    """R1 per cloud: (models [ncl, 4], inliers [ncl], iterations [ncl])."""
+    m = np.zeros((batch.ncl, 4), np.float32)
+    inl = np.zeros(batch.ncl, np.int32)
+    its = np.zeros(batch.ncl, np.int32)
+    call("rvp_batch_ransac_plane", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, max(batch.n, 1),
+         float(dist_thresh), int(max_iters), float(collinear_thresh), float(probability),
+         m.ctypes.data_as(C.POINTER(C.c_float)), inl.ctypes.data_as(C.POINTER(C.c_int)),
+         its.ctypes.data_as(C.POINTER(C.c_int)), stream)
+    return m, inl, its

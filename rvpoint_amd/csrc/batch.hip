@@ -37,12 +37,29 @@ __global__ __launch_bounds__(64) void seg_offsets_k(const uint32_t *__restrict__
   }
 }
 
+// float <-> uint order-preserving map (for atomic min/max of finite floats)
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void seg_minmax_init_k(int ncl, uint32_t *__restrict__ mmu) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 6 * ncl) mmu[i] = f2ord((i % 6) < 3 ? FLT_MAX : -FLT_MAX);
+}
+
+// per-cloud bbox: blockIdx.y = cloud, blockIdx.x strides over its points; strict comparisons as the
+// reference's scalar bbox (NaN never updates, spatial_hashing.cpp:75-84)
 __global__ __launch_bounds__(256) void seg_minmax_k(const float *__restrict__ x, const float *__restrict__ y,
                                                      const float *__restrict__ z, const uint32_t *__restrict__ off,
-                                                     float *__restrict__ mm) {
-  const int c = blockIdx.x;
+                                                     uint32_t *__restrict__ mmu) {
+  const int c = blockIdx.y;
   float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
-  for (uint32_t i = off[c] + threadIdx.x; i < off[c + 1]; i += blockDim.x) {
+  const uint32_t e = off[c + 1];
+  for (uint32_t i = off[c] + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
     const float a = x[i], b = y[i], cz = z[i];
     if (a < v[0]) v[0] = a;
     if (b < v[1]) v[1] = b;
@@ -63,14 +80,23 @@ __global__ __launch_bounds__(256) void seg_minmax_k(const float *__restrict__ x,
   if (lane() == 0)
     for (int j = 0; j < 6; ++j) sm[threadIdx.x >> 6][j] = v[j];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < 4; ++w)
-      for (int j = 0; j < 3; ++j) {
-        if (sm[w][j] < v[j]) v[j] = sm[w][j];
-        if (sm[w][j + 3] > v[j + 3]) v[j + 3] = sm[w][j + 3];
-      }
-    for (int j = 0; j < 6; ++j) mm[6 * c + j] = v[j];
+  if (threadIdx.x < 6) {
+    const int j = threadIdx.x;
+    float r = sm[0][j];
+    for (int w = 1; w < 4; ++w) r = j < 3 ? fminf(r, sm[w][j]) : fmaxf(r, sm[w][j]);
+    if (j < 3) atomicMin(&mmu[6 * c + j], f2ord(r));
+    else atomicMax(&mmu[6 * c + j], f2ord(r));
   }
+}
+
+__global__ void seg_minmax_final_k(int ncl, const uint32_t *__restrict__ mmu, float *__restrict__ mm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 6 * ncl) mm[i] = ord2f(mmu[i]);
+}
+
+int seg_clamp_offsets(const uint32_t *in, int ncl, size_t n_bound, uint32_t *out, hipStream_t s) {
+  RVP_LAUNCH("seg_offsets_k", s, seg_offsets_k<<<1, 64, 0, s>>>(in, ncl, (uint32_t)n_bound, out));
+  return RVP_OK;
 }
 
 // cells per axis for an edge: floor((max - origin) / edge) + 1 (the device cell function's arithmetic)
@@ -78,7 +104,8 @@ __device__ __forceinline__ double axis_cells(float mx, float origin, float edge)
   return (double)floorf((mx - origin) / edge) + 1.0;
 }
 
-constexpr int OCC_BITS = 1 << 16;  // occupancy bitmap cells per cloud for the k-NN edge estimate
+constexpr int OCC_BITS = 1 << 16;  // occupancy cells per cloud (one byte each) for the k-NN edge estimate
+constexpr uint32_t SEG_MIN_CELLS = 1u << 17;  // dense-table cells every cloud may use at least
 
 // k-NN edge, step 1: volume-based edge c0 (coarsened until its grid fits the occupancy bitmap)
 __global__ void seg_knn_c0_k(const float *__restrict__ mm, const uint32_t *__restrict__ off, int ncl, int K,
@@ -124,44 +151,55 @@ __global__ void seg_knn_c0_k(const float *__restrict__ mm, const uint32_t *__res
 // k-NN edge, step 2: occupied cells of the c0 grid (one bit per cell)
 __global__ void seg_occ_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
                           const uint32_t *__restrict__ off, int ncl, const SegDesc *__restrict__ d0,
-                          uint32_t *__restrict__ bits, uint32_t *__restrict__ occ) {
+                          uint8_t *__restrict__ flags) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool fresh = false;
-  int c = 0;
   if (i < off[ncl]) {
-    c = cloud_of(off, ncl, i);
+    const int c = cloud_of(off, ncl, i);
     const SegDesc d = d0[c];
     const uint32_t k = (uint32_t)cell_clamped(x[i], d.mn[0], d.cell, d.g[0]) +
                        (uint32_t)d.g[0] * ((uint32_t)cell_clamped(y[i], d.mn[1], d.cell, d.g[1]) +
                                            (uint32_t)d.g[1] * (uint32_t)cell_clamped(z[i], d.mn[2], d.cell, d.g[2]));
-    uint32_t *w = bits + (size_t)c * (OCC_BITS / 32) + (k >> 5);
-    const uint32_t b = 1u << (k & 31);
-    if (!(__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & b)) fresh = !(atomicOr(w, b) & b);
+    uint8_t *f = flags + (size_t)c * OCC_BITS + k;  // racing plain stores of one value: no atomics
+    if (!*f) *f = 1;
   }
-  if (fresh) atomicAdd(&occ[c], 1u);
 }
 
-// k-NN edge, step 3: surface density m / c0^2 from m points per occupied cell gives the k-th neighbour
-// distance R = c0 sqrt(K / (pi m)); the edge is R / 2.8
-__global__ void seg_knn_cell_k(const uint32_t *__restrict__ off, int ncl, int K, const float *__restrict__ c0,
-                               const uint32_t *__restrict__ occ, float *__restrict__ cell) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncl) return;
+// k-NN edge, step 3 (block per cloud): occupied cells = set bits; surface density m / c0^2 from m points
+// per occupied cell gives the k-th neighbour distance R = c0 sqrt(K / (pi m)); the edge is R / div
+__global__ __launch_bounds__(256) void seg_knn_cell_k(const uint32_t *__restrict__ off, int K,
+                                                       const float *__restrict__ c0, const uint32_t *__restrict__ words,
+                                                       double div, float *__restrict__ cell) {
+  const int c = blockIdx.x;
+  uint32_t cnt = 0;
+  for (int j = threadIdx.x; j < OCC_BITS / 4; j += 256) {
+    uint32_t w = words[(size_t)c * (OCC_BITS / 4) + j];  // four cells: count the nonzero bytes
+    w |= w >> 1;
+    w |= w >> 2;
+    w |= w >> 4;
+    cnt += __popc(w & 0x01010101u);
+  }
+  cnt = wave_sum(cnt);
+  __shared__ uint32_t sm[4];
+  if (lane() == 0) sm[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const uint32_t occ = sm[0] + sm[1] + sm[2] + sm[3];
   const double n = (double)(off[c + 1] - off[c]);
-  const double m = n / (double)max(occ[c], 1u);
+  const double m = n / (double)max(occ, 1u);
   const double e0 = (double)c0[c];
   const double R = e0 * sqrt((double)K / (3.14159265358979 * fmax(m, 1e-30)));
-  cell[c] = (float)fmin(fmax(R / 2.8, e0 / 64.0), e0 * 2.0);
+  cell[c] = (float)fmin(fmax(R / div, e0 / 64.0), e0 * 2.0);
 }
 
 // per cloud: origin (reference margin), dims, coarsened until the cloud's dense table fits its budget
 __global__ void seg_geom_k(const float *__restrict__ mm, const uint32_t *__restrict__ off, int ncl,
-                           const float *__restrict__ cell, SegDesc *__restrict__ desc, uint32_t *__restrict__ ncells) {
+                           const float *__restrict__ cell, double per_point, SegDesc *__restrict__ desc,
+                           uint32_t *__restrict__ ncells) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncl) return;
   const float *m = mm + 6 * c;
   const uint32_t n = off[c + 1] - off[c];
-  const double budget = fmax(16384.0, 4.0 * (double)n);
+  const double budget = fmax((double)SEG_MIN_CELLS, per_point * (double)n);
   SegDesc d{};
   const float req = cell[c];
   const float mg = req * 0.01f;
@@ -226,18 +264,28 @@ struct SegGap {
   uint32_t lo, hi, val;
 };
 constexpr uint32_t SEG_SHORT_GAP = 32;
+constexpr uint32_t SEG_GAP_CHUNK = 8192;  // cells per fill record, so that long empty runs spread over blocks
+
+// records [lo, hi] -> val in pieces of at most SEG_GAP_CHUNK cells
+__device__ __forceinline__ void push_gap(SegGap *gaps, uint32_t *ngaps, uint32_t lo, uint32_t hi, uint32_t val) {
+  const uint32_t pieces = (hi - lo) / SEG_GAP_CHUNK + 1;
+  const uint32_t g = atomicAdd(ngaps, pieces);
+  for (uint32_t p = 0; p < pieces; ++p) {
+    const uint32_t a = lo + p * SEG_GAP_CHUNK;
+    gaps[g + p] = SegGap{a, min(hi, a + SEG_GAP_CHUNK - 1), val};
+  }
+}
 
 // cstart[g] = first cell-order position whose key >= g, for g in [0, table_size]
+// (queries read cs up to the last cloud's end, total_cells = toff[ncl]; the table beyond stays unwritten)
 __global__ void seg_cell_start_k(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ off, int ncl,
-                                 uint32_t table_size, uint32_t *__restrict__ cs, SegGap *__restrict__ gaps,
-                                 uint32_t *__restrict__ ngaps) {
+                                 const uint32_t *__restrict__ total_cells, uint32_t *__restrict__ cs,
+                                 SegGap *__restrict__ gaps, uint32_t *__restrict__ ngaps) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = off[ncl];
+  const uint32_t table_size = *total_cells;
   if (n == 0) {
-    if (i == 0) {
-      const uint32_t g = atomicAdd(ngaps, 1u);
-      gaps[g] = SegGap{0, table_size, 0};
-    }
+    if (i == 0) push_gap(gaps, ngaps, 0, table_size, 0);
     return;
   }
   if (i >= n) return;
@@ -247,14 +295,10 @@ __global__ void seg_cell_start_k(const uint32_t *__restrict__ keys, const uint32
     if (k - lo + 1 <= SEG_SHORT_GAP) {
       for (uint32_t c = lo; c <= k; ++c) cs[c] = i;
     } else {
-      const uint32_t g = atomicAdd(ngaps, 1u);
-      gaps[g] = SegGap{lo, k, i};
+      push_gap(gaps, ngaps, lo, k, i);
     }
   }
-  if (i == n - 1) {
-    const uint32_t g = atomicAdd(ngaps, 1u);
-    gaps[g] = SegGap{k + 1, table_size, n};
-  }
+  if (i == n - 1) push_gap(gaps, ngaps, k + 1, table_size, n);
 }
 
 __global__ void seg_fill_gaps_k(const SegGap *__restrict__ gaps, const uint32_t *__restrict__ ngaps,
@@ -274,7 +318,7 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_CHECK(n_bound > 0, RVP_E_ARG, "n_bound must be positive");
   RVP_CHECK(n_bound < 0xfffffff0ull / 5, RVP_E_UNSUPPORTED, "batch too large");
   RVP_TRY(offs.alloc((ncl + 1) * sizeof(uint32_t), s));
-  RVP_LAUNCH("seg_offsets_k", s, seg_offsets_k<<<1, 64, 0, s>>>(off_, ncl, (uint32_t)n_bound, offs.as<uint32_t>()));
+  RVP_TRY(seg_clamp_offsets(off_, ncl, n_bound, offs.as<uint32_t>(), s));
   off = offs.as<uint32_t>();
   const unsigned cb = ceil_div(ncl, 64);
   RVP_TRY(mm.alloc(ncl * 6 * sizeof(float), s));
@@ -282,31 +326,39 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_TRY(cellv.alloc(ncl * sizeof(float), s));
   RVP_TRY(ncells.alloc((ncl + 1) * sizeof(uint32_t), s));
   RVP_TRY(toff.alloc((ncl + 1) * sizeof(uint32_t), s));
-  RVP_LAUNCH("seg_minmax_k", s, seg_minmax_k<<<ncl, 256, 0, s>>>(x, y, z, off, mm.as<float>()));
+  {
+    Tmp mmu;
+    RVP_TRY(mmu.alloc(ncl * 6 * sizeof(uint32_t), s));
+    const unsigned b6 = ceil_div(6 * ncl, 256);
+    RVP_LAUNCH("seg_minmax_init_k", s, seg_minmax_init_k<<<b6, 256, 0, s>>>(ncl, mmu.as<uint32_t>()));
+    const unsigned per = (unsigned)std::min<size_t>(64, std::max<size_t>(1, n_bound / ((size_t)ncl * 4096)));
+    RVP_LAUNCH("seg_minmax_k", s, seg_minmax_k<<<dim3(per, ncl), 256, 0, s>>>(x, y, z, off, mmu.as<uint32_t>()));
+    RVP_LAUNCH("seg_minmax_final_k", s, seg_minmax_final_k<<<b6, 256, 0, s>>>(ncl, mmu.as<uint32_t>(),
+                                                                            mm.as<float>()));
+  }
   const float *cells = cell_dev;
   if (mode_knn) {
-    Tmp c0, d0, bits, occ;
+    Tmp c0, d0, bits;
     RVP_TRY(c0.alloc(ncl * sizeof(float), s));
     RVP_TRY(d0.alloc(ncl * sizeof(SegDesc), s));
-    RVP_TRY(bits.alloc((size_t)ncl * (OCC_BITS / 8), s));
-    RVP_TRY(occ.alloc(ncl * sizeof(uint32_t), s));
-    RVP_HIP(hipMemsetAsync(bits.p, 0, (size_t)ncl * (OCC_BITS / 8), s));
-    RVP_HIP(hipMemsetAsync(occ.p, 0, ncl * sizeof(uint32_t), s));
+    RVP_TRY(bits.alloc((size_t)ncl * OCC_BITS, s));
+    RVP_HIP(hipMemsetAsync(bits.p, 0, (size_t)ncl * OCC_BITS, s));
     RVP_LAUNCH("seg_knn_c0_k", s, seg_knn_c0_k<<<cb, 64, 0, s>>>(mm.as<float>(), off, ncl, K, c0.as<float>(),
                                                                  d0.as<SegDesc>()));
     RVP_LAUNCH("seg_occ_k", s, seg_occ_k<<<ceil_div(n_bound, 256), 256, 0, s>>>(
-                                   x, y, z, off, ncl, d0.as<SegDesc>(), bits.as<uint32_t>(), occ.as<uint32_t>()));
-    RVP_LAUNCH("seg_knn_cell_k", s, seg_knn_cell_k<<<cb, 64, 0, s>>>(off, ncl, K, c0.as<float>(),
-                                                                     occ.as<uint32_t>(), cellv.as<float>()));
+                                   x, y, z, off, ncl, d0.as<SegDesc>(), bits.as<uint8_t>()));
+    RVP_LAUNCH("seg_knn_cell_k", s, seg_knn_cell_k<<<ncl, 256, 0, s>>>(off, K, c0.as<float>(), bits.as<uint32_t>(),
+                                                                       tune_knn_cell_div(), cellv.as<float>()));
     cells = cellv.as<float>();
   }
-  RVP_LAUNCH("seg_geom_k", s, seg_geom_k<<<cb, 64, 0, s>>>(mm.as<float>(), off, ncl, cells, desc.as<SegDesc>(),
-                                                           ncells.as<uint32_t>()));
+  const double per_point = tune_seg_cells_per_point();
+  RVP_LAUNCH("seg_geom_k", s, seg_geom_k<<<cb, 64, 0, s>>>(mm.as<float>(), off, ncl, cells, per_point,
+                                                           desc.as<SegDesc>(), ncells.as<uint32_t>()));
   RVP_TRY(exclusive_scan_u32(ncells.as<uint32_t>(), toff.as<uint32_t>(), ncl, toff.as<uint32_t>() + ncl, s));
   RVP_LAUNCH("seg_toff_k", s, seg_toff_k<<<cb, 64, 0, s>>>(toff.as<uint32_t>(), ncl, desc.as<SegDesc>()));
-  // the table holds at most sum(max(2^14, 4 n_c)) cells
-  const uint64_t table = 4ull * n_bound + 16384ull * (uint64_t)ncl;
-  RVP_CHECK(table < 0xfffffff0ull, RVP_E_UNSUPPORTED, "batch cell table too large");
+  // the table holds at most sum(max(SEG_MIN_CELLS, per_point n_c)) cells
+  const uint64_t table = (uint64_t)std::ceil(per_point * (double)n_bound) + (uint64_t)SEG_MIN_CELLS * (uint64_t)ncl;
+  RVP_CHECK(table < 0xfff00000ull, RVP_E_UNSUPPORTED, "batch cell table too large");
   const uint32_t n32 = (uint32_t)n_bound;
   Tmp kt, vt;
   RVP_TRY(keys.alloc(n_bound * sizeof(uint32_t), s));
@@ -317,7 +369,7 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_LAUNCH("seg_keys_k", s, seg_keys_k<<<nb, 256, 0, s>>>(x, y, z, off, ncl, desc.as<SegDesc>(), n32,
                                                             (uint32_t)table, keys.as<uint32_t>(), perm.as<uint32_t>()));
   uint32_t *kp = keys.as<uint32_t>(), *vp = perm.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
-  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n_bound, bits_for(table), s));
+  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n_bound, bits_for(table), s, off + ncl));
   if (kp != keys.as<uint32_t>()) {
     keys.swap(kt);
     perm.swap(vt);
@@ -329,11 +381,11 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
                                                                 xs.as<float>(), ys.as<float>(), zs.as<float>()));
   RVP_TRY(cstart.alloc((table + 1) * sizeof(uint32_t), s));
   Tmp gaps, ng;
-  RVP_TRY(gaps.alloc((n_bound + 2) * sizeof(SegGap), s));
+  RVP_TRY(gaps.alloc((n_bound + 2 + table / SEG_GAP_CHUNK + 2) * sizeof(SegGap), s));
   RVP_TRY(ng.alloc(sizeof(uint32_t), s));
   RVP_HIP(hipMemsetAsync(ng.p, 0, sizeof(uint32_t), s));
   RVP_LAUNCH("seg_cell_start_k", s, seg_cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), off, ncl,
-                                                                        (uint32_t)table, cstart.as<uint32_t>(),
+                                                                        toff.as<uint32_t>() + ncl, cstart.as<uint32_t>(),
                                                                         gaps.as<SegGap>(), ng.as<uint32_t>()));
   RVP_LAUNCH("seg_fill_gaps_k", s, seg_fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<SegGap>(), ng.as<uint32_t>(),
                                                                         cstart.as<uint32_t>()));

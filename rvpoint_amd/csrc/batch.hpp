@@ -50,6 +50,9 @@ __device__ __forceinline__ GridView seg_view(const SegArrays &A, int c) {
   return v;
 }
 
+// out[c] = running max of min(in[c], n_bound), out[0] = 0: the caller's offsets as a valid partition
+int seg_clamp_offsets(const uint32_t *in, int ncl, size_t n_bound, uint32_t *out, hipStream_t s);
+
 // Host owner of a segmented grid (stream-ordered temporaries).
 struct SegGrid {
   Tmp desc, mm, cellv, ncells, toff, xs, ys, zs, perm, keys, cstart, offs;

@@ -46,13 +46,17 @@ int fail(int code, const std::string &msg);
 #define RVP_LAUNCHED() RVP_HIP(hipGetLastError())
 
 // ------------------------------------------------------------------------------------------------
-// stream-ordered temporary device buffer (hipMallocAsync pool, cached across calls)
+// temporaries: a caching allocator keyed by (device, stream).  A block freed on a stream is handed out
+// again only on that stream, so stream order makes the reuse safe without events; blocks stay cached until
+// rvp_cache_trim (or an allocation failure) returns them to the driver.
 // ------------------------------------------------------------------------------------------------
 int pool_init();
+void *cache_alloc(size_t bytes, hipStream_t s, size_t *cap);
+void cache_free(void *p, size_t cap, hipStream_t s);
 
-// A temporary lives in the stream-ordered pool and is freed on its stream; a persistent buffer (owned by
-// a handle that may outlive the stream it was built on) uses hipMalloc/hipFree.  Buffers are reused when
-// a later request of the same kind fits.
+// A temporary comes from the cache and goes back to it on its stream; a persistent buffer (owned by a
+// handle that may outlive the stream it was built on) uses hipMalloc/hipFree.  Buffers are reused when
+// large enough.
 struct Tmp {
   void *p = nullptr;
   size_t cap = 0;
@@ -65,7 +69,7 @@ struct Tmp {
   void release() {
     if (p) {
       if (persistent) (void)hipFree(p);
-      else (void)hipFreeAsync(p, s);
+      else cache_free(p, cap, s);
     }
     p = nullptr;
     cap = 0;
@@ -76,13 +80,17 @@ struct Tmp {
     release();
     s = stream;
     persistent = keep;
-    hipError_t e = keep ? hipMalloc(&p, bytes) : hipMallocAsync(&p, bytes, stream);
-    if (e != hipSuccess) {
-      p = nullptr;
-      return fail(RVP_E_OOM, std::string(keep ? "hipMalloc(" : "hipMallocAsync(") + std::to_string(bytes) +
-                                 "): " + hipGetErrorString(e));
+    if (keep) {
+      const hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) {
+        p = nullptr;
+        return fail(RVP_E_OOM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+      }
+      cap = bytes;
+    } else {
+      p = cache_alloc(bytes, stream, &cap);
+      if (!p) return fail(RVP_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
     }
-    cap = bytes;
     return RVP_OK;
   }
   template <class T> T *as() const { return static_cast<T *>(p); }
@@ -102,6 +110,10 @@ inline hipStream_t S(rvp_stream_t s) { return static_cast<hipStream_t>(s); }
 // ------------------------------------------------------------------------------------------------
 bool prof_enabled();
 bool prof_want(const char *name);
+// tuning knob read by the k-NN grid builders (rvp_tune_set "knn_cell_div")
+double tune_knn_cell_div();
+// dense-table cells per point a batched grid may use (rvp_tune_set "seg_cells_per_point")
+double tune_seg_cells_per_point();
 hipEvent_t prof_start(hipStream_t s);
 void prof_stop(const char *name, hipStream_t s, hipEvent_t start);
 

@@ -92,20 +92,35 @@ static double cells_on_axis(float data_max, float origin, float edge) {
   return std::floor((double)t) + 1.0;
 }
 
+// one byte per cell; racing plain stores of the same value need no atomics (read first: most cells are
+// already marked)
 __global__ void occupancy_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
                             size_t n, float mnx, float mny, float mnz, float c, int gx, int gy, int gz,
-                            uint32_t *__restrict__ flags, uint32_t *__restrict__ count) {
+                            uint8_t *__restrict__ flags) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t k = (uint32_t)cell_clamped(x[i], mnx, c, gx) +
                      (uint32_t)gx * ((uint32_t)cell_clamped(y[i], mny, c, gy) +
                                      (uint32_t)gy * (uint32_t)cell_clamped(z[i], mnz, c, gz));
-  // most points find their cell already marked: read before the atomic, and count one mark per wave
-  bool first = false;
-  if (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
-    first = atomicExch(&flags[k], 1u) == 0u;
-  const int marked = popc(ballot(first));
-  if (lane() == 0 && marked) atomicAdd(count, (uint32_t)marked);
+  if (!flags[k]) flags[k] = 1;
+}
+
+// nonzero bytes of a word
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t w) {
+  w |= w >> 1;
+  w |= w >> 2;
+  w |= w >> 4;
+  return __popc(w & 0x01010101u);
+}
+
+// marked cells (flags padded to a multiple of 4 bytes with zeros)
+__global__ __launch_bounds__(256) void count_marked_k(const uint32_t *__restrict__ words, size_t nw,
+                                                       uint32_t *__restrict__ count) {
+  uint32_t c = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (size_t)gridDim.x * blockDim.x)
+    c += nonzero_bytes(words[i]);
+  c = wave_sum(c);
+  if (lane() == 0 && c) atomicAdd(count, c);
 }
 
 __global__ void keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
@@ -296,7 +311,8 @@ size_t Grid::occupied_cells() const {
 
 // Cell edge for k-nearest queries.  A first edge c0 from the bbox volume (k points per cell if the cloud
 // filled it) gives an occupancy count; with m points per occupied cell the surface density is about
-// m / c0^2, so the k-th neighbour distance is about R = c0 sqrt(k / (pi m)), and the edge is R / 2.8:
+// m / c0^2, so the k-th neighbour distance is about R = c0 sqrt(k / (pi m)), and the edge is R / div
+// (div = tuning knob knn_cell_div, default 2.8):
 // the certified box then spans about 6 cells per axis, which keeps the candidates near 2-3x the k
 // neighbours for surfaces and volumes alike.
 int Grid::build_for_knn(const float *x_, const float *y_, const float *z_, size_t n_, int k, hipStream_t s) {
@@ -320,20 +336,23 @@ int Grid::build_for_knn(const float *x_, const float *y_, const float *z_, size_
                g2 = cells_on_axis(mm[5], b0[2], (float)c0);
   if (g0 * g1 * g2 <= 4194304.0) {
     const size_t nc = (size_t)(g0 * g1 * g2);
+    const size_t nw = ceil_div(nc, 4);
     Tmp flags, cnt;
-    RVP_TRY(flags.alloc(nc * sizeof(uint32_t), s));
+    RVP_TRY(flags.alloc(nw * 4, s));
     RVP_TRY(cnt.alloc(sizeof(uint32_t), s));
-    RVP_HIP(hipMemsetAsync(flags.p, 0, nc * sizeof(uint32_t), s));
+    RVP_HIP(hipMemsetAsync(flags.p, 0, nw * 4, s));
     RVP_HIP(hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), s));
     RVP_LAUNCH("occupancy_k", s, occupancy_k<<<ceil_div(n_, 256), 256, 0, s>>>(
                                      x_, y_, z_, n_, b0[0], b0[1], b0[2], (float)c0, (int)g0, (int)g1, (int)g2,
-                                     flags.as<uint32_t>(), cnt.as<uint32_t>()));
+                                     flags.as<uint8_t>()));
+    RVP_LAUNCH("count_marked_k", s, count_marked_k<<<(unsigned)std::min<size_t>(1024, ceil_div(nw, 256)), 256, 0, s>>>(
+                                        flags.as<uint32_t>(), nw, cnt.as<uint32_t>()));
     uint32_t occupied0 = 0;
     RVP_HIP(hipMemcpyAsync(&occupied0, cnt.p, sizeof(occupied0), hipMemcpyDeviceToHost, s));
     RVP_HIP(hipStreamSynchronize(s));
     const double m = (double)n_ / (double)std::max<uint32_t>(occupied0, 1);
     const double R = c0 * std::sqrt((double)k / (3.14159265358979 * m));
-    c = std::min(std::max(R / 2.8, c0 / 64.0), c0 * 2.0);
+    c = std::min(std::max(R / tune_knn_cell_div(), c0 / 64.0), c0 * 2.0);
   }
   return build_from(mm, (float)c, s);
 }

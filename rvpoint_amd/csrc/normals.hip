@@ -306,10 +306,13 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
   solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
 }
 
-// N1 over a batch: every query's k+1 nearest within its own cloud
-__global__ __launch_bounds__(QT) void seg_normals_knn_k(SegArrays A, int K, float vpx, float vpy, float vpz,
-                                                         float *__restrict__ nx, float *__restrict__ ny,
-                                                         float *__restrict__ nz, unsigned long long *fallbacks) {
+// N1 / N2 over a batch: every query's neighbourhood within its own cloud (MODE 0: K nearest incl. self;
+// MODE 1: radius[c] list capped at K)
+template <int MODE>
+__global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius, float vpx,
+                                                     float vpy, float vpz, float *__restrict__ nx,
+                                                     float *__restrict__ ny, float *__restrict__ nz,
+                                                     unsigned long long *fallbacks) {
   __shared__ WaveLds lds[QW];
   __shared__ NormLds nl[QW];
   const int w = threadIdx.x >> 6;
@@ -320,15 +323,25 @@ __global__ __launch_bounds__(QT) void seg_normals_knn_k(SegArrays A, int K, floa
   const int nbatch = (int)min<uint32_t>(NB, total - base);
   for (int j = 0; j < nbatch; ++j) {
     const uint32_t q = base + j;
-    const GridView g = seg_view(A, cloud_of(A.off, A.ncl, q));
+    const int c = cloud_of(A.off, A.ncl, q);
+    const GridView g = seg_view(A, c);
     const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
     TopK t;
     t.init(K);
-    knn_search<true>(g, x, y, z, t, lds[w]);
+    if (MODE == 0) {
+      knn_search<true>(g, x, y, z, t, lds[w]);
+    } else {
+      const float r = radius[c];
+      int lo[3], hi[3];
+      window3(g, x, y, z, r, lo, hi);
+      TopKSink<true> sink{t, lds[w]};
+      scan_box(g, x, y, z, lo, hi, false, lo, hi, __fmul_rn(r, r), true, lds[w], sink);
+      topk_flush<true>(t, lds[w]);
+    }
     const int m = min(t.cnt, K);
-    float c[6];
-    wave_cov(g, lane() < m, t.pay, m, x, y, z, c);
-    stash(N, j, c, x, y, z, g.perm[q], m);
+    float c6[6];
+    wave_cov(g, lane() < m, t.pay, m, x, y, z, c6);
+    stash(N, j, c6, x, y, z, g.perm[q], m);
   }
   solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
 }
@@ -405,8 +418,25 @@ int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const 
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, true, k + 1, nullptr, s));
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
-  RVP_LAUNCH("seg_normals_knn_k", s, seg_normals_knn_k<<<ceil_div(n_bound, QW * NB), QT, 0, s>>>(
-                                         G.arrays(), k + 1, vpx, vpy, vpz, nx, ny, nz, fb));
+  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0><<<ceil_div(n_bound, QW * NB), QT, 0, s>>>(
+                                     G.arrays(), k + 1, nullptr, vpx, vpy, vpz, nx, ny, nz, fb));
+  return RVP_OK;
+}
+
+int rvp_batch_normals_radius(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                             size_t n_bound, const float *radius, const float *cell, int max_nn, float vpx, float vpy,
+                             float vpz, float *nx, float *ny, float *nz, rvp_stream_t st) {
+  RVP_CHECK(x && y && z && offsets && radius && nx && ny && nz, RVP_E_ARG, "null pointer");
+  RVP_CHECK(nclouds > 0, RVP_E_ARG, "nclouds must be positive");
+  RVP_CHECK(max_nn >= 1 && max_nn <= MAX_TOPK, RVP_E_UNSUPPORTED, "max_nn must be in [1, 64]");
+  RVP_TRY(pool_init());
+  hipStream_t s = S(st);
+  SegGrid G;
+  RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, false, 0, cell ? cell : radius, s));
+  unsigned long long *fb = nullptr;
+  RVP_TRY(fallback_counter(s, &fb));
+  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1><<<ceil_div(n_bound, QW * NB), QT, 0, s>>>(
+                                     G.arrays(), max_nn, radius, vpx, vpy, vpz, nx, ny, nz, fb));
   return RVP_OK;
 }
 

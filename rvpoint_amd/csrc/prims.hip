@@ -92,9 +92,20 @@ int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *out, size_t n, uint64_t
 constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = PRIM_THREADS * RS_ITEMS;
 
+// element count of a device-sized sort: min(*n_dev, n) (n = the host bound the launch was sized for)
+__device__ __forceinline__ size_t dev_count(size_t n, const uint32_t *n_dev) {
+  return n_dev ? min((size_t)*n_dev, n) : n;
+}
+// a pass whose digit lies above the device-side key width only copies (the ping-pong stays in step)
+__device__ __forceinline__ bool copy_pass(int shift, const int *bits_dev) { return bits_dev && shift >= *bits_dev; }
+
 template <class K>
 __global__ __launch_bounds__(PRIM_THREADS) void radix_hist_k(const K *__restrict__ keys, size_t n, int shift,
-                                                              uint32_t *__restrict__ hist, unsigned nb) {
+                                                              uint32_t *__restrict__ hist, unsigned nb,
+                                                              const uint32_t *__restrict__ n_dev,
+                                                              const int *__restrict__ bits_dev) {
+  if (copy_pass(shift, bits_dev)) return;
+  n = dev_count(n, n_dev);
   __shared__ uint32_t h[PRIM_THREADS / WAVE][256];
   const int t = threadIdx.x, w = t >> 6;
 #pragma unroll
@@ -115,12 +126,26 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_scatter_k(const K *__restr
                                                                  const uint32_t *__restrict__ vin,
                                                                  K *__restrict__ kout, uint32_t *__restrict__ vout,
                                                                  size_t n, int shift,
-                                                                 const uint32_t *__restrict__ offs, unsigned nb) {
+                                                                 const uint32_t *__restrict__ offs, unsigned nb,
+                                                                 const uint32_t *__restrict__ n_dev,
+                                                                 const int *__restrict__ bits_dev) {
   __shared__ uint32_t run[256];
   __shared__ uint32_t wc[PRIM_THREADS / WAVE][256];
   const int t = threadIdx.x, w = t >> 6;
-  run[t] = offs[(size_t)t * nb + blockIdx.x];
+  n = dev_count(n, n_dev);
   const size_t base = (size_t)blockIdx.x * RS_TILE;
+  if (copy_pass(shift, bits_dev)) {
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+      const size_t i = base + (size_t)j * PRIM_THREADS + t;
+      if (i < n) {
+        kout[i] = kin[i];
+        vout[i] = vin[i];
+      }
+    }
+    return;
+  }
+  run[t] = offs[(size_t)t * nb + blockIdx.x];
   for (int j = 0; j < RS_ITEMS; ++j) {
     const size_t i = base + (size_t)j * PRIM_THREADS + t;
     const bool valid = i < n;
@@ -160,16 +185,19 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_scatter_k(const K *__restr
 }
 
 template <class K>
-static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, size_t n, int bits, hipStream_t s) {
+static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, size_t n, int bits, hipStream_t s,
+                           const uint32_t *n_dev, const int *bits_dev) {
   if (n <= 1 || bits <= 0) return RVP_OK;
   RVP_CHECK(n < 0xffffffffull, RVP_E_UNSUPPORTED, "radix sort: more than 2^32-1 elements");
   const unsigned nb = ceil_div(n, RS_TILE);
   Tmp hist;
   RVP_TRY(hist.alloc((size_t)256 * nb * sizeof(uint32_t), s));
   for (int shift = 0; shift < bits; shift += 8) {
-    RVP_LAUNCH("radix_hist_k", s, radix_hist_k<K><<<nb, PRIM_THREADS, 0, s>>>(*keys, n, shift, hist.as<uint32_t>(), nb));
+    RVP_LAUNCH("radix_hist_k", s, radix_hist_k<K><<<nb, PRIM_THREADS, 0, s>>>(*keys, n, shift, hist.as<uint32_t>(), nb,
+                                                                             n_dev, bits_dev));
     RVP_TRY(exclusive_scan_u32(hist.as<uint32_t>(), hist.as<uint32_t>(), (size_t)256 * nb, nullptr, s));
-    RVP_LAUNCH("radix_scatter_k", s, radix_scatter_k<K><<<nb, PRIM_THREADS, 0, s>>>(*keys, *vals, *kt, *vt, n, shift, hist.as<uint32_t>(), nb));
+    RVP_LAUNCH("radix_scatter_k", s, radix_scatter_k<K><<<nb, PRIM_THREADS, 0, s>>>(
+                                         *keys, *vals, *kt, *vt, n, shift, hist.as<uint32_t>(), nb, n_dev, bits_dev));
     std::swap(*keys, *kt);
     std::swap(*vals, *vt);
   }
@@ -177,12 +205,12 @@ static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, siz
 }
 
 int radix_sort_pairs_u32(uint32_t **keys, uint32_t **vals, uint32_t **kt, uint32_t **vt, size_t n, int bits,
-                         hipStream_t s) {
-  return radix_sort_impl<uint32_t>(keys, vals, kt, vt, n, bits, s);
+                         hipStream_t s, const uint32_t *n_dev, const int *bits_dev) {
+  return radix_sort_impl<uint32_t>(keys, vals, kt, vt, n, bits, s, n_dev, bits_dev);
 }
 int radix_sort_pairs_u64(uint64_t **keys, uint32_t **vals, uint64_t **kt, uint32_t **vt, size_t n, int bits,
-                         hipStream_t s) {
-  return radix_sort_impl<uint64_t>(keys, vals, kt, vt, n, bits, s);
+                         hipStream_t s, const uint32_t *n_dev, const int *bits_dev) {
+  return radix_sort_impl<uint64_t>(keys, vals, kt, vt, n, bits, s, n_dev, bits_dev);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -236,3 +264,4 @@ int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_
 }
 
 }  // namespace rvp
+

@@ -16,11 +16,15 @@ int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *out, size_t n, uint64_t
 
 // Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the key (8-bit digits).
 // On return the sorted data is in (*keys, *vals): the function swaps the pointers with the temporaries
-// when an odd number of passes leaves it there.
+// when an odd number of passes leaves it there.  Device-sized form: n_dev (may be null) holds the element
+// count (<= n, the bound the launches are sized for); bits_dev (may be null) the key width, passes above
+// it only copy.  Neither is read on the host.
 int radix_sort_pairs_u32(uint32_t **keys, uint32_t **vals, uint32_t **keys_tmp, uint32_t **vals_tmp,
-                         size_t n, int bits, hipStream_t s);
+                         size_t n, int bits, hipStream_t s, const uint32_t *n_dev = nullptr,
+                         const int *bits_dev = nullptr);
 int radix_sort_pairs_u64(uint64_t **keys, uint32_t **vals, uint64_t **keys_tmp, uint32_t **vals_tmp,
-                         size_t n, int bits, hipStream_t s);
+                         size_t n, int bits, hipStream_t s, const uint32_t *n_dev = nullptr,
+                         const int *bits_dev = nullptr);
 
 // Stable compaction of points whose flag equals `want` into AoS (float[3*m]); *count_dev = m (device).
 int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want,

@@ -61,22 +61,43 @@ __device__ __forceinline__ void cx(uint64_t &k, uint32_t &p, int partner_mask, b
   }
 }
 
-template <bool PAY> __device__ __forceinline__ void bitonic_sort64(uint64_t &k, uint32_t &p) {
+// sorts every group of `size` lanes ascending (size: power of two <= 64, wave-uniform)
+template <bool PAY> __device__ __forceinline__ void bitonic_sort_groups(uint64_t &k, uint32_t &p, int size) {
   const int l = lane();
 #pragma unroll
   for (int s = 2; s <= WAVE; s <<= 1) {
+    if (s > size) break;
     cx<PAY>(k, p, s - 1, (l & (s >> 1)) == 0);
 #pragma unroll
     for (int j = s >> 2; j > 0; j >>= 1) cx<PAY>(k, p, j, (l & j) == 0);
   }
 }
 
-template <bool PAY> __device__ __forceinline__ void bitonic_clean64(uint64_t &k, uint32_t &p) {
-  const int l = lane();
-#pragma unroll
-  for (int j = WAVE / 2; j > 0; j >>= 1) cx<PAY>(k, p, j, (l & j) == 0);
+template <bool PAY> __device__ __forceinline__ void bitonic_sort64(uint64_t &k, uint32_t &p) {
+  bitonic_sort_groups<PAY>(k, p, WAVE);
 }
 
+// sorts a bitonic sequence held in each group of `size` lanes
+template <bool PAY> __device__ __forceinline__ void bitonic_clean_groups(uint64_t &k, uint32_t &p, int size) {
+  const int l = lane();
+#pragma unroll
+  for (int j = WAVE / 2; j > 0; j >>= 1)
+    if (j < size) cx<PAY>(k, p, j, (l & j) == 0);
+}
+
+template <bool PAY> __device__ __forceinline__ void bitonic_clean64(uint64_t &k, uint32_t &p) {
+  bitonic_clean_groups<PAY>(k, p, WAVE);
+}
+
+__device__ __forceinline__ int pow2_at_least(int v) {
+  int s = 1;
+  while (s < v) s <<= 1;
+  return s;
+}
+
+// Merge the staged hits into the best list.  Only the K best are ever read, so the list spans
+// W = pow2 >= K lanes: the staged batch is sorted in groups of pow2 >= nb lanes, its W smallest are
+// reversed onto lanes [0, W) and min'ed with the list (a bitonic sequence), and W lanes are cleaned.
 template <bool PAY> __device__ __forceinline__ void topk_flush(TopK &t, WaveLds &L) {
   if (t.nb == 0) return;
   const int l = lane();
@@ -84,21 +105,26 @@ template <bool PAY> __device__ __forceinline__ void topk_flush(TopK &t, WaveLds 
   uint64_t nk = l < t.nb ? L.bk[l] : ~0ull;
   uint32_t np = (PAY && l < t.nb) ? L.bp[l] : 0u;
   wave_lds_sync();
-  bitonic_sort64<PAY>(nk, np);
+  const int sb = pow2_at_least(max(t.nb, 2));
+  bitonic_sort_groups<PAY>(nk, np, sb);
+  const int W = pow2_at_least(max(t.K, 2));
   if (t.cnt == 0) {
+    // lanes [sb, 64) hold ~0 after the group sort; lanes >= W are never read
     t.key = nk;
     t.pay = np;
   } else {
-    // min(best[l], new[63-l]) holds the 64 smallest of the union as a bitonic sequence
-    uint64_t rk = shfl64(nk, WAVE - 1 - l);
-    uint32_t rp = PAY ? (uint32_t)__shfl((int)np, WAVE - 1 - l) : 0u;
+    // lane l < W takes new[W-1-l] = new[l ^ (W-1)] (the batch's W smallest, reversed)
+    uint64_t rk = shfl_xor64(nk, W - 1);
+    uint32_t rp = PAY ? (uint32_t)__shfl_xor((int)np, W - 1) : 0u;
+    if (l >= W) rk = ~0ull;
     if (rk < t.key) {
       t.key = rk;
       t.pay = rp;
     }
-    bitonic_clean64<PAY>(t.key, t.pay);
+    bitonic_clean_groups<PAY>(t.key, t.pay, W);
+    if (l >= W) t.key = ~0ull;
   }
-  t.cnt = min(WAVE, t.cnt + t.nb);
+  t.cnt = min(W, t.cnt + t.nb);
   t.nb = 0;
   t.kth = (t.cnt >= t.K) ? readlane64(t.key, t.K - 1) : ~0ull;
 }
@@ -215,9 +241,11 @@ __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy
   scan_box(g, qx, qy, qz, lo, hi, false, lo, hi, 0.0f, false, L, sink);
   int step = 1;
   for (;;) {
-    topk_flush<PAY>(t, L);
-    if (lo[0] == 0 && lo[1] == 0 && lo[2] == 0 && hi[0] == gd[0] - 1 && hi[1] == gd[1] - 1 && hi[2] == gd[2] - 1)
-      break;
+    const bool whole = lo[0] == 0 && lo[1] == 0 && lo[2] == 0 && hi[0] == gd[0] - 1 && hi[1] == gd[1] - 1 &&
+                       hi[2] == gd[2] - 1;
+    // while fewer than K candidates are known the box only grows: merging can wait
+    if (whole || t.cnt + t.nb >= t.K) topk_flush<PAY>(t, L);
+    if (whole) break;
     int nlo[3], nhi[3];
     if (t.cnt >= t.K) {
       const float dk = key_d2(t.kth);

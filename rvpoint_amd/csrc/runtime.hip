@@ -22,20 +22,80 @@ int fail(int code, const std::string &msg) {
 
 // Keep freed pool memory cached (release threshold = max) so per-call temporaries do not go back to
 // the driver at every synchronisation.
-int pool_init() {
-  static std::mutex mu;
-  static bool done[64] = {false};
+int pool_init() { return RVP_OK; }
+
+// ---- caching allocator ------------------------------------------------------------------------------
+// size classes: 4 per octave above 4 KiB (at most 19% slack), 4 KiB below
+static size_t size_class(size_t bytes) {
+  if (bytes <= 4096) return 4096;
+  int e = 63 - __builtin_clzll(bytes - 1);  // 2^e < bytes <= 2^(e+1)
+  const size_t step = (size_t)1 << (e - 2);
+  return (bytes + step - 1) / step * step;
+}
+
+struct CacheKey {
+  int dev;
+  hipStream_t s;
+  size_t cls;
+  bool operator<(const CacheKey &o) const {
+    if (dev != o.dev) return dev < o.dev;
+    if (s != o.s) return s < o.s;
+    return cls < o.cls;
+  }
+};
+static std::mutex g_cache_mu;
+static std::map<CacheKey, std::vector<void *>> g_cache;
+
+static void cache_release_all(int dev, bool only_stream, hipStream_t s) {
+  std::vector<void *> drop;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (auto it = g_cache.begin(); it != g_cache.end();) {
+      if (it->first.dev == dev && (!only_stream || it->first.s == s)) {
+        drop.insert(drop.end(), it->second.begin(), it->second.end());
+        it = g_cache.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  if (drop.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (void *p : drop) (void)hipFree(p);
+}
+
+void *cache_alloc(size_t bytes, hipStream_t s, size_t *cap) {
   int dev = 0;
-  RVP_HIP(hipGetDevice(&dev));
-  if (dev < 0 || dev >= 64) return RVP_OK;
-  std::lock_guard<std::mutex> lk(mu);
-  if (done[dev]) return RVP_OK;
-  hipMemPool_t pool;
-  RVP_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
-  uint64_t thr = ~0ull;
-  RVP_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
-  done[dev] = true;
-  return RVP_OK;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  const size_t cls = size_class(bytes);
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_cache.find(CacheKey{dev, s, cls});
+    if (it != g_cache.end() && !it->second.empty()) {
+      void *p = it->second.back();
+      it->second.pop_back();
+      *cap = cls;
+      return p;
+    }
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, cls) != hipSuccess) {
+    (void)hipGetLastError();
+    cache_release_all(dev, false, nullptr);  // give cached blocks back and retry once
+    if (hipMalloc(&p, cls) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  *cap = cls;
+  return p;
+}
+
+void cache_free(void *p, size_t cap, hipStream_t s) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  g_cache[CacheKey{dev, s, cap}].push_back(p);
 }
 
 // ---- kernel timer ---------------------------------------------------------------------------------
@@ -95,6 +155,11 @@ void prof_stop(const char *name, hipStream_t s, hipEvent_t a) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
   g_prof.push_back(ProfRec{name, a, b});
 }
+
+static std::atomic<double> g_knn_cell_div{2.8};
+static std::atomic<double> g_seg_cells_per_point{32.0};
+double tune_knn_cell_div() { return g_knn_cell_div.load(); }
+double tune_seg_cells_per_point() { return g_seg_cells_per_point.load(); }
 
 }  // namespace rvp
 
@@ -166,7 +231,19 @@ int rvp_stream_create(rvp_stream_t *s) {
 }
 
 int rvp_stream_destroy(rvp_stream_t s) {
-  if (s) RVP_HIP(hipStreamDestroy(S(s)));
+  if (s) {
+    int dev = 0;
+    RVP_HIP(hipGetDevice(&dev));
+    rvp::cache_release_all(dev, true, S(s));
+    RVP_HIP(hipStreamDestroy(S(s)));
+  }
+  return RVP_OK;
+}
+
+int rvp_cache_trim(void) {
+  int dev = 0;
+  RVP_HIP(hipGetDevice(&dev));
+  rvp::cache_release_all(dev, false, nullptr);
   return RVP_OK;
 }
 
@@ -215,6 +292,29 @@ int rvp_exclusive_scan(const int32_t *counts, size_t n, int64_t *offsets, int64_
     RVP_HIP(hipStreamSynchronize(S(s)));
   }
   return RVP_OK;
+}
+
+int rvp_tune_set(const char *name, double value) {
+  RVP_CHECK(name, RVP_E_ARG, "null name");
+  if (std::string(name) == "knn_cell_div") {
+    RVP_CHECK(value >= 0.25 && value <= 16.0, RVP_E_ARG, "knn_cell_div must be in [0.25, 16]");
+    rvp::g_knn_cell_div.store(value);
+    return RVP_OK;
+  }
+  return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);
+}
+
+int rvp_tune_get(const char *name, double *value) {
+  RVP_CHECK(name && value, RVP_E_ARG, "null argument");
+  if (std::string(name) == "knn_cell_div") {
+    *value = rvp::g_knn_cell_div.load();
+    return RVP_OK;
+  }
+  if (std::string(name) == "seg_cells_per_point") {
+    *value = rvp::g_seg_cells_per_point.load();
+    return RVP_OK;
+  }
+  return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);
 }
 
 int rvp_profile_enable(int on) {

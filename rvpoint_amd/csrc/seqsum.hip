@@ -153,7 +153,29 @@ __global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, 
   if (l == 0) out[c] = SSChunk{fe, fo, e, 1};
 }
 
-// one wave per segment walks its chunks in order
+// element-by-element replay of one chunk with the fp32 adder (the reference's own loop)
+__device__ __forceinline__ float ss_replay(float acc, const float *__restrict__ v, uint32_t a, uint32_t e_end,
+                                           int mode, float mean) {
+  const int l = lane();
+  float x[SS_LANE];
+#pragma unroll
+  for (int q = 0; q < SS_LANE; ++q) {
+    const uint32_t i = a + (uint32_t)l * SS_LANE + q;
+    x[q] = i < e_end ? ss_term(v[i], mode, mean) : 0.0f;  // + 0 leaves the sum unchanged
+  }
+  for (int q = 0; q < WAVE; ++q) {
+#pragma unroll
+    for (int r = 0; r < SS_LANE; ++r) acc = acc + readlanef(x[r], q);
+  }
+  return acc;
+}
+
+// One wave per segment walks its chunks in order, 64 at a time (lane j holds chunk j's map).  A run of
+// consecutive clean chunks predicted in the running sum's binade is applied at once: an inclusive scan
+// composes their maps across the lanes, every lane evaluates the state after its chunk from the run's
+// start state, and the run is accepted up to the first chunk whose state leaves the binade (M >= 2^24;
+// the sum never decreases, so every earlier state stayed inside).  That chunk, and any chunk outside such
+// a run, is replayed element by element.
 __global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, const uint32_t *__restrict__ seg,
                                                   const uint32_t *__restrict__ coff, int nseg, int mode,
                                                   const float *__restrict__ mean_dev,
@@ -167,36 +189,42 @@ __global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, co
   for (uint32_t c0 = c_begin; c0 < c_end; c0 += WAVE) {
     const SSChunk mine = (c0 + l < c_end) ? ch[c0 + l] : SSChunk{0, 0, 0, 0};
     const int cnt = (int)min<uint32_t>(WAVE, c_end - c0);
-    for (int j = 0; j < cnt; ++j) {
-      const uint32_t c = c0 + j;
-      const int clean = __builtin_amdgcn_readlane(mine.clean, j);
-      bool ok = false;
-      if (clean) {
-        const int e = __builtin_amdgcn_readlane(mine.e, j);
-        if (acc >= ldexpf(1.0f, e) && acc < ldexpf(1.0f, e + 1)) {
-          const long long ie = (long long)readlane64((uint64_t)mine.ie, j);
-          const long long io = (long long)readlane64((uint64_t)mine.io, j);
-          const long long M = (long long)ldexpf(acc, 23 - e);
-          const long long Mn = M + ((M & 1) ? io : ie);
-          if (Mn < (1ll << 24)) {
-            acc = ldexpf((float)Mn, e - 23);
-            ok = true;
+    int pos = 0;
+    while (pos < cnt) {
+      // binade of the running sum (normal, finite, positive), else no map applies
+      int e_acc = -100000;
+      if (acc >= 0x1p-126f && acc < 0x1p127f) e_acc = ilogbf(acc);
+      const uint64_t elig = ballot(l >= pos && l < cnt && mine.clean && mine.e == e_acc);
+      const uint64_t stop = ~elig & (~0ull << pos);
+      const int run_end = stop ? min(cnt, __ffsll((unsigned long long)stop) - 1) : cnt;
+      if (run_end > pos) {
+        long long fe = 0, fo = 0;  // identity outside the run
+        if (l >= pos && l < run_end) {
+          fe = mine.ie;
+          fo = mine.io;
+        }
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+          long long pe = __shfl_up(fe, d), po = __shfl_up(fo, d);
+          if (l >= d) {
+            ss_compose(pe, po, fe, fo);  // earlier maps first
+            fe = pe;
+            fo = po;
           }
         }
-      }
-      if (!ok) {
-        const uint32_t a = seg[s] + (c - c_begin) * SS_C;
-        float x[SS_LANE];
-#pragma unroll
-        for (int q = 0; q < SS_LANE; ++q) {
-          const uint32_t i = a + (uint32_t)l * SS_LANE + q;
-          x[q] = i < e_end ? ss_term(v[i], mode, mean) : 0.0f;  // + 0 leaves the sum unchanged
-        }
-        for (int q = 0; q < WAVE; ++q) {
-#pragma unroll
-          for (int r = 0; r < SS_LANE; ++r) acc = acc + readlanef(x[r], q);
+        const long long M0 = (long long)ldexpf(acc, 23 - e_acc);
+        const long long Ml = M0 + ((M0 & 1) ? fo : fe);
+        const uint64_t bad = ballot(l >= pos && l < run_end && Ml >= (1ll << 24));
+        const int upto = bad ? __ffsll((unsigned long long)bad) - 1 : run_end;  // chunks [pos, upto) accepted
+        if (upto > pos) {
+          const long long Mf = (long long)readlane64((uint64_t)Ml, upto - 1);
+          acc = ldexpf((float)Mf, e_acc - 23);
+          pos = upto;
+          continue;
         }
       }
+      acc = ss_replay(acc, v, seg[s] + (c0 + pos - c_begin) * SS_C, e_end, mode, mean);
+      ++pos;
     }
   }
   if (l == 0) out[s] = acc;

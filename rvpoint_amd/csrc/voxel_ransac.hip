@@ -6,6 +6,7 @@
 #include <limits>
 #include <vector>
 
+#include "batch.hpp"
 #include "prims.hpp"
 
 namespace rvp {
@@ -63,27 +64,32 @@ __global__ void voxel_keys_k(const float *__restrict__ x, const float *__restric
   vals[i] = (uint32_t)i;
 }
 
+// n_dev (may be null): device element count <= n, the bound the launch is sized for
 template <class K>
-__global__ void run_flags_k(const K *__restrict__ keys, size_t n, uint32_t *__restrict__ f) {
+__global__ void run_flags_k(const K *__restrict__ keys, size_t n, const uint32_t *__restrict__ n_dev,
+                            uint32_t *__restrict__ f) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  f[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+  const size_t nn = n_dev ? min((size_t)*n_dev, n) : n;
+  f[i] = (i < nn && (i == 0 || keys[i] != keys[i - 1])) ? 1u : 0u;
 }
 
 template <class K>
-__global__ void run_starts_k(const K *__restrict__ keys, size_t n, const uint32_t *__restrict__ vid,
-                             uint32_t *__restrict__ start) {
+__global__ void run_starts_k(const K *__restrict__ keys, size_t n, const uint32_t *__restrict__ n_dev,
+                             const uint32_t *__restrict__ vid, uint32_t *__restrict__ start) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const size_t nn = n_dev ? min((size_t)*n_dev, n) : n;
+  if (i >= nn) return;
   if (i == 0 || keys[i] != keys[i - 1]) start[vid[i]] = (uint32_t)i;
 }
 
 __global__ void voxel_centroid_k(const float *__restrict__ x, const float *__restrict__ y,
                                  const float *__restrict__ z, const uint32_t *__restrict__ perm,
-                                 const uint32_t *__restrict__ start, const uint32_t *__restrict__ nvox, uint32_t n,
-                                 float *__restrict__ out) {
+                                 const uint32_t *__restrict__ start, const uint32_t *__restrict__ nvox, uint32_t n_host,
+                                 const uint32_t *__restrict__ n_dev, float *__restrict__ out) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t V = *nvox;
+  const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
   if (v >= V) return;
   const uint32_t s = start[v], e = (v + 1 < V) ? start[v + 1] : n;
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
@@ -117,11 +123,13 @@ static int voxel_sorted(const float *x, const float *y, const float *z, size_t n
   else RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n, bits, s));
   RVP_TRY(vid.alloc(n * sizeof(uint32_t), s));
   RVP_TRY(nv.alloc(sizeof(uint32_t), s));
-  RVP_LAUNCH("run_flags_k", s, run_flags_k<K><<<nb, 256, 0, s>>>(kp, n, vid.as<uint32_t>()));
+  RVP_LAUNCH("run_flags_k", s, run_flags_k<K><<<nb, 256, 0, s>>>(kp, n, nullptr, vid.as<uint32_t>()));
   RVP_TRY(exclusive_scan_u32(vid.as<uint32_t>(), vid.as<uint32_t>(), n, nv.as<uint32_t>(), s));
   RVP_TRY(start.alloc(n * sizeof(uint32_t), s));
-  RVP_LAUNCH("run_starts_k", s, run_starts_k<K><<<nb, 256, 0, s>>>(kp, n, vid.as<uint32_t>(), start.as<uint32_t>()));
-  RVP_LAUNCH("voxel_centroid_k", s, voxel_centroid_k<<<nb, 256, 0, s>>>(x, y, z, vp, start.as<uint32_t>(), nv.as<uint32_t>(), (uint32_t)n, out));
+  RVP_LAUNCH("run_starts_k", s, run_starts_k<K><<<nb, 256, 0, s>>>(kp, n, nullptr, vid.as<uint32_t>(),
+                                                                    start.as<uint32_t>()));
+  RVP_LAUNCH("voxel_centroid_k", s, voxel_centroid_k<<<nb, 256, 0, s>>>(x, y, z, vp, start.as<uint32_t>(),
+                                                                        nv.as<uint32_t>(), (uint32_t)n, nullptr, out));
   uint32_t V = 0;
   RVP_HIP(hipMemcpyAsync(&V, nv.p, sizeof(V), hipMemcpyDeviceToHost, s));
   RVP_HIP(hipStreamSynchronize(s));
@@ -232,6 +240,45 @@ __global__ __launch_bounds__(256) void plane_count_k(const float *__restrict__ x
   if (threadIdx.x < hn && sc[threadIdx.x]) atomicAdd(&counts[h0 + threadIdx.x], sc[threadIdx.x]);
 }
 
+// Replay of the reference's sequential loop (ransac_plane.cpp:68-107) over counted hypotheses, fed in
+// order; the state carries across feeds.  Host fp64 log / pow as the reference's.
+struct RansacReplay {
+  int best = 0;
+  float model[4] = {0, 0, 0, 0};
+  int k_iters, max_iters, iter = 0;
+  size_t n;
+  double log_p;
+  bool done;
+  RansacReplay(size_t n_, int max_iters_, float probability)
+      : k_iters(max_iters_), max_iters(max_iters_), n(n_),
+        log_p(std::log(1.0 - std::clamp((double)probability, 0.5, 0.9999))), done(max_iters_ <= 0) {}
+  void feed(const float *models, const int32_t *valid, const int32_t *counts, int H) {
+    for (int h = 0; h < H && !done; ++h, ++iter) {
+      if (!(iter < k_iters && iter < max_iters)) {
+        done = true;
+        break;
+      }
+      if (!valid[h]) continue;
+      const int cur = counts[h];
+      if (cur > best) {
+        best = cur;
+        for (int k = 0; k < 4; ++k) model[k] = models[4 * h + k];
+        const double w = (double)best / (double)n;
+        double p_no = 1.0 - std::pow(w, 3.0);
+        p_no = std::max(std::numeric_limits<double>::epsilon(), p_no);
+        p_no = std::min(1.0 - std::numeric_limits<double>::epsilon(), p_no);
+        const double log_no = std::log(p_no);
+        if (std::abs(log_no) > 1e-7) {
+          const int dyn = (int)std::ceil(log_p / log_no);
+          if (dyn > 0 && dyn < k_iters) k_iters = dyn;
+        }
+      }
+    }
+    if (iter >= k_iters || iter >= max_iters) done = true;
+  }
+  int iterations() const { return iter; }  // iterations the reference's loop runs
+};
+
 static int plane_counts_dev(const float *x, const float *y, const float *z, size_t n, const float *models,
                             const int32_t *valid, int H, float thr, int32_t *counts, hipStream_t s) {
   RVP_HIP(hipMemsetAsync(counts, 0, (size_t)H * sizeof(int32_t), s));
@@ -250,6 +297,238 @@ __global__ void plane_flags_k(const float *__restrict__ x, const float *__restri
   if (i >= n) return;
   const float dist = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(a, x[i]), __fmul_rn(b, y[i])), __fmul_rn(c, z[i])), d);
   f[i] = (dist >= -t && dist <= t) ? 1 : 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// batched V1 and R1: clouds [off[c], off[c+1]) of one SoA batch, counts on the device, no host sync
+// except the final read of the RANSAC counts.
+// ------------------------------------------------------------------------------------------------
+__global__ void seg_vox_init_k(int ncl, int *__restrict__ vmm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 6 * ncl) vmm[i] = (i % 6) < 3 ? INT_MAX : INT_MIN;
+}
+
+// voxel coordinate bounds per cloud (blockIdx.y = cloud); inv = fl(1 / leaf) as voxel_grid_downsamp.cpp:189
+__global__ __launch_bounds__(256) void seg_vox_minmax_k(const float *__restrict__ x, const float *__restrict__ y,
+                                                         const float *__restrict__ z, const uint32_t *__restrict__ off,
+                                                         const float *__restrict__ leaf, int *__restrict__ vmm) {
+  const int c = blockIdx.y;
+  const float inv = __fdiv_rn(1.0f, leaf[c]);
+  int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+  const uint32_t e = off[c + 1];
+  for (uint32_t i = off[c] + blockIdx.x * blockDim.x + threadIdx.x; i < e; i += gridDim.x * blockDim.x) {
+    const int a = vcoord(x[i], inv), b = vcoord(y[i], inv), cc = vcoord(z[i], inv);
+    v[0] = min(v[0], a); v[1] = min(v[1], b); v[2] = min(v[2], cc);
+    v[3] = max(v[3], a); v[4] = max(v[4], b); v[5] = max(v[5], cc);
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1)
+    for (int j = 0; j < 3; ++j) {
+      v[j] = min(v[j], __shfl_xor(v[j], d));
+      v[j + 3] = max(v[j + 3], __shfl_xor(v[j + 3], d));
+    }
+  __shared__ int sm[4][6];
+  if (lane() == 0)
+    for (int j = 0; j < 6; ++j) sm[threadIdx.x >> 6][j] = v[j];
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int j = threadIdx.x;
+    atomicMin(&vmm[6 * c + j], min(min(sm[0][j], sm[1][j]), min(sm[2][j], sm[3][j])));
+    atomicMax(&vmm[6 * c + j + 3], max(max(sm[0][j + 3], sm[1][j + 3]), max(sm[2][j + 3], sm[3][j + 3])));
+  }
+}
+
+// key space of each cloud: rx * ry * rz voxels (1 for an empty cloud or one whose range overflows 2^48,
+// which is then flagged in status[c]); base[c] = exclusive prefix (one wave); *bits = key width
+__global__ __launch_bounds__(64) void seg_vox_base_k(int ncl, const uint32_t *__restrict__ off,
+                                                     const int *__restrict__ vmm, uint64_t *__restrict__ ryz,
+                                                     uint64_t *__restrict__ base, int *__restrict__ bits,
+                                                     uint32_t *__restrict__ status) {
+  const int l = lane();
+  uint64_t carry = 0;
+  for (int b0 = 0; b0 < ncl; b0 += 64) {
+    const int c = b0 + l;
+    uint64_t r = 0;
+    if (c < ncl) {
+      uint32_t st = 0;
+      uint64_t ry = 1, rz = 1;
+      r = 1;
+      if (off[c + 1] > off[c]) {
+        const int *m = vmm + 6 * c;
+        const uint64_t rx = (uint64_t)((int64_t)m[3] - m[0] + 1);
+        ry = (uint64_t)((int64_t)m[4] - m[1] + 1);
+        rz = (uint64_t)((int64_t)m[5] - m[2] + 1);
+        const double cells = (double)rx * (double)ry * (double)rz;
+        if (cells < 0x1p48) r = rx * ry * rz;
+        else st = 1;
+      }
+      ryz[2 * c] = ry;
+      ryz[2 * c + 1] = rz;
+      status[c] = st;
+    }
+    uint64_t inc = r;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t t = __shfl_up(inc, d);
+      if (l >= d) inc += t;
+    }
+    if (c < ncl) base[c] = carry + inc - r;
+    carry += __shfl(inc, 63);
+  }
+  if (l == 0) {
+    base[ncl] = carry;
+    int b = 0;
+    const uint64_t mx = carry ? carry - 1 : 0;
+    while (b < 64 && (mx >> b) != 0) ++b;
+    *bits = b;
+  }
+}
+
+__global__ void seg_vox_keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+                               const uint32_t *__restrict__ off, int ncl, const float *__restrict__ leaf,
+                               const int *__restrict__ vmm, const uint64_t *__restrict__ ryz,
+                               const uint64_t *__restrict__ base, const uint32_t *__restrict__ status,
+                               uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= off[ncl]) return;
+  const int c = cloud_of(off, ncl, i);
+  uint64_t k = base[c];
+  if (!status[c]) {
+    const float inv = __fdiv_rn(1.0f, leaf[c]);
+    const int *m = vmm + 6 * c;
+    const uint64_t a = (uint64_t)(uint32_t)(vcoord(x[i], inv) - m[0]);
+    const uint64_t b = (uint64_t)(uint32_t)(vcoord(y[i], inv) - m[1]);
+    const uint64_t cc = (uint64_t)(uint32_t)(vcoord(z[i], inv) - m[2]);
+    k += (a * ryz[2 * c] + b) * ryz[2 * c + 1] + cc;
+  }
+  keys[i] = k;
+  vals[i] = i;
+}
+
+// first output voxel of each cloud: the run index at its first sorted point (clouds are key-major)
+__global__ void seg_vox_offsets_k(const uint32_t *__restrict__ off, int ncl, const uint32_t *__restrict__ vid,
+                                  const uint32_t *__restrict__ nvox, uint32_t *__restrict__ out_off) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncl) return;
+  const uint32_t o = off[c];
+  out_off[c] = o < off[ncl] ? vid[o] : *nvox;
+}
+
+// hypotheses of every cloud from one rand() stream (each cloud's call starts with srand(0)):
+// draws r[3h..3h+2] % n_c (ransac_plane.cpp:69-71), model as compute_plane_coefficients
+__global__ void seg_plane_models_k(const float *__restrict__ x, const float *__restrict__ y,
+                                   const float *__restrict__ z, const uint32_t *__restrict__ off, int ncl,
+                                   const int32_t *__restrict__ draws, int H, float collinear,
+                                   float *__restrict__ models, int32_t *__restrict__ valid) {
+  const size_t gh = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gh >= (size_t)ncl * H) return;
+  const int c = (int)(gh / H), h = (int)(gh % H);
+  const uint32_t n = off[c + 1] - off[c];
+  valid[gh] = 0;
+  if (n < 3) return;
+  const uint32_t i1 = (uint32_t)draws[3 * h] % n, i2 = (uint32_t)draws[3 * h + 1] % n,
+                 i3 = (uint32_t)draws[3 * h + 2] % n;
+  if (i1 == i2 || i1 == i3 || i2 == i3) return;
+  const uint32_t o = off[c];
+  const float x1 = x[o + i1], y1 = y[o + i1], z1 = z[o + i1];
+  const float v1x = __fsub_rn(x[o + i2], x1), v1y = __fsub_rn(y[o + i2], y1), v1z = __fsub_rn(z[o + i2], z1);
+  const float v2x = __fsub_rn(x[o + i3], x1), v2y = __fsub_rn(y[o + i3], y1), v2z = __fsub_rn(z[o + i3], z1);
+  float a = __fsub_rn(__fmul_rn(v1y, v2z), __fmul_rn(v1z, v2y));
+  float b = __fsub_rn(__fmul_rn(v1z, v2x), __fmul_rn(v1x, v2z));
+  float cc = __fsub_rn(__fmul_rn(v1x, v2y), __fmul_rn(v1y, v2x));
+  const float norm = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(cc, cc)));
+  if (norm < collinear) return;
+  a = __fdiv_rn(a, norm);
+  b = __fdiv_rn(b, norm);
+  cc = __fdiv_rn(cc, norm);
+  const float d = -__fadd_rn(__fadd_rn(__fmul_rn(a, x1), __fmul_rn(b, y1)), __fmul_rn(cc, z1));
+  models[4 * gh + 0] = a;
+  models[4 * gh + 1] = b;
+  models[4 * gh + 2] = cc;
+  models[4 * gh + 3] = d;
+  valid[gh] = 1;
+}
+
+// tiles of RC_TILE points that never straddle clouds: tile_off[c] = first tile of cloud c (one wave)
+__global__ __launch_bounds__(64) void seg_tiles_k(const uint32_t *__restrict__ off, int ncl,
+                                                  uint32_t *__restrict__ tile_off) {
+  const int l = lane();
+  uint32_t carry = 0;
+  for (int b0 = 0; b0 < ncl; b0 += 64) {
+    const int c = b0 + l;
+    const uint32_t t = c < ncl ? (off[c + 1] - off[c] + RC_TILE - 1) / RC_TILE : 0u;
+    const uint32_t inc = wave_incl_scan(t);
+    if (c < ncl) tile_off[c] = carry + inc - t;
+    carry += __shfl(inc, 63);
+  }
+  if (l == 0) tile_off[ncl] = carry;
+}
+
+__global__ __launch_bounds__(256) void seg_plane_count_k(const float *__restrict__ x, const float *__restrict__ y,
+                                                          const float *__restrict__ z, const uint32_t *__restrict__ off,
+                                                          int ncl, const uint32_t *__restrict__ tile_off,
+                                                          const float *__restrict__ models,
+                                                          const int32_t *__restrict__ valid, int H, float thr,
+                                                          int32_t *__restrict__ counts) {
+  const uint32_t tile = blockIdx.x;
+  if (tile >= tile_off[ncl]) return;
+  const int c = cloud_of(tile_off, ncl, tile);
+  __shared__ int sc[RC_HYP];
+  if (threadIdx.x < RC_HYP) sc[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t first = off[c] + (tile - tile_off[c]) * RC_TILE, end = off[c + 1];
+  float px[RC_PTS], py[RC_PTS], pz[RC_PTS];
+  bool ok[RC_PTS];
+#pragma unroll
+  for (int j = 0; j < RC_PTS; ++j) {
+    const uint32_t i = first + j * 256 + threadIdx.x;
+    ok[j] = i < end;
+    px[j] = ok[j] ? x[i] : 0.f;
+    py[j] = ok[j] ? y[i] : 0.f;
+    pz[j] = ok[j] ? z[i] : 0.f;
+  }
+  const int h0 = blockIdx.y * RC_HYP;
+  const int hn = min(RC_HYP, H - h0);
+  const size_t mb = (size_t)c * H;
+  for (int hh = 0; hh < hn; ++hh) {
+    const size_t h = mb + h0 + hh;
+    if (!valid[h]) continue;
+    const float a = models[4 * h], b = models[4 * h + 1], cc = models[4 * h + 2], d = models[4 * h + 3];
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < RC_PTS; ++j) {
+      const float dist = fabsf(__fadd_rn(
+          __fadd_rn(__fadd_rn(__fmul_rn(a, px[j]), __fmul_rn(b, py[j])), __fmul_rn(cc, pz[j])), d));
+      cnt += popc(ballot(ok[j] && dist <= thr));
+    }
+    if (lane() == 0 && cnt) atomicAdd(&sc[hh], cnt);
+  }
+  __syncthreads();
+  if (threadIdx.x < hn && sc[threadIdx.x]) atomicAdd(&counts[mb + h0 + threadIdx.x], sc[threadIdx.x]);
+}
+
+// glibc rand() after srand(0), cached per device and grown on demand
+static int32_t *g_draws_dev[64] = {nullptr};
+static size_t g_draws_len[64] = {0};
+
+static int rand_draws(size_t count, const int32_t **out) {
+  int dev = 0;
+  RVP_HIP(hipGetDevice(&dev));
+  RVP_CHECK(dev >= 0 && dev < 64, RVP_E_UNSUPPORTED, "device index");
+  if (g_draws_len[dev] < count) {
+    const size_t len = std::max<size_t>(count, 3072);
+    std::vector<int32_t> h(len);
+    GlibcRand rng(0);
+    for (auto &v : h) v = rng.next();
+    int32_t *d = nullptr;
+    RVP_HIP(hipMalloc(&d, len * sizeof(int32_t)));
+    RVP_HIP(hipMemcpy(d, h.data(), len * sizeof(int32_t), hipMemcpyHostToDevice));
+    // a shorter previous table is kept: launches on other streams may still read it
+    g_draws_dev[dev] = d;
+    g_draws_len[dev] = len;
+  }
+  *out = g_draws_dev[dev];
+  return RVP_OK;
 }
 
 }  // namespace rvp
@@ -315,17 +594,11 @@ int rvp_ransac_plane(const float *x, const float *y, const float *z, size_t n, f
   hipStream_t s = S(st);
 
   GlibcRand rng(0);  // std::srand(0) (ransac_plane.cpp:60)
-  int best = 0;
-  float best_model[4] = {0, 0, 0, 0};
-  int k_iters = max_iters;
-  const double log_p = std::log(1.0 - std::clamp((double)probability, 0.5, 0.9999));
-
+  RansacReplay rp(n, max_iters, probability);
   Tmp idx_d, models_d, valid_d, counts_d;
-  int iter = 0;
-  int chunk = 128;
-  bool done = max_iters <= 0;
-  while (!done) {
-    const int H = std::min(chunk, max_iters - iter);
+  int chunk = 128, drawn = 0;
+  while (!rp.done) {
+    const int H = std::min(chunk, max_iters - drawn);
     if (H <= 0) break;
     // three draws per iteration whether or not it is skipped (ransac_plane.cpp:69-72)
     std::vector<int32_t> idx3((size_t)H * 3);
@@ -338,6 +611,7 @@ int rvp_ransac_plane(const float *x, const float *y, const float *z, size_t n, f
       idx3[3 * h + 1] = i2;
       idx3[3 * h + 2] = i3;
     }
+    drawn += H;
     RVP_TRY(idx_d.alloc(idx3.size() * sizeof(int32_t), s));
     RVP_TRY(models_d.alloc((size_t)H * 4 * sizeof(float), s));
     RVP_TRY(valid_d.alloc((size_t)H * sizeof(int32_t), s));
@@ -354,34 +628,12 @@ int rvp_ransac_plane(const float *x, const float *y, const float *z, size_t n, f
     RVP_HIP(hipMemcpyAsync(valid.data(), valid_d.p, valid.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     RVP_HIP(hipMemcpyAsync(counts.data(), counts_d.p, counts.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     RVP_HIP(hipStreamSynchronize(s));
-    // replay of the sequential loop (ransac_plane.cpp:68-107)
-    for (int h = 0; h < H; ++h, ++iter) {
-      if (!(iter < k_iters && iter < max_iters)) {
-        done = true;
-        break;
-      }
-      if (!valid[h]) continue;
-      const int cur = counts[h];
-      if (cur > best) {
-        best = cur;
-        for (int k = 0; k < 4; ++k) best_model[k] = models[4 * h + k];
-        const double w = (double)best / (double)n;
-        double p_no = 1.0 - std::pow(w, 3.0);
-        p_no = std::max(std::numeric_limits<double>::epsilon(), p_no);
-        p_no = std::min(1.0 - std::numeric_limits<double>::epsilon(), p_no);
-        const double log_no = std::log(p_no);
-        if (std::abs(log_no) > 1e-7) {
-          const int dyn = (int)std::ceil(log_p / log_no);
-          if (dyn > 0 && dyn < k_iters) k_iters = dyn;
-        }
-      }
-    }
-    if (iter >= k_iters || iter >= max_iters) done = true;
+    rp.feed(models.data(), valid.data(), counts.data(), H);
     chunk = std::min(chunk * 2, 1024);
   }
-  for (int k = 0; k < 4; ++k) model[k] = best_model[k];
-  *inliers = best;
-  if (iterations) *iterations = std::min(iter, std::min(k_iters, max_iters));
+  for (int k = 0; k < 4; ++k) model[k] = rp.model[k];
+  *inliers = rp.best;
+  if (iterations) *iterations = rp.iterations();
   return RVP_OK;
 }
 
@@ -403,6 +655,113 @@ int rvp_plane_extract(const float *x, const float *y, const float *z, size_t n, 
   RVP_HIP(hipStreamSynchronize(s));
   *n_in = h[0];
   *n_out = h[1];
+  return RVP_OK;
+}
+
+int rvp_batch_voxel(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                    size_t n_bound, const float *leaf, float *out_aos, uint32_t *out_offsets, uint32_t *status,
+                    rvp_stream_t st) {
+  RVP_CHECK(x && y && z && offsets && leaf && out_aos && out_offsets && status, RVP_E_ARG, "null pointer");
+  RVP_CHECK(nclouds > 0 && n_bound > 0, RVP_E_ARG, "empty batch");
+  RVP_CHECK(n_bound < 0xfffffff0ull, RVP_E_UNSUPPORTED, "more than 2^32-16 points");
+  RVP_TRY(pool_init());
+  hipStream_t s = S(st);
+  SegGrid G;  // only its offset clamp is used
+  RVP_TRY(G.offs.alloc((nclouds + 1) * sizeof(uint32_t), s));
+  RVP_TRY(seg_clamp_offsets(offsets, nclouds, n_bound, G.offs.as<uint32_t>(), s));
+  const uint32_t *off = G.offs.as<uint32_t>();
+  const unsigned b6 = ceil_div(6 * nclouds, 256);
+  Tmp vmm, ryz, base, bits, keys, vals, kt, vt, vid, start, nv;
+  RVP_TRY(vmm.alloc(6 * nclouds * sizeof(int), s));
+  RVP_TRY(ryz.alloc(2 * nclouds * sizeof(uint64_t), s));
+  RVP_TRY(base.alloc((nclouds + 1) * sizeof(uint64_t), s));
+  RVP_TRY(bits.alloc(sizeof(int), s));
+  RVP_LAUNCH("seg_vox_init_k", s, seg_vox_init_k<<<b6, 256, 0, s>>>(nclouds, vmm.as<int>()));
+  const unsigned per = (unsigned)std::min<size_t>(64, std::max<size_t>(1, n_bound / ((size_t)nclouds * 4096)));
+  RVP_LAUNCH("seg_vox_minmax_k", s, seg_vox_minmax_k<<<dim3(per, nclouds), 256, 0, s>>>(x, y, z, off, leaf,
+                                                                                     vmm.as<int>()));
+  RVP_LAUNCH("seg_vox_base_k", s, seg_vox_base_k<<<1, 64, 0, s>>>(nclouds, off, vmm.as<int>(), ryz.as<uint64_t>(),
+                                                                  base.as<uint64_t>(), bits.as<int>(), status));
+  RVP_TRY(keys.alloc(n_bound * sizeof(uint64_t), s));
+  RVP_TRY(vals.alloc(n_bound * sizeof(uint32_t), s));
+  RVP_TRY(kt.alloc(n_bound * sizeof(uint64_t), s));
+  RVP_TRY(vt.alloc(n_bound * sizeof(uint32_t), s));
+  const unsigned nb = ceil_div(n_bound, 256);
+  RVP_LAUNCH("seg_vox_keys_k", s, seg_vox_keys_k<<<nb, 256, 0, s>>>(x, y, z, off, nclouds, leaf, vmm.as<int>(),
+                                                                    ryz.as<uint64_t>(), base.as<uint64_t>(), status,
+                                                                    keys.as<uint64_t>(), vals.as<uint32_t>()));
+  uint64_t *kp = keys.as<uint64_t>(), *ktp = kt.as<uint64_t>();
+  uint32_t *vp = vals.as<uint32_t>(), *vtp = vt.as<uint32_t>();
+  const uint32_t *n_dev = off + nclouds;
+  RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n_bound, 64, s, n_dev, bits.as<int>()));
+  RVP_TRY(vid.alloc(n_bound * sizeof(uint32_t), s));
+  RVP_TRY(nv.alloc(sizeof(uint32_t), s));
+  RVP_LAUNCH("run_flags_k", s, run_flags_k<uint64_t><<<nb, 256, 0, s>>>(kp, n_bound, n_dev, vid.as<uint32_t>()));
+  RVP_TRY(exclusive_scan_u32(vid.as<uint32_t>(), vid.as<uint32_t>(), n_bound, nv.as<uint32_t>(), s));
+  RVP_TRY(start.alloc(n_bound * sizeof(uint32_t), s));
+  RVP_LAUNCH("run_starts_k", s, run_starts_k<uint64_t><<<nb, 256, 0, s>>>(kp, n_bound, n_dev, vid.as<uint32_t>(),
+                                                                          start.as<uint32_t>()));
+  RVP_LAUNCH("voxel_centroid_k", s, voxel_centroid_k<<<nb, 256, 0, s>>>(x, y, z, vp, start.as<uint32_t>(),
+                                                                        nv.as<uint32_t>(), (uint32_t)n_bound, n_dev,
+                                                                        out_aos));
+  RVP_LAUNCH("seg_vox_offsets_k", s, seg_vox_offsets_k<<<ceil_div(nclouds + 1, 64), 64, 0, s>>>(
+                                         off, nclouds, vid.as<uint32_t>(), nv.as<uint32_t>(), out_offsets));
+  return RVP_OK;
+}
+
+int rvp_batch_ransac_plane(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                           size_t n_bound, float dist_thresh, int max_iters, float collinear_thresh,
+                           float probability, float *models_out, int *inliers_out, int *iterations_out,
+                           rvp_stream_t st) {
+  RVP_CHECK(x && y && z && offsets && models_out && inliers_out, RVP_E_ARG, "null pointer");
+  RVP_CHECK(nclouds > 0 && n_bound > 0, RVP_E_ARG, "empty batch");
+  RVP_CHECK(n_bound < 0xfffffff0ull, RVP_E_UNSUPPORTED, "more than 2^32-16 points");
+  RVP_CHECK(max_iters >= 0 && (size_t)nclouds * (size_t)std::max(max_iters, 1) < (1ull << 31), RVP_E_UNSUPPORTED,
+            "clouds x hypotheses beyond 2^31");
+  RVP_TRY(pool_init());
+  hipStream_t s = S(st);
+  const int H = std::max(max_iters, 1);
+  const size_t NH = (size_t)nclouds * H;
+  SegGrid G;
+  RVP_TRY(G.offs.alloc((nclouds + 1) * sizeof(uint32_t), s));
+  RVP_TRY(seg_clamp_offsets(offsets, nclouds, n_bound, G.offs.as<uint32_t>(), s));
+  const uint32_t *off = G.offs.as<uint32_t>();
+  const int32_t *draws = nullptr;
+  RVP_TRY(rand_draws((size_t)3 * H, &draws));
+  Tmp models, valid, counts, tiles;
+  RVP_TRY(models.alloc(NH * 4 * sizeof(float), s));
+  RVP_TRY(valid.alloc(NH * sizeof(int32_t), s));
+  RVP_TRY(counts.alloc(NH * sizeof(int32_t), s));
+  RVP_TRY(tiles.alloc((nclouds + 1) * sizeof(uint32_t), s));
+  RVP_LAUNCH("seg_plane_models_k", s, seg_plane_models_k<<<ceil_div(NH, 256), 256, 0, s>>>(
+                                          x, y, z, off, nclouds, draws, H, collinear_thresh, models.as<float>(),
+                                          valid.as<int32_t>()));
+  RVP_HIP(hipMemsetAsync(counts.p, 0, NH * sizeof(int32_t), s));
+  RVP_LAUNCH("seg_tiles_k", s, seg_tiles_k<<<1, 64, 0, s>>>(off, nclouds, tiles.as<uint32_t>()));
+  const dim3 grid((unsigned)(ceil_div(n_bound, RC_TILE) + nclouds), ceil_div((size_t)H, RC_HYP));
+  RVP_LAUNCH("seg_plane_count_k", s, seg_plane_count_k<<<grid, 256, 0, s>>>(
+                                         x, y, z, off, nclouds, tiles.as<uint32_t>(), models.as<float>(),
+                                         valid.as<int32_t>(), H, dist_thresh, counts.as<int32_t>()));
+  std::vector<float> hm(NH * 4);
+  std::vector<int32_t> hv(NH), hc(NH);
+  std::vector<uint32_t> ho(nclouds + 1);
+  RVP_HIP(hipMemcpyAsync(hm.data(), models.p, hm.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  RVP_HIP(hipMemcpyAsync(hv.data(), valid.p, hv.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  RVP_HIP(hipMemcpyAsync(hc.data(), counts.p, hc.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  RVP_HIP(hipMemcpyAsync(ho.data(), off, ho.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  RVP_HIP(hipStreamSynchronize(s));
+  for (int c = 0; c < nclouds; ++c) {
+    const size_t n = ho[c + 1] - ho[c];
+    for (int k = 0; k < 4; ++k) models_out[4 * c + k] = 0.0f;
+    inliers_out[c] = 0;
+    if (iterations_out) iterations_out[c] = 0;
+    if (n < 3) continue;  // ransac_plane.cpp:59
+    RansacReplay rp(n, max_iters, probability);
+    rp.feed(hm.data() + (size_t)c * H * 4, hv.data() + (size_t)c * H, hc.data() + (size_t)c * H, max_iters);
+    for (int k = 0; k < 4; ++k) models_out[4 * c + k] = rp.model[k];
+    inliers_out[c] = rp.best;
+    if (iterations_out) iterations_out[c] = rp.iterations();
+  }
   return RVP_OK;
 }
 

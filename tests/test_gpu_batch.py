@@ -65,3 +65,99 @@ def test_batch_normals_equal_single_cloud(clouds):
         cosang = np.abs((single.astype(np.float64) * ref).sum(1))
         err = np.degrees(np.arccos(np.clip(cosang, -1, 1)))
         assert np.median(err) <= 0.01
+
+
+def test_batch_voxel_matches_oracle(clouds):
+    cl, scales = clouds
+    b = R.Batch(cl)
+    leaf = np.array([0.01 * s for s in scales], np.float32)
+    vox, status = R.batch_voxel(b, leaf)
+    assert (status == 0).all()
+    for c, p in enumerate(cl):
+        ref = O.voxel_sc(p, float(leaf[c])) if len(p) else np.zeros((0, 3), np.float32)
+        assert vox[c].shape == ref.shape and (vox[c] == ref).all(), c
+
+
+def test_batch_voxel_wide_and_negative_keys():
+    rng = np.random.default_rng(7)
+    a = (rng.random((3000, 3)) * 400 - 200).astype(np.float32)   # wide range: keys above 2^32 at leaf 0.01
+    bcl = (rng.random((500, 3)) - 0.5).astype(np.float32)
+    b = R.Batch([a, bcl])
+    vox, status = R.batch_voxel(b, [0.01, 0.05])
+    assert (status == 0).all()
+    for c, (p, lf) in enumerate([(a, 0.01), (bcl, 0.05)]):
+        ref = O.voxel_sc(p, lf)
+        assert (vox[c] == ref).all()
+
+
+def test_batch_normals_radius_equal_single_cloud(clouds):
+    cl, scales = clouds
+    b = R.Batch(cl)
+    radius = np.array([0.03 * s for s in scales], np.float32)
+    nrm = R.batch_normals_radius(b, radius, 32)
+    for c, p in enumerate(cl):
+        if len(p) == 0:
+            continue
+        h = R.SpatialHash()
+        h.setInputCloud(p, float(radius[c]))
+        h.build()
+        single = np.stack(R.normal_estimation_rvv(p, h, 10, float(radius[c]), max_nn=32), 1)
+        assert (b.split(nrm)[c] == single).all(), c
+
+
+def test_batch_ransac_matches_single_and_oracle(clouds):
+    cl, scales = clouds
+    b = R.Batch(cl)
+    models, inl, its = R.batch_ransac_plane(b, 0.005 * 5, 300)
+    for c, p in enumerate(cl):
+        best, model = R.ransac_plane_sc(p, 0.005 * 5, 300)
+        assert inl[c] == best and (models[c] == model).all(), c
+        if len(p) >= 3:
+            ob, om, oit = O.ransac_plane(*soa(p), 0.005 * 5, 300)
+            assert ob == inl[c] and (om == models[c]).all() and oit == its[c], c
+
+
+def test_batch_chained_stages_without_host_sync():
+    """voxel -> SOR -> radius normals -> RANSAC on device offsets equals the per-cloud chain."""
+    import ctypes as C
+
+    from rvpoint_amd._lib import call
+    cl = [scenes.plane_sphere_cloud(n, seed=200 + i, scale=s) for i, (n, s) in
+          enumerate([(3000, 1.0), (800, 2.0), (5000, 4.0)])]
+    b = R.Batch(cl)
+    n, ncl = b.n, b.ncl
+    D = R.DeviceArray
+    leaf = D.from_numpy(np.array([0.01, 0.02, 0.04], np.float32))
+    vox, voff, st = D((n, 3)), D(ncl + 1, np.uint32), D(ncl, np.uint32)
+    call("rvp_batch_voxel", *b.dev.ptrs(), b.offsets.ptr, ncl, n, leaf.ptr, vox.ptr, voff.ptr, st.ptr, None)
+    vx, vy, vz = D(n), D(n), D(n)
+    call("rvp_aos_to_soa", vox.ptr, n, vx.ptr, vy.ptr, vz.ptr, None)
+    sor, soff = D((n, 3)), D(ncl + 1, np.uint32)
+    call("rvp_batch_sor_exact", vx.ptr, vy.ptr, vz.ptr, voff.ptr, ncl, n, 10, 1.0, sor.ptr, soff.ptr, None, None,
+         None)
+    sx, sy, sz = D(n), D(n), D(n)
+    call("rvp_aos_to_soa", sor.ptr, n, sx.ptr, sy.ptr, sz.ptr, None)
+    rad = D.from_numpy(np.array([0.05, 0.1, 0.2], np.float32))
+    nx, ny, nz = D(n), D(n), D(n)
+    call("rvp_batch_normals_radius", sx.ptr, sy.ptr, sz.ptr, soff.ptr, ncl, n, rad.ptr, None, 64, 0.0, 0.0, 0.0,
+         nx.ptr, ny.ptr, nz.ptr, None)
+    m = np.zeros((ncl, 4), np.float32)
+    inl = np.zeros(ncl, np.int32)
+    its = np.zeros(ncl, np.int32)
+    call("rvp_batch_ransac_plane", sx.ptr, sy.ptr, sz.ptr, soff.ptr, ncl, n, 0.02, 200, 1e-6, 0.99,
+         m.ctypes.data_as(C.POINTER(C.c_float)), inl.ctypes.data_as(C.POINTER(C.c_int)),
+         its.ctypes.data_as(C.POINTER(C.c_int)), None)
+    so = soff.numpy()
+    S = sor.numpy()
+    N = np.stack([nx.numpy(), ny.numpy(), nz.numpy()], 1)
+    for c, (p, lf, r) in enumerate(zip(cl, [0.01, 0.02, 0.04], [0.05, 0.1, 0.2])):
+        v = O.voxel_sc(p, lf)
+        kept, _ = O.sor_sc(v, 10, 1.0)
+        assert (S[so[c]:so[c + 1]] == kept).all(), c
+        h = R.SpatialHash()
+        h.setInputCloud(kept, r)
+        h.build()
+        single = np.stack(R.normal_estimation_rvv(kept, h, 10, r, max_nn=64), 1)
+        assert (N[so[c]:so[c + 1]] == single).all(), c
+        best, model = R.ransac_plane_sc(kept, 0.02, 200)
+        assert inl[c] == best and (m[c] == model).all(), c

@@ -322,3 +322,4 @@ def test_exact_sequential_sum(kind):
     if kind != "nan":
         mean = np.float32(exp / np.float32(len(v)))
         assert _gpu_seq_sum(v, True, mean) == O.seq_sum(v, True, mean)
+
