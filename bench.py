@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--cpu-normals-sample", type=int, default=100000)
     ap.add_argument("--batch-steps", type=int, default=3, help="timed passes of the 256-cloud batch leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs per-cloud leg")
+    ap.add_argument("--no-indoor", action="store_true", help="skip the 20M-point indoor leg")
+    ap.add_argument("--indoor-steps", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device-to-device copy of this many MiB before warmup (HBM counter calibration)")
     return ap.parse_args()
@@ -272,6 +274,64 @@ def time_steps(R, fn, steps):
     return (time.perf_counter() - t_all) * 1e3 / steps, times
 
 
+def indoor_leg(R, args):
+    """Dense indoor scan, 20M points: hash build (cell 0.02), radius lists r=0.02 capped at 64 with indices
+    and squared distances for every point, kNN normals k=30.  The radius kernel's HBM roofline uses its
+    algorithmic bytes: 12 (query) + 12 (candidates once) + 4 (count) + 8 per returned neighbour."""
+    from rvpoint_amd import _lib, scenes
+    from rvpoint_amd._lib import call
+    print("bench: generating 20M-point indoor scan", file=sys.stderr, flush=True)
+    pts = scenes.indoor_scan()
+    n = len(pts)
+    D = R.DeviceArray
+    dev = R.DeviceCloud(pts)
+    del pts
+    h = C.c_void_p()
+    call("rvp_hash_create", C.byref(h))
+    cap, r = 64, 0.02
+    idx, d2, cnt = D((n, cap), np.int32), D((n, cap)), D(n, np.int32)
+    nx, ny, nz = D(n), D(n), D(n)
+
+    def build():
+        call("rvp_hash_build", h.value, *dev.ptrs(), n, r, None)
+
+    def radius():
+        call("rvp_radius_search", h.value, None, None, None, n, r, cap, idx.ptr, d2.ptr, cnt.ptr, None, None)
+
+    def normals():
+        call("rvp_normals_knn", *dev.ptrs(), n, 30, 0.0, 0.0, 0.0, nx.ptr, ny.ptr, nz.ptr, None)
+
+    out = {"workload": "indoor scan 20M points (10x8x3 m room, 60 boxes, 20 cylinders, sigma 3 mm): hash build "
+                       "cell 0.02 -> radius lists r=0.02 cap 64 (idx + d2, every point) ; normals k=30",
+           "points": n}
+    for name, fn, kern in (("hash_build", build, None), ("radius", radius, "radius_topk_k"),
+                           ("normals_k30", normals, "normals_topk_k")):
+        fn()
+        R.synchronize()
+        if kern:
+            _lib.profile_enable(True, kern)
+        ms, _ = time_steps(R, fn, args.indoor_steps)
+        if kern:
+            _lib.profile_enable(False)
+            prof = _lib.profile_collect()
+            launches, total = prof.get(kern, (0, 0.0))
+            out[name + "_kernel_ms"] = round(total / max(launches, 1), 3)
+        out[name + "_ms"] = round(ms, 3)
+        out[name + "_mpts_s"] = round(n / (ms * 1e-3) / 1e6, 2)
+        if name == "hash_build":
+            build()  # the hash the queries use
+    m = cnt.numpy().astype(np.int64)
+    mean_nn = float(m.mean())
+    bytes_per_launch = n * (12 + 12 + 4) + 8 * int(m.sum())
+    kms = out["radius_kernel_ms"]
+    achieved = bytes_per_launch / (kms * 1e-3) / 1e9
+    out["radius_mean_neighbours"] = round(mean_nn, 2)
+    out["radius_roofline"] = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(achieved / HBM_PEAK_GBS, 5), "bytes_per_launch": bytes_per_launch}
+    call("rvp_hash_destroy", h.value)
+    return out
+
+
 def cpu_baseline(frame, info, args):
     """The reference's scalar path (oracle/_ref when present, else the oracle port) on a bounded sample of the
     same workload, scaled to the full frame: voxel and RANSAC at full size, exact-kNN SOR on a subset scaled by
@@ -435,6 +495,7 @@ def main():
                                                                      key=lambda kv: -kv[1][1])[:12]},
     }
     line["batched"] = None if args.no_batch else batch_leg(R, args)
+    line["indoor"] = None if args.no_indoor else indoor_leg(R, args)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (reference scalar path, 1 thread)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(frame, info, args)

@@ -153,6 +153,11 @@ int rvp_seq_sum(const float *v, size_t n, int squared_deviation, const float *me
  * (normal_estimation.cpp:55-67).  nx/ny/nz device float[n]. */
 int rvp_normals_knn(const float *x, const float *y, const float *z, size_t n, int k, float vpx, float vpy,
                     float vpz, float *nx, float *ny, float *nz, rvp_stream_t s);
+/* The same with the neighbourhood centroid of every point (the mean of its k+1 neighbours, computed
+ * re-centred at the point with compensated sums): cx/cy/cz device float[n], all NULL or none. */
+int rvp_normals_knn_centroids(const float *x, const float *y, const float *z, size_t n, int k, float vpx, float vpy,
+                              float vpz, float *nx, float *ny, float *nz, float *cx, float *cy, float *cz,
+                              rvp_stream_t s);
 
 /* ---- N2: normal_estimation_rvv(in, SpatialHash&, ...) (normal_estimation.cpp:189-214) --------------
  * neighbourhood = H3 list within radius (max_nn > 0 caps it, 0 = unlimited); fewer than 3 -> zero normal.

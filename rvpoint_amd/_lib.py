@@ -75,6 +75,7 @@ _SIGS = {
     "rvp_sor_radius": [vp, i32, fl, fl, vp, psz, vp, vp, vp],
     "rvp_seq_sum": [vp, sz, i32, vp, vp, vp],
     "rvp_normals_knn": [vp, vp, vp, sz, i32, fl, fl, fl, vp, vp, vp, vp],
+    "rvp_normals_knn_centroids": [vp, vp, vp, sz, i32, fl, fl, fl, vp, vp, vp, vp, vp, vp, vp],
     "rvp_normals_radius": [vp, fl, i32, fl, fl, fl, vp, vp, vp, vp],
     "rvp_normals_fallback_count": [psz],
     "rvp_voxel": [vp, vp, vp, sz, fl, vp, psz, vp],

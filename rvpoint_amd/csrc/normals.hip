@@ -34,7 +34,7 @@ static int fallback_counter(hipStream_t s, unsigned long long **out) {
 constexpr int NB = 8;
 
 struct NormLds {
-  float c[6][NB];  // covariance of query j
+  float c[9][NB];  // covariance of query j (6), then the neighbourhood mean relative to the query (3)
   float q[3][NB];  // query point
   uint32_t row[NB];
   int m[NB];
@@ -148,9 +148,10 @@ __device__ __forceinline__ void flip_toward(float px, float py, float pz, float 
 }
 
 // covariance of the m <= 64 neighbours held one per lane (payload = cell-order position), coordinates
-// re-centred at the query point, compensated butterfly sums; result uniform across the wave
+// re-centred at the query point, compensated butterfly sums; result uniform across the wave.  c[6..8]
+// receives the neighbourhood mean relative to the query.
 __device__ __forceinline__ void wave_cov(const GridView &g, bool valid, uint32_t pos, int m, float qx, float qy,
-                                         float qz, float c[6]) {
+                                         float qz, float c[9]) {
   float px = 0.f, py = 0.f, pz = 0.f;
   if (valid) {
     px = __fsub_rn(g.xs[pos], qx);
@@ -170,12 +171,17 @@ __device__ __forceinline__ void wave_cov(const GridView &g, bool valid, uint32_t
   c[3] = wave_sum_comp(__fmul_rn(dy, dy));
   c[4] = wave_sum_comp(__fmul_rn(dy, dz));
   c[5] = wave_sum_comp(__fmul_rn(dz, dz));
+  c[6] = mx;
+  c[7] = my;
+  c[8] = mz;
 }
 
 // lane j solves query j of the wave's batch
+// (cx may be null; else cx/cy/cz receive the neighbourhood centroid q + mean)
 __device__ __forceinline__ void solve_batch(NormLds &N, int nbatch, float vpx, float vpy, float vpz,
                                             float *__restrict__ nx, float *__restrict__ ny, float *__restrict__ nz,
-                                            unsigned long long *fallbacks) {
+                                            unsigned long long *fallbacks, float *__restrict__ cx = nullptr,
+                                            float *__restrict__ cy = nullptr, float *__restrict__ cz = nullptr) {
   wave_lds_sync();
   const int l = lane();
   bool fb = false;
@@ -190,16 +196,21 @@ __device__ __forceinline__ void solve_batch(NormLds &N, int nbatch, float vpx, f
     nx[r] = n[0];
     ny[r] = n[1];
     nz[r] = n[2];
+    if (cx) {
+      cx[r] = __fadd_rn(N.q[0][l], N.c[6][l]);
+      cy[r] = __fadd_rn(N.q[1][l], N.c[7][l]);
+      cz[r] = __fadd_rn(N.q[2][l], N.c[8][l]);
+    }
   }
   const int nf = popc(ballot(fb));
   if (l == 0 && nf) atomicAdd(fallbacks, (unsigned long long)nf);
   wave_lds_sync();
 }
 
-__device__ __forceinline__ void stash(NormLds &N, int j, const float c[6], float x, float y, float z, uint32_t row,
+__device__ __forceinline__ void stash(NormLds &N, int j, const float c[9], float x, float y, float z, uint32_t row,
                                       int m) {
   if (lane() == 0) {
-    for (int e = 0; e < 6; ++e) N.c[e][j] = c[e];
+    for (int e = 0; e < 9; ++e) N.c[e][j] = c[e];
     N.q[0][j] = x;
     N.q[1][j] = y;
     N.q[2][j] = z;
@@ -212,7 +223,9 @@ __device__ __forceinline__ void stash(NormLds &N, int j, const float c[6], float
 template <int MODE>
 __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, float vpx, float vpy, float vpz,
                                                       float *__restrict__ nx, float *__restrict__ ny,
-                                                      float *__restrict__ nz, unsigned long long *fallbacks) {
+                                                      float *__restrict__ nz, unsigned long long *fallbacks,
+                                                      float *__restrict__ cx, float *__restrict__ cy,
+                                                      float *__restrict__ cz) {
   __shared__ WaveLds lds[QW];
   __shared__ NormLds nl[QW];
   const int w = threadIdx.x >> 6;
@@ -237,11 +250,11 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
       topk_flush<true>(t, L);
     }
     const int m = min(t.cnt, K);
-    float c[6];
+    float c[9];
     wave_cov(g, lane() < m, t.pay, m, x, y, z, c);
     stash(N, j, c, x, y, z, g.perm[q], m);
   }
-  solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
+  solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks, cx, cy, cz);
 }
 
 // N2 without a cap: neighbour positions in CSR rows (row = original index), covariance in chunks of 64
@@ -262,7 +275,7 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
     const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
     const int64_t b = offsets[row];
     const int m = (int)min<int64_t>(offsets[row + 1] - b, (int64_t)cap);
-    float c[6] = {0, 0, 0, 0, 0, 0};
+    float c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (m <= WAVE) {
       wave_cov(g, l < m, l < m ? pos[b + l] : 0u, m, x, y, z, c);
     } else {
@@ -300,6 +313,7 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
         }
       }
       for (int a = 0; a < 6; ++a) c[a] = wave_sum_comp(cs[a], ce[a]);
+      for (int a = 0; a < 3; ++a) c[6 + a] = mu[a];
     }
     stash(N, j, c, x, y, z, row, m);
   }
@@ -339,7 +353,7 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
       topk_flush<true>(t, lds[w]);
     }
     const int m = min(t.cnt, K);
-    float c6[6];
+    float c6[9];
     wave_cov(g, lane() < m, t.pay, m, x, y, z, c6);
     stash(N, j, c6, x, y, z, g.perm[q], m);
   }
@@ -353,8 +367,10 @@ using namespace rvp;
 
 extern "C" {
 
-int rvp_normals_knn(const float *x, const float *y, const float *z, size_t n, int k, float vpx, float vpy, float vpz,
-                    float *nx, float *ny, float *nz, rvp_stream_t st) {
+int rvp_normals_knn_centroids(const float *x, const float *y, const float *z, size_t n, int k, float vpx, float vpy,
+                              float vpz, float *nx, float *ny, float *nz, float *cx, float *cy, float *cz,
+                              rvp_stream_t st) {
+  RVP_CHECK((cx && cy && cz) || (!cx && !cy && !cz), RVP_E_ARG, "centroid outputs: all or none");
   RVP_CHECK(k >= 1 && k + 1 <= MAX_TOPK, RVP_E_UNSUPPORTED, "k must be in [1, 63]");
   RVP_CHECK(nx && ny && nz, RVP_E_ARG, "null output");
   if (n == 0) return RVP_OK;
@@ -365,9 +381,14 @@ int rvp_normals_knn(const float *x, const float *y, const float *z, size_t n, in
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
   RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0><<<ceil_div(n, QW * NB), QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz, nx, ny, nz,
-                                                          fb));
+                                                          fb, cx, cy, cz));
   RVP_LAUNCHED();
   return RVP_OK;
+}
+
+int rvp_normals_knn(const float *x, const float *y, const float *z, size_t n, int k, float vpx, float vpy, float vpz,
+                    float *nx, float *ny, float *nz, rvp_stream_t st) {
+  return rvp_normals_knn_centroids(x, y, z, n, k, vpx, vpy, vpz, nx, ny, nz, nullptr, nullptr, nullptr, st);
 }
 
 int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, float vpy, float vpz, float *nx,
@@ -383,7 +404,7 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
   const GridView g = h->g.view();
   if (max_nn > 0 && max_nn <= MAX_TOPK) {
     RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1><<<ceil_div(n, QW * NB), QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx, ny, nz,
-                                                            fb));
+                                                            fb, nullptr, nullptr, nullptr));
     RVP_LAUNCHED();
   } else {
     // unlimited (or a cap above 64): full lists, sorted by (d2, idx), truncated by the row offsets
