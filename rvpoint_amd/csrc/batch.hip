@@ -403,12 +403,12 @@ __global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, float 
   if (q >= A.off[A.ncl]) return;
   const int c = cloud_of(A.off, A.ncl, q);
   const GridView g = seg_view(A, c);
-  TopK t;
+  TopKT<uint32_t> t;
   t.init(k + 1);
   knn_search<false>(g, g.xs[q], g.ys[q], g.zs[q], t, lds[w]);
   const int l = lane();
   const int m = min(t.cnt, k + 1);
-  const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(key_d2(t.key)) : 0.0f;
+  const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
   float s = 0.0f;
   for (int j = 1; j <= k; ++j) s = s + readlanef(sq, j);
   if (l == 0) means[g.perm[q]] = s / __int2float_rn(k);

@@ -311,12 +311,12 @@ __global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, float *
   const int w = threadIdx.x >> 6;
   const uint32_t q = blockIdx.x * QW + w;
   if (q >= g.n) return;
-  TopK t;
+  TopKT<uint32_t> t;  // distances only: the means read the multiset of the k+1 smallest
   t.init(k + 1);
   knn_search<false>(g, g.xs[q], g.ys[q], g.zs[q], t, lds[w]);
   const int l = lane();
   const int m = min(t.cnt, k + 1);
-  const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(key_d2(t.key)) : 0.0f;
+  const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
   float s = 0.0f;
   for (int j = 1; j <= k; ++j) s = __fadd_rn(s, readlanef(sq, j));  // +0 beyond m leaves s unchanged
   if (l == 0) means[g.perm[q]] = __fdiv_rn(s, __int2float_rn(k));
@@ -334,9 +334,9 @@ __global__ __launch_bounds__(QT) void sor_radius_means_k(GridView g, int k, floa
   const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
   int lo[3], hi[3];
   window3(g, x, y, z, radius, lo, hi);
-  TopK t;
+  TopKT<uint32_t> t;
   t.init(k + 1);
-  TopKSink<false> sink{t, L};
+  TopKSink<false, uint32_t> sink{t, L};
   scan_box(g, x, y, z, lo, hi, false, lo, hi, __fmul_rn(radius, radius), true, L, sink);
   topk_flush<false>(t, L);
   const int l = lane();
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(QT) void sor_radius_means_k(GridView g, int k, floa
   float mean = radius;
   if (m > 1) {
     const int valid_k = min(k, m - 1);
-    const float sq = (l >= 1 && l <= valid_k) ? sqrtf(key_d2(t.key)) : 0.0f;
+    const float sq = (l >= 1 && l <= valid_k) ? sqrtf(kd2(t.key)) : 0.0f;
     float s = 0.0f;
     for (int j = 1; j <= valid_k; ++j) s = __fadd_rn(s, readlanef(sq, j));
     mean = __fdiv_rn(s, __int2float_rn(valid_k));

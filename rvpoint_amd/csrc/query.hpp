@@ -30,29 +30,45 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-struct TopK {
-  uint64_t key;  // this lane's entry; lanes [0, cnt) sorted ascending, the rest ~0
+// KT = uint64_t: key (d2 bits << 32 | idx), the reference's (d2, idx) order; KT = uint32_t: d2 bits only, for
+// callers that read nothing but the multiset of the K smallest distances (SOR's means)
+template <class KT> struct TopKT {
+  KT key;        // this lane's entry; lanes [0, cnt) sorted ascending, the rest all-ones
   uint32_t pay;
   int cnt;       // valid entries (<= 64)
   int K;
-  uint64_t kth;  // key at K-1 once cnt >= K, else ~0 (hits must beat it to be staged)
+  KT kth;        // key at K-1 once cnt >= K, else all-ones (hits must beat it to be staged)
   int nb;        // staged entries
   uint32_t hits; // all hits offered (untruncated count)
   __device__ __forceinline__ void init(int k) {
-    key = ~0ull;
+    key = ~(KT)0;
     pay = 0;
     cnt = 0;
     K = k;
-    kth = ~0ull;
+    kth = ~(KT)0;
     nb = 0;
     hits = 0;
   }
 };
+using TopK = TopKT<uint64_t>;
+
+__device__ __forceinline__ uint64_t xor_lanes(uint64_t v, int m) { return shfl_xor64(v, m); }
+__device__ __forceinline__ uint32_t xor_lanes(uint32_t v, int m) { return (uint32_t)__shfl_xor((int)v, m); }
+__device__ __forceinline__ uint64_t read_lane(uint64_t v, int l) { return readlane64(v, l); }
+__device__ __forceinline__ uint32_t read_lane(uint32_t v, int l) { return readlaneu(v, l); }
+__device__ __forceinline__ float kd2(uint64_t k) { return key_d2(k); }
+__device__ __forceinline__ float kd2(uint32_t k) { return __uint_as_float(k); }
+// the TopK key of a packed (d2, idx) key
+template <class KT> __device__ __forceinline__ KT as_key(uint64_t k) { return (KT)(k >> (64 - 8 * sizeof(KT))); }
 
 // all compare-exchanges keep the minimum at the lower lane ("mirror" bitonic formulation)
-template <bool PAY>
-__device__ __forceinline__ void cx(uint64_t &k, uint32_t &p, int partner_mask, bool lower) {
-  uint64_t ok = shfl_xor64(k, partner_mask);
+template <bool PAY, class KT>
+__device__ __forceinline__ void cx(KT &k, uint32_t &p, int partner_mask, bool lower) {
+  const KT ok = xor_lanes(k, partner_mask);
+  if (!PAY && sizeof(KT) == 4) {
+    k = lower ? min(k, ok) : max(k, ok);
+    return;
+  }
   uint32_t op = PAY ? (uint32_t)__shfl_xor((int)p, partner_mask) : 0u;
   bool take = lower ? (ok < k) : (ok > k);
   if (take) {
@@ -62,7 +78,7 @@ __device__ __forceinline__ void cx(uint64_t &k, uint32_t &p, int partner_mask, b
 }
 
 // sorts every group of `size` lanes ascending (size: power of two <= 64, wave-uniform)
-template <bool PAY> __device__ __forceinline__ void bitonic_sort_groups(uint64_t &k, uint32_t &p, int size) {
+template <bool PAY, class KT> __device__ __forceinline__ void bitonic_sort_groups(KT &k, uint32_t &p, int size) {
   const int l = lane();
 #pragma unroll
   for (int s = 2; s <= WAVE; s <<= 1) {
@@ -73,19 +89,19 @@ template <bool PAY> __device__ __forceinline__ void bitonic_sort_groups(uint64_t
   }
 }
 
-template <bool PAY> __device__ __forceinline__ void bitonic_sort64(uint64_t &k, uint32_t &p) {
+template <bool PAY, class KT> __device__ __forceinline__ void bitonic_sort64(KT &k, uint32_t &p) {
   bitonic_sort_groups<PAY>(k, p, WAVE);
 }
 
 // sorts a bitonic sequence held in each group of `size` lanes
-template <bool PAY> __device__ __forceinline__ void bitonic_clean_groups(uint64_t &k, uint32_t &p, int size) {
+template <bool PAY, class KT> __device__ __forceinline__ void bitonic_clean_groups(KT &k, uint32_t &p, int size) {
   const int l = lane();
 #pragma unroll
   for (int j = WAVE / 2; j > 0; j >>= 1)
     if (j < size) cx<PAY>(k, p, j, (l & j) == 0);
 }
 
-template <bool PAY> __device__ __forceinline__ void bitonic_clean64(uint64_t &k, uint32_t &p) {
+template <bool PAY, class KT> __device__ __forceinline__ void bitonic_clean64(KT &k, uint32_t &p) {
   bitonic_clean_groups<PAY>(k, p, WAVE);
 }
 
@@ -98,11 +114,11 @@ __device__ __forceinline__ int pow2_at_least(int v) {
 // Merge the staged hits into the best list.  Only the K best are ever read, so the list spans
 // W = pow2 >= K lanes: the staged batch is sorted in groups of pow2 >= nb lanes, its W smallest are
 // reversed onto lanes [0, W) and min'ed with the list (a bitonic sequence), and W lanes are cleaned.
-template <bool PAY> __device__ __forceinline__ void topk_flush(TopK &t, WaveLds &L) {
+template <bool PAY, class KT> __device__ __forceinline__ void topk_flush(TopKT<KT> &t, WaveLds &L) {
   if (t.nb == 0) return;
   const int l = lane();
   wave_lds_sync();
-  uint64_t nk = l < t.nb ? L.bk[l] : ~0ull;
+  KT nk = l < t.nb ? (KT)L.bk[l] : ~(KT)0;
   uint32_t np = (PAY && l < t.nb) ? L.bp[l] : 0u;
   wave_lds_sync();
   const int sb = pow2_at_least(max(t.nb, 2));
@@ -114,23 +130,23 @@ template <bool PAY> __device__ __forceinline__ void topk_flush(TopK &t, WaveLds 
     t.pay = np;
   } else {
     // lane l < W takes new[W-1-l] = new[l ^ (W-1)] (the batch's W smallest, reversed)
-    uint64_t rk = shfl_xor64(nk, W - 1);
+    KT rk = xor_lanes(nk, W - 1);
     uint32_t rp = PAY ? (uint32_t)__shfl_xor((int)np, W - 1) : 0u;
-    if (l >= W) rk = ~0ull;
+    if (l >= W) rk = ~(KT)0;
     if (rk < t.key) {
       t.key = rk;
       t.pay = rp;
     }
     bitonic_clean_groups<PAY>(t.key, t.pay, W);
-    if (l >= W) t.key = ~0ull;
+    if (l >= W) t.key = ~(KT)0;
   }
   t.cnt = min(W, t.cnt + t.nb);
   t.nb = 0;
-  t.kth = (t.cnt >= t.K) ? readlane64(t.key, t.K - 1) : ~0ull;
+  t.kth = (t.cnt >= t.K) ? read_lane(t.key, t.K - 1) : ~(KT)0;
 }
 
-template <bool PAY>
-__device__ __forceinline__ void topk_offer(TopK &t, WaveLds &L, bool hit, uint64_t key, uint32_t pay) {
+template <bool PAY, class KT>
+__device__ __forceinline__ void topk_offer(TopKT<KT> &t, WaveLds &L, bool hit, KT key, uint32_t pay) {
   t.hits += (uint32_t)popc(ballot(hit));
   bool pass = hit && key < t.kth;
   uint64_t m = ballot(pass);
@@ -215,19 +231,20 @@ __device__ __forceinline__ void scan_box(const GridView &g, float qx, float qy, 
   }
 }
 
-template <bool PAY> struct TopKSink {
-  TopK &t;
+template <bool PAY, class KT = uint64_t> struct TopKSink {
+  TopKT<KT> &t;
   WaveLds &L;
   __device__ __forceinline__ void operator()(bool, bool hit, uint64_t key, uint32_t pos) {
-    topk_offer<PAY>(t, L, hit, key, pos);
+    topk_offer<PAY>(t, L, hit, as_key<KT>(key), pos);
   }
 };
 
 // Exact k nearest by (d2, idx): 3x3x3 cells around the query, doubling growth until k candidates are
 // known, then the box certified by window3(q, R) with fl(R*R) > d2_k: every point outside it has a
 // larger d2, so the k best are final.
-template <bool PAY>
-__device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy, float qz, TopK &t, WaveLds &L) {
+template <bool PAY, class KT>
+__device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy, float qz, TopKT<KT> &t,
+                                           WaveLds &L) {
   const int c[3] = {cell_clamped(qx, g.mnx, g.cell, g.gx), cell_clamped(qy, g.mny, g.cell, g.gy),
                     cell_clamped(qz, g.mnz, g.cell, g.gz)};
   const int gd[3] = {g.gx, g.gy, g.gz};
@@ -237,7 +254,7 @@ __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy
     lo[a] = max(c[a] - 1, 0);
     hi[a] = min(c[a] + 1, gd[a] - 1);
   }
-  TopKSink<PAY> sink{t, L};
+  TopKSink<PAY, KT> sink{t, L};
   scan_box(g, qx, qy, qz, lo, hi, false, lo, hi, 0.0f, false, L, sink);
   int step = 1;
   for (;;) {
@@ -248,7 +265,7 @@ __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy
     if (whole) break;
     int nlo[3], nhi[3];
     if (t.cnt >= t.K) {
-      const float dk = key_d2(t.kth);
+      const float dk = kd2(t.kth);
       const float R = next_up(__fmul_rn(sqrtf(dk), 1.0f + 0x1p-22f));
       if (!(R < FLT_MAX)) {
 #pragma unroll
