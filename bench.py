@@ -60,6 +60,10 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched vs per-cloud leg")
     ap.add_argument("--no-indoor", action="store_true", help="skip the 20M-point indoor leg")
     ap.add_argument("--indoor-steps", type=int, default=3)
+    ap.add_argument("--no-serve", action="store_true", help="skip the streaming (serving) leg")
+    ap.add_argument("--serve-frames", type=int, default=6)
+    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device-to-device copy of this many MiB before warmup (HBM counter calibration)")
     return ap.parse_args()
@@ -332,6 +336,50 @@ def indoor_leg(R, args):
     return out
 
 
+def serving_leg(R, args):
+    """Streaming LiDAR-like frames (rings in {16, 32, 64, 128}) through the frame server (continuous batching,
+    admission by an HBM budget): capacity under a burst, then Poisson arrivals at fractions of it with
+    frames/s, Mpts/s and p50/p99 latency from submission to results."""
+    from rvpoint_amd import scenes
+    from rvpoint_amd.serve import FrameParams, FrameServer, poisson_run
+    plan = scenes.lidar_stream(frames=args.serve_frames)
+    print(f"bench: generating {len(plan)} streaming frames", file=sys.stderr, flush=True)
+    frames = [scenes.lidar_frame(rings=r, seed=sd) for r, sd in plan]
+    sizes = [len(f) for f in frames]
+    srv = FrameServer(FrameParams.lidar(), slots=args.serve_slots, slot_points=4_000_000, hbm_budget=64 << 30)
+    try:
+        for f in frames:  # warm every size once
+            srv.wait(srv.submit(f))
+        burst = frames * 3
+        t0 = time.perf_counter()
+        res = [srv.wait(t) for t in [srv.submit(f) for f in burst]]
+        dt = time.perf_counter() - t0
+        cap_fps = len(burst) / dt
+        out = {"workload": "LiDAR-like frames, rings in {16,32,64,128} x 18750 azimuths, voxel 0.1 -> SOR k=50 -> "
+                           "normals r=0.5 cap 64 -> RANSAC 1000; continuous batching over "
+                           f"{srv.slots} slots of 4M points, HBM budget 64 GiB",
+               "frame_points": sizes, "slots": srv.slots,
+               "burst": {"frames": len(burst), "frames_s": round(cap_fps, 2),
+                         "mpts_s": round(sum(r["n_in"] for r in res) / dt / 1e6, 2),
+                         "mean_batch_frames": round(float(np.mean([r["batch_frames"] for r in res])), 2)},
+               "offered": []}
+        for frac in (0.3, 0.6, 0.9):
+            rate = frac * cap_fps
+            done = poisson_run(srv, frames, rate, args.serve_seconds)
+            lat = np.array([(r["t_done"] - r["t_submit"]) * 1e3 for r in done])
+            span = max(r["t_done"] for r in done) - min(r["t_submit"] for r in done)
+            out["offered"].append({"rate_frames_s": round(rate, 2), "frames": len(done),
+                                   "frames_s": round(len(done) / span, 2),
+                                   "mpts_s": round(sum(r["n_in"] for r in done) / span / 1e6, 2),
+                                   "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                                   "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                                   "mean_batch_frames": round(float(np.mean([r["batch_frames"] for r in done])), 2)})
+            print(f"bench: offered {rate:.1f} frames/s -> {out['offered'][-1]}", file=sys.stderr, flush=True)
+        return out
+    finally:
+        srv.close()
+
+
 def cpu_baseline(frame, info, args):
     """The reference's scalar path (oracle/_ref when present, else the oracle port) on a bounded sample of the
     same workload, scaled to the full frame: voxel and RANSAC at full size, exact-kNN SOR on a subset scaled by
@@ -496,6 +544,7 @@ def main():
     }
     line["batched"] = None if args.no_batch else batch_leg(R, args)
     line["indoor"] = None if args.no_indoor else indoor_leg(R, args)
+    line["serving"] = None if args.no_serve else serving_leg(R, args)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (reference scalar path, 1 thread)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(frame, info, args)

@@ -233,6 +233,47 @@ int rvp_batch_ransac_plane(const float *x, const float *y, const float *z, const
                            float probability, float *models_out, int *inliers_out, int *iterations_out,
                            rvp_stream_t s);
 
+/* ---- streaming frames (no counterpart in the reference: the serving loop of the streaming workload) ---
+ * A server runs `slots` batches at once, each on its own stream with buffers for slot_points points.  Frames
+ * wait in one FIFO queue; a free slot takes as many as fit (continuous batching) and runs them through
+ * voxel -> exact SOR -> radius normals (capped) -> RANSAC, each frame with its own results, equal to the
+ * frame run alone.  hbm_budget bounds the device memory of the slots (512 B per slot point): fewer slots
+ * run when it holds fewer.  submit copies the frame (AoS, host) and returns at once; wait blocks until the
+ * frame's batch is done and hands over the result arrays (free them with rvp_result_free).  Times are
+ * steady-clock seconds. */
+typedef struct {
+  float leaf;                           /* V1 leaf */
+  int sor_k;                            /* S1 k, in [1, 63] */
+  float sor_alpha;                      /* S1 std multiplier */
+  float normal_radius;                  /* N2 radius */
+  int normal_max_nn;                    /* N2 cap, in [1, 64] */
+  float vp[3];                          /* F1 viewpoint */
+  float ransac_thresh;                  /* R1 distance threshold */
+  int ransac_iters;                     /* R1 max iterations */
+  float ransac_prob;                    /* R1 probability */
+} rvp_frame_params;
+
+typedef struct {
+  uint64_t ticket;
+  size_t n_in, n_voxel, n_sor;          /* points in, voxels, points kept by SOR */
+  float *points;                        /* kept points, AoS float[3 * n_sor] (owned by the result) */
+  float *normals;                       /* their normals, float[3 * n_sor] */
+  float model[4];                       /* RANSAC plane */
+  int inliers, iterations;
+  uint32_t voxel_status;                /* as rvp_batch_voxel's status */
+  int batch_frames;                     /* frames in the batch this one ran in */
+  double t_submit, t_start, t_done;
+} rvp_frame_result;
+
+typedef struct rvp_server rvp_server;
+int rvp_server_create(rvp_server **srv, const rvp_frame_params *params, int slots, size_t slot_points,
+                      size_t hbm_budget);
+int rvp_server_slots(const rvp_server *srv, int *slots);
+int rvp_server_submit(rvp_server *srv, const float *xyz_aos, size_t n, uint64_t *ticket);
+int rvp_server_wait(rvp_server *srv, uint64_t ticket, rvp_frame_result *out);
+int rvp_result_free(rvp_frame_result *r);
+int rvp_server_destroy(rvp_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

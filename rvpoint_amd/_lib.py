@@ -88,6 +88,12 @@ _SIGS = {
     "rvp_batch_normals_radius": [vp, vp, vp, vp, i32, sz, vp, vp, i32, fl, fl, fl, vp, vp, vp, vp],
     "rvp_batch_voxel": [vp, vp, vp, vp, i32, sz, vp, vp, vp, vp, vp],
     "rvp_batch_ransac_plane": [vp, vp, vp, vp, i32, sz, fl, i32, fl, fl, pfl, pi32, pi32, vp],
+    "rvp_server_create": [vp, vp, i32, sz, sz],
+    "rvp_server_slots": [vp, pi32],
+    "rvp_server_submit": [vp, pfl, sz, vp],
+    "rvp_server_wait": [vp, C.c_uint64, vp],
+    "rvp_result_free": [vp],
+    "rvp_server_destroy": [vp],
 }
 _RESTYPE = {"rvp_last_error": C.c_char_p, "rvp_version": C.c_char_p}
 

@@ -4,6 +4,7 @@
 #include <climits>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 #include "batch.hpp"
@@ -511,10 +512,13 @@ __global__ __launch_bounds__(256) void seg_plane_count_k(const float *__restrict
 static int32_t *g_draws_dev[64] = {nullptr};
 static size_t g_draws_len[64] = {0};
 
+static std::mutex g_draws_mu;
+
 static int rand_draws(size_t count, const int32_t **out) {
   int dev = 0;
   RVP_HIP(hipGetDevice(&dev));
   RVP_CHECK(dev >= 0 && dev < 64, RVP_E_UNSUPPORTED, "device index");
+  std::lock_guard<std::mutex> lk(g_draws_mu);
   if (g_draws_len[dev] < count) {
     const size_t len = std::max<size_t>(count, 3072);
     std::vector<int32_t> h(len);

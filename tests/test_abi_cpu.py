@@ -61,3 +61,8 @@ def test_errors_without_device(built):
     assert L.rvp_batch_radius_search(p, p, p, p, 1, 8, None, None, 8, None, None, p, None) == _lib.RVP_E_ARG
     assert L.rvp_batch_normals_knn(p, p, p, p, 1, 8, 0, C.c_float(0), C.c_float(0), C.c_float(0), p, p, p,
                                    None) == _lib.RVP_E_UNSUPPORTED
+    # server: slots and sizes are checked first
+    from rvpoint_amd.serve import FrameParams
+    h = C.c_void_p()
+    assert L.rvp_server_create(C.byref(h), C.byref(FrameParams.lidar()), 0, 1 << 20, 1 << 30) == _lib.RVP_E_ARG
+    assert L.rvp_server_create(C.byref(h), C.byref(FrameParams.lidar()), 1, 16, 1 << 30) == _lib.RVP_E_ARG
