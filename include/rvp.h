@@ -117,6 +117,10 @@ int rvp_radius_fill(const rvp_hash *h, const float *qx, const float *qy, const f
                     float radius, const int64_t *offsets, int32_t *idx_out, float *d2_out, rvp_stream_t s);
 /* exclusive scan helper: offsets[0..n] of counts[0..n) (device), host total */
 int rvp_exclusive_scan(const int32_t *counts, size_t n, int64_t *offsets, int64_t *total_host, rvp_stream_t s);
+/* 32-bit exclusive scan (device in/out, may alias; total_dev may be NULL): the single-pass scan the sorts and
+ * compactions use; rvp_scan_wait_errors reports look-back waits that hit their spin limit (0 when healthy). */
+int rvp_exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev, rvp_stream_t s);
+int rvp_scan_wait_errors(unsigned long long *count);
 
 /* ---- H4: radius_search_sc / radius_search_rvv, batched (src/search/radius_search.cpp:12-59) -------
  * Brute force: the first max_nn hits in index order; counts[i] = min(total, max_nn). */

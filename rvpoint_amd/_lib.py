@@ -55,6 +55,8 @@ _SIGS = {
     "rvp_profile_collect": [C.c_char_p, sz, psz],
     "rvp_profile_filter": [C.c_char_p],
     "rvp_cache_trim": [],
+    "rvp_exclusive_scan_u32": [vp, vp, sz, vp, vp],
+    "rvp_scan_wait_errors": [C.POINTER(C.c_ulonglong)],
     "rvp_tune_set": [C.c_char_p, C.c_double],
     "rvp_tune_get": [C.c_char_p, C.POINTER(C.c_double)],
     "rvp_dist_sq": [vp, vp, vp, sz, fl, fl, fl, vp, vp],

@@ -399,19 +399,20 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
 __global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, float *__restrict__ means) {
   __shared__ WaveLds lds[QW];
   const int w = threadIdx.x >> 6;
-  const uint32_t q = blockIdx.x * QW + w;
-  if (q >= A.off[A.ncl]) return;
-  const int c = cloud_of(A.off, A.ncl, q);
-  const GridView g = seg_view(A, c);
-  TopKT<uint32_t> t;
-  t.init(k + 1);
-  knn_search<false>(g, g.xs[q], g.ys[q], g.zs[q], t, lds[w]);
-  const int l = lane();
-  const int m = min(t.cnt, k + 1);
-  const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
-  float s = 0.0f;
-  for (int j = 1; j <= k; ++j) s = s + readlanef(sq, j);
-  if (l == 0) means[g.perm[q]] = s / __int2float_rn(k);
+  const uint32_t total = A.off[A.ncl];
+  for (uint32_t q = blockIdx.x * QW + w; q < total; q += gridDim.x * QW) {
+    const int c = cloud_of(A.off, A.ncl, q);
+    const GridView g = seg_view(A, c);
+    TopKT<uint32_t> t;
+    t.init(k + 1);
+    knn_search<false>(g, g.xs[q], g.ys[q], g.zs[q], t, lds[w]);
+    const int l = lane();
+    const int m = min(t.cnt, k + 1);
+    const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
+    float s = 0.0f;
+    for (int j = 1; j <= k; ++j) s = s + readlanef(sq, j);
+    if (l == 0) means[g.perm[q]] = s / __int2float_rn(k);
+  }
 }
 
 __global__ void seg_mean_k(const uint32_t *__restrict__ off, int ncl, const float *__restrict__ sums,
@@ -454,28 +455,29 @@ __global__ __launch_bounds__(QT) void seg_radius_topk_k(SegArrays A, const float
                                                          int32_t *__restrict__ counts) {
   __shared__ WaveLds lds[QW];
   const int w = threadIdx.x >> 6;
-  const uint32_t q = blockIdx.x * QW + w;
-  if (q >= A.off[A.ncl]) return;
-  const int c = cloud_of(A.off, A.ncl, q);
-  const GridView g = seg_view(A, c);
-  const float r = radius[c];
-  const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
-  int lo[3], hi[3];
-  window3(g, x, y, z, r, lo, hi);
-  TopK t;
-  t.init(K);
-  TopKSink<false> sink{t, lds[w]};
-  scan_box(g, x, y, z, lo, hi, false, lo, hi, r * r, true, lds[w], sink);
-  topk_flush<false>(t, lds[w]);
-  const int m = min(t.cnt, K);
-  const uint32_t row = g.perm[q];
-  const uint32_t base = A.off[c];
-  const int l = lane();
-  if (l < m) {
-    if (idx_out) idx_out[(size_t)row * K + l] = (int32_t)(key_idx(t.key) - base);
-    if (d2_out) d2_out[(size_t)row * K + l] = key_d2(t.key);
+  const uint32_t nq = A.off[A.ncl];
+  for (uint32_t q = blockIdx.x * QW + w; q < nq; q += gridDim.x * QW) {
+    const int c = cloud_of(A.off, A.ncl, q);
+    const GridView g = seg_view(A, c);
+    const float r = radius[c];
+    const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
+    int lo[3], hi[3];
+    window3(g, x, y, z, r, lo, hi);
+    TopK t;
+    t.init(K);
+    TopKSink<false> sink{t, lds[w]};
+    scan_box(g, x, y, z, lo, hi, false, lo, hi, r * r, true, lds[w], sink);
+    topk_flush<false>(t, lds[w]);
+    const int m = min(t.cnt, K);
+    const uint32_t row = g.perm[q];
+    const uint32_t base = A.off[c];
+    const int l = lane();
+    if (l < m) {
+      if (idx_out) idx_out[(size_t)row * K + l] = (int32_t)(key_idx(t.key) - base);
+      if (d2_out) d2_out[(size_t)row * K + l] = key_d2(t.key);
+    }
+    if (l == 0) counts[row] = m;
   }
-  if (l == 0) counts[row] = m;
 }
 
 }  // namespace rvp
@@ -500,7 +502,7 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
     RVP_TRY(mt.alloc(n_bound * sizeof(float), s));
     mp = mt.as<float>();
   }
-  RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<<<ceil_div(n_bound, QW), QT, 0, s>>>(G.arrays(), k, mp));
+  RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<<<query_blocks(n_bound), QT, 0, s>>>(G.arrays(), k, mp));
   RVP_TRY(sums.alloc(nclouds * sizeof(float), s));
   RVP_TRY(mean.alloc(nclouds * sizeof(float), s));
   RVP_TRY(vsum.alloc(nclouds * sizeof(float), s));
@@ -539,7 +541,7 @@ int rvp_batch_radius_search(const float *x, const float *y, const float *z, cons
   hipStream_t s = S(st);
   SegGrid G;
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, false, 0, cell ? cell : radius, s));
-  RVP_LAUNCH("seg_radius_topk_k", s, seg_radius_topk_k<<<ceil_div(n_bound, QW), QT, 0, s>>>(
+  RVP_LAUNCH("seg_radius_topk_k", s, seg_radius_topk_k<<<query_blocks(n_bound), QT, 0, s>>>(
                                          G.arrays(), radius, max_nn, idx_out, d2_out, counts));
   return RVP_OK;
 }

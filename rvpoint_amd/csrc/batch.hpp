@@ -1,6 +1,8 @@
 // batch.hpp -- segmented (many-cloud) grid shared by the batched stages.
 #pragma once
 
+#include <algorithm>
+
 #include "query.hpp"
 
 namespace rvp {
@@ -48,6 +50,12 @@ __device__ __forceinline__ GridView seg_view(const SegArrays &A, int c) {
   v.gz = d.g[2];
   v.n = A.off[A.ncl];
   return v;
+}
+
+// grid of the batched query kernels: they stride over the true total read from the offsets, so the
+// launch need not scale with the bound
+inline unsigned query_blocks(size_t n_bound, size_t per_wave = 1) {
+  return (unsigned)std::min<size_t>(8192, std::max<size_t>(1, ceil_div(n_bound, QW * per_wave)));
 }
 
 // out[c] = running max of min(in[c], n_bound), out[0] = 0: the caller's offsets as a valid partition

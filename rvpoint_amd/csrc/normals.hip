@@ -331,33 +331,33 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
   __shared__ NormLds nl[QW];
   const int w = threadIdx.x >> 6;
   const uint32_t total = A.off[A.ncl];
-  const uint32_t base = (blockIdx.x * QW + w) * NB;
-  if (base >= total) return;
   NormLds &N = nl[w];
-  const int nbatch = (int)min<uint32_t>(NB, total - base);
-  for (int j = 0; j < nbatch; ++j) {
-    const uint32_t q = base + j;
-    const int c = cloud_of(A.off, A.ncl, q);
-    const GridView g = seg_view(A, c);
-    const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
-    TopK t;
-    t.init(K);
-    if (MODE == 0) {
-      knn_search<true>(g, x, y, z, t, lds[w]);
-    } else {
-      const float r = radius[c];
-      int lo[3], hi[3];
-      window3(g, x, y, z, r, lo, hi);
-      TopKSink<true> sink{t, lds[w]};
-      scan_box(g, x, y, z, lo, hi, false, lo, hi, __fmul_rn(r, r), true, lds[w], sink);
-      topk_flush<true>(t, lds[w]);
+  for (uint32_t base = (blockIdx.x * QW + w) * NB; base < total; base += gridDim.x * QW * NB) {
+    const int nbatch = (int)min<uint32_t>(NB, total - base);
+    for (int j = 0; j < nbatch; ++j) {
+      const uint32_t q = base + j;
+      const int c = cloud_of(A.off, A.ncl, q);
+      const GridView g = seg_view(A, c);
+      const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
+      TopK t;
+      t.init(K);
+      if (MODE == 0) {
+        knn_search<true>(g, x, y, z, t, lds[w]);
+      } else {
+        const float r = radius[c];
+        int lo[3], hi[3];
+        window3(g, x, y, z, r, lo, hi);
+        TopKSink<true> sink{t, lds[w]};
+        scan_box(g, x, y, z, lo, hi, false, lo, hi, __fmul_rn(r, r), true, lds[w], sink);
+        topk_flush<true>(t, lds[w]);
+      }
+      const int m = min(t.cnt, K);
+      float c6[9];
+      wave_cov(g, lane() < m, t.pay, m, x, y, z, c6);
+      stash(N, j, c6, x, y, z, g.perm[q], m);
     }
-    const int m = min(t.cnt, K);
-    float c6[9];
-    wave_cov(g, lane() < m, t.pay, m, x, y, z, c6);
-    stash(N, j, c6, x, y, z, g.perm[q], m);
+    solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
   }
-  solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
 }
 
 }  // namespace rvp
@@ -439,7 +439,7 @@ int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const 
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, true, k + 1, nullptr, s));
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
-  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0><<<ceil_div(n_bound, QW * NB), QT, 0, s>>>(
+  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0><<<query_blocks(n_bound, NB), QT, 0, s>>>(
                                      G.arrays(), k + 1, nullptr, vpx, vpy, vpz, nx, ny, nz, fb));
   return RVP_OK;
 }
@@ -456,7 +456,7 @@ int rvp_batch_normals_radius(const float *x, const float *y, const float *z, con
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, false, 0, cell ? cell : radius, s));
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
-  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1><<<ceil_div(n_bound, QW * NB), QT, 0, s>>>(
+  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1><<<query_blocks(n_bound, NB), QT, 0, s>>>(
                                      G.arrays(), max_nn, radius, vpx, vpy, vpz, nx, ny, nz, fb));
   return RVP_OK;
 }

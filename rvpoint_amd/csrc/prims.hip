@@ -1,6 +1,8 @@
 // prims.hip -- exclusive scan, stable LSD radix sort (8-bit digits, wave-ballot ranking), compaction.
 #include "prims.hpp"
 
+#include <map>
+#include <mutex>
 #include <utility>
 
 namespace rvp {
@@ -78,11 +80,165 @@ static int exclusive_scan_impl(const uint32_t *in, Out *out, size_t n, Out *tota
   return RVP_OK;
 }
 
+// ------------------------------------------------------------------------------------------------
+// single-pass exclusive scan (decoupled look-back) for 32-bit sums.  Tiles take their index from a ticket
+// counter in dispatch order (atomicInc wraps it back to 0 after the last tile), publish their aggregate,
+// and wave 0 looks back over up to 64 predecessors at a time for the nearest inclusive prefix.  A tile only
+// waits on tiles with smaller tickets, which were dispatched before it, so the wait ends; a spin limit
+// turns a wait that would not end into a counted error instead of a hang.  Status words carry the call's
+// epoch (30 bits), so the per-stream status array never needs clearing.
+// ------------------------------------------------------------------------------------------------
+constexpr uint32_t SP_AGG = 1, SP_INC = 2;
+constexpr uint32_t SP_SPIN_LIMIT = 1u << 24;
+__device__ unsigned long long g_scan_spin_errors;
+
+__device__ __forceinline__ uint64_t sp_word(uint32_t epoch, uint32_t flag, uint32_t v) {
+  return ((uint64_t)((epoch << 2) | flag) << 32) | v;
+}
+
+// (in and out may be the same array: a tile reads all its inputs before it writes)
+__global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in, uint32_t *out, size_t n,
+                                                              uint64_t *__restrict__ status,
+                                                              uint32_t *__restrict__ ticket, uint32_t nb,
+                                                              uint32_t epoch, uint32_t *__restrict__ total) {
+  __shared__ uint32_t sm[PRIM_THREADS / WAVE];
+  __shared__ uint32_t s_tile, s_prefix;
+  const int t = threadIdx.x;
+  if (t == 0) s_tile = atomicInc(ticket, nb - 1);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const size_t base = (size_t)tile * SCAN_TILE;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + (size_t)j * PRIM_THREADS + t;
+    v[j] = i < n ? in[i] : 0u;
+    acc += v[j];
+  }
+  acc = wave_sum(acc);
+  if (lane() == 0) sm[t >> 6] = acc;
+  __syncthreads();
+  const uint32_t agg = sm[0] + sm[1] + sm[2] + sm[3];
+  if (t < WAVE) {
+    const int l = t;
+    uint32_t prefix = 0;
+    if (tile == 0) {
+      if (l == 0)
+        __hip_atomic_store(&status[0], sp_word(epoch, SP_INC, agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      if (l == 0)
+        __hip_atomic_store(&status[tile], sp_word(epoch, SP_AGG, agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      int64_t w0 = (int64_t)tile - 1;
+      bool bad = false;
+      for (;;) {
+        const int64_t idx = w0 - l;
+        uint64_t word = sp_word(epoch, SP_INC, 0);  // before tile 0: an inclusive zero
+        uint32_t spins = 0;
+        for (;;) {
+          bool ready = true;
+          if (idx >= 0) {
+            word = __hip_atomic_load(&status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            ready = (uint32_t)(word >> 34) == epoch && ((word >> 32) & 3u) != 0u;
+          }
+          if (!ballot(!ready)) break;
+          if (++spins >= SP_SPIN_LIMIT) {
+            bad = true;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        if (bad) break;
+        const uint64_t inc = ballot(((word >> 32) & 3u) == SP_INC);
+        const uint32_t val = (uint32_t)word;
+        if (inc) {
+          const int first = __ffsll((unsigned long long)inc) - 1;  // nearest inclusive predecessor
+          prefix += wave_sum(l <= first ? val : 0u);
+          break;
+        }
+        prefix += wave_sum(val);
+        w0 -= WAVE;
+      }
+      if (bad && l == 0) atomicAdd(&g_scan_spin_errors, 1ull);
+      if (l == 0)
+        __hip_atomic_store(&status[tile], sp_word(epoch, SP_INC, prefix + agg), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (l == 0) {
+      s_prefix = prefix;
+      if (total && tile == nb - 1) *total = prefix + agg;
+    }
+  }
+  __syncthreads();
+  uint32_t carry = s_prefix;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + (size_t)j * PRIM_THREADS + t;
+    uint32_t bt;
+    const uint32_t ex = block_excl_scan_256(v[j], bt, sm);
+    if (i < n) out[i] = carry + ex;
+    carry += bt;
+  }
+}
+
+// per-(device, stream) look-back state: status words, ticket counter, epoch
+struct ScanState {
+  uint64_t *status = nullptr;
+  uint32_t *ticket = nullptr;
+  size_t cap = 0;
+  uint32_t epoch = 0;
+};
+static std::mutex g_scan_mu;
+static std::map<std::pair<int, hipStream_t>, ScanState> g_scan;
+
+static int scan_state(hipStream_t s, size_t tiles, uint64_t **status, uint32_t **ticket, uint32_t *epoch) {
+  int dev = 0;
+  RVP_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scan_mu);
+  ScanState &st = g_scan[{dev, s}];
+  if (!st.ticket) {
+    RVP_HIP(hipMalloc(&st.ticket, sizeof(uint32_t)));
+    RVP_HIP(hipMemset(st.ticket, 0, sizeof(uint32_t)));
+  }
+  if (st.cap < tiles) {
+    // the stream may still run a scan on the old array: wait before replacing it
+    RVP_HIP(hipStreamSynchronize(s));
+    if (st.status) RVP_HIP(hipFree(st.status));
+    const size_t cap = std::max<size_t>(tiles, 4096);
+    RVP_HIP(hipMalloc(&st.status, cap * sizeof(uint64_t)));
+    RVP_HIP(hipMemset(st.status, 0, cap * sizeof(uint64_t)));
+    st.cap = cap;
+  }
+  st.epoch = (st.epoch + 1) & 0x3fffffffu;
+  if (st.epoch == 0) st.epoch = 1;  // 0 is the cleared state
+  *status = st.status;
+  *ticket = st.ticket;
+  *epoch = st.epoch;
+  return RVP_OK;
+}
+
 int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev, hipStream_t s) {
-  return exclusive_scan_impl<uint32_t>(in, out, n, total_dev, s);
+  if (n == 0) {
+    if (total_dev) RVP_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), s));
+    return RVP_OK;
+  }
+  const size_t nb = ceil_div(n, SCAN_TILE);
+  RVP_CHECK(nb < 0xffffffffull, RVP_E_UNSUPPORTED, "scan too large");
+  uint64_t *status = nullptr;
+  uint32_t *ticket = nullptr, epoch = 0;
+  RVP_TRY(scan_state(s, nb, &status, &ticket, &epoch));
+  RVP_LAUNCH("scan_1pass_k", s, scan_1pass_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, out, n, status, ticket,
+                                                                                  (uint32_t)nb, epoch, total_dev));
+  return RVP_OK;
 }
 int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *out, size_t n, uint64_t *total_dev, hipStream_t s) {
   return exclusive_scan_impl<uint64_t>(in, out, n, total_dev, s);
+}
+
+unsigned long long scan_spin_errors() {
+  unsigned long long h = 0;
+  if (hipMemcpyFromSymbol(&h, HIP_SYMBOL(g_scan_spin_errors), sizeof(h)) != hipSuccess) return ~0ull;
+  return h;
 }
 
 // ------------------------------------------------------------------------------------------------

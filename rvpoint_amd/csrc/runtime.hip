@@ -294,6 +294,20 @@ int rvp_exclusive_scan(const int32_t *counts, size_t n, int64_t *offsets, int64_
   return RVP_OK;
 }
 
+int rvp_exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev, rvp_stream_t s) {
+  RVP_CHECK((in && out) || n == 0, RVP_E_ARG, "null array");
+  RVP_TRY(pool_init());
+  return exclusive_scan_u32(in, out, n, total_dev, S(s));
+}
+
+int rvp_scan_wait_errors(unsigned long long *count) {
+  RVP_CHECK(count, RVP_E_ARG, "null argument");
+  RVP_HIP(hipDeviceSynchronize());
+  *count = scan_spin_errors();
+  RVP_CHECK(*count != ~0ull, RVP_E_DEVICE, "reading the scan error counter failed");
+  return RVP_OK;
+}
+
 int rvp_tune_set(const char *name, double value) {
   RVP_CHECK(name, RVP_E_ARG, "null name");
   if (std::string(name) == "knn_cell_div") {

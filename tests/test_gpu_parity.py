@@ -323,3 +323,24 @@ def test_exact_sequential_sum(kind):
         mean = np.float32(exp / np.float32(len(v)))
         assert _gpu_seq_sum(v, True, mean) == O.seq_sum(v, True, mean)
 
+
+
+@pytest.mark.parametrize("n", [1, 5, 2048, 2049, 130_000, 1_000_007, 4_194_304])
+def test_single_pass_scan(n):
+    """The single-pass (look-back) exclusive scan against numpy, out of place and in place, with the total."""
+    import ctypes as C
+
+    from rvpoint_amd import _lib
+    rng = np.random.default_rng(n)
+    v = rng.integers(0, 300, n, dtype=np.uint32)
+    d_in = R.DeviceArray.from_numpy(v)
+    d_out = R.DeviceArray(n, np.uint32)
+    tot = R.DeviceArray(1, np.uint32)
+    _lib.call("rvp_exclusive_scan_u32", d_in.ptr, d_out.ptr, n, tot.ptr, None)
+    ref = np.concatenate([[0], np.cumsum(v, dtype=np.uint64)[:-1]]).astype(np.uint32)
+    assert (d_out.numpy() == ref).all() and int(tot.numpy()[0]) == int(v.sum(dtype=np.uint64)) & 0xffffffff
+    _lib.call("rvp_exclusive_scan_u32", d_in.ptr, d_in.ptr, n, None, None)   # in place
+    assert (d_in.numpy() == ref).all()
+    errs = C.c_ulonglong()
+    _lib.call("rvp_scan_wait_errors", C.byref(errs))
+    assert errs.value == 0
