@@ -237,6 +237,29 @@ int rvp_batch_ransac_plane(const float *x, const float *y, const float *z, const
                            float probability, float *models_out, int *inliers_out, int *iterations_out,
                            rvp_stream_t s);
 
+/* ---- PCD ingest (src/io/simple_pcd_loader.h:143-328) --------------------------------------------------
+ * DATA ascii, binary and binary_compressed (LZF), x/y/z as 4-byte fields of any record layout, header rules
+ * of the reference's parser.  info: point count from the header, data mode 0 ascii / 1 binary / 2
+ * binary_compressed / -1 other.  read: host SoA; read_device: device SoA (binary records de-interleaved on
+ * the GPU, compressed blocks copied straight in; returns after the copies).  *n receives the count even
+ * when cap is too small (RVP_E_ARG).  write: as savePCD (:295-328), FIELDS x y z, ascii or binary. */
+int rvp_pcd_info(const char *path, size_t *n_points, int *data_mode);
+int rvp_pcd_read(const char *path, float *x, float *y, float *z, size_t cap, size_t *n);
+int rvp_pcd_read_device(const char *path, float *x, float *y, float *z, size_t cap, size_t *n, rvp_stream_t s);
+int rvp_pcd_write(const char *path, const float *x, const float *y, const float *z, size_t n, int binary);
+
+/* ---- Euclidean clustering (src/segmentation/euclidean_clustering.cpp:113-189) -----------------------
+ * Connected components of the graph linking points of the hash's cloud within `tolerance`
+ * (((dx*dx + dy*dy) + dz*dz) <= fl(tol*tol), as radiusQueryUnvisited :98-109), which are the clusters the
+ * reference's breadth-first search grows.  Clusters of min_size..max_size points, in the order of their
+ * smallest index, each ascending: indices_out (device int32[n]) back to back, offsets_out (device
+ * int64[n+1]) their bounds, *nclusters (host).  labels (device int32[n], may be NULL): every point's
+ * component id, its smallest index.  Any grid edge works (the radius window adapts). */
+int rvp_euclidean_cluster(const rvp_hash *h, float tolerance, int min_size, int max_size, int32_t *indices_out,
+                          int64_t *offsets_out, size_t *nclusters, int32_t *labels, rvp_stream_t s);
+/* union-find steps that hit their bound since load (0 when healthy) */
+int rvp_cluster_errors(unsigned long long *count);
+
 /* ---- streaming frames (no counterpart in the reference: the serving loop of the streaming workload) ---
  * A server runs `slots` batches at once, each on its own stream with buffers for slot_points points.  Frames
  * wait in one FIFO queue; a free slot takes as many as fit (continuous batching) and runs them through

@@ -5,10 +5,12 @@
 #include "core/point_types.h"
 #include "core/rvv_common.h"
 #include "features/normal_estimation.h"
+#include "io/simple_pcd_loader.h"
 #include "filters/statistical_outlier_removal.h"
 #include "filters/voxel_grid.h"
 #include "search/radius_search.h"
 #include "search/spatial_hashing.h"
+#include "segmentation/euclidean_clustering.h"
 #include "segmentation/ransac_plane.h"
 
 namespace rvv_pcl {

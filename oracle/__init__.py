@@ -31,6 +31,9 @@ def build(with_ref: bool | None = None) -> None:
         with_ref = os.path.isdir(REF_SRC)
     if with_ref:
         subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={REF_SRC}"], check=True)
+        if os.path.exists(os.path.join(os.path.dirname(HERE), "rvpoint_amd", "lib", "librvpoint_amd.so")):
+            # the reference's unit tests against the drop-in library (run by the GPU tests)
+            subprocess.run(["make", "-s", "-C", HERE, "reftests", f"REF={REF_SRC}"], check=True)
 
 
 _lib = None
@@ -78,6 +81,10 @@ def lib():
                                        _f32p, C.POINTER(C.c_int)]
         L.orc_plane_extract.argtypes = [_f32p, _f32p, _f32p, _sz, _f32p, C.c_float, _f32p,
                                         C.POINTER(_sz), _f32p, C.POINTER(_sz)]
+        L.orc_euclidean_cluster.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, C.c_int, C.c_int,
+                                            np.ctypeslib.ndpointer(np.int32, flags="C"),
+                                            np.ctypeslib.ndpointer(np.int64, flags="C")]
+        L.orc_euclidean_cluster.restype = _sz
         _lib = L
     return _lib
 
@@ -121,6 +128,13 @@ def ref():
         R.ref_ransac_rvv.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, C.c_int, _f32p, C.c_float, C.c_float]
         R.ref_extract.argtypes = [_f32p, _f32p, _f32p, _sz, _f32p, C.c_float, _f32p, _f32p, C.POINTER(_sz),
                                   C.POINTER(_sz)]
+        R.ref_euclidean_cluster.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, C.c_int, C.c_int,
+                                            np.ctypeslib.ndpointer(np.int32, flags="C"),
+                                            np.ctypeslib.ndpointer(np.int64, flags="C")]
+        R.ref_euclidean_cluster.restype = _sz
+        R.ref_load_pcd.argtypes = [C.c_char_p, _f32p, _sz]
+        R.ref_load_pcd.restype = C.c_int
+        R.ref_save_pcd.argtypes = [C.c_char_p, _f32p, _sz, C.c_int]
         _ref = R
     return _ref
 
@@ -291,3 +305,31 @@ def plane_extract(x, y, z, model, thr):
     lib().orc_plane_extract(_c(x), _c(y), _c(z), n, _c(model), thr, inl.reshape(-1), C.byref(ni),
                             outl.reshape(-1), C.byref(no))
     return inl[:ni.value], outl[:no.value]
+
+
+def _clusters(fn, x, y, z, tol, min_size, max_size):
+    n = len(x)
+    idx = np.zeros(max(n, 1), np.int32)
+    off = np.zeros(n + 2, np.int64)
+    m = fn(_c(x), _c(y), _c(z), n, tol, int(min_size), int(max_size), idx, off)
+    return [idx[off[i]:off[i + 1]].copy() for i in range(m)]
+
+
+def euclidean_cluster(x, y, z, tol, min_size=1, max_size=2**31 - 1):
+    """Clusters (sorted index arrays) in the order of their smallest index."""
+    return _clusters(lib().orc_euclidean_cluster, x, y, z, tol, min_size, max_size)
+
+
+def ref_euclidean_cluster(x, y, z, tol, min_size=1, max_size=2**31 - 1):
+    return _clusters(ref().ref_euclidean_cluster, x, y, z, tol, min_size, max_size)
+
+
+def ref_load_pcd(path, cap):
+    out = np.zeros((max(cap, 1), 3), np.float32)
+    n = ref().ref_load_pcd(str(path).encode(), out.reshape(-1), cap)
+    return out[:max(n, 0)] if n >= 0 else None
+
+
+def ref_save_pcd(path, xyz, binary):
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    ref().ref_save_pcd(str(path).encode(), xyz.reshape(-1), len(xyz), 1 if binary else 0)

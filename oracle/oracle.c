@@ -687,3 +687,48 @@ void orc_plane_extract(const float *x, const float *y, const float *z, size_t n,
     }
     *n_in = ni; *n_out = no;
 }
+
+/* Euclidean clustering (euclidean_clustering.cpp:113-189, scalar neighbour query :98-109). */
+static int cmp_i32(const void *a, const void *b) {
+    const int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+    return (x > y) - (x < y);
+}
+
+size_t orc_euclidean_cluster(const float *x, const float *y, const float *z, size_t n, float tol, int min_size,
+                             int max_size, int32_t *indices, int64_t *offsets) {
+    offsets[0] = 0;
+    if (n == 0) return 0;
+    const float tol_sq = tol * tol;
+    unsigned char *visited = (unsigned char *)calloc(n, 1);
+    int32_t *queue = (int32_t *)malloc(n * sizeof(int32_t));
+    size_t ncl = 0, out = 0;
+    for (size_t seed = 0; seed < n; ++seed) {
+        if (visited[seed]) continue;
+        size_t head = 0, tail = 0;
+        visited[seed] = 1;
+        queue[tail++] = (int32_t)seed;
+        while (head < tail) {
+            const int32_t cur = queue[head++];
+            const float qx = x[cur], qy = y[cur], qz = z[cur];
+            for (size_t i = 0; i < n; ++i) {
+                if (visited[i]) continue;
+                const float dx = x[i] - qx, dy = y[i] - qy, dz = z[i] - qz;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 <= tol_sq) {
+                    visited[i] = 1;
+                    queue[tail++] = (int32_t)i;
+                }
+            }
+        }
+        const int sz = (int)tail;
+        if (sz >= min_size && sz <= max_size) {
+            qsort(queue, tail, sizeof(int32_t), cmp_i32);
+            memcpy(indices + out, queue, tail * sizeof(int32_t));
+            out += tail;
+            offsets[++ncl] = (int64_t)out;
+        }
+    }
+    free(visited);
+    free(queue);
+    return ncl;
+}

@@ -22,6 +22,7 @@
  *   orc_rand_*                      glibc rand() of the host (ctypes)
  *   orc_ransac_plane                oracle/_ref ransac_plane_sc (count + model, bit-exact)
  *   orc_plane_extract               oracle/_ref extract_plane_inliers_outliers_rvv
+ *   orc_euclidean_cluster           oracle/_ref EuclideanClustering::extract; union-find components (numpy)
  */
 #ifndef RVP_ORACLE_H
 #define RVP_ORACLE_H
@@ -127,6 +128,14 @@ int orc_ransac_plane(const float *x, const float *y, const float *z, size_t n, f
 /* X1: extract_plane_inliers_outliers scalar branch (ransac_plane.cpp:374-382). AoS outputs. */
 void orc_plane_extract(const float *x, const float *y, const float *z, size_t n, const float model[4],
                        float thr, float *inl, size_t *n_in, float *outl, size_t *n_out);
+
+/* Euclidean clustering, scalar branch without a search structure (euclidean_clustering.cpp:49-111,
+ * 113-189): breadth-first search from every unvisited seed in index order; the neighbours of a point q are
+ * the unvisited i with ((dx*dx + dy*dy) + dz*dz) <= fl(tol*tol), dx = x_i - q_x, each op rounded once;
+ * clusters of min_size..max_size points are kept, their indices sorted.  CSR output: indices[<= n],
+ * offsets[clusters + 1]; returns the number of clusters. */
+size_t orc_euclidean_cluster(const float *x, const float *y, const float *z, size_t n, float tol, int min_size,
+                             int max_size, int32_t *indices, int64_t *offsets);
 
 #ifdef __cplusplus
 }

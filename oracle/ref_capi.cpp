@@ -8,6 +8,8 @@
 #include "filters/voxel_grid.h"
 #include "filters/statistical_outlier_removal.h"
 #include "features/normal_estimation.h"
+#include "io/simple_pcd_loader.h"
+#include "segmentation/euclidean_clustering.h"
 #include "segmentation/ransac_plane.h"
 
 #include <cstdint>
@@ -110,6 +112,42 @@ void ref_extract(const float *x, const float *y, const float *z, size_t n, const
                  float *inl, float *outl, size_t *ni, size_t *no) {
   extract_plane_inliers_outliers_rvv(soa(x, y, z, n), model, thr, reinterpret_cast<PointXYZ *>(inl),
                                      reinterpret_cast<PointXYZ *>(outl), *ni, *no);
+}
+
+size_t ref_euclidean_cluster(const float *x, const float *y, const float *z, size_t n, float tol, int min_size,
+                             int max_size, int32_t *indices, int64_t *offsets) {
+  rvpoint::PointCloudSoA c{const_cast<float *>(x), const_cast<float *>(y), const_cast<float *>(z), n};
+  rvpoint::EuclideanClustering ec;
+  ec.setInputCloud(c);
+  ec.setClusterTolerance(tol);
+  ec.setMinClusterSize(min_size);
+  ec.setMaxClusterSize(max_size);
+  const auto clusters = ec.extract();
+  size_t out = 0;
+  offsets[0] = 0;
+  for (size_t i = 0; i < clusters.size(); ++i) {
+    for (int v : clusters[i].indices) indices[out++] = v;
+    offsets[i + 1] = (int64_t)out;
+  }
+  return clusters.size();
+}
+
+int ref_load_pcd(const char *path, float *aos, size_t cap) {
+  std::vector<rvpoint::PointXYZ> pts;
+  const int n = rvpoint::loadPCD(path, pts);
+  if (n < 0) return n;
+  for (size_t i = 0; i < pts.size() && i < cap; ++i) {
+    aos[3 * i] = pts[i].x;
+    aos[3 * i + 1] = pts[i].y;
+    aos[3 * i + 2] = pts[i].z;
+  }
+  return n;
+}
+
+void ref_save_pcd(const char *path, const float *aos, size_t n, int binary) {
+  std::vector<rvpoint::PointXYZ> pts(n);
+  for (size_t i = 0; i < n; ++i) pts[i] = {aos[3 * i], aos[3 * i + 1], aos[3 * i + 2]};
+  rvpoint::savePCD(path, pts, binary != 0);
 }
 
 }  // extern "C"

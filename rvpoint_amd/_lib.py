@@ -90,6 +90,12 @@ _SIGS = {
     "rvp_batch_normals_radius": [vp, vp, vp, vp, i32, sz, vp, vp, i32, fl, fl, fl, vp, vp, vp, vp],
     "rvp_batch_voxel": [vp, vp, vp, vp, i32, sz, vp, vp, vp, vp, vp],
     "rvp_batch_ransac_plane": [vp, vp, vp, vp, i32, sz, fl, i32, fl, fl, pfl, pi32, pi32, vp],
+    "rvp_euclidean_cluster": [vp, fl, i32, i32, vp, vp, psz, vp, vp],
+    "rvp_pcd_info": [C.c_char_p, psz, pi32],
+    "rvp_pcd_read": [C.c_char_p, pfl, pfl, pfl, sz, psz],
+    "rvp_pcd_read_device": [C.c_char_p, vp, vp, vp, sz, psz, vp],
+    "rvp_pcd_write": [C.c_char_p, pfl, pfl, pfl, sz, i32],
+    "rvp_cluster_errors": [C.POINTER(C.c_ulonglong)],
     "rvp_server_create": [vp, vp, i32, sz, sz],
     "rvp_server_slots": [vp, pi32],
     "rvp_server_submit": [vp, pfl, sz, vp],

@@ -447,3 +447,72 @@ def batch_ransac_plane(batch: Batch, dist_thresh, max_iters, collinear_thresh=1e
          m.ctypes.data_as(C.POINTER(C.c_float)), inl.ctypes.data_as(C.POINTER(C.c_int)),
          its.ctypes.data_as(C.POINTER(C.c_int)), stream)
     return m, inl, its
+
+
+# ----------------------------------------------------------------------------------------------------
+# Euclidean clustering
+# ----------------------------------------------------------------------------------------------------
+def euclidean_cluster(cloud, tolerance, min_size=1, max_size=2**31 - 1, hash_=None, stream=None):
+    """EuclideanClustering::extract (euclidean_clustering.cpp:113-189): sorted index arrays, one per cluster of
+    min_size..max_size points, in the order of their smallest index.  hash_: a SpatialHash already built on
+    the cloud (any cell); else one with cell = tolerance is built."""
+    if hash_ is None:
+        c = _soa(cloud)
+        if c.n == 0:
+            return []
+        hash_ = SpatialHash()
+        ext = float(max(np.ptp(c.x), np.ptp(c.y), np.ptp(c.z), 1e-30))
+        hash_.setInputCloud(c, max(float(tolerance), ext * 1e-6))
+        hash_.build()
+    n = hash_.dev.n
+    if n == 0:
+        return []
+    idx = DeviceArray(n, np.int32)
+    off = DeviceArray(n + 1, np.int64)
+    m = C.c_size_t()
+    call("rvp_euclidean_cluster", hash_.handle, float(tolerance), int(min_size), int(max_size), idx.ptr, off.ptr,
+         C.byref(m), None, stream)
+    o = off.numpy()[:m.value + 1]
+    I = idx.numpy()
+    return [I[o[i]:o[i + 1]].copy() for i in range(m.value)]
+
+
+# ----------------------------------------------------------------------------------------------------
+# PCD ingest
+# ----------------------------------------------------------------------------------------------------
+def pcd_info(path):
+    n = C.c_size_t()
+    mode = C.c_int()
+    call("rvp_pcd_info", str(path).encode(), C.byref(n), C.byref(mode))
+    return n.value, {0: "ascii", 1: "binary", 2: "binary_compressed"}.get(mode.value, "other")
+
+
+def load_pcd(path):
+    """loadPCD (simple_pcd_loader.h:143-293): (n, 3) float32 xyz."""
+    n, _ = pcd_info(path)
+    x, y, z = (np.zeros(max(n, 1), np.float32) for _ in range(3))
+    m = C.c_size_t()
+    P = C.POINTER(C.c_float)
+    call("rvp_pcd_read", str(path).encode(), x.ctypes.data_as(P), y.ctypes.data_as(P), z.ctypes.data_as(P), n,
+         C.byref(m))
+    k = m.value
+    return np.stack([x[:k], y[:k], z[:k]], 1)
+
+
+def load_pcd_device(path, stream=None):
+    """The same straight into device SoA (a DeviceCloud)."""
+    n, _ = pcd_info(path)
+    d = DeviceCloud.__new__(DeviceCloud)
+    d.x, d.y, d.z = (DeviceArray(max(n, 1), np.float32) for _ in range(3))
+    m = C.c_size_t()
+    call("rvp_pcd_read_device", str(path).encode(), d.x.ptr, d.y.ptr, d.z.ptr, n, C.byref(m), stream)
+    d.n = m.value
+    return d
+
+
+def save_pcd(path, xyz, binary=False):
+    """savePCD (simple_pcd_loader.h:295-328): FIELDS x y z, ascii or binary."""
+    c = _soa(xyz)
+    P = C.POINTER(C.c_float)
+    call("rvp_pcd_write", str(path).encode(), c.x.ctypes.data_as(P), c.y.ctypes.data_as(P), c.z.ctypes.data_as(P),
+         c.n, 1 if binary else 0)

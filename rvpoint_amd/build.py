@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib")
 OBJ = os.path.join(LIB, "obj")
 
-HIP_SOURCES = ["runtime.hip", "prims.hip", "grid.hip", "query.hip", "normals.hip", "voxel_ransac.hip", "distance.hip", "seqsum.hip", "batch.hip", "server.hip"]
+HIP_SOURCES = ["runtime.hip", "prims.hip", "grid.hip", "query.hip", "normals.hip", "voxel_ransac.hip", "distance.hip", "seqsum.hip", "batch.hip", "server.hip", "cluster.hip", "pcd.hip"]
 CXX_SOURCES = ["rvpoint_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

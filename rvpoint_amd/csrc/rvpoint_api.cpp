@@ -398,4 +398,72 @@ std::size_t extract_plane_outliers_rvv(const PointCloudSoA &cloud, const float *
   return no;
 }
 
+// ---------------------------------------------------------------------------------------------------
+// Euclidean clustering (euclidean_clustering.cpp:113-189)
+// ---------------------------------------------------------------------------------------------------
+std::vector<ClusterIndices> EuclideanClustering::extract() const {
+  std::vector<ClusterIndices> result;
+  if (!cloud_ || cloud_->n == 0) return result;
+  const PointCloudSoA &c = *cloud_;
+  const std::size_t n = c.n;
+  float mn[3] = {c.x[0], c.y[0], c.z[0]}, mx[3] = {c.x[0], c.y[0], c.z[0]};
+  for (std::size_t i = 0; i < n; ++i) {
+    const float v[3] = {c.x[i], c.y[i], c.z[i]};
+    for (int a = 0; a < 3; ++a) {
+      if (v[a] < mn[a]) mn[a] = v[a];
+      if (v[a] > mx[a]) mx[a] = v[a];
+    }
+  }
+  float ext = 0.0f;
+  for (int a = 0; a < 3; ++a)
+    if (mx[a] - mn[a] > ext) ext = mx[a] - mn[a];
+  // the grid edge only sets the query window; a zero tolerance still needs a positive edge
+  const float cell = clusterTolerance_ > ext * 1e-6f ? clusterTolerance_ : (ext > 0.0f ? ext * 1e-6f : 1.0f);
+  DevCloud d = upload_soa(c.x, c.y, c.z, n);
+  rvp_hash *h = nullptr;
+  check(rvp_hash_create(&h), "rvp_hash_create");
+  struct HashGuard {
+    rvp_hash *h;
+    ~HashGuard() { rvp_hash_destroy(h); }
+  } guard{h};
+  check(rvp_hash_build(h, d.x.as<float>(), d.y.as<float>(), d.z.as<float>(), n, cell, nullptr), "rvp_hash_build");
+  Dev idx(n * sizeof(int32_t)), off((n + 1) * sizeof(int64_t));
+  std::size_t m = 0;
+  check(rvp_euclidean_cluster(h, clusterTolerance_, minClusterSize_, maxClusterSize_, idx.as<int32_t>(),
+                              off.as<int64_t>(), &m, nullptr, nullptr),
+        "rvp_euclidean_cluster");
+  std::vector<int64_t> ho(m + 1);
+  download(ho.data(), off.p, (m + 1) * sizeof(int64_t));
+  std::vector<int32_t> hi((std::size_t)ho[m]);
+  if (!hi.empty()) download(hi.data(), idx.p, hi.size() * sizeof(int32_t));
+  result.resize(m);
+  for (std::size_t k = 0; k < m; ++k) result[k].indices.assign(hi.begin() + ho[k], hi.begin() + ho[k + 1]);
+  return result;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// PCD I/O (simple_pcd_loader.h:143-328)
+// ---------------------------------------------------------------------------------------------------
+int loadPCD(const std::string &file_path, std::vector<PointXYZ> &points) {
+  std::size_t n = 0;
+  int mode = -1;
+  if (rvp_pcd_info(file_path.c_str(), &n, &mode) != RVP_OK) return -1;
+  std::vector<float> x(n ? n : 1), y(n ? n : 1), z(n ? n : 1);
+  std::size_t m = 0;
+  if (rvp_pcd_read(file_path.c_str(), x.data(), y.data(), z.data(), n, &m) != RVP_OK) return -1;
+  points.resize(m);
+  for (std::size_t i = 0; i < m; ++i) points[i] = {x[i], y[i], z[i]};
+  return static_cast<int>(m);
+}
+
+void savePCD(const std::string &filename, const std::vector<PointXYZ> &points, bool binary) {
+  std::vector<float> x(points.size()), y(points.size()), z(points.size());
+  for (std::size_t i = 0; i < points.size(); ++i) {
+    x[i] = points[i].x;
+    y[i] = points[i].y;
+    z[i] = points[i].z;
+  }
+  (void)rvp_pcd_write(filename.c_str(), x.data(), y.data(), z.data(), points.size(), binary ? 1 : 0);
+}
+
 }  // namespace rvpoint

@@ -277,3 +277,55 @@ def test_plane_extract_equals_reference():
     assert a.value == len(inl) and b.value == len(outl)
     assert (ri[:a.value] == inl).all() and (ro[:b.value] == outl).all()
     assert len(inl) + len(outl) == len(p)
+
+
+def _components_unionfind(p, tol):
+    """Connected components of the fl(d2) <= fl(tol*tol) graph by brute force and union-find."""
+    n = len(p)
+    tol_sq = np.float32(tol) * np.float32(tol)
+    parent = list(range(n))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+    for i in range(n):
+        d = p - p[i]                      # float32, one rounding each
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        for j in np.nonzero(d2 <= tol_sq)[0]:
+            a, b = find(i), find(int(j))
+            if a != b:
+                parent[max(a, b)] = min(a, b)
+    groups = {}
+    for i in range(n):
+        groups.setdefault(find(i), []).append(i)
+    return [np.array(groups[r], np.int32) for r in sorted(groups)]
+
+
+@pytest.mark.parametrize("tol", [0.02, 0.08])
+def test_euclidean_cluster_equals_union_find(tol):
+    rng = np.random.default_rng(31)
+    p = np.concatenate([rng.normal(c, 0.03, (150, 3)) for c in ([0, 0, 0], [0.5, 0, 0], [0, 0.6, 0.2])]
+                       + [rng.random((100, 3)) * 2]).astype(np.float32)
+    x, y, z = (np.ascontiguousarray(p[:, i]) for i in range(3))
+    got = O.euclidean_cluster(x, y, z, tol)
+    ref = _components_unionfind(p, tol)
+    assert len(got) == len(ref) and all((a == b).all() for a, b in zip(got, ref))
+    # size filters keep the same clusters in the same order
+    sizes = [len(c) for c in ref]
+    lo, hi = 2, max(sizes) - 1
+    f = O.euclidean_cluster(x, y, z, tol, lo, hi)
+    keep = [c for c in ref if lo <= len(c) <= hi]
+    assert len(f) == len(keep) and all((a == b).all() for a, b in zip(f, keep))
+
+
+@needs_ref
+@pytest.mark.parametrize("tol,mn,mx", [(0.02, 1, 1 << 30), (0.05, 5, 400), (0.0, 1, 1 << 30)])
+def test_euclidean_cluster_equals_reference(tol, mn, mx):
+    from rvpoint_amd import scenes
+    p = scenes.plane_sphere_cloud(3000, seed=77)
+    x, y, z = (np.ascontiguousarray(p[:, i]) for i in range(3))
+    got = O.euclidean_cluster(x, y, z, tol, mn, mx)
+    ref = O.ref_euclidean_cluster(x, y, z, tol, mn, mx)
+    assert len(got) == len(ref) and all((a == b).all() for a, b in zip(got, ref))
