@@ -121,6 +121,14 @@ int rvp_exclusive_scan(const int32_t *counts, size_t n, int64_t *offsets, int64_
  * compactions use; rvp_scan_wait_errors reports look-back waits that hit their spin limit (0 when healthy). */
 int rvp_exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev, rvp_stream_t s);
 int rvp_scan_wait_errors(unsigned long long *count);
+/* stable LSD radix sort of (key, value) pairs on the low `bits` bits of the key, 8-bit digits, one sweep per
+ * digit: the sorted pairs end in (keys, vals), or in (keys_tmp, vals_tmp) when *in_tmp is set on return.
+ * Device-sized form: n_dev (device, may be NULL) holds the count to sort (<= n), bits_dev (device, may be NULL)
+ * the key width (<= bits); digits above it are not sorted on.  Neither is read on the host. */
+int rvp_sort_pairs_u32(uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp, size_t n, int bits,
+                       const uint32_t *n_dev, const int *bits_dev, int *in_tmp, rvp_stream_t s);
+int rvp_sort_pairs_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_tmp, uint32_t *vals_tmp, size_t n, int bits,
+                       const uint32_t *n_dev, const int *bits_dev, int *in_tmp, rvp_stream_t s);
 
 /* ---- H4: radius_search_sc / radius_search_rvv, batched (src/search/radius_search.cpp:12-59) -------
  * Brute force: the first max_nn hits in index order; counts[i] = min(total, max_nn). */

@@ -57,6 +57,8 @@ _SIGS = {
     "rvp_cache_trim": [],
     "rvp_exclusive_scan_u32": [vp, vp, sz, vp, vp],
     "rvp_scan_wait_errors": [C.POINTER(C.c_ulonglong)],
+    "rvp_sort_pairs_u32": [vp, vp, vp, vp, sz, C.c_int, vp, vp, C.POINTER(C.c_int), vp],
+    "rvp_sort_pairs_u64": [vp, vp, vp, vp, sz, C.c_int, vp, vp, C.POINTER(C.c_int), vp],
     "rvp_tune_set": [C.c_char_p, C.c_double],
     "rvp_tune_get": [C.c_char_p, C.POINTER(C.c_double)],
     "rvp_dist_sq": [vp, vp, vp, sz, fl, fl, fl, vp, vp],

@@ -86,7 +86,10 @@ static int exclusive_scan_impl(const uint32_t *in, Out *out, size_t n, Out *tota
 // and wave 0 looks back over up to 64 predecessors at a time for the nearest inclusive prefix.  A tile only
 // waits on tiles with smaller tickets, which were dispatched before it, so the wait ends; a spin limit
 // turns a wait that would not end into a counted error instead of a hang.  Status words carry the call's
-// epoch (30 bits), so the per-stream status array never needs clearing.
+// epoch (30 bits), so the per-stream status array never needs clearing.  The value travels in the same
+// 64-bit word as its flag, so the status accesses are relaxed device-scope atomics: no other memory has to
+// become visible with them, and an acquire / release pair would write back and invalidate the XCD's L2 on
+// every status access.
 // ------------------------------------------------------------------------------------------------
 constexpr uint32_t SP_AGG = 1, SP_INC = 2;
 constexpr uint32_t SP_SPIN_LIMIT = 1u << 24;
@@ -107,6 +110,10 @@ __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in,
   if (t == 0) s_tile = atomicInc(ticket, nb - 1);
   __syncthreads();
   const uint32_t tile = s_tile;
+  if (tile >= nb) {  // a ticket counter left off zero: count it rather than index past the status words
+    if (t == 0) atomicAdd(&g_scan_spin_errors, 1ull);
+    return;
+  }
   const size_t base = (size_t)tile * SCAN_TILE;
   uint32_t v[SCAN_ITEMS];
   uint32_t acc = 0;
@@ -125,10 +132,10 @@ __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in,
     uint32_t prefix = 0;
     if (tile == 0) {
       if (l == 0)
-        __hip_atomic_store(&status[0], sp_word(epoch, SP_INC, agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&status[0], sp_word(epoch, SP_INC, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       if (l == 0)
-        __hip_atomic_store(&status[tile], sp_word(epoch, SP_AGG, agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&status[tile], sp_word(epoch, SP_AGG, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       int64_t w0 = (int64_t)tile - 1;
       bool bad = false;
       for (;;) {
@@ -138,7 +145,7 @@ __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in,
         for (;;) {
           bool ready = true;
           if (idx >= 0) {
-            word = __hip_atomic_load(&status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            word = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             ready = (uint32_t)(word >> 34) == epoch && ((word >> 32) & 3u) != 0u;
           }
           if (!ballot(!ready)) break;
@@ -161,7 +168,7 @@ __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in,
       }
       if (bad && l == 0) atomicAdd(&g_scan_spin_errors, 1ull);
       if (l == 0)
-        __hip_atomic_store(&status[tile], sp_word(epoch, SP_INC, prefix + agg), __ATOMIC_RELEASE,
+        __hip_atomic_store(&status[tile], sp_word(epoch, SP_INC, prefix + agg), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
     if (l == 0) {
@@ -242,101 +249,221 @@ unsigned long long scan_spin_errors() {
 }
 
 // ------------------------------------------------------------------------------------------------
-// radix sort: per pass, a digit histogram per tile (digit-major), an exclusive scan over it, and a
-// stable scatter that ranks equal digits inside each wave with 8 ballots (match-any).
+// radix sort, one sweep per digit (after Adinets & Merrill, "Onesweep", 2022): one kernel histograms
+// every digit position in a single read of the keys; each pass is then ONE kernel in which a tile ranks
+// its keys (stable, waves own consecutive sub-ranges), publishes its per-digit counts, looks back over
+// its predecessors' per-digit status words for its global offsets, stages the keys in LDS in digit order
+// and writes them out in runs.  Status words are the scan's: (epoch, flag, value) in one relaxed 64-bit
+// word, tiles ordered by a ticket so a tile waits only on tiles already running.
 // ------------------------------------------------------------------------------------------------
 constexpr int RS_ITEMS = 8;
-constexpr int RS_TILE = PRIM_THREADS * RS_ITEMS;
+constexpr int RS_TILE = PRIM_THREADS * RS_ITEMS;  // 2048 keys per tile
+constexpr int RS_MAX_PASSES = 8;
+constexpr int RS_LOOKBACK = 8;  // predecessor status words one look-back step reads at once
 
 // element count of a device-sized sort: min(*n_dev, n) (n = the host bound the launch was sized for)
 __device__ __forceinline__ size_t dev_count(size_t n, const uint32_t *n_dev) {
   return n_dev ? min((size_t)*n_dev, n) : n;
 }
-// a pass whose digit lies above the device-side key width only copies (the ping-pong stays in step)
-__device__ __forceinline__ bool copy_pass(int shift, const int *bits_dev) { return bits_dev && shift >= *bits_dev; }
+// digit positions that carry key bits: all `passes`, or those below the device-side key width
+__device__ __forceinline__ int active_passes(int passes, const int *bits_dev) {
+  return bits_dev ? min(passes, (max(*bits_dev, 0) + 7) / 8) : passes;
+}
 
+__device__ __forceinline__ void lds_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ghist[p * 256 + d] += number of keys whose digit p is d, for every active position p (ghist zeroed)
 template <class K>
-__global__ __launch_bounds__(PRIM_THREADS) void radix_hist_k(const K *__restrict__ keys, size_t n, int shift,
-                                                              uint32_t *__restrict__ hist, unsigned nb,
-                                                              const uint32_t *__restrict__ n_dev,
-                                                              const int *__restrict__ bits_dev) {
-  if (copy_pass(shift, bits_dev)) return;
+__global__ __launch_bounds__(PRIM_THREADS) void radix_hist_all_k(const K *__restrict__ keys, size_t n, int passes,
+                                                                  uint32_t *__restrict__ ghist,
+                                                                  const uint32_t *__restrict__ n_dev,
+                                                                  const int *__restrict__ bits_dev) {
+  __shared__ uint32_t h[RS_MAX_PASSES][256];
+  const int t = threadIdx.x;
+  const int act = active_passes(passes, bits_dev);
   n = dev_count(n, n_dev);
-  __shared__ uint32_t h[PRIM_THREADS / WAVE][256];
-  const int t = threadIdx.x, w = t >> 6;
-#pragma unroll
-  for (int j = 0; j < PRIM_THREADS / WAVE; ++j) h[j][t] = 0;
+  for (int p = 0; p < act; ++p) h[p][t] = 0;
   __syncthreads();
-  const size_t base = (size_t)blockIdx.x * RS_TILE;
-#pragma unroll
-  for (int j = 0; j < RS_ITEMS; ++j) {
-    size_t i = base + (size_t)j * PRIM_THREADS + t;
-    if (i < n) atomicAdd(&h[w][(uint32_t)(keys[i] >> shift) & 255u], 1u);
+  for (size_t i0 = (size_t)blockIdx.x * PRIM_THREADS; i0 < n; i0 += (size_t)gridDim.x * PRIM_THREADS) {
+    const size_t i = i0 + t;
+    const bool ok = i < n;
+    const K k = ok ? keys[i] : (K)0;
+    for (int p = 0; p < act; ++p) {
+      const uint32_t d = (uint32_t)(k >> (8 * p)) & 255u;
+      // a digit shared by the whole wave (the high digits of ordered input) is one add, not 64 conflicts
+      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+      const uint64_t okm = ballot(ok);
+      if (!ballot(ok && d != d0)) {
+        if (lane() == __ffsll((unsigned long long)okm) - 1) atomicAdd(&h[p][d0], (uint32_t)popc(okm));
+      } else if (ok) {
+        atomicAdd(&h[p][d], 1u);
+      }
+    }
   }
   __syncthreads();
-  hist[(size_t)t * nb + blockIdx.x] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
+  for (int p = 0; p < act; ++p)
+    if (h[p][t]) atomicAdd(&ghist[p * 256 + t], h[p][t]);
 }
 
 template <class K>
-__global__ __launch_bounds__(PRIM_THREADS) void radix_scatter_k(const K *__restrict__ kin,
-                                                                 const uint32_t *__restrict__ vin,
-                                                                 K *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                                 size_t n, int shift,
-                                                                 const uint32_t *__restrict__ offs, unsigned nb,
-                                                                 const uint32_t *__restrict__ n_dev,
-                                                                 const int *__restrict__ bits_dev) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wc[PRIM_THREADS / WAVE][256];
-  const int t = threadIdx.x, w = t >> 6;
+__global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                              K *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                              size_t n, int pass, int passes,
+                                                              const uint32_t *__restrict__ ghist,
+                                                              uint64_t *__restrict__ status,
+                                                              uint32_t *__restrict__ ticket, uint32_t nb,
+                                                              uint32_t epoch, const uint32_t *__restrict__ n_dev,
+                                                              const int *__restrict__ bits_dev) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t wcnt[PRIM_THREADS / WAVE][256];  // per-wave digit counters, then per-wave offsets
+  __shared__ uint32_t lstart[256];                     // first slot of digit d in the staged tile
+  __shared__ uint32_t gbase[256];                      // global position of staged slot 0 of digit d's run
+  __shared__ uint32_t sm[PRIM_THREADS / WAVE];
+  __shared__ K skey[RS_TILE];
+  __shared__ uint32_t sval[RS_TILE];
+  const int t = threadIdx.x, w = t >> 6, l = lane();
   n = dev_count(n, n_dev);
-  const size_t base = (size_t)blockIdx.x * RS_TILE;
-  if (copy_pass(shift, bits_dev)) {
-#pragma unroll
-    for (int j = 0; j < RS_ITEMS; ++j) {
-      const size_t i = base + (size_t)j * PRIM_THREADS + t;
-      if (i < n) {
+  const int act = active_passes(passes, bits_dev);
+  if (pass >= act) {
+    // no key bits left: the data stays put, except that one copy lands it in the buffer the remaining
+    // (even or odd) number of pointer swaps ends on; uniform over the launch, so no tile takes a ticket
+    if (pass == act && ((passes - act) & 1)) {
+      for (size_t i = (size_t)blockIdx.x * PRIM_THREADS + t; i < n; i += (size_t)gridDim.x * PRIM_THREADS) {
         kout[i] = kin[i];
         vout[i] = vin[i];
       }
     }
     return;
   }
-  run[t] = offs[(size_t)t * nb + blockIdx.x];
-  for (int j = 0; j < RS_ITEMS; ++j) {
-    const size_t i = base + (size_t)j * PRIM_THREADS + t;
-    const bool valid = i < n;
-    K k = valid ? kin[i] : (K)0;
-    uint32_t v = valid ? vin[i] : 0u;
-    uint32_t d = (uint32_t)(k >> shift) & 255u;
+  if (t == 0) s_tile = atomicInc(ticket, nb - 1);
 #pragma unroll
-    for (int q = 0; q < PRIM_THREADS / WAVE; ++q) wc[q][t] = 0;
-    __syncthreads();
-    uint64_t peers = ballot(valid);
+  for (int q = 0; q < 4; ++q) wcnt[w][l + 64 * q] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  if (tile >= nb) {  // a ticket counter left off zero: count it rather than index past the status words
+    if (t == 0) atomicAdd(&g_scan_spin_errors, 1ull);
+    return;
+  }
+  const size_t base = (size_t)tile * RS_TILE;
+  if (base >= n) return;  // past the device-side count: nothing to rank, and no later tile looks at it
+  const int shift = 8 * pass;
+  // rank: wave w owns keys [base + w*512, base + (w+1)*512), in index order
+  K k[RS_ITEMS];
+  uint32_t v[RS_ITEMS], r[RS_ITEMS];
+  const size_t wb = base + (size_t)w * (RS_ITEMS * WAVE);
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    const size_t i = wb + (size_t)j * WAVE + l;
+    const bool ok = i < n;
+    k[j] = ok ? kin[i] : (K)0;
+    v[j] = ok ? vin[i] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    const bool ok = wb + (size_t)j * WAVE + l < n;
+    const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+    uint64_t peers = ballot(ok);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t bb = ballot(bit);
       peers &= bit ? bb : ~bb;
     }
-    const int rank = popc(peers & lanemask_lt());
-    if (valid && rank == 0) wc[w][d] = (uint32_t)popc(peers);
-    __syncthreads();
-    {
-      uint32_t s = run[t];
-      uint32_t c0 = wc[0][t], c1 = wc[1][t], c2 = wc[2][t], c3 = wc[3][t];
-      wc[0][t] = s; s += c0;
-      wc[1][t] = s; s += c1;
-      wc[2][t] = s; s += c2;
-      wc[3][t] = s; s += c3;
-      run[t] = s;
+    const uint32_t rank = (uint32_t)popc(peers & lanemask_lt());
+    const uint32_t pre = ok ? wcnt[w][d] : 0u;
+    lds_wave_sync();
+    if (ok && rank == 0) wcnt[w][d] = pre + (uint32_t)popc(peers);
+    lds_wave_sync();
+    r[j] = pre + rank;
+  }
+  __syncthreads();
+  // thread t = digit t: per-wave offsets, the tile's count, its first staged slot
+  uint32_t cnt;
+  {
+    const uint32_t c0 = wcnt[0][t], c1 = wcnt[1][t], c2 = wcnt[2][t], c3 = wcnt[3][t];
+    cnt = c0 + c1 + c2 + c3;
+    wcnt[0][t] = 0;
+    wcnt[1][t] = c0;
+    wcnt[2][t] = c0 + c1;
+    wcnt[3][t] = c0 + c1 + c2;
+  }
+  if (tile == 0)
+    __hip_atomic_store(&status[t], sp_word(epoch, SP_INC, cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    __hip_atomic_store(&status[(size_t)tile * 256 + t], sp_word(epoch, SP_AGG, cnt), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t tot;
+  const uint32_t ls = block_excl_scan_256(cnt, tot, sm);
+  const uint32_t gofs = block_excl_scan_256(ghist[pass * 256 + t], tot, sm);
+  // look back over the preceding tiles for the keys of digit t ahead of this tile, RS_LOOKBACK status words
+  // in flight at a time: the walk consumes ready words in order up to the nearest inclusive one
+  uint32_t excl = 0;
+  if (tile > 0) {
+    int64_t j = (int64_t)tile - 1;  // next predecessor to consume
+    uint32_t spins = 0;
+    bool done = false, bad = false;
+    while (!done) {
+      uint64_t wd[RS_LOOKBACK];
+#pragma unroll
+      for (int q = 0; q < RS_LOOKBACK; ++q)
+        wd[q] = j - q >= 0 ? __hip_atomic_load(&status[(size_t)(j - q) * 256 + t], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : sp_word(epoch, SP_INC, 0);  // before tile 0: an inclusive zero
+      int used = 0;
+      bool stop = false;
+#pragma unroll
+      for (int q = 0; q < RS_LOOKBACK; ++q) {
+        if (stop) continue;
+        const uint32_t flag = (uint32_t)(wd[q] >> 32) & 3u;
+        if ((uint32_t)(wd[q] >> 34) != epoch || flag == 0u) {
+          stop = true;
+          continue;
+        }
+        excl += (uint32_t)wd[q];
+        ++used;
+        if (flag == SP_INC) stop = done = true;
+      }
+      j -= used;
+      if (!done && used == 0) {
+        if (++spins >= SP_SPIN_LIMIT) {
+          bad = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
     }
-    __syncthreads();
-    if (valid) {
-      const uint32_t pos = wc[w][d] + (uint32_t)rank;
-      kout[pos] = k;
-      vout[pos] = v;
+    if (bad) atomicAdd(&g_scan_spin_errors, 1ull);
+    __hip_atomic_store(&status[(size_t)tile * 256 + t], sp_word(epoch, SP_INC, excl + cnt), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  lstart[t] = ls;
+  gbase[t] = gofs + excl - ls;  // mod 2^32: slot s of digit t's run lands at gbase[t] + s
+  __syncthreads();
+  // stage in digit order, then write runs
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    if (wb + (size_t)j * WAVE + l < n) {
+      const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+      const uint32_t s = lstart[d] + wcnt[w][d] + r[j];
+      skey[s] = k[j];
+      sval[s] = v[j];
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  const uint32_t m = (uint32_t)min((size_t)RS_TILE, n - base);
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    const uint32_t s = (uint32_t)(j * PRIM_THREADS + t);
+    if (s < m) {
+      const K kk = skey[s];
+      const uint32_t pos = gbase[(uint32_t)(kk >> shift) & 255u] + s;
+      kout[pos] = kk;
+      vout[pos] = sval[s];
+    }
   }
 }
 
@@ -345,15 +472,22 @@ static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, siz
                            const uint32_t *n_dev, const int *bits_dev) {
   if (n <= 1 || bits <= 0) return RVP_OK;
   RVP_CHECK(n < 0xffffffffull, RVP_E_UNSUPPORTED, "radix sort: more than 2^32-1 elements");
+  const int passes = std::min<int>((bits + 7) / 8, (int)sizeof(K));
   const unsigned nb = ceil_div(n, RS_TILE);
-  Tmp hist;
-  RVP_TRY(hist.alloc((size_t)256 * nb * sizeof(uint32_t), s));
-  for (int shift = 0; shift < bits; shift += 8) {
-    RVP_LAUNCH("radix_hist_k", s, radix_hist_k<K><<<nb, PRIM_THREADS, 0, s>>>(*keys, n, shift, hist.as<uint32_t>(), nb,
-                                                                             n_dev, bits_dev));
-    RVP_TRY(exclusive_scan_u32(hist.as<uint32_t>(), hist.as<uint32_t>(), (size_t)256 * nb, nullptr, s));
-    RVP_LAUNCH("radix_scatter_k", s, radix_scatter_k<K><<<nb, PRIM_THREADS, 0, s>>>(
-                                         *keys, *vals, *kt, *vt, n, shift, hist.as<uint32_t>(), nb, n_dev, bits_dev));
+  Tmp gh;
+  RVP_TRY(gh.alloc((size_t)passes * 256 * sizeof(uint32_t), s));
+  RVP_HIP(hipMemsetAsync(gh.p, 0, (size_t)passes * 256 * sizeof(uint32_t), s));
+  const unsigned hb = std::min(nb * 2, 2048u);
+  RVP_LAUNCH("radix_hist_all_k", s, radix_hist_all_k<K><<<hb, PRIM_THREADS, 0, s>>>(*keys, n, passes,
+                                                                                    gh.as<uint32_t>(), n_dev,
+                                                                                    bits_dev));
+  for (int p = 0; p < passes; ++p) {
+    uint64_t *status = nullptr;
+    uint32_t *ticket = nullptr, epoch = 0;
+    RVP_TRY(scan_state(s, (size_t)nb * 256, &status, &ticket, &epoch));
+    RVP_LAUNCH("radix_pass_k", s, radix_pass_k<K><<<nb, PRIM_THREADS, 0, s>>>(*keys, *vals, *kt, *vt, n, p, passes,
+                                                                            gh.as<uint32_t>(), status, ticket, nb,
+                                                                            epoch, n_dev, bits_dev));
     std::swap(*keys, *kt);
     std::swap(*vals, *vt);
   }

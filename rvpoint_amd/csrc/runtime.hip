@@ -300,6 +300,29 @@ int rvp_exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t
   return exclusive_scan_u32(in, out, n, total_dev, S(s));
 }
 
+int rvp_sort_pairs_u32(uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp, size_t n, int bits,
+                       const uint32_t *n_dev, const int *bits_dev, int *in_tmp, rvp_stream_t s) {
+  RVP_CHECK(in_tmp && ((keys && vals && keys_tmp && vals_tmp) || n == 0), RVP_E_ARG, "null argument");
+  RVP_CHECK(bits >= 0 && bits <= 32, RVP_E_ARG, "bits must be in [0, 32]");
+  RVP_TRY(pool_init());
+  uint32_t *k = keys, *v = vals, *kt = keys_tmp, *vt = vals_tmp;
+  RVP_TRY(radix_sort_pairs_u32(&k, &v, &kt, &vt, n, bits, S(s), n_dev, bits_dev));
+  *in_tmp = k != keys;
+  return RVP_OK;
+}
+
+int rvp_sort_pairs_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_tmp, uint32_t *vals_tmp, size_t n, int bits,
+                       const uint32_t *n_dev, const int *bits_dev, int *in_tmp, rvp_stream_t s) {
+  RVP_CHECK(in_tmp && ((keys && vals && keys_tmp && vals_tmp) || n == 0), RVP_E_ARG, "null argument");
+  RVP_CHECK(bits >= 0 && bits <= 64, RVP_E_ARG, "bits must be in [0, 64]");
+  RVP_TRY(pool_init());
+  uint64_t *k = keys, *kt = keys_tmp;
+  uint32_t *v = vals, *vt = vals_tmp;
+  RVP_TRY(radix_sort_pairs_u64(&k, &v, &kt, &vt, n, bits, S(s), n_dev, bits_dev));
+  *in_tmp = k != keys;
+  return RVP_OK;
+}
+
 int rvp_scan_wait_errors(unsigned long long *count) {
   RVP_CHECK(count, RVP_E_ARG, "null argument");
   RVP_HIP(hipDeviceSynchronize());

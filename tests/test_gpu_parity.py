@@ -344,3 +344,56 @@ def test_single_pass_scan(n):
     errs = C.c_ulonglong()
     _lib.call("rvp_scan_wait_errors", C.byref(errs))
     assert errs.value == 0
+
+
+def _sort_case(dtype, n, bits, n_dev=None, bits_dev=None, seed=0, ordered=False):
+    import ctypes as C
+
+    from rvpoint_amd import _lib
+    rng = np.random.default_rng(seed)
+    top = (1 << 64) - 1 if dtype == np.uint64 else (1 << 32) - 1
+    keys = rng.integers(0, top, n, dtype=dtype, endpoint=True)
+    if ordered:  # long runs of equal high digits, as cell keys of scan-ordered points have
+        keys = np.sort(keys >> dtype(24)).astype(dtype) | (keys & dtype(0xff))
+    vals = np.arange(n, dtype=np.uint32)
+    dk, dv = R.DeviceArray.from_numpy(keys), R.DeviceArray.from_numpy(vals)
+    tk, tv = R.DeviceArray(n, dtype), R.DeviceArray(n, np.uint32)
+    nd = R.DeviceArray.from_numpy(np.array([n_dev], np.uint32)) if n_dev is not None else None
+    bd = R.DeviceArray.from_numpy(np.array([bits_dev], np.int32)) if bits_dev is not None else None
+    in_tmp = C.c_int(-1)
+    name = "rvp_sort_pairs_u64" if dtype == np.uint64 else "rvp_sort_pairs_u32"
+    print(f"sort case {name} n={n} bits={bits} n_dev={n_dev} bits_dev={bits_dev} ordered={ordered}", flush=True)
+    _lib.call(name, dk.ptr, dv.ptr, tk.ptr, tv.ptr, n, bits, nd.ptr if nd else None, bd.ptr if bd else None,
+              C.byref(in_tmp), None)
+    ok, ov = (tk, tv) if in_tmp.value else (dk, dv)
+    m = n if n_dev is None else min(n_dev, n)
+    eff = bits if bits_dev is None else min(bits, bits_dev)
+    sort_bits = min(8 * ((eff + 7) // 8), 8 * np.dtype(dtype).itemsize)
+    mask = dtype(top) if sort_bits >= 8 * np.dtype(dtype).itemsize else dtype((1 << sort_bits) - 1)
+    order = np.argsort(keys[:m] & mask, kind="stable")
+    assert np.array_equal(ov.numpy()[:m], vals[:m][order])
+    assert np.array_equal(ok.numpy()[:m], keys[:m][order])
+
+
+@pytest.mark.parametrize("n", [2, 1000, 2047, 2048, 2049, 100003, 1 << 20])
+@pytest.mark.parametrize("bits", [8, 17, 32])
+def test_radix_sort_u32(n, bits):
+    """The one-sweep radix sort: keys and values against numpy's stable argsort on the sorted digits."""
+    _sort_case(np.uint32, n, bits, seed=n + bits)
+
+
+@pytest.mark.parametrize("n_dev,bits_dev", [(None, None), (None, 0), (None, 20), (None, 40), (None, 64),
+                                            (333333, None), (333333, 33), (0, None), (1, 64)])
+def test_radix_sort_u64_device_sized(n_dev, bits_dev):
+    _sort_case(np.uint64, 1000003, 64, n_dev, bits_dev, seed=7)
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.uint64])
+def test_radix_sort_ordered_input_and_wait_errors(dtype):
+    import ctypes as C
+
+    from rvpoint_amd import _lib
+    _sort_case(dtype, 1 << 21, 8 * np.dtype(dtype).itemsize, seed=3, ordered=True)
+    errs = C.c_ulonglong()
+    _lib.call("rvp_scan_wait_errors", C.byref(errs))
+    assert errs.value == 0
