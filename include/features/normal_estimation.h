@@ -9,6 +9,8 @@
 namespace rvpoint {
 
 class SpatialHash;
+class Octree;
+class PointerOctree;
 
 // k+1 nearest (incl. the point itself) per point; normals flipped toward (vp_x, vp_y, vp_z).
 void normal_estimation_sc(const PointXYZ *in, std::size_t n, float *nx, float *ny, float *nz, int k, float radius,
@@ -18,6 +20,15 @@ void normal_estimation_sc(const PointXYZ *in, std::size_t n, float *nx, float *n
 void normal_estimation_rvv(const PointCloudSoA &in, const SpatialHash &hash, float *nx, float *ny, float *nz,
                            int k, float radius, float vp_x = 0, float vp_y = 0, float vp_z = 0,
                            int eigen_iters = 4);
+
+// The same over the tree facades (octree.h, pointer_octree.h), and self-contained (a grid of edge `radius`).
+void normal_estimation_rvv(const PointCloudSoA &in, const Octree &octree, float *nx, float *ny, float *nz, int k,
+                           float radius, float vp_x = 0, float vp_y = 0, float vp_z = 0, int eigen_iters = 4);
+void normal_estimation_rvv(const PointCloudSoA &in, const PointerOctree &octree, float *nx, float *ny, float *nz,
+                           int k, float radius, float vp_x = 0, float vp_y = 0, float vp_z = 0,
+                           int eigen_iters = 4);
+void normal_estimation_rvv(const PointCloudSoA &in, float *nx, float *ny, float *nz, int k, float radius,
+                           float vp_x = 0, float vp_y = 0, float vp_z = 0, int eigen_iters = 4);
 
 // Extension: the same with the (d2, idx)-sorted neighbour list capped at max_nn.
 void normal_estimation_rvv_capped(const PointCloudSoA &in, const SpatialHash &hash, float *nx, float *ny,

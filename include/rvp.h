@@ -197,10 +197,19 @@ int rvp_plane_counts(const float *x, const float *y, const float *z, size_t n, c
                      float dist_thresh, int32_t *counts, rvp_stream_t s);
 
 /* ---- X1: extract_plane_inliers_outliers_rvv (ransac_plane.cpp:374-382, scalar order) -------------
- * Stable split into inliers / outliers (AoS, device); counts host.  Either output may be NULL. */
+ * Stable split into inliers / outliers (AoS, device); counts host.  Either output may be NULL.  One pass:
+ * each tile's inlier rank comes from a decoupled look-back, its outlier rank is its index minus that. */
 int rvp_plane_extract(const float *x, const float *y, const float *z, size_t n, const float *model_host,
                       float dist_thresh, float *inliers_aos, size_t *n_in, float *outliers_aos,
                       size_t *n_out, rvp_stream_t s);
+
+/* ---- R1 + X1 fused: the stage split of pipeline_export.cpp:457-498 --------------------------------
+ * rvp_ransac_plane, then (only when a model was found, *inliers > 0) the one-pass split of the same points by
+ * that model into inliers / outliers (AoS, device, index order; either may be NULL); counts host. */
+int rvp_ransac_plane_extract(const float *x, const float *y, const float *z, size_t n, float dist_thresh,
+                             int max_iters, float collinear_thresh, float probability, float *model, int *inliers,
+                             int *iterations, float *inliers_aos, size_t *n_in, float *outliers_aos,
+                             size_t *n_out, rvp_stream_t s);
 
 /* ---- batches: many clouds, one launch per stage ---------------------------------------------------
  * The clouds lie back to back in x/y/z; offsets[nclouds+1] (device, uint32) delimit them; n_bound >= the

@@ -8,6 +8,8 @@
 #include "io/simple_pcd_loader.h"
 #include "filters/statistical_outlier_removal.h"
 #include "filters/voxel_grid.h"
+#include "search/octree.h"
+#include "search/pointer_octree.h"
 #include "search/radius_search.h"
 #include "search/spatial_hashing.h"
 #include "segmentation/euclidean_clustering.h"

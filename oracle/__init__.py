@@ -116,6 +116,13 @@ def ref():
         R.ref_sor_spatial_hash.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, _f32p, C.c_int, C.c_float,
                                            C.c_float]
         R.ref_sor_spatial_hash.restype = _sz
+        R.ref_sor_octree.argtypes = [_f32p, _f32p, _f32p, _sz, _f32p, C.c_int, C.c_float, C.c_float, C.c_int]
+        R.ref_sor_octree.restype = _sz
+        R.ref_octree_radius.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, C.c_float, C.c_float, C.c_float,
+                                        C.c_int, np.ctypeslib.ndpointer(np.int32, flags="C"), _f32p, _sz]
+        R.ref_octree_radius.restype = _sz
+        R.ref_normal_octree.argtypes = [_f32p, _f32p, _f32p, _sz, _f32p, _f32p, _f32p, C.c_int, C.c_float,
+                                        C.c_float, C.c_float, C.c_float, C.c_int]
         R.ref_voxel_sc.argtypes = [_f32p, _sz, _f32p, C.c_float]
         R.ref_voxel_sc.restype = _sz
         R.ref_voxel_v2.argtypes = [_f32p, _f32p, _f32p, _sz, _f32p, C.c_float]

@@ -4,6 +4,8 @@
 #include "core/point_types.h"
 #include "core/rvv_common.h"
 #include "search/spatial_hashing.h"
+#include "search/octree.h"
+#include "search/pointer_octree.h"
 #include "search/radius_search.h"
 #include "filters/voxel_grid.h"
 #include "filters/statistical_outlier_removal.h"
@@ -75,6 +77,54 @@ size_t ref_sor_spatial_hash(const float *x, const float *y, const float *z, size
   h.setInputCloud(c, cell);
   h.build();
   return sor_spatial_hash(c, h, reinterpret_cast<PointXYZ *>(out), k, alpha, search_radius);
+}
+
+size_t ref_sor_octree(const float *x, const float *y, const float *z, size_t n, float *out, int k, float alpha,
+                      float search_radius, int pointer_tree) {
+  PointCloudSoA c = soa(x, y, z, n);
+  if (pointer_tree) {
+    PointerOctree t;
+    t.setInputCloud(c);
+    t.build();
+    return sor_pointer_octree(c, t, reinterpret_cast<PointXYZ *>(out), k, alpha, search_radius);
+  }
+  Octree t;
+  t.setInputCloud(c);
+  t.build();
+  return sor_octree(c, t, reinterpret_cast<PointXYZ *>(out), k, alpha, search_radius);
+}
+
+// one query's neighbours from the reference's Octree (pointer_tree 0) or PointerOctree (1), traversal order
+size_t ref_octree_radius(const float *x, const float *y, const float *z, size_t n, float qx, float qy, float qz,
+                         float r, int pointer_tree, int *idx, float *d2, size_t cap) {
+  PointCloudSoA c = soa(x, y, z, n);
+  std::vector<int> vi;
+  std::vector<float> vd;
+  if (pointer_tree) {
+    PointerOctree t;
+    t.setInputCloud(c);
+    t.build();
+    t.radiusSearch(PointXYZ{qx, qy, qz}, r, vi, vd);
+  } else {
+    Octree t;
+    t.setInputCloud(c);
+    t.build();
+    t.radiusSearch(PointXYZ{qx, qy, qz}, r, vi, vd);
+  }
+  for (size_t i = 0; i < vi.size() && i < cap; ++i) {
+    idx[i] = vi[i];
+    d2[i] = vd[i];
+  }
+  return vi.size();
+}
+
+void ref_normal_octree(const float *x, const float *y, const float *z, size_t n, float *nx, float *ny, float *nz,
+                       int k, float radius, float vpx, float vpy, float vpz, int iters) {
+  PointCloudSoA c = soa(x, y, z, n);
+  Octree t;
+  t.setInputCloud(c);
+  t.build();
+  normal_estimation_rvv(c, t, nx, ny, nz, k, radius, vpx, vpy, vpz, iters);
 }
 
 size_t ref_voxel_sc(const float *aos, size_t n, float *out, float leaf) {

@@ -333,6 +333,25 @@ def ransac_plane_rvv(cloud_soa, dist_thresh, max_iters, collinear_thresh=1e-6, p
     return best, model
 
 
+def ransac_plane_extract(cloud_soa, dist_thresh, max_iters, collinear_thresh=1e-6, probability=0.99):
+    """RANSAC then the inlier / outlier split by the best model in one call, the stage split of
+    pipeline_export.cpp:457-498: (inliers, model[4], iterations, inliers AoS, outliers AoS).  The split runs
+    only when a model was found."""
+    c = _soa(cloud_soa)
+    empty = np.zeros((0, 3), np.float32)
+    if c.n < 3:
+        return 0, np.zeros(4, np.float32), 0, empty, empty
+    dc = DeviceCloud.from_soa(c.x, c.y, c.z)
+    a = DeviceArray((c.n, 3), np.float32)
+    b = DeviceArray((c.n, 3), np.float32)
+    model = (C.c_float * 4)()
+    best, iters = C.c_int(), C.c_int()
+    ni, no = C.c_size_t(), C.c_size_t()
+    call("rvp_ransac_plane_extract", *dc.ptrs(), c.n, float(dist_thresh), int(max_iters), float(collinear_thresh),
+         float(probability), model, C.byref(best), C.byref(iters), a.ptr, C.byref(ni), b.ptr, C.byref(no), None)
+    return best.value, np.array(model, np.float32), iters.value, a.numpy()[:ni.value], b.numpy()[:no.value]
+
+
 def extract_plane_inliers_outliers_rvv(cloud_soa, model, dist_thresh):
     """extract_plane_inliers_outliers_rvv (ransac_plane.cpp:374-382): (inliers AoS, outliers AoS)."""
     c = _soa(cloud_soa)

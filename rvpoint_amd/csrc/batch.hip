@@ -227,9 +227,15 @@ __global__ void seg_geom_k(const float *__restrict__ mm, const uint32_t *__restr
   ncells[c] = (uint32_t)d.g[0] * (uint32_t)d.g[1] * (uint32_t)d.g[2];
 }
 
-__global__ void seg_toff_k(const uint32_t *__restrict__ toff, int ncl, SegDesc *__restrict__ desc) {
+// also the width of the largest cell key, total - 1, for the key sort
+__global__ void seg_toff_k(const uint32_t *__restrict__ toff, int ncl, SegDesc *__restrict__ desc,
+                           int *__restrict__ key_bits) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < ncl) desc[c].toff = toff[c];
+  if (c == 0) {
+    const uint32_t total = toff[ncl];
+    *key_bits = total > 1 ? 32 - __clz((int)(total - 1)) : 0;
+  }
 }
 
 __global__ void seg_keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
@@ -355,7 +361,10 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_LAUNCH("seg_geom_k", s, seg_geom_k<<<cb, 64, 0, s>>>(mm.as<float>(), off, ncl, cells, per_point,
                                                            desc.as<SegDesc>(), ncells.as<uint32_t>()));
   RVP_TRY(exclusive_scan_u32(ncells.as<uint32_t>(), toff.as<uint32_t>(), ncl, toff.as<uint32_t>() + ncl, s));
-  RVP_LAUNCH("seg_toff_k", s, seg_toff_k<<<cb, 64, 0, s>>>(toff.as<uint32_t>(), ncl, desc.as<SegDesc>()));
+  Tmp key_bits;
+  RVP_TRY(key_bits.alloc(sizeof(int), s));
+  RVP_LAUNCH("seg_toff_k", s, seg_toff_k<<<cb, 64, 0, s>>>(toff.as<uint32_t>(), ncl, desc.as<SegDesc>(),
+                                                           key_bits.as<int>()));
   // the table holds at most sum(max(SEG_MIN_CELLS, per_point n_c)) cells
   const uint64_t table = (uint64_t)std::ceil(per_point * (double)n_bound) + (uint64_t)SEG_MIN_CELLS * (uint64_t)ncl;
   RVP_CHECK(table < 0xfff00000ull, RVP_E_UNSUPPORTED, "batch cell table too large");
@@ -369,7 +378,7 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_LAUNCH("seg_keys_k", s, seg_keys_k<<<nb, 256, 0, s>>>(x, y, z, off, ncl, desc.as<SegDesc>(), n32,
                                                             (uint32_t)table, keys.as<uint32_t>(), perm.as<uint32_t>()));
   uint32_t *kp = keys.as<uint32_t>(), *vp = perm.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
-  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n_bound, bits_for(table), s, off + ncl));
+  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n_bound, bits_for(table), s, off + ncl, key_bits.as<int>()));
   if (kp != keys.as<uint32_t>()) {
     keys.swap(kt);
     perm.swap(vt);

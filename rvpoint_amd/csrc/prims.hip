@@ -99,6 +99,66 @@ __device__ __forceinline__ uint64_t sp_word(uint32_t epoch, uint32_t flag, uint3
   return ((uint64_t)((epoch << 2) | flag) << 32) | v;
 }
 
+// Wave 0 of a tile: publish the tile's aggregate, look back for the exclusive prefix of the tiles before it,
+// publish the inclusive value; returns the prefix on every lane of the wave.
+__device__ uint32_t tile_lookback(uint64_t *__restrict__ status, uint32_t tile, uint32_t epoch, uint32_t agg) {
+  const int l = lane();
+  uint32_t prefix = 0;
+  if (tile == 0) {
+    if (l == 0)
+      __hip_atomic_store(&status[0], sp_word(epoch, SP_INC, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  if (l == 0)
+    __hip_atomic_store(&status[tile], sp_word(epoch, SP_AGG, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int64_t w0 = (int64_t)tile - 1;
+  bool bad = false;
+  for (;;) {
+    const int64_t idx = w0 - l;
+    uint64_t word = sp_word(epoch, SP_INC, 0);  // before tile 0: an inclusive zero
+    uint32_t spins = 0;
+    for (;;) {
+      bool ready = true;
+      if (idx >= 0) {
+        word = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ready = (uint32_t)(word >> 34) == epoch && ((word >> 32) & 3u) != 0u;
+      }
+      if (!ballot(!ready)) break;
+      if (++spins >= SP_SPIN_LIMIT) {
+        bad = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (bad) break;
+    const uint64_t inc = ballot(((word >> 32) & 3u) == SP_INC);
+    const uint32_t val = (uint32_t)word;
+    if (inc) {
+      const int first = __ffsll((unsigned long long)inc) - 1;  // nearest inclusive predecessor
+      prefix += wave_sum(l <= first ? val : 0u);
+      break;
+    }
+    prefix += wave_sum(val);
+    w0 -= WAVE;
+  }
+  if (bad && l == 0) atomicAdd(&g_scan_spin_errors, 1ull);
+  if (l == 0)
+    __hip_atomic_store(&status[tile], sp_word(epoch, SP_INC, prefix + agg), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  return prefix;
+}
+
+// a tile index from the ticket counter; nb on a counter left off zero (counted, the tile does nothing)
+__device__ __forceinline__ uint32_t take_tile(uint32_t *ticket, uint32_t nb, uint32_t *s_tile) {
+  if (threadIdx.x == 0) {
+    const uint32_t t = atomicInc(ticket, nb - 1);
+    if (t >= nb) atomicAdd(&g_scan_spin_errors, 1ull);
+    *s_tile = t < nb ? t : nb;
+  }
+  __syncthreads();
+  return *s_tile;
+}
+
 // (in and out may be the same array: a tile reads all its inputs before it writes)
 __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in, uint32_t *out, size_t n,
                                                               uint64_t *__restrict__ status,
@@ -107,13 +167,8 @@ __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in,
   __shared__ uint32_t sm[PRIM_THREADS / WAVE];
   __shared__ uint32_t s_tile, s_prefix;
   const int t = threadIdx.x;
-  if (t == 0) s_tile = atomicInc(ticket, nb - 1);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  if (tile >= nb) {  // a ticket counter left off zero: count it rather than index past the status words
-    if (t == 0) atomicAdd(&g_scan_spin_errors, 1ull);
-    return;
-  }
+  const uint32_t tile = take_tile(ticket, nb, &s_tile);
+  if (tile >= nb) return;
   const size_t base = (size_t)tile * SCAN_TILE;
   uint32_t v[SCAN_ITEMS];
   uint32_t acc = 0;
@@ -128,50 +183,8 @@ __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in,
   __syncthreads();
   const uint32_t agg = sm[0] + sm[1] + sm[2] + sm[3];
   if (t < WAVE) {
-    const int l = t;
-    uint32_t prefix = 0;
-    if (tile == 0) {
-      if (l == 0)
-        __hip_atomic_store(&status[0], sp_word(epoch, SP_INC, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      if (l == 0)
-        __hip_atomic_store(&status[tile], sp_word(epoch, SP_AGG, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int64_t w0 = (int64_t)tile - 1;
-      bool bad = false;
-      for (;;) {
-        const int64_t idx = w0 - l;
-        uint64_t word = sp_word(epoch, SP_INC, 0);  // before tile 0: an inclusive zero
-        uint32_t spins = 0;
-        for (;;) {
-          bool ready = true;
-          if (idx >= 0) {
-            word = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ready = (uint32_t)(word >> 34) == epoch && ((word >> 32) & 3u) != 0u;
-          }
-          if (!ballot(!ready)) break;
-          if (++spins >= SP_SPIN_LIMIT) {
-            bad = true;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
-        if (bad) break;
-        const uint64_t inc = ballot(((word >> 32) & 3u) == SP_INC);
-        const uint32_t val = (uint32_t)word;
-        if (inc) {
-          const int first = __ffsll((unsigned long long)inc) - 1;  // nearest inclusive predecessor
-          prefix += wave_sum(l <= first ? val : 0u);
-          break;
-        }
-        prefix += wave_sum(val);
-        w0 -= WAVE;
-      }
-      if (bad && l == 0) atomicAdd(&g_scan_spin_errors, 1ull);
-      if (l == 0)
-        __hip_atomic_store(&status[tile], sp_word(epoch, SP_INC, prefix + agg), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (l == 0) {
+    const uint32_t prefix = tile_lookback(status, tile, epoch, agg);
+    if (t == 0) {
       s_prefix = prefix;
       if (total && tile == nb - 1) *total = prefix + agg;
     }
@@ -184,6 +197,76 @@ __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in,
     uint32_t bt;
     const uint32_t ex = block_excl_scan_256(v[j], bt, sm);
     if (i < n) out[i] = carry + ex;
+    carry += bt;
+  }
+}
+
+// X1 split in one pass (ransac_plane.cpp:374-382): dist = ((a*x + b*y) + c*z) + d, inlier iff
+// dist >= -t && dist <= t; inliers and outliers each compacted to AoS in index order.  A tile's inlier
+// rank comes from the look-back; its outlier rank is its index minus that.
+__global__ __launch_bounds__(PRIM_THREADS) void plane_split_k(const float *__restrict__ x,
+                                                               const float *__restrict__ y,
+                                                               const float *__restrict__ z, size_t n, float a,
+                                                               float b, float c, float d, float th,
+                                                               uint64_t *__restrict__ status,
+                                                               uint32_t *__restrict__ ticket, uint32_t nb,
+                                                               uint32_t epoch, float *__restrict__ in_aos,
+                                                               float *__restrict__ out_aos,
+                                                               uint32_t *__restrict__ counts) {
+  __shared__ uint32_t sm[PRIM_THREADS / WAVE];
+  __shared__ uint32_t s_tile, s_prefix;
+  const int t = threadIdx.x;
+  const uint32_t tile = take_tile(ticket, nb, &s_tile);
+  if (tile >= nb) return;
+  const size_t base = (size_t)tile * SCAN_TILE;
+  float px[SCAN_ITEMS], py[SCAN_ITEMS], pz[SCAN_ITEMS];
+  uint32_t f[SCAN_ITEMS];
+  uint32_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + (size_t)j * PRIM_THREADS + t;
+    f[j] = 0u;
+    px[j] = py[j] = pz[j] = 0.0f;
+    if (i < n) {
+      px[j] = x[i];
+      py[j] = y[i];
+      pz[j] = z[i];
+      const float dist =
+          __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(a, px[j]), __fmul_rn(b, py[j])), __fmul_rn(c, pz[j])), d);
+      f[j] = (dist >= -th && dist <= th) ? 1u : 0u;
+    }
+    acc += f[j];
+  }
+  acc = wave_sum(acc);
+  if (lane() == 0) sm[t >> 6] = acc;
+  __syncthreads();
+  const uint32_t agg = sm[0] + sm[1] + sm[2] + sm[3];
+  if (t < WAVE) {
+    const uint32_t prefix = tile_lookback(status, tile, epoch, agg);
+    if (t == 0) {
+      s_prefix = prefix;
+      if (tile == nb - 1) {
+        counts[0] = prefix + agg;
+        counts[1] = (uint32_t)n - (prefix + agg);
+      }
+    }
+  }
+  __syncthreads();
+  uint32_t carry = s_prefix;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + (size_t)j * PRIM_THREADS + t;
+    uint32_t bt;
+    const uint32_t r = carry + block_excl_scan_256(f[j], bt, sm);
+    if (i < n) {
+      float *o = f[j] ? in_aos : out_aos;
+      const size_t q = f[j] ? (size_t)r : i - (size_t)r;
+      if (o) {
+        o[3 * q] = px[j];
+        o[3 * q + 1] = py[j];
+        o[3 * q + 2] = pz[j];
+      }
+    }
     carry += bt;
   }
 }
@@ -238,6 +321,23 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *to
                                                                                   (uint32_t)nb, epoch, total_dev));
   return RVP_OK;
 }
+int plane_split_aos(const float *x, const float *y, const float *z, size_t n, const float *m, float t, float *in_aos,
+                    float *out_aos, uint32_t *counts_dev, hipStream_t s) {
+  if (n == 0) {
+    RVP_HIP(hipMemsetAsync(counts_dev, 0, 2 * sizeof(uint32_t), s));
+    return RVP_OK;
+  }
+  const size_t nb = ceil_div(n, SCAN_TILE);
+  RVP_CHECK(n < 0xffffffffull, RVP_E_UNSUPPORTED, "plane split: more than 2^32-1 points");
+  uint64_t *status = nullptr;
+  uint32_t *ticket = nullptr, epoch = 0;
+  RVP_TRY(scan_state(s, nb, &status, &ticket, &epoch));
+  RVP_LAUNCH("plane_split_k", s, plane_split_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(
+                                     x, y, z, n, m[0], m[1], m[2], m[3], t, status, ticket, (uint32_t)nb, epoch,
+                                     in_aos, out_aos, counts_dev));
+  return RVP_OK;
+}
+
 int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *out, size_t n, uint64_t *total_dev, hipStream_t s) {
   return exclusive_scan_impl<uint64_t>(in, out, n, total_dev, s);
 }
@@ -276,10 +376,11 @@ __device__ __forceinline__ void lds_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// ghist[p * 256 + d] += number of keys whose digit p is d, for every active position p (ghist zeroed)
+// Per-block digit histograms of every active position, stored (not added) to part[block][p][256]: global
+// atomics on 256 * passes shared counters serialise across the blocks.
 template <class K>
 __global__ __launch_bounds__(PRIM_THREADS) void radix_hist_all_k(const K *__restrict__ keys, size_t n, int passes,
-                                                                  uint32_t *__restrict__ ghist,
+                                                                  uint32_t *__restrict__ part,
                                                                   const uint32_t *__restrict__ n_dev,
                                                                   const int *__restrict__ bits_dev) {
   __shared__ uint32_t h[RS_MAX_PASSES][256];
@@ -305,8 +406,24 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_hist_all_k(const K *__rest
     }
   }
   __syncthreads();
-  for (int p = 0; p < act; ++p)
-    if (h[p][t]) atomicAdd(&ghist[p * 256 + t], h[p][t]);
+  for (int p = 0; p < act; ++p) part[((size_t)blockIdx.x * RS_MAX_PASSES + p) * 256 + t] = h[p][t];
+}
+
+// ghist[p][d] = sum over the hb partial histograms; block (p, q) sums blocks [q*RS_SUM, (q+1)*RS_SUM)
+constexpr int RS_SUM = 16;
+__global__ __launch_bounds__(PRIM_THREADS) void radix_hist_sum_k(const uint32_t *__restrict__ part, unsigned hb,
+                                                                  int passes, uint32_t *__restrict__ ghist,
+                                                                  const int *__restrict__ bits_dev) {
+  const int p = blockIdx.x % passes, q = blockIdx.x / passes, t = threadIdx.x;
+  if (p >= active_passes(passes, bits_dev)) return;
+  const unsigned b0 = (unsigned)q * RS_SUM, b1 = min(hb, b0 + RS_SUM);
+  uint32_t v[RS_SUM];
+#pragma unroll
+  for (int j = 0; j < RS_SUM; ++j) v[j] = b0 + j < b1 ? part[((size_t)(b0 + j) * RS_MAX_PASSES + p) * 256 + t] : 0u;
+  uint32_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < RS_SUM; ++j) acc += v[j];
+  if (acc) atomicAdd(&ghist[p * 256 + t], acc);
 }
 
 template <class K>
@@ -339,15 +456,11 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict
     }
     return;
   }
-  if (t == 0) s_tile = atomicInc(ticket, nb - 1);
+  const uint32_t gcount = ghist[pass * 256 + t];  // issued before the ticket round trip
 #pragma unroll
   for (int q = 0; q < 4; ++q) wcnt[w][l + 64 * q] = 0;
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  if (tile >= nb) {  // a ticket counter left off zero: count it rather than index past the status words
-    if (t == 0) atomicAdd(&g_scan_spin_errors, 1ull);
-    return;
-  }
+  const uint32_t tile = take_tile(ticket, nb, &s_tile);
+  if (tile >= nb) return;
   const size_t base = (size_t)tile * RS_TILE;
   if (base >= n) return;  // past the device-side count: nothing to rank, and no later tile looks at it
   const int shift = 8 * pass;
@@ -398,7 +511,7 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict
                        __HIP_MEMORY_SCOPE_AGENT);
   uint32_t tot;
   const uint32_t ls = block_excl_scan_256(cnt, tot, sm);
-  const uint32_t gofs = block_excl_scan_256(ghist[pass * 256 + t], tot, sm);
+  const uint32_t gofs = block_excl_scan_256(gcount, tot, sm);
   // look back over the preceding tiles for the keys of digit t ahead of this tile, RS_LOOKBACK status words
   // in flight at a time: the walk consumes ready words in order up to the nearest inclusive one
   uint32_t excl = 0;
@@ -474,13 +587,16 @@ static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, siz
   RVP_CHECK(n < 0xffffffffull, RVP_E_UNSUPPORTED, "radix sort: more than 2^32-1 elements");
   const int passes = std::min<int>((bits + 7) / 8, (int)sizeof(K));
   const unsigned nb = ceil_div(n, RS_TILE);
-  Tmp gh;
+  const unsigned hb = std::min(nb * 2, 1024u);
+  Tmp gh, part;
   RVP_TRY(gh.alloc((size_t)passes * 256 * sizeof(uint32_t), s));
+  RVP_TRY(part.alloc((size_t)hb * RS_MAX_PASSES * 256 * sizeof(uint32_t), s));
   RVP_HIP(hipMemsetAsync(gh.p, 0, (size_t)passes * 256 * sizeof(uint32_t), s));
-  const unsigned hb = std::min(nb * 2, 2048u);
   RVP_LAUNCH("radix_hist_all_k", s, radix_hist_all_k<K><<<hb, PRIM_THREADS, 0, s>>>(*keys, n, passes,
-                                                                                    gh.as<uint32_t>(), n_dev,
+                                                                                    part.as<uint32_t>(), n_dev,
                                                                                     bits_dev));
+  RVP_LAUNCH("radix_hist_sum_k", s, radix_hist_sum_k<<<(unsigned)passes * ceil_div(hb, RS_SUM), PRIM_THREADS, 0, s>>>(
+                                        part.as<uint32_t>(), hb, passes, gh.as<uint32_t>(), bits_dev));
   for (int p = 0; p < passes; ++p) {
     uint64_t *status = nullptr;
     uint32_t *ticket = nullptr, epoch = 0;

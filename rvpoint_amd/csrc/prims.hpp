@@ -35,6 +35,12 @@ int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_
 int compact_xyz_aos_pos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want,
                         size_t n, float *out_aos, uint32_t *pos, uint32_t *count_dev, hipStream_t s);
 
+// X1 (ransac_plane.cpp:374-382) in one pass: points with |((a*x + b*y) + c*z) + d| <= t (as dist >= -t &&
+// dist <= t) to in_aos, the rest to out_aos, each in index order (either may be null); counts_dev[0..1] =
+// (inliers, outliers) on the device.  m = {a, b, c, d} on the host.
+int plane_split_aos(const float *x, const float *y, const float *z, size_t n, const float *m, float t, float *in_aos,
+                    float *out_aos, uint32_t *counts_dev, hipStream_t s);
+
 // The reference's sequential fp32 sum s = fl(s + x_i) over i = 0..n-1 of non-negative terms, exact, in
 // parallel (seqsum.hip).  mode 0: x_i = v[i]; mode 1: x_i = fl(fl(v[i] - *mean_dev)^2).  *out_dev = s.
 int exact_seq_sum(const float *v, size_t n, int mode, const float *mean_dev, float *out_dev, hipStream_t s);

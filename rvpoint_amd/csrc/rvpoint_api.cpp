@@ -1,8 +1,11 @@
 // rvpoint_api.cpp -- the reference's rvpoint:: API (the "rvv_pcl.h" signatures) implemented over the C ABI
 // of librvp_hip.so.  Host buffers are staged to HBM, the GPU computes, results are copied back; there is no
 // CPU compute path.  Device errors throw rvpoint::DeviceError (the reference has no error channel).
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -338,6 +341,183 @@ void normal_estimation_rvv_capped(const PointCloudSoA &in, const SpatialHash &ha
 void normal_estimation_rvv(const PointCloudSoA &in, const SpatialHash &hash, float *nx, float *ny, float *nz, int,
                            float radius, float vp_x, float vp_y, float vp_z, int) {
   normal_estimation_rvv_capped(in, hash, nx, ny, nz, radius, 0, vp_x, vp_y, vp_z);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Octree / PointerOctree facades (octree.h:11-48, pointer_octree.h:8-58) over the device grid
+// ---------------------------------------------------------------------------------------------------
+namespace {
+
+// A grid edge for searches of any radius: about `per_cell` points per cell over the dimensions the cloud
+// actually spans (a plane is 2-D, a line 1-D).  Search results do not depend on it.
+float facade_cell(const PointCloudSoA &c, int per_cell) {
+  float mn[3], mx[3];
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = std::numeric_limits<float>::infinity();
+    mx[a] = -std::numeric_limits<float>::infinity();
+  }
+  for (std::size_t i = 0; i < c.n; ++i) {
+    const float v[3] = {c.x[i], c.y[i], c.z[i]};
+    for (int a = 0; a < 3; ++a) {
+      if (v[a] < mn[a]) mn[a] = v[a];
+      if (v[a] > mx[a]) mx[a] = v[a];
+    }
+  }
+  double ext[3], emax = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    ext[a] = mx[a] >= mn[a] ? (double)mx[a] - (double)mn[a] : 0.0;
+    emax = std::max(emax, ext[a]);
+  }
+  if (!(emax > 0.0) || !std::isfinite(emax)) return 1.0f;
+  double vol = 1.0;
+  int dims = 0;
+  for (int a = 0; a < 3; ++a)
+    if (ext[a] > emax * 1e-3) {
+      vol *= ext[a];
+      ++dims;
+    }
+  const double cell = std::pow(vol * (double)std::max(per_cell, 1) / (double)std::max<std::size_t>(c.n, 1),
+                               1.0 / dims);
+  return (float)std::max(cell, emax * 1e-6);
+}
+
+std::unique_ptr<SpatialHash> facade_build(const PointCloudSoA &cloud, int leaf_points) {
+  auto h = std::make_unique<SpatialHash>();
+  h->setInputCloud(cloud, facade_cell(cloud, std::max(1, leaf_points / 8)));
+  h->build();
+  return h;
+}
+
+}  // namespace
+
+OctreeNode::~OctreeNode() {
+  for (OctreeNode *c : children) delete c;
+}
+
+Octree::Octree() = default;
+Octree::~Octree() = default;
+
+void Octree::setInputCloud(const PointCloudSoA &cloud) { cloud_ = cloud; }
+
+void Octree::build() { hash_ = facade_build(cloud_, max_points_per_leaf_); }
+
+std::size_t Octree::radiusSearch(const PointXYZ &query, float radius, std::vector<int> &indices,
+                                 std::vector<float> &dists, int) const {
+  indices.clear();
+  dists.clear();
+  if (!hash_) return 0;  // octree.cpp:117: no tree, no neighbours
+  return hash_->radiusSearch(query, radius, indices, dists, 0);
+}
+
+const SpatialHash &Octree::hash() const {
+  if (!hash_) throw std::logic_error("Octree: build() has not been called");
+  return *hash_;
+}
+
+PointerOctreeNode::PointerOctreeNode() = default;
+PointerOctreeNode::~PointerOctreeNode() {
+  for (PointerOctreeNode *c : children) delete c;
+}
+
+PointerOctree::PointerOctree() = default;
+PointerOctree::~PointerOctree() = default;
+
+void PointerOctree::setInputCloud(const PointCloudSoA &cloud) { cloud_ = cloud; }
+
+void PointerOctree::build() {
+  hash_ = facade_build(cloud_, max_points_per_leaf_);
+  root_.reset();
+}
+
+std::size_t PointerOctree::radiusSearch(const PointXYZ &query, float radius, std::vector<int> &indices,
+                                        std::vector<float> &dists) const {
+  indices.clear();
+  dists.clear();
+  if (!hash_) return 0;
+  return hash_->radiusSearch(query, radius, indices, dists, 0);
+}
+
+std::size_t PointerOctree::radiusSearchScalar(const PointXYZ &query, float radius, std::vector<int> &indices,
+                                              std::vector<float> &dists) const {
+  return radiusSearch(query, radius, indices, dists);
+}
+
+const PointerOctreeNode *PointerOctree::getRoot() const {
+  if (!hash_) return nullptr;
+  if (!root_) {
+    auto r = std::make_unique<PointerOctreeNode>();
+    const std::size_t n = cloud_.n;
+    float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    if (n) {
+      mn[0] = mx[0] = cloud_.x[0];
+      mn[1] = mx[1] = cloud_.y[0];
+      mn[2] = mx[2] = cloud_.z[0];
+    }
+    for (std::size_t i = 0; i < n; ++i) {
+      const float v[3] = {cloud_.x[i], cloud_.y[i], cloud_.z[i]};
+      for (int a = 0; a < 3; ++a) {
+        if (v[a] < mn[a]) mn[a] = v[a];
+        if (v[a] > mx[a]) mx[a] = v[a];
+      }
+    }
+    r->min_x = mn[0];
+    r->min_y = mn[1];
+    r->min_z = mn[2];
+    r->max_x = mx[0];
+    r->max_y = mx[1];
+    r->max_z = mx[2];
+    r->indices.resize(n);
+    for (std::size_t i = 0; i < n; ++i) r->indices[i] = (int)i;
+    r->leaf_x.assign(cloud_.x, cloud_.x + n);
+    r->leaf_y.assign(cloud_.y, cloud_.y + n);
+    r->leaf_z.assign(cloud_.z, cloud_.z + n);
+    root_ = std::move(r);
+  }
+  return root_.get();
+}
+
+const SpatialHash &PointerOctree::hash() const {
+  if (!hash_) throw std::logic_error("PointerOctree: build() has not been called");
+  return *hash_;
+}
+
+// normal_estimation.cpp:155-187, 218-246: every neighbour within the radius (the trees ignore max_nn)
+void normal_estimation_rvv(const PointCloudSoA &in, const Octree &octree, float *nx, float *ny, float *nz, int,
+                           float radius, float vp_x, float vp_y, float vp_z, int) {
+  if (in.n == 0) return;
+  normal_estimation_rvv_capped(in, octree.hash(), nx, ny, nz, radius, 0, vp_x, vp_y, vp_z);
+}
+
+void normal_estimation_rvv(const PointCloudSoA &in, const PointerOctree &octree, float *nx, float *ny, float *nz,
+                           int, float radius, float vp_x, float vp_y, float vp_z, int) {
+  if (in.n == 0) return;
+  normal_estimation_rvv_capped(in, octree.hash(), nx, ny, nz, radius, 0, vp_x, vp_y, vp_z);
+}
+
+// normal_estimation.cpp:248-255 builds a PointerOctree on the cloud; here a grid with the radius as its edge
+void normal_estimation_rvv(const PointCloudSoA &in, float *nx, float *ny, float *nz, int, float radius,
+                           float vp_x, float vp_y, float vp_z, int) {
+  if (in.n == 0) return;
+  SpatialHash h;
+  h.setInputCloud(in, radius > 0.0f && std::isfinite(radius) ? radius : facade_cell(in, 8));
+  h.build();
+  normal_estimation_rvv_capped(in, h, nx, ny, nz, radius, 0, vp_x, vp_y, vp_z);
+}
+
+// statistical_outlier_removal.cpp:220-244: all neighbours within the radius, sorted, the k smallest after
+// the first -- the same sum, in the same order, as the hash variant's capped (d2, idx) list
+std::size_t sor_octree(const PointCloudSoA &in, const Octree &tree, PointXYZ *out, int k, float alpha,
+                       float search_radius) {
+  if (in.n == 0) return 0;
+  return sor_spatial_hash(in, tree.hash(), out, k, alpha, search_radius);
+}
+
+// statistical_outlier_removal.cpp:272-301 sums the k smallest in nth_element's order, which the C++ library
+// leaves unspecified; here they are summed ascending, as in sor_octree
+std::size_t sor_pointer_octree(const PointCloudSoA &in, const PointerOctree &tree, PointXYZ *out, int k,
+                               float alpha, float search_radius) {
+  if (in.n == 0) return 0;
+  return sor_spatial_hash(in, tree.hash(), out, k, alpha, search_radius);
 }
 
 // ---------------------------------------------------------------------------------------------------

@@ -290,17 +290,6 @@ static int plane_counts_dev(const float *x, const float *y, const float *z, size
 }
 
 // ------------------------------------------------------------------------------------------------
-// X1: dist = ((a*x + b*y) + c*z) + d (no fabs); inlier iff dist >= -t && dist <= t (ransac_plane.cpp:374-382)
-// ------------------------------------------------------------------------------------------------
-__global__ void plane_flags_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-                              size_t n, float a, float b, float c, float d, float t, uint8_t *__restrict__ f) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float dist = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(a, x[i]), __fmul_rn(b, y[i])), __fmul_rn(c, z[i])), d);
-  f[i] = (dist >= -t && dist <= t) ? 1 : 0;
-}
-
-// ------------------------------------------------------------------------------------------------
 // batched V1 and R1: clouds [off[c], off[c+1]) of one SoA batch, counts on the device, no host sync
 // except the final read of the RANSAC counts.
 // ------------------------------------------------------------------------------------------------
@@ -648,18 +637,28 @@ int rvp_plane_extract(const float *x, const float *y, const float *z, size_t n, 
   if (n == 0) return RVP_OK;
   RVP_TRY(pool_init());
   hipStream_t s = S(st);
-  Tmp f, c;
-  RVP_TRY(f.alloc(n, s));
+  Tmp c;
   RVP_TRY(c.alloc(2 * sizeof(uint32_t), s));
-  RVP_LAUNCH("plane_flags_k", s, plane_flags_k<<<ceil_div(n, 256), 256, 0, s>>>(x, y, z, n, m[0], m[1], m[2], m[3], t, f.as<uint8_t>()));
-  RVP_TRY(compact_xyz_aos(x, y, z, f.as<uint8_t>(), 1, n, in_aos, c.as<uint32_t>(), s));
-  RVP_TRY(compact_xyz_aos(x, y, z, f.as<uint8_t>(), 0, n, out_aos, c.as<uint32_t>() + 1, s));
+  RVP_TRY(plane_split_aos(x, y, z, n, m, t, in_aos, out_aos, c.as<uint32_t>(), s));
   uint32_t h[2];
   RVP_HIP(hipMemcpyAsync(h, c.p, sizeof(h), hipMemcpyDeviceToHost, s));
   RVP_HIP(hipStreamSynchronize(s));
   *n_in = h[0];
   *n_out = h[1];
   return RVP_OK;
+}
+
+int rvp_ransac_plane_extract(const float *x, const float *y, const float *z, size_t n, float dist_thresh,
+                             int max_iters, float collinear_thresh, float probability, float *model, int *inliers,
+                             int *iterations, float *in_aos, size_t *n_in, float *out_aos, size_t *n_out,
+                             rvp_stream_t st) {
+  RVP_CHECK(model && inliers && n_in && n_out, RVP_E_ARG, "null output");
+  *n_in = *n_out = 0;
+  RVP_TRY(rvp_ransac_plane(x, y, z, n, dist_thresh, max_iters, collinear_thresh, probability, model, inliers,
+                           iterations, st));
+  // pipeline_export.cpp:471-476: the split runs only when a model was found
+  if (*inliers <= 0) return RVP_OK;
+  return rvp_plane_extract(x, y, z, n, model, dist_thresh, in_aos, n_in, out_aos, n_out, st);
 }
 
 int rvp_batch_voxel(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,

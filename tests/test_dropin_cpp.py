@@ -15,7 +15,7 @@ EXE = os.path.join(ROOT, "tests", "cpp", "_build", "unit_checks")
 
 @pytest.mark.skipif(not os.path.isdir(REF_TESTS), reason="reference tree not mounted")
 @pytest.mark.parametrize("name", ["test_radius.cpp", "test_sor.cpp", "test_voxel_grid.cpp", "test_ransac.cpp",
-                                  "test_euclidean_clustering.cpp"])
+                                  "test_euclidean_clustering.cpp", "test_octree.cpp", "test_normal.cpp"])
 def test_reference_unit_tests_compile_against_dropin(name):
     inc = os.path.join(ROOT, "include")
     dirs = [inc] + [os.path.join(inc, d) for d in ("core", "search", "filters", "features", "segmentation")]
@@ -30,7 +30,7 @@ REF_UNIT = os.path.join(ROOT, "oracle", "_ref", "unit")
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["test_radius", "test_sor", "test_voxel_grid", "test_ransac",
-                                  "test_euclidean_clustering"])
+                                  "test_euclidean_clustering", "test_octree", "test_normal"])
 def test_reference_unit_tests_pass_on_the_gpu(name):
     """The reference's own unit-test programs, built unchanged against the drop-in (oracle/_ref/unit)."""
     exe = os.path.join(REF_UNIT, name)

@@ -239,6 +239,44 @@ def test_plane_extract():
     assert (a == oa).all() and (b == ob).all()
 
 
+@pytest.mark.parametrize("n", [1, 2047, 2048, 2049, 100000, 1000003])
+def test_plane_split_one_pass_sizes(n):
+    """The one-pass split at tile edges and past the look-back window (489 tiles at 10^6 points): both outputs
+    against the oracle, NaN coordinates on the outlier side, and each output alone."""
+    import ctypes as C
+
+    from rvpoint_amd import _lib
+    rng = np.random.default_rng(n)
+    p = np.c_[rng.uniform(-5, 5, (n, 2)), rng.normal(0, 0.05, n)].astype(np.float32)
+    p[::97, 2] = np.nan
+    m = np.array([0.0, 0.0, 1.0, 0.0], np.float32)
+    print(f"plane split n={n}", flush=True)
+    a, b = R.extract_plane_inliers_outliers_rvv(R.PointCloudSoA(*soa(p)), m, 0.05)
+    oa, ob = O.plane_extract(*soa(p), m, 0.05)
+    assert len(a) + len(b) == n
+    assert np.array_equal(a.view(np.uint32), oa.view(np.uint32)) and np.array_equal(b.view(np.uint32),
+                                                                                     ob.view(np.uint32))
+    dc = R.DeviceCloud.from_soa(*soa(p))
+    out = R.DeviceArray((n, 3), np.float32)
+    mm = (C.c_float * 4)(*[float(v) for v in m])
+    ni, no = C.c_size_t(), C.c_size_t()
+    _lib.call("rvp_plane_extract", *dc.ptrs(), n, mm, 0.05, None, C.byref(ni), out.ptr, C.byref(no), None)
+    assert ni.value == len(oa) and np.array_equal(out.numpy()[:no.value].view(np.uint32), ob.view(np.uint32))
+    _lib.call("rvp_plane_extract", *dc.ptrs(), n, mm, 0.05, out.ptr, C.byref(ni), None, C.byref(no), None)
+    assert no.value == len(ob) and np.array_equal(out.numpy()[:ni.value].view(np.uint32), oa.view(np.uint32))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_ransac_plane_extract_fused(seed):
+    """RANSAC and the split by its best model in one call equal the two reference steps in sequence."""
+    p = c0_small(30000, 40 + seed)
+    best, model, iters, a, b = R.ransac_plane_extract(R.PointCloudSoA(*soa(p)), 0.005, 500)
+    ob, om, oit = O.ransac_plane(*soa(p), 0.005, 500)
+    assert best == ob and (model == om).all() and iters == oit
+    oa, obb = O.plane_extract(*soa(p), om, 0.005)
+    assert (a == oa).all() and (b == obb).all() and len(a) + len(b) == len(p)
+
+
 # ------------------------------------------------------------------------------------------------ normals
 def _angles_deg(n, ref):
     n = np.asarray(n, np.float64)

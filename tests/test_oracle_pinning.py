@@ -329,3 +329,50 @@ def test_euclidean_cluster_equals_reference(tol, mn, mx):
     got = O.euclidean_cluster(x, y, z, tol, mn, mx)
     ref = O.ref_euclidean_cluster(x, y, z, tol, mn, mx)
     assert len(got) == len(ref) and all((a == b).all() for a, b in zip(got, ref))
+
+
+@needs_ref
+@pytest.mark.parametrize("r,k", [(0.3, 10), (0.6, 32)])
+def test_reference_sor_octree_equals_its_hash_variant(r, k):
+    """The reference's own sor_octree (every neighbour in the radius, sorted, the k smallest after the first)
+    keeps exactly what its sor_spatial_hash keeps (the k+1 smallest by (d2, idx)): the drop-in's Octree facade
+    runs the hash variant."""
+    R = O.ref()
+    p, (x, y, z) = cloud(3000, 31)
+    a, b = np.zeros_like(p), np.zeros_like(p)
+    ma = R.ref_sor_octree(x, y, z, len(p), a.reshape(-1), k, 1.0, r, 0)
+    mb = R.ref_sor_spatial_hash(x, y, z, len(p), r, b.reshape(-1), k, 1.0, r)
+    assert ma == mb and (a[:ma] == b[:mb]).all()
+    out, _ = O.sor_radius(x, y, z, r, k, 1.0, r)
+    assert ma == len(out) and (a[:ma] == out).all()
+
+
+@needs_ref
+def test_reference_pointer_octree_sor_is_not_a_reproducible_target():
+    """sor_pointer_octree (statistical_outlier_removal.cpp:288-295) calls nth_element(begin, begin + valid_k,
+    end) and sums positions 1..valid_k: position 0 holds an unspecified one of the valid_k smallest, not
+    necessarily the point itself, so the sum drops an arbitrary neighbour (chosen by libstdc++'s introselect on
+    the tree's traversal order) instead of the self distance.  Its kept set differs from sor_octree's; the
+    drop-in's PointerOctree variant follows sor_octree's definition."""
+    R = O.ref()
+    p, (x, y, z) = cloud(3000, 32)
+    a, b = np.zeros_like(p), np.zeros_like(p)
+    ma = R.ref_sor_octree(x, y, z, len(p), a.reshape(-1), 20, 1.0, 0.5, 0)
+    mb = R.ref_sor_octree(x, y, z, len(p), b.reshape(-1), 20, 1.0, 0.5, 1)
+    assert ma != mb
+
+
+@needs_ref
+@pytest.mark.parametrize("tree", [0, 1])
+def test_reference_octree_radius_is_the_bruteforce_set(tree):
+    """The reference trees return the brute-force set (in traversal order): the facades return the same set
+    in (d2, idx) order, which the hash radius search is pinned to above."""
+    R = O.ref()
+    p, (x, y, z) = cloud(4000, 33)
+    rng = np.random.default_rng(34)
+    for q in rng.normal(0, 1, (20, 3)).astype(np.float32):
+        idx = np.zeros(4000, np.int32)
+        d2 = np.zeros(4000, np.float32)
+        m = R.ref_octree_radius(x, y, z, len(p), float(q[0]), float(q[1]), float(q[2]), 0.4, tree, idx, d2, 4000)
+        bi, _ = O.radius_bruteforce(x, y, z, q, 0.4, len(x))
+        assert sorted(idx[:m].tolist()) == sorted(bi.tolist())
