@@ -406,17 +406,21 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_hist_all_k(const K *__rest
     }
   }
   __syncthreads();
+  if ((size_t)blockIdx.x * PRIM_THREADS >= n) return;  // saw no key: the sum stops before this block
   for (int p = 0; p < act; ++p) part[((size_t)blockIdx.x * RS_MAX_PASSES + p) * 256 + t] = h[p][t];
 }
 
 // ghist[p][d] = sum over the hb partial histograms; block (p, q) sums blocks [q*RS_SUM, (q+1)*RS_SUM)
 constexpr int RS_SUM = 16;
 __global__ __launch_bounds__(PRIM_THREADS) void radix_hist_sum_k(const uint32_t *__restrict__ part, unsigned hb,
-                                                                  int passes, uint32_t *__restrict__ ghist,
+                                                                  size_t n, int passes, uint32_t *__restrict__ ghist,
+                                                                  const uint32_t *__restrict__ n_dev,
                                                                   const int *__restrict__ bits_dev) {
   const int p = blockIdx.x % passes, q = blockIdx.x / passes, t = threadIdx.x;
   if (p >= active_passes(passes, bits_dev)) return;
-  const unsigned b0 = (unsigned)q * RS_SUM, b1 = min(hb, b0 + RS_SUM);
+  // histogram blocks that saw keys: the first ceil(n / PRIM_THREADS) of them
+  const unsigned used = (unsigned)min((size_t)hb, (dev_count(n, n_dev) + PRIM_THREADS - 1) / PRIM_THREADS);
+  const unsigned b0 = (unsigned)q * RS_SUM, b1 = min(used, b0 + RS_SUM);
   uint32_t v[RS_SUM];
 #pragma unroll
   for (int j = 0; j < RS_SUM; ++j) v[j] = b0 + j < b1 ? part[((size_t)(b0 + j) * RS_MAX_PASSES + p) * 256 + t] : 0u;
@@ -596,7 +600,7 @@ static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, siz
                                                                                     part.as<uint32_t>(), n_dev,
                                                                                     bits_dev));
   RVP_LAUNCH("radix_hist_sum_k", s, radix_hist_sum_k<<<(unsigned)passes * ceil_div(hb, RS_SUM), PRIM_THREADS, 0, s>>>(
-                                        part.as<uint32_t>(), hb, passes, gh.as<uint32_t>(), bits_dev));
+                                        part.as<uint32_t>(), hb, n, passes, gh.as<uint32_t>(), n_dev, bits_dev));
   for (int p = 0; p < passes; ++p) {
     uint64_t *status = nullptr;
     uint32_t *ticket = nullptr, epoch = 0;
