@@ -94,7 +94,27 @@ __global__ void voxel_centroid_k(const float *__restrict__ x, const float *__res
   if (v >= V) return;
   const uint32_t s = start[v], e = (v + 1 < V) ? start[v + 1] : n;
   float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-  for (uint32_t j = s; j < e; ++j) {
+  uint32_t j = s;
+  // the loads run ahead of the sums (8 indices, then their coordinates); the adds stay in input order
+  for (; j + 8 <= e; j += 8) {
+    uint32_t p[8];
+    float vx[8], vy[8], vz[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) p[u] = perm[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      vx[u] = x[p[u]];
+      vy[u] = y[p[u]];
+      vz[u] = z[p[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      sx = __fadd_rn(sx, vx[u]);
+      sy = __fadd_rn(sy, vy[u]);
+      sz = __fadd_rn(sz, vz[u]);
+    }
+  }
+  for (; j < e; ++j) {
     const uint32_t p = perm[j];
     sx = __fadd_rn(sx, x[p]);
     sy = __fadd_rn(sy, y[p]);
