@@ -409,7 +409,7 @@ __global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, float 
   __shared__ WaveLds lds[QW];
   const int w = threadIdx.x >> 6;
   const uint32_t total = A.off[A.ncl];
-  for (uint32_t q = blockIdx.x * QW + w; q < total; q += gridDim.x * QW) {
+  for (uint32_t q = wave_id(); q < total; q += wave_count()) {
     const int c = cloud_of(A.off, A.ncl, q);
     const GridView g = seg_view(A, c);
     TopKT<uint32_t> t;
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(QT) void seg_radius_topk_k(SegArrays A, const float
   __shared__ WaveLds lds[QW];
   const int w = threadIdx.x >> 6;
   const uint32_t nq = A.off[A.ncl];
-  for (uint32_t q = blockIdx.x * QW + w; q < nq; q += gridDim.x * QW) {
+  for (uint32_t q = wave_id(); q < nq; q += wave_count()) {
     const int c = cloud_of(A.off, A.ncl, q);
     const GridView g = seg_view(A, c);
     const float r = radius[c];

@@ -52,10 +52,51 @@ __device__ __forceinline__ GridView seg_view(const SegArrays &A, int c) {
   return v;
 }
 
-// grid of the batched query kernels: they stride over the true total read from the offsets, so the
-// launch need not scale with the bound
-inline unsigned query_blocks(size_t n_bound, size_t per_wave = 1) {
-  return (unsigned)std::min<size_t>(8192, std::max<size_t>(1, ceil_div(n_bound, QW * per_wave)));
+// Grid of the batched one-query-per-wave kernels: wave g of G takes queries g, g + G, ... of the true total
+// (read from the offsets on the device), so the launch need not scale with the bound; 32768 waves are several
+// rounds of the resident ones, which lets the dispatcher even out queries of uneven cost.
+inline unsigned query_blocks(size_t n_bound) {
+  return (unsigned)std::min<size_t>(8192, std::max<size_t>(1, ceil_div(n_bound, QW)));
+}
+
+// Grid and mode of a batch-solving query kernel (WaveBatches): strided over two rounds of resident waves
+// while the batches would fill fewer than 32 rounds, else one batch of NB per wave.
+template <int NB, class K> int batch_blocks(K kernel, size_t n_bound, unsigned *blocks, bool *strided) {
+  unsigned res = 1;
+  RVP_TRY(resident_blocks(reinterpret_cast<const void *>(kernel), QT, &res));
+  const size_t batches = ceil_div(n_bound, (size_t)NB);
+  *strided = batches <= (size_t)res * QW * 32;
+  *blocks = *strided ? (unsigned)std::max<size_t>(1, std::min<size_t>((size_t)res * 2, ceil_div(n_bound, QW)))
+                     : (unsigned)ceil_div(batches, QW);
+  return RVP_OK;
+}
+
+// this wave's index g and the wave count G of the launch
+__device__ __forceinline__ uint32_t wave_id() { return blockIdx.x * QW + (threadIdx.x >> 6); }
+__device__ __forceinline__ uint32_t wave_count() { return gridDim.x * QW; }
+
+// Batch k (up to NB queries) of a wave that solves a batch at a time; false when the wave is done.
+// STRIDED (a grid of two rounds of resident waves, for counts that fill only a few rounds of batches): batch
+// k holds the wave's queries g + (k*NB + j)*G, so every wave gets the same count to within one and no
+// partial last round of whole batches is left.  Otherwise (a grid of one batch per wave, which the
+// dispatcher balances): the one batch of NB neighbouring queries NB*g + j.
+template <bool STRIDED, int NB>
+__device__ __forceinline__ bool wave_batch(uint32_t k, uint32_t total, uint32_t &first, uint32_t &step,
+                                           int &nbatch) {
+  if (STRIDED) {
+    const uint64_t G = wave_count(), f = (uint64_t)wave_id() + (uint64_t)k * NB * G;
+    if (f >= total) return false;
+    first = (uint32_t)f;
+    step = (uint32_t)G;
+    nbatch = (int)min<uint64_t>(NB, ((uint64_t)total - f + G - 1) / G);
+    return true;
+  }
+  const uint64_t f = (uint64_t)wave_id() * NB;
+  if (k > 0 || f >= total) return false;
+  first = (uint32_t)f;
+  step = 1;
+  nbatch = (int)min<uint64_t>(NB, (uint64_t)total - f);
+  return true;
 }
 
 // out[c] = running max of min(in[c], n_bound), out[0] = 0: the caller's offsets as a valid partition

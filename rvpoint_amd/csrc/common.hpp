@@ -114,6 +114,8 @@ bool prof_want(const char *name);
 double tune_knn_cell_div();
 // dense-table cells per point a batched grid may use (rvp_tune_set "seg_cells_per_point")
 double tune_seg_cells_per_point();
+// blocks of `threads` threads of `kernel` the current device holds at once (occupancy x CUs), cached
+int resident_blocks(const void *kernel, int threads, unsigned *blocks);
 hipEvent_t prof_start(hipStream_t s);
 void prof_stop(const char *name, hipStream_t s, hipEvent_t start);
 

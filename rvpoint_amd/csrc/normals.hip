@@ -220,7 +220,7 @@ __device__ __forceinline__ void stash(NormLds &N, int j, const float c[9], float
 }
 
 // MODE 0: k nearest (N1, K = k+1 incl. self); MODE 1: radius list capped at K (N2)
-template <int MODE>
+template <int MODE, bool STRIDED>
 __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, float vpx, float vpy, float vpz,
                                                       float *__restrict__ nx, float *__restrict__ ny,
                                                       float *__restrict__ nz, unsigned long long *fallbacks,
@@ -229,32 +229,33 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
   __shared__ WaveLds lds[QW];
   __shared__ NormLds nl[QW];
   const int w = threadIdx.x >> 6;
-  const uint32_t base = (blockIdx.x * QW + w) * NB;
-  if (base >= g.n) return;
   WaveLds &L = lds[w];
   NormLds &N = nl[w];
-  const int nbatch = (int)min<uint32_t>(NB, g.n - base);
   const float r2 = __fmul_rn(radius, radius);
-  for (int j = 0; j < nbatch; ++j) {
-    const uint32_t q = base + j;
-    const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
-    TopK t;
-    t.init(K);
-    if (MODE == 0) {
-      knn_search<true>(g, x, y, z, t, L);
-    } else {
-      int lo[3], hi[3];
-      window3(g, x, y, z, radius, lo, hi);
-      TopKSink<true> sink{t, L};
-      scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
-      topk_flush<true>(t, L);
+  uint32_t first = 0, step = 1;
+  int nbatch = 0;
+  for (uint32_t k = 0; wave_batch<STRIDED, NB>(k, g.n, first, step, nbatch); ++k) {
+    for (int j = 0; j < nbatch; ++j) {
+      const uint32_t q = first + (uint32_t)j * step;
+      const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
+      TopK t;
+      t.init(K);
+      if (MODE == 0) {
+        knn_search<true>(g, x, y, z, t, L);
+      } else {
+        int lo[3], hi[3];
+        window3(g, x, y, z, radius, lo, hi);
+        TopKSink<true> sink{t, L};
+        scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
+        topk_flush<true>(t, L);
+      }
+      const int m = min(t.cnt, K);
+      float c[9];
+      wave_cov(g, lane() < m, t.pay, m, x, y, z, c);
+      stash(N, j, c, x, y, z, g.perm[q], m);
     }
-    const int m = min(t.cnt, K);
-    float c[9];
-    wave_cov(g, lane() < m, t.pay, m, x, y, z, c);
-    stash(N, j, c, x, y, z, g.perm[q], m);
+    solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks, cx, cy, cz);
   }
-  solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks, cx, cy, cz);
 }
 
 // N2 without a cap: neighbour positions in CSR rows (row = original index), covariance in chunks of 64
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
 
 // N1 / N2 over a batch: every query's neighbourhood within its own cloud (MODE 0: K nearest incl. self;
 // MODE 1: radius[c] list capped at K)
-template <int MODE>
+template <int MODE, bool STRIDED>
 __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius, float vpx,
                                                      float vpy, float vpz, float *__restrict__ nx,
                                                      float *__restrict__ ny, float *__restrict__ nz,
@@ -330,12 +331,13 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
   __shared__ WaveLds lds[QW];
   __shared__ NormLds nl[QW];
   const int w = threadIdx.x >> 6;
-  const uint32_t total = A.off[A.ncl];
   NormLds &N = nl[w];
-  for (uint32_t base = (blockIdx.x * QW + w) * NB; base < total; base += gridDim.x * QW * NB) {
-    const int nbatch = (int)min<uint32_t>(NB, total - base);
+  const uint32_t total = A.off[A.ncl];
+  uint32_t first = 0, step = 1;
+  int nbatch = 0;
+  for (uint32_t k = 0; wave_batch<STRIDED, NB>(k, total, first, step, nbatch); ++k) {
     for (int j = 0; j < nbatch; ++j) {
-      const uint32_t q = base + j;
+      const uint32_t q = first + (uint32_t)j * step;
       const int c = cloud_of(A.off, A.ncl, q);
       const GridView g = seg_view(A, c);
       const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
@@ -380,8 +382,15 @@ int rvp_normals_knn_centroids(const float *x, const float *y, const float *z, si
   RVP_TRY(G.build_for_knn(x, y, z, n, k + 1, s));
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
-  RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0><<<ceil_div(n, QW * NB), QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz, nx, ny, nz,
-                                                          fb, cx, cy, cz));
+  unsigned qb = 1;
+  bool strided = false;
+  RVP_TRY(batch_blocks<NB>(normals_topk_k<0, true>, n, &qb, &strided));
+  if (strided)
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, true><<<qb, QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz,
+                                                                              nx, ny, nz, fb, cx, cy, cz));
+  else
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, false><<<qb, QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz,
+                                                                               nx, ny, nz, fb, cx, cy, cz));
   RVP_LAUNCHED();
   return RVP_OK;
 }
@@ -403,8 +412,17 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
   RVP_TRY(fallback_counter(s, &fb));
   const GridView g = h->g.view();
   if (max_nn > 0 && max_nn <= MAX_TOPK) {
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1><<<ceil_div(n, QW * NB), QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx, ny, nz,
-                                                            fb, nullptr, nullptr, nullptr));
+    unsigned qb = 1;
+    bool strided = false;
+    RVP_TRY(batch_blocks<NB>(normals_topk_k<1, true>, n, &qb, &strided));
+    if (strided)
+      RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, true><<<qb, QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx,
+                                                                                ny, nz, fb, nullptr, nullptr,
+                                                                                nullptr));
+    else
+      RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, false><<<qb, QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx,
+                                                                                 ny, nz, fb, nullptr, nullptr,
+                                                                                 nullptr));
     RVP_LAUNCHED();
   } else {
     // unlimited (or a cap above 64): full lists, sorted by (d2, idx), truncated by the row offsets
@@ -439,8 +457,15 @@ int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const 
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, true, k + 1, nullptr, s));
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
-  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0><<<query_blocks(n_bound, NB), QT, 0, s>>>(
-                                     G.arrays(), k + 1, nullptr, vpx, vpy, vpz, nx, ny, nz, fb));
+  unsigned qb = 1;
+  bool strided = false;
+  RVP_TRY(batch_blocks<NB>(seg_normals_k<0, true>, n_bound, &qb, &strided));
+  if (strided)
+    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true><<<qb, QT, 0, s>>>(G.arrays(), k + 1, nullptr, vpx, vpy,
+                                                                            vpz, nx, ny, nz, fb));
+  else
+    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false><<<qb, QT, 0, s>>>(G.arrays(), k + 1, nullptr, vpx, vpy,
+                                                                             vpz, nx, ny, nz, fb));
   return RVP_OK;
 }
 
@@ -456,8 +481,15 @@ int rvp_batch_normals_radius(const float *x, const float *y, const float *z, con
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, false, 0, cell ? cell : radius, s));
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
-  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1><<<query_blocks(n_bound, NB), QT, 0, s>>>(
-                                     G.arrays(), max_nn, radius, vpx, vpy, vpz, nx, ny, nz, fb));
+  unsigned qb = 1;
+  bool strided = false;
+  RVP_TRY(batch_blocks<NB>(seg_normals_k<1, true>, n_bound, &qb, &strided));
+  if (strided)
+    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, true><<<qb, QT, 0, s>>>(G.arrays(), max_nn, radius, vpx, vpy, vpz,
+                                                                            nx, ny, nz, fb));
+  else
+    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(G.arrays(), max_nn, radius, vpx, vpy,
+                                                                             vpz, nx, ny, nz, fb));
   return RVP_OK;
 }
 

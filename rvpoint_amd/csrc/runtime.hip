@@ -3,6 +3,7 @@
 #include <atomic>
 #include <cstdio>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <vector>
 
@@ -160,6 +161,25 @@ static std::atomic<double> g_knn_cell_div{2.8};
 static std::atomic<double> g_seg_cells_per_point{32.0};
 double tune_knn_cell_div() { return g_knn_cell_div.load(); }
 double tune_seg_cells_per_point() { return g_seg_cells_per_point.load(); }
+
+static std::mutex g_occ_mu;
+static std::map<std::tuple<int, const void *, int>, unsigned> g_occ;
+
+int resident_blocks(const void *kernel, int threads, unsigned *blocks) {
+  int dev = 0;
+  RVP_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_occ_mu);
+  const auto key = std::make_tuple(dev, kernel, threads);
+  auto it = g_occ.find(key);
+  if (it == g_occ.end()) {
+    int per_cu = 0, cus = 0;
+    RVP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+    RVP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    it = g_occ.emplace(key, (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, cus)).first;
+  }
+  *blocks = it->second;
+  return RVP_OK;
+}
 
 }  // namespace rvp
 
