@@ -116,6 +116,16 @@ double tune_knn_cell_div();
 double tune_seg_cells_per_point();
 // blocks of `threads` threads of `kernel` the current device holds at once (occupancy x CUs), cached
 int resident_blocks(const void *kernel, int threads, unsigned *blocks);
+
+// Pinned host staging for small device-to-host read-backs, one growing buffer per (device, stream), held
+// (locked) by this object until it goes out of scope: copies into it are true async DMA, and callers that
+// share a stream take turns.
+struct PinnedStage {
+  struct Slot;
+  Slot *slot = nullptr;
+  int get(hipStream_t s, size_t bytes, void **host);
+  ~PinnedStage();
+};
 hipEvent_t prof_start(hipStream_t s);
 void prof_stop(const char *name, hipStream_t s, hipEvent_t start);
 

@@ -162,6 +162,40 @@ static std::atomic<double> g_seg_cells_per_point{32.0};
 double tune_knn_cell_div() { return g_knn_cell_div.load(); }
 double tune_seg_cells_per_point() { return g_seg_cells_per_point.load(); }
 
+struct PinnedStage::Slot {
+  std::mutex mu;
+  void *p = nullptr;
+  size_t cap = 0;
+};
+static std::mutex g_pin_mu;
+static std::map<std::pair<int, hipStream_t>, PinnedStage::Slot *> g_pin;  // kept until the process ends
+
+int PinnedStage::get(hipStream_t s, size_t bytes, void **host) {
+  int dev = 0;
+  RVP_HIP(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    Slot *&sl = g_pin[{dev, s}];
+    if (!sl) sl = new Slot;
+    slot = sl;
+  }
+  slot->mu.lock();
+  if (slot->cap < bytes) {
+    if (slot->p) RVP_HIP(hipHostFree(slot->p));
+    slot->p = nullptr;
+    slot->cap = 0;
+    const size_t cap = std::max<size_t>(bytes, 64 << 10);
+    RVP_HIP(hipHostMalloc(&slot->p, cap, hipHostMallocDefault));
+    slot->cap = cap;
+  }
+  *host = slot->p;
+  return RVP_OK;
+}
+
+PinnedStage::~PinnedStage() {
+  if (slot) slot->mu.unlock();
+}
+
 static std::mutex g_occ_mu;
 static std::map<std::tuple<int, const void *, int>, unsigned> g_occ;
 

@@ -429,12 +429,14 @@ __global__ void seg_vox_offsets_k(const uint32_t *__restrict__ off, int ncl, con
 __global__ void seg_plane_models_k(const float *__restrict__ x, const float *__restrict__ y,
                                    const float *__restrict__ z, const uint32_t *__restrict__ off, int ncl,
                                    const int32_t *__restrict__ draws, int H, float collinear,
-                                   float *__restrict__ models, int32_t *__restrict__ valid) {
+                                   float *__restrict__ models, int32_t *__restrict__ valid,
+                                   int32_t *__restrict__ counts) {
   const size_t gh = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gh >= (size_t)ncl * H) return;
   const int c = (int)(gh / H), h = (int)(gh % H);
   const uint32_t n = off[c + 1] - off[c];
   valid[gh] = 0;
+  counts[gh] = 0;  // the count kernel adds into it
   if (n < 3) return;
   const uint32_t i1 = (uint32_t)draws[3 * h] % n, i2 = (uint32_t)draws[3 * h + 1] % n,
                  i3 = (uint32_t)draws[3 * h + 2] % n;
@@ -460,18 +462,25 @@ __global__ void seg_plane_models_k(const float *__restrict__ x, const float *__r
 }
 
 // tiles of RC_TILE points that never straddle clouds: tile_off[c] = first tile of cloud c (one wave)
+// (also copies the offsets into the read-back buffer)
 __global__ __launch_bounds__(64) void seg_tiles_k(const uint32_t *__restrict__ off, int ncl,
-                                                  uint32_t *__restrict__ tile_off) {
+                                                  uint32_t *__restrict__ tile_off, uint32_t *__restrict__ off_copy) {
   const int l = lane();
   uint32_t carry = 0;
   for (int b0 = 0; b0 < ncl; b0 += 64) {
     const int c = b0 + l;
     const uint32_t t = c < ncl ? (off[c + 1] - off[c] + RC_TILE - 1) / RC_TILE : 0u;
     const uint32_t inc = wave_incl_scan(t);
-    if (c < ncl) tile_off[c] = carry + inc - t;
+    if (c < ncl) {
+      tile_off[c] = carry + inc - t;
+      off_copy[c] = off[c];
+    }
     carry += __shfl(inc, 63);
   }
-  if (l == 0) tile_off[ncl] = carry;
+  if (l == 0) {
+    tile_off[ncl] = carry;
+    off_copy[ncl] = off[ncl];
+  }
 }
 
 __global__ __launch_bounds__(256) void seg_plane_count_k(const float *__restrict__ x, const float *__restrict__ y,
@@ -608,13 +617,18 @@ int rvp_ransac_plane(const float *x, const float *y, const float *z, size_t n, f
 
   GlibcRand rng(0);  // std::srand(0) (ransac_plane.cpp:60)
   RansacReplay rp(n, max_iters, probability);
-  Tmp idx_d, models_d, valid_d, counts_d;
+  // per chunk of H: the draws go up in one copy, models [H][4] + valid [H] + counts [H] come back in one,
+  // both through pinned staging (draws in words [0, 3H), results after them)
+  Tmp idx_d, pack;
+  PinnedStage stage;
+  int32_t *host = nullptr;
+  RVP_TRY(stage.get(s, (size_t)1024 * 9 * sizeof(int32_t), reinterpret_cast<void **>(&host)));
   int chunk = 128, drawn = 0;
   while (!rp.done) {
     const int H = std::min(chunk, max_iters - drawn);
     if (H <= 0) break;
     // three draws per iteration whether or not it is skipped (ransac_plane.cpp:69-72)
-    std::vector<int32_t> idx3((size_t)H * 3);
+    int32_t *idx3 = host;
     for (int h = 0; h < H; ++h) {
       const int i1 = (int)((size_t)rng.next() % n);
       const int i2 = (int)((size_t)rng.next() % n);
@@ -625,23 +639,19 @@ int rvp_ransac_plane(const float *x, const float *y, const float *z, size_t n, f
       idx3[3 * h + 2] = i3;
     }
     drawn += H;
-    RVP_TRY(idx_d.alloc(idx3.size() * sizeof(int32_t), s));
-    RVP_TRY(models_d.alloc((size_t)H * 4 * sizeof(float), s));
-    RVP_TRY(valid_d.alloc((size_t)H * sizeof(int32_t), s));
-    RVP_TRY(counts_d.alloc((size_t)H * sizeof(int32_t), s));
-    RVP_HIP(hipMemcpyAsync(idx_d.p, idx3.data(), idx3.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-    RVP_LAUNCH("plane_models_k", s, plane_models_k<<<ceil_div(H, 64), 64, 0, s>>>(x, y, z, idx_d.as<int32_t>(), H, collinear_thresh,
-                                                  models_d.as<float>(), valid_d.as<int32_t>()));
-    RVP_LAUNCHED();
-    RVP_TRY(plane_counts_dev(x, y, z, n, models_d.as<float>(), valid_d.as<int32_t>(), H, dist_thresh,
-                             counts_d.as<int32_t>(), s));
-    std::vector<float> models((size_t)H * 4);
-    std::vector<int32_t> valid(H), counts(H);
-    RVP_HIP(hipMemcpyAsync(models.data(), models_d.p, models.size() * sizeof(float), hipMemcpyDeviceToHost, s));
-    RVP_HIP(hipMemcpyAsync(valid.data(), valid_d.p, valid.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    RVP_HIP(hipMemcpyAsync(counts.data(), counts_d.p, counts.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    RVP_TRY(idx_d.alloc((size_t)H * 3 * sizeof(int32_t), s));
+    RVP_TRY(pack.alloc((size_t)H * 6 * sizeof(int32_t), s));
+    float *models_d = pack.as<float>();
+    int32_t *valid_d = reinterpret_cast<int32_t *>(models_d + (size_t)H * 4), *counts_d = valid_d + H;
+    RVP_HIP(hipMemcpyAsync(idx_d.p, idx3, (size_t)H * 3 * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    RVP_LAUNCH("plane_models_k", s, plane_models_k<<<ceil_div(H, 64), 64, 0, s>>>(x, y, z, idx_d.as<int32_t>(), H,
+                                                                                  collinear_thresh, models_d, valid_d));
+    RVP_TRY(plane_counts_dev(x, y, z, n, models_d, valid_d, H, dist_thresh, counts_d, s));
+    int32_t *res = host + (size_t)H * 3;
+    RVP_HIP(hipMemcpyAsync(res, pack.p, (size_t)H * 6 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     RVP_HIP(hipStreamSynchronize(s));
-    rp.feed(models.data(), valid.data(), counts.data(), H);
+    const float *hm = reinterpret_cast<const float *>(res);
+    rp.feed(hm, res + (size_t)H * 4, res + (size_t)H * 5, H);
     chunk = std::min(chunk * 2, 1024);
   }
   for (int k = 0; k < 4; ++k) model[k] = rp.model[k];
@@ -751,28 +761,29 @@ int rvp_batch_ransac_plane(const float *x, const float *y, const float *z, const
   const uint32_t *off = G.offs.as<uint32_t>();
   const int32_t *draws = nullptr;
   RVP_TRY(rand_draws((size_t)3 * H, &draws));
-  Tmp models, valid, counts, tiles;
-  RVP_TRY(models.alloc(NH * 4 * sizeof(float), s));
-  RVP_TRY(valid.alloc(NH * sizeof(int32_t), s));
-  RVP_TRY(counts.alloc(NH * sizeof(int32_t), s));
+  // models [NH][4], valid [NH], counts [NH], offsets [nclouds + 1]: one buffer, read back in one copy
+  const size_t words = NH * 6 + (size_t)nclouds + 1;
+  Tmp pack, tiles;
+  RVP_TRY(pack.alloc(words * sizeof(uint32_t), s));
+  float *models = pack.as<float>();
+  int32_t *valid = reinterpret_cast<int32_t *>(models + NH * 4), *counts = valid + NH;
+  uint32_t *offs_copy = reinterpret_cast<uint32_t *>(counts + NH);
   RVP_TRY(tiles.alloc((nclouds + 1) * sizeof(uint32_t), s));
   RVP_LAUNCH("seg_plane_models_k", s, seg_plane_models_k<<<ceil_div(NH, 256), 256, 0, s>>>(
-                                          x, y, z, off, nclouds, draws, H, collinear_thresh, models.as<float>(),
-                                          valid.as<int32_t>()));
-  RVP_HIP(hipMemsetAsync(counts.p, 0, NH * sizeof(int32_t), s));
-  RVP_LAUNCH("seg_tiles_k", s, seg_tiles_k<<<1, 64, 0, s>>>(off, nclouds, tiles.as<uint32_t>()));
+                                          x, y, z, off, nclouds, draws, H, collinear_thresh, models, valid, counts));
+  RVP_LAUNCH("seg_tiles_k", s, seg_tiles_k<<<1, 64, 0, s>>>(off, nclouds, tiles.as<uint32_t>(), offs_copy));
   const dim3 grid((unsigned)(ceil_div(n_bound, RC_TILE) + nclouds), ceil_div((size_t)H, RC_HYP));
   RVP_LAUNCH("seg_plane_count_k", s, seg_plane_count_k<<<grid, 256, 0, s>>>(
-                                         x, y, z, off, nclouds, tiles.as<uint32_t>(), models.as<float>(),
-                                         valid.as<int32_t>(), H, dist_thresh, counts.as<int32_t>()));
-  std::vector<float> hm(NH * 4);
-  std::vector<int32_t> hv(NH), hc(NH);
-  std::vector<uint32_t> ho(nclouds + 1);
-  RVP_HIP(hipMemcpyAsync(hm.data(), models.p, hm.size() * sizeof(float), hipMemcpyDeviceToHost, s));
-  RVP_HIP(hipMemcpyAsync(hv.data(), valid.p, hv.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  RVP_HIP(hipMemcpyAsync(hc.data(), counts.p, hc.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  RVP_HIP(hipMemcpyAsync(ho.data(), off, ho.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                                         x, y, z, off, nclouds, tiles.as<uint32_t>(), models, valid, H, dist_thresh,
+                                         counts));
+  PinnedStage stage;
+  uint32_t *host = nullptr;
+  RVP_TRY(stage.get(s, words * sizeof(uint32_t), reinterpret_cast<void **>(&host)));
+  RVP_HIP(hipMemcpyAsync(host, pack.p, words * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   RVP_HIP(hipStreamSynchronize(s));
+  const float *hm = reinterpret_cast<const float *>(host);
+  const int32_t *hv = reinterpret_cast<const int32_t *>(hm + NH * 4), *hc = hv + NH;
+  const uint32_t *ho = reinterpret_cast<const uint32_t *>(hc + NH);
   for (int c = 0; c < nclouds; ++c) {
     const size_t n = ho[c + 1] - ho[c];
     for (int k = 0; k < 4; ++k) models_out[4 * c + k] = 0.0f;
@@ -780,7 +791,7 @@ int rvp_batch_ransac_plane(const float *x, const float *y, const float *z, const
     if (iterations_out) iterations_out[c] = 0;
     if (n < 3) continue;  // ransac_plane.cpp:59
     RansacReplay rp(n, max_iters, probability);
-    rp.feed(hm.data() + (size_t)c * H * 4, hv.data() + (size_t)c * H, hc.data() + (size_t)c * H, max_iters);
+    rp.feed(hm + (size_t)c * H * 4, hv + (size_t)c * H, hc + (size_t)c * H, max_iters);
     for (int k = 0; k < 4; ++k) models_out[4 * c + k] = rp.model[k];
     inliers_out[c] = rp.best;
     if (iterations_out) iterations_out[c] = rp.iterations();
