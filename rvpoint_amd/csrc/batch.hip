@@ -424,29 +424,24 @@ __global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, float 
   }
 }
 
-__global__ void seg_mean_k(const uint32_t *__restrict__ off, int ncl, const float *__restrict__ sums,
-                           float *__restrict__ mean) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < ncl) mean[c] = sums[c] / __uint2float_rn(off[c + 1] - off[c]);
-}
-
-__global__ void seg_threshold_k(const uint32_t *__restrict__ off, int ncl, float alpha, const float *__restrict__ mean,
-                                const float *__restrict__ vsum, float *__restrict__ thr) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncl) return;
-  const float sd = sqrtf(vsum[c] / __uint2float_rn(off[c + 1] - off[c]));
-  thr[c] = mean[c] + alpha * sd;
-}
-
+// keep flag (u8, and u32 for the compaction scan) of every point: mean_i <= gmean + alpha * std of its cloud,
+// gmean = fl(sum / n), std = sqrtf(fl(sq / n)) (statistical_outlier_removal.cpp:53-62)
 __global__ void seg_keep_k(const float *__restrict__ means, const uint32_t *__restrict__ off, int ncl,
-                           const float *__restrict__ thr, uint32_t n_bound, uint8_t *__restrict__ keep) {
+                           const float *__restrict__ sums, const float *__restrict__ sqs, float alpha,
+                           uint32_t n_bound, uint8_t *__restrict__ keep, uint32_t *__restrict__ keep32) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_bound) return;
   if (i >= off[ncl]) {
     keep[i] = 0;
+    keep32[i] = 0;
     return;
   }
-  keep[i] = means[i] <= thr[cloud_of(off, ncl, i)] ? 1 : 0;
+  const int c = cloud_of(off, ncl, i);
+  const float len = __uint2float_rn(off[c + 1] - off[c]);
+  const float thr = __fadd_rn(__fdiv_rn(sums[c], len), __fmul_rn(alpha, sqrtf(__fdiv_rn(sqs[c], len))));
+  const bool k = means[i] <= thr;
+  keep[i] = k ? 1 : 0;
+  keep32[i] = k ? 1u : 0u;
 }
 
 __global__ void seg_out_offsets_k(const uint32_t *__restrict__ off, int ncl, uint32_t n_bound,
@@ -505,7 +500,7 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
   hipStream_t s = S(st);
   SegGrid G;
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, true, k + 1, nullptr, s));
-  Tmp mt, kt, sums, mean, vsum, thr, pos, kept;
+  Tmp mt, kt, sums, vsum, pos, kept;
   float *mp = means;
   if (!mp) {
     RVP_TRY(mt.alloc(n_bound * sizeof(float), s));
@@ -513,27 +508,23 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
   }
   RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<<<query_blocks(n_bound), QT, 0, s>>>(G.arrays(), k, mp));
   RVP_TRY(sums.alloc(nclouds * sizeof(float), s));
-  RVP_TRY(mean.alloc(nclouds * sizeof(float), s));
   RVP_TRY(vsum.alloc(nclouds * sizeof(float), s));
-  RVP_TRY(thr.alloc(nclouds * sizeof(float), s));
-  const unsigned cb = ceil_div(nclouds, 64);
-  RVP_TRY(exact_seq_sum_seg(mp, G.off, nclouds, n_bound, 0, nullptr, sums.as<float>(), s));
-  RVP_LAUNCH("seg_mean_k", s, seg_mean_k<<<cb, 64, 0, s>>>(G.off, nclouds, sums.as<float>(), mean.as<float>()));
-  RVP_TRY(exact_seq_sum_seg(mp, G.off, nclouds, n_bound, 1, mean.as<float>(), vsum.as<float>(), s));
-  RVP_LAUNCH("seg_threshold_k", s, seg_threshold_k<<<cb, 64, 0, s>>>(G.off, nclouds, alpha, mean.as<float>(),
-                                                                     vsum.as<float>(), thr.as<float>()));
+  // statistical_outlier_removal.cpp:44-51: the sum of the means, then of the squared deviations from their
+  // mean, both sequential
+  RVP_TRY(exact_seq_mean_var_seg(mp, G.off, nclouds, n_bound, sums.as<float>(), vsum.as<float>(), s));
   uint8_t *kp = keep;
   if (!kp) {
     RVP_TRY(kt.alloc(n_bound, s));
     kp = kt.as<uint8_t>();
   }
-  const unsigned nb = ceil_div(n_bound, 256);
-  RVP_LAUNCH("seg_keep_k", s, seg_keep_k<<<nb, 256, 0, s>>>(mp, G.off, nclouds, thr.as<float>(), (uint32_t)n_bound, kp));
-  // stable compaction over the whole batch keeps the clouds in order; a cloud's output starts where the
-  // compaction positions its first point
   RVP_TRY(pos.alloc(n_bound * sizeof(uint32_t), s));
   RVP_TRY(kept.alloc(sizeof(uint32_t), s));
-  RVP_TRY(compact_xyz_aos_pos(x, y, z, kp, 1, n_bound, out_aos, pos.as<uint32_t>(), kept.as<uint32_t>(), s));
+  const unsigned nb = ceil_div(n_bound, 256);
+  RVP_LAUNCH("seg_keep_k", s, seg_keep_k<<<nb, 256, 0, s>>>(mp, G.off, nclouds, sums.as<float>(), vsum.as<float>(),
+                                                            alpha, (uint32_t)n_bound, kp, pos.as<uint32_t>()));
+  // stable compaction over the whole batch keeps the clouds in order; a cloud's output starts where the
+  // compaction positions its first point
+  RVP_TRY(compact_xyz_aos_flagged(x, y, z, kp, n_bound, out_aos, pos.as<uint32_t>(), kept.as<uint32_t>(), s));
   RVP_LAUNCH("seg_out_offsets_k", s, seg_out_offsets_k<<<ceil_div(nclouds + 1, 64), 64, 0, s>>>(
                                          G.off, nclouds, (uint32_t)n_bound, pos.as<uint32_t>(), kept.as<uint32_t>(),
                                          out_offsets));

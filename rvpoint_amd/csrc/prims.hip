@@ -643,16 +643,15 @@ __global__ void scatter_aos_k(const float *__restrict__ x, const float *__restri
   }
 }
 
-int compact_xyz_aos_pos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want, size_t n,
-                        float *out_aos, uint32_t *pos, uint32_t *count_dev, hipStream_t s) {
+int compact_xyz_aos_flagged(const float *x, const float *y, const float *z, const uint8_t *flags, size_t n,
+                            float *out_aos, uint32_t *pos, uint32_t *count_dev, hipStream_t s) {
   if (n == 0) {
     RVP_HIP(hipMemsetAsync(count_dev, 0, sizeof(uint32_t), s));
     return RVP_OK;
   }
-  const unsigned nb = ceil_div(n, 256);
-  RVP_LAUNCH("flag_eq_k", s, flag_eq_k<<<nb, 256, 0, s>>>(flags, want, n, pos));
   RVP_TRY(exclusive_scan_u32(pos, pos, n, count_dev, s));
-  if (out_aos) RVP_LAUNCH("scatter_aos_k", s, scatter_aos_k<<<nb, 256, 0, s>>>(x, y, z, flags, want, n, pos, out_aos));
+  if (out_aos)
+    RVP_LAUNCH("scatter_aos_k", s, scatter_aos_k<<<ceil_div(n, 256), 256, 0, s>>>(x, y, z, flags, 1, n, pos, out_aos));
   return RVP_OK;
 }
 

@@ -31,9 +31,6 @@ int radix_sort_pairs_u64(uint64_t **keys, uint32_t **vals, uint64_t **keys_tmp, 
 // Stable compaction of points whose flag equals `want` into AoS (float[3*m]); *count_dev = m (device).
 int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want,
                     size_t n, float *out_aos, uint32_t *count_dev, hipStream_t s);
-// the same, keeping the exclusive-scan positions (device uint32[n])
-int compact_xyz_aos_pos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want,
-                        size_t n, float *out_aos, uint32_t *pos, uint32_t *count_dev, hipStream_t s);
 
 // X1 (ransac_plane.cpp:374-382) in one pass: points with |((a*x + b*y) + c*z) + d| <= t (as dist >= -t &&
 // dist <= t) to in_aos, the rest to out_aos, each in index order (either may be null); counts_dev[0..1] =
@@ -48,6 +45,15 @@ int exact_seq_sum(const float *v, size_t n, int mode, const float *mean_dev, flo
 // mode 1 reads one mean per segment from mean_dev[s]; out[s] receives segment s's sum.  No host sync.
 int exact_seq_sum_seg(const float *v, const uint32_t *seg_off, int nseg, size_t n_bound, int mode,
                       const float *mean_dev, float *out, hipStream_t s);
+// Both sums of the SOR statistics per segment, one chunking: sum_out[s] = the sequential sum of v, and
+// sq_out[s] = the sequential sum of fl(fl(v[i] - mean)^2) with mean = fl(sum_out[s] / len(s)).
+int exact_seq_mean_var_seg(const float *v, const uint32_t *seg_off, int nseg, size_t n_bound, float *sum_out,
+                           float *sq_out, hipStream_t s);
+
+// Stable compaction of points whose u8 flag equals 1, given those flags also as u32 in pos (scanned in place
+// into the output positions); *count_dev = kept count (device).
+int compact_xyz_aos_flagged(const float *x, const float *y, const float *z, const uint8_t *flags, size_t n,
+                            float *out_aos, uint32_t *pos, uint32_t *count_dev, hipStream_t s);
 
 inline int bits_for(uint64_t max_value) {
   int b = 0;

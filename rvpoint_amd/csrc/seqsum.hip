@@ -28,11 +28,19 @@ struct SSChunk {
   int clean;
 };
 
-// mode 0: x_i = v[i]; mode 1: x_i = (v[i] - mean)^2 with both ops rounded once (the variance terms)
+// mode 0: x_i = v[i]; modes 1, 2: x_i = (v[i] - mean)^2 with both ops rounded once (the variance terms)
 __device__ __forceinline__ float ss_term(float v, int mode, float mean) {
   if (mode == 0) return v;
   const float d = v - mean;
   return d * d;
+}
+
+// the segment's mean: mode 1 reads it, mode 2 derives it from the segment's sum as fl(sum / len)
+__device__ __forceinline__ float ss_mean(int mode, const float *__restrict__ mean_dev,
+                                         const uint32_t *__restrict__ seg, int s) {
+  if (mode == 0) return 0.0f;
+  if (mode == 1) return mean_dev[s];
+  return __fdiv_rn(mean_dev[s], __uint2float_rn(seg[s + 1] - seg[s]));
 }
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
@@ -85,7 +93,7 @@ __global__ __launch_bounds__(64) void ss_chunksum_k(const float *__restrict__ v,
                                                      const float *__restrict__ mean_dev, double *__restrict__ csum) {
   ChunkRef r;
   if (!chunk_ref(seg, coff, nseg, blockIdx.x, r)) return;
-  const float mean = mode ? mean_dev[r.s] : 0.0f;
+  const float mean = ss_mean(mode, mean_dev, seg, r.s);
   double acc = 0.0;
 #pragma unroll
   for (int q = 0; q < SS_LANE; ++q) {
@@ -121,7 +129,7 @@ __global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, 
     if (l == 0) out[c] = SSChunk{0, 0, 0, 0};
     return;
   }
-  const float mean = mode ? mean_dev[r.s] : 0.0f;
+  const float mean = ss_mean(mode, mean_dev, seg, r.s);
   const double scale = ldexp(1.0, 23 - e);
   long long fe = 0, fo = 0;
 #pragma unroll
@@ -183,7 +191,7 @@ __global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, co
   const int s = blockIdx.x;
   if (s >= nseg) return;
   const int l = lane();
-  const float mean = mode ? mean_dev[s] : 0.0f;
+  const float mean = ss_mean(mode, mean_dev, seg, s);
   const uint32_t c_begin = coff[s], c_end = coff[s + 1], e_end = seg[s + 1];
   float acc = 0.0f;
   for (uint32_t c0 = c_begin; c0 < c_end; c0 += WAVE) {
@@ -230,24 +238,46 @@ __global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, co
   if (l == 0) out[s] = acc;
 }
 
+// chunk offsets of the segments (coff[nseg + 1], exclusive scan of ceil(len / SS_C)); returns the bound
+static int ss_chunk_offsets(const uint32_t *seg_off, int nseg, size_t n_bound, Tmp &coff, size_t *bound,
+                            hipStream_t s) {
+  *bound = n_bound / SS_C + (size_t)nseg + 1;  // chunks: sum of ceil(len / SS_C)
+  RVP_TRY(coff.alloc((nseg + 1) * sizeof(uint32_t), s));
+  RVP_LAUNCH("ss_nchunks_k", s, ss_nchunks_k<<<ceil_div(nseg, 256), 256, 0, s>>>(seg_off, nseg, coff.as<uint32_t>()));
+  return exclusive_scan_u32(coff.as<uint32_t>(), coff.as<uint32_t>(), nseg, coff.as<uint32_t>() + nseg, s);
+}
+
+static int ss_run(const float *v, const uint32_t *seg_off, const uint32_t *coff, int nseg, size_t bound, int mode,
+                  const float *mean_dev, float *out, hipStream_t s) {
+  Tmp cs, ch;
+  RVP_TRY(cs.alloc(bound * sizeof(double), s));
+  RVP_TRY(ch.alloc(bound * sizeof(SSChunk), s));
+  RVP_LAUNCH("ss_chunksum_k", s, ss_chunksum_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff, nseg, mode, mean_dev,
+                                                                              cs.as<double>()));
+  RVP_LAUNCH("ss_compose_k", s, ss_compose_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff, nseg, mode, mean_dev,
+                                                                            cs.as<double>(), ch.as<SSChunk>()));
+  RVP_LAUNCH("ss_drive_k", s, ss_drive_k<<<nseg, 64, 0, s>>>(v, seg_off, coff, nseg, mode, mean_dev,
+                                                             ch.as<SSChunk>(), out));
+  return RVP_OK;
+}
+
 int exact_seq_sum_seg(const float *v, const uint32_t *seg_off, int nseg, size_t n_bound, int mode,
                       const float *mean_dev, float *out, hipStream_t s) {
   if (nseg <= 0) return RVP_OK;
-  const size_t bound = n_bound / SS_C + (size_t)nseg + 1;  // chunks: sum of ceil(len / 256)
-  Tmp coff, cs, ch;
-  RVP_TRY(coff.alloc((nseg + 1) * sizeof(uint32_t), s));
-  RVP_TRY(cs.alloc(bound * sizeof(double), s));
-  RVP_TRY(ch.alloc(bound * sizeof(SSChunk), s));
-  RVP_LAUNCH("ss_nchunks_k", s, ss_nchunks_k<<<ceil_div(nseg, 256), 256, 0, s>>>(seg_off, nseg, coff.as<uint32_t>()));
-  RVP_TRY(exclusive_scan_u32(coff.as<uint32_t>(), coff.as<uint32_t>(), nseg, coff.as<uint32_t>() + nseg, s));
-  RVP_LAUNCH("ss_chunksum_k", s, ss_chunksum_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff.as<uint32_t>(), nseg,
-                                                                              mode, mean_dev, cs.as<double>()));
-  RVP_LAUNCH("ss_compose_k", s, ss_compose_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff.as<uint32_t>(), nseg,
-                                                                            mode, mean_dev, cs.as<double>(),
-                                                                            ch.as<SSChunk>()));
-  RVP_LAUNCH("ss_drive_k", s, ss_drive_k<<<nseg, 64, 0, s>>>(v, seg_off, coff.as<uint32_t>(), nseg, mode, mean_dev,
-                                                             ch.as<SSChunk>(), out));
-  return RVP_OK;
+  Tmp coff;
+  size_t bound = 0;
+  RVP_TRY(ss_chunk_offsets(seg_off, nseg, n_bound, coff, &bound, s));
+  return ss_run(v, seg_off, coff.as<uint32_t>(), nseg, bound, mode, mean_dev, out, s);
+}
+
+int exact_seq_mean_var_seg(const float *v, const uint32_t *seg_off, int nseg, size_t n_bound, float *sum_out,
+                           float *sq_out, hipStream_t s) {
+  if (nseg <= 0) return RVP_OK;
+  Tmp coff;
+  size_t bound = 0;
+  RVP_TRY(ss_chunk_offsets(seg_off, nseg, n_bound, coff, &bound, s));
+  RVP_TRY(ss_run(v, seg_off, coff.as<uint32_t>(), nseg, bound, 0, nullptr, sum_out, s));
+  return ss_run(v, seg_off, coff.as<uint32_t>(), nseg, bound, 2, sum_out, sq_out, s);
 }
 
 __global__ void one_segment_k(uint32_t n, uint32_t *__restrict__ seg) {
