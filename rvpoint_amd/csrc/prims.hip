@@ -271,6 +271,54 @@ __global__ __launch_bounds__(PRIM_THREADS) void plane_split_k(const float *__res
   }
 }
 
+// Heads of the equal-key runs of sorted keys[0, nn) (nn = min(*n_dev, n)) in one pass: vid[i] = number of
+// runs starting before i (i < n; nn.. get the total), start[r] = first index of run r, *total = run count.
+template <class K>
+__global__ __launch_bounds__(PRIM_THREADS) void run_heads_k(const K *__restrict__ keys, size_t n,
+                                                             const uint32_t *__restrict__ n_dev,
+                                                             uint32_t *__restrict__ vid, uint32_t *__restrict__ start,
+                                                             uint64_t *__restrict__ status,
+                                                             uint32_t *__restrict__ ticket, uint32_t nb,
+                                                             uint32_t epoch, uint32_t *__restrict__ total) {
+  __shared__ uint32_t sm[PRIM_THREADS / WAVE];
+  __shared__ uint32_t s_tile, s_prefix;
+  const int t = threadIdx.x;
+  const uint32_t tile = take_tile(ticket, nb, &s_tile);
+  if (tile >= nb) return;
+  const size_t nn = n_dev ? min((size_t)*n_dev, n) : n;
+  const size_t base = (size_t)tile * SCAN_TILE;
+  uint32_t f[SCAN_ITEMS];
+  uint32_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + (size_t)j * PRIM_THREADS + t;
+    f[j] = (i < nn && (i == 0 || keys[i] != keys[i - 1])) ? 1u : 0u;
+    acc += f[j];
+  }
+  acc = wave_sum(acc);
+  if (lane() == 0) sm[t >> 6] = acc;
+  __syncthreads();
+  const uint32_t agg = sm[0] + sm[1] + sm[2] + sm[3];
+  if (t < WAVE) {
+    const uint32_t prefix = tile_lookback(status, tile, epoch, agg);
+    if (t == 0) {
+      s_prefix = prefix;
+      if (tile == nb - 1) *total = prefix + agg;
+    }
+  }
+  __syncthreads();
+  uint32_t carry = s_prefix;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + (size_t)j * PRIM_THREADS + t;
+    uint32_t bt;
+    const uint32_t ex = carry + block_excl_scan_256(f[j], bt, sm);
+    if (i < n) vid[i] = ex;
+    if (f[j]) start[ex] = (uint32_t)i;
+    carry += bt;
+  }
+}
+
 // per-(device, stream) look-back state: status words, ticket counter, epoch
 struct ScanState {
   uint64_t *status = nullptr;
@@ -338,6 +386,32 @@ int plane_split_aos(const float *x, const float *y, const float *z, size_t n, co
   return RVP_OK;
 }
 
+template <class K>
+static int run_heads_impl(const K *keys, size_t n, const uint32_t *n_dev, uint32_t *vid, uint32_t *start,
+                          uint32_t *total_dev, hipStream_t s) {
+  if (n == 0) {
+    RVP_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), s));
+    return RVP_OK;
+  }
+  RVP_CHECK(n < 0xffffffffull, RVP_E_UNSUPPORTED, "run heads: more than 2^32-1 keys");
+  const size_t nb = ceil_div(n, SCAN_TILE);
+  uint64_t *status = nullptr;
+  uint32_t *ticket = nullptr, epoch = 0;
+  RVP_TRY(scan_state(s, nb, &status, &ticket, &epoch));
+  RVP_LAUNCH("run_heads_k", s, run_heads_k<K><<<(unsigned)nb, PRIM_THREADS, 0, s>>>(keys, n, n_dev, vid, start, status,
+                                                                                  ticket, (uint32_t)nb, epoch,
+                                                                                  total_dev));
+  return RVP_OK;
+}
+int run_heads(const uint32_t *keys, size_t n, const uint32_t *n_dev, uint32_t *vid, uint32_t *start,
+              uint32_t *total_dev, hipStream_t s) {
+  return run_heads_impl(keys, n, n_dev, vid, start, total_dev, s);
+}
+int run_heads(const uint64_t *keys, size_t n, const uint32_t *n_dev, uint32_t *vid, uint32_t *start,
+              uint32_t *total_dev, hipStream_t s) {
+  return run_heads_impl(keys, n, n_dev, vid, start, total_dev, s);
+}
+
 int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *out, size_t n, uint64_t *total_dev, hipStream_t s) {
   return exclusive_scan_impl<uint64_t>(in, out, n, total_dev, s);
 }
@@ -381,6 +455,7 @@ __device__ __forceinline__ void lds_wave_sync() {
 template <class K>
 __global__ __launch_bounds__(PRIM_THREADS) void radix_hist_all_k(const K *__restrict__ keys, size_t n, int passes,
                                                                   uint32_t *__restrict__ part,
+                                                                  uint32_t *__restrict__ ghist,
                                                                   const uint32_t *__restrict__ n_dev,
                                                                   const int *__restrict__ bits_dev) {
   __shared__ uint32_t h[RS_MAX_PASSES][256];
@@ -388,6 +463,8 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_hist_all_k(const K *__rest
   const int act = active_passes(passes, bits_dev);
   n = dev_count(n, n_dev);
   for (int p = 0; p < act; ++p) h[p][t] = 0;
+  if (blockIdx.x == 0)  // the reduction that follows adds into ghist
+    for (int p = 0; p < act; ++p) ghist[p * 256 + t] = 0;
   __syncthreads();
   for (size_t i0 = (size_t)blockIdx.x * PRIM_THREADS; i0 < n; i0 += (size_t)gridDim.x * PRIM_THREADS) {
     const size_t i = i0 + t;
@@ -595,9 +672,9 @@ static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, siz
   Tmp gh, part;
   RVP_TRY(gh.alloc((size_t)passes * 256 * sizeof(uint32_t), s));
   RVP_TRY(part.alloc((size_t)hb * RS_MAX_PASSES * 256 * sizeof(uint32_t), s));
-  RVP_HIP(hipMemsetAsync(gh.p, 0, (size_t)passes * 256 * sizeof(uint32_t), s));
   RVP_LAUNCH("radix_hist_all_k", s, radix_hist_all_k<K><<<hb, PRIM_THREADS, 0, s>>>(*keys, n, passes,
-                                                                                    part.as<uint32_t>(), n_dev,
+                                                                                    part.as<uint32_t>(),
+                                                                                    gh.as<uint32_t>(), n_dev,
                                                                                     bits_dev));
   RVP_LAUNCH("radix_hist_sum_k", s, radix_hist_sum_k<<<(unsigned)passes * ceil_div(hb, RS_SUM), PRIM_THREADS, 0, s>>>(
                                         part.as<uint32_t>(), hb, n, passes, gh.as<uint32_t>(), n_dev, bits_dev));

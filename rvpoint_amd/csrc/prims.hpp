@@ -28,6 +28,13 @@ int radix_sort_pairs_u64(uint64_t **keys, uint32_t **vals, uint64_t **keys_tmp, 
                          size_t n, int bits, hipStream_t s, const uint32_t *n_dev = nullptr,
                          const int *bits_dev = nullptr);
 
+// Heads of the equal-key runs of sorted keys in one pass: vid[i] (i < n) = runs starting before i,
+// start[r] = first index of run r, *total_dev = run count; n_dev (may be null) bounds the keys on the device.
+int run_heads(const uint32_t *keys, size_t n, const uint32_t *n_dev, uint32_t *vid, uint32_t *start,
+              uint32_t *total_dev, hipStream_t s);
+int run_heads(const uint64_t *keys, size_t n, const uint32_t *n_dev, uint32_t *vid, uint32_t *start,
+              uint32_t *total_dev, hipStream_t s);
+
 // Stable compaction of points whose flag equals `want` into AoS (float[3*m]); *count_dev = m (device).
 int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want,
                     size_t n, float *out_aos, uint32_t *count_dev, hipStream_t s);

@@ -65,25 +65,6 @@ __global__ void voxel_keys_k(const float *__restrict__ x, const float *__restric
   vals[i] = (uint32_t)i;
 }
 
-// n_dev (may be null): device element count <= n, the bound the launch is sized for
-template <class K>
-__global__ void run_flags_k(const K *__restrict__ keys, size_t n, const uint32_t *__restrict__ n_dev,
-                            uint32_t *__restrict__ f) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const size_t nn = n_dev ? min((size_t)*n_dev, n) : n;
-  f[i] = (i < nn && (i == 0 || keys[i] != keys[i - 1])) ? 1u : 0u;
-}
-
-template <class K>
-__global__ void run_starts_k(const K *__restrict__ keys, size_t n, const uint32_t *__restrict__ n_dev,
-                             const uint32_t *__restrict__ vid, uint32_t *__restrict__ start) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t nn = n_dev ? min((size_t)*n_dev, n) : n;
-  if (i >= nn) return;
-  if (i == 0 || keys[i] != keys[i - 1]) start[vid[i]] = (uint32_t)i;
-}
-
 __global__ void voxel_centroid_k(const float *__restrict__ x, const float *__restrict__ y,
                                  const float *__restrict__ z, const uint32_t *__restrict__ perm,
                                  const uint32_t *__restrict__ start, const uint32_t *__restrict__ nvox, uint32_t n_host,
@@ -144,11 +125,8 @@ static int voxel_sorted(const float *x, const float *y, const float *z, size_t n
   else RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n, bits, s));
   RVP_TRY(vid.alloc(n * sizeof(uint32_t), s));
   RVP_TRY(nv.alloc(sizeof(uint32_t), s));
-  RVP_LAUNCH("run_flags_k", s, run_flags_k<K><<<nb, 256, 0, s>>>(kp, n, nullptr, vid.as<uint32_t>()));
-  RVP_TRY(exclusive_scan_u32(vid.as<uint32_t>(), vid.as<uint32_t>(), n, nv.as<uint32_t>(), s));
   RVP_TRY(start.alloc(n * sizeof(uint32_t), s));
-  RVP_LAUNCH("run_starts_k", s, run_starts_k<K><<<nb, 256, 0, s>>>(kp, n, nullptr, vid.as<uint32_t>(),
-                                                                    start.as<uint32_t>()));
+  RVP_TRY(run_heads(kp, n, nullptr, vid.as<uint32_t>(), start.as<uint32_t>(), nv.as<uint32_t>(), s));
   RVP_LAUNCH("voxel_centroid_k", s, voxel_centroid_k<<<nb, 256, 0, s>>>(x, y, z, vp, start.as<uint32_t>(),
                                                                         nv.as<uint32_t>(), (uint32_t)n, nullptr, out));
   uint32_t V = 0;
@@ -729,11 +707,8 @@ int rvp_batch_voxel(const float *x, const float *y, const float *z, const uint32
   RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n_bound, 64, s, n_dev, bits.as<int>()));
   RVP_TRY(vid.alloc(n_bound * sizeof(uint32_t), s));
   RVP_TRY(nv.alloc(sizeof(uint32_t), s));
-  RVP_LAUNCH("run_flags_k", s, run_flags_k<uint64_t><<<nb, 256, 0, s>>>(kp, n_bound, n_dev, vid.as<uint32_t>()));
-  RVP_TRY(exclusive_scan_u32(vid.as<uint32_t>(), vid.as<uint32_t>(), n_bound, nv.as<uint32_t>(), s));
   RVP_TRY(start.alloc(n_bound * sizeof(uint32_t), s));
-  RVP_LAUNCH("run_starts_k", s, run_starts_k<uint64_t><<<nb, 256, 0, s>>>(kp, n_bound, n_dev, vid.as<uint32_t>(),
-                                                                          start.as<uint32_t>()));
+  RVP_TRY(run_heads(kp, n_bound, n_dev, vid.as<uint32_t>(), start.as<uint32_t>(), nv.as<uint32_t>(), s));
   RVP_LAUNCH("voxel_centroid_k", s, voxel_centroid_k<<<nb, 256, 0, s>>>(x, y, z, vp, start.as<uint32_t>(),
                                                                         nv.as<uint32_t>(), (uint32_t)n_bound, n_dev,
                                                                         out_aos));
