@@ -171,6 +171,7 @@ __global__ __launch_bounds__(256) void seg_knn_cell_k(const uint32_t *__restrict
                                                        double div, float *__restrict__ cell) {
   const int c = blockIdx.x;
   uint32_t cnt = 0;
+#pragma unroll 16
   for (int j = threadIdx.x; j < OCC_BITS / 4; j += 256) {
     uint32_t w = words[(size_t)c * (OCC_BITS / 4) + j];  // four cells: count the nonzero bytes
     w |= w >> 1;

@@ -291,15 +291,11 @@ static int plane_counts_dev(const float *x, const float *y, const float *z, size
 // batched V1 and R1: clouds [off[c], off[c+1]) of one SoA batch, counts on the device, no host sync
 // except the final read of the RANSAC counts.
 // ------------------------------------------------------------------------------------------------
-__global__ void seg_vox_init_k(int ncl, int *__restrict__ vmm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 6 * ncl) vmm[i] = (i % 6) < 3 ? INT_MAX : INT_MIN;
-}
-
-// voxel coordinate bounds per cloud (blockIdx.y = cloud); inv = fl(1 / leaf) as voxel_grid_downsamp.cpp:189
+// voxel coordinate bounds per cloud (blockIdx.y = cloud), one partial per block in pmm[c][block][6];
+// inv = fl(1 / leaf) as voxel_grid_downsamp.cpp:189
 __global__ __launch_bounds__(256) void seg_vox_minmax_k(const float *__restrict__ x, const float *__restrict__ y,
                                                          const float *__restrict__ z, const uint32_t *__restrict__ off,
-                                                         const float *__restrict__ leaf, int *__restrict__ vmm) {
+                                                         const float *__restrict__ leaf, int *__restrict__ pmm) {
   const int c = blockIdx.y;
   const float inv = __fdiv_rn(1.0f, leaf[c]);
   int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
@@ -321,9 +317,31 @@ __global__ __launch_bounds__(256) void seg_vox_minmax_k(const float *__restrict_
   __syncthreads();
   if (threadIdx.x < 3) {
     const int j = threadIdx.x;
-    atomicMin(&vmm[6 * c + j], min(min(sm[0][j], sm[1][j]), min(sm[2][j], sm[3][j])));
-    atomicMax(&vmm[6 * c + j + 3], max(max(sm[0][j + 3], sm[1][j + 3]), max(sm[2][j + 3], sm[3][j + 3])));
+    int *p = pmm + ((size_t)c * gridDim.x + blockIdx.x) * 6;
+    p[j] = min(min(sm[0][j], sm[1][j]), min(sm[2][j], sm[3][j]));
+    p[j + 3] = max(max(sm[0][j + 3], sm[1][j + 3]), max(sm[2][j + 3], sm[3][j + 3]));
   }
+}
+
+// the per-block partials of cloud c (blockIdx.x) -> vmm[c][6]
+__global__ __launch_bounds__(64) void seg_vox_reduce_k(const int *__restrict__ pmm, int per, int *__restrict__ vmm) {
+  const int c = blockIdx.x, l = lane();
+  int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+  for (int b = l; b < per; b += WAVE) {
+    const int *p = pmm + ((size_t)c * per + b) * 6;
+    for (int j = 0; j < 3; ++j) {
+      v[j] = min(v[j], p[j]);
+      v[j + 3] = max(v[j + 3], p[j + 3]);
+    }
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1)
+    for (int j = 0; j < 3; ++j) {
+      v[j] = min(v[j], __shfl_xor(v[j], d));
+      v[j + 3] = max(v[j + 3], __shfl_xor(v[j + 3], d));
+    }
+  if (l == 0)
+    for (int j = 0; j < 6; ++j) vmm[6 * c + j] = v[j];
 }
 
 // key space of each cloud: rx * ry * rz voxels (1 for an empty cloud or one whose range overflows 2^48,
@@ -681,16 +699,17 @@ int rvp_batch_voxel(const float *x, const float *y, const float *z, const uint32
   RVP_TRY(G.offs.alloc((nclouds + 1) * sizeof(uint32_t), s));
   RVP_TRY(seg_clamp_offsets(offsets, nclouds, n_bound, G.offs.as<uint32_t>(), s));
   const uint32_t *off = G.offs.as<uint32_t>();
-  const unsigned b6 = ceil_div(6 * nclouds, 256);
-  Tmp vmm, ryz, base, bits, keys, vals, kt, vt, vid, start, nv;
+  Tmp vmm, pmm, ryz, base, bits, keys, vals, kt, vt, vid, start, nv;
+  const unsigned per = (unsigned)std::min<size_t>(256, std::max<size_t>(1, n_bound / ((size_t)nclouds * 2048)));
   RVP_TRY(vmm.alloc(6 * nclouds * sizeof(int), s));
+  RVP_TRY(pmm.alloc((size_t)6 * nclouds * per * sizeof(int), s));
   RVP_TRY(ryz.alloc(2 * nclouds * sizeof(uint64_t), s));
   RVP_TRY(base.alloc((nclouds + 1) * sizeof(uint64_t), s));
   RVP_TRY(bits.alloc(sizeof(int), s));
-  RVP_LAUNCH("seg_vox_init_k", s, seg_vox_init_k<<<b6, 256, 0, s>>>(nclouds, vmm.as<int>()));
-  const unsigned per = (unsigned)std::min<size_t>(64, std::max<size_t>(1, n_bound / ((size_t)nclouds * 4096)));
   RVP_LAUNCH("seg_vox_minmax_k", s, seg_vox_minmax_k<<<dim3(per, nclouds), 256, 0, s>>>(x, y, z, off, leaf,
-                                                                                     vmm.as<int>()));
+                                                                                     pmm.as<int>()));
+  RVP_LAUNCH("seg_vox_reduce_k", s, seg_vox_reduce_k<<<nclouds, 64, 0, s>>>(pmm.as<int>(), (int)per,
+                                                                            vmm.as<int>()));
   RVP_LAUNCH("seg_vox_base_k", s, seg_vox_base_k<<<1, 64, 0, s>>>(nclouds, off, vmm.as<int>(), ryz.as<uint64_t>(),
                                                                   base.as<uint64_t>(), bits.as<int>(), status));
   RVP_TRY(keys.alloc(n_bound * sizeof(uint64_t), s));
