@@ -537,13 +537,16 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict
     }
     return;
   }
+  // only the tiles the device-side count needs take tickets (the ticket counter wraps after exactly that
+  // many); the launch's other blocks leave at once instead of queueing on the counter
+  const uint32_t nb_act = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+  if (blockIdx.x >= nb_act) return;
   const uint32_t gcount = ghist[pass * 256 + t];  // issued before the ticket round trip
 #pragma unroll
   for (int q = 0; q < 4; ++q) wcnt[w][l + 64 * q] = 0;
-  const uint32_t tile = take_tile(ticket, nb, &s_tile);
-  if (tile >= nb) return;
+  const uint32_t tile = take_tile(ticket, nb_act, &s_tile);
+  if (tile >= nb_act) return;
   const size_t base = (size_t)tile * RS_TILE;
-  if (base >= n) return;  // past the device-side count: nothing to rank, and no later tile looks at it
   const int shift = 8 * pass;
   // rank: wave w owns keys [base + w*512, base + (w+1)*512), in index order
   K k[RS_ITEMS];
