@@ -168,7 +168,7 @@ int rvp_euclidean_cluster(const rvp_hash *h, float tolerance, int min_size, int 
   RVP_TRY(vt.alloc(n * sizeof(uint32_t), s));
   RVP_LAUNCH("uf_init_k", s, uf_init_k<<<nb, 256, 0, s>>>((uint32_t)n, vals.as<uint32_t>()));
   uint32_t *kp = label.as<uint32_t>(), *vp = vals.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
-  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n, bits_for(n - 1), s));
+  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n, bits_for(n - 1), s, nullptr, nullptr, n >= (1u << 19)));
   RVP_TRY(keep.alloc(n * sizeof(uint32_t), s));
   RVP_TRY(kpos.alloc(n * sizeof(uint32_t), s));
   RVP_TRY(first.alloc(n * sizeof(uint32_t), s));

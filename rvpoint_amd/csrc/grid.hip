@@ -274,7 +274,7 @@ int Grid::build_from(const float mm[6], float cell_, hipStream_t s) {
   RVP_LAUNCH("keys_k", s, keys_k<<<nb, 256, 0, s>>>(x, y, z, n, bb[0], bb[1], bb[2], cell, dims[0], dims[1],
                                                     dims[2], keys.as<uint32_t>(), perm.as<uint32_t>()));
   uint32_t *kp = keys.as<uint32_t>(), *vp = perm.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
-  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n, bits_for(ncells - 1), s));
+  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n, bits_for(ncells - 1), s, nullptr, nullptr, n >= (1u << 19)));
   if (kp != keys.as<uint32_t>()) {  // odd pass count: the sorted data sits in the temporaries
     RVP_HIP(hipMemcpyAsync(keys.p, kp, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     RVP_HIP(hipMemcpyAsync(perm.p, vp, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));

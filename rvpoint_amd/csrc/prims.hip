@@ -430,7 +430,7 @@ unsigned long long scan_spin_errors() {
 // and writes them out in runs.  Status words are the scan's: (epoch, flag, value) in one relaxed 64-bit
 // word, tiles ordered by a ticket so a tile waits only on tiles already running.
 // ------------------------------------------------------------------------------------------------
-constexpr int RS_ITEMS = 8;
+constexpr int RS_ITEMS = 8;                       // keys per thread of a pass tile (16 for large sorts)
 constexpr int RS_TILE = PRIM_THREADS * RS_ITEMS;  // 2048 keys per tile
 constexpr int RS_MAX_PASSES = 8;
 constexpr int RS_LOOKBACK = 8;  // predecessor status words one look-back step reads at once
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_hist_sum_k(const uint32_t 
   if (acc) atomicAdd(&ghist[p * 256 + t], acc);
 }
 
-template <class K>
+template <class K, int ITEMS>
 __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                               K *__restrict__ kout, uint32_t *__restrict__ vout,
                                                               size_t n, int pass, int passes,
@@ -516,13 +516,14 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict
                                                               uint32_t *__restrict__ ticket, uint32_t nb,
                                                               uint32_t epoch, const uint32_t *__restrict__ n_dev,
                                                               const int *__restrict__ bits_dev) {
+  constexpr int TILE = PRIM_THREADS * ITEMS;
   __shared__ uint32_t s_tile;
   __shared__ uint32_t wcnt[PRIM_THREADS / WAVE][256];  // per-wave digit counters, then per-wave offsets
   __shared__ uint32_t lstart[256];                     // first slot of digit d in the staged tile
   __shared__ uint32_t gbase[256];                      // global position of staged slot 0 of digit d's run
   __shared__ uint32_t sm[PRIM_THREADS / WAVE];
-  __shared__ K skey[RS_TILE];
-  __shared__ uint32_t sval[RS_TILE];
+  __shared__ K skey[TILE];
+  __shared__ uint32_t sval[TILE];
   const int t = threadIdx.x, w = t >> 6, l = lane();
   n = dev_count(n, n_dev);
   const int act = active_passes(passes, bits_dev);
@@ -539,28 +540,28 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict
   }
   // only the tiles the device-side count needs take tickets (the ticket counter wraps after exactly that
   // many); the launch's other blocks leave at once instead of queueing on the counter
-  const uint32_t nb_act = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+  const uint32_t nb_act = (uint32_t)((n + TILE - 1) / TILE);
   if (blockIdx.x >= nb_act) return;
   const uint32_t gcount = ghist[pass * 256 + t];  // issued before the ticket round trip
 #pragma unroll
   for (int q = 0; q < 4; ++q) wcnt[w][l + 64 * q] = 0;
   const uint32_t tile = take_tile(ticket, nb_act, &s_tile);
   if (tile >= nb_act) return;
-  const size_t base = (size_t)tile * RS_TILE;
+  const size_t base = (size_t)tile * TILE;
   const int shift = 8 * pass;
   // rank: wave w owns keys [base + w*512, base + (w+1)*512), in index order
-  K k[RS_ITEMS];
-  uint32_t v[RS_ITEMS], r[RS_ITEMS];
-  const size_t wb = base + (size_t)w * (RS_ITEMS * WAVE);
+  K k[ITEMS];
+  uint32_t v[ITEMS], r[ITEMS];
+  const size_t wb = base + (size_t)w * (ITEMS * WAVE);
 #pragma unroll
-  for (int j = 0; j < RS_ITEMS; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     const size_t i = wb + (size_t)j * WAVE + l;
     const bool ok = i < n;
     k[j] = ok ? kin[i] : (K)0;
     v[j] = ok ? vin[i] : 0u;
   }
 #pragma unroll
-  for (int j = 0; j < RS_ITEMS; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     const bool ok = wb + (size_t)j * WAVE + l < n;
     const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
     uint64_t peers = ballot(ok);
@@ -642,7 +643,7 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict
   __syncthreads();
   // stage in digit order, then write runs
 #pragma unroll
-  for (int j = 0; j < RS_ITEMS; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     if (wb + (size_t)j * WAVE + l < n) {
       const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
       const uint32_t s = lstart[d] + wcnt[w][d] + r[j];
@@ -651,9 +652,9 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict
     }
   }
   __syncthreads();
-  const uint32_t m = (uint32_t)min((size_t)RS_TILE, n - base);
+  const uint32_t m = (uint32_t)min((size_t)TILE, n - base);
 #pragma unroll
-  for (int j = 0; j < RS_ITEMS; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     const uint32_t s = (uint32_t)(j * PRIM_THREADS + t);
     if (s < m) {
       const K kk = skey[s];
@@ -666,12 +667,14 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict
 
 template <class K>
 static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, size_t n, int bits, hipStream_t s,
-                           const uint32_t *n_dev, const int *bits_dev) {
+                           const uint32_t *n_dev, const int *bits_dev, bool large_tiles) {
   if (n <= 1 || bits <= 0) return RVP_OK;
   RVP_CHECK(n < 0xffffffffull, RVP_E_UNSUPPORTED, "radix sort: more than 2^32-1 elements");
   const int passes = std::min<int>((bits + 7) / 8, (int)sizeof(K));
-  const unsigned nb = ceil_div(n, RS_TILE);
-  const unsigned hb = std::min(nb * 2, 1024u);
+  const unsigned hb = std::min(ceil_div(n, RS_TILE) * 2, 1024u);
+  // large tiles: half the tiles, so half the look-back chain and its status traffic (worth it from about
+  // half a million keys; below that the longer tile is the critical path)
+  const unsigned nb = ceil_div(n, (size_t)PRIM_THREADS * (large_tiles ? 2 * RS_ITEMS : RS_ITEMS));
   Tmp gh, part;
   RVP_TRY(gh.alloc((size_t)passes * 256 * sizeof(uint32_t), s));
   RVP_TRY(part.alloc((size_t)hb * RS_MAX_PASSES * 256 * sizeof(uint32_t), s));
@@ -685,9 +688,14 @@ static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, siz
     uint64_t *status = nullptr;
     uint32_t *ticket = nullptr, epoch = 0;
     RVP_TRY(scan_state(s, (size_t)nb * 256, &status, &ticket, &epoch));
-    RVP_LAUNCH("radix_pass_k", s, radix_pass_k<K><<<nb, PRIM_THREADS, 0, s>>>(*keys, *vals, *kt, *vt, n, p, passes,
-                                                                            gh.as<uint32_t>(), status, ticket, nb,
-                                                                            epoch, n_dev, bits_dev));
+    if (large_tiles)
+      RVP_LAUNCH("radix_pass_k", s, radix_pass_k<K, 2 * RS_ITEMS><<<nb, PRIM_THREADS, 0, s>>>(
+                                        *keys, *vals, *kt, *vt, n, p, passes, gh.as<uint32_t>(), status, ticket, nb,
+                                        epoch, n_dev, bits_dev));
+    else
+      RVP_LAUNCH("radix_pass_k", s, radix_pass_k<K, RS_ITEMS><<<nb, PRIM_THREADS, 0, s>>>(
+                                        *keys, *vals, *kt, *vt, n, p, passes, gh.as<uint32_t>(), status, ticket, nb,
+                                        epoch, n_dev, bits_dev));
     std::swap(*keys, *kt);
     std::swap(*vals, *vt);
   }
@@ -695,12 +703,12 @@ static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, siz
 }
 
 int radix_sort_pairs_u32(uint32_t **keys, uint32_t **vals, uint32_t **kt, uint32_t **vt, size_t n, int bits,
-                         hipStream_t s, const uint32_t *n_dev, const int *bits_dev) {
-  return radix_sort_impl<uint32_t>(keys, vals, kt, vt, n, bits, s, n_dev, bits_dev);
+                         hipStream_t s, const uint32_t *n_dev, const int *bits_dev, bool large_tiles) {
+  return radix_sort_impl<uint32_t>(keys, vals, kt, vt, n, bits, s, n_dev, bits_dev, large_tiles);
 }
 int radix_sort_pairs_u64(uint64_t **keys, uint32_t **vals, uint64_t **kt, uint32_t **vt, size_t n, int bits,
-                         hipStream_t s, const uint32_t *n_dev, const int *bits_dev) {
-  return radix_sort_impl<uint64_t>(keys, vals, kt, vt, n, bits, s, n_dev, bits_dev);
+                         hipStream_t s, const uint32_t *n_dev, const int *bits_dev, bool large_tiles) {
+  return radix_sort_impl<uint64_t>(keys, vals, kt, vt, n, bits, s, n_dev, bits_dev, large_tiles);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -20,13 +20,14 @@ int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *out, size_t n, uint64_t
 // On return the sorted data is in (*keys, *vals): the function swaps the pointers with the temporaries
 // when an odd number of passes leaves it there.  Device-sized form: n_dev (may be null) holds the element
 // count (<= n, the bound the launches are sized for); bits_dev (may be null) the key width, passes above
-// it only copy.  Neither is read on the host.
+// it only copy.  Neither is read on the host.  large_tiles: 4096-key tiles instead of 2048, for callers whose
+// n is about the real count and at least half a million.
 int radix_sort_pairs_u32(uint32_t **keys, uint32_t **vals, uint32_t **keys_tmp, uint32_t **vals_tmp,
                          size_t n, int bits, hipStream_t s, const uint32_t *n_dev = nullptr,
-                         const int *bits_dev = nullptr);
+                         const int *bits_dev = nullptr, bool large_tiles = false);
 int radix_sort_pairs_u64(uint64_t **keys, uint32_t **vals, uint64_t **keys_tmp, uint32_t **vals_tmp,
                          size_t n, int bits, hipStream_t s, const uint32_t *n_dev = nullptr,
-                         const int *bits_dev = nullptr);
+                         const int *bits_dev = nullptr, bool large_tiles = false);
 
 // Heads of the equal-key runs of sorted keys in one pass: vid[i] (i < n) = runs starting before i,
 // start[r] = first index of run r, *total_dev = run count; n_dev (may be null) bounds the keys on the device.

@@ -360,7 +360,7 @@ int rvp_sort_pairs_u32(uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint3
   RVP_CHECK(bits >= 0 && bits <= 32, RVP_E_ARG, "bits must be in [0, 32]");
   RVP_TRY(pool_init());
   uint32_t *k = keys, *v = vals, *kt = keys_tmp, *vt = vals_tmp;
-  RVP_TRY(radix_sort_pairs_u32(&k, &v, &kt, &vt, n, bits, S(s), n_dev, bits_dev));
+  RVP_TRY(radix_sort_pairs_u32(&k, &v, &kt, &vt, n, bits, S(s), n_dev, bits_dev, !n_dev && n >= (1u << 19)));
   *in_tmp = k != keys;
   return RVP_OK;
 }
@@ -372,7 +372,7 @@ int rvp_sort_pairs_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_tmp, uint3
   RVP_TRY(pool_init());
   uint64_t *k = keys, *kt = keys_tmp;
   uint32_t *v = vals, *vt = vals_tmp;
-  RVP_TRY(radix_sort_pairs_u64(&k, &v, &kt, &vt, n, bits, S(s), n_dev, bits_dev));
+  RVP_TRY(radix_sort_pairs_u64(&k, &v, &kt, &vt, n, bits, S(s), n_dev, bits_dev, !n_dev && n >= (1u << 19)));
   *in_tmp = k != keys;
   return RVP_OK;
 }

@@ -121,8 +121,9 @@ static int voxel_sorted(const float *x, const float *y, const float *z, size_t n
   RVP_LAUNCHED();
   K *kp = keys.as<K>(), *ktp = kt.as<K>();
   uint32_t *vp = vals.as<uint32_t>(), *vtp = vt.as<uint32_t>();
-  if constexpr (sizeof(K) == 4) RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n, bits, s));
-  else RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n, bits, s));
+  const bool large = n >= (1u << 19);
+  if constexpr (sizeof(K) == 4) RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n, bits, s, nullptr, nullptr, large));
+  else RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n, bits, s, nullptr, nullptr, large));
   RVP_TRY(vid.alloc(n * sizeof(uint32_t), s));
   RVP_TRY(nv.alloc(sizeof(uint32_t), s));
   RVP_TRY(start.alloc(n * sizeof(uint32_t), s));
@@ -723,7 +724,8 @@ int rvp_batch_voxel(const float *x, const float *y, const float *z, const uint32
   uint64_t *kp = keys.as<uint64_t>(), *ktp = kt.as<uint64_t>();
   uint32_t *vp = vals.as<uint32_t>(), *vtp = vt.as<uint32_t>();
   const uint32_t *n_dev = off + nclouds;
-  RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n_bound, 64, s, n_dev, bits.as<int>()));
+  // every input point has a key: the bound is the count, so large sorts take the large tiles
+  RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n_bound, 64, s, n_dev, bits.as<int>(), n_bound >= (1u << 19)));
   RVP_TRY(vid.alloc(n_bound * sizeof(uint32_t), s));
   RVP_TRY(nv.alloc(sizeof(uint32_t), s));
   RVP_TRY(start.alloc(n_bound * sizeof(uint32_t), s));
