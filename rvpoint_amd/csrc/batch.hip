@@ -256,15 +256,18 @@ __global__ void seg_keys_k(const float *__restrict__ x, const float *__restrict_
                                 (uint32_t)d.g[1] * (uint32_t)cell_clamped(z[i], d.mn[2], d.cell, d.g[2]));
 }
 
+// cell-order SoA, and the inverse permutation rank[perm[i]] = i
 __global__ void seg_gather_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
                              const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off, int ncl,
-                             float *__restrict__ xs, float *__restrict__ ys, float *__restrict__ zs) {
+                             float *__restrict__ xs, float *__restrict__ ys, float *__restrict__ zs,
+                             uint32_t *__restrict__ rank) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= off[ncl]) return;
   const uint32_t p = perm[i];
   xs[i] = x[p];
   ys[i] = y[p];
   zs[i] = z[p];
+  rank[p] = i;
 }
 
 struct SegGap {
@@ -387,8 +390,10 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_TRY(xs.alloc(n_bound * sizeof(float), s));
   RVP_TRY(ys.alloc(n_bound * sizeof(float), s));
   RVP_TRY(zs.alloc(n_bound * sizeof(float), s));
+  RVP_TRY(rank.alloc(n_bound * sizeof(uint32_t), s));
   RVP_LAUNCH("seg_gather_k", s, seg_gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), off, ncl,
-                                                                xs.as<float>(), ys.as<float>(), zs.as<float>()));
+                                                                xs.as<float>(), ys.as<float>(), zs.as<float>(),
+                                                                rank.as<uint32_t>()));
   RVP_TRY(cstart.alloc((table + 1) * sizeof(uint32_t), s));
   Tmp gaps, ng;
   RVP_TRY(gaps.alloc((n_bound + 2 + table / SEG_GAP_CHUNK + 2) * sizeof(SegGap), s));

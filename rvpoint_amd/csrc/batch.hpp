@@ -17,6 +17,7 @@ struct SegDesc {
 struct SegArrays {
   const float *xs, *ys, *zs;
   const uint32_t *perm;
+  const uint32_t *rank;  // batch position -> cell-order position
   const uint32_t *cstart;
   const SegDesc *desc;
   const uint32_t *off;  // [ncl + 1]
@@ -40,6 +41,7 @@ __device__ __forceinline__ GridView seg_view(const SegArrays &A, int c) {
   v.ys = A.ys;
   v.zs = A.zs;
   v.perm = A.perm;
+  v.rank = A.rank;
   v.cell_start = A.cstart + d.toff;
   v.mnx = d.mn[0];
   v.mny = d.mn[1];
@@ -104,14 +106,14 @@ int seg_clamp_offsets(const uint32_t *in, int ncl, size_t n_bound, uint32_t *out
 
 // Host owner of a segmented grid (stream-ordered temporaries).
 struct SegGrid {
-  Tmp desc, mm, cellv, ncells, toff, xs, ys, zs, perm, keys, cstart, offs;
+  Tmp desc, mm, cellv, ncells, toff, xs, ys, zs, perm, rank, keys, cstart, offs;
   int ncl = 0;
   const uint32_t *off = nullptr;  // the caller's offsets made non-decreasing, off[0] = 0, off[ncl] <= n_bound
   size_t n_bound = 0;
 
   SegArrays arrays() const {
-    return SegArrays{xs.as<float>(), ys.as<float>(), zs.as<float>(), perm.as<uint32_t>(), cstart.as<uint32_t>(),
-                     desc.as<SegDesc>(), off, ncl};
+    return SegArrays{xs.as<float>(), ys.as<float>(), zs.as<float>(), perm.as<uint32_t>(), rank.as<uint32_t>(),
+                     cstart.as<uint32_t>(), desc.as<SegDesc>(), off, ncl};
   }
 
   // Build the grids of the ncl clouds [off[c], off[c+1]) of x/y/z (device offsets, n_bound >= off[ncl]).

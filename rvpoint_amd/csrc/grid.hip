@@ -135,15 +135,17 @@ __global__ void keys_k(const float *__restrict__ x, const float *__restrict__ y,
   vals[i] = (uint32_t)i;
 }
 
+// cell-order SoA, and the inverse permutation rank[perm[i]] = i
 __global__ void gather_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
                          const uint32_t *__restrict__ perm, size_t n, float *__restrict__ xs,
-                         float *__restrict__ ys, float *__restrict__ zs) {
+                         float *__restrict__ ys, float *__restrict__ zs, uint32_t *__restrict__ rank) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t p = perm[i];
   xs[i] = x[p];
   ys[i] = y[p];
   zs[i] = z[p];
+  rank[p] = (uint32_t)i;
 }
 
 // cell_start[c] = first sorted position whose key >= c, for c in [0, ncells]; short gaps are filled by the
@@ -219,6 +221,7 @@ int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, fl
     RVP_TRY(ys.alloc(4, s, persistent));
     RVP_TRY(zs.alloc(4, s, persistent));
     RVP_TRY(perm.alloc(4, s, persistent));
+    RVP_TRY(rank.alloc(4, s, persistent));
     RVP_TRY(occ.alloc(sizeof(uint32_t), s, persistent));
     RVP_HIP(hipMemsetAsync(occ.p, 0, sizeof(uint32_t), s));
     built = true;
@@ -282,8 +285,9 @@ int Grid::build_from(const float mm[6], float cell_, hipStream_t s) {
   RVP_TRY(xs.alloc(n * sizeof(float), s, persistent));
   RVP_TRY(ys.alloc(n * sizeof(float), s, persistent));
   RVP_TRY(zs.alloc(n * sizeof(float), s, persistent));
+  RVP_TRY(rank.alloc(n * sizeof(uint32_t), s, persistent));
   RVP_LAUNCH("gather_k", s, gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), n, xs.as<float>(),
-                                                        ys.as<float>(), zs.as<float>()));
+                                                        ys.as<float>(), zs.as<float>(), rank.as<uint32_t>()));
 
   RVP_TRY(cstart.alloc((ncells + 1) * sizeof(uint32_t), s, persistent));
   // long gaps are at most one per run start plus the tail: n + 1 entries always suffice
@@ -363,6 +367,7 @@ GridView Grid::view() const {
   v.ys = ys.as<float>();
   v.zs = zs.as<float>();
   v.perm = perm.as<uint32_t>();
+  v.rank = rank.as<uint32_t>();
   v.cell_start = cstart.as<uint32_t>();
   v.mnx = bb[0];
   v.mny = bb[1];

@@ -11,6 +11,7 @@ namespace rvp {
 struct GridView {
   const float *xs, *ys, *zs;   // points in cell order (SoA)
   const uint32_t *perm;        // cell-order position -> original index
+  const uint32_t *rank;        // original index -> cell-order position (the inverse of perm)
   const uint32_t *cell_start;  // [ncells + 1]: points of cell c are [cell_start[c], cell_start[c+1])
   float mnx, mny, mnz;         // grid origin = bbox min after the 0.01*cell margin (spatial_hashing.cpp:141-143)
   float cell;                  // cell edge of this grid (>= the requested one, see Grid::build)
@@ -60,7 +61,7 @@ struct Grid {
   uint64_t ncells = 0;
   bool built = false;
   bool persistent = false;  // owned by an rvp_hash: hipMalloc'd buffers that outlive the build stream
-  Tmp xs, ys, zs, perm, keys, cstart, occ;
+  Tmp xs, ys, zs, perm, rank, keys, cstart, occ;
   hipStream_t stream = nullptr;
 
   // Build from device SoA.  The device grid uses the requested cell unless its dense table would exceed

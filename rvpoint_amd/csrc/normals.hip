@@ -241,17 +241,18 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
       TopK t;
       t.init(K);
       if (MODE == 0) {
-        knn_search<true>(g, x, y, z, t, L);
+        knn_search<false>(g, x, y, z, t, L);
       } else {
         int lo[3], hi[3];
         window3(g, x, y, z, radius, lo, hi);
-        TopKSink<true> sink{t, L};
+        TopKSink<false> sink{t, L};
         scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
-        topk_flush<true>(t, L);
+        topk_flush<false>(t, L);
       }
       const int m = min(t.cnt, K);
       float c[9];
-      wave_cov(g, lane() < m, t.pay, m, x, y, z, c);
+      // the list holds (d2, index) keys only: the neighbour's cell-order position is rank[index]
+      wave_cov(g, lane() < m, lane() < m ? g.rank[key_idx(t.key)] : 0u, m, x, y, z, c);
       stash(N, j, c, x, y, z, g.perm[q], m);
     }
     solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks, cx, cy, cz);
@@ -344,18 +345,18 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
       TopK t;
       t.init(K);
       if (MODE == 0) {
-        knn_search<true>(g, x, y, z, t, lds[w]);
+        knn_search<false>(g, x, y, z, t, lds[w]);
       } else {
         const float r = radius[c];
         int lo[3], hi[3];
         window3(g, x, y, z, r, lo, hi);
-        TopKSink<true> sink{t, lds[w]};
+        TopKSink<false> sink{t, lds[w]};
         scan_box(g, x, y, z, lo, hi, false, lo, hi, __fmul_rn(r, r), true, lds[w], sink);
-        topk_flush<true>(t, lds[w]);
+        topk_flush<false>(t, lds[w]);
       }
       const int m = min(t.cnt, K);
       float c6[9];
-      wave_cov(g, lane() < m, t.pay, m, x, y, z, c6);
+      wave_cov(g, lane() < m, lane() < m ? g.rank[key_idx(t.key)] : 0u, m, x, y, z, c6);
       stash(N, j, c6, x, y, z, g.perm[q], m);
     }
     solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
