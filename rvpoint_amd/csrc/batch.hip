@@ -71,9 +71,9 @@ __global__ __launch_bounds__(256) void seg_minmax_k(const float *__restrict__ x,
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1)
     for (int j = 0; j < 3; ++j) {
-      const float o = __shfl_xor(v[j], d);
+      const float o = shfl_xor(v[j], d);
       if (o < v[j]) v[j] = o;
-      const float p = __shfl_xor(v[j + 3], d);
+      const float p = shfl_xor(v[j + 3], d);
       if (p > v[j + 3]) v[j + 3] = p;
     }
   __shared__ float sm[4][6];

@@ -172,18 +172,31 @@ __device__ __forceinline__ float dist2(float px, float py, float pz, float qx, f
   return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
 }
 
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
-  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-  lo = __shfl_xor(lo, m);
-  hi = __shfl_xor(hi, m);
+// Lane permutes through ds_bpermute with the byte address computed here (lane index * 4, taken mod 64
+// lanes), without the bounds handling of HIP's __shfl family: every caller's partner lies in the wave.
+__device__ __forceinline__ uint32_t bperm(uint32_t v, int addr) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v);
+}
+__device__ __forceinline__ int bperm(int v, int addr) { return __builtin_amdgcn_ds_bpermute(addr, v); }
+__device__ __forceinline__ float bperm(float v, int addr) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ uint64_t bperm(uint64_t v, int addr) {
+  const uint32_t lo = bperm((uint32_t)v, addr), hi = bperm((uint32_t)(v >> 32), addr);
   return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
-  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-  lo = __shfl(lo, src);
-  hi = __shfl(hi, src);
-  return ((uint64_t)hi << 32) | lo;
+__device__ __forceinline__ double bperm(double v, int addr) {
+  return __longlong_as_double((long long)bperm((uint64_t)__double_as_longlong(v), addr));
 }
+__device__ __forceinline__ long long bperm(long long v, int addr) { return (long long)bperm((uint64_t)v, addr); }
+// byte address of lane l ^ m, of lane l - d, of lane l + d (the last two wrap around the wave)
+__device__ __forceinline__ int xor_addr(int m) { return (lane() ^ m) << 2; }
+__device__ __forceinline__ int up_addr(int d) { return ((lane() - d) << 2) & 0xfc; }
+__device__ __forceinline__ int down_addr(int d) { return ((lane() + d) << 2) & 0xfc; }
+template <class T> __device__ __forceinline__ T shfl_xor(T v, int m) { return bperm(v, xor_addr(m)); }
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) { return shfl_xor(v, m); }
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) { return bperm(v, src << 2); }
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
   uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
   uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
@@ -200,7 +213,7 @@ template <class T> __device__ __forceinline__ T wave_incl_scan(T v) {
   const int l = lane();
 #pragma unroll
   for (int d = 1; d < WAVE; d <<= 1) {
-    T t = __shfl_up(v, d);
+    const T t = bperm(v, up_addr(d));
     if (l >= d) v += t;
   }
   return v;
@@ -208,7 +221,7 @@ template <class T> __device__ __forceinline__ T wave_incl_scan(T v) {
 
 template <class T> __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
-  for (int d = WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  for (int d = WAVE / 2; d > 0; d >>= 1) v += shfl_xor(v, d);
   return v;
 }
 

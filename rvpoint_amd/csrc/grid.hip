@@ -34,9 +34,9 @@ __global__ __launch_bounds__(256) void bbox_partial_k(const float *__restrict__ 
   for (int d = 32; d > 0; d >>= 1) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      float o = __shfl_xor(v[j], d);
+      float o = shfl_xor(v[j], d);
       if (o < v[j]) v[j] = o;
-      float p = __shfl_xor(v[j + 3], d);
+      float p = shfl_xor(v[j + 3], d);
       if (p > v[j + 3]) v[j + 3] = p;
     }
   }

@@ -52,8 +52,6 @@ template <class KT> struct TopKT {
 };
 using TopK = TopKT<uint64_t>;
 
-__device__ __forceinline__ uint64_t xor_lanes(uint64_t v, int m) { return shfl_xor64(v, m); }
-__device__ __forceinline__ uint32_t xor_lanes(uint32_t v, int m) { return (uint32_t)__shfl_xor((int)v, m); }
 __device__ __forceinline__ uint64_t read_lane(uint64_t v, int l) { return readlane64(v, l); }
 __device__ __forceinline__ uint32_t read_lane(uint32_t v, int l) { return readlaneu(v, l); }
 __device__ __forceinline__ float kd2(uint64_t k) { return key_d2(k); }
@@ -61,48 +59,76 @@ __device__ __forceinline__ float kd2(uint32_t k) { return __uint_as_float(k); }
 // the TopK key of a packed (d2, idx) key
 template <class KT> __device__ __forceinline__ KT as_key(uint64_t k) { return (KT)(k >> (64 - 8 * sizeof(KT))); }
 
-// all compare-exchanges keep the minimum at the lower lane ("mirror" bitonic formulation)
+// lanes whose lane index has bit b set (b a power of two below 64)
+__host__ __device__ constexpr uint64_t lanes_with(int b) {
+  return b == 1    ? 0xAAAAAAAAAAAAAAAAull
+         : b == 2  ? 0xCCCCCCCCCCCCCCCCull
+         : b == 4  ? 0xF0F0F0F0F0F0F0F0ull
+         : b == 8  ? 0xFF00FF00FF00FF00ull
+         : b == 16 ? 0xFFFF0000FFFF0000ull
+                   : 0xFFFFFFFF00000000ull;
+}
+
+// Compare-exchange with lane l ^ m: lanes of `upper` keep the larger key, the others the smaller ("mirror"
+// bitonic formulation).  The comparison is one wave-wide mask, flipped for the upper lanes by a scalar xor.
+// Without a payload, equal keys may be taken from either side (they are the same value); with one, each
+// side keeps its own entry on a tie.
 template <bool PAY, class KT>
-__device__ __forceinline__ void cx(KT &k, uint32_t &p, int partner_mask, bool lower) {
-  const KT ok = xor_lanes(k, partner_mask);
-  if (!PAY && sizeof(KT) == 4) {
-    k = lower ? min(k, ok) : max(k, ok);
-    return;
-  }
-  uint32_t op = PAY ? (uint32_t)__shfl_xor((int)p, partner_mask) : 0u;
-  bool take = lower ? (ok < k) : (ok > k);
-  if (take) {
+__device__ __forceinline__ void cx(KT &k, uint32_t &p, int m, uint64_t upper) {
+  const int a = xor_addr(m);
+  const KT ok = bperm(k, a);
+  uint64_t take;
+  if (PAY) take = (ballot(ok < k) & ~upper) | (ballot(ok > k) & upper);
+  else take = ballot(ok < k) ^ upper;
+  const uint32_t op = PAY ? bperm(p, a) : 0u;
+  if (__builtin_amdgcn_inverse_ballot_w64(take)) {
     k = ok;
     if (PAY) p = op;
   }
 }
 
-// sorts every group of `size` lanes ascending (size: power of two <= 64, wave-uniform)
-template <bool PAY, class KT> __device__ __forceinline__ void bitonic_sort_groups(KT &k, uint32_t &p, int size) {
-  const int l = lane();
+// sorts every group of SIZE lanes ascending
+template <bool PAY, int SIZE, class KT> __device__ __forceinline__ void bitonic_sort_n(KT &k, uint32_t &p) {
 #pragma unroll
-  for (int s = 2; s <= WAVE; s <<= 1) {
-    if (s > size) break;
-    cx<PAY>(k, p, s - 1, (l & (s >> 1)) == 0);
+  for (int s = 2; s <= SIZE; s <<= 1) {
+    cx<PAY>(k, p, s - 1, lanes_with(s >> 1));
 #pragma unroll
-    for (int j = s >> 2; j > 0; j >>= 1) cx<PAY>(k, p, j, (l & j) == 0);
+    for (int j = s >> 2; j > 0; j >>= 1) cx<PAY>(k, p, j, lanes_with(j));
   }
 }
 
+// sorts a bitonic sequence held in each group of SIZE lanes
+template <bool PAY, int SIZE, class KT> __device__ __forceinline__ void bitonic_clean_n(KT &k, uint32_t &p) {
+#pragma unroll
+  for (int j = SIZE / 2; j > 0; j >>= 1) cx<PAY>(k, p, j, lanes_with(j));
+}
+
+// sorts every group of `size` lanes ascending (size: power of two <= 64, wave-uniform)
+template <bool PAY, class KT> __device__ __forceinline__ void bitonic_sort_groups(KT &k, uint32_t &p, int size) {
+  if (size > 32) bitonic_sort_n<PAY, 64>(k, p);
+  else if (size > 16) bitonic_sort_n<PAY, 32>(k, p);
+  else if (size > 8) bitonic_sort_n<PAY, 16>(k, p);
+  else if (size > 4) bitonic_sort_n<PAY, 8>(k, p);
+  else if (size > 2) bitonic_sort_n<PAY, 4>(k, p);
+  else if (size > 1) bitonic_sort_n<PAY, 2>(k, p);
+}
+
 template <bool PAY, class KT> __device__ __forceinline__ void bitonic_sort64(KT &k, uint32_t &p) {
-  bitonic_sort_groups<PAY>(k, p, WAVE);
+  bitonic_sort_n<PAY, WAVE>(k, p);
 }
 
 // sorts a bitonic sequence held in each group of `size` lanes
 template <bool PAY, class KT> __device__ __forceinline__ void bitonic_clean_groups(KT &k, uint32_t &p, int size) {
-  const int l = lane();
-#pragma unroll
-  for (int j = WAVE / 2; j > 0; j >>= 1)
-    if (j < size) cx<PAY>(k, p, j, (l & j) == 0);
+  if (size > 32) bitonic_clean_n<PAY, 64>(k, p);
+  else if (size > 16) bitonic_clean_n<PAY, 32>(k, p);
+  else if (size > 8) bitonic_clean_n<PAY, 16>(k, p);
+  else if (size > 4) bitonic_clean_n<PAY, 8>(k, p);
+  else if (size > 2) bitonic_clean_n<PAY, 4>(k, p);
+  else if (size > 1) bitonic_clean_n<PAY, 2>(k, p);
 }
 
 template <bool PAY, class KT> __device__ __forceinline__ void bitonic_clean64(KT &k, uint32_t &p) {
-  bitonic_clean_groups<PAY>(k, p, WAVE);
+  bitonic_clean_n<PAY, WAVE>(k, p);
 }
 
 __device__ __forceinline__ int pow2_at_least(int v) {
@@ -130,8 +156,9 @@ template <bool PAY, class KT> __device__ __forceinline__ void topk_flush(TopKT<K
     t.pay = np;
   } else {
     // lane l < W takes new[W-1-l] = new[l ^ (W-1)] (the batch's W smallest, reversed)
-    KT rk = xor_lanes(nk, W - 1);
-    uint32_t rp = PAY ? (uint32_t)__shfl_xor((int)np, W - 1) : 0u;
+    const int ra = xor_addr(W - 1);
+    KT rk = bperm(nk, ra);
+    uint32_t rp = PAY ? bperm(np, ra) : 0u;
     if (l >= W) rk = ~(KT)0;
     if (rk < t.key) {
       t.key = rk;
@@ -314,7 +341,8 @@ __device__ __forceinline__ float wave_sum_comp(float v, float e_in = 0.0f) {
   float s = v, e = e_in;
 #pragma unroll
   for (int d = WAVE / 2; d > 0; d >>= 1) {
-    const float so = __shfl_xor(s, d), eo = __shfl_xor(e, d);
+    const int a = xor_addr(d);
+    const float so = bperm(s, a), eo = bperm(e, a);
     float t, err;
     two_sum(s, so, t, err);
     e = __fadd_rn(__fadd_rn(e, eo), err);

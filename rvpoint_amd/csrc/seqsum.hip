@@ -45,7 +45,7 @@ __device__ __forceinline__ float ss_mean(int mode, const float *__restrict__ mea
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
-  for (int d = WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  for (int d = WAVE / 2; d > 0; d >>= 1) v += shfl_xor(v, d);
   return v;
 }
 

@@ -38,8 +38,8 @@ __global__ __launch_bounds__(256) void voxel_minmax_k(const float *__restrict__ 
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1)
     for (int j = 0; j < 3; ++j) {
-      v[j] = min(v[j], __shfl_xor(v[j], d));
-      v[j + 3] = max(v[j + 3], __shfl_xor(v[j + 3], d));
+      v[j] = min(v[j], shfl_xor(v[j], d));
+      v[j + 3] = max(v[j + 3], shfl_xor(v[j + 3], d));
     }
   __shared__ int sm[4][6];
   if (lane() == 0)
@@ -309,8 +309,8 @@ __global__ __launch_bounds__(256) void seg_vox_minmax_k(const float *__restrict_
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1)
     for (int j = 0; j < 3; ++j) {
-      v[j] = min(v[j], __shfl_xor(v[j], d));
-      v[j + 3] = max(v[j + 3], __shfl_xor(v[j + 3], d));
+      v[j] = min(v[j], shfl_xor(v[j], d));
+      v[j + 3] = max(v[j + 3], shfl_xor(v[j + 3], d));
     }
   __shared__ int sm[4][6];
   if (lane() == 0)
@@ -338,8 +338,8 @@ __global__ __launch_bounds__(64) void seg_vox_reduce_k(const int *__restrict__ p
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1)
     for (int j = 0; j < 3; ++j) {
-      v[j] = min(v[j], __shfl_xor(v[j], d));
-      v[j + 3] = max(v[j + 3], __shfl_xor(v[j + 3], d));
+      v[j] = min(v[j], shfl_xor(v[j], d));
+      v[j + 3] = max(v[j + 3], shfl_xor(v[j + 3], d));
     }
   if (l == 0)
     for (int j = 0; j < 6; ++j) vmm[6 * c + j] = v[j];
