@@ -13,40 +13,45 @@ namespace rvp {
 template <bool SELF>
 __global__ __launch_bounds__(QT) void radius_topk_k(GridView g, const float *__restrict__ qx,
                                                      const float *__restrict__ qy, const float *__restrict__ qz,
-                                                     uint32_t nq, float r, int K, int32_t *__restrict__ idx_out,
-                                                     float *__restrict__ d2_out, int32_t *__restrict__ counts,
-                                                     int32_t *__restrict__ total) {
+                                                     uint32_t nq, int per_wave, float r, int K,
+                                                     int32_t *__restrict__ idx_out, float *__restrict__ d2_out,
+                                                     int32_t *__restrict__ counts, int32_t *__restrict__ total) {
   __shared__ WaveLds lds[QW];
   const int w = threadIdx.x >> 6;
-  const uint32_t q = blockIdx.x * QW + w;
-  if (q >= nq) return;
   WaveLds &L = lds[w];
-  float x, y, z;
-  uint32_t row;
-  if (SELF) {
-    x = g.xs[q]; y = g.ys[q]; z = g.zs[q];
-    row = g.perm[q];
-  } else {
-    x = qx[q]; y = qy[q]; z = qz[q];
-    row = q;
+  // this wave's queries [q0, q0 + nb): lane j loads query q0 + j and computes its window once
+  const uint64_t q0 = (uint64_t)(blockIdx.x * QW + w) * (uint64_t)per_wave;
+  if (q0 >= nq) return;
+  const int nb = (int)min<uint64_t>((uint64_t)per_wave, nq - q0);
+  LaneQuery mine;
+  if (lane() < nb) {
+    const uint32_t q = (uint32_t)q0 + (uint32_t)lane();
+    if (SELF) mine.load(g, g.xs[q], g.ys[q], g.zs[q], g.perm[q], r);
+    else mine.load(g, qx[q], qy[q], qz[q], q, r);
   }
   const float r2 = __fmul_rn(r, r);
-  int lo[3], hi[3];
-  window3(g, x, y, z, r, lo, hi);
-  TopK t;
-  t.init(K);
-  TopKSink<false> sink{t, L};
-  scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
-  topk_flush<false>(t, L);
-  const int m = min(t.cnt, K);
-  const int l = lane();
-  if (l < m) {
-    if (idx_out) idx_out[(size_t)row * K + l] = (int32_t)key_idx(t.key);
-    if (d2_out) d2_out[(size_t)row * K + l] = key_d2(t.key);
+  int32_t my_count = 0, my_total = 0;
+  for (int j = 0; j < nb; ++j) {
+    const LaneQuery qj = mine.of_lane(j);
+    TopK t;
+    t.init(K);
+    TopKSink<false> sink{t, L};
+    scan_box(g, qj.x, qj.y, qj.z, qj.lo, qj.hi, false, qj.lo, qj.hi, r2, true, L, sink);
+    topk_flush<false>(t, L);
+    const int m = min(t.cnt, K);
+    const int l = lane();
+    if (l < m) {
+      if (idx_out) idx_out[(size_t)qj.row * K + l] = (int32_t)key_idx(t.key);
+      if (d2_out) d2_out[(size_t)qj.row * K + l] = key_d2(t.key);
+    }
+    if (l == j) {
+      my_count = m;
+      my_total = (int32_t)t.hits;
+    }
   }
-  if (l == 0) {
-    counts[row] = m;
-    if (total) total[row] = (int32_t)t.hits;
+  if (lane() < nb) {
+    counts[mine.row] = my_count;
+    if (total) total[mine.row] = my_total;
   }
 }
 
@@ -456,10 +461,11 @@ int rvp_radius_search(const rvp_hash *h, const float *qx, const float *qy, const
   hipStream_t s = S(st);
   const GridView g = h->g.view();
   if (max_nn <= MAX_TOPK) {
-    const unsigned nb = ceil_div(nq, QW);
-    if (!qx) RVP_LAUNCH("radius_topk_k", s, radius_topk_k<true><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, radius, max_nn, idx_out, d2_out,
+    const int pw = queries_per_wave(nq);
+    const unsigned nb = ceil_div(ceil_div(nq, pw), QW);
+    if (!qx) RVP_LAUNCH("radius_topk_k", s, radius_topk_k<true><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, pw, radius, max_nn, idx_out, d2_out,
                                                    counts, total));
-    else RVP_LAUNCH("radius_topk_k", s, radius_topk_k<false><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, radius, max_nn, idx_out, d2_out,
+    else RVP_LAUNCH("radius_topk_k", s, radius_topk_k<false><<<nb, QT, 0, s>>>(g, qx, qy, qz, (uint32_t)nq, pw, radius, max_nn, idx_out, d2_out,
                                                 counts, total));
     RVP_LAUNCHED();
   } else {

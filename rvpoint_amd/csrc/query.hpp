@@ -9,6 +9,8 @@
 // the order is the reference's (d2, idx) pair order (spatial_hashing.cpp:203-209).
 #pragma once
 
+#include <algorithm>
+
 #include "grid.hpp"
 
 namespace rvp {
@@ -87,20 +89,23 @@ __device__ __forceinline__ void cx(KT &k, uint32_t &p, int m, uint64_t upper) {
   }
 }
 
-// sorts every group of SIZE lanes ascending
+constexpr int log2i(int v) { return v <= 1 ? 0 : 1 + log2i(v >> 1); }
+
+// sorts every group of SIZE lanes ascending (counted loops, so that both unroll completely)
 template <bool PAY, int SIZE, class KT> __device__ __forceinline__ void bitonic_sort_n(KT &k, uint32_t &p) {
 #pragma unroll
-  for (int s = 2; s <= SIZE; s <<= 1) {
+  for (int a = 1; a <= log2i(SIZE); ++a) {
+    const int s = 1 << a;
     cx<PAY>(k, p, s - 1, lanes_with(s >> 1));
 #pragma unroll
-    for (int j = s >> 2; j > 0; j >>= 1) cx<PAY>(k, p, j, lanes_with(j));
+    for (int b = a - 2; b >= 0; --b) cx<PAY>(k, p, 1 << b, lanes_with(1 << b));
   }
 }
 
 // sorts a bitonic sequence held in each group of SIZE lanes
 template <bool PAY, int SIZE, class KT> __device__ __forceinline__ void bitonic_clean_n(KT &k, uint32_t &p) {
 #pragma unroll
-  for (int j = SIZE / 2; j > 0; j >>= 1) cx<PAY>(k, p, j, lanes_with(j));
+  for (int b = log2i(SIZE) - 1; b >= 0; --b) cx<PAY>(k, p, 1 << b, lanes_with(1 << b));
 }
 
 // sorts every group of `size` lanes ascending (size: power of two <= 64, wave-uniform)
@@ -256,6 +261,42 @@ __device__ __forceinline__ void scan_box(const GridView &g, float qx, float qy, 
     }
     wave_lds_sync();
   }
+}
+
+// One query of a wave's run of queries.  Lane j of the wave loads query j of the run and computes its window
+// (the per-query set-up, divisions included, done once for the run, lane-parallel); of_lane(j) hands query j to
+// the whole wave (read from lane j: wave-uniform values).
+struct LaneQuery {
+  float x = 0.f, y = 0.f, z = 0.f;
+  uint32_t row = 0;
+  int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+  __device__ __forceinline__ void load(const GridView &g, float qx, float qy, float qz, uint32_t out_row, float r) {
+    x = qx;
+    y = qy;
+    z = qz;
+    row = out_row;
+    window3(g, x, y, z, r, lo, hi);
+  }
+  __device__ __forceinline__ LaneQuery of_lane(int j) const {
+    LaneQuery q;
+    q.x = readlanef(x, j);
+    q.y = readlanef(y, j);
+    q.z = readlanef(z, j);
+    q.row = readlaneu(row, j);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      q.lo[a] = __builtin_amdgcn_readlane(lo[a], j);
+      q.hi[a] = __builtin_amdgcn_readlane(hi[a], j);
+    }
+    return q;
+  }
+};
+
+// queries per wave for a run-of-queries kernel: up to 64, fewer when the count would leave the chip
+// (256 CUs x 32 wave slots) less than two rounds of waves
+inline int queries_per_wave(size_t nq) {
+  const size_t target_waves = 16384;
+  return (int)std::max<size_t>(1, std::min<size_t>(WAVE, nq / target_waves));
 }
 
 template <bool PAY, class KT = uint64_t> struct TopKSink {
