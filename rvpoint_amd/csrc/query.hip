@@ -17,8 +17,10 @@ __global__ __launch_bounds__(QT) void radius_topk_k(GridView g, const float *__r
                                                      int32_t *__restrict__ idx_out, float *__restrict__ d2_out,
                                                      int32_t *__restrict__ counts, int32_t *__restrict__ total) {
   __shared__ WaveLds lds[QW];
+  __shared__ StageLds stage[QW];
   const int w = threadIdx.x >> 6;
   WaveLds &L = lds[w];
+  StageLds &S = stage[w];
   // this wave's queries [q0, q0 + nb): lane j loads query q0 + j and computes its window once
   const uint64_t q0 = (uint64_t)(blockIdx.x * QW + w) * (uint64_t)per_wave;
   if (q0 >= nq) return;
@@ -31,12 +33,15 @@ __global__ __launch_bounds__(QT) void radius_topk_k(GridView g, const float *__r
   }
   const float r2 = __fmul_rn(r, r);
   int32_t my_count = 0, my_total = 0;
+  StagedWindow win;
   for (int j = 0; j < nb; ++j) {
     const LaneQuery qj = mine.of_lane(j);
     TopK t;
     t.init(K);
     TopKSink<false> sink{t, L};
-    scan_box(g, qj.x, qj.y, qj.z, qj.lo, qj.hi, false, qj.lo, qj.hi, r2, true, L, sink);
+    win.ensure(g, qj.lo, qj.hi, L, S);
+    if (win.total >= 0) scan_staged(S, win.total, qj.x, qj.y, qj.z, r2, sink);
+    else scan_box(g, qj.x, qj.y, qj.z, qj.lo, qj.hi, false, qj.lo, qj.hi, r2, true, L, sink);
     topk_flush<false>(t, L);
     const int m = min(t.cnt, K);
     const int l = lane();

@@ -299,6 +299,90 @@ inline int queries_per_wave(size_t nq) {
   return (int)std::max<size_t>(1, std::min<size_t>(WAVE, nq / target_waves));
 }
 
+// A wave's copy of the candidates of one window: x, y, z and the original index (as float bits) of every point
+// of the box, in slice order.  Consecutive queries of a run are mostly in one cell and then have the same
+// window, so one staging serves them all and each query reads its candidates from LDS, 64 per instruction.
+constexpr int STAGE_CAP = 256;
+struct StageLds {
+  float4 c[STAGE_CAP];
+};
+
+// Stage the points of box [lo, hi] into S; returns their count, or -1 when the box has more than 64 (y, z)
+// rows or more than STAGE_CAP points (the caller then walks the box with scan_box).
+__device__ __forceinline__ int stage_box(const GridView &g, const int lo[3], const int hi[3], WaveLds &L,
+                                         StageLds &S) {
+  const int l = lane();
+  const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+  if (ny <= 0 || nz <= 0 || hi[0] < lo[0]) return 0;
+  const int nrows = ny * nz;
+  if (nrows > WAVE) return -1;
+  uint32_t s0 = 0, l0 = 0;
+  if (l < nrows) {
+    const int iy = lo[1] + l % ny, iz = lo[2] + l / ny;
+    const size_t base = ((size_t)iz * (size_t)g.gy + (size_t)iy) * (size_t)g.gx;
+    s0 = g.cell_start[base + lo[0]];
+    l0 = g.cell_start[base + hi[0] + 1] - s0;
+  }
+  const uint32_t inc = wave_incl_scan(l0);
+  const uint32_t total = readlaneu(inc, WAVE - 1);
+  if (total > (uint32_t)STAGE_CAP) return -1;
+  wave_lds_sync();
+  L.excl[l] = inc - l0;  // rows past nrows hold `total`, so the search below never selects them
+  L.s0[l] = s0;
+  wave_lds_sync();
+  for (uint32_t c0 = 0; c0 < total; c0 += WAVE) {
+    const uint32_t v = c0 + (uint32_t)l;
+    if (v < total) {
+      int o = 0;
+#pragma unroll
+      for (int st = WAVE / 2; st > 0; st >>= 1)
+        if (L.excl[o + st] <= v) o += st;
+      const uint32_t pos = L.s0[o] + (v - L.excl[o]);
+      S.c[v] = make_float4(g.xs[pos], g.ys[pos], g.zs[pos], __uint_as_float(g.perm[pos]));
+    }
+  }
+  wave_lds_sync();
+  return (int)total;
+}
+
+// The staged counterpart of scan_box in radius mode: sink(valid, hit, key, 0) for each group of 64 candidates.
+template <class Sink>
+__device__ __forceinline__ void scan_staged(const StageLds &S, int total, float qx, float qy, float qz, float r2,
+                                            Sink &sink) {
+  for (int c0 = 0; c0 < total; c0 += WAVE) {
+    const int v = c0 + lane();
+    const bool valid = v < total;
+    float d2 = 0.0f;
+    uint32_t idx = 0;
+    bool hit = false;
+    if (valid) {
+      const float4 p = S.c[v];
+      d2 = dist2(p.x, p.y, p.z, qx, qy, qz);
+      idx = __float_as_uint(p.w);
+      hit = d2 <= r2;
+    }
+    sink(valid, hit, pack_key(d2, idx), 0u);
+  }
+}
+
+// Window cache of a run of queries: restages only when a query's window differs from the staged one.
+struct StagedWindow {
+  int lo[3] = {1, 1, 1}, hi[3] = {0, 0, 0};
+  int total = -2;  // -2: nothing staged; -1: window too large to stage
+  __device__ __forceinline__ void ensure(const GridView &g, const int qlo[3], const int qhi[3], WaveLds &L,
+                                         StageLds &S) {
+    if (total != -2 && qlo[0] == lo[0] && qlo[1] == lo[1] && qlo[2] == lo[2] && qhi[0] == hi[0] &&
+        qhi[1] == hi[1] && qhi[2] == hi[2])
+      return;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = qlo[a];
+      hi[a] = qhi[a];
+    }
+    total = stage_box(g, lo, hi, L, S);
+  }
+};
+
 template <bool PAY, class KT = uint64_t> struct TopKSink {
   TopKT<KT> &t;
   WaveLds &L;
