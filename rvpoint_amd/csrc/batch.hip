@@ -461,32 +461,52 @@ __global__ void seg_out_offsets_k(const uint32_t *__restrict__ off, int ncl, uin
 
 // H3 per cloud: capped radius lists (cloud-local indices), radius per cloud
 __global__ __launch_bounds__(QT) void seg_radius_topk_k(SegArrays A, const float *__restrict__ radius, int K,
-                                                         int32_t *__restrict__ idx_out, float *__restrict__ d2_out,
-                                                         int32_t *__restrict__ counts) {
+                                                         int per_wave, int32_t *__restrict__ idx_out,
+                                                         float *__restrict__ d2_out, int32_t *__restrict__ counts) {
   __shared__ WaveLds lds[QW];
+  __shared__ StageLds stage[QW];
   const int w = threadIdx.x >> 6;
+  WaveLds &L = lds[w];
+  StageLds &S = stage[w];
   const uint32_t nq = A.off[A.ncl];
-  for (uint32_t q = wave_id(); q < nq; q += wave_count()) {
-    const int c = cloud_of(A.off, A.ncl, q);
-    const GridView g = seg_view(A, c);
-    const float r = radius[c];
-    const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
-    int lo[3], hi[3];
-    window3(g, x, y, z, r, lo, hi);
-    TopK t;
-    t.init(K);
-    TopKSink<false> sink{t, lds[w]};
-    scan_box(g, x, y, z, lo, hi, false, lo, hi, r * r, true, lds[w], sink);
-    topk_flush<false>(t, lds[w]);
-    const int m = min(t.cnt, K);
-    const uint32_t row = g.perm[q];
-    const uint32_t base = A.off[c];
-    const int l = lane();
-    if (l < m) {
-      if (idx_out) idx_out[(size_t)row * K + l] = (int32_t)(key_idx(t.key) - base);
-      if (d2_out) d2_out[(size_t)row * K + l] = key_d2(t.key);
+  const uint64_t runs = ((uint64_t)nq + (uint64_t)per_wave - 1) / (uint64_t)per_wave;
+  const int l = lane();
+  // runs of per_wave consecutive cell-order positions, strided over the launch's waves
+  for (uint64_t run = wave_id(); run < runs; run += wave_count()) {
+    const uint64_t q0 = run * (uint64_t)per_wave;
+    const int nb = (int)min<uint64_t>((uint64_t)per_wave, (uint64_t)nq - q0);
+    LaneQuery mine;
+    int mc = 0;
+    if (l < nb) {
+      const uint32_t q = (uint32_t)q0 + (uint32_t)l;
+      mc = cloud_of(A.off, A.ncl, q);
+      const GridView g = seg_view(A, mc);
+      mine.load(g, g.xs[q], g.ys[q], g.zs[q], g.perm[q], radius[mc]);
     }
-    if (l == 0) counts[row] = m;
+    StagedWindow win;
+    int32_t my_count = 0;
+    for (int j = 0; j < nb; ++j) {
+      const LaneQuery qj = mine.of_lane(j);
+      const int c = __builtin_amdgcn_readlane(mc, j);
+      const GridView g = seg_view(A, c);
+      const float r = radius[c];
+      const float r2 = __fmul_rn(r, r);
+      TopK t;
+      t.init(K);
+      TopKSink<false> sink{t, L};
+      win.ensure(g, qj.lo, qj.hi, c, L, S);
+      if (win.total >= 0) scan_staged(S, win.total, qj.x, qj.y, qj.z, r2, sink);
+      else scan_box(g, qj.x, qj.y, qj.z, qj.lo, qj.hi, false, qj.lo, qj.hi, r2, true, L, sink);
+      topk_flush<false>(t, L);
+      const int m = min(t.cnt, K);
+      const uint32_t base = A.off[c];
+      if (l < m) {
+        if (idx_out) idx_out[(size_t)qj.row * K + l] = (int32_t)(key_idx(t.key) - base);
+        if (d2_out) d2_out[(size_t)qj.row * K + l] = key_d2(t.key);
+      }
+      if (l == j) my_count = m;
+    }
+    if (l < nb) counts[mine.row] = my_count;
   }
 }
 
@@ -547,8 +567,9 @@ int rvp_batch_radius_search(const float *x, const float *y, const float *z, cons
   hipStream_t s = S(st);
   SegGrid G;
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, false, 0, cell ? cell : radius, s));
-  RVP_LAUNCH("seg_radius_topk_k", s, seg_radius_topk_k<<<query_blocks(n_bound), QT, 0, s>>>(
-                                         G.arrays(), radius, max_nn, idx_out, d2_out, counts));
+  const int pw = queries_per_wave(n_bound);
+  RVP_LAUNCH("seg_radius_topk_k", s, seg_radius_topk_k<<<query_blocks(ceil_div(n_bound, pw)), QT, 0, s>>>(
+                                         G.arrays(), radius, max_nn, pw, idx_out, d2_out, counts));
   return RVP_OK;
 }
 

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(QT) void radius_topk_k(GridView g, const float *__r
     TopK t;
     t.init(K);
     TopKSink<false> sink{t, L};
-    win.ensure(g, qj.lo, qj.hi, L, S);
+    win.ensure(g, qj.lo, qj.hi, 0, L, S);
     if (win.total >= 0) scan_staged(S, win.total, qj.x, qj.y, qj.z, r2, sink);
     else scan_box(g, qj.x, qj.y, qj.z, qj.lo, qj.hi, false, qj.lo, qj.hi, r2, true, L, sink);
     topk_flush<false>(t, L);

@@ -366,19 +366,22 @@ __device__ __forceinline__ void scan_staged(const StageLds &S, int total, float 
 }
 
 // Window cache of a run of queries: restages only when a query's window differs from the staged one.
+// `tag` names the grid the window belongs to (the cloud of a batch; 0 for a single grid).
 struct StagedWindow {
   int lo[3] = {1, 1, 1}, hi[3] = {0, 0, 0};
+  int tag = -1;
   int total = -2;  // -2: nothing staged; -1: window too large to stage
-  __device__ __forceinline__ void ensure(const GridView &g, const int qlo[3], const int qhi[3], WaveLds &L,
-                                         StageLds &S) {
-    if (total != -2 && qlo[0] == lo[0] && qlo[1] == lo[1] && qlo[2] == lo[2] && qhi[0] == hi[0] &&
-        qhi[1] == hi[1] && qhi[2] == hi[2])
+  __device__ __forceinline__ void ensure(const GridView &g, const int qlo[3], const int qhi[3], int qtag,
+                                         WaveLds &L, StageLds &S) {
+    if (total != -2 && qtag == tag && qlo[0] == lo[0] && qlo[1] == lo[1] && qlo[2] == lo[2] &&
+        qhi[0] == hi[0] && qhi[1] == hi[1] && qhi[2] == hi[2])
       return;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       lo[a] = qlo[a];
       hi[a] = qhi[a];
     }
+    tag = qtag;
     total = stage_box(g, lo, hi, L, S);
   }
 };
