@@ -121,6 +121,7 @@ __global__ void seg_knn_c0_k(const float *__restrict__ mm, const uint32_t *__res
   }
   SegDesc d{};  // one cell unless a grid below fits: every point then maps to cell 0
   d.cell = 1.0f;
+  d.icell = 1.0f;
   d.g[0] = d.g[1] = d.g[2] = 1;
   if (!(n > 0.0) || !(emax > 0.0) || !(emax < 1e30)) {
     c0[c] = 1.0f;
@@ -145,6 +146,7 @@ __global__ void seg_knn_c0_k(const float *__restrict__ mm, const uint32_t *__res
     e *= 2.0f;
   }
   c0[c] = e;
+  d.icell = __fdiv_rn(1.0f, d.cell);
   d0[c] = d;
 }
 
@@ -224,6 +226,7 @@ __global__ void seg_geom_k(const float *__restrict__ mm, const uint32_t *__restr
       e *= 2.0f;
     }
   }
+  d.icell = __fdiv_rn(1.0f, d.cell);
   desc[c] = d;
   ncells[c] = (uint32_t)d.g[0] * (uint32_t)d.g[1] * (uint32_t)d.g[2];
 }

@@ -10,6 +10,7 @@ namespace rvp {
 struct SegDesc {
   float mn[3];     // grid origin: the cloud's bbox min after the 0.01*edge margin
   float cell;      // grid edge
+  float icell;     // fl(1 / cell)
   int g[3];        // cells per axis
   uint32_t toff;   // first cell in the shared table
 };
@@ -47,6 +48,7 @@ __device__ __forceinline__ GridView seg_view(const SegArrays &A, int c) {
   v.mny = d.mn[1];
   v.mnz = d.mn[2];
   v.cell = d.cell;
+  v.icell = d.icell;
   v.gx = d.g[0];
   v.gy = d.g[1];
   v.gz = d.g[2];

@@ -373,6 +373,7 @@ GridView Grid::view() const {
   v.mny = bb[1];
   v.mnz = bb[2];
   v.cell = cell;
+  v.icell = 1.0f / cell;
   v.gx = dims[0];
   v.gy = dims[1];
   v.gz = dims[2];

@@ -15,6 +15,7 @@ struct GridView {
   const uint32_t *cell_start;  // [ncells + 1]: points of cell c are [cell_start[c], cell_start[c+1])
   float mnx, mny, mnz;         // grid origin = bbox min after the 0.01*cell margin (spatial_hashing.cpp:141-143)
   float cell;                  // cell edge of this grid (>= the requested one, see Grid::build)
+  float icell;                 // fl(1 / cell)
   int gx, gy, gz;              // cells per axis; the linear cell id is ix + gx*(iy + gy*iz) (x fastest)
   uint32_t n;
 };
@@ -28,24 +29,38 @@ __device__ __forceinline__ int cell_clamped(float v, float mn, float c, int g) {
   return (int)f;
 }
 
-// Cells [lo, hi] on one axis that can hold a point p with fl(d2(p, q)) <= fl(r*r).
-// Such a point has |p - q| <= r (1 + 3u) (u = 2^-24); the bounds below are floats strictly beyond
-// q -/+ r (1 + 2^-20), so by monotonicity of cell_clamped every such p lies in [lo, hi].
-__device__ __forceinline__ void window_axis(float q, float r, float mn, float c, int g, int &lo, int &hi) {
-  const double rr = (double)r * (1.0 + 0x1p-20);
-  float bl = (float)((double)q - rr);
-  float bh = (float)((double)q + rr);
-  bl = next_down(bl);
-  bh = next_up(bh);
-  lo = cell_clamped(bl, mn, c, g);
-  hi = cell_clamped(bh, mn, c, g);
+// Cell-unit coordinate of v: fl(fl(v - mn) * icell), icell = fl(1 / cell).  Within 3 * 2^-24 (relative) of the
+// exact (v - mn) / cell.
+__device__ __forceinline__ float cell_coord(float v, float mn, float icell) {
+  return __fmul_rn(__fsub_rn(v, mn), icell);
+}
+
+// floor(f) clamped to [0, g-1] (NaN -> 0)
+__device__ __forceinline__ int floor_clamped(float f, int g) {
+  if (!(f >= 0.0f)) return 0;
+  if (f >= (float)(g - 1)) return g - 1;
+  return (int)f;
+}
+
+// Cells [lo, hi] on one axis that hold every point p within R (1 + 3 * 2^-24) of q, the farthest a point with
+// fl(d2(p, q)) <= fl(R * R) can lie.  u = cell_coord(q), rc = fl(R * icell).  The grid puts p in floor(D),
+// D = fl(fl(p - mn) / cell); the roundings in D, u and rc (relative errors of at most 8 * 2^-24 together) keep
+// D within (|u| + rc) * 2^-21 of [u - rc, u + rc], and the bounds below lie beyond that interval by at least
+// (|u| + rc) * 0.875 * 2^-20 after their own rounding, so by monotonicity of floor every such p's cell is in
+// [lo, hi].  The bounds are conservative: a window may hold one cell layer more than the exact one, never
+// fewer (results do not depend on the window, only the candidates scanned).
+__device__ __forceinline__ void window_axis(float u, float rc, int g, int &lo, int &hi) {
+  const float m = __fmul_rn(__fadd_rn(fabsf(u), rc), 0x1p-20f);
+  lo = floor_clamped(__fsub_rn(__fsub_rn(u, rc), m), g);
+  hi = floor_clamped(__fadd_rn(__fadd_rn(u, rc), m), g);
 }
 
 __device__ __forceinline__ void window3(const GridView &g, float qx, float qy, float qz, float r, int lo[3],
                                         int hi[3]) {
-  window_axis(qx, r, g.mnx, g.cell, g.gx, lo[0], hi[0]);
-  window_axis(qy, r, g.mny, g.cell, g.gy, lo[1], hi[1]);
-  window_axis(qz, r, g.mnz, g.cell, g.gz, lo[2], hi[2]);
+  const float rc = __fmul_rn(r, g.icell);
+  window_axis(cell_coord(qx, g.mnx, g.icell), rc, g.gx, lo[0], hi[0]);
+  window_axis(cell_coord(qy, g.mny, g.icell), rc, g.gy, lo[1], hi[1]);
+  window_axis(cell_coord(qz, g.mnz, g.icell), rc, g.gz, lo[2], hi[2]);
 }
 
 // Host-side owner of a grid's device buffers (stream-ordered pool memory).

@@ -400,8 +400,10 @@ template <bool PAY, class KT = uint64_t> struct TopKSink {
 template <bool PAY, class KT>
 __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy, float qz, TopKT<KT> &t,
                                            WaveLds &L) {
-  const int c[3] = {cell_clamped(qx, g.mnx, g.cell, g.gx), cell_clamped(qy, g.mny, g.cell, g.gy),
-                    cell_clamped(qz, g.mnz, g.cell, g.gz)};
+  // the query's cell up to rounding: any box is a valid start, the certification below makes the result exact
+  const int c[3] = {floor_clamped(cell_coord(qx, g.mnx, g.icell), g.gx),
+                    floor_clamped(cell_coord(qy, g.mny, g.icell), g.gy),
+                    floor_clamped(cell_coord(qz, g.mnz, g.icell), g.gz)};
   const int gd[3] = {g.gx, g.gy, g.gz};
   int lo[3], hi[3];
 #pragma unroll
