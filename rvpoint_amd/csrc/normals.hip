@@ -159,18 +159,16 @@ __device__ __forceinline__ void wave_cov(const GridView &g, bool valid, uint32_t
     pz = __fsub_rn(g.zs[pos], qz);
   }
   const float fm = __int2float_rn(m);
-  const float mx = __fdiv_rn(wave_sum_comp(px), fm);
-  const float my = __fdiv_rn(wave_sum_comp(py), fm);
-  const float mz = __fdiv_rn(wave_sum_comp(pz), fm);
+  const float ps[3] = {px, py, pz}, zero[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float sum[3];
+  wave_sum_comp3(ps, zero, sum);
+  const float mx = __fdiv_rn(sum[0], fm), my = __fdiv_rn(sum[1], fm), mz = __fdiv_rn(sum[2], fm);
   const float dx = valid ? __fsub_rn(px, mx) : 0.f;
   const float dy = valid ? __fsub_rn(py, my) : 0.f;
   const float dz = valid ? __fsub_rn(pz, mz) : 0.f;
-  c[0] = wave_sum_comp(__fmul_rn(dx, dx));
-  c[1] = wave_sum_comp(__fmul_rn(dx, dy));
-  c[2] = wave_sum_comp(__fmul_rn(dx, dz));
-  c[3] = wave_sum_comp(__fmul_rn(dy, dy));
-  c[4] = wave_sum_comp(__fmul_rn(dy, dz));
-  c[5] = wave_sum_comp(__fmul_rn(dz, dz));
+  const float pr[6] = {__fmul_rn(dx, dx), __fmul_rn(dx, dy), __fmul_rn(dx, dz),
+                       __fmul_rn(dy, dy), __fmul_rn(dy, dz), __fmul_rn(dz, dz)};
+  wave_sum_comp6(pr, zero, c);
   c[6] = mx;
   c[7] = my;
   c[8] = mz;
@@ -297,7 +295,8 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
       }
       const float fm = __int2float_rn(m);
       float mu[3];
-      for (int a = 0; a < 3; ++a) mu[a] = __fdiv_rn(wave_sum_comp(s[a], e[a]), fm);
+      wave_sum_comp3(s, e, mu);
+      for (int a = 0; a < 3; ++a) mu[a] = __fdiv_rn(mu[a], fm);
       float cs[6] = {0, 0, 0, 0, 0, 0}, ce[6] = {0, 0, 0, 0, 0, 0};
       for (int c0 = 0; c0 < m; c0 += WAVE) {
         if (c0 + l < m) {
@@ -314,7 +313,7 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
           }
         }
       }
-      for (int a = 0; a < 6; ++a) c[a] = wave_sum_comp(cs[a], ce[a]);
+      wave_sum_comp6(cs, ce, c);
       for (int a = 0; a < 3; ++a) c[6 + a] = mu[a];
     }
     stash(N, j, c, x, y, z, row, m);
