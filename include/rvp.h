@@ -69,7 +69,7 @@ int rvp_profile_enable(int on);
 int rvp_profile_filter(const char *kernel);
 int rvp_profile_collect(char *buf, size_t buflen, size_t *needed);
 /* Process-wide tuning knobs (no counterpart in the reference; results never depend on them, only speed):
- *   "knn_cell_div"         k-NN grid edge = estimated k-th neighbour distance / this (default 2.8, [0.25, 16])
+ *   "knn_cell_div"         k-NN grid edge = estimated k-th neighbour distance / this (default 2.4, [0.25, 16])
  *   "seg_cells_per_point"  dense cells a batched grid may give each point of a cloud before coarsening
  *                          (default 32, [1, 64]; at least 2^17 cells per cloud)
  * set() returns RVP_E_ARG for an unknown name or a value out of range. */

@@ -157,7 +157,7 @@ void prof_stop(const char *name, hipStream_t s, hipEvent_t a) {
   g_prof.push_back(ProfRec{name, a, b});
 }
 
-static std::atomic<double> g_knn_cell_div{2.8};
+static std::atomic<double> g_knn_cell_div{2.4};
 static std::atomic<double> g_seg_cells_per_point{32.0};
 double tune_knn_cell_div() { return g_knn_cell_div.load(); }
 double tune_seg_cells_per_point() { return g_seg_cells_per_point.load(); }
