@@ -293,8 +293,10 @@ struct LaneQuery {
 };
 
 // queries per wave for a run-of-queries kernel: up to 64, fewer when the count would leave the chip
-// (256 CUs x 32 wave slots) less than two rounds of waves
+// (256 CUs x 32 wave slots) less than two rounds of waves; the "queries_per_wave" knob overrides it
 inline int queries_per_wave(size_t nq) {
+  const int forced = tune_queries_per_wave();
+  if (forced > 0) return forced;
   const size_t target_waves = 16384;
   return (int)std::max<size_t>(1, std::min<size_t>(WAVE, nq / target_waves));
 }

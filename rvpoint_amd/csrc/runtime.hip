@@ -159,8 +159,10 @@ void prof_stop(const char *name, hipStream_t s, hipEvent_t a) {
 
 static std::atomic<double> g_knn_cell_div{2.4};
 static std::atomic<double> g_seg_cells_per_point{32.0};
+static std::atomic<int> g_queries_per_wave{0};
 double tune_knn_cell_div() { return g_knn_cell_div.load(); }
 double tune_seg_cells_per_point() { return g_seg_cells_per_point.load(); }
+int tune_queries_per_wave() { return g_queries_per_wave.load(); }
 
 struct PinnedStage::Slot {
   std::mutex mu;
@@ -392,6 +394,17 @@ int rvp_tune_set(const char *name, double value) {
     rvp::g_knn_cell_div.store(value);
     return RVP_OK;
   }
+  if (std::string(name) == "seg_cells_per_point") {
+    RVP_CHECK(value >= 1.0 && value <= 64.0, RVP_E_ARG, "seg_cells_per_point must be in [1, 64]");
+    rvp::g_seg_cells_per_point.store(value);
+    return RVP_OK;
+  }
+  if (std::string(name) == "queries_per_wave") {
+    RVP_CHECK(value >= 0.0 && value <= 64.0 && value == (double)(int)value, RVP_E_ARG,
+              "queries_per_wave must be an integer in [0, 64]");
+    rvp::g_queries_per_wave.store((int)value);
+    return RVP_OK;
+  }
   return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);
 }
 
@@ -403,6 +416,10 @@ int rvp_tune_get(const char *name, double *value) {
   }
   if (std::string(name) == "seg_cells_per_point") {
     *value = rvp::g_seg_cells_per_point.load();
+    return RVP_OK;
+  }
+  if (std::string(name) == "queries_per_wave") {
+    *value = (double)rvp::g_queries_per_wave.load();
     return RVP_OK;
   }
   return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);

@@ -66,3 +66,24 @@ def test_errors_without_device(built):
     h = C.c_void_p()
     assert L.rvp_server_create(C.byref(h), C.byref(FrameParams.lidar()), 0, 1 << 20, 1 << 30) == _lib.RVP_E_ARG
     assert L.rvp_server_create(C.byref(h), C.byref(FrameParams.lidar()), 1, 16, 1 << 30) == _lib.RVP_E_ARG
+
+
+def test_tuning_knobs_set_get_and_ranges(built):
+    from rvpoint_amd import _lib
+    L = _lib.lib()
+    for name, good, bad in (("knn_cell_div", 1.7, 20.0), ("seg_cells_per_point", 8.0, 0.5),
+                            ("queries_per_wave", 7, 65)):
+        old = _lib.tune(name)
+        try:
+            assert _lib.tune(name, good) == good
+            with pytest.raises(_lib.RvpError):
+                _lib.tune(name, bad)
+            assert _lib.tune(name) == good  # a rejected value leaves the knob as it was
+        finally:
+            _lib.tune(name, old)
+    assert _lib.tune("knn_cell_div") == 2.4 and _lib.tune("queries_per_wave") == 0
+    with pytest.raises(_lib.RvpError):
+        _lib.tune("queries_per_wave", 2.5)
+    with pytest.raises(_lib.RvpError):
+        _lib.tune("no_such_knob")
+    assert L.rvp_tune_get(None, None) == _lib.RVP_E_ARG

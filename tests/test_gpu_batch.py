@@ -52,6 +52,34 @@ def test_batch_radius_matches_oracle(clouds):
             assert (ci[i, :m] == oi[i, :m]).all() and (cd[i, :m] == od[i, :m]).all(), (c, i)
 
 
+@pytest.mark.parametrize("per_wave", [1, 64])
+def test_batch_radius_runs_cross_clouds_and_dense_windows(clouds, per_wave):
+    """Runs of consecutive positions that span cloud boundaries (tiny and empty clouds between), one cloud with
+    a dense blob whose windows are too large for one LDS staging."""
+    cl, scales = clouds
+    blob = np.random.default_rng(8).normal(0.3, 0.004, (900, 3)).astype(np.float32)
+    cl = [cl[1], np.concatenate([cl[0], blob]), cl[2], cl[3], cl[5]]
+    sc = [scales[1], scales[0], scales[2], scales[3], scales[5]]
+    b = R.Batch(cl)
+    radius = np.array([0.02 * s for s in sc], np.float32)
+    old = R._lib.tune("queries_per_wave")
+    try:
+        R._lib.tune("queries_per_wave", per_wave)
+        idx, d2, cnt = R.batch_radius_search(b, radius, 48)
+    finally:
+        R._lib.tune("queries_per_wave", old)
+    for c, p in enumerate(cl):
+        if len(p) == 0:
+            continue
+        x, y, z = soa(p)
+        oi, od, oc, _ = O.hash_radius(x, y, z, float(radius[c]), x, y, z, float(radius[c]), 48)
+        ci, cd, cc = b.split(idx)[c], b.split(d2)[c], b.split(cnt)[c]
+        assert (cc == oc).all(), c
+        for i in range(len(p)):
+            m = oc[i]
+            assert (ci[i, :m] == oi[i, :m]).all() and (cd[i, :m] == od[i, :m]).all(), (c, i)
+
+
 def test_batch_normals_equal_single_cloud(clouds):
     cl, _ = clouds
     b = R.Batch(cl)

@@ -64,6 +64,39 @@ def test_radius_capped_external_queries(cell, r, max_nn):
     _check_lists(got, idx, d2, cnt)
 
 
+def dense_blob_cloud():
+    """c0-like surfaces plus a blob of 1200 points within ~0.02 of one spot: its windows hold far more points
+    than one LDS staging takes, the rest stage; consecutive points share windows."""
+    blob = np.random.default_rng(5).normal(0.5, 0.006, (1200, 3)).astype(np.float32)
+    return np.concatenate([c0_small(4000, 21), blob])
+
+
+@pytest.mark.parametrize("per_wave", [0, 1, 7, 64])
+def test_radius_capped_runs_of_queries(per_wave):
+    """Capped lists with each wave serving runs of consecutive queries (knob forced; 0 = from the count), over
+    windows that stage in LDS and windows too large to stage; the cloud's own points and external queries."""
+    p = dense_blob_cloud()
+    x, y, z = soa(p)
+    rng = np.random.default_rng(9)
+    q = np.concatenate([p[::3], rng.uniform(-0.1, 1.1, (200, 3)).astype(np.float32)])
+    h = R.SpatialHash()
+    h.setInputCloud(R.PointCloudSoA(x, y, z), 0.02)
+    h.build()
+    old = R._lib.tune("queries_per_wave")
+    try:
+        R._lib.tune("queries_per_wave", per_wave)
+        for r, mnn in ((0.02, 64), (0.03, 17)):
+            got = h.radius_search_batch(None, r, mnn)
+            idx, d2, cnt, _ = O.hash_radius(x, y, z, 0.02, x, y, z, r, mnn)
+            _check_lists(got, idx, d2, cnt)
+            assert cnt.max() == mnn  # the blob's lists are truncated
+            got = h.radius_search_batch(q, r, mnn)
+            idx, d2, cnt, _ = O.hash_radius(x, y, z, 0.02, *soa(q), r, mnn)
+            _check_lists(got, idx, d2, cnt)
+    finally:
+        R._lib.tune("queries_per_wave", old)
+
+
 @pytest.mark.parametrize("cell,r", [(0.02, 0.02), (0.05, 0.08)])
 def test_radius_unlimited_self(cell, r):
     p = c0_small(3000, 12)
