@@ -247,7 +247,8 @@ int Grid::build_from(const float mm[6], float cell_, hipStream_t s) {
   margin_bbox(mm, cell_, bb);
   // all-NaN cloud: min > max; every point then lands in cell 0 and never matches a query
   const bool degenerate = !(bb[0] <= bb[3] && bb[1] <= bb[4] && bb[2] <= bb[5]);
-  const double limit = std::max<double>(16777216.0, 4.0 * (double)n);
+  // cell ids are 32-bit (sort keys, table offsets): the table stays below 2^32 cells
+  const double limit = std::min<double>(std::max<double>(16777216.0, 4.0 * (double)n), 4294967040.0);
   cell = cell_;
   dims[0] = dims[1] = dims[2] = 1;
   if (!degenerate) {

@@ -80,7 +80,8 @@ struct Grid {
   hipStream_t stream = nullptr;
 
   // Build from device SoA.  The device grid uses the requested cell unless its dense table would exceed
-  // max(2^24, 4n) cells; then the edge doubles until it fits (results do not depend on the edge).
+  // max(2^24, 4n) cells (and always fewer than 2^32); then the edge doubles until it fits (results do not
+  // depend on the edge).
   int build(const float *x, const float *y, const float *z, size_t n, float cell, hipStream_t s);
   // Build with a cell edge chosen for k-nearest-neighbour queries.
   int build_for_knn(const float *x, const float *y, const float *z, size_t n, int k, hipStream_t s);

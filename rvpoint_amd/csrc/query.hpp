@@ -210,12 +210,18 @@ __device__ __forceinline__ void scan_box(const GridView &g, float qx, float qy, 
   const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
   if (ny <= 0 || nz <= 0 || hi[0] < lo[0]) return;
   const int nrows = ny * nz;
+  const float rny = __frcp_rn((float)ny);
+  const bool fq = nrows < (1 << 22);
   for (int r0 = 0; r0 < nrows; r0 += WAVE) {
     const int row = r0 + l;
     uint32_t s0 = 0, l0 = 0, s1 = 0, l1 = 0;
     if (row < nrows) {
-      const int iy = lo[1] + row % ny, iz = lo[2] + row / ny;
-      const size_t base = ((size_t)iz * (size_t)g.gy + (size_t)iy) * (size_t)g.gx;
+      // (row + 0.5) / ny is at least 0.5 / ny from an integer, more than the rounding of the product for
+      // row < 2^22, so the truncation is the exact quotient; larger boxes divide
+      const int qz = fq ? (int)__fmul_rn((float)row + 0.5f, rny) : row / ny;
+      const int iy = lo[1] + (row - qz * ny), iz = lo[2] + qz;
+      // the dense table has fewer than 2^32 cells (Grid::build bounds it)
+      const uint32_t base = ((uint32_t)iz * (uint32_t)g.gy + (uint32_t)iy) * (uint32_t)g.gx;
       if (inner && iy >= ilo[1] && iy <= ihi[1] && iz >= ilo[2] && iz <= ihi[2]) {
         if (ilo[0] > lo[0]) {
           s0 = g.cell_start[base + lo[0]];
@@ -240,13 +246,16 @@ __device__ __forceinline__ void scan_box(const GridView &g, float qx, float qy, 
     L.l0[l] = l0;
     L.s1[l] = s1;
     wave_lds_sync();
+    // rows of this pass: the search below needs only as many halvings as its row count
+    const int pass_rows = min(WAVE, nrows - r0);
+    const int top = pass_rows > 16 ? WAVE / 2 : pass_rows > 4 ? 8 : 2;
     for (uint32_t c0 = 0; c0 < total; c0 += WAVE) {
       const uint32_t v = c0 + (uint32_t)l;
       const bool valid = v < total;
       int o = 0;
 #pragma unroll
       for (int st = WAVE / 2; st > 0; st >>= 1)
-        if (L.excl[o + st] <= v) o += st;
+        if (st <= top && L.excl[o + st] <= v) o += st;
       const uint32_t off = v - L.excl[o];
       const uint32_t pos = off < L.l0[o] ? L.s0[o] + off : L.s1[o] + (off - L.l0[o]);
       float d2 = 0.0f;
@@ -320,8 +329,9 @@ __device__ __forceinline__ int stage_box(const GridView &g, const int lo[3], con
   if (nrows > WAVE) return -1;
   uint32_t s0 = 0, l0 = 0;
   if (l < nrows) {
-    const int iy = lo[1] + l % ny, iz = lo[2] + l / ny;
-    const size_t base = ((size_t)iz * (size_t)g.gy + (size_t)iy) * (size_t)g.gx;
+    const int qz = (int)__fmul_rn((float)l + 0.5f, __frcp_rn((float)ny));  // exact quotient (see scan_box)
+    const int iy = lo[1] + (l - qz * ny), iz = lo[2] + qz;
+    const uint32_t base = ((uint32_t)iz * (uint32_t)g.gy + (uint32_t)iy) * (uint32_t)g.gx;
     s0 = g.cell_start[base + lo[0]];
     l0 = g.cell_start[base + hi[0] + 1] - s0;
   }
@@ -332,13 +342,14 @@ __device__ __forceinline__ int stage_box(const GridView &g, const int lo[3], con
   L.excl[l] = inc - l0;  // rows past nrows hold `total`, so the search below never selects them
   L.s0[l] = s0;
   wave_lds_sync();
+  const int top = nrows > 16 ? WAVE / 2 : nrows > 4 ? 8 : 2;
   for (uint32_t c0 = 0; c0 < total; c0 += WAVE) {
     const uint32_t v = c0 + (uint32_t)l;
     if (v < total) {
       int o = 0;
 #pragma unroll
       for (int st = WAVE / 2; st > 0; st >>= 1)
-        if (L.excl[o + st] <= v) o += st;
+        if (st <= top && L.excl[o + st] <= v) o += st;
       const uint32_t pos = L.s0[o] + (v - L.excl[o]);
       S.c[v] = make_float4(g.xs[pos], g.ys[pos], g.zs[pos], __uint_as_float(g.perm[pos]));
     }
