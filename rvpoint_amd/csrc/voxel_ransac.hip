@@ -480,15 +480,18 @@ __global__ __launch_bounds__(64) void seg_tiles_k(const uint32_t *__restrict__ o
   }
 }
 
+// hypotheses [h_begin, h_end) of every cloud still active (active[c] != 0)
 __global__ __launch_bounds__(256) void seg_plane_count_k(const float *__restrict__ x, const float *__restrict__ y,
                                                           const float *__restrict__ z, const uint32_t *__restrict__ off,
                                                           int ncl, const uint32_t *__restrict__ tile_off,
                                                           const float *__restrict__ models,
-                                                          const int32_t *__restrict__ valid, int H, float thr,
+                                                          const int32_t *__restrict__ valid, int H, int h_begin,
+                                                          int h_end, const uint8_t *__restrict__ active, float thr,
                                                           int32_t *__restrict__ counts) {
   const uint32_t tile = blockIdx.x;
   if (tile >= tile_off[ncl]) return;
   const int c = cloud_of(tile_off, ncl, tile);
+  if (!active[c]) return;
   __shared__ int sc[RC_HYP];
   if (threadIdx.x < RC_HYP) sc[threadIdx.x] = 0;
   __syncthreads();
@@ -503,8 +506,8 @@ __global__ __launch_bounds__(256) void seg_plane_count_k(const float *__restrict
     py[j] = ok[j] ? y[i] : 0.f;
     pz[j] = ok[j] ? z[i] : 0.f;
   }
-  const int h0 = blockIdx.y * RC_HYP;
-  const int hn = min(RC_HYP, H - h0);
+  const int h0 = h_begin + (int)blockIdx.y * RC_HYP;
+  const int hn = min(RC_HYP, h_end - h0);
   const size_t mb = (size_t)c * H;
   for (int hh = 0; hh < hn; ++hh) {
     const size_t h = mb + h0 + hh;
@@ -768,26 +771,62 @@ int rvp_batch_ransac_plane(const float *x, const float *y, const float *z, const
   RVP_LAUNCH("seg_plane_models_k", s, seg_plane_models_k<<<ceil_div(NH, 256), 256, 0, s>>>(
                                           x, y, z, off, nclouds, draws, H, collinear_thresh, models, valid, counts));
   RVP_LAUNCH("seg_tiles_k", s, seg_tiles_k<<<1, 64, 0, s>>>(off, nclouds, tiles.as<uint32_t>(), offs_copy));
-  const dim3 grid((unsigned)(ceil_div(n_bound, RC_TILE) + nclouds), ceil_div((size_t)H, RC_HYP));
-  RVP_LAUNCH("seg_plane_count_k", s, seg_plane_count_k<<<grid, 256, 0, s>>>(
-                                         x, y, z, off, nclouds, tiles.as<uint32_t>(), models, valid, H, dist_thresh,
-                                         counts));
+  // Hypotheses are counted in rounds of 64, 128, ... and replayed after each: the reference's loop usually
+  // stops long before max_iters (adaptively, ransac_plane.cpp:95-105), and a cloud whose loop has stopped
+  // takes no part in later rounds.  The first read-back brings the models, flags and offsets too.
+  Tmp act;
+  RVP_TRY(act.alloc((size_t)nclouds, s));
   PinnedStage stage;
-  uint32_t *host = nullptr;
-  RVP_TRY(stage.get(s, words * sizeof(uint32_t), reinterpret_cast<void **>(&host)));
-  RVP_HIP(hipMemcpyAsync(host, pack.p, words * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  RVP_HIP(hipStreamSynchronize(s));
+  uint8_t *host = nullptr;
+  const size_t act_at = (words * sizeof(uint32_t) + 255) & ~(size_t)255;
+  RVP_TRY(stage.get(s, act_at + (size_t)nclouds, reinterpret_cast<void **>(&host)));
+  uint8_t *hact = host + act_at;
+  std::memset(hact, 1, (size_t)nclouds);
+  RVP_HIP(hipMemcpyAsync(act.p, hact, (size_t)nclouds, hipMemcpyHostToDevice, s));
   const float *hm = reinterpret_cast<const float *>(host);
   const int32_t *hv = reinterpret_cast<const int32_t *>(hm + NH * 4), *hc = hv + NH;
   const uint32_t *ho = reinterpret_cast<const uint32_t *>(hc + NH);
+  std::vector<RansacReplay> rps;
+  int h_begin = 0, chunk = 64;
+  for (;;) {
+    const int h_end = std::min(H, h_begin + chunk);
+    const dim3 grid((unsigned)(ceil_div(n_bound, RC_TILE) + nclouds), ceil_div((size_t)(h_end - h_begin), RC_HYP));
+    RVP_LAUNCH("seg_plane_count_k", s, seg_plane_count_k<<<grid, 256, 0, s>>>(
+                                           x, y, z, off, nclouds, tiles.as<uint32_t>(), models, valid, H, h_begin,
+                                           h_end, act.as<uint8_t>(), dist_thresh, counts));
+    if (h_begin == 0) RVP_HIP(hipMemcpyAsync(host, pack.p, words * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    else RVP_HIP(hipMemcpyAsync(host + NH * 5 * sizeof(int32_t), counts, NH * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    RVP_HIP(hipStreamSynchronize(s));
+    if (h_begin == 0) {
+      rps.reserve(nclouds);
+      for (int c = 0; c < nclouds; ++c) {
+        const size_t n = ho[c + 1] - ho[c];
+        rps.emplace_back(n, max_iters, probability);
+        if (n < 3) rps.back().done = true;  // ransac_plane.cpp:59
+      }
+    }
+    bool more = false;
+    for (int c = 0; c < nclouds; ++c) {
+      RansacReplay &rp = rps[c];
+      if (!rp.done) {
+        const size_t b = (size_t)c * H + h_begin;
+        rp.feed(hm + b * 4, hv + b, hc + b, h_end - h_begin);
+      }
+      hact[c] = rp.done ? 0 : 1;
+      more = more || !rp.done;
+    }
+    if (!more || h_end >= H) break;
+    RVP_HIP(hipMemcpyAsync(act.p, hact, (size_t)nclouds, hipMemcpyHostToDevice, s));
+    h_begin = h_end;
+    chunk = std::min(chunk * 2, 1024);
+  }
   for (int c = 0; c < nclouds; ++c) {
     const size_t n = ho[c + 1] - ho[c];
     for (int k = 0; k < 4; ++k) models_out[4 * c + k] = 0.0f;
     inliers_out[c] = 0;
     if (iterations_out) iterations_out[c] = 0;
-    if (n < 3) continue;  // ransac_plane.cpp:59
-    RansacReplay rp(n, max_iters, probability);
-    rp.feed(hm + (size_t)c * H * 4, hv + (size_t)c * H, hc + (size_t)c * H, max_iters);
+    if (n < 3) continue;
+    const RansacReplay &rp = rps[c];
     for (int k = 0; k < 4; ++k) models_out[4 * c + k] = rp.model[k];
     inliers_out[c] = rp.best;
     if (iterations_out) iterations_out[c] = rp.iterations();
