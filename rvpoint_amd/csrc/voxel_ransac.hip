@@ -223,10 +223,19 @@ __global__ __launch_bounds__(256) void plane_count_k(const float *__restrict__ x
   }
   const int h0 = blockIdx.y * RC_HYP;
   const int hn = min(RC_HYP, H - h0);
+  // the block's models in LDS first: the loop below then reads no global memory
+  __shared__ float4 sm[RC_HYP];
+  __shared__ int sv[RC_HYP];
+  if (threadIdx.x < hn) {
+    const int h = h0 + threadIdx.x;
+    sm[threadIdx.x] = make_float4(models[4 * h], models[4 * h + 1], models[4 * h + 2], models[4 * h + 3]);
+    sv[threadIdx.x] = valid[h];
+  }
+  __syncthreads();
   for (int hh = 0; hh < hn; ++hh) {
-    const int h = h0 + hh;
-    if (!valid[h]) continue;
-    const float a = models[4 * h], b = models[4 * h + 1], c = models[4 * h + 2], d = models[4 * h + 3];
+    if (!sv[hh]) continue;
+    const float4 m = sm[hh];
+    const float a = m.x, b = m.y, c = m.z, d = m.w;
     int cnt = 0;
 #pragma unroll
     for (int j = 0; j < RC_PTS; ++j) {
@@ -509,10 +518,19 @@ __global__ __launch_bounds__(256) void seg_plane_count_k(const float *__restrict
   const int h0 = h_begin + (int)blockIdx.y * RC_HYP;
   const int hn = min(RC_HYP, h_end - h0);
   const size_t mb = (size_t)c * H;
+  // the block's models in LDS first: the loop below then reads no global memory
+  __shared__ float4 sm[RC_HYP];
+  __shared__ int sv[RC_HYP];
+  if (threadIdx.x < hn) {
+    const size_t h = mb + h0 + threadIdx.x;
+    sm[threadIdx.x] = make_float4(models[4 * h], models[4 * h + 1], models[4 * h + 2], models[4 * h + 3]);
+    sv[threadIdx.x] = valid[h];
+  }
+  __syncthreads();
   for (int hh = 0; hh < hn; ++hh) {
-    const size_t h = mb + h0 + hh;
-    if (!valid[h]) continue;
-    const float a = models[4 * h], b = models[4 * h + 1], cc = models[4 * h + 2], d = models[4 * h + 3];
+    if (!sv[hh]) continue;
+    const float4 m = sm[hh];
+    const float a = m.x, b = m.y, cc = m.z, d = m.w;
     int cnt = 0;
 #pragma unroll
     for (int j = 0; j < RC_PTS; ++j) {
