@@ -46,9 +46,33 @@ __device__ __forceinline__ float ord2f(uint32_t u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-__global__ void seg_minmax_init_k(int ncl, uint32_t *__restrict__ mmu) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 6 * ncl) mmu[i] = f2ord((i % 6) < 3 ? FLT_MAX : -FLT_MAX);
+// The first launch of a grid build: the caller's offsets made a valid partition (wave 0 of block 0, as
+// seg_offsets_k), the integer bbox at its identity, `words` 32-bit words of `zero` and the word zero1 cleared.
+__global__ __launch_bounds__(256) void seg_prep_k(const uint32_t *__restrict__ in, int ncl, uint32_t n_bound,
+                                                  uint32_t *__restrict__ off, uint32_t *__restrict__ mmu,
+                                                  uint32_t *__restrict__ zero, size_t words,
+                                                  uint32_t *__restrict__ zero1) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *zero1 = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < WAVE) {
+    const int l = lane();
+    uint32_t carry = 0;
+    for (int b = 0; b <= ncl; b += WAVE) {
+      const int c = b + l;
+      uint32_t v = c == 0 ? 0u : (c <= ncl ? min(in[c], n_bound) : n_bound);
+      v = max(v, carry);
+#pragma unroll
+      for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t o = bperm(v, up_addr(d));
+        if (l >= d) v = max(v, o);
+      }
+      if (c <= ncl) off[c] = v;
+      carry = readlaneu(v, WAVE - 1);
+    }
+  }
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)6 * ncl; i += stride)
+    mmu[i] = f2ord((i % 6) < 3 ? FLT_MAX : -FLT_MAX);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) zero[i] = 0u;
 }
 
 // per-cloud bbox: blockIdx.y = cloud, blockIdx.x strides over its points; strict comparisons as the
@@ -108,11 +132,15 @@ constexpr int OCC_BITS = 1 << 16;  // occupancy cells per cloud (one byte each) 
 constexpr uint32_t SEG_MIN_CELLS = 1u << 17;  // dense-table cells every cloud may use at least
 
 // k-NN edge, step 1: volume-based edge c0 (coarsened until its grid fits the occupancy bitmap)
-__global__ void seg_knn_c0_k(const float *__restrict__ mm, const uint32_t *__restrict__ off, int ncl, int K,
-                             float *__restrict__ c0, SegDesc *__restrict__ d0) {
+__global__ void seg_knn_c0_k(const uint32_t *__restrict__ mmu, float *__restrict__ mm, const uint32_t *__restrict__ off,
+                             int ncl, int K, float *__restrict__ c0, SegDesc *__restrict__ d0) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncl) return;
-  const float *m = mm + 6 * c;
+  float m[6];  // the cloud's bbox (as seg_minmax_final_k), kept in mm for the later stages
+  for (int j = 0; j < 6; ++j) {
+    m[j] = ord2f(mmu[6 * c + j]);
+    mm[6 * c + j] = m[j];
+  }
   const double n = (double)(off[c + 1] - off[c]);
   double ext[3], emax = 0.0, vol = 1.0;
   for (int j = 0; j < 3; ++j) {
@@ -195,16 +223,9 @@ __global__ __launch_bounds__(256) void seg_knn_cell_k(const uint32_t *__restrict
 }
 
 // per cloud: origin (reference margin), dims, coarsened until the cloud's dense table fits its budget
-__global__ void seg_geom_k(const float *__restrict__ mm, const uint32_t *__restrict__ off, int ncl,
-                           const float *__restrict__ cell, double per_point, SegDesc *__restrict__ desc,
-                           uint32_t *__restrict__ ncells) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncl) return;
-  const float *m = mm + 6 * c;
-  const uint32_t n = off[c + 1] - off[c];
+__device__ __forceinline__ SegDesc seg_geom(const float *m, uint32_t n, float req, double per_point) {
   const double budget = fmax((double)SEG_MIN_CELLS, per_point * (double)n);
   SegDesc d{};
-  const float req = cell[c];
   const float mg = req * 0.01f;
   d.mn[0] = m[0] - mg;
   d.mn[1] = m[1] - mg;
@@ -227,8 +248,61 @@ __global__ void seg_geom_k(const float *__restrict__ mm, const uint32_t *__restr
     }
   }
   d.icell = __fdiv_rn(1.0f, d.cell);
+  return d;
+}
+
+__global__ void seg_geom_k(const float *__restrict__ mm, const uint32_t *__restrict__ off, int ncl,
+                           const float *__restrict__ cell, double per_point, SegDesc *__restrict__ desc,
+                           uint32_t *__restrict__ ncells) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  const SegDesc d = seg_geom(mm + 6 * c, off[c + 1] - off[c], cell[c], per_point);
   desc[c] = d;
   ncells[c] = (uint32_t)d.g[0] * (uint32_t)d.g[1] * (uint32_t)d.g[2];
+}
+
+// Up to LAYOUT_MAX clouds in one block: (from the integer bbox when mmu is given) each cloud's geometry, the
+// exclusive scan of the clouds' cell counts into table offsets, and the key width; the work of
+// seg_minmax_final_k, seg_geom_k, the scan and seg_toff_k in one launch.
+constexpr int LAYOUT_MAX = 1024;
+__global__ __launch_bounds__(LAYOUT_MAX) void seg_layout_k(const uint32_t *__restrict__ mmu, float *__restrict__ mm,
+                                                           const uint32_t *__restrict__ off, int ncl,
+                                                           const float *__restrict__ cell, double per_point,
+                                                           SegDesc *__restrict__ desc, uint32_t *__restrict__ ncells,
+                                                           uint32_t *__restrict__ toff, int *__restrict__ key_bits) {
+  const int c = threadIdx.x;
+  SegDesc d{};
+  uint32_t nc = 0;
+  if (c < ncl) {
+    float m[6];
+    for (int j = 0; j < 6; ++j) {
+      m[j] = mmu ? ord2f(mmu[6 * c + j]) : mm[6 * c + j];
+      if (mmu) mm[6 * c + j] = m[j];
+    }
+    d = seg_geom(m, off[c + 1] - off[c], cell[c], per_point);
+    nc = (uint32_t)d.g[0] * (uint32_t)d.g[1] * (uint32_t)d.g[2];
+    ncells[c] = nc;
+  }
+  // block-wide exclusive scan: inclusive per wave, then the wave totals
+  __shared__ uint32_t wt[LAYOUT_MAX / WAVE];
+  const uint32_t inc = wave_incl_scan(nc);
+  const int w = c >> 6;
+  if (lane() == WAVE - 1) wt[w] = inc;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (int i = 0; i < LAYOUT_MAX / WAVE; ++i) {
+    if (i < w) before += wt[i];
+    total += wt[i];
+  }
+  if (c < ncl) {
+    d.toff = before + inc - nc;
+    desc[c] = d;
+    toff[c] = d.toff;
+  }
+  if (c == 0) {
+    toff[ncl] = total;
+    *key_bits = total > 1 ? 32 - __clz((int)(total - 1)) : 0;
+  }
 }
 
 // also the width of the largest cell key, total - 1, for the key sort
@@ -331,7 +405,6 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_CHECK(n_bound > 0, RVP_E_ARG, "n_bound must be positive");
   RVP_CHECK(n_bound < 0xfffffff0ull / 5, RVP_E_UNSUPPORTED, "batch too large");
   RVP_TRY(offs.alloc((ncl + 1) * sizeof(uint32_t), s));
-  RVP_TRY(seg_clamp_offsets(off_, ncl, n_bound, offs.as<uint32_t>(), s));
   off = offs.as<uint32_t>();
   const unsigned cb = ceil_div(ncl, 64);
   RVP_TRY(mm.alloc(ncl * 6 * sizeof(float), s));
@@ -339,25 +412,26 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_TRY(cellv.alloc(ncl * sizeof(float), s));
   RVP_TRY(ncells.alloc((ncl + 1) * sizeof(uint32_t), s));
   RVP_TRY(toff.alloc((ncl + 1) * sizeof(uint32_t), s));
-  {
-    Tmp mmu;
-    RVP_TRY(mmu.alloc(ncl * 6 * sizeof(uint32_t), s));
-    const unsigned b6 = ceil_div(6 * ncl, 256);
-    RVP_LAUNCH("seg_minmax_init_k", s, seg_minmax_init_k<<<b6, 256, 0, s>>>(ncl, mmu.as<uint32_t>()));
-    const unsigned per = (unsigned)std::min<size_t>(64, std::max<size_t>(1, n_bound / ((size_t)ncl * 4096)));
-    RVP_LAUNCH("seg_minmax_k", s, seg_minmax_k<<<dim3(per, ncl), 256, 0, s>>>(x, y, z, off, mmu.as<uint32_t>()));
-    RVP_LAUNCH("seg_minmax_final_k", s, seg_minmax_final_k<<<b6, 256, 0, s>>>(ncl, mmu.as<uint32_t>(),
-                                                                            mm.as<float>()));
-  }
+  Tmp mmu, bits, ng, key_bits;
+  RVP_TRY(mmu.alloc(ncl * 6 * sizeof(uint32_t), s));
+  RVP_TRY(ng.alloc(sizeof(uint32_t), s));
+  RVP_TRY(key_bits.alloc(sizeof(int), s));
+  const size_t bit_words = mode_knn ? (size_t)ncl * OCC_BITS / 4 : 0;
+  if (mode_knn) RVP_TRY(bits.alloc((size_t)ncl * OCC_BITS, s));
+  // offsets clamp, bbox identity, occupancy bytes and the gap counter cleared: one launch
+  const unsigned pb = (unsigned)std::min<size_t>(1024, std::max<size_t>(1, ceil_div(std::max<size_t>(6 * (size_t)ncl, bit_words), 256)));
+  RVP_LAUNCH("seg_prep_k", s, seg_prep_k<<<pb, 256, 0, s>>>(off_, ncl, (uint32_t)n_bound, offs.as<uint32_t>(),
+                                                            mmu.as<uint32_t>(), bits.as<uint32_t>(), bit_words,
+                                                            ng.as<uint32_t>()));
+  const unsigned per = (unsigned)std::min<size_t>(64, std::max<size_t>(1, n_bound / ((size_t)ncl * 4096)));
+  RVP_LAUNCH("seg_minmax_k", s, seg_minmax_k<<<dim3(per, ncl), 256, 0, s>>>(x, y, z, off, mmu.as<uint32_t>()));
   const float *cells = cell_dev;
   if (mode_knn) {
-    Tmp c0, d0, bits;
+    Tmp c0, d0;
     RVP_TRY(c0.alloc(ncl * sizeof(float), s));
     RVP_TRY(d0.alloc(ncl * sizeof(SegDesc), s));
-    RVP_TRY(bits.alloc((size_t)ncl * OCC_BITS, s));
-    RVP_HIP(hipMemsetAsync(bits.p, 0, (size_t)ncl * OCC_BITS, s));
-    RVP_LAUNCH("seg_knn_c0_k", s, seg_knn_c0_k<<<cb, 64, 0, s>>>(mm.as<float>(), off, ncl, K, c0.as<float>(),
-                                                                 d0.as<SegDesc>()));
+    RVP_LAUNCH("seg_knn_c0_k", s, seg_knn_c0_k<<<cb, 64, 0, s>>>(mmu.as<uint32_t>(), mm.as<float>(), off, ncl, K,
+                                                                 c0.as<float>(), d0.as<SegDesc>()));
     RVP_LAUNCH("seg_occ_k", s, seg_occ_k<<<ceil_div(n_bound, 256), 256, 0, s>>>(
                                    x, y, z, off, ncl, d0.as<SegDesc>(), bits.as<uint8_t>()));
     RVP_LAUNCH("seg_knn_cell_k", s, seg_knn_cell_k<<<ncl, 256, 0, s>>>(off, K, c0.as<float>(), bits.as<uint32_t>(),
@@ -365,13 +439,22 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
     cells = cellv.as<float>();
   }
   const double per_point = tune_seg_cells_per_point();
-  RVP_LAUNCH("seg_geom_k", s, seg_geom_k<<<cb, 64, 0, s>>>(mm.as<float>(), off, ncl, cells, per_point,
-                                                           desc.as<SegDesc>(), ncells.as<uint32_t>()));
-  RVP_TRY(exclusive_scan_u32(ncells.as<uint32_t>(), toff.as<uint32_t>(), ncl, toff.as<uint32_t>() + ncl, s));
-  Tmp key_bits;
-  RVP_TRY(key_bits.alloc(sizeof(int), s));
-  RVP_LAUNCH("seg_toff_k", s, seg_toff_k<<<cb, 64, 0, s>>>(toff.as<uint32_t>(), ncl, desc.as<SegDesc>(),
-                                                           key_bits.as<int>()));
+  if (ncl <= LAYOUT_MAX) {
+    // the k-NN path has the float bbox already (seg_knn_c0_k)
+    RVP_LAUNCH("seg_layout_k", s, seg_layout_k<<<1, LAYOUT_MAX, 0, s>>>(
+                                      mode_knn ? nullptr : mmu.as<uint32_t>(), mm.as<float>(), off, ncl, cells,
+                                      per_point, desc.as<SegDesc>(), ncells.as<uint32_t>(), toff.as<uint32_t>(),
+                                      key_bits.as<int>()));
+  } else {
+    if (!mode_knn)
+      RVP_LAUNCH("seg_minmax_final_k", s, seg_minmax_final_k<<<ceil_div(6 * ncl, 256), 256, 0, s>>>(
+                                              ncl, mmu.as<uint32_t>(), mm.as<float>()));
+    RVP_LAUNCH("seg_geom_k", s, seg_geom_k<<<cb, 64, 0, s>>>(mm.as<float>(), off, ncl, cells, per_point,
+                                                             desc.as<SegDesc>(), ncells.as<uint32_t>()));
+    RVP_TRY(exclusive_scan_u32(ncells.as<uint32_t>(), toff.as<uint32_t>(), ncl, toff.as<uint32_t>() + ncl, s));
+    RVP_LAUNCH("seg_toff_k", s, seg_toff_k<<<cb, 64, 0, s>>>(toff.as<uint32_t>(), ncl, desc.as<SegDesc>(),
+                                                             key_bits.as<int>()));
+  }
   // the table holds at most sum(max(SEG_MIN_CELLS, per_point n_c)) cells
   const uint64_t table = (uint64_t)std::ceil(per_point * (double)n_bound) + (uint64_t)SEG_MIN_CELLS * (uint64_t)ncl;
   RVP_CHECK(table < 0xfff00000ull, RVP_E_UNSUPPORTED, "batch cell table too large");
@@ -398,10 +481,8 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
                                                                 xs.as<float>(), ys.as<float>(), zs.as<float>(),
                                                                 rank.as<uint32_t>()));
   RVP_TRY(cstart.alloc((table + 1) * sizeof(uint32_t), s));
-  Tmp gaps, ng;
+  Tmp gaps;
   RVP_TRY(gaps.alloc((n_bound + 2 + table / SEG_GAP_CHUNK + 2) * sizeof(SegGap), s));
-  RVP_TRY(ng.alloc(sizeof(uint32_t), s));
-  RVP_HIP(hipMemsetAsync(ng.p, 0, sizeof(uint32_t), s));
   RVP_LAUNCH("seg_cell_start_k", s, seg_cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), off, ncl,
                                                                         toff.as<uint32_t>() + ncl, cstart.as<uint32_t>(),
                                                                         gaps.as<SegGap>(), ng.as<uint32_t>()));

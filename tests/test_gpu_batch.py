@@ -80,6 +80,27 @@ def test_batch_radius_runs_cross_clouds_and_dense_windows(clouds, per_wave):
             assert (ci[i, :m] == oi[i, :m]).all() and (cd[i, :m] == od[i, :m]).all(), (c, i)
 
 
+def test_batch_many_small_clouds_layout_fallback():
+    """More clouds than one layout block takes (1,100): the multi-launch layout path, radius and k-NN grids."""
+    rng = np.random.default_rng(31)
+    cl = [rng.uniform(0, 0.3, (int(rng.integers(5, 40)), 3)).astype(np.float32) for _ in range(1100)]
+    b = R.Batch(cl)
+    idx, d2, cnt = R.batch_radius_search(b, 0.05, 16)
+    kept, means = R.batch_sor_exact(b, 3, 1.0)
+    for c in range(0, 1100, 37):
+        p = cl[c]
+        x, y, z = soa(p)
+        oi, od, oc, _ = O.hash_radius(x, y, z, 0.05, x, y, z, 0.05, 16)
+        ci, cd, cc = b.split(idx)[c], b.split(d2)[c], b.split(cnt)[c]
+        assert (cc == oc).all(), c
+        for i in range(len(p)):
+            m = oc[i]
+            assert (ci[i, :m] == oi[i, :m]).all() and (cd[i, :m] == od[i, :m]).all(), (c, i)
+        oout, omeans = O.sor_sc(p, 3, 1.0)
+        assert (b.split(means)[c] == omeans).all(), c
+        assert kept[c].shape == oout.shape and (kept[c] == oout).all(), c
+
+
 def test_batch_normals_equal_single_cloud(clouds):
     cl, _ = clouds
     b = R.Batch(cl)
