@@ -634,7 +634,7 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
                                                             alpha, (uint32_t)n_bound, kp, pos.as<uint32_t>()));
   // stable compaction over the whole batch keeps the clouds in order; a cloud's output starts where the
   // compaction positions its first point
-  RVP_TRY(compact_xyz_aos_flagged(x, y, z, kp, n_bound, out_aos, pos.as<uint32_t>(), kept.as<uint32_t>(), s));
+  RVP_TRY(compact_flagged(x, y, z, pos.as<uint32_t>(), n_bound, G.off + nclouds, out_aos, kept.as<uint32_t>(), s));
   RVP_LAUNCH("seg_out_offsets_k", s, seg_out_offsets_k<<<ceil_div(nclouds + 1, 64), 64, 0, s>>>(
                                          G.off, nclouds, (uint32_t)n_bound, pos.as<uint32_t>(), kept.as<uint32_t>(),
                                          out_offsets));

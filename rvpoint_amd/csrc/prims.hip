@@ -271,6 +271,70 @@ __global__ __launch_bounds__(PRIM_THREADS) void plane_split_k(const float *__res
   }
 }
 
+// Stable compaction of flagged points in one pass: flags in pos[i] (0 or 1) for i < nn = min(*n_dev, n);
+// pos[i] becomes the number of flagged points before i, a flagged point i goes to out_aos[pos[i]], and
+// *count = the number flagged.  The tile count comes from nn on the device: only those tiles take tickets,
+// the rest of a launch sized for the bound n leaves at once.
+__global__ __launch_bounds__(PRIM_THREADS) void compact_flagged_k(const float *__restrict__ x,
+                                                                   const float *__restrict__ y,
+                                                                   const float *__restrict__ z,
+                                                                   uint32_t *__restrict__ pos, size_t n,
+                                                                   const uint32_t *__restrict__ n_dev,
+                                                                   float *__restrict__ out_aos,
+                                                                   uint64_t *__restrict__ status,
+                                                                   uint32_t *__restrict__ ticket, uint32_t epoch,
+                                                                   uint32_t *__restrict__ count) {
+  __shared__ uint32_t sm[PRIM_THREADS / WAVE];
+  __shared__ uint32_t s_tile, s_prefix;
+  const size_t nn = n_dev ? min((size_t)*n_dev, n) : n;
+  const uint32_t nb = (uint32_t)((nn + SCAN_TILE - 1) / SCAN_TILE);
+  if (nb == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0u;
+    return;
+  }
+  if (blockIdx.x >= nb) return;
+  const int t = threadIdx.x;
+  const uint32_t tile = take_tile(ticket, nb, &s_tile);
+  if (tile >= nb) return;
+  const size_t base = (size_t)tile * SCAN_TILE;
+  uint32_t f[SCAN_ITEMS];
+  uint32_t acc = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + (size_t)j * PRIM_THREADS + t;
+    f[j] = i < nn ? pos[i] : 0u;
+    acc += f[j];
+  }
+  acc = wave_sum(acc);
+  if (lane() == 0) sm[t >> 6] = acc;
+  __syncthreads();
+  const uint32_t agg = sm[0] + sm[1] + sm[2] + sm[3];
+  if (t < WAVE) {
+    const uint32_t prefix = tile_lookback(status, tile, epoch, agg);
+    if (t == 0) {
+      s_prefix = prefix;
+      if (tile == nb - 1) *count = prefix + agg;
+    }
+  }
+  __syncthreads();
+  uint32_t carry = s_prefix;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + (size_t)j * PRIM_THREADS + t;
+    uint32_t bt;
+    const uint32_t r = carry + block_excl_scan_256(f[j], bt, sm);
+    if (i < nn) {
+      pos[i] = r;
+      if (f[j] && out_aos) {
+        out_aos[3 * (size_t)r] = x[i];
+        out_aos[3 * (size_t)r + 1] = y[i];
+        out_aos[3 * (size_t)r + 2] = z[i];
+      }
+    }
+    carry += bt;
+  }
+}
+
 // Heads of the equal-key runs of sorted keys[0, nn) (nn = min(*n_dev, n)) in one pass: vid[i] = number of
 // runs starting before i (i < n; nn.. get the total), start[r] = first index of run r, *total = run count.
 template <class K>
@@ -729,6 +793,22 @@ __global__ void scatter_aos_k(const float *__restrict__ x, const float *__restri
     out[3 * p + 1] = y[i];
     out[3 * p + 2] = z[i];
   }
+}
+
+int compact_flagged(const float *x, const float *y, const float *z, uint32_t *pos, size_t n, const uint32_t *n_dev,
+                    float *out_aos, uint32_t *count_dev, hipStream_t s) {
+  if (n == 0) {
+    RVP_HIP(hipMemsetAsync(count_dev, 0, sizeof(uint32_t), s));
+    return RVP_OK;
+  }
+  RVP_CHECK(n < 0xffffffffull, RVP_E_UNSUPPORTED, "compaction: more than 2^32-1 points");
+  const size_t nb = ceil_div(n, SCAN_TILE);
+  uint64_t *status = nullptr;
+  uint32_t *ticket = nullptr, epoch = 0;
+  RVP_TRY(scan_state(s, nb, &status, &ticket, &epoch));
+  RVP_LAUNCH("compact_flagged_k", s, compact_flagged_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(
+                                         x, y, z, pos, n, n_dev, out_aos, status, ticket, epoch, count_dev));
+  return RVP_OK;
 }
 
 int compact_xyz_aos_flagged(const float *x, const float *y, const float *z, const uint8_t *flags, size_t n,

@@ -58,6 +58,11 @@ int exact_seq_sum_seg(const float *v, const uint32_t *seg_off, int nseg, size_t 
 int exact_seq_mean_var_seg(const float *v, const uint32_t *seg_off, int nseg, size_t n_bound, float *sum_out,
                            float *sq_out, hipStream_t s);
 
+// Stable compaction in one pass of the points i < min(*n_dev, n) (n_dev may be null) whose u32 flag in pos is 1:
+// pos is scanned in place into the output positions, out_aos (may be null) receives the flagged points,
+// *count_dev = their count (device).
+int compact_flagged(const float *x, const float *y, const float *z, uint32_t *pos, size_t n, const uint32_t *n_dev,
+                    float *out_aos, uint32_t *count_dev, hipStream_t s);
 // Stable compaction of points whose u8 flag equals 1, given those flags also as u32 in pos (scanned in place
 // into the output positions); *count_dev = kept count (device).
 int compact_xyz_aos_flagged(const float *x, const float *y, const float *z, const uint8_t *flags, size_t n,
