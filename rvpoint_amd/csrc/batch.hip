@@ -378,11 +378,26 @@ __global__ void seg_cell_start_k(const uint32_t *__restrict__ keys, const uint32
   if (i >= n) return;
   const uint32_t k = keys[i];
   const uint32_t lo = i == 0 ? 0u : keys[i - 1] + 1;
-  if (i == 0 || k != keys[i - 1]) {
-    if (k - lo + 1 <= SEG_SHORT_GAP) {
+  const bool start = i == 0 || k != keys[i - 1];
+  const bool long_gap = start && k - lo + 1 > SEG_SHORT_GAP;
+  // the wave's long gaps take their queue slots with one atomic (a counter shared by the whole launch
+  // serialises per-thread atomics)
+  const uint32_t pieces = long_gap ? (k - lo) / SEG_GAP_CHUNK + 1 : 0u;
+  const uint32_t inc = wave_incl_scan(pieces);
+  const uint64_t active = ballot(true);
+  const int leader = __ffsll((unsigned long long)active) - 1, last = 63 - __clzll((long long)active);
+  const uint32_t wave_pieces = readlaneu(inc, last);
+  uint32_t gbase = 0;
+  if (lane() == leader && wave_pieces) gbase = atomicAdd(ngaps, wave_pieces);
+  gbase = readlaneu(gbase, leader) + inc - pieces;
+  if (start) {
+    if (!long_gap) {
       for (uint32_t c = lo; c <= k; ++c) cs[c] = i;
     } else {
-      push_gap(gaps, ngaps, lo, k, i);
+      for (uint32_t p = 0; p < pieces; ++p) {
+        const uint32_t a = lo + p * SEG_GAP_CHUNK;
+        gaps[gbase + p] = SegGap{a, min(k, a + SEG_GAP_CHUNK - 1), i};
+      }
     }
   }
   if (i == n - 1) push_gap(gaps, ngaps, k + 1, table_size, n);

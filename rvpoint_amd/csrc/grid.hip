@@ -148,57 +148,41 @@ __global__ void gather_k(const float *__restrict__ x, const float *__restrict__ 
   rank[p] = (uint32_t)i;
 }
 
-// cell_start[c] = first sorted position whose key >= c, for c in [0, ncells]; short gaps are filled by the
-// point that ends them, long ones are queued for fill_gaps_k.
-struct Gap {
-  uint32_t lo, hi, val;  // cells [lo, hi] get val
-};
-constexpr uint32_t SHORT_GAP = 32;
-
-__global__ void cell_start_k(const uint32_t *__restrict__ keys, size_t n, uint32_t ncells,
-                             uint32_t *__restrict__ cs, uint32_t *__restrict__ occupied, Gap *__restrict__ gaps,
-                             uint32_t *__restrict__ ngaps, uint32_t gap_cap) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// cs[k] = length of the run of key k, written by the run's first position (the table is zeroed before): the
+// exclusive scan of these counts is the cell-start table.  The run end comes from an exponential, then binary
+// search over the sorted keys.
+__global__ void run_lengths_k(const uint32_t *__restrict__ keys, size_t n, uint32_t *__restrict__ cs) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t k = keys[i];
-  uint32_t lo;
-  bool start;
-  if (i == 0) {
-    lo = 0;
-    start = true;
-  } else {
-    const uint32_t kp = keys[i - 1];
-    start = k != kp;
-    lo = kp + 1;
+  if (i > 0 && keys[i - 1] == k) return;
+  size_t lo = i, hi, step = 1;  // keys[lo] == k
+  for (;;) {
+    hi = lo + step;
+    if (hi >= n || keys[hi] != k) break;
+    lo = hi;
+    step <<= 1;
   }
-  if (start) {
-    atomicAdd(occupied, 1u);
-    if (k - lo + 1 <= SHORT_GAP) {
-      for (uint32_t c = lo; c <= k; ++c) cs[c] = (uint32_t)i;
-    } else {
-      uint32_t g = atomicAdd(ngaps, 1u);
-      if (g < gap_cap) gaps[g] = Gap{lo, k, (uint32_t)i};
-    }
+  if (hi > n) hi = n;  // keys[hi] != k or hi == n
+  while (hi - lo > 1) {
+    const size_t m = lo + (hi - lo) / 2;
+    if (keys[m] == k) lo = m;
+    else hi = m;
   }
-  if (i == n - 1) {
-    // tail: cells (k, ncells] start at n
-    if (ncells - k <= SHORT_GAP) {
-      for (uint32_t c = k + 1; c <= ncells; ++c) cs[c] = (uint32_t)n;
-    } else {
-      uint32_t g = atomicAdd(ngaps, 1u);
-      if (g < gap_cap) gaps[g] = Gap{k + 1, ncells, (uint32_t)n};
-    }
-  }
+  cs[k] = (uint32_t)(hi - i);
 }
 
-__global__ void fill_gaps_k(const Gap *__restrict__ gaps, const uint32_t *__restrict__ ngaps, uint32_t gap_cap,
-                            uint32_t *__restrict__ cs) {
-  uint32_t ng = *ngaps;
-  if (ng > gap_cap) ng = gap_cap;
-  for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x) {
-    Gap gp = gaps[g];
-    for (uint64_t c = (uint64_t)gp.lo + threadIdx.x; c <= gp.hi; c += blockDim.x) cs[c] = gp.val;
-  }
+// occupied cells: c with cs[c + 1] > cs[c]
+__global__ __launch_bounds__(256) void count_occupied_k(const uint32_t *__restrict__ cs, uint32_t ncells,
+                                                         uint32_t *__restrict__ out) {
+  uint32_t cnt = 0;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x)
+    cnt += cs[c + 1] > cs[c] ? 1u : 0u;
+  cnt = wave_sum(cnt);
+  __shared__ uint32_t sm[4];
+  if (lane() == 0) sm[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, sm[0] + sm[1] + sm[2] + sm[3]);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -222,8 +206,6 @@ int Grid::build(const float *x_, const float *y_, const float *z_, size_t n_, fl
     RVP_TRY(zs.alloc(4, s, persistent));
     RVP_TRY(perm.alloc(4, s, persistent));
     RVP_TRY(rank.alloc(4, s, persistent));
-    RVP_TRY(occ.alloc(sizeof(uint32_t), s, persistent));
-    RVP_HIP(hipMemsetAsync(occ.p, 0, sizeof(uint32_t), s));
     built = true;
     return RVP_OK;
   }
@@ -290,26 +272,25 @@ int Grid::build_from(const float mm[6], float cell_, hipStream_t s) {
   RVP_LAUNCH("gather_k", s, gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), n, xs.as<float>(),
                                                         ys.as<float>(), zs.as<float>(), rank.as<uint32_t>()));
 
+  // cell-start table: run length of every occupied cell, then one exclusive scan over the table (cells past
+  // the last key get n); per-run writes and a streaming scan instead of per-run gap filling
   RVP_TRY(cstart.alloc((ncells + 1) * sizeof(uint32_t), s, persistent));
-  // long gaps are at most one per run start plus the tail: n + 1 entries always suffice
-  const uint32_t gap_cap = (uint32_t)(n + 2);
-  Tmp gaps;
-  RVP_TRY(gaps.alloc(gap_cap * sizeof(Gap), s));
-  RVP_TRY(occ.alloc(2 * sizeof(uint32_t), s, persistent));
-  RVP_HIP(hipMemsetAsync(occ.p, 0, 2 * sizeof(uint32_t), s));
-  RVP_LAUNCH("cell_start_k", s, cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), n, (uint32_t)ncells,
-                                                                cstart.as<uint32_t>(), occ.as<uint32_t>(), gaps.as<Gap>(),
-                                                                occ.as<uint32_t>() + 1, gap_cap));
-  RVP_LAUNCH("fill_gaps_k", s, fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<Gap>(), occ.as<uint32_t>() + 1, gap_cap,
-                                                                cstart.as<uint32_t>()));
+  RVP_HIP(hipMemsetAsync(cstart.p, 0, (ncells + 1) * sizeof(uint32_t), s));
+  RVP_LAUNCH("run_lengths_k", s, run_lengths_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), n, cstart.as<uint32_t>()));
+  RVP_TRY(exclusive_scan_u32(cstart.as<uint32_t>(), cstart.as<uint32_t>(), ncells + 1, nullptr, s));
   built = true;
   return RVP_OK;
 }
 
 size_t Grid::occupied_cells() const {
-  if (!built || !occ.p) return 0;
+  if (!built || !cstart.p || n == 0) return 0;
+  Tmp cnt;
+  if (cnt.alloc(sizeof(uint32_t), stream) != RVP_OK) return 0;
+  if (hipMemsetAsync(cnt.p, 0, sizeof(uint32_t), stream) != hipSuccess) return 0;
+  const unsigned nb = (unsigned)std::min<size_t>(1024, std::max<size_t>(1, ceil_div(ncells, 256)));
+  count_occupied_k<<<nb, 256, 0, stream>>>(cstart.as<uint32_t>(), (uint32_t)ncells, cnt.as<uint32_t>());
   uint32_t v = 0;
-  if (hipMemcpyAsync(&v, occ.p, sizeof(v), hipMemcpyDeviceToHost, stream) != hipSuccess) return 0;
+  if (hipMemcpyAsync(&v, cnt.p, sizeof(v), hipMemcpyDeviceToHost, stream) != hipSuccess) return 0;
   if (hipStreamSynchronize(stream) != hipSuccess) return 0;
   return v;
 }

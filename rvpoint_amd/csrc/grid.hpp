@@ -76,7 +76,7 @@ struct Grid {
   uint64_t ncells = 0;
   bool built = false;
   bool persistent = false;  // owned by an rvp_hash: hipMalloc'd buffers that outlive the build stream
-  Tmp xs, ys, zs, perm, rank, keys, cstart, occ;
+  Tmp xs, ys, zs, perm, rank, keys, cstart;
   hipStream_t stream = nullptr;
 
   // Build from device SoA.  The device grid uses the requested cell unless its dense table would exceed
