@@ -74,6 +74,8 @@ int rvp_profile_collect(char *buf, size_t buflen, size_t *needed);
  *                          (default 32, [1, 64]; at least 2^17 cells per cloud)
  *   "queries_per_wave"     consecutive queries one wave serves in the capped radius searches (0 = from the
  *                          query count: 64 for a few million queries and more, fewer below; [0, 64])
+ *   "seg_scan_table_min"   batches of at least this many points (their bound) build the cell table from run
+ *                          lengths and one scan; smaller ones fill the gaps between runs (default 4194304)
  * set() returns RVP_E_ARG for an unknown name or a value out of range. */
 int rvp_tune_set(const char *name, double value);
 int rvp_tune_get(const char *name, double *value);

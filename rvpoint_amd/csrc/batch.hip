@@ -412,6 +412,38 @@ __global__ void seg_fill_gaps_k(const SegGap *__restrict__ gaps, const uint32_t 
   }
 }
 
+// The cell-start table of the batch: cstart[g] = first cell-order position whose key >= g, for g in
+// [0, table_size] (table_size = toff[ncl]; queries read no further).  The table is zeroed, each run of equal
+// keys writes its length at its key, and one exclusive scan turns the counts into starts.
+__global__ void seg_zero_table_k(uint32_t *__restrict__ cs, size_t bound, const uint32_t *__restrict__ total_cells) {
+  const size_t m = min((size_t)*total_cells + 1, bound);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
+    cs[i] = 0u;
+}
+
+__global__ void seg_run_lengths_k(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ off, int ncl,
+                                  uint32_t *__restrict__ cs) {
+  const uint32_t n = off[ncl];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  if (i > 0 && keys[i - 1] == k) return;
+  uint32_t lo = i, hi, step = 1;  // keys[lo] == k; hi: first position past the run
+  for (;;) {
+    hi = lo + step;
+    if (hi >= n || keys[hi] != k) break;
+    lo = hi;
+    step <<= 1;
+  }
+  if (hi > n) hi = n;
+  while (hi - lo > 1) {
+    const uint32_t m = lo + (hi - lo) / 2;
+    if (keys[m] == k) lo = m;
+    else hi = m;
+  }
+  cs[k] = hi - i;
+}
+
 int SegGrid::build(const float *x, const float *y, const float *z, const uint32_t *off_, int ncl_, size_t n_bound_,
                  bool mode_knn, int K, const float *cell_dev, hipStream_t s) {
   ncl = ncl_;
@@ -496,13 +528,25 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
                                                                 xs.as<float>(), ys.as<float>(), zs.as<float>(),
                                                                 rank.as<uint32_t>()));
   RVP_TRY(cstart.alloc((table + 1) * sizeof(uint32_t), s));
-  Tmp gaps;
-  RVP_TRY(gaps.alloc((n_bound + 2 + table / SEG_GAP_CHUNK + 2) * sizeof(SegGap), s));
-  RVP_LAUNCH("seg_cell_start_k", s, seg_cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), off, ncl,
-                                                                        toff.as<uint32_t>() + ncl, cstart.as<uint32_t>(),
-                                                                        gaps.as<SegGap>(), ng.as<uint32_t>()));
-  RVP_LAUNCH("seg_fill_gaps_k", s, seg_fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<SegGap>(), ng.as<uint32_t>(),
-                                                                        cstart.as<uint32_t>()));
+  const uint32_t *total_cells = toff.as<uint32_t>() + ncl;
+  if ((double)n_bound >= tune_seg_scan_table_min()) {
+    // run lengths and one streaming scan of the table
+    RVP_LAUNCH("seg_zero_table_k", s, seg_zero_table_k<<<(unsigned)std::min<size_t>(4096, ceil_div(table + 1, 256)), 256, 0, s>>>(
+                                          cstart.as<uint32_t>(), table + 1, total_cells));
+    RVP_LAUNCH("seg_run_lengths_k", s, seg_run_lengths_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), off, ncl,
+                                                                            cstart.as<uint32_t>()));
+    RVP_TRY(exclusive_scan_u32_bounded(cstart.as<uint32_t>(), cstart.as<uint32_t>(), table + 1, total_cells, 1,
+                                       nullptr, s));
+  } else {
+    // small batches: gaps filled by the run that ends them (long ones queued), no pass over the whole table
+    Tmp gaps;
+    RVP_TRY(gaps.alloc((n_bound + 2 + table / SEG_GAP_CHUNK + 2) * sizeof(SegGap), s));
+    RVP_LAUNCH("seg_cell_start_k", s, seg_cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), off, ncl, total_cells,
+                                                                          cstart.as<uint32_t>(), gaps.as<SegGap>(),
+                                                                          ng.as<uint32_t>()));
+    RVP_LAUNCH("seg_fill_gaps_k", s, seg_fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<SegGap>(), ng.as<uint32_t>(),
+                                                                          cstart.as<uint32_t>()));
+  }
   return RVP_OK;
 }
 

@@ -115,6 +115,7 @@ double tune_knn_cell_div();
 // dense-table cells per point a batched grid may use (rvp_tune_set "seg_cells_per_point")
 double tune_seg_cells_per_point();
 int tune_queries_per_wave();  // 0: chosen from the query count
+double tune_seg_scan_table_min();  // batch bound (points) from which a batched grid's cell table is scanned
 // blocks of `threads` threads of `kernel` the current device holds at once (occupancy x CUs), cached
 int resident_blocks(const void *kernel, int threads, unsigned *blocks);
 

@@ -53,15 +53,23 @@ __global__ __launch_bounds__(256) void bbox_partial_k(const float *__restrict__ 
   }
 }
 
-__global__ void bbox_final_k(const float *__restrict__ part, int nb, float *__restrict__ out) {
-  if (threadIdx.x != 0) return;
+// one wave: lanes stride over the block partials, then a butterfly (min / max are exact in any order; the
+// partials hold no NaN)
+__global__ __launch_bounds__(64) void bbox_final_k(const float *__restrict__ part, int nb, float *__restrict__ out) {
   float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
-  for (int b = 0; b < nb; ++b)
+  for (int b = threadIdx.x; b < nb; b += WAVE)
     for (int j = 0; j < 3; ++j) {
-      if (part[b * 6 + j] < v[j]) v[j] = part[b * 6 + j];
-      if (part[b * 6 + j + 3] > v[j + 3]) v[j + 3] = part[b * 6 + j + 3];
+      v[j] = fminf(v[j], part[b * 6 + j]);
+      v[j + 3] = fmaxf(v[j + 3], part[b * 6 + j + 3]);
     }
-  for (int j = 0; j < 6; ++j) out[j] = v[j];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1)
+    for (int j = 0; j < 3; ++j) {
+      v[j] = fminf(v[j], shfl_xor(v[j], d));
+      v[j + 3] = fmaxf(v[j + 3], shfl_xor(v[j + 3], d));
+    }
+  if (threadIdx.x == 0)
+    for (int j = 0; j < 6; ++j) out[j] = v[j];
 }
 
 int bbox_minmax(const float *x, const float *y, const float *z, size_t n, float *out6, hipStream_t s) {

@@ -160,12 +160,22 @@ __device__ __forceinline__ uint32_t take_tile(uint32_t *ticket, uint32_t nb, uin
 }
 
 // (in and out may be the same array: a tile reads all its inputs before it writes)
-__global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in, uint32_t *out, size_t n,
+// n_dev (may be null) bounds the element count on the device: nn = min(*n_dev + add, n); only the tiles nn
+// needs take tickets, the rest of a launch sized for n leaves at once
+__global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in, uint32_t *out, size_t n_bound,
+                                                              const uint32_t *__restrict__ n_dev, uint32_t add,
                                                               uint64_t *__restrict__ status,
-                                                              uint32_t *__restrict__ ticket, uint32_t nb,
-                                                              uint32_t epoch, uint32_t *__restrict__ total) {
+                                                              uint32_t *__restrict__ ticket, uint32_t epoch,
+                                                              uint32_t *__restrict__ total) {
   __shared__ uint32_t sm[PRIM_THREADS / WAVE];
   __shared__ uint32_t s_tile, s_prefix;
+  const size_t n = n_dev ? min((size_t)*n_dev + add, n_bound) : n_bound;
+  const uint32_t nb = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
+  if (nb == 0) {
+    if (total && blockIdx.x == 0 && threadIdx.x == 0) *total = 0u;
+    return;
+  }
+  if (blockIdx.x >= nb) return;
   const int t = threadIdx.x;
   const uint32_t tile = take_tile(ticket, nb, &s_tile);
   if (tile >= nb) return;
@@ -419,19 +429,24 @@ static int scan_state(hipStream_t s, size_t tiles, uint64_t **status, uint32_t *
   return RVP_OK;
 }
 
-int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev, hipStream_t s) {
+int exclusive_scan_u32_bounded(const uint32_t *in, uint32_t *out, size_t n, const uint32_t *n_dev, uint32_t add,
+                               uint32_t *total_dev, hipStream_t s) {
   if (n == 0) {
     if (total_dev) RVP_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), s));
     return RVP_OK;
   }
   const size_t nb = ceil_div(n, SCAN_TILE);
-  RVP_CHECK(nb < 0xffffffffull, RVP_E_UNSUPPORTED, "scan too large");
+  RVP_CHECK(nb < 0xffffffffull && n < 0xffffffffull, RVP_E_UNSUPPORTED, "scan too large");
   uint64_t *status = nullptr;
   uint32_t *ticket = nullptr, epoch = 0;
   RVP_TRY(scan_state(s, nb, &status, &ticket, &epoch));
-  RVP_LAUNCH("scan_1pass_k", s, scan_1pass_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, out, n, status, ticket,
-                                                                                  (uint32_t)nb, epoch, total_dev));
+  RVP_LAUNCH("scan_1pass_k", s, scan_1pass_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, out, n, n_dev, add, status,
+                                                                                  ticket, epoch, total_dev));
   return RVP_OK;
+}
+
+int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev, hipStream_t s) {
+  return exclusive_scan_u32_bounded(in, out, n, nullptr, 0, total_dev, s);
 }
 int plane_split_aos(const float *x, const float *y, const float *z, size_t n, const float *m, float t, float *in_aos,
                     float *out_aos, uint32_t *counts_dev, hipStream_t s) {

@@ -12,6 +12,9 @@ constexpr int SCAN_TILE = PRIM_THREADS * SCAN_ITEMS; // 2048
 // Exclusive scan out[i] = sum(in[0..i)), i < n; if total_dev != nullptr it receives sum(in[0..n)).
 // In/Out may alias.  Out is uint32_t or uint64_t (int64_t via reinterpretation).
 int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev, hipStream_t s);
+// the same over the first min(*n_dev + add, n) elements (n_dev on the device, may be null)
+int exclusive_scan_u32_bounded(const uint32_t *in, uint32_t *out, size_t n, const uint32_t *n_dev, uint32_t add,
+                               uint32_t *total_dev, hipStream_t s);
 // look-back waits of the single-pass scan that hit their spin limit since load (~0 if unreadable)
 unsigned long long scan_spin_errors();
 int exclusive_scan_u32_u64(const uint32_t *in, uint64_t *out, size_t n, uint64_t *total_dev, hipStream_t s);

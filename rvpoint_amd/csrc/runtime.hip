@@ -160,9 +160,11 @@ void prof_stop(const char *name, hipStream_t s, hipEvent_t a) {
 static std::atomic<double> g_knn_cell_div{2.4};
 static std::atomic<double> g_seg_cells_per_point{32.0};
 static std::atomic<int> g_queries_per_wave{0};
+static std::atomic<double> g_seg_scan_table_min{4194304.0};
 double tune_knn_cell_div() { return g_knn_cell_div.load(); }
 double tune_seg_cells_per_point() { return g_seg_cells_per_point.load(); }
 int tune_queries_per_wave() { return g_queries_per_wave.load(); }
+double tune_seg_scan_table_min() { return g_seg_scan_table_min.load(); }
 
 struct PinnedStage::Slot {
   std::mutex mu;
@@ -405,6 +407,11 @@ int rvp_tune_set(const char *name, double value) {
     rvp::g_queries_per_wave.store((int)value);
     return RVP_OK;
   }
+  if (std::string(name) == "seg_scan_table_min") {
+    RVP_CHECK(value >= 0.0 && value <= 4294967295.0, RVP_E_ARG, "seg_scan_table_min must be in [0, 2^32)");
+    rvp::g_seg_scan_table_min.store(value);
+    return RVP_OK;
+  }
   return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);
 }
 
@@ -420,6 +427,10 @@ int rvp_tune_get(const char *name, double *value) {
   }
   if (std::string(name) == "queries_per_wave") {
     *value = (double)rvp::g_queries_per_wave.load();
+    return RVP_OK;
+  }
+  if (std::string(name) == "seg_scan_table_min") {
+    *value = rvp::g_seg_scan_table_min.load();
     return RVP_OK;
   }
   return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);

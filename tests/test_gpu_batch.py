@@ -101,6 +101,30 @@ def test_batch_many_small_clouds_layout_fallback():
         assert kept[c].shape == oout.shape and (kept[c] == oout).all(), c
 
 
+@pytest.mark.parametrize("scan_min", [0, 1 << 40])
+def test_batch_cell_table_both_builds(clouds, scan_min):
+    """The batched cell table built from run lengths + a scan, and by gap filling: same radius lists."""
+    cl, scales = clouds
+    b = R.Batch(cl)
+    radius = np.array([0.02 * s for s in scales], np.float32)
+    old = R._lib.tune("seg_scan_table_min")
+    try:
+        R._lib.tune("seg_scan_table_min", min(scan_min, 4294967295))
+        idx, d2, cnt = R.batch_radius_search(b, radius, 64)
+    finally:
+        R._lib.tune("seg_scan_table_min", old)
+    for c, p in enumerate(cl):
+        if len(p) == 0:
+            continue
+        x, y, z = soa(p)
+        oi, od, oc, _ = O.hash_radius(x, y, z, float(radius[c]), x, y, z, float(radius[c]), 64)
+        ci, cd, cc = b.split(idx)[c], b.split(d2)[c], b.split(cnt)[c]
+        assert (cc == oc).all(), c
+        for i in range(len(p)):
+            m = oc[i]
+            assert (ci[i, :m] == oi[i, :m]).all() and (cd[i, :m] == od[i, :m]).all(), (c, i)
+
+
 def test_batch_normals_equal_single_cloud(clouds):
     cl, _ = clouds
     b = R.Batch(cl)
