@@ -316,9 +316,12 @@ __global__ void seg_toff_k(const uint32_t *__restrict__ toff, int ncl, SegDesc *
   }
 }
 
+// also each point as one 16-byte record in input order, so that the gather after the sort reads one sector
+// per point instead of three
 __global__ void seg_keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
                            const uint32_t *__restrict__ off, int ncl, const SegDesc *__restrict__ desc, uint32_t n_bound,
-                           uint32_t pad_key, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+                           uint32_t pad_key, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                           float4 *__restrict__ xyz) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_bound) return;
   vals[i] = i;
@@ -328,22 +331,24 @@ __global__ void seg_keys_k(const float *__restrict__ x, const float *__restrict_
   }
   const int c = cloud_of(off, ncl, i);
   const SegDesc d = desc[c];
-  keys[i] = d.toff + (uint32_t)cell_clamped(x[i], d.mn[0], d.cell, d.g[0]) +
-            (uint32_t)d.g[0] * ((uint32_t)cell_clamped(y[i], d.mn[1], d.cell, d.g[1]) +
-                                (uint32_t)d.g[1] * (uint32_t)cell_clamped(z[i], d.mn[2], d.cell, d.g[2]));
+  const float a = x[i], b = y[i], e = z[i];
+  keys[i] = d.toff + (uint32_t)cell_clamped(a, d.mn[0], d.cell, d.g[0]) +
+            (uint32_t)d.g[0] * ((uint32_t)cell_clamped(b, d.mn[1], d.cell, d.g[1]) +
+                                (uint32_t)d.g[1] * (uint32_t)cell_clamped(e, d.mn[2], d.cell, d.g[2]));
+  xyz[i] = make_float4(a, b, e, 0.0f);
 }
 
 // cell-order SoA, and the inverse permutation rank[perm[i]] = i
-__global__ void seg_gather_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-                             const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off, int ncl,
-                             float *__restrict__ xs, float *__restrict__ ys, float *__restrict__ zs,
-                             uint32_t *__restrict__ rank) {
+__global__ void seg_gather_k(const float4 *__restrict__ xyz, const uint32_t *__restrict__ perm,
+                             const uint32_t *__restrict__ off, int ncl, float *__restrict__ xs,
+                             float *__restrict__ ys, float *__restrict__ zs, uint32_t *__restrict__ rank) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= off[ncl]) return;
   const uint32_t p = perm[i];
-  xs[i] = x[p];
-  ys[i] = y[p];
-  zs[i] = z[p];
+  const float4 v = xyz[p];
+  xs[i] = v.x;
+  ys[i] = v.y;
+  zs[i] = v.z;
   rank[p] = i;
 }
 
@@ -512,8 +517,11 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_TRY(kt.alloc(n_bound * sizeof(uint32_t), s));
   RVP_TRY(vt.alloc(n_bound * sizeof(uint32_t), s));
   const unsigned nb = ceil_div(n_bound, 256);
+  Tmp xyz;
+  RVP_TRY(xyz.alloc(n_bound * sizeof(float4), s));
   RVP_LAUNCH("seg_keys_k", s, seg_keys_k<<<nb, 256, 0, s>>>(x, y, z, off, ncl, desc.as<SegDesc>(), n32,
-                                                            (uint32_t)table, keys.as<uint32_t>(), perm.as<uint32_t>()));
+                                                            (uint32_t)table, keys.as<uint32_t>(), perm.as<uint32_t>(),
+                                                            xyz.as<float4>()));
   uint32_t *kp = keys.as<uint32_t>(), *vp = perm.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
   RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n_bound, bits_for(table), s, off + ncl, key_bits.as<int>()));
   if (kp != keys.as<uint32_t>()) {
@@ -524,7 +532,7 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_TRY(ys.alloc(n_bound * sizeof(float), s));
   RVP_TRY(zs.alloc(n_bound * sizeof(float), s));
   RVP_TRY(rank.alloc(n_bound * sizeof(uint32_t), s));
-  RVP_LAUNCH("seg_gather_k", s, seg_gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), off, ncl,
+  RVP_LAUNCH("seg_gather_k", s, seg_gather_k<<<nb, 256, 0, s>>>(xyz.as<float4>(), perm.as<uint32_t>(), off, ncl,
                                                                 xs.as<float>(), ys.as<float>(), zs.as<float>(),
                                                                 rank.as<uint32_t>()));
   RVP_TRY(cstart.alloc((table + 1) * sizeof(uint32_t), s));

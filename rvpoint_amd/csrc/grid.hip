@@ -131,28 +131,33 @@ __global__ __launch_bounds__(256) void count_marked_k(const uint32_t *__restrict
   if (lane() == 0 && c) atomicAdd(count, c);
 }
 
+// also each point as one 16-byte record in input order, so that the gather after the sort reads one sector
+// per point instead of three
 __global__ void keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
                        size_t n, float mnx, float mny, float mnz, float c, int gx, int gy, int gz,
-                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, float4 *__restrict__ xyz) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int ix = cell_clamped(x[i], mnx, c, gx);
-  int iy = cell_clamped(y[i], mny, c, gy);
-  int iz = cell_clamped(z[i], mnz, c, gz);
+  const float a = x[i], b = y[i], d = z[i];
+  int ix = cell_clamped(a, mnx, c, gx);
+  int iy = cell_clamped(b, mny, c, gy);
+  int iz = cell_clamped(d, mnz, c, gz);
   keys[i] = (uint32_t)ix + (uint32_t)gx * ((uint32_t)iy + (uint32_t)gy * (uint32_t)iz);
   vals[i] = (uint32_t)i;
+  xyz[i] = make_float4(a, b, d, 0.0f);
 }
 
 // cell-order SoA, and the inverse permutation rank[perm[i]] = i
-__global__ void gather_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-                         const uint32_t *__restrict__ perm, size_t n, float *__restrict__ xs,
-                         float *__restrict__ ys, float *__restrict__ zs, uint32_t *__restrict__ rank) {
+__global__ void gather_k(const float4 *__restrict__ xyz, const uint32_t *__restrict__ perm, size_t n,
+                         float *__restrict__ xs, float *__restrict__ ys, float *__restrict__ zs,
+                         uint32_t *__restrict__ rank) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t p = perm[i];
-  xs[i] = x[p];
-  ys[i] = y[p];
-  zs[i] = z[p];
+  const float4 v = xyz[p];
+  xs[i] = v.x;
+  ys[i] = v.y;
+  zs[i] = v.z;
   rank[p] = (uint32_t)i;
 }
 
@@ -265,8 +270,11 @@ int Grid::build_from(const float mm[6], float cell_, hipStream_t s) {
   RVP_TRY(kt.alloc(n * sizeof(uint32_t), s));
   RVP_TRY(vt.alloc(n * sizeof(uint32_t), s));
   const unsigned nb = ceil_div(n, 256);
+  Tmp xyz;
+  RVP_TRY(xyz.alloc(n * sizeof(float4), s));
   RVP_LAUNCH("keys_k", s, keys_k<<<nb, 256, 0, s>>>(x, y, z, n, bb[0], bb[1], bb[2], cell, dims[0], dims[1],
-                                                    dims[2], keys.as<uint32_t>(), perm.as<uint32_t>()));
+                                                    dims[2], keys.as<uint32_t>(), perm.as<uint32_t>(),
+                                                    xyz.as<float4>()));
   uint32_t *kp = keys.as<uint32_t>(), *vp = perm.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
   RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n, bits_for(ncells - 1), s, nullptr, nullptr, n >= (1u << 19)));
   if (kp != keys.as<uint32_t>()) {  // odd pass count: the sorted data sits in the temporaries
@@ -277,7 +285,7 @@ int Grid::build_from(const float mm[6], float cell_, hipStream_t s) {
   RVP_TRY(ys.alloc(n * sizeof(float), s, persistent));
   RVP_TRY(zs.alloc(n * sizeof(float), s, persistent));
   RVP_TRY(rank.alloc(n * sizeof(uint32_t), s, persistent));
-  RVP_LAUNCH("gather_k", s, gather_k<<<nb, 256, 0, s>>>(x, y, z, perm.as<uint32_t>(), n, xs.as<float>(),
+  RVP_LAUNCH("gather_k", s, gather_k<<<nb, 256, 0, s>>>(xyz.as<float4>(), perm.as<uint32_t>(), n, xs.as<float>(),
                                                         ys.as<float>(), zs.as<float>(), rank.as<uint32_t>()));
 
   // cell-start table: run length of every occupied cell, then one exclusive scan over the table (cells past
