@@ -85,7 +85,8 @@ def test_radius_capped_runs_of_queries(per_wave):
     old = R._lib.tune("queries_per_wave")
     try:
         R._lib.tune("queries_per_wave", per_wave)
-        for r, mnn in ((0.02, 64), (0.03, 17)):
+        # r = 0.2 over 0.02 cells: windows of more than 64 (y, z) rows, walked without staging
+        for r, mnn in ((0.02, 64), (0.03, 17), (0.2, 8)):
             got = h.radius_search_batch(None, r, mnn)
             idx, d2, cnt, _ = O.hash_radius(x, y, z, 0.02, x, y, z, r, mnn)
             _check_lists(got, idx, d2, cnt)
