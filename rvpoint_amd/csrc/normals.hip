@@ -8,6 +8,8 @@
 // reproduced; results are graded against an FP64 eigen solve of the same neighbourhood.
 #include <atomic>
 
+#include <type_traits>
+
 #include "batch.hpp"
 #include "prims.hpp"
 #include "query.hpp"
@@ -226,6 +228,9 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
                                                       float *__restrict__ cz) {
   __shared__ WaveLds lds[QW];
   __shared__ NormLds nl[QW];
+  // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
+  constexpr bool STAGE = MODE == 1 && !STRIDED;
+  __shared__ typename std::conditional<STAGE, StageLds, char>::type stage[QW];
   const int w = threadIdx.x >> 6;
   WaveLds &L = lds[w];
   NormLds &N = nl[w];
@@ -233,6 +238,7 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
   uint32_t first = 0, step = 1;
   int nbatch = 0;
   for (uint32_t k = 0; wave_batch<STRIDED, NB>(k, g.n, first, step, nbatch); ++k) {
+    StagedWindow win;
     for (int j = 0; j < nbatch; ++j) {
       const uint32_t q = first + (uint32_t)j * step;
       const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
@@ -244,7 +250,13 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
         int lo[3], hi[3];
         window3(g, x, y, z, radius, lo, hi);
         TopKSink<false> sink{t, L};
-        scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
+        if constexpr (STAGE) {
+          win.ensure(g, lo, hi, 0, L, stage[w]);
+          if (win.total >= 0) scan_staged(stage[w], win.total, x, y, z, r2, sink);
+          else scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
+        } else {
+          scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
+        }
         topk_flush<false>(t, L);
       }
       const int m = min(t.cnt, K);
@@ -330,12 +342,16 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
                                                      unsigned long long *fallbacks) {
   __shared__ WaveLds lds[QW];
   __shared__ NormLds nl[QW];
+  // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
+  constexpr bool STAGE = MODE == 1 && !STRIDED;
+  __shared__ typename std::conditional<STAGE, StageLds, char>::type stage[QW];
   const int w = threadIdx.x >> 6;
   NormLds &N = nl[w];
   const uint32_t total = A.off[A.ncl];
   uint32_t first = 0, step = 1;
   int nbatch = 0;
   for (uint32_t k = 0; wave_batch<STRIDED, NB>(k, total, first, step, nbatch); ++k) {
+    StagedWindow win;
     for (int j = 0; j < nbatch; ++j) {
       const uint32_t q = first + (uint32_t)j * step;
       const int c = cloud_of(A.off, A.ncl, q);
@@ -347,10 +363,17 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
         knn_search<false>(g, x, y, z, t, lds[w]);
       } else {
         const float r = radius[c];
+        const float r2 = __fmul_rn(r, r);
         int lo[3], hi[3];
         window3(g, x, y, z, r, lo, hi);
         TopKSink<false> sink{t, lds[w]};
-        scan_box(g, x, y, z, lo, hi, false, lo, hi, __fmul_rn(r, r), true, lds[w], sink);
+        if constexpr (STAGE) {
+          win.ensure(g, lo, hi, c, lds[w], stage[w]);
+          if (win.total >= 0) scan_staged(stage[w], win.total, x, y, z, r2, sink);
+          else scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, lds[w], sink);
+        } else {
+          scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, lds[w], sink);
+        }
         topk_flush<false>(t, lds[w]);
       }
       const int m = min(t.cnt, K);
@@ -412,17 +435,10 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
   RVP_TRY(fallback_counter(s, &fb));
   const GridView g = h->g.view();
   if (max_nn > 0 && max_nn <= MAX_TOPK) {
-    unsigned qb = 1;
-    bool strided = false;
-    RVP_TRY(batch_blocks<NB>(normals_topk_k<1, true>, n, &qb, &strided));
-    if (strided)
-      RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, true><<<qb, QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx,
-                                                                                ny, nz, fb, nullptr, nullptr,
-                                                                                nullptr));
-    else
-      RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, false><<<qb, QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx,
-                                                                                 ny, nz, fb, nullptr, nullptr,
-                                                                                 nullptr));
+    // batches of NB consecutive queries with staged windows (as the batched radius normals)
+    const unsigned qb = ceil_div(ceil_div(n, NB), QW);
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, false><<<qb, QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx,
+                                                                               ny, nz, fb, nullptr, nullptr, nullptr));
     RVP_LAUNCHED();
   } else {
     // unlimited (or a cap above 64): full lists, sorted by (d2, idx), truncated by the row offsets
@@ -481,15 +497,11 @@ int rvp_batch_normals_radius(const float *x, const float *y, const float *z, con
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, false, 0, cell ? cell : radius, s));
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
-  unsigned qb = 1;
-  bool strided = false;
-  RVP_TRY(batch_blocks<NB>(seg_normals_k<1, true>, n_bound, &qb, &strided));
-  if (strided)
-    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, true><<<qb, QT, 0, s>>>(G.arrays(), max_nn, radius, vpx, vpy, vpz,
-                                                                            nx, ny, nz, fb));
-  else
-    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(G.arrays(), max_nn, radius, vpx, vpy,
-                                                                             vpz, nx, ny, nz, fb));
+  // radius lists: batches of NB consecutive queries, whose shared windows stage once in LDS (faster than the
+  // strided batches that balance k-NN queries, also for small counts)
+  const unsigned qb = ceil_div(ceil_div(n_bound, NB), QW);
+  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(G.arrays(), max_nn, radius, vpx, vpy,
+                                                                           vpz, nx, ny, nz, fb));
   return RVP_OK;
 }
 
