@@ -314,7 +314,7 @@ size_t Grid::occupied_cells() const {
 // Cell edge for k-nearest queries.  A first edge c0 from the bbox volume (k points per cell if the cloud
 // filled it) gives an occupancy count; with m points per occupied cell the surface density is about
 // m / c0^2, so the k-th neighbour distance is about R = c0 sqrt(k / (pi m)), and the edge is R / div
-// (div = tuning knob knn_cell_div, default 2.8):
+// (div = tuning knob knn_cell_div, default 2.4):
 // the certified box then spans about 6 cells per axis, which keeps the candidates near 2-3x the k
 // neighbours for surfaces and volumes alike.
 int Grid::build_for_knn(const float *x_, const float *y_, const float *z_, size_t n_, int k, hipStream_t s) {

@@ -13,8 +13,9 @@
 // by an ordered tree reduction over its elements.  One wave per segment then walks its chunks in order: a
 // chunk whose map applies (the running sum is in the predicted binade and M stays below 2^24) costs one
 // integer step; any other chunk (the first one, binade crossings, zeros, subnormals, non-finite values)
-// is replayed element by element with the fp32 adder.  Every applied map is checked, so the result equals
-// the sequential sum exactly; the prediction only decides how many chunks take the slow path.
+// is replayed: lane maps (SS_LANE terms each) in the running sum's binade, up to the lane where it leaves
+// the binade, which the fp32 adder replays term by term.  Every applied map is checked, so the result
+// equals the sequential sum exactly; the prediction only decides how many chunks take the slow path.
 #include "prims.hpp"
 
 namespace rvp {
@@ -22,8 +23,14 @@ namespace rvp {
 constexpr int SS_LANE = 4;                  // elements per lane
 constexpr int SS_C = WAVE * SS_LANE;        // elements per chunk
 
+// Maps are kept in 32 bits, saturated at SS_SAT = 2^25: a composed value whose exact counterpart is below
+// 2^24 is exact (its inputs are, so the parity selections agree), and one whose exact counterpart is 2^24
+// or more stays at 2^24 or more.  A state M0 + map (M0 < 2^24) then fails the check M < 2^24 exactly when
+// the exact one does.
+constexpr int SS_SAT = 1 << 25;
+
 struct SSChunk {
-  long long ie, io;
+  int ie, io;
   int e;
   int clean;
 };
@@ -49,11 +56,53 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
   return v;
 }
 
-__device__ __forceinline__ void ss_compose(long long &fe, long long &fo, long long ge, long long go) {
-  const long long ne = fe + ((fe & 1) ? go : ge);
-  const long long no = fo + ((fo & 1) ? ge : go);
+__device__ __forceinline__ void ss_compose(int &fe, int &fo, int ge, int go) {
+  const int ne = min(fe + ((fe & 1) ? go : ge), SS_SAT);
+  const int no = min(fo + ((fo & 1) ? ge : go), SS_SAT);
   fe = ne;
   fo = no;
+}
+
+// The step map (ae, ao) of one term x in binade e, scale = 2^(23 - e): M -> M + (M even ? ae : ao).  A
+// term that is negative, non-finite or of SS_SAT units or more gets the saturated map, so the state check
+// (M < 2^24) fails and the term is added by the fp32 adder instead.
+__device__ __forceinline__ void ss_term_map(float x, double scale, int &ae, int &ao) {
+  double t = (double)x * scale;  // exact: power-of-two scaling
+  if (!(t >= 0.0 && t < (double)SS_SAT)) t = (double)SS_SAT;
+  const double a = floor(t), fr = t - a;
+  const int ai = (int)a;
+  if (fr > 0.5) {
+    ae = ao = ai + 1;
+  } else if (fr < 0.5) {
+    ae = ao = ai;
+  } else {  // exact tie: the result M + a (+1) must be even
+    ae = ai + (ai & 1);
+    ao = ai + 1 - (ai & 1);
+  }
+}
+
+// one scan step: compose the map of the lane the DPP control selects (identity (0, 0) where it selects
+// none or the row is masked off) in front of this lane's
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void ss_scan_step(int &fe, int &fo) {
+  int pe = __builtin_amdgcn_update_dpp(0, fe, CTRL, ROW_MASK, 0xF, true);
+  int po = __builtin_amdgcn_update_dpp(0, fo, CTRL, ROW_MASK, 0xF, true);
+  ss_compose(pe, po, fe, fo);
+  fe = pe;
+  fo = po;
+}
+
+// inclusive scan of step maps over the wave, earlier lanes' maps first ((0, 0) is the identity): within
+// rows of 16 lanes by row_shr 1, 2, 4, 8, then lane 15 of rows 0 and 2 into rows 1 and 3 (row_bcast:15),
+// then lane 31 into rows 2 and 3 (row_bcast:31).  The walk over the chunks is one wave, so the scan's
+// latency is its cost: DPP moves instead of LDS permutes.
+__device__ __forceinline__ void ss_scan_maps(int &fe, int &fo) {
+  ss_scan_step<0x111, 0xF>(fe, fo);
+  ss_scan_step<0x112, 0xF>(fe, fo);
+  ss_scan_step<0x114, 0xF>(fe, fo);
+  ss_scan_step<0x118, 0xF>(fe, fo);
+  ss_scan_step<0x142, 0xA>(fe, fo);
+  ss_scan_step<0x143, 0xC>(fe, fo);
 }
 
 // largest s in [0, nseg) with off[s] <= v (off non-decreasing, off[0] == 0)
@@ -131,49 +180,84 @@ __global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, 
   }
   const float mean = ss_mean(mode, mean_dev, seg, r.s);
   const double scale = ldexp(1.0, 23 - e);
-  long long fe = 0, fo = 0;
+  int fe = 0, fo = 0;
 #pragma unroll
   for (int q = 0; q < SS_LANE; ++q) {
     const uint32_t i = r.a + (uint32_t)l * SS_LANE + q;
     if (i < r.b) {
-      double t = (double)ss_term(v[i], mode, mean) * scale;  // exact: power-of-two scaling
-      if (!(t < 1099511627776.0)) t = 1099511627776.0;       // 2^40: saturate (the chunk then fails its check)
-      const double a = floor(t), fr = t - a;
-      const long long ai = (long long)a;
-      long long ae, ao;
-      if (fr > 0.5) {
-        ae = ao = ai + 1;
-      } else if (fr < 0.5) {
-        ae = ao = ai;
-      } else {  // exact tie: the result M + a (+1) must be even
-        ae = ai + (ai & 1);
-        ao = ai + 1 - (ai & 1);
-      }
+      int ae, ao;
+      ss_term_map(ss_term(v[i], mode, mean), scale, ae, ao);
       ss_compose(fe, fo, ae, ao);
     }
   }
   // ordered tree: lane l composes its segment with the one of lane l + d
 #pragma unroll
   for (int d = 1; d < WAVE; d <<= 1) {
-    const long long ge = __shfl_down(fe, d), go = __shfl_down(fo, d);
+    const int ge = __shfl_down(fe, d), go = __shfl_down(fo, d);
     if ((l & (2 * d - 1)) == 0 && l + d < WAVE) ss_compose(fe, fo, ge, go);
   }
   if (l == 0) out[c] = SSChunk{fe, fo, e, 1};
 }
 
-// element-by-element replay of one chunk with the fp32 adder (the reference's own loop)
-__device__ __forceinline__ float ss_replay(float acc, const float *__restrict__ v, uint32_t a, uint32_t e_end,
-                                           int mode, float mean) {
-  const int l = lane();
+// One chunk whose map does not apply, a lane's SS_LANE terms at a time.  While the running sum is a
+// positive normal number, the maps of the remaining lanes' terms in its binade are built and scanned, and
+// applied up to the first lane whose state leaves the binade (M >= 2^24); that lane alone is added term by
+// term with the fp32 adder (the reference's own loop), which carries the sum into the next binade.  A +0
+// sum skips the lanes whose terms are all zero (+0 plus +-0 is +0).  Every other state (subnormal,
+// non-finite, negative) is also advanced with the fp32 adder.
+struct SSTerms {
   float x[SS_LANE];
+};
+
+// the chunk's terms starting at element a, SS_LANE per lane; + 0 past the segment leaves the sum unchanged
+__device__ __forceinline__ SSTerms ss_load_terms(const float *__restrict__ v, uint32_t a, uint32_t e_end, int mode,
+                                                 float mean) {
+  SSTerms t;
 #pragma unroll
   for (int q = 0; q < SS_LANE; ++q) {
-    const uint32_t i = a + (uint32_t)l * SS_LANE + q;
-    x[q] = i < e_end ? ss_term(v[i], mode, mean) : 0.0f;  // + 0 leaves the sum unchanged
+    const uint32_t i = a + (uint32_t)lane() * SS_LANE + q;
+    t.x[q] = i < e_end ? ss_term(v[i], mode, mean) : 0.0f;
   }
-  for (int q = 0; q < WAVE; ++q) {
+  return t;
+}
+
+__device__ __forceinline__ float ss_replay(float acc, const SSTerms &t) {
+  const int l = lane();
+  const float *x = t.x;
+  bool nz = false;
 #pragma unroll
-    for (int r = 0; r < SS_LANE; ++r) acc = acc + readlanef(x[r], q);
+  for (int q = 0; q < SS_LANE; ++q) nz = nz || x[q] != 0.0f;
+  int pos = 0;
+  while (pos < WAVE) {
+    if (acc >= 0x1p-126f && acc < 0x1p127f) {
+      const int e = ilogbf(acc);
+      const double scale = ldexp(1.0, 23 - e);
+      int fe = 0, fo = 0;  // identity for the lanes already added
+      if (l >= pos) {
+#pragma unroll
+        for (int q = 0; q < SS_LANE; ++q) {
+          int ae, ao;
+          ss_term_map(x[q], scale, ae, ao);
+          ss_compose(fe, fo, ae, ao);
+        }
+      }
+      ss_scan_maps(fe, fo);
+      const int M0 = (int)ldexpf(acc, 23 - e);
+      const int Ml = M0 + ((M0 & 1) ? fo : fe);
+      const uint64_t bad = ballot(l >= pos && Ml >= (1 << 24));
+      const int upto = bad ? __ffsll((unsigned long long)bad) - 1 : WAVE;  // lanes [pos, upto) accepted
+      if (upto > pos) {
+        acc = ldexpf((float)__builtin_amdgcn_readlane(Ml, upto - 1), e - 23);
+        pos = upto;
+      }
+    } else if (__float_as_uint(acc) == 0u) {
+      const uint64_t live = ballot(l >= pos && nz);
+      pos = live ? __ffsll((unsigned long long)live) - 1 : WAVE;
+    }
+    if (pos >= WAVE) break;
+#pragma unroll
+    for (int r = 0; r < SS_LANE; ++r) acc = acc + readlanef(x[r], pos);
+    ++pos;
   }
   return acc;
 }
@@ -183,7 +267,7 @@ __device__ __forceinline__ float ss_replay(float acc, const float *__restrict__ 
 // composes their maps across the lanes, every lane evaluates the state after its chunk from the run's
 // start state, and the run is accepted up to the first chunk whose state leaves the binade (M >= 2^24;
 // the sum never decreases, so every earlier state stayed inside).  That chunk, and any chunk outside such
-// a run, is replayed element by element.
+// a run, goes through ss_replay.
 __global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, const uint32_t *__restrict__ seg,
                                                   const uint32_t *__restrict__ coff, int nseg, int mode,
                                                   const float *__restrict__ mean_dev,
@@ -194,9 +278,17 @@ __global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, co
   const float mean = ss_mean(mode, mean_dev, seg, s);
   const uint32_t c_begin = coff[s], c_end = coff[s + 1], e_end = seg[s + 1];
   float acc = 0.0f;
+  const uint32_t e_begin = seg[s];
+  // the next window's maps and the terms of the next chunk that is known to be replayed (one the binade
+  // prediction left out) are loaded ahead, so their latency overlaps the work before them
+  SSChunk mine = (c_begin + l < c_end) ? ch[c_begin + l] : SSChunk{0, 0, 0, 0};
   for (uint32_t c0 = c_begin; c0 < c_end; c0 += WAVE) {
-    const SSChunk mine = (c0 + l < c_end) ? ch[c0 + l] : SSChunk{0, 0, 0, 0};
+    const SSChunk nxt = (c0 + WAVE + l < c_end) ? ch[c0 + WAVE + l] : SSChunk{0, 0, 0, 0};
     const int cnt = (int)min<uint32_t>(WAVE, c_end - c0);
+    const uint64_t unclean = ballot(l < cnt && !mine.clean);
+    int pf = unclean ? __ffsll((unsigned long long)unclean) - 1 : -1;
+    SSTerms pft;
+    if (pf >= 0) pft = ss_load_terms(v, e_begin + (c0 + pf - c_begin) * SS_C, e_end, mode, mean);
     int pos = 0;
     while (pos < cnt) {
       // binade of the running sum (normal, finite, positive), else no map applies
@@ -206,34 +298,33 @@ __global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, co
       const uint64_t stop = ~elig & (~0ull << pos);
       const int run_end = stop ? min(cnt, __ffsll((unsigned long long)stop) - 1) : cnt;
       if (run_end > pos) {
-        long long fe = 0, fo = 0;  // identity outside the run
+        int fe = 0, fo = 0;  // identity outside the run
         if (l >= pos && l < run_end) {
           fe = mine.ie;
           fo = mine.io;
         }
-#pragma unroll
-        for (int d = 1; d < WAVE; d <<= 1) {
-          long long pe = __shfl_up(fe, d), po = __shfl_up(fo, d);
-          if (l >= d) {
-            ss_compose(pe, po, fe, fo);  // earlier maps first
-            fe = pe;
-            fo = po;
-          }
-        }
-        const long long M0 = (long long)ldexpf(acc, 23 - e_acc);
-        const long long Ml = M0 + ((M0 & 1) ? fo : fe);
-        const uint64_t bad = ballot(l >= pos && l < run_end && Ml >= (1ll << 24));
+        ss_scan_maps(fe, fo);
+        const int M0 = (int)ldexpf(acc, 23 - e_acc);
+        const int Ml = M0 + ((M0 & 1) ? fo : fe);
+        const uint64_t bad = ballot(l >= pos && l < run_end && Ml >= (1 << 24));
         const int upto = bad ? __ffsll((unsigned long long)bad) - 1 : run_end;  // chunks [pos, upto) accepted
         if (upto > pos) {
-          const long long Mf = (long long)readlane64((uint64_t)Ml, upto - 1);
-          acc = ldexpf((float)Mf, e_acc - 23);
+          acc = ldexpf((float)__builtin_amdgcn_readlane(Ml, upto - 1), e_acc - 23);
           pos = upto;
           continue;
         }
       }
-      acc = ss_replay(acc, v, seg[s] + (c0 + pos - c_begin) * SS_C, e_end, mode, mean);
+      if (pos == pf) {
+        acc = ss_replay(acc, pft);
+        const uint64_t later = unclean & (~0ull << pos) & ~(1ull << pos);
+        pf = later ? __ffsll((unsigned long long)later) - 1 : -1;
+        if (pf >= 0) pft = ss_load_terms(v, e_begin + (c0 + pf - c_begin) * SS_C, e_end, mode, mean);
+      } else {
+        acc = ss_replay(acc, ss_load_terms(v, e_begin + (c0 + pos - c_begin) * SS_C, e_end, mode, mean));
+      }
       ++pos;
     }
+    mine = nxt;
   }
   if (l == 0) out[s] = acc;
 }

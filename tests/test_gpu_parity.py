@@ -366,7 +366,8 @@ def _gpu_seq_sum(v, squared=False, mean=0.0):
     return out.numpy()[0]
 
 
-SUM_KINDS = ["uniform", "ties", "wide", "zeros_first", "tiny", "large", "nan"]
+SUM_KINDS = ["uniform", "ties", "wide", "zeros_first", "tiny", "large", "nan", "growing", "sparse", "subnormal",
+             "inf", "signed"]
 
 
 @pytest.mark.parametrize("kind", SUM_KINDS)
@@ -384,6 +385,18 @@ def test_exact_sequential_sum(kind):
         v = rng.random(37)
     elif kind == "large":
         v = rng.random(3_000_000) * 0.1
+    elif kind == "growing":   # the sum crosses a binade every few hundred terms, inside chunks and lanes
+        v = 1.02 ** np.arange(1500) * (1 + rng.random(1500))
+    elif kind == "sparse":    # long runs of zeros, also at the start of the sum
+        v = np.where(rng.random(120_000) < 0.01, rng.random(120_000), 0.0)
+        v[:700] = 0.0
+    elif kind == "subnormal":  # a subnormal running sum for a while, then normal
+        v = np.concatenate([np.full(3000, 2.0 ** -140), rng.random(20_000) * 2.0 ** -100, rng.random(5000)])
+    elif kind == "inf":
+        v = rng.random(20_000)
+        v[12_345] = np.inf
+    elif kind == "signed":    # outside the SOR contract (terms >= 0); the sum is still the fp32 adder's
+        v = rng.normal(0, 1, 30_000)
     else:
         v = rng.random(10_000)
         v[7777] = np.nan
@@ -391,7 +404,7 @@ def test_exact_sequential_sum(kind):
     got = _gpu_seq_sum(v)
     exp = O.seq_sum(v)
     assert (np.isnan(got) and np.isnan(exp)) or got == exp
-    if kind != "nan":
+    if kind not in ("nan", "inf"):
         mean = np.float32(exp / np.float32(len(v)))
         assert _gpu_seq_sum(v, True, mean) == O.seq_sum(v, True, mean)
 
