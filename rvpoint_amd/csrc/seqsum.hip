@@ -63,17 +63,18 @@ __device__ __forceinline__ void ss_compose(int &fe, int &fo, int ge, int go) {
   fo = no;
 }
 
-// The step map (ae, ao) of one term x in binade e, scale = 2^(23 - e): M -> M + (M even ? ae : ao).  A
-// term that is negative, non-finite or of SS_SAT units or more gets the saturated map, so the state check
-// (M < 2^24) fails and the term is added by the fp32 adder instead.
-__device__ __forceinline__ void ss_term_map(float x, double scale, int &ae, int &ao) {
-  double t = (double)x * scale;  // exact: power-of-two scaling
-  if (!(t >= 0.0 && t < (double)SS_SAT)) t = (double)SS_SAT;
-  const double a = floor(t), fr = t - a;
+// The step map (ae, ao) of one term x in binade e, k = 23 - e: M -> M + (M even ? ae : ao).  A term that
+// is negative, non-finite or of SS_SAT units or more gets the saturated map, so the state check (M < 2^24)
+// fails and the term is added by the fp32 adder instead.  t = x * 2^k is exact unless it underflows, and
+// an underflowed t is below 2^-126, which maps to 0 like the exact one; floor and t - floor(t) are exact.
+__device__ __forceinline__ void ss_term_map(float x, int k, int &ae, int &ao) {
+  float t = ldexpf(x, k);
+  if (!(t >= 0.0f && t < (float)SS_SAT)) t = (float)SS_SAT;
+  const float a = floorf(t), fr = t - a;
   const int ai = (int)a;
-  if (fr > 0.5) {
+  if (fr > 0.5f) {
     ae = ao = ai + 1;
-  } else if (fr < 0.5) {
+  } else if (fr < 0.5f) {
     ae = ao = ai;
   } else {  // exact tie: the result M + a (+1) must be even
     ae = ai + (ai & 1);
@@ -121,6 +122,21 @@ __global__ void ss_nchunks_k(const uint32_t *__restrict__ seg, int nseg, uint32_
   if (s < nseg) nch[s] = (seg[s + 1] - seg[s] + SS_C - 1) / SS_C;
 }
 
+// the same counts and their exclusive scan (coff[nseg] = total) in one wave, for few segments
+__global__ __launch_bounds__(64) void ss_chunk_offsets_k(const uint32_t *__restrict__ seg, int nseg,
+                                                          uint32_t *__restrict__ coff) {
+  const int l = lane();
+  uint32_t carry = 0;
+  for (int b0 = 0; b0 < nseg; b0 += WAVE) {
+    const int s = b0 + l;
+    const uint32_t c = s < nseg ? (seg[s + 1] - seg[s] + SS_C - 1) / SS_C : 0u;
+    const uint32_t inc = wave_incl_scan(c);
+    if (s < nseg) coff[s] = carry + inc - c;
+    carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
+  }
+  if (l == 0) coff[nseg] = carry;
+}
+
 struct ChunkRef {
   int s;          // segment
   uint32_t a, b;  // element range
@@ -137,34 +153,71 @@ __device__ __forceinline__ bool chunk_ref(const uint32_t *__restrict__ seg, cons
   return true;
 }
 
+// chunk sums of the terms in fp64 (csum[c]); with csq, mode 0 also stores the sums of their squares, from
+// which the variance pass (mode 2) predicts its chunk sums as S2 - 2 m S1 + len m^2 without a pass of its own
 __global__ __launch_bounds__(64) void ss_chunksum_k(const float *__restrict__ v, const uint32_t *__restrict__ seg,
                                                      const uint32_t *__restrict__ coff, int nseg, int mode,
-                                                     const float *__restrict__ mean_dev, double *__restrict__ csum) {
+                                                     const float *__restrict__ mean_dev, double *__restrict__ csum,
+                                                     double *__restrict__ csq) {
   ChunkRef r;
   if (!chunk_ref(seg, coff, nseg, blockIdx.x, r)) return;
   const float mean = ss_mean(mode, mean_dev, seg, r.s);
-  double acc = 0.0;
+  double acc = 0.0, acc2 = 0.0;
 #pragma unroll
   for (int q = 0; q < SS_LANE; ++q) {
     const uint32_t i = r.a + (uint32_t)lane() * SS_LANE + q;
-    if (i < r.b) acc += (double)ss_term(v[i], mode, mean);
+    if (i < r.b) {
+      const double t = (double)ss_term(v[i], mode, mean);
+      acc += t;
+      acc2 += t * t;
+    }
   }
   acc = wave_sum_f64(acc);
-  if (lane() == 0) csum[blockIdx.x] = acc;
+  if (csq) acc2 = wave_sum_f64(acc2);
+  if (lane() == 0) {
+    csum[blockIdx.x] = acc;
+    if (csq) csq[blockIdx.x] = acc2;
+  }
+}
+
+// predicted fp64 sum of chunk j's terms: csum[j], or in mode 2 from the moments of mode 0's terms
+__device__ __forceinline__ double ss_chunk_pred(const double *__restrict__ csum, const double *__restrict__ csq,
+                                                uint32_t j, double m, double len) {
+  return csq ? csq[j] - 2.0 * m * csum[j] + len * m * m : csum[j];
 }
 
 __global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, const uint32_t *__restrict__ seg,
                                                     const uint32_t *__restrict__ coff, int nseg, int mode,
                                                     const float *__restrict__ mean_dev,
-                                                    const double *__restrict__ csum, SSChunk *__restrict__ out) {
+                                                    const double *__restrict__ csum,
+                                                    const double *__restrict__ csq, SSChunk *__restrict__ out) {
   const uint32_t c = blockIdx.x;
   ChunkRef r;
   if (!chunk_ref(seg, coff, nseg, c, r)) return;
   const int l = lane();
+  // the chunk's terms are read before the prefix sum, so the two round trips overlap
+  const float mean = ss_mean(mode, mean_dev, seg, r.s);
+  float x[SS_LANE];
+#pragma unroll
+  for (int q = 0; q < SS_LANE; ++q) {
+    const uint32_t i = r.a + (uint32_t)l * SS_LANE + q;
+    x[q] = i < r.b ? ss_term(v[i], mode, mean) : 0.0f;  // map (0, 0): the identity
+  }
+  // the segment's earlier chunk sums, four loads in flight per lane (chunks before c are full)
+  const double md = (double)mean;
   double p = 0.0;
-  for (uint32_t j = r.c0 + l; j < c; j += WAVE) p += csum[j];
+  for (uint32_t j0 = r.c0; j0 < c; j0 += 4 * WAVE) {
+    double t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t j = j0 + (uint32_t)(u * WAVE + l);
+      t[u] = j < c ? ss_chunk_pred(csum, csq, j, md, (double)SS_C) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p += t[u];
+  }
   p = wave_sum_f64(p);
-  const double lo = p, hi = p + csum[c];
+  const double lo = p, hi = p + ss_chunk_pred(csum, csq, c, md, (double)(r.b - r.a));
   int e = 0;
   bool clean = false;
   if (c > r.c0 && lo > 0.0 && hi < 1e38) {
@@ -178,17 +231,12 @@ __global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, 
     if (l == 0) out[c] = SSChunk{0, 0, 0, 0};
     return;
   }
-  const float mean = ss_mean(mode, mean_dev, seg, r.s);
-  const double scale = ldexp(1.0, 23 - e);
   int fe = 0, fo = 0;
 #pragma unroll
   for (int q = 0; q < SS_LANE; ++q) {
-    const uint32_t i = r.a + (uint32_t)l * SS_LANE + q;
-    if (i < r.b) {
-      int ae, ao;
-      ss_term_map(ss_term(v[i], mode, mean), scale, ae, ao);
-      ss_compose(fe, fo, ae, ao);
-    }
+    int ae, ao;
+    ss_term_map(x[q], 23 - e, ae, ao);
+    ss_compose(fe, fo, ae, ao);
   }
   // ordered tree: lane l composes its segment with the one of lane l + d
 #pragma unroll
@@ -231,13 +279,12 @@ __device__ __forceinline__ float ss_replay(float acc, const SSTerms &t) {
   while (pos < WAVE) {
     if (acc >= 0x1p-126f && acc < 0x1p127f) {
       const int e = ilogbf(acc);
-      const double scale = ldexp(1.0, 23 - e);
       int fe = 0, fo = 0;  // identity for the lanes already added
       if (l >= pos) {
 #pragma unroll
         for (int q = 0; q < SS_LANE; ++q) {
           int ae, ao;
-          ss_term_map(x[q], scale, ae, ao);
+          ss_term_map(x[q], 23 - e, ae, ao);
           ss_compose(fe, fo, ae, ao);
         }
       }
@@ -334,19 +381,32 @@ static int ss_chunk_offsets(const uint32_t *seg_off, int nseg, size_t n_bound, T
                             hipStream_t s) {
   *bound = n_bound / SS_C + (size_t)nseg + 1;  // chunks: sum of ceil(len / SS_C)
   RVP_TRY(coff.alloc((nseg + 1) * sizeof(uint32_t), s));
+  if (nseg <= 4096) {  // one wave walks the segments: one launch instead of two
+    RVP_LAUNCH("ss_chunk_offsets_k", s, ss_chunk_offsets_k<<<1, 64, 0, s>>>(seg_off, nseg, coff.as<uint32_t>()));
+    return RVP_OK;
+  }
   RVP_LAUNCH("ss_nchunks_k", s, ss_nchunks_k<<<ceil_div(nseg, 256), 256, 0, s>>>(seg_off, nseg, coff.as<uint32_t>()));
   return exclusive_scan_u32(coff.as<uint32_t>(), coff.as<uint32_t>(), nseg, coff.as<uint32_t>() + nseg, s);
 }
 
+// One sum.  moments (2 * bound doubles) links a mode 0 run to the mode 2 run after it: mode 0 fills it
+// with the chunk sums and sums of squares of its terms, mode 2 predicts from it and skips its own
+// chunk-sum pass.
 static int ss_run(const float *v, const uint32_t *seg_off, const uint32_t *coff, int nseg, size_t bound, int mode,
-                  const float *mean_dev, float *out, hipStream_t s) {
+                  const float *mean_dev, float *out, double *moments, hipStream_t s) {
   Tmp cs, ch;
-  RVP_TRY(cs.alloc(bound * sizeof(double), s));
+  double *csum = moments, *csq = moments ? moments + bound : nullptr;
+  if (!moments) {
+    RVP_TRY(cs.alloc(bound * sizeof(double), s));
+    csum = cs.as<double>();
+  }
   RVP_TRY(ch.alloc(bound * sizeof(SSChunk), s));
-  RVP_LAUNCH("ss_chunksum_k", s, ss_chunksum_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff, nseg, mode, mean_dev,
-                                                                              cs.as<double>()));
+  if (mode != 2 || !moments)
+    RVP_LAUNCH("ss_chunksum_k", s, ss_chunksum_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff, nseg, mode,
+                                                                                mean_dev, csum, csq));
   RVP_LAUNCH("ss_compose_k", s, ss_compose_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff, nseg, mode, mean_dev,
-                                                                            cs.as<double>(), ch.as<SSChunk>()));
+                                                                            csum, mode == 2 ? csq : nullptr,
+                                                                            ch.as<SSChunk>()));
   RVP_LAUNCH("ss_drive_k", s, ss_drive_k<<<nseg, 64, 0, s>>>(v, seg_off, coff, nseg, mode, mean_dev,
                                                              ch.as<SSChunk>(), out));
   return RVP_OK;
@@ -358,17 +418,18 @@ int exact_seq_sum_seg(const float *v, const uint32_t *seg_off, int nseg, size_t 
   Tmp coff;
   size_t bound = 0;
   RVP_TRY(ss_chunk_offsets(seg_off, nseg, n_bound, coff, &bound, s));
-  return ss_run(v, seg_off, coff.as<uint32_t>(), nseg, bound, mode, mean_dev, out, s);
+  return ss_run(v, seg_off, coff.as<uint32_t>(), nseg, bound, mode, mean_dev, out, nullptr, s);
 }
 
 int exact_seq_mean_var_seg(const float *v, const uint32_t *seg_off, int nseg, size_t n_bound, float *sum_out,
                            float *sq_out, hipStream_t s) {
   if (nseg <= 0) return RVP_OK;
-  Tmp coff;
+  Tmp coff, moments;
   size_t bound = 0;
   RVP_TRY(ss_chunk_offsets(seg_off, nseg, n_bound, coff, &bound, s));
-  RVP_TRY(ss_run(v, seg_off, coff.as<uint32_t>(), nseg, bound, 0, nullptr, sum_out, s));
-  return ss_run(v, seg_off, coff.as<uint32_t>(), nseg, bound, 2, sum_out, sq_out, s);
+  RVP_TRY(moments.alloc(2 * bound * sizeof(double), s));
+  RVP_TRY(ss_run(v, seg_off, coff.as<uint32_t>(), nseg, bound, 0, nullptr, sum_out, moments.as<double>(), s));
+  return ss_run(v, seg_off, coff.as<uint32_t>(), nseg, bound, 2, sum_out, sq_out, moments.as<double>(), s);
 }
 
 __global__ void one_segment_k(uint32_t n, uint32_t *__restrict__ seg) {

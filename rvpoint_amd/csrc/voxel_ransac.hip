@@ -745,8 +745,12 @@ int rvp_batch_voxel(const float *x, const float *y, const float *z, const uint32
   uint64_t *kp = keys.as<uint64_t>(), *ktp = kt.as<uint64_t>();
   uint32_t *vp = vals.as<uint32_t>(), *vtp = vt.as<uint32_t>();
   const uint32_t *n_dev = off + nclouds;
-  // every input point has a key: the bound is the count, so large sorts take the large tiles
-  RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n_bound, 64, s, n_dev, bits.as<int>(), n_bound >= (1u << 19)));
+  // every input point has a key: the bound is the count, so large sorts take the large tiles.  Each cloud
+  // spans fewer than 2^48 keys (seg_vox_base_k), so the keys fit 48 + bits_for(nclouds - 1) bits: the
+  // passes above that are not launched at all (the device-side width skips the empty ones below it).
+  const int key_bits = std::min(64, 48 + bits_for((uint64_t)nclouds - 1));
+  RVP_TRY(radix_sort_pairs_u64(&kp, &vp, &ktp, &vtp, n_bound, key_bits, s, n_dev, bits.as<int>(),
+                               n_bound >= (1u << 19)));
   RVP_TRY(vid.alloc(n_bound * sizeof(uint32_t), s));
   RVP_TRY(nv.alloc(sizeof(uint32_t), s));
   RVP_TRY(start.alloc(n_bound * sizeof(uint32_t), s));
