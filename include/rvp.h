@@ -72,8 +72,9 @@ int rvp_profile_collect(char *buf, size_t buflen, size_t *needed);
  *   "knn_cell_div"         k-NN grid edge = estimated k-th neighbour distance / this (default 2.4, [0.25, 16])
  *   "seg_cells_per_point"  dense cells a batched grid may give each point of a cloud before coarsening
  *                          (default 32, [1, 64]; at least 2^17 cells per cloud)
- *   "queries_per_wave"     consecutive queries one wave serves in the capped radius searches (0 = from the
- *                          query count: 64 for a few million queries and more, fewer below; [0, 64])
+ *   "queries_per_wave"     consecutive queries one wave serves in the capped radius searches and the k-NN
+ *                          SOR means (0 = from the query count: radius 64 for a few million queries and more,
+ *                          fewer below; SOR means 2 up to a quarter million, at most 16; [0, 64])
  *   "seg_scan_table_min"   batches of at least this many points (their bound) build the cell table from run
  *                          lengths and one scan; smaller ones fill the gaps between runs (default 4194304)
  * set() returns RVP_E_ARG for an unknown name or a value out of range. */

@@ -562,22 +562,40 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
 // batched stages
 // ------------------------------------------------------------------------------------------------
 // S1 per cloud: k+1 nearest within the query's cloud, mean of sqrt of entries 1..k (sequential)
-__global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, float *__restrict__ means) {
+// Waves take runs of consecutive queries (cell order, run length from the device-side count; `forced` > 0
+// fixes it); within a cloud, the previous query's k-th distance bounds the next one's (knn_bound_from).
+// Six waves per SIMD: the register budget is 80 VGPRs.
+__global__ __launch_bounds__(QT) __attribute__((amdgpu_waves_per_eu(6))) void seg_sor_means_k(SegArrays A, int k, int forced,
+                                                                                           float *__restrict__ means) {
   __shared__ WaveLds lds[QW];
   const int w = threadIdx.x >> 6;
   const uint32_t total = A.off[A.ncl];
-  for (uint32_t q = wave_id(); q < total; q += wave_count()) {
-    const int c = cloud_of(A.off, A.ncl, q);
-    const GridView g = seg_view(A, c);
-    TopKT<uint32_t> t;
-    t.init(k + 1);
-    knn_search<false>(g, g.xs[q], g.ys[q], g.zs[q], t, lds[w]);
-    const int l = lane();
-    const int m = min(t.cnt, k + 1);
-    const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
-    float s = 0.0f;
-    for (int j = 1; j <= k; ++j) s = s + readlanef(sq, j);
-    if (l == 0) means[g.perm[q]] = s / __int2float_rn(k);
+  const uint32_t pw = (uint32_t)knn_run_length(total, forced);
+  const uint32_t nruns = (total + pw - 1) / pw;
+  for (uint32_t run = wave_id(); run < nruns; run += wave_count()) {
+    const uint32_t q1 = min(run * pw + pw, total);
+    int pc = -1;  // cloud of the previous query whose k-th d2 is pd2
+    float px = 0.f, py = 0.f, pz = 0.f, pd2 = -1.0f;
+    for (uint32_t q = run * pw; q < q1; ++q) {
+      const int c = cloud_of(A.off, A.ncl, q);
+      const GridView g = seg_view(A, c);
+      const float x = uniform(g.xs[q]), y = uniform(g.ys[q]), z = uniform(g.zs[q]);
+      TopKT<uint32_t> t;
+      t.init(k + 1);
+      knn_search<false>(g, x, y, z, t, lds[w],
+                        c == pc ? knn_bound_from(x, y, z, px, py, pz, pd2) : __builtin_inff());
+      const int l = lane();
+      const int m = min(t.cnt, k + 1);
+      const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
+      float s = 0.0f;
+      for (int j = 1; j <= k; ++j) s = s + readlanef(sq, j);
+      if (l == 0) means[g.perm[q]] = s / __int2float_rn(k);
+      pc = c;
+      px = x;
+      py = y;
+      pz = z;
+      pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
+    }
   }
 }
 
@@ -683,7 +701,8 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
     RVP_TRY(mt.alloc(n_bound * sizeof(float), s));
     mp = mt.as<float>();
   }
-  RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<<<query_blocks(n_bound), QT, 0, s>>>(G.arrays(), k, mp));
+  RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<<<query_blocks(n_bound), QT, 0, s>>>(G.arrays(), k,
+                                                                                      tune_queries_per_wave(), mp));
   RVP_TRY(sums.alloc(nclouds * sizeof(float), s));
   RVP_TRY(vsum.alloc(nclouds * sizeof(float), s));
   // statistical_outlier_removal.cpp:44-51: the sum of the means, then of the squared deviations from their

@@ -207,6 +207,10 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
 __device__ __forceinline__ float readlanef(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
+// a value every lane holds alike, moved to a scalar register
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 __device__ __forceinline__ uint32_t readlaneu(uint32_t v, int l) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
 }

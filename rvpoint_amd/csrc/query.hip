@@ -316,20 +316,32 @@ __global__ __launch_bounds__(QT) void knn_k(GridView g, const float *__restrict_
 // ------------------------------------------------------------------------------------------------
 // S1 (statistical_outlier_removal.cpp:27-41): k+1 smallest d2 incl. self; sum of sqrt of entries 1..k in
 // ascending order, sequentially; divide by k.
-__global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, float *__restrict__ means) {
+// Each wave takes a run of per_wave consecutive queries (cell order): from the second on, the previous
+// query's k-th distance bounds the current one (knn_bound_from), so the first box is already the certified
+// one and only hits within the bound are staged.
+__global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, int per_wave, float *__restrict__ means) {
   __shared__ WaveLds lds[QW];
   const int w = threadIdx.x >> 6;
-  const uint32_t q = blockIdx.x * QW + w;
-  if (q >= g.n) return;
-  TopKT<uint32_t> t;  // distances only: the means read the multiset of the k+1 smallest
-  t.init(k + 1);
-  knn_search<false>(g, g.xs[q], g.ys[q], g.zs[q], t, lds[w]);
-  const int l = lane();
-  const int m = min(t.cnt, k + 1);
-  const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
-  float s = 0.0f;
-  for (int j = 1; j <= k; ++j) s = __fadd_rn(s, readlanef(sq, j));  // +0 beyond m leaves s unchanged
-  if (l == 0) means[g.perm[q]] = __fdiv_rn(s, __int2float_rn(k));
+  const uint32_t q0 = (blockIdx.x * QW + w) * (uint32_t)per_wave;
+  if (q0 >= g.n) return;
+  const uint32_t q1 = min(q0 + (uint32_t)per_wave, g.n);
+  float px = 0.f, py = 0.f, pz = 0.f, pd2 = -1.0f;  // previous query and its k-th d2 (< 0: none)
+  for (uint32_t q = q0; q < q1; ++q) {
+    const float x = uniform(g.xs[q]), y = uniform(g.ys[q]), z = uniform(g.zs[q]);
+    TopKT<uint32_t> t;  // distances only: the means read the multiset of the k+1 smallest
+    t.init(k + 1);
+    knn_search<false>(g, x, y, z, t, lds[w], knn_bound_from(x, y, z, px, py, pz, pd2));
+    const int l = lane();
+    const int m = min(t.cnt, k + 1);
+    const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
+    float s = 0.0f;
+    for (int j = 1; j <= k; ++j) s = __fadd_rn(s, readlanef(sq, j));  // +0 beyond m leaves s unchanged
+    if (l == 0) means[g.perm[q]] = __fdiv_rn(s, __int2float_rn(k));
+    px = x;
+    py = y;
+    pz = z;
+    pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
+  }
 }
 
 // S2 (statistical_outlier_removal.cpp:246-269): radius list capped at k+1; mean over min(k, m-1) entries
@@ -564,7 +576,8 @@ int rvp_sor_exact(const float *x, const float *y, const float *z, size_t n, int 
     RVP_TRY(mt.alloc(n * sizeof(float), s));
     mp = mt.as<float>();
   }
-  RVP_LAUNCH("sor_knn_means_k", s, sor_knn_means_k<<<ceil_div(n, QW), QT, 0, s>>>(G.view(), k, mp));
+  const int pw = knn_run_length(n, tune_queries_per_wave());
+  RVP_LAUNCH("sor_knn_means_k", s, sor_knn_means_k<<<ceil_div(ceil_div(n, pw), QW), QT, 0, s>>>(G.view(), k, pw, mp));
   return sor_tail(G, x, y, z, n, alpha, mp, out_aos, n_out, keep, s);
 }
 

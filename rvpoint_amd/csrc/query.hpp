@@ -39,7 +39,8 @@ template <class KT> struct TopKT {
   uint32_t pay;
   int cnt;       // valid entries (<= 64)
   int K;
-  KT kth;        // key at K-1 once cnt >= K, else all-ones (hits must beat it to be staged)
+  KT kth;        // key at K-1 once cnt >= K, else `bound` (hits must beat it to be staged)
+  KT bound;      // a key known to exceed the K-th smallest (all-ones: none known)
   int nb;        // staged entries
   uint32_t hits; // all hits offered (untruncated count)
   __device__ __forceinline__ void init(int k) {
@@ -48,6 +49,7 @@ template <class KT> struct TopKT {
     cnt = 0;
     K = k;
     kth = ~(KT)0;
+    bound = ~(KT)0;
     nb = 0;
     hits = 0;
   }
@@ -174,7 +176,7 @@ template <bool PAY, class KT> __device__ __forceinline__ void topk_flush(TopKT<K
   }
   t.cnt = min(W, t.cnt + t.nb);
   t.nb = 0;
-  t.kth = (t.cnt >= t.K) ? read_lane(t.key, t.K - 1) : ~(KT)0;
+  t.kth = (t.cnt >= t.K) ? read_lane(t.key, t.K - 1) : t.bound;
 }
 
 template <bool PAY, class KT>
@@ -310,6 +312,15 @@ inline int queries_per_wave(size_t nq) {
   return (int)std::max<size_t>(1, std::min<size_t>(WAVE, nq / target_waves));
 }
 
+// consecutive queries per wave for the k-NN means: a run ends the chip's work unevenly (the last runs of
+// the last round), so runs stay short until there are dozens of rounds of resident waves: 2 up to a quarter
+// million queries, nq / 2^17 above, at most 16; a positive `forced` (the queries_per_wave knob) overrides it
+__host__ __device__ inline int knn_run_length(size_t nq, int forced) {
+  if (forced > 0) return forced;
+  const size_t r = nq >> 17;
+  return (int)(r < 2 ? 2 : r > 16 ? 16 : r);
+}
+
 // A wave's copy of the candidates of one window: x, y, z and the original index (as float bits) of every point
 // of the box, in slice order.  Consecutive queries of a run are mostly in one cell and then have the same
 // window, so one staging serves them all and each query reads its candidates from LDS, 64 per instruction.
@@ -407,32 +418,73 @@ template <bool PAY, class KT = uint64_t> struct TopKSink {
   }
 };
 
+// the smallest key above every key whose d2 is at most b (b finite, >= 0)
+template <class KT> __device__ __forceinline__ KT bound_key(float b) {
+  return as_key<KT>((uint64_t)(__float_as_uint(b) + 1u) << 32);
+}
+
+// An upper bound on the K-th smallest d2 at query q from the previous query p of the same cloud, whose K-th
+// smallest d2 is pd2: p's K nearest lie within sqrt(pd2) + |q - p| of q (triangle inequality).  Computed
+// d2 values are within a relative 5 * 2^-24 of the true squared distances, and the evaluation below loses
+// at most 6 * 2^-24 more, so the 2^-16 margin covers both.  +inf (no bound) when the step exceeds half the
+// previous K-th distance (the bound would admit too many candidates to pay), for tiny or non-finite pd2.
+__device__ __forceinline__ float knn_bound_from(float qx, float qy, float qz, float px, float py, float pz,
+                                                float pd2) {
+  if (!(pd2 >= 0x1p-60f && pd2 < 0x1p100f)) return __builtin_inff();
+  const float dx = __fsub_rn(qx, px), dy = __fsub_rn(qy, py), dz = __fsub_rn(qz, pz);
+  const float dd = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz)));
+  const float dp = sqrtf(pd2);
+  if (!(dd <= 0.5f * dp)) return __builtin_inff();
+  const float sm = __fadd_rn(dp, dd);
+  return __fmul_rn(__fmul_rn(sm, sm), 1.0f + 0x1p-16f);
+}
+
 // Exact k nearest by (d2, idx): 3x3x3 cells around the query, doubling growth until k candidates are
 // known, then the box certified by window3(q, R) with fl(R*R) > d2_k: every point outside it has a
-// larger d2, so the k best are final.
+// larger d2, so the k best are final.  With a finite bound_d2 (at least the K-th smallest d2), the first
+// box is the one certified for bound_d2 and only hits within it are staged: once it yields K entries the
+// certification holds at once; if it yields fewer, the search starts over without the bound.
 template <bool PAY, class KT>
 __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy, float qz, TopKT<KT> &t,
-                                           WaveLds &L) {
-  // the query's cell up to rounding: any box is a valid start, the certification below makes the result exact
-  const int c[3] = {floor_clamped(cell_coord(qx, g.mnx, g.icell), g.gx),
-                    floor_clamped(cell_coord(qy, g.mny, g.icell), g.gy),
-                    floor_clamped(cell_coord(qz, g.mnz, g.icell), g.gz)};
+                                           WaveLds &L, float bound_d2 = __builtin_inff()) {
   const int gd[3] = {g.gx, g.gy, g.gz};
   int lo[3], hi[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    lo[a] = max(c[a] - 1, 0);
-    hi[a] = min(c[a] + 1, gd[a] - 1);
-  }
   TopKSink<PAY, KT> sink{t, L};
+  const bool bounded = bound_d2 < FLT_MAX;
+  if (bounded) {
+    const float R = next_up(__fmul_rn(sqrtf(bound_d2), 1.0f + 0x1p-22f));
+    window3(g, qx, qy, qz, R, lo, hi);
+    t.bound = t.kth = bound_key<KT>(bound_d2);
+  } else {
+    // the query's cell up to rounding: any box is a valid start, the certification below makes the result
+    // exact
+    const int c[3] = {floor_clamped(cell_coord(qx, g.mnx, g.icell), g.gx),
+                      floor_clamped(cell_coord(qy, g.mny, g.icell), g.gy),
+                      floor_clamped(cell_coord(qz, g.mnz, g.icell), g.gz)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = max(c[a] - 1, 0);
+      hi[a] = min(c[a] + 1, gd[a] - 1);
+    }
+  }
   scan_box(g, qx, qy, qz, lo, hi, false, lo, hi, 0.0f, false, L, sink);
+  // a bounded box with fewer than K entries (a bound below the K-th d2): forget it and grow from the box,
+  // rescanning it whole
+  bool rescan = false;
+  if (bounded) {
+    topk_flush<PAY>(t, L);
+    if (t.cnt < t.K) {
+      t.init(t.K);
+      rescan = true;
+    }
+  }
   int step = 1;
   for (;;) {
     const bool whole = lo[0] == 0 && lo[1] == 0 && lo[2] == 0 && hi[0] == gd[0] - 1 && hi[1] == gd[1] - 1 &&
                        hi[2] == gd[2] - 1;
     // while fewer than K candidates are known the box only grows: merging can wait
     if (whole || t.cnt + t.nb >= t.K) topk_flush<PAY>(t, L);
-    if (whole) break;
+    if (whole && !rescan) break;
     int nlo[3], nhi[3];
     if (t.cnt >= t.K) {
       const float dk = kd2(t.kth);
@@ -463,7 +515,8 @@ __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy
       }
       step = step < (1 << 20) ? step * 2 : step;
     }
-    scan_box(g, qx, qy, qz, nlo, nhi, true, lo, hi, 0.0f, false, L, sink);
+    scan_box(g, qx, qy, qz, nlo, nhi, !rescan, lo, hi, 0.0f, false, L, sink);
+    rescan = false;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       lo[a] = nlo[a];
