@@ -239,13 +239,19 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
   int nbatch = 0;
   for (uint32_t k = 0; wave_batch<STRIDED, NB>(k, g.n, first, step, nbatch); ++k) {
     StagedWindow win;
+    float px = 0.f, py = 0.f, pz = 0.f, pd2 = -1.0f;  // previous query of the batch, its K-th d2 (< 0: none)
     for (int j = 0; j < nbatch; ++j) {
       const uint32_t q = first + (uint32_t)j * step;
-      const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
+      const float x = uniform(g.xs[q]), y = uniform(g.ys[q]), z = uniform(g.zs[q]);
       TopK t;
       t.init(K);
       if (MODE == 0) {
-        knn_search<false>(g, x, y, z, t, L);
+        // consecutive queries: the previous one's K-th distance bounds this one's (knn_bound_from)
+        knn_search<false>(g, x, y, z, t, L, STRIDED ? __builtin_inff() : knn_bound_from(x, y, z, px, py, pz, pd2));
+        px = x;
+        py = y;
+        pz = z;
+        pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
       } else {
         int lo[3], hi[3];
         window3(g, x, y, z, radius, lo, hi);
@@ -352,15 +358,24 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
   int nbatch = 0;
   for (uint32_t k = 0; wave_batch<STRIDED, NB>(k, total, first, step, nbatch); ++k) {
     StagedWindow win;
+    int pc = -1;  // cloud of the previous query of the batch, whose K-th d2 is pd2
+    float px = 0.f, py = 0.f, pz = 0.f, pd2 = -1.0f;
     for (int j = 0; j < nbatch; ++j) {
       const uint32_t q = first + (uint32_t)j * step;
       const int c = cloud_of(A.off, A.ncl, q);
       const GridView g = seg_view(A, c);
-      const float x = g.xs[q], y = g.ys[q], z = g.zs[q];
+      const float x = uniform(g.xs[q]), y = uniform(g.ys[q]), z = uniform(g.zs[q]);
       TopK t;
       t.init(K);
       if (MODE == 0) {
-        knn_search<false>(g, x, y, z, t, lds[w]);
+        // consecutive queries of one cloud: the previous one's K-th distance bounds this one's
+        knn_search<false>(g, x, y, z, t, lds[w],
+                          (!STRIDED && c == pc) ? knn_bound_from(x, y, z, px, py, pz, pd2) : __builtin_inff());
+        pc = c;
+        px = x;
+        py = y;
+        pz = z;
+        pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
       } else {
         const float r = radius[c];
         const float r2 = __fmul_rn(r, r);
