@@ -470,8 +470,7 @@ __global__ void seg_plane_models_k(const float *__restrict__ x, const float *__r
 // tiles of RC_TILE points that never straddle clouds: tile_off[c] = first tile of cloud c (one wave)
 // (also copies the offsets into the read-back buffer)
 __global__ __launch_bounds__(64) void seg_tiles_k(const uint32_t *__restrict__ off, int ncl,
-                                                  uint32_t *__restrict__ tile_off, uint32_t *__restrict__ off_copy,
-                                                  uint8_t *__restrict__ active) {
+                                                  uint32_t *__restrict__ tile_off, uint32_t *__restrict__ off_copy) {
   const int l = lane();
   uint32_t carry = 0;
   for (int b0 = 0; b0 < ncl; b0 += 64) {
@@ -481,7 +480,6 @@ __global__ __launch_bounds__(64) void seg_tiles_k(const uint32_t *__restrict__ o
     if (c < ncl) {
       tile_off[c] = carry + inc - t;
       off_copy[c] = off[c];
-      active[c] = 1;  // every cloud takes part in the first round
     }
     carry += __shfl(inc, 63);
   }
@@ -794,18 +792,19 @@ int rvp_batch_ransac_plane(const float *x, const float *y, const float *z, const
   RVP_TRY(tiles.alloc((nclouds + 1) * sizeof(uint32_t), s));
   RVP_LAUNCH("seg_plane_models_k", s, seg_plane_models_k<<<ceil_div(NH, 256), 256, 0, s>>>(
                                           x, y, z, off, nclouds, draws, H, collinear_thresh, models, valid, counts));
+  RVP_LAUNCH("seg_tiles_k", s, seg_tiles_k<<<1, 64, 0, s>>>(off, nclouds, tiles.as<uint32_t>(), offs_copy));
   // Hypotheses are counted in rounds of 64, 128, ... and replayed after each: the reference's loop usually
   // stops long before max_iters (adaptively, ransac_plane.cpp:95-105), and a cloud whose loop has stopped
   // takes no part in later rounds.  The first read-back brings the models, flags and offsets too.
   Tmp act;
   RVP_TRY(act.alloc((size_t)nclouds, s));
-  RVP_LAUNCH("seg_tiles_k", s, seg_tiles_k<<<1, 64, 0, s>>>(off, nclouds, tiles.as<uint32_t>(), offs_copy,
-                                                            act.as<uint8_t>()));
   PinnedStage stage;
   uint8_t *host = nullptr;
   const size_t act_at = (words * sizeof(uint32_t) + 255) & ~(size_t)255;
   RVP_TRY(stage.get(s, act_at + (size_t)nclouds, reinterpret_cast<void **>(&host)));
   uint8_t *hact = host + act_at;
+  std::memset(hact, 1, (size_t)nclouds);
+  RVP_HIP(hipMemcpyAsync(act.p, hact, (size_t)nclouds, hipMemcpyHostToDevice, s));
   const float *hm = reinterpret_cast<const float *>(host);
   const int32_t *hv = reinterpret_cast<const int32_t *>(hm + NH * 4), *hc = hv + NH;
   const uint32_t *ho = reinterpret_cast<const uint32_t *>(hc + NH);
