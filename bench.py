@@ -142,7 +142,9 @@ class BatchPipeline:
         call, s, n = self.call, self.s, self.n
         call("rvp_batch_sor_exact", *self.cloud.ptrs(), self.off.ptr, self.ncl, n, self.K_SOR, 1.0, self.sor.ptr,
              self.sor_off.ptr, None, None, s)
-        call("rvp_aos_to_soa", self.sor.ptr, n, self.sx.ptr, self.sy.ptr, self.sz.ptr, s)
+        # the kept total is the last entry of the batch's output offsets, read on the device
+        call("rvp_aos_to_soa_bounded", self.sor.ptr, n, self.sor_off.ptr + 4 * self.ncl, self.sx.ptr, self.sy.ptr,
+             self.sz.ptr, s)
         kept = (self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, self.ncl, n)
         call("rvp_batch_radius_search", *kept, self.radius.ptr, None, self.CAP, self.idx.ptr, self.d2.ptr,
              self.cnt.ptr, s)
@@ -253,10 +255,11 @@ class DeviceFramePipeline:
         call, s, n = self.call, self.s, self.n
         call("rvp_batch_voxel", *self.frame.ptrs(), self.off.ptr, 1, n, self.leaf.ptr, self.vox.ptr,
              self.vox_off.ptr, self.status.ptr, s)
-        call("rvp_aos_to_soa", self.vox.ptr, n, self.vx.ptr, self.vy.ptr, self.vz.ptr, s)
+        # what the stage kept, counted on the device (the second entry of its output offsets)
+        call("rvp_aos_to_soa_bounded", self.vox.ptr, n, self.vox_off.ptr + 4, self.vx.ptr, self.vy.ptr, self.vz.ptr, s)
         call("rvp_batch_sor_exact", self.vx.ptr, self.vy.ptr, self.vz.ptr, self.vox_off.ptr, 1, n, self.k,
              self.alpha, self.sor.ptr, self.sor_off.ptr, None, None, s)
-        call("rvp_aos_to_soa", self.sor.ptr, n, self.sx.ptr, self.sy.ptr, self.sz.ptr, s)
+        call("rvp_aos_to_soa_bounded", self.sor.ptr, n, self.sor_off.ptr + 4, self.sx.ptr, self.sy.ptr, self.sz.ptr, s)
         kept = (self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, 1, n)
         call("rvp_batch_normals_radius", *kept, self.radius.ptr, None, self.max_nn, *self.vp, self.nx.ptr,
              self.ny.ptr, self.nz.ptr, s)

@@ -61,6 +61,11 @@ int rvp_event_elapsed(void *start, void *stop, float *ms);
 /* AoS float[3n] -> SoA x/y/z (the copy the reference's drivers make between stages,
  * src/tools/pipeline_export.cpp:336-342) */
 int rvp_aos_to_soa(const float *aos, size_t n, float *x, float *y, float *z, rvp_stream_t s);
+/* The same for min(*n_dev, n_bound) points, the count read on the device (n_dev: a device pointer, e.g.
+ * the last entry of a batched stage's output offsets), so a pipeline converts what a stage kept without a
+ * read-back (no counterpart in the reference, whose drivers know their counts on the host). */
+int rvp_aos_to_soa_bounded(const float *aos, size_t n_bound, const uint32_t *n_dev, float *x, float *y, float *z,
+                           rvp_stream_t s);
 /* Per-kernel timing: while enabled, every kernel launch of the library is bracketed by a HIP event pair on
  * the stream it runs on.  collect() synchronises, writes "kernel launches total_ms" lines into buf
  * (NUL-terminated, *needed = size required) and clears the records. */

@@ -51,6 +51,7 @@ _SIGS = {
     "rvp_event_record": [vp, vp],
     "rvp_event_elapsed": [vp, vp, pfl],
     "rvp_aos_to_soa": [vp, sz, vp, vp, vp, vp],
+    "rvp_aos_to_soa_bounded": [vp, sz, vp, vp, vp, vp, vp],
     "rvp_profile_enable": [i32],
     "rvp_profile_collect": [C.c_char_p, sz, psz],
     "rvp_profile_filter": [C.c_char_p],

@@ -497,11 +497,31 @@ __global__ void aos_to_soa_k(const float *__restrict__ aos, size_t n, float *__r
   y[i] = aos[3 * i + 1];
   z[i] = aos[3 * i + 2];
 }
+// grid-stride over min(*n_dev, n_bound) points
+__global__ void aos_to_soa_bounded_k(const float *__restrict__ aos, size_t n_bound, const uint32_t *__restrict__ n_dev,
+                                     float *__restrict__ x, float *__restrict__ y, float *__restrict__ z) {
+  const size_t n = min((size_t)*n_dev, n_bound);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    x[i] = aos[3 * i];
+    y[i] = aos[3 * i + 1];
+    z[i] = aos[3 * i + 2];
+  }
+}
 }  // namespace rvp
 
 extern "C" int rvp_aos_to_soa(const float *aos, size_t n, float *x, float *y, float *z, rvp_stream_t st) {
   if (n == 0) return RVP_OK;
   RVP_CHECK(aos && x && y && z, RVP_E_ARG, "null pointer");
   RVP_LAUNCH("aos_to_soa", S(st), aos_to_soa_k<<<ceil_div(n, 256), 256, 0, S(st)>>>(aos, n, x, y, z));
+  return RVP_OK;
+}
+
+extern "C" int rvp_aos_to_soa_bounded(const float *aos, size_t n_bound, const uint32_t *n_dev, float *x, float *y,
+                                      float *z, rvp_stream_t st) {
+  if (n_bound == 0) return RVP_OK;
+  RVP_CHECK(aos && n_dev && x && y && z, RVP_E_ARG, "null pointer");
+  // a quarter million threads: one pass for a frame's kept points, strided beyond
+  const unsigned nb = (unsigned)std::min<size_t>(1024, ceil_div(n_bound, 256));
+  RVP_LAUNCH("aos_to_soa_bounded", S(st), aos_to_soa_bounded_k<<<nb, 256, 0, S(st)>>>(aos, n_bound, n_dev, x, y, z));
   return RVP_OK;
 }

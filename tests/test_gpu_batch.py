@@ -190,6 +190,24 @@ def test_batch_ransac_matches_single_and_oracle(clouds):
             assert ob == inl[c] and (om == models[c]).all() and oit == its[c], c
 
 
+@pytest.mark.parametrize("nb,count", [(5000, 0), (5000, 1), (5000, 777), (5000, 5000), (5000, 6000),
+                                      (300_000, 290_000)])
+def test_aos_to_soa_bounded_reads_the_device_count(nb, count):
+    """Converts min(*n_dev, n_bound) points (grid-stride beyond 262,144) and leaves the rest untouched."""
+    from rvpoint_amd._lib import call
+    rng = np.random.default_rng(count)
+    aos = rng.random((nb, 3)).astype(np.float32)
+    D = R.DeviceArray
+    a = D.from_numpy(aos)
+    cnt = D.from_numpy(np.array([count], np.uint32))
+    outs = [D.from_numpy(np.full(nb, -1.0, np.float32)) for _ in range(3)]
+    call("rvp_aos_to_soa_bounded", a.ptr, nb, cnt.ptr, *(o.ptr for o in outs), None)
+    m = min(count, nb)
+    for j, o in enumerate(outs):
+        got = o.numpy()
+        assert (got[:m] == aos[:m, j]).all() and (got[m:] == -1.0).all(), j
+
+
 def test_batch_chained_stages_without_host_sync():
     """voxel -> SOR -> radius normals -> RANSAC on device offsets equals the per-cloud chain."""
     import ctypes as C
