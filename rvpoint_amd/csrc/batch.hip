@@ -564,9 +564,8 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
 // S1 per cloud: k+1 nearest within the query's cloud, mean of sqrt of entries 1..k (sequential)
 // Waves take runs of consecutive queries (cell order, run length from the device-side count; `forced` > 0
 // fixes it); within a cloud, the previous query's k-th distance bounds the next one's (knn_bound_from).
-// Seven waves per SIMD (72 VGPRs, a few spilled): the walk is latency-bound, and 7 waves beat 6 (80 VGPRs, frame
-// SOR 0.217 -> 0.211 ms), 5 (unconstrained, 0.89 ms per frame against 0.86) and 8 (64 VGPRs, 0.224 ms).
-__global__ __launch_bounds__(QT) __attribute__((amdgpu_waves_per_eu(7))) void seg_sor_means_k(SegArrays A, int k, int forced,
+// Six waves per SIMD: the register budget is 80 VGPRs.
+__global__ __launch_bounds__(QT) __attribute__((amdgpu_waves_per_eu(6))) void seg_sor_means_k(SegArrays A, int k, int forced,
                                                                                            float *__restrict__ means) {
   __shared__ WaveLds lds[QW];
   const int w = threadIdx.x >> 6;
