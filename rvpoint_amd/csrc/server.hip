@@ -89,6 +89,9 @@ static int slot_alloc(Slot &S, int dev, size_t cap, int max_frames) {
 }
 
 static void slot_free(Slot &S) {
+  // the slot's stream is drained before any of its buffers is released (a batch that ended in an error can
+  // leave work queued on it)
+  if (S.s) (void)hipStreamSynchronize(S.s);
   for (Tmp *t : {&S.aos, &S.x, &S.y, &S.z, &S.vox, &S.vx, &S.vy, &S.vz, &S.sor, &S.sx, &S.sy, &S.sz, &S.nx, &S.ny,
                  &S.nz, &S.off, &S.voff, &S.soff, &S.status, &S.leaf, &S.radius})
     t->release();
@@ -96,12 +99,7 @@ static void slot_free(Slot &S) {
   if (S.h_pts) (void)hipHostFree(S.h_pts);
   if (S.h_nrm) (void)hipHostFree(S.h_nrm);
   if (S.h_off) (void)hipHostFree(S.h_off);
-  if (S.s) {
-    (void)hipStreamSynchronize(S.s);
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    (void)rvp_stream_destroy(S.s);
-  }
+  if (S.s) (void)rvp_stream_destroy(S.s);
 }
 
 // one batch through the pipeline; fills every request's results
