@@ -564,9 +564,9 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
 // S1 per cloud: k+1 nearest within the query's cloud, mean of sqrt of entries 1..k (sequential)
 // Waves take runs of consecutive queries (cell order, run length from the device-side count; `forced` > 0
 // fixes it); within a cloud, the previous query's k-th distance bounds the next one's (knn_bound_from).
-// Six waves per SIMD: the register budget is 80 VGPRs.
-__global__ __launch_bounds__(QT) __attribute__((amdgpu_waves_per_eu(6))) void seg_sor_means_k(SegArrays A, int k, int forced,
-                                                                                           float *__restrict__ means) {
+// Only that k-th d2 is carried between queries (the previous position is re-read), which keeps the kernel at
+// 79 VGPRs: six waves per SIMD and no scratch.
+__global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, int forced, float *__restrict__ means) {
   __shared__ WaveLds lds[QW];
   const int w = threadIdx.x >> 6;
   const uint32_t total = A.off[A.ncl];
@@ -574,26 +574,25 @@ __global__ __launch_bounds__(QT) __attribute__((amdgpu_waves_per_eu(6))) void se
   const uint32_t nruns = (total + pw - 1) / pw;
   for (uint32_t run = wave_id(); run < nruns; run += wave_count()) {
     const uint32_t q1 = min(run * pw + pw, total);
-    int pc = -1;  // cloud of the previous query whose k-th d2 is pd2
-    float px = 0.f, py = 0.f, pz = 0.f, pd2 = -1.0f;
+    float pd2 = -1.0f;  // k-th d2 of query q - 1 when it is in the run (< 0: none)
     for (uint32_t q = run * pw; q < q1; ++q) {
       const int c = cloud_of(A.off, A.ncl, q);
       const GridView g = seg_view(A, c);
       const float x = uniform(g.xs[q]), y = uniform(g.ys[q]), z = uniform(g.zs[q]);
       TopKT<uint32_t> t;
       t.init(k + 1);
+      // the previous query of the run is q - 1, re-read from the cache; it bounds q only within one cloud
       knn_search<false>(g, x, y, z, t, lds[w],
-                        c == pc ? knn_bound_from(x, y, z, px, py, pz, pd2) : __builtin_inff());
+                        (pd2 >= 0.0f && q > A.off[c])
+                            ? knn_bound_from(x, y, z, uniform(g.xs[q - 1]), uniform(g.ys[q - 1]), uniform(g.zs[q - 1]),
+                                             pd2)
+                            : __builtin_inff());
       const int l = lane();
       const int m = min(t.cnt, k + 1);
       const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
       float s = 0.0f;
       for (int j = 1; j <= k; ++j) s = s + readlanef(sq, j);
       if (l == 0) means[g.perm[q]] = s / __int2float_rn(k);
-      pc = c;
-      px = x;
-      py = y;
-      pz = z;
       pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
     }
   }
