@@ -186,7 +186,9 @@ int rvp_normals_knn_centroids(const float *x, const float *y, const float *z, si
  * Queries are the hash's own points. */
 int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, float vpy, float vpz,
                        float *nx, float *ny, float *nz, rvp_stream_t s);
-/* number of points of the last normals call that took the FP64 eigen fallback (synchronises the device) */
+/* number of points that took the FP64 eigen fallback in the last normals call made by the calling host thread
+ * (synchronises that call's stream; every stream has its own counter, so calls on other streams or threads do
+ * not disturb it) */
 int rvp_normals_fallback_count(size_t *count);
 
 /* ---- V1: voxel_grid_downsamp_sc (src/filters/voxel_grid_downsamp.cpp:183-213) ---------------------

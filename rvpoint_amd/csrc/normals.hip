@@ -6,8 +6,8 @@
 // a SpatialHash (:189-214), flipNormalTowardsViewpoint (:55-67).  The reference's own eigen solver
 // (simple_eigen3x3_smallest, :16-52: four non-converged Jacobi steps with a sign error) is deliberately not
 // reproduced; results are graded against an FP64 eigen solve of the same neighbourhood.
-#include <atomic>
-
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include "batch.hpp"
@@ -16,18 +16,37 @@
 
 namespace rvp {
 
-// FP64-fallback counter of the last normals call, per device (device memory, read on request)
-static unsigned long long *g_fallback_dev[64] = {nullptr};
-static int g_fallback_last_dev = -1;
+// FP64-fallback counters: one device word per (device, stream), allocated once under a lock and kept until the
+// process ends (a stream handle the driver reuses gets the same word back).  A normals call clears its stream's
+// word with a kernel in stream order (no memset node, so the call can be captured into a graph) and its solve
+// kernels add to it; calls on other streams never touch it.  Each host thread remembers the stream of its last
+// normals call, which rvp_normals_fallback_count reads.
+static std::mutex g_fallback_mu;
+static std::map<std::pair<int, hipStream_t>, unsigned long long *> g_fallback;
+struct LastNormalsCall {
+  int dev = -1;
+  hipStream_t s = nullptr;
+  unsigned long long *counter = nullptr;
+};
+static thread_local LastNormalsCall g_last_normals;
+
+__global__ void zero_u64_k(unsigned long long *p) {
+  if (threadIdx.x == 0) *p = 0ull;
+}
 
 static int fallback_counter(hipStream_t s, unsigned long long **out) {
   int dev = 0;
   RVP_HIP(hipGetDevice(&dev));
-  RVP_CHECK(dev >= 0 && dev < 64, RVP_E_UNSUPPORTED, "device index");
-  if (!g_fallback_dev[dev]) RVP_HIP(hipMalloc(&g_fallback_dev[dev], sizeof(unsigned long long)));
-  RVP_HIP(hipMemsetAsync(g_fallback_dev[dev], 0, sizeof(unsigned long long), s));
-  g_fallback_last_dev = dev;
-  *out = g_fallback_dev[dev];
+  unsigned long long *p = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_fallback_mu);
+    unsigned long long *&slot = g_fallback[{dev, s}];
+    if (!slot) RVP_HIP(hipMalloc(&slot, sizeof(unsigned long long)));
+    p = slot;
+  }
+  RVP_LAUNCH("zero_u64_k", s, zero_u64_k<<<1, WAVE, 0, s>>>(p));
+  g_last_normals = LastNormalsCall{dev, s, p};
+  *out = p;
   return RVP_OK;
 }
 
@@ -45,43 +64,55 @@ struct NormLds {
 // cyclic Jacobi in double precision (symmetric 3x3), eigenvector of the smallest eigenvalue
 __device__ void jacobi_smallest_f64(double a00, double a01, double a02, double a11, double a12, double a22,
                                     double v[3]) {
+  // every index below is a compile-time constant (unrolled loops, selects for the result column), so the
+  // arrays live in registers and the kernels need no scratch memory
   double A[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
   double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   for (int sweep = 0; sweep < 32; ++sweep) {
     const double off = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
     const double dia = fabs(A[0][0]) + fabs(A[1][1]) + fabs(A[2][2]);
     if (off == 0.0 || off < 1e-18 * dia) break;
+#pragma unroll
     for (int p = 0; p < 2; ++p)
+#pragma unroll
       for (int q = p + 1; q < 3; ++q) {
         const double apq = A[p][q];
-        if (apq == 0.0) continue;
-        const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < 3; ++k) {
-          const double akp = A[k][p], akq = A[k][q];
-          A[k][p] = c * akp - s * akq;
-          A[k][q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < 3; ++k) {
-          const double apk = A[p][k], aqk = A[q][k];
-          A[p][k] = c * apk - s * aqk;
-          A[q][k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < 3; ++k) {
-          const double vkp = V[k][p], vkq = V[k][q];
-          V[k][p] = c * vkp - s * vkq;
-          V[k][q] = s * vkp + c * vkq;
+        if (apq != 0.0) {
+          const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+          const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const double akp = A[k][p], akq = A[k][q];
+            A[k][p] = c * akp - s * akq;
+            A[k][q] = s * akp + c * akq;
+          }
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const double apk = A[p][k], aqk = A[q][k];
+            A[p][k] = c * apk - s * aqk;
+            A[q][k] = s * apk + c * aqk;
+          }
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const double vkp = V[k][p], vkq = V[k][q];
+            V[k][p] = c * vkp - s * vkq;
+            V[k][q] = s * vkp + c * vkq;
+          }
         }
       }
   }
-  int mi = 0;
-  if (A[1][1] < A[mi][mi]) mi = 1;
-  if (A[2][2] < A[mi][mi]) mi = 2;
-  const double nn = sqrt(V[0][mi] * V[0][mi] + V[1][mi] * V[1][mi] + V[2][mi] * V[2][mi]);
-  v[0] = V[0][mi] / nn;
-  v[1] = V[1][mi] / nn;
-  v[2] = V[2][mi] / nn;
+  // the column of the smallest diagonal entry (the first one on a tie)
+  const bool c1 = A[1][1] < A[0][0];
+  const double dmin = c1 ? A[1][1] : A[0][0];
+  const bool c2 = A[2][2] < dmin;
+  double u[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) u[k] = c2 ? V[k][2] : (c1 ? V[k][1] : V[k][0]);
+  const double nn = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  v[0] = u[0] / nn;
+  v[1] = u[1] / nn;
+  v[2] = u[2] / nn;
 }
 
 // Smallest-eigenvalue eigenvector of a symmetric 3x3 (Smith's trigonometric eigenvalues, eigenvector from
@@ -118,17 +149,18 @@ __device__ bool smallest_eigvec(const float c[6], float n[3]) {
       v[0][0] = r0[1] * r1[2] - r0[2] * r1[1]; v[0][1] = r0[2] * r1[0] - r0[0] * r1[2]; v[0][2] = r0[0] * r1[1] - r0[1] * r1[0];
       v[1][0] = r0[1] * r2[2] - r0[2] * r2[1]; v[1][1] = r0[2] * r2[0] - r0[0] * r2[2]; v[1][2] = r0[0] * r2[1] - r0[1] * r2[0];
       v[2][0] = r1[1] * r2[2] - r1[2] * r2[1]; v[2][1] = r1[2] * r2[0] - r1[0] * r2[2]; v[2][2] = r1[0] * r2[1] - r1[1] * r2[0];
-      float best = -1.0f;
-      int bi = 0;
+      // the longest cross product (the first one on a tie), chosen by selects so that v stays in registers
+      float best = -1.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f;
+#pragma unroll
       for (int i = 0; i < 3; ++i) {
         const float nn = v[i][0] * v[i][0] + v[i][1] * v[i][1] + v[i][2] * v[i][2];
-        if (nn > best) { best = nn; bi = i; }
+        if (nn > best) { best = nn; b0 = v[i][0]; b1 = v[i][1]; b2 = v[i][2]; }
       }
       if (!(best > 0.0f)) {
         fallback = true;
       } else {
         const float inv = 1.0f / sqrtf(best);
-        n[0] = v[bi][0] * inv; n[1] = v[bi][1] * inv; n[2] = v[bi][2] * inv;
+        n[0] = b0 * inv; n[1] = b1 * inv; n[2] = b2 * inv;
       }
     }
   }
@@ -523,11 +555,17 @@ int rvp_batch_normals_radius(const float *x, const float *y, const float *z, con
 int rvp_normals_fallback_count(size_t *count) {
   RVP_CHECK(count, RVP_E_ARG, "null argument");
   *count = 0;
-  if (g_fallback_last_dev < 0) return RVP_OK;
-  RVP_HIP(hipSetDevice(g_fallback_last_dev));
-  RVP_HIP(hipDeviceSynchronize());
+  const LastNormalsCall last = g_last_normals;
+  if (!last.counter) return RVP_OK;
+  // read on the call's own stream and device; the caller's current device is restored
+  int cur = 0;
+  RVP_HIP(hipGetDevice(&cur));
+  if (cur != last.dev) RVP_HIP(hipSetDevice(last.dev));
   unsigned long long h = 0;
-  RVP_HIP(hipMemcpy(&h, g_fallback_dev[g_fallback_last_dev], sizeof(h), hipMemcpyDeviceToHost));
+  hipError_t e = hipMemcpyAsync(&h, last.counter, sizeof(h), hipMemcpyDeviceToHost, last.s);
+  if (e == hipSuccess) e = hipStreamSynchronize(last.s);
+  if (cur != last.dev) (void)hipSetDevice(cur);
+  if (e != hipSuccess) return fail(RVP_E_DEVICE, std::string("reading the fallback counter: ") + hipGetErrorString(e));
   *count = (size_t)h;
   return RVP_OK;
 }

@@ -393,32 +393,56 @@ __global__ __launch_bounds__(PRIM_THREADS) void run_heads_k(const K *__restrict_
   }
 }
 
-// per-(device, stream) look-back state: status words, ticket counter, epoch
+// Per-(device, stream) look-back state: status words, ticket counter, epoch.  Launches on one stream run in
+// stream order whichever host thread enqueued them, and each accepts only status words of its own epoch, so
+// several threads may drive one stream.  A status array that has been handed out is never freed while the
+// process runs: a larger one replaces it and the old one is kept (retired), because a launch enqueued by
+// another thread may still read it.
 struct ScanState {
   uint64_t *status = nullptr;
   uint32_t *ticket = nullptr;
   size_t cap = 0;
   uint32_t epoch = 0;
+  std::vector<uint64_t *> retired;
 };
 static std::mutex g_scan_mu;
 static std::map<std::pair<int, hipStream_t>, ScanState> g_scan;
+
+// Clears fresh look-back state in the order of the stream that will use it.  A plain hipMemset is enqueued on
+// the null stream, which does not order against non-blocking streams: the first look-back launch on such a
+// stream could then read the allocation's previous contents (a nonzero ticket skips a tile; a stale status
+// word of a matching epoch hands out a wrong prefix), and the sort scatters or gathers built on those prefixes
+// would leave their arrays.
+__global__ void state_clear_k(uint64_t *__restrict__ status, size_t words, uint32_t *__restrict__ ticket) {
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ticket && i0 == 0) *ticket = 0u;
+  for (size_t i = i0; i < words; i += (size_t)gridDim.x * blockDim.x) status[i] = 0ull;
+}
 
 static int scan_state(hipStream_t s, size_t tiles, uint64_t **status, uint32_t **ticket, uint32_t *epoch) {
   int dev = 0;
   RVP_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_scan_mu);
   ScanState &st = g_scan[{dev, s}];
+  uint32_t *clear_ticket = nullptr;
   if (!st.ticket) {
     RVP_HIP(hipMalloc(&st.ticket, sizeof(uint32_t)));
-    RVP_HIP(hipMemset(st.ticket, 0, sizeof(uint32_t)));
+    clear_ticket = st.ticket;
   }
+  uint64_t *fresh = nullptr;
+  size_t cap = st.cap;
   if (st.cap < tiles) {
-    // the stream may still run a scan on the old array: wait before replacing it
-    RVP_HIP(hipStreamSynchronize(s));
-    if (st.status) RVP_HIP(hipFree(st.status));
-    const size_t cap = std::max<size_t>(tiles, 4096);
-    RVP_HIP(hipMalloc(&st.status, cap * sizeof(uint64_t)));
-    RVP_HIP(hipMemset(st.status, 0, cap * sizeof(uint64_t)));
+    cap = std::max<size_t>(std::max<size_t>(tiles, 4096), 2 * st.cap);
+    RVP_HIP(hipMalloc(&fresh, cap * sizeof(uint64_t)));
+  }
+  if (fresh || clear_ticket) {
+    const size_t words = fresh ? cap : 0;
+    const unsigned nb = (unsigned)std::min<size_t>(1024, std::max<size_t>(1, ceil_div(words, PRIM_THREADS)));
+    RVP_LAUNCH("state_clear_k", s, state_clear_k<<<nb, PRIM_THREADS, 0, s>>>(fresh, words, clear_ticket));
+  }
+  if (fresh) {
+    if (st.status) st.retired.push_back(st.status);
+    st.status = fresh;
     st.cap = cap;
   }
   st.epoch = (st.epoch + 1) & 0x3fffffffu;

@@ -550,7 +550,10 @@ static size_t g_draws_len[64] = {0};
 
 static std::mutex g_draws_mu;
 
-static int rand_draws(size_t count, const int32_t **out) {
+// The table is copied on the caller's stream and waited for before it is published, so every launch that
+// receives the pointer (on any stream) reads it complete; a plain hipMemcpy from pageable memory may return
+// before the copy has landed, and the null stream it uses does not order against non-blocking streams.
+static int rand_draws(size_t count, hipStream_t s, const int32_t **out) {
   int dev = 0;
   RVP_HIP(hipGetDevice(&dev));
   RVP_CHECK(dev >= 0 && dev < 64, RVP_E_UNSUPPORTED, "device index");
@@ -562,7 +565,8 @@ static int rand_draws(size_t count, const int32_t **out) {
     for (auto &v : h) v = rng.next();
     int32_t *d = nullptr;
     RVP_HIP(hipMalloc(&d, len * sizeof(int32_t)));
-    RVP_HIP(hipMemcpy(d, h.data(), len * sizeof(int32_t), hipMemcpyHostToDevice));
+    RVP_HIP(hipMemcpyAsync(d, h.data(), len * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    RVP_HIP(hipStreamSynchronize(s));
     // a shorter previous table is kept: launches on other streams may still read it
     g_draws_dev[dev] = d;
     g_draws_len[dev] = len;
@@ -781,7 +785,7 @@ int rvp_batch_ransac_plane(const float *x, const float *y, const float *z, const
   RVP_TRY(seg_clamp_offsets(offsets, nclouds, n_bound, G.offs.as<uint32_t>(), s));
   const uint32_t *off = G.offs.as<uint32_t>();
   const int32_t *draws = nullptr;
-  RVP_TRY(rand_draws((size_t)3 * H, &draws));
+  RVP_TRY(rand_draws((size_t)3 * H, s, &draws));
   // models [NH][4], valid [NH], counts [NH], offsets [nclouds + 1]: one buffer, read back in one copy
   const size_t words = NH * 6 + (size_t)nclouds + 1;
   Tmp pack, tiles;
