@@ -226,54 +226,6 @@ def batch_leg(R, args):
     }
 
 
-class DeviceFramePipeline:
-    """The same frame as a batch of one: every stage reads its point count from the previous stage's device
-    offsets, so nothing waits on the host until RANSAC's adaptive stop is replayed at the end."""
-
-    def __init__(self, frame_xyz, stream=None):
-        import rvpoint_amd as R
-        from rvpoint_amd._lib import call
-        self.call, self.s = call, stream
-        self.n = n = len(frame_xyz)
-        D = R.DeviceArray
-        self.frame = R.DeviceCloud(frame_xyz)
-        self.off = D.from_numpy(np.array([0, n], np.uint32))
-        self.leaf = D.from_numpy(np.array([0.1], np.float32))
-        self.radius = D.from_numpy(np.array([0.5], np.float32))
-        self.vox, self.vox_off, self.status = D((n, 3)), D(2, np.uint32), D(1, np.uint32)
-        self.vx, self.vy, self.vz = D(n), D(n), D(n)
-        self.sor, self.sor_off = D((n, 3)), D(2, np.uint32)
-        self.sx, self.sy, self.sz = D(n), D(n), D(n)
-        self.nx, self.ny, self.nz = D(n), D(n), D(n)
-        self.k, self.alpha, self.max_nn, self.vp = 50, 1.0, 64, (0.0, 0.0, 1.7)
-        self.thr, self.iters = 0.05, 1000
-        self.model = np.zeros(4, np.float32)
-        self.inl = np.zeros(1, np.int32)
-        self.its = np.zeros(1, np.int32)
-
-    def step(self):
-        call, s, n = self.call, self.s, self.n
-        call("rvp_batch_voxel", *self.frame.ptrs(), self.off.ptr, 1, n, self.leaf.ptr, self.vox.ptr,
-             self.vox_off.ptr, self.status.ptr, s)
-        # what the stage kept, counted on the device (the second entry of its output offsets)
-        call("rvp_aos_to_soa_bounded", self.vox.ptr, n, self.vox_off.ptr + 4, self.vx.ptr, self.vy.ptr, self.vz.ptr, s)
-        call("rvp_batch_sor_exact", self.vx.ptr, self.vy.ptr, self.vz.ptr, self.vox_off.ptr, 1, n, self.k,
-             self.alpha, self.sor.ptr, self.sor_off.ptr, None, None, s)
-        call("rvp_aos_to_soa_bounded", self.sor.ptr, n, self.sor_off.ptr + 4, self.sx.ptr, self.sy.ptr, self.sz.ptr, s)
-        kept = (self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, 1, n)
-        call("rvp_batch_normals_radius", *kept, self.radius.ptr, None, self.max_nn, *self.vp, self.nx.ptr,
-             self.ny.ptr, self.nz.ptr, s)
-        P = C.POINTER
-        call("rvp_batch_ransac_plane", *kept, self.thr, self.iters, 1e-6, 0.99,
-             self.model.ctypes.data_as(P(C.c_float)), self.inl.ctypes.data_as(P(C.c_int)),
-             self.its.ctypes.data_as(P(C.c_int)), s)
-
-    def info(self):
-        return {"n_in": self.n, "n_voxel": int(self.vox_off.numpy()[1]), "n_sor": int(self.sor_off.numpy()[1]),
-                "inliers": int(self.inl[0]), "ransac_iters": int(self.its[0]),
-                "model": [float(v) for v in self.model]}
-
-
 def time_steps(R, fn, steps):
     times = []
     t_all = time.perf_counter()
@@ -531,6 +483,7 @@ def main():
     info_plain = plain.step()
     R.synchronize()
     # the measured path: the frame as a batch of one, stages chained on device offsets
+    from rvpoint_amd.pipeline import DeviceFramePipeline
     pipe = DeviceFramePipeline(frame)
     for _ in range(args.warmup):
         pipe.step()

@@ -46,6 +46,10 @@ def lib():
         if not os.path.exists(LIB_PATH):
             build(with_ref=False)
         L = C.CDLL(LIB_PATH)
+        L.orc_dist_sq.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, C.c_float, C.c_float, _f32p]
+        L.orc_inds_in_radius.argtypes = [_f32p, _f32p, _f32p, _i32p, _sz, C.c_float, C.c_float, C.c_float, C.c_float,
+                                         _i32p, _f32p, _sz]
+        L.orc_inds_in_radius.restype = _sz
         L.orc_bbox.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, _f32p]
         L.orc_cell_coords.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, _i32p, C.c_void_p]
         L.orc_hash_radius.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, _f32p, _f32p, _f32p, _sz,
@@ -57,6 +61,10 @@ def lib():
         L.orc_knn.argtypes = [_f32p, _f32p, _f32p, _sz, _f32p, _f32p, _f32p, _sz, C.c_int, _i32p, _f32p]
         L.orc_sor_sc.argtypes = [_f32p, _sz, C.c_int, C.c_float, _f32p, _f32p]
         L.orc_sor_sc.restype = _sz
+        L.orc_sor_sc_mt.argtypes = [_f32p, _sz, C.c_int, C.c_float, _f32p, _f32p, C.c_int]
+        L.orc_sor_sc_mt.restype = _sz
+        L.orc_normals_hash_e1_mt.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, C.c_float, _f32p, C.c_int,
+                                             _f32p, _f32p, _f32p, C.c_int]
         L.orc_sor_radius.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, C.c_int, C.c_float, C.c_float,
                                      _f32p, _f32p]
         L.orc_sor_radius.restype = _sz
@@ -101,6 +109,9 @@ def ref():
             raise FileNotFoundError(REF_PATH)
         R = C.CDLL(REF_PATH)
         R.ref_get_dist_sq.argtypes = [_f32p, _f32p, _f32p, C.c_float, C.c_float, C.c_float, _f32p, _sz]
+        R.ref_inds_in_radius.argtypes = [_f32p, _f32p, _f32p, _i32p, _sz, C.c_float, C.c_float, C.c_float,
+                                         C.c_float, _i32p, _f32p, _sz]
+        R.ref_inds_in_radius.restype = _sz
         R.ref_hash_radius.argtypes = [_f32p, _f32p, _f32p, _sz, C.c_float, _f32p, _f32p, _f32p, _sz, C.c_float,
                                       C.c_int, _sz, C.c_void_p, C.c_void_p, _i32p]
         R.ref_radius_search_sc.argtypes = [_f32p, _sz, C.c_float, C.c_float, C.c_float, C.c_float, _i32p, _f32p,
@@ -156,6 +167,24 @@ def _c(a, dt=np.float32):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def dist_sq(x, y, z, q):
+    out = np.zeros(len(x), np.float32)
+    lib().orc_dist_sq(_c(x), _c(y), _c(z), len(x), q[0], q[1], q[2], out)
+    return out
+
+
+def inds_in_radius(x, y, z, subset, q, r2, prefix_idx=(), prefix_d2=()):
+    """D2 appending after the given prefix; returns (indices, d2) including the prefix."""
+    subset = _c(subset, np.int32)
+    have = len(prefix_idx)
+    oi = np.zeros(have + len(subset), np.int32)
+    od = np.zeros(have + len(subset), np.float32)
+    oi[:have] = prefix_idx
+    od[:have] = prefix_d2
+    m = lib().orc_inds_in_radius(_c(x), _c(y), _c(z), subset, len(subset), q[0], q[1], q[2], r2, oi, od, have)
+    return oi[:m], od[:m]
 
 
 def bbox(x, y, z, cell):
@@ -216,6 +245,32 @@ def sor_sc(aos, k, alpha):
     means = np.zeros(max(n, 1), np.float32)
     m = lib().orc_sor_sc(aos.reshape(-1), n, k, alpha, out.reshape(-1), means)
     return out[:m], means[:n]
+
+
+def threads() -> int:
+    """Host threads the multi-threaded oracle functions use: the CPUs this process may run on."""
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except (AttributeError, OSError):
+        return max(1, os.cpu_count() or 1)
+
+
+def sor_sc_mt(aos, k, alpha, nthreads=None):
+    """sor_sc with the per-point means on several threads; bit-identical to sor_sc."""
+    aos = _c(aos).reshape(-1, 3)
+    n = len(aos)
+    out = np.zeros((max(n, 1), 3), np.float32)
+    means = np.zeros(max(n, 1), np.float32)
+    m = lib().orc_sor_sc_mt(aos.reshape(-1), n, k, alpha, out.reshape(-1), means, nthreads or threads())
+    return out[:m], means[:n]
+
+
+def normals_hash_e1(x, y, z, cell, radius, vp=(0.0, 0.0, 0.0), eigen_iters=4, nthreads=1):
+    """normal_estimation_rvv with a SpatialHash, reference semantics (uncapped radius list, E1, flip)."""
+    n = len(x)
+    nx, ny, nz = (np.zeros(n, np.float32) for _ in range(3))
+    lib().orc_normals_hash_e1_mt(_c(x), _c(y), _c(z), n, cell, radius, _c(vp), eigen_iters, nx, ny, nz, nthreads)
+    return nx, ny, nz
 
 
 def sor_radius(x, y, z, cell, k, alpha, search_radius):

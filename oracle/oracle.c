@@ -72,6 +72,24 @@ static void select_sorted_u64(uint64_t *v, size_t n, size_t k) {
 /* ------------------------------------------------------------------ */
 
 /* spatial_hashing.cpp:76-146 (scalar branch :131-138, margin :141-146). */
+/* get_dist_sq_rvv, scalar branch (rvv_common.cpp:36-43): out[i] = D1(p_i, q). */
+void orc_dist_sq(const float *x, const float *y, const float *z, size_t n, float qx, float qy, float qz, float *out) {
+    for (size_t i = 0; i < n; ++i) out[i] = d1(x[i], y[i], z[i], qx, qy, qz);
+}
+
+/* get_inds_in_radius_rvv, scalar branch (rvv_common.cpp:55-68): gather the subset, D1, and append
+ * (subset[i], d2) for d2 <= r2 in subset order after the `have` entries already in the outputs; returns the new
+ * length. */
+size_t orc_inds_in_radius(const float *x, const float *y, const float *z, const int32_t *subset, size_t m,
+                          float qx, float qy, float qz, float r2, int32_t *out_idx, float *out_d2, size_t have) {
+    for (size_t i = 0; i < m; ++i) {
+        const int32_t j = subset[i];
+        const float d = d1(x[j], y[j], z[j], qx, qy, qz);
+        if (d <= r2) { out_idx[have] = j; out_d2[have] = d; ++have; }
+    }
+    return have;
+}
+
 void orc_bbox(const float *x, const float *y, const float *z, size_t n, float cell, float out[6]) {
     float mnx = FLT_MAX, mny = FLT_MAX, mnz = FLT_MAX;
     float mxx = -FLT_MAX, mxy = -FLT_MAX, mxz = -FLT_MAX;
@@ -340,6 +358,37 @@ size_t orc_sor_sc(const float *aos, size_t n, int k, float alpha, float *out, fl
     return c;
 }
 
+/* sor_sc with the per-point means computed by `threads` OpenMP threads (each point's k nearest and its mean
+ * are the same arithmetic as orc_sor_sc; the global mean and deviation sums stay sequential in index order,
+ * statistical_outlier_removal.cpp:43-52), so the outputs are bit-identical to orc_sor_sc.  The loop over
+ * independent points with thread-private buffers follows the reference's own OpenMP pattern
+ * (tests/unit/test_openmp_rvv_scaling.cpp:61-104). */
+size_t orc_sor_sc_mt(const float *aos, size_t n, int k, float alpha, float *out, float *means, int threads) {
+    if (n == 0) return 0;
+    float *md = (float *)malloc(n * sizeof(float));
+    float *x = (float *)malloc(n * 4), *y = (float *)malloc(n * 4), *z = (float *)malloc(n * 4);
+    for (size_t i = 0; i < n; ++i) { x[i] = aos[3 * i]; y[i] = aos[3 * i + 1]; z[i] = aos[3 * i + 2]; }
+    const size_t take = (size_t)k + 1 < n ? (size_t)k + 1 : n;
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+    {
+        uint64_t *buf = (uint64_t *)malloc(n * 8);
+#pragma omp for schedule(dynamic, 64)
+        for (long long ii = 0; ii < (long long)n; ++ii) {
+            const size_t i = (size_t)ii;
+            for (size_t j = 0; j < n; ++j) buf[j] = pack(d1(x[i], y[i], z[i], x[j], y[j], z[j]), (int32_t)j);
+            select_sorted_u64(buf, n, take);
+            float sum = 0.0f;
+            for (int j = 1; j <= k && (size_t)j < take; ++j) sum += sqrtf(bitsf((uint32_t)(buf[j] >> 32)));
+            md[i] = sum / (float)k;
+        }
+        free(buf);
+    }
+    size_t c = orc_sor_filter(x, y, z, n, md, alpha, out, NULL, NULL);
+    if (means) memcpy(means, md, n * sizeof(float));
+    free(md); free(x); free(y); free(z);
+    return c;
+}
+
 /* sor_spatial_hash (statistical_outlier_removal.cpp:246-270). */
 size_t orc_sor_radius(const float *x, const float *y, const float *z, size_t n, float cell,
                       int k, float alpha, float search_radius, float *out, float *means) {
@@ -483,22 +532,28 @@ static void flip_f64(double px, double py, double pz, const float vp[3], double 
 void orc_normals_knn_f64(const float *x, const float *y, const float *z, size_t n, int k,
                          const float vp[3], double *nx, double *ny, double *nz) {
     if (n == 0) return;
-    size_t K = (size_t)k + 1 < n ? (size_t)k + 1 : n;
-    uint64_t *buf = (uint64_t *)malloc(n * 8);
-    double *px = (double *)malloc(K * 8), *py = (double *)malloc(K * 8), *pz = (double *)malloc(K * 8);
-    for (size_t i = 0; i < n; ++i) {
-        for (size_t j = 0; j < n; ++j) buf[j] = pack(d1(x[i], y[i], z[i], x[j], y[j], z[j]), (int32_t)j);
-        select_sorted_u64(buf, n, K);
-        for (size_t j = 0; j < K; ++j) {
-            int32_t id = (int32_t)(buf[j] & 0xffffffffu);
-            px[j] = x[id]; py[j] = y[id]; pz[j] = z[id];
+    const size_t K = (size_t)k + 1 < n ? (size_t)k + 1 : n;
+    /* points are independent: OpenMP over them with thread-private buffers (results do not depend on it) */
+#pragma omp parallel
+    {
+        uint64_t *buf = (uint64_t *)malloc(n * 8);
+        double *px = (double *)malloc(K * 8), *py = (double *)malloc(K * 8), *pz = (double *)malloc(K * 8);
+#pragma omp for schedule(dynamic, 64)
+        for (long long ii = 0; ii < (long long)n; ++ii) {
+            const size_t i = (size_t)ii;
+            for (size_t j = 0; j < n; ++j) buf[j] = pack(d1(x[i], y[i], z[i], x[j], y[j], z[j]), (int32_t)j);
+            select_sorted_u64(buf, n, K);
+            for (size_t j = 0; j < K; ++j) {
+                int32_t id = (int32_t)(buf[j] & 0xffffffffu);
+                px[j] = x[id]; py[j] = y[id]; pz[j] = z[id];
+            }
+            double nv[3];
+            orc_normal_from_points_f64(px, py, pz, K, nv, NULL);
+            flip_f64(x[i], y[i], z[i], vp, nv);
+            nx[i] = nv[0]; ny[i] = nv[1]; nz[i] = nv[2];
         }
-        double nv[3];
-        orc_normal_from_points_f64(px, py, pz, K, nv, NULL);
-        flip_f64(x[i], y[i], z[i], vp, nv);
-        nx[i] = nv[0]; ny[i] = nv[1]; nz[i] = nv[2];
+        free(buf); free(px); free(py); free(pz);
     }
-    free(buf); free(px); free(py); free(pz);
 }
 
 void orc_normals_radius_f64(const float *x, const float *y, const float *z, size_t n, float cell,
@@ -507,27 +562,33 @@ void orc_normals_radius_f64(const float *x, const float *y, const float *z, size
     if (n == 0) return;
     orc_grid g;
     grid_build(&g, x, y, z, n, cell);
-    uint64_t *buf = NULL; size_t cap = 0;
-    size_t pcap = 64;
-    double *px = (double *)malloc(pcap * 8), *py = (double *)malloc(pcap * 8), *pz = (double *)malloc(pcap * 8);
-    for (size_t i = 0; i < n; ++i) {
-        size_t m = grid_query(&g, x, y, z, x[i], y[i], z[i], radius, &buf, &cap);
-        if (max_nn > 0 && m > (size_t)max_nn) m = (size_t)max_nn;
-        if (m < 3) { nx[i] = ny[i] = nz[i] = 0.0; continue; }
-        if (m > pcap) {
-            pcap = m;
-            px = (double *)realloc(px, pcap * 8); py = (double *)realloc(py, pcap * 8); pz = (double *)realloc(pz, pcap * 8);
+#pragma omp parallel
+    {
+        uint64_t *buf = NULL; size_t cap = 0;
+        size_t pcap = 64;
+        double *px = (double *)malloc(pcap * 8), *py = (double *)malloc(pcap * 8), *pz = (double *)malloc(pcap * 8);
+#pragma omp for schedule(dynamic, 256)
+        for (long long ii = 0; ii < (long long)n; ++ii) {
+            const size_t i = (size_t)ii;
+            size_t m = grid_query(&g, x, y, z, x[i], y[i], z[i], radius, &buf, &cap);
+            if (max_nn > 0 && m > (size_t)max_nn) m = (size_t)max_nn;
+            if (m < 3) { nx[i] = ny[i] = nz[i] = 0.0; continue; }
+            if (m > pcap) {
+                pcap = m;
+                px = (double *)realloc(px, pcap * 8); py = (double *)realloc(py, pcap * 8); pz = (double *)realloc(pz, pcap * 8);
+            }
+            for (size_t j = 0; j < m; ++j) {
+                int32_t id = (int32_t)(buf[j] & 0xffffffffu);
+                px[j] = x[id]; py[j] = y[id]; pz[j] = z[id];
+            }
+            double nv[3];
+            orc_normal_from_points_f64(px, py, pz, m, nv, NULL);
+            flip_f64(x[i], y[i], z[i], vp, nv);
+            nx[i] = nv[0]; ny[i] = nv[1]; nz[i] = nv[2];
         }
-        for (size_t j = 0; j < m; ++j) {
-            int32_t id = (int32_t)(buf[j] & 0xffffffffu);
-            px[j] = x[id]; py[j] = y[id]; pz[j] = z[id];
-        }
-        double nv[3];
-        orc_normal_from_points_f64(px, py, pz, m, nv, NULL);
-        flip_f64(x[i], y[i], z[i], vp, nv);
-        nx[i] = nv[0]; ny[i] = nv[1]; nz[i] = nv[2];
+        free(buf); free(px); free(py); free(pz);
     }
-    free(buf); free(px); free(py); free(pz); free(g.t);
+    free(g.t);
 }
 
 /* simple_eigen3x3_smallest (normal_estimation.cpp:16-52), op for op. */
@@ -565,6 +626,50 @@ void orc_flip(float px, float py, float pz, float vx_, float vy_, float vz_, flo
     float vx = vx_ - px, vy = vy_ - py, vz = vz_ - pz;
     float dot = vx * n[0] + vy * n[1] + vz * n[2];
     if (dot < 0) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
+}
+
+/* normal_estimation_rvv(in, SpatialHash, ...) (normal_estimation.cpp:189-214) over `threads` OpenMP threads:
+ * per point the hash radius list (no cap), compute_covariance_rvv's sequential centroid and raw covariance
+ * sums in list order (:117-142), the four-step eigen step E1 (:16-52) and the viewpoint flip (:55-67); fewer
+ * than 3 neighbours give a zero normal.  Points are independent, so the result does not depend on threads. */
+void orc_normals_hash_e1_mt(const float *x, const float *y, const float *z, size_t n, float cell, float radius,
+                            const float vp[3], int eigen_iters, float *nx, float *ny, float *nz, int threads) {
+    if (n == 0) return;
+    orc_grid g;
+    grid_build(&g, x, y, z, n, cell);
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+    {
+        uint64_t *buf = NULL;
+        size_t cap = 0;
+#pragma omp for schedule(dynamic, 256)
+        for (long long ii = 0; ii < (long long)n; ++ii) {
+            const size_t i = (size_t)ii;
+            const size_t m = grid_query(&g, x, y, z, x[i], y[i], z[i], radius, &buf, &cap);
+            if (m < 3) { nx[i] = ny[i] = nz[i] = 0.0f; continue; }
+            float cx = 0, cy = 0, cz = 0;
+            for (size_t j = 0; j < m; ++j) {
+                const uint32_t q = (uint32_t)(buf[j] & 0xffffffffu);
+                cx += x[q]; cy += y[q]; cz += z[q];
+            }
+            const float inv_n = 1.0f / (float)m;
+            cx *= inv_n; cy *= inv_n; cz *= inv_n;
+            float c00 = 0, c01 = 0, c02 = 0, c11 = 0, c12 = 0, c22 = 0;
+            for (size_t j = 0; j < m; ++j) {
+                const uint32_t q = (uint32_t)(buf[j] & 0xffffffffu);
+                const float dx = x[q] - cx, dy = y[q] - cy, dz = z[q] - cz;
+                c00 += dx * dx; c01 += dx * dy; c02 += dx * dz;
+                c11 += dy * dy; c12 += dy * dz;
+                c22 += dz * dz;
+            }
+            const float cov[9] = {c00, c01, c02, c01, c11, c12, c02, c12, c22};
+            float nv[3];
+            orc_eigen_e1(cov, eigen_iters, nv);
+            orc_flip(x[i], y[i], z[i], vp[0], vp[1], vp[2], nv);
+            nx[i] = nv[0]; ny[i] = nv[1]; nz[i] = nv[2];
+        }
+        free(buf);
+    }
+    free(g.t);
 }
 
 /* ------------------------------------------------------------------ */

@@ -35,6 +35,10 @@ extern "C" {
 #endif
 
 /* H1: bbox with the 0.01*cell margin (spatial_hashing.cpp:76-146 scalar branch). out = {minx,miny,minz,maxx,maxy,maxz} */
+/* D1 / D2 (rvv_common.cpp:36-43, :55-68); D2 appends after `have` entries and returns the new length */
+void orc_dist_sq(const float *x, const float *y, const float *z, size_t n, float qx, float qy, float qz, float *out);
+size_t orc_inds_in_radius(const float *x, const float *y, const float *z, const int32_t *subset, size_t m,
+                          float qx, float qy, float qz, float r2, int32_t *out_idx, float *out_d2, size_t have);
 void orc_bbox(const float *x, const float *y, const float *z, size_t n, float cell, float out[6]);
 
 /* H2: per-point cell coordinates (spatial_hashing.h:47-52) and the int64 hash key (spatial_hashing.h:43-45). */
@@ -72,6 +76,8 @@ size_t orc_sor_sc(const float *aos, size_t n, int k, float alpha, float *out, fl
 
 /* S2: sor_spatial_hash + filter_by_mean_dists (statistical_outlier_removal.cpp:246-270, 193-218),
  * hash built on the same cloud with cell size `cell`. */
+/* orc_sor_sc with the per-point means on `threads` OpenMP threads (global sums sequential; same bits) */
+size_t orc_sor_sc_mt(const float *aos, size_t n, int k, float alpha, float *out, float *means, int threads);
 size_t orc_sor_radius(const float *x, const float *y, const float *z, size_t n, float cell,
                       int k, float alpha, float search_radius, float *out, float *means);
 
@@ -107,6 +113,9 @@ void orc_normal_from_points_f64(const double *px, const double *py, const double
 void orc_eigen_e1(const float cov[9], int iters, float n_out[3]);
 
 /* F1: flipNormalTowardsViewpoint (normal_estimation.cpp:55-67), fp32. */
+/* normal_estimation_rvv with a SpatialHash (reference semantics: uncapped radius list, E1, flip), threaded */
+void orc_normals_hash_e1_mt(const float *x, const float *y, const float *z, size_t n, float cell, float radius,
+                            const float vp[3], int eigen_iters, float *nx, float *ny, float *nz, int threads);
 void orc_flip(float px, float py, float pz, float vx, float vy, float vz, float n[3]);
 
 /* glibc TYPE_3 random_r restatement (srand/rand). */

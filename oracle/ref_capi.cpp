@@ -32,6 +32,20 @@ void ref_get_dist_sq(const float *x, const float *y, const float *z, float qx, f
   get_dist_sq_rvv(x, y, z, qx, qy, qz, out, n);
 }
 
+// get_inds_in_radius_rvv appending to vectors that already hold `have` entries (copied in first); returns the
+// new length (rvv_common.cpp:49-68)
+size_t ref_inds_in_radius(const float *x, const float *y, const float *z, const int32_t *subset, size_t m, float qx,
+                          float qy, float qz, float r2, int32_t *out_idx, float *out_d2, size_t have) {
+  std::vector<int> idx(out_idx, out_idx + have);
+  std::vector<float> d2(out_d2, out_d2 + have);
+  get_inds_in_radius_rvv(x, y, z, subset, m, qx, qy, qz, r2, idx, d2);
+  for (size_t i = 0; i < idx.size(); ++i) {
+    out_idx[i] = idx[i];
+    out_d2[i] = d2[i];
+  }
+  return idx.size();
+}
+
 // SpatialHash::setInputCloud + build, then radiusSearch per query (spatial_hashing.cpp:76-223)
 void ref_hash_radius(const float *x, const float *y, const float *z, size_t n, float cell,
                      const float *qx, const float *qy, const float *qz, size_t nq, float radius, int max_nn,

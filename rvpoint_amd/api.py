@@ -171,7 +171,7 @@ class SpatialHash:
             cnt = DeviceArray(nq, np.int32)
             call("rvp_radius_search", self.handle, *qp, nq, float(radius), int(max_nn), idx.ptr, d2.ptr, cnt.ptr,
                  None, stream)
-            I, D, N = idx.numpy(), d2.numpy(), cnt.numpy()
+            I, D, N = idx.numpy(stream=stream), d2.numpy(stream=stream), cnt.numpy(stream=stream)
             return [(I[i, :N[i]].copy(), D[i, :N[i]].copy()) for i in range(nq)]
         cnt = DeviceArray(nq, np.int32)
         call("rvp_radius_count", self.handle, *qp, nq, float(radius), cnt.ptr, stream)
@@ -181,8 +181,8 @@ class SpatialHash:
         idx = DeviceArray(max(tot.value, 1), np.int32)
         d2 = DeviceArray(max(tot.value, 1), np.float32)
         call("rvp_radius_fill", self.handle, *qp, nq, float(radius), off.ptr, idx.ptr, d2.ptr, stream)
-        O = off.numpy()
-        I, D = idx.numpy(), d2.numpy()
+        O = off.numpy(stream=stream)
+        I, D = idx.numpy(stream=stream), d2.numpy(stream=stream)
         return [(I[O[i]:O[i + 1]].copy(), D[O[i]:O[i + 1]].copy()) for i in range(nq)]
 
     def radiusSearch(self, query, radius, max_nn=0):
@@ -403,9 +403,9 @@ def batch_sor_exact(batch: Batch, k, alpha, stream=None):
     means = DeviceArray(nb, np.float32)
     call("rvp_batch_sor_exact", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, int(k), float(alpha), out.ptr,
          out_off.ptr, None, means.ptr, stream)
-    o = out_off.numpy()
-    res = out.numpy()
-    return [res[o[c]:o[c + 1]].copy() for c in range(batch.ncl)], means.numpy()[:batch.n]
+    o = out_off.numpy(stream=stream)
+    res = out.numpy(stream=stream)
+    return [res[o[c]:o[c + 1]].copy() for c in range(batch.ncl)], means.numpy(stream=stream)[:batch.n]
 
 
 def batch_radius_search(batch: Batch, radius, max_nn, cell=None, stream=None):
@@ -420,7 +420,7 @@ def batch_radius_search(batch: Batch, radius, max_nn, cell=None, stream=None):
     cnt = DeviceArray(nb, np.int32)
     call("rvp_batch_radius_search", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, dr.ptr,
          dc.ptr if dc is not None else None, int(max_nn), idx.ptr, d2.ptr, cnt.ptr, stream)
-    return idx.numpy()[:batch.n], d2.numpy()[:batch.n], cnt.numpy()[:batch.n]
+    return tuple(a.numpy(stream=stream)[:batch.n] for a in (idx, d2, cnt))
 
 
 def batch_normals_knn(batch: Batch, k, vp=(0.0, 0.0, 0.0), stream=None):
@@ -429,7 +429,7 @@ def batch_normals_knn(batch: Batch, k, vp=(0.0, 0.0, 0.0), stream=None):
     nx, ny, nz = (DeviceArray(nb, np.float32) for _ in range(3))
     call("rvp_batch_normals_knn", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, int(k), float(vp[0]),
          float(vp[1]), float(vp[2]), nx.ptr, ny.ptr, nz.ptr, stream)
-    return np.stack([nx.numpy()[:batch.n], ny.numpy()[:batch.n], nz.numpy()[:batch.n]], 1)
+    return np.stack([a.numpy(stream=stream)[:batch.n] for a in (nx, ny, nz)], 1)
 
 
 def batch_normals_radius(batch: Batch, radius, max_nn, vp=(0.0, 0.0, 0.0), stream=None):
@@ -439,7 +439,7 @@ def batch_normals_radius(batch: Batch, radius, max_nn, vp=(0.0, 0.0, 0.0), strea
     nx, ny, nz = (DeviceArray(nb, np.float32) for _ in range(3))
     call("rvp_batch_normals_radius", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, r.ptr, None, int(max_nn),
          float(vp[0]), float(vp[1]), float(vp[2]), nx.ptr, ny.ptr, nz.ptr, stream)
-    return np.stack([nx.numpy()[:batch.n], ny.numpy()[:batch.n], nz.numpy()[:batch.n]], 1)
+    return np.stack([a.numpy(stream=stream)[:batch.n] for a in (nx, ny, nz)], 1)
 
 
 def batch_voxel(batch: Batch, leaf, stream=None):
@@ -451,9 +451,9 @@ def batch_voxel(batch: Batch, leaf, stream=None):
     status = DeviceArray(batch.ncl, np.uint32)
     call("rvp_batch_voxel", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, lf.ptr, out.ptr, out_off.ptr,
          status.ptr, stream)
-    o = out_off.numpy()
-    res = out.numpy()
-    return [res[o[c]:o[c + 1]].copy() for c in range(batch.ncl)], status.numpy()
+    o = out_off.numpy(stream=stream)
+    res = out.numpy(stream=stream)
+    return [res[o[c]:o[c + 1]].copy() for c in range(batch.ncl)], status.numpy(stream=stream)
 
 
 def batch_ransac_plane(batch: Batch, dist_thresh, max_iters, collinear_thresh=1e-6, probability=0.99, stream=None):
@@ -491,8 +491,8 @@ def euclidean_cluster(cloud, tolerance, min_size=1, max_size=2**31 - 1, hash_=No
     m = C.c_size_t()
     call("rvp_euclidean_cluster", hash_.handle, float(tolerance), int(min_size), int(max_size), idx.ptr, off.ptr,
          C.byref(m), None, stream)
-    o = off.numpy()[:m.value + 1]
-    I = idx.numpy()
+    o = off.numpy(stream=stream)[:m.value + 1]
+    I = idx.numpy(stream=stream)
     return [I[o[i]:o[i + 1]].copy() for i in range(m.value)]
 
 

@@ -116,8 +116,9 @@ __device__ void jacobi_smallest_f64(double a00, double a01, double a02, double a
 }
 
 // Smallest-eigenvalue eigenvector of a symmetric 3x3 (Smith's trigonometric eigenvalues, eigenvector from
-// the largest cross product of two rows of A - l3 I).  Returns true when it fell back to FP64 Jacobi
-// (relative gap (l2 - l3) / l1 < 1e-4, or an isotropic spectrum).
+// the largest cross product of two rows of A - l3 I).  Returns true when it fell back to FP64 Jacobi: a
+// relative gap (l2 - l3) / l1 that may lie below 1e-4 once the rounding of the trigonometric solve is
+// allowed for, or an isotropic spectrum.
 __device__ bool smallest_eigvec(const float c[6], float n[3]) {
   float a00 = c[0], a01 = c[1], a02 = c[2], a11 = c[3], a12 = c[4], a22 = c[5];
   float scale = fmaxf(fmaxf(fmaxf(fabsf(a00), fabsf(a01)), fmaxf(fabsf(a02), fabsf(a11))),
@@ -141,7 +142,11 @@ __device__ bool smallest_eigvec(const float c[6], float n[3]) {
     const float l1 = q + 2.0f * p * cosf(phi);
     const float l3 = q + 2.0f * p * cosf(phi + 2.0943951023931953f);
     const float l2 = 3.0f * q - l1 - l3;
-    if (!((l2 - l3) >= 1e-4f * fmaxf(fabsf(l1), 1e-30f))) {
+    // Near a double smallest root (phi -> 0) acos loses half the digits of r: with r within 2^-18 of its true
+    // value, phi is off by up to sqrt(2 * 2^-18) / 3 < 9.2e-4, so l2 - l3 = 2 sqrt(3) p sin(phi) is off by up to
+    // 3.2e-3 p.  The FP64 solve therefore also takes every spectrum whose computed gap does not exceed the
+    // 1e-4 relative threshold by that bound: a true gap below 1e-4 |l1| never reaches the closed form.
+    if (!((l2 - l3) >= 1e-4f * fmaxf(fabsf(l1), 1e-30f) + 3.2e-3f * p)) {
       fallback = true;
     } else {
       const float r0[3] = {a00 - l3, a01, a02}, r1[3] = {a01, a11 - l3, a12}, r2[3] = {a02, a12, a22 - l3};

@@ -152,16 +152,23 @@ def batch_sizes(count=256, lo=2_000, hi=500_000, seed=3000):
     return np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
 
 
-def batch_clouds(count=256, seed=3000, with_scales=False):
+def batch_cloud(i, n, seed=3000):
+    """Cloud i of the batch (n points): a plane+sphere cloud scaled by s ~ U[1, 20]; returns (cloud, s)."""
+    rng = _rng(seed + 1 + i)
+    s = rng.uniform(1.0, 20.0)
+    return plane_sphere_cloud(int(n), seed=seed + 1 + i, noise=0.002, scale=s), s
+
+
+def batch_clouds(count=256, seed=3000, with_scales=False, which=None):
     """`count` plane+sphere clouds with log-uniform sizes in [2e3, 5e5], each scaled by s ~ U[1, 20] (all
-    lengths, noise included); with_scales also returns the scales (the radius of cloud i is 0.02 s_i)."""
+    lengths, noise included); with_scales also returns the scales (the radius of cloud i is 0.02 s_i).
+    which: indices of the clouds to generate (default all), in that order."""
     sizes = batch_sizes(count, seed=seed)
     out, scales = [], []
-    for i, n in enumerate(sizes):
-        rng = _rng(seed + 1 + i)
-        s = rng.uniform(1.0, 20.0)
+    for i in (range(count) if which is None else which):
+        c, s = batch_cloud(i, sizes[i], seed)
+        out.append(c)
         scales.append(s)
-        out.append(plane_sphere_cloud(int(n), seed=seed + 1 + i, noise=0.002, scale=s))
     return (out, np.array(scales)) if with_scales else out
 
 

@@ -1,9 +1,14 @@
 // unit_checks.cpp -- the checks of the reference's own unit tests (tests/unit/test_radius.cpp,
 // test_sor.cpp, test_normal.cpp, test_voxel_grid.cpp, test_ransac.cpp), run against the drop-in API
-// (include/rvv_pcl.h) with the same std::mt19937(42) inputs.  Exit status 0 = all pass.
-// test_normal.cpp builds an Octree; the drop-in has no Octree, so that check uses the SpatialHash overload.
+// (include/rvv_pcl.h) with the same std::mt19937(42) inputs, plus the D1 / D2 helpers of rvv_common.h.
+// Exit status 0 = all pass.  The normals check uses the SpatialHash overload of normal_estimation_rvv (the
+// reference's test_normal.cpp itself, with its Octree, runs unchanged as its own program).
+// With "--dump <file>", the D1 / D2 inputs and outputs are also written there (raw little-endian arrays) so that
+// a test can compare the C++ layer with the oracle.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <iostream>
 #include <random>
@@ -176,7 +181,52 @@ static void ransac_check() {  // tests/unit/test_ransac.cpp:11-64
   EXPECT(ni + no == N && ni >= 450, "extract: inliers + outliers partition the cloud");
 }
 
-int main() {
+// D1 / D2 (rvv_common.cpp:36-68): get_inds_in_radius_rvv appends after what the vectors hold, in subset order,
+// and its d2 values are get_dist_sq_rvv's.
+static void distance_check(const char *dump) {
+  const size_t N = 3000, M = 2000;
+  std::vector<float> x(N), y(N), z(N);
+  std::mt19937 gen(42);
+  std::normal_distribution<float> dist(0.0f, 1.0f);
+  for (size_t i = 0; i < N; ++i) { x[i] = dist(gen); y[i] = dist(gen); z[i] = dist(gen); }
+  std::vector<int> subset(M);
+  for (size_t i = 0; i < M; ++i) subset[i] = (int)((i * 7919u + 13u) % N);  // a permuted subset
+  const float qx = 0.1f, qy = -0.2f, qz = 0.3f, r2 = 0.8f;
+  std::vector<float> d(N);
+  get_dist_sq_rvv(x.data(), y.data(), z.data(), qx, qy, qz, d.data(), N);
+  std::vector<int> idx = {-1, -2, -3};
+  std::vector<float> dd = {9.0f, 8.0f, 7.0f};
+  get_inds_in_radius_rvv(x.data(), y.data(), z.data(), subset.data(), M, qx, qy, qz, r2, idx, dd);
+  bool prefix = idx.size() == dd.size() && idx.size() >= 3 && idx[0] == -1 && idx[1] == -2 && idx[2] == -3 &&
+                dd[0] == 9.0f && dd[1] == 8.0f && dd[2] == 7.0f;
+  EXPECT(prefix, "distance: get_inds_in_radius_rvv appends without clearing");
+  size_t j = 3;
+  bool order = true;
+  for (size_t i = 0; i < M; ++i)
+    if (d[subset[i]] <= r2) {
+      order = order && j < idx.size() && idx[j] == subset[i] && dd[j] == d[subset[i]];
+      ++j;
+    }
+  EXPECT(order && j == idx.size(), "distance: hits in subset order with get_dist_sq_rvv's d2");
+  if (dump) {
+    FILE *f = std::fopen(dump, "wb");
+    if (!f) { ++fails; return; }
+    const uint64_t hdr[3] = {N, M, idx.size()};
+    const float q[4] = {qx, qy, qz, r2};
+    std::fwrite(hdr, sizeof(hdr), 1, f);
+    std::fwrite(q, sizeof(q), 1, f);
+    std::fwrite(x.data(), 4, N, f); std::fwrite(y.data(), 4, N, f); std::fwrite(z.data(), 4, N, f);
+    std::fwrite(subset.data(), 4, M, f);
+    std::fwrite(d.data(), 4, N, f);
+    std::fwrite(idx.data(), 4, idx.size(), f);
+    std::fwrite(dd.data(), 4, dd.size(), f);
+    std::fclose(f);
+  }
+}
+
+int main(int argc, char **argv) {
+  const char *dump = (argc == 3 && std::strcmp(argv[1], "--dump") == 0) ? argv[2] : nullptr;
+  distance_check(dump);
   radius_check();
   sor_check();
   normal_check();

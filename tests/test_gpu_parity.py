@@ -198,8 +198,6 @@ def test_sor_exact(k, alpha):
     assert (out2 == oout).all()
 
 
-@pytest.mark.skip(reason="added with a GPU run (tests, smoke, bench) that faulted the GPU before this test had "
-                         "passed once; the fault's cause is not identified, so the test stays off the GPU until it is")
 @pytest.mark.parametrize("case", ["offset", "duplicates", "tiny_steps"])
 def test_sor_exact_previous_query_bound(case):
     """Exact k-NN means where the bound from the previous query is tight or void: coordinates far from the

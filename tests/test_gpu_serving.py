@@ -1,10 +1,13 @@
 """Streaming frames with continuous batching: every frame's results in a mixed batch equal the frame run
-alone, and the voxel -> SOR chain equals the oracle's."""
+alone, the voxel -> SOR -> RANSAC chain equals the oracle's, and the served normals meet the FP64 oracle's
+bar (the same radius neighbourhoods, capped as the server caps them)."""
 import numpy as np
 import pytest
 
 import oracle as O
 from rvpoint_amd import scenes
+
+from precision import angles_deg, report
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +64,30 @@ def test_served_chain_matches_oracle(frames):
             assert np.allclose(np.linalg.norm(r["normals"][r["normals"].any(1)], axis=1), 1.0, atol=1e-5)
     finally:
         srv.close()
+
+
+def test_served_normals_vs_fp64(frames):
+    """Two slots on two streams with frames queued together: each frame's normals against the FP64 oracle's
+    radius normals (r = 0.5, cap 32, viewpoint (0, 0, 1.7)) of its kept points."""
+    from rvpoint_amd.serve import FrameServer
+    P = params()
+    srv = FrameServer(P, slots=2, slot_points=200_000, hbm_budget=4 << 30)
+    try:
+        res = [srv.wait(t) for t in [srv.submit(f) for f in frames + frames[::-1]]]
+    finally:
+        srv.close()
+    angs = []
+    for r in res:
+        kept = r["points"]
+        x, y, z = (np.ascontiguousarray(kept[:, i]) for i in range(3))
+        ref = O.normals_radius_f64(x, y, z, P.normal_radius, P.normal_radius, P.normal_max_nn,
+                                   (P.vp[0], P.vp[1], P.vp[2]))
+        zero_ref = np.linalg.norm(ref, axis=1) == 0
+        got = r["normals"]
+        assert ((np.linalg.norm(got, axis=1) == 0) == zero_ref).all()
+        angs.append(angles_deg(got[~zero_ref], ref[~zero_ref]))
+    rep = report("served frames (2 slots): radius 0.5 cap 32 normals", np.concatenate(angs))
+    assert rep["angle_deg_median"] <= 0.01 and rep["angle_deg_p99"] <= 1.0
 
 
 def test_server_rejects_oversized_frames(frames):

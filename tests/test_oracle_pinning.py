@@ -25,6 +25,31 @@ def test_glibc_rand_matches_host_libc():
         assert (O.rand_stream(seed, 2000) == host).all()
 
 
+@needs_ref
+def test_dist_sq_and_inds_in_radius_equal_reference():
+    """D1 / D2 (rvv_common.cpp:36-68): per-element bits, subset order, and appending after existing entries."""
+    R = O.ref()
+    p, (x, y, z) = cloud(4000, 16)
+    q = np.float32([0.1, -0.2, 0.3])
+    ref = np.zeros(len(x), np.float32)
+    R.ref_get_dist_sq(x, y, z, *q, ref, len(x))
+    d = O.dist_sq(x, y, z, q)
+    assert (d == ref).all()
+    # numpy fp32 in the reference's order: ((dx*dx + dy*dy) + dz*dz), dx = p - q
+    dx, dy, dz = x - q[0], y - q[1], z - q[2]
+    assert (d == ((dx * dx + dy * dy) + dz * dz)).all()
+    sub = np.random.default_rng(2).permutation(len(x))[:2500].astype(np.int32)
+    for r2 in (np.float32(0.0), np.float32(0.7), np.float32(100.0)):
+        oi = np.zeros(len(sub) + 3, np.int32)
+        od = np.zeros(len(sub) + 3, np.float32)
+        oi[:3], od[:3] = [5, 6, 7], [0.5, 0.25, 0.125]
+        m = R.ref_inds_in_radius(x, y, z, sub, len(sub), *q, r2, oi, od, 3)
+        gi, gd = O.inds_in_radius(x, y, z, sub, q, r2, [5, 6, 7], [0.5, 0.25, 0.125])
+        assert m == len(gi) and (oi[:m] == gi).all() and (od[:m] == gd).all()
+        hit = d[sub] <= r2
+        assert (gi[3:] == sub[hit]).all() and (gd[3:] == d[sub][hit]).all()
+
+
 def test_cell_coords_closed_form_edges():
     # points exactly on cell edges, negative coordinates
     pts = np.array([[0.0, 0.0, 0.0], [1.0, -1.0, 2.0], [-3.0, 0.5, 0.25], [0.5, 0.5, 0.5]], np.float32)
@@ -108,6 +133,36 @@ def test_sor_sc_equals_reference(k):
     assert m == len(out) and (rout[:m] == out).all()
     m2 = R.ref_sor_rvv(x, y, z, len(p), rout.reshape(-1), k, 1.0)
     assert m2 == len(out) and (rout[:m2] == out).all()
+
+
+@needs_ref
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_sor_sc_threaded_equals_reference(threads):
+    """The all-core CPU baseline's SOR: per-point means on several threads, global sums sequential."""
+    R = O.ref()
+    p, _ = cloud(1800, 14)
+    out, means = O.sor_sc_mt(p, 12, 1.0, nthreads=threads)
+    out1, means1 = O.sor_sc(p, 12, 1.0)
+    assert (means == means1).all() and out.shape == out1.shape and (out == out1).all()
+    rout = np.zeros_like(p)
+    m = R.ref_sor_sc(p.reshape(-1).copy(), len(p), rout.reshape(-1), 12, 1.0)
+    assert m == len(out) and (rout[:m] == out).all()
+
+
+@needs_ref
+@pytest.mark.parametrize("threads", [1, 4])
+def test_normals_hash_e1_threaded_equals_reference(threads):
+    """The all-core CPU baseline's normals: normal_estimation_rvv(SpatialHash) semantics restated, bit-identical
+    to the reference's own function (same libm), on any thread count."""
+    R = O.ref()
+    p, (x, y, z) = cloud(3000, 15, 0.2)
+    n = len(x)
+    for cell, r in ((0.05, 0.05), (0.1, 0.07)):
+        rx, ry, rz = (np.zeros(n, np.float32) for _ in range(3))
+        R.ref_normal_hash(x, y, z, n, cell, rx, ry, rz, 10, r, 0.1, -0.2, 0.3, 4)
+        ox, oy, oz = O.normals_hash_e1(x, y, z, cell, r, (0.1, -0.2, 0.3), 4, nthreads=threads)
+        assert (rx == ox).all() and (ry == oy).all() and (rz == oz).all()
+        assert (np.abs(ox) + np.abs(oy) + np.abs(oz) == 0).sum() < n  # not all degenerate
 
 
 def test_sor_sc_brute_force_small():
