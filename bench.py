@@ -70,6 +70,10 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="one device-to-device copy of this many MiB before warmup (HBM counter calibration)")
+    ap.add_argument("--only", choices=["frame", "batched", "indoor", "serving"], default=None,
+                    help="run one leg alone (profiling runs); the JSON line then carries that leg only")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the all-core CPU leg (0: all this "
+                    "process may use)")
     return ap.parse_args()
 
 
@@ -408,6 +412,98 @@ def cpu_baseline(frame, info, args):
             "stage_seconds": {k: round(v, 4) for k, v in t.items()}}
 
 
+def host_cpu():
+    """CPUs this process may use and the host's CPU model name."""
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        ncpu = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return ncpu, model
+
+
+def cpu_all_core(frame, gpu_info, args):
+    """The same frame through the oracle's restatement of the scalar path on every host core: the per-point work
+    (SOR's k nearest and mean, the hash normals) in OpenMP threads over points, the global SOR sums, voxel sums
+    and the RANSAC loop sequential as in the reference; its outputs equal the reference's bit for bit (the
+    threaded functions are pinned against oracle/_ref on the CPU)."""
+    import oracle as O
+    ncpu, model = host_cpu()
+    th = args.cpu_threads or ncpu
+    pts = np.ascontiguousarray(frame, np.float32)
+    n = len(pts)
+    t = {}
+    t0 = time.perf_counter()
+    vox = O.voxel_sc(pts, 0.1)
+    t["voxel"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    kept, _ = O.sor_sc_mt(vox, 50, 1.0, nthreads=th)
+    t["sor"] = time.perf_counter() - t0
+    x, y, z = (np.ascontiguousarray(kept[:, i]) for i in range(3))
+    t0 = time.perf_counter()
+    O.normals_hash_e1(x, y, z, 0.5, 0.5, (0.0, 0.0, 1.7), 4, nthreads=th)
+    t["normals"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.ransac_plane(x, y, z, 0.05, 1000)
+    t["ransac"] = time.perf_counter() - t0
+    total = sum(t.values())
+    return {"value": round(n / total / 1e6, 6), "unit": "Mpts/s", "cores": th, "kind": "port",
+            "host_cpus": ncpu, "cpu_model": model,
+            "sample": f"whole frame, oracle port on {th} threads (OpenMP over points; global sums, voxel and RANSAC "
+                      f"sequential): voxel 0.1 ({n} -> {len(vox)}), sor_sc k=50 ({len(vox)} -> {len(kept)}), "
+                      f"hash radius-0.5 normals with the reference's eigen step, ransac 1000; {total:.2f} s",
+            "same_counts_as_gpu": bool(len(vox) == gpu_info["n_voxel"] and len(kept) == gpu_info["n_sor"]),
+            "stage_seconds": {k: round(v, 4) for k, v in t.items()}}
+
+
+def frame_precision(pipe):
+    """The frame's normals (radius 0.5, cap 64, viewpoint (0, 0, 1.7)) against the FP64 oracle's normals of the
+    same capped (d2, idx) neighbourhoods: median / p99 / max angle in degrees and the FP64-fallback count."""
+    import oracle as O
+    from rvpoint_amd import normals_fallback_count
+    res = pipe.results()
+    kept, got = res["kept"], res["normals"].astype(np.float64)
+    x, y, z = (np.ascontiguousarray(kept[:, i]) for i in range(3))
+    ref = O.normals_radius_f64(x, y, z, 0.5, 0.5, 64, (0.0, 0.0, 1.7))
+    ok = np.linalg.norm(ref, axis=1) > 0
+    c = np.abs((got[ok] * ref[ok]).sum(1)) / np.maximum(np.linalg.norm(got[ok], axis=1), 1e-300)
+    ang = np.degrees(np.arccos(np.clip(c, -1.0, 1.0)))
+    return {"normals_vs_fp64_deg": {"median": float(np.median(ang)), "p99": float(np.percentile(ang, 99)),
+                                    "max": float(ang.max()), "points": int(ok.sum())},
+            "fp64_fallbacks": int(normals_fallback_count()), "zero_normal_sets_equal":
+            bool(((np.linalg.norm(got, axis=1) == 0) == ~ok).all())}
+
+
+def copy_bandwidth(R, mib=1024, reps=5):
+    """Measured device-to-device copy bandwidth (read + write bytes per second), HIP events on one stream."""
+    from rvpoint_amd._lib import call
+    nb = mib << 20
+    a, b = R.DeviceArray(nb // 4), R.DeviceArray(nb // 4)
+    ev = [C.c_void_p(), C.c_void_p()]
+    for e in ev:
+        call("rvp_event_create", C.byref(e))
+    call("rvp_memcpy_d2d", b.ptr, a.ptr, nb, None)
+    best = None
+    for _ in range(reps):
+        call("rvp_event_record", ev[0], None)
+        call("rvp_memcpy_d2d", b.ptr, a.ptr, nb, None)
+        call("rvp_event_record", ev[1], None)
+        ms = C.c_float()
+        call("rvp_event_elapsed", ev[0], ev[1], C.byref(ms))
+        best = ms.value if best is None else min(best, ms.value)
+    for e in ev:
+        call("rvp_event_destroy", e)
+    del a, b
+    return round(2 * nb / (best * 1e-3) / 1e9, 1)
+
+
 def measured_counters(kernel):
     """HBM bytes (FETCH_SIZE and WRITE_SIZE, scaled by a device copy of known size) and VALU instructions per
     dispatch of `kernel`, from the committed counter summary of this workload; (None, None) when absent."""
@@ -455,35 +551,33 @@ def roofline(prof, info, steps):
     return out, prof
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU, and "
-              "the multi-GPU split of this workload is described in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-    import rvpoint_amd as R
-    from rvpoint_amd import _lib, scenes
-    if not R.gpu_available():
-        print("no GPU visible to librvp_hip.so", file=sys.stderr)
-        sys.exit(1)
-    print("bench: generating LiDAR frame (64 rings x 18,750 azimuths)", file=sys.stderr, flush=True)
-    frame = scenes.lidar_frame()
-    if args.copy_mib > 0:
-        nb = args.copy_mib << 20
-        a, b = R.DeviceArray(nb // 4), R.DeviceArray(nb // 4)
-        _lib.call("rvp_memcpy_d2d", b.ptr, a.ptr, nb, None)
-        R.synchronize()
-        del a, b
+def pipeline_roofline(info, per_kernel, ms):
+    """The whole frame against HBM: algorithmic bytes of every stage (inputs read once, outputs written once) over
+    the frame's wall time.  voxel 20 B/pt + 16 B/voxel; SOR 41 B/pt; the normals' grid 28 B/pt and normals
+    36 B/pt; RANSAC 12 B/pt per counting launch."""
+    rounds = per_kernel.get("seg_plane_count_k", (1, 0.0))[0]
+    parts = {"voxel": 20 * info["n_in"] + 16 * info["n_voxel"], "sor": 41 * info["n_voxel"],
+             "normals_grid": 28 * info["n_sor"], "normals": 36 * info["n_sor"],
+             "ransac": int(12 * info["n_sor"] * rounds)}
+    total = sum(parts.values())
+    achieved = total / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "bytes_per_frame": int(total), "stage_bytes": parts, "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5)}
+
+
+def frame_leg(R, args, frame):
+    """The headline leg: the LiDAR frame through the plain path and the device frame pipeline."""
+    from rvpoint_amd import _lib
+    from rvpoint_amd.pipeline import DeviceFramePipeline
     # the plain path: single-cloud entry points, a host round trip per stage
     plain = FramePipeline(frame)
     for _ in range(args.warmup):
-        info_plain = plain.step()
+        plain.step()
     R.synchronize()
     plain_ms, plain_times = time_steps(R, plain.step, args.steps)
     info_plain = plain.step()
     R.synchronize()
     # the measured path: the frame as a batch of one, stages chained on device offsets
-    from rvpoint_amd.pipeline import DeviceFramePipeline
     pipe = DeviceFramePipeline(frame)
     for _ in range(args.warmup):
         pipe.step()
@@ -507,20 +601,10 @@ def main():
     same = all(info[k] == info_plain[k] for k in ("n_voxel", "n_sor", "inliers", "model"))
     rl, prof = roofline(prof, info, args.steps)
     line = {
-        "metric": "Mpts/s hash-grid radius/kNN+SOR+normals; batched vs per-cloud; p99 ms/frame",
         "value": round(info["n_in"] / (ms * 1e-3) / 1e6, 4),
-        "unit": "Mpts/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
         "p50_ms": round(float(np.percentile(times, 50)), 4),
         "p99_ms": round(float(np.percentile(times, 99)), 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
         "config": {"workload": "lidar_frame: 64 rings x 18750 azimuths (1.2M rays), voxel 0.1 -> SOR k=50 "
                                "-> normals r=0.5 cap 64 -> RANSAC 1000 thr 0.05",
                    "points": info["n_in"], "n_voxel": info["n_voxel"], "n_sor": info["n_sor"],
@@ -531,16 +615,58 @@ def main():
                        "mpts_s": round(info["n_in"] / (plain_ms * 1e-3) / 1e6, 4),
                        "same_results": bool(same)},
         "roofline": rl,
+        "pipeline_roofline": pipeline_roofline(info, per_kernel, ms),
         "kernels_ms_per_step": {k: round(v[1], 4) for k, v in sorted(per_kernel.items(),
                                                                      key=lambda kv: -kv[1][1])[:12]},
     }
-    line["batched"] = None if args.no_batch else batch_leg(R, args)
-    line["indoor"] = None if args.no_indoor else indoor_leg(R, args)
-    line["serving"] = None if args.no_serve else serving_leg(R, args)
-    if not args.no_cpu_baseline:
+    return line, pipe, info
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU, and "
+              "the multi-GPU split of this workload is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import rvpoint_amd as R
+    from rvpoint_amd import _lib, scenes
+    if not R.gpu_available():
+        print("no GPU visible to librvp_hip.so", file=sys.stderr)
+        sys.exit(1)
+    if args.copy_mib > 0:
+        nb = args.copy_mib << 20
+        a, b = R.DeviceArray(nb // 4), R.DeviceArray(nb // 4)
+        _lib.call("rvp_memcpy_d2d", b.ptr, a.ptr, nb, None)
+        R.synchronize()
+        del a, b
+    line = {"metric": "Mpts/s hash-grid radius/kNN+SOR+normals; batched vs per-cloud; p99 ms/frame", "value": None,
+            "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
+    legs = [args.only] if args.only else ["frame", "batched", "indoor", "serving"]
+    frame = pipe = info = None
+    if "frame" in legs:
+        print("bench: generating LiDAR frame (64 rings x 18,750 azimuths)", file=sys.stderr, flush=True)
+        frame = scenes.lidar_frame()
+        part, pipe, info = frame_leg(R, args, frame)
+        line.update(part)
+        line["hbm_copy_gbs"] = copy_bandwidth(R)
+    if "batched" in legs:
+        line["batched"] = None if args.no_batch else batch_leg(R, args)
+    if "indoor" in legs:
+        line["indoor"] = None if args.no_indoor else indoor_leg(R, args)
+    if "serving" in legs:
+        line["serving"] = None if args.no_serve else serving_leg(R, args)
+    if "frame" in legs and not args.no_cpu_baseline:
+        print("bench: frame precision against the FP64 oracle", file=sys.stderr, flush=True)
+        line["precision"] = frame_precision(pipe)
         print("bench: CPU baseline (reference scalar path, 1 thread)", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(frame, info, args)
-    else:
+        single = cpu_baseline(frame, info, args)
+        ncpu, model = host_cpu()
+        single["host_cpus"], single["cpu_model"] = ncpu, model
+        print("bench: CPU baseline (all cores)", file=sys.stderr, flush=True)
+        single["all_cores"] = cpu_all_core(frame, info, args)
+        line["cpu_baseline"] = single
+    elif "frame" in legs:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
 
