@@ -215,6 +215,16 @@ int rvp_plane_extract(const float *x, const float *y, const float *z, size_t n, 
                       float dist_thresh, float *inliers_aos, size_t *n_in, float *outliers_aos,
                       size_t *n_out, rvp_stream_t s);
 
+/* Allocation-free variant of rvp_plane_extract (same arguments, same results).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under graph capture, tight serving
+ * loops).  It calls no hipMalloc / hipFree: its look-back status words, ticket counter and counts live in one
+ * fixed array per device (0.5 MiB, held until the process ends), so n is at most 134,217,728 (larger n is
+ * rejected with RVP_E_UNSUPPORTED).  Every call on a device shares that array: calls that overlap on one device
+ * (other streams or threads) race and must be serialised by the caller. */
+int rvp_plane_extract_noalloc(const float *x, const float *y, const float *z, size_t n, const float *model_host,
+                              float dist_thresh, float *inliers_aos, size_t *n_in, float *outliers_aos,
+                              size_t *n_out, rvp_stream_t s);
+
 /* ---- R1 + X1 fused: the stage split of pipeline_export.cpp:457-498 --------------------------------
  * rvp_ransac_plane, then (only when a model was found, *inliers > 0) the one-pass split of the same points by
  * that model into inliers / outliers (AoS, device, index order; either may be NULL); counts host. */

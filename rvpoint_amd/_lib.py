@@ -87,6 +87,7 @@ _SIGS = {
     "rvp_ransac_plane": [vp, vp, vp, sz, fl, i32, fl, fl, pfl, pi32, pi32, vp],
     "rvp_plane_counts": [vp, vp, vp, sz, vp, i32, fl, vp, vp],
     "rvp_plane_extract": [vp, vp, vp, sz, pfl, fl, vp, psz, vp, psz, vp],
+    "rvp_plane_extract_noalloc": [vp, vp, vp, sz, pfl, fl, vp, psz, vp, psz, vp],
     "rvp_ransac_plane_extract": [vp, vp, vp, sz, fl, i32, fl, fl, pfl, pi32, pi32, vp, psz, vp, psz, vp],
     "rvp_batch_sor_exact": [vp, vp, vp, vp, i32, sz, i32, fl, vp, vp, vp, vp, vp],
     "rvp_batch_radius_search": [vp, vp, vp, vp, i32, sz, vp, vp, i32, vp, vp, vp, vp],

@@ -881,3 +881,82 @@ int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_
 
 }  // namespace rvp
 
+
+// ------------------------------------------------------------------------------------------------
+// Allocation-free plane split: the X1 split of rvp_plane_extract with its look-back state (status words,
+// ticket counter) and its two counts in a fixed per-device array instead of stream-ordered allocations, for
+// callers that cannot allocate device memory (code under graph capture, serving loops).
+// ------------------------------------------------------------------------------------------------
+namespace rvp {
+
+constexpr size_t NOALLOC_TILES = size_t(1) << 16;                  // status words: one per tile
+constexpr size_t NOALLOC_MAX_POINTS = NOALLOC_TILES * SCAN_TILE;   // 134,217,728 points
+// [0, NOALLOC_TILES): status words; NOALLOC_TILES: ticket counter (low half); NOALLOC_TILES + 1: the counts
+static __device__ __align__(256) uint64_t scratch[NOALLOC_TILES + 2];
+
+struct NoAllocDevice {
+  uint64_t *ptr = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
+  uint32_t epoch = 0;
+};
+static std::mutex g_noalloc_mu;
+static NoAllocDevice g_noalloc[64];
+
+__global__ void noalloc_ticket_clear_k(uint64_t *sc) {
+  if (threadIdx.x == 0) reinterpret_cast<uint32_t *>(sc + NOALLOC_TILES)[0] = 0u;
+}
+
+// this device's scratch address (looked up once per device, with the device current) and the call's epoch
+static int noalloc_scratch(uint64_t **ptr, uint32_t *epoch) {
+  int dev = 0;
+  RVP_HIP(hipGetDevice(&dev));
+  RVP_CHECK(dev >= 0 && dev < 64, RVP_E_UNSUPPORTED, "device index");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  NoAllocDevice &d = g_noalloc[dev];
+  if (!d.ptr) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    RVP_CHECK(e == hipSuccess && p, RVP_E_DEVICE,
+              std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    d.ptr = static_cast<uint64_t *>(p);
+  }
+  // the array starts zeroed (epoch 0 = cleared); status words carry the call's epoch
+  d.epoch = (d.epoch + 1) & 0x3fffffffu;
+  if (d.epoch == 0) d.epoch = 1;
+  *ptr = d.ptr;
+  *epoch = d.epoch;
+  return RVP_OK;
+}
+
+}  // namespace rvp
+
+extern "C" int rvp_plane_extract_noalloc(const float *x, const float *y, const float *z, size_t n,
+                                         const float *model_host, float dist_thresh, float *inliers_aos,
+                                         size_t *n_in, float *outliers_aos, size_t *n_out, rvp_stream_t st) {
+  using namespace rvp;
+  RVP_CHECK(model_host && n_in && n_out, RVP_E_ARG, "null argument");
+  *n_in = *n_out = 0;
+  if (n == 0) return RVP_OK;
+  RVP_CHECK(x && y && z, RVP_E_ARG, "null coordinate pointer");
+  RVP_CHECK(n <= NOALLOC_MAX_POINTS, RVP_E_UNSUPPORTED,
+            "rvp_plane_extract_noalloc: more points than its fixed scratch holds (134,217,728)");
+  hipStream_t s = S(st);
+  uint64_t *sc = nullptr;
+  uint32_t epoch = 0;
+  RVP_TRY(noalloc_scratch(&sc, &epoch));
+  const size_t nb = ceil_div(n, SCAN_TILE);
+  uint32_t *ticket = reinterpret_cast<uint32_t *>(sc + NOALLOC_TILES);
+  uint32_t *counts = reinterpret_cast<uint32_t *>(sc + NOALLOC_TILES + 1);
+  // the ticket must start at zero (a launch leaves it there, but a failed one may not): cleared in stream order
+  noalloc_ticket_clear_k<<<1, WAVE, 0, s>>>(sc);
+  RVP_HIP(hipGetLastError());
+  plane_split_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(x, y, z, n, model_host[0], model_host[1], model_host[2],
+                                                        model_host[3], dist_thresh, sc, ticket, (uint32_t)nb, epoch,
+                                                        inliers_aos, outliers_aos, counts);
+  RVP_HIP(hipGetLastError());
+  uint32_t h[2] = {0, 0};
+  RVP_HIP(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, s));
+  RVP_HIP(hipStreamSynchronize(s));
+  *n_in = h[0];
+  *n_out = h[1];
+  return RVP_OK;
+}
