@@ -27,6 +27,23 @@ def soa(p):
     return [np.ascontiguousarray(p[:, i]) for i in range(3)]
 
 
+def voxel_centroid_error(p, leaf, got):
+    """Voxel centroids (fp32, the reference's sequential sums) against the FP64 mean of each voxel's points:
+    voxels keyed as voxel_grid_downsamp_sc keys them (floor(fl(x * fl(1/leaf))), voxel_grid_downsamp.cpp:189-194)
+    and ordered lexicographically; returns the median and max distance in metres."""
+    inv = np.float32(1.0) / np.float32(leaf)
+    keys = np.floor(p * inv).astype(np.int64)
+    order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
+    k = keys[order]
+    head = np.r_[True, (k[1:] != k[:-1]).any(1)]
+    gid = np.cumsum(head) - 1
+    cnt = np.bincount(gid)
+    mean = np.stack([np.bincount(gid, p[order, i].astype(np.float64)) for i in range(3)], 1) / cnt[:, None]
+    assert len(mean) == len(got)
+    err = np.linalg.norm(got.astype(np.float64) - mean, axis=1)
+    return {"median": float(np.median(err)), "max": float(err.max())}
+
+
 def test_lidar_frame_pipeline_vs_oracle():
     frame = scenes.lidar_frame()
     assert 1_000_000 < len(frame) < 1_100_000
@@ -52,7 +69,8 @@ def test_lidar_frame_pipeline_vs_oracle():
     assert ((v * got.astype(np.float64)).sum(1)[~zero_ref] >= -1e-4).all()
     rep = report("c1 lidar frame: radius 0.5 cap 64 normals of the SOR output", ang,
                  {"voxels": len(vox), "kept": len(kept), "zero_normals": int(zero_ref.sum()),
-                  "fp64_fallbacks": R.normals_fallback_count()})
+                  "fp64_fallbacks": R.normals_fallback_count(),
+                  "voxel_centroid_err_m": voxel_centroid_error(frame, st.leaf, res["voxels"])})
     assert rep["angle_deg_median"] <= 0.01 and rep["angle_deg_p99"] <= 1.0
 
 

@@ -3,13 +3,14 @@ FP32-compensated normals (k=30) and neighbourhood centroids against the FP64 ora
 neighbourhoods, and spatial-hash radius lists (r = 0.02, cap 64) against the oracle's hash search on a
 sample of queries.  Prints the precision report (median / p99 angle and centroid error, FP64 fallbacks)."""
 import ctypes as C
-import json
 
 import numpy as np
 import pytest
 
 import oracle as O
 from rvpoint_amd import scenes
+
+from precision import report
 
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
@@ -47,15 +48,13 @@ def test_indoor_normals_precision_vs_fp64(indoor):
         n64, _ = O.normal_from_points_f64(P)
         ang[j] = np.degrees(np.arccos(min(1.0, abs(float(nrm[j] @ n64)))))
         cerr[j] = np.linalg.norm(cen[j] - P.mean(0))
-    report = {"points": n, "k": K, "sample": SAMPLE,
-              "angle_deg_median": float(np.median(ang)), "angle_deg_p99": float(np.percentile(ang, 99)),
-              "angle_deg_max": float(ang.max()),
-              "centroid_err_m_median": float(np.median(cerr)), "centroid_err_m_p99": float(np.percentile(cerr, 99)),
-              "centroid_err_m_max": float(cerr.max()), "fp64_fallback_fraction": fb / n}
-    print("indoor precision:", json.dumps(report))
-    assert report["angle_deg_median"] <= 0.01
-    assert report["angle_deg_p99"] <= 1.0
-    assert report["centroid_err_m_max"] <= 1e-5
+    rep = report("c2 indoor 20M: k=30 normals and neighbourhood centroids (20k-point sample)", ang,
+                 {"cloud_points": n, "k": K, "centroid_err_m_median": float(np.median(cerr)),
+                  "centroid_err_m_p99": float(np.percentile(cerr, 99)), "centroid_err_m_max": float(cerr.max()),
+                  "fp64_fallback_fraction": fb / n})
+    assert rep["angle_deg_median"] <= 0.01
+    assert rep["angle_deg_p99"] <= 1.0
+    assert rep["centroid_err_m_max"] <= 1e-5
 
 
 def test_indoor_radius_lists_vs_oracle_sample(indoor):
