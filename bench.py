@@ -600,6 +600,23 @@ def frame_leg(R, args, frame):
     info = pipe.info()
     same = all(info[k] == info_plain[k] for k in ("n_voxel", "n_sor", "inliers", "model"))
     rl, prof = roofline(prof, info, args.steps)
+    eager = {"ms_per_step": round(ms, 4), "p50_ms": round(float(np.percentile(times, 50)), 4),
+             "p99_ms": round(float(np.percentile(times, 99)), 4), "mpts_s": round(info["n_in"] / (ms * 1e-3) / 1e6, 4)}
+    # the same chain with voxel -> SOR -> normals replayed as one captured graph (RANSAC's loop stays eager)
+    gpipe = DeviceFramePipeline(frame, graph=True)
+    for _ in range(max(args.warmup, 1)):
+        gpipe.step()
+    R.synchronize()
+    gms, gtimes = time_steps(R, gpipe.step, args.steps)
+    ginfo = gpipe.info()
+    gsame = all(ginfo[k] == info[k] for k in ("n_voxel", "n_sor", "inliers", "model"))
+    graph = {"ms_per_step": round(gms, 4), "p50_ms": round(float(np.percentile(gtimes, 50)), 4),
+             "p99_ms": round(float(np.percentile(gtimes, 99)), 4),
+             "mpts_s": round(info["n_in"] / (gms * 1e-3) / 1e6, 4), "same_results": bool(gsame)}
+    gpipe.close()
+    use_graph = gsame and gms < ms
+    if use_graph:
+        ms, times = gms, gtimes
     line = {
         "value": round(info["n_in"] / (ms * 1e-3) / 1e6, 4),
         "ms_per_step": round(ms, 4),
@@ -609,7 +626,10 @@ def frame_leg(R, args, frame):
                                "-> normals r=0.5 cap 64 -> RANSAC 1000 thr 0.05",
                    "points": info["n_in"], "n_voxel": info["n_voxel"], "n_sor": info["n_sor"],
                    "inliers": info["inliers"], "parallelism": "single GPU",
-                   "path": "frame as a batch of one: stages chained on device offsets, one host wait (RANSAC)"},
+                   "path": ("captured graph of voxel -> SOR -> normals, RANSAC eager; " if use_graph else "") +
+                           "frame as a batch of one: stages chained on device offsets, one host wait (RANSAC)"},
+        "device_eager": eager,
+        "graph": graph,
         "plain_path": {"ms_per_step": round(plain_ms, 4), "p50_ms": round(float(np.percentile(plain_times, 50)), 4),
                        "p99_ms": round(float(np.percentile(plain_times, 99)), 4),
                        "mpts_s": round(info["n_in"] / (plain_ms * 1e-3) / 1e6, 4),

@@ -66,6 +66,18 @@ int rvp_aos_to_soa(const float *aos, size_t n, float *x, float *y, float *z, rvp
  * read-back (no counterpart in the reference, whose drivers know their counts on the host). */
 int rvp_aos_to_soa_bounded(const float *aos, size_t n_bound, const uint32_t *n_dev, float *x, float *y, float *z,
                            rvp_stream_t s);
+/* Graphs (no counterpart in the reference, which has no device): capture the calls made on a created stream
+ * between begin and end into one graph, and replay it on that stream.  The batched stages (rvp_batch_*, the
+ * frame pipeline's voxel, SOR and normals; not RANSAC, whose adaptive stop is replayed on the host) record only
+ * kernels: their look-back epochs live on the device and their temporaries come from an arena the graph keeps,
+ * so every replay recomputes from the inputs' current contents.  Run the calls once eagerly on the same stream
+ * before capturing (first-use allocations happen then).  Replays must not overlap with eager calls on other
+ * streams that write the same buffers; destroy() waits for the stream. */
+typedef struct rvp_graph rvp_graph;
+int rvp_graph_capture_begin(rvp_stream_t s);
+int rvp_graph_capture_end(rvp_stream_t s, rvp_graph **graph);
+int rvp_graph_launch(rvp_graph *graph, rvp_stream_t s);
+int rvp_graph_destroy(rvp_graph *graph);
 /* Per-kernel timing: while enabled, every kernel launch of the library is bracketed by a HIP event pair on
  * the stream it runs on.  collect() synchronises, writes "kernel launches total_ms" lines into buf
  * (NUL-terminated, *needed = size required) and clears the records. */

@@ -52,6 +52,10 @@ _SIGS = {
     "rvp_event_elapsed": [vp, vp, pfl],
     "rvp_aos_to_soa": [vp, sz, vp, vp, vp, vp],
     "rvp_aos_to_soa_bounded": [vp, sz, vp, vp, vp, vp, vp],
+    "rvp_graph_capture_begin": [vp],
+    "rvp_graph_capture_end": [vp, pvp],
+    "rvp_graph_launch": [vp, vp],
+    "rvp_graph_destroy": [vp],
     "rvp_profile_enable": [i32],
     "rvp_profile_collect": [C.c_char_p, sz, psz],
     "rvp_profile_filter": [C.c_char_p],

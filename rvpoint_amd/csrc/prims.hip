@@ -65,10 +65,21 @@ __global__ __launch_bounds__(PRIM_THREADS) void scan_down_k(const uint32_t *in, 
   }
 }
 
+// zeroes `words` 32-bit words in stream order (a kernel rather than hipMemsetAsync, so that a stream capture
+// records no memset node)
+__global__ void zero_words_k(uint32_t *__restrict__ p, uint32_t words) {
+  if (threadIdx.x < words) p[threadIdx.x] = 0u;
+}
+int zero_words(void *p, uint32_t words, hipStream_t s) {
+  RVP_CHECK(words <= 256, RVP_E_ARG, "zero_words: at most 256 words");
+  RVP_LAUNCH("zero_words_k", s, zero_words_k<<<1, 256, 0, s>>>(static_cast<uint32_t *>(p), words));
+  return RVP_OK;
+}
+
 template <class Out>
 static int exclusive_scan_impl(const uint32_t *in, Out *out, size_t n, Out *total_dev, hipStream_t s) {
   if (n == 0) {
-    if (total_dev) RVP_HIP(hipMemsetAsync(total_dev, 0, sizeof(Out), s));
+    if (total_dev) RVP_TRY(zero_words(total_dev, sizeof(Out) / 4, s));
     return RVP_OK;
   }
   const size_t nb = ceil_div(n, SCAN_TILE);
@@ -81,15 +92,14 @@ static int exclusive_scan_impl(const uint32_t *in, Out *out, size_t n, Out *tota
 }
 
 // ------------------------------------------------------------------------------------------------
-// single-pass exclusive scan (decoupled look-back) for 32-bit sums.  Tiles take their index from a ticket
-// counter in dispatch order (atomicInc wraps it back to 0 after the last tile), publish their aggregate,
-// and wave 0 looks back over up to 64 predecessors at a time for the nearest inclusive prefix.  A tile only
-// waits on tiles with smaller tickets, which were dispatched before it, so the wait ends; a spin limit
-// turns a wait that would not end into a counted error instead of a hang.  Status words carry the call's
-// epoch (30 bits), so the per-stream status array never needs clearing.  The value travels in the same
-// 64-bit word as its flag, so the status accesses are relaxed device-scope atomics: no other memory has to
-// become visible with them, and an acquire / release pair would write back and invalidate the XCD's L2 on
-// every status access.
+// single-pass exclusive scan (decoupled look-back) for 32-bit sums.  Tiles take their index from the stream's
+// control word in dispatch order (take_tile), publish their aggregate, and wave 0 looks back over up to 64
+// predecessors at a time for the nearest inclusive prefix.  A tile only waits on tiles with smaller tickets,
+// which were dispatched before it, so the wait ends; a spin limit turns a wait that would not end into a
+// counted error instead of a hang.  Status words carry the launch's epoch (30 bits, kept on the device), so
+// the per-stream status array is cleared only when it is created.  The value travels in the same 64-bit word
+// as its flag, so the status accesses are relaxed device-scope atomics: no other memory has to become visible
+// with them, and an acquire / release pair would write back and invalidate the XCD's L2 on every status access.
 // ------------------------------------------------------------------------------------------------
 constexpr uint32_t SP_AGG = 1, SP_INC = 2;
 constexpr uint32_t SP_SPIN_LIMIT = 1u << 24;
@@ -148,12 +158,20 @@ __device__ uint32_t tile_lookback(uint64_t *__restrict__ status, uint32_t tile, 
   return prefix;
 }
 
-// a tile index from the ticket counter; nb on a counter left off zero (counted, the tile does nothing)
-__device__ __forceinline__ uint32_t take_tile(uint32_t *ticket, uint32_t nb, uint32_t *s_tile) {
+// A tile index and the launch's epoch from the stream's control word ctl = (generation << 32) | ticket, in one
+// atomic: tiles take tickets in dispatch order, and the taker of the last ticket (nb - 1) moves the word to the
+// next generation with the ticket back at zero, so every tile of a launch reads the same generation and the next
+// launch on the stream the next one.  The epoch of the status words is that generation mod 2^30 - 1, plus one
+// (0 is the cleared state).  Kept on the device, the epoch needs no host argument, so a captured graph replays
+// with fresh epochs.  nb on a counter left off zero (counted, the tile does nothing).
+__device__ __forceinline__ uint32_t take_tile(uint64_t *ctl, uint32_t nb, uint32_t *s_tile, uint32_t *s_epoch) {
   if (threadIdx.x == 0) {
-    const uint32_t t = atomicInc(ticket, nb - 1);
+    const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long *>(ctl), 1ull);
+    const uint32_t t = (uint32_t)old;
+    if (t == nb - 1) atomicAdd(reinterpret_cast<unsigned long long *>(ctl), (1ull << 32) - (unsigned long long)nb);
     if (t >= nb) atomicAdd(&g_scan_spin_errors, 1ull);
     *s_tile = t < nb ? t : nb;
+    *s_epoch = (uint32_t)((old >> 32) % 0x3fffffffull) + 1u;
   }
   __syncthreads();
   return *s_tile;
@@ -165,10 +183,10 @@ __device__ __forceinline__ uint32_t take_tile(uint32_t *ticket, uint32_t nb, uin
 __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in, uint32_t *out, size_t n_bound,
                                                               const uint32_t *__restrict__ n_dev, uint32_t add,
                                                               uint64_t *__restrict__ status,
-                                                              uint32_t *__restrict__ ticket, uint32_t epoch,
+                                                              uint64_t *__restrict__ ctl,
                                                               uint32_t *__restrict__ total) {
   __shared__ uint32_t sm[PRIM_THREADS / WAVE];
-  __shared__ uint32_t s_tile, s_prefix;
+  __shared__ uint32_t s_tile, s_prefix, s_epoch;
   const size_t n = n_dev ? min((size_t)*n_dev + add, n_bound) : n_bound;
   const uint32_t nb = (uint32_t)((n + SCAN_TILE - 1) / SCAN_TILE);
   if (nb == 0) {
@@ -177,7 +195,8 @@ __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in,
   }
   if (blockIdx.x >= nb) return;
   const int t = threadIdx.x;
-  const uint32_t tile = take_tile(ticket, nb, &s_tile);
+  const uint32_t tile = take_tile(ctl, nb, &s_tile, &s_epoch);
+  const uint32_t epoch = s_epoch;
   if (tile >= nb) return;
   const size_t base = (size_t)tile * SCAN_TILE;
   uint32_t v[SCAN_ITEMS];
@@ -219,14 +238,15 @@ __global__ __launch_bounds__(PRIM_THREADS) void plane_split_k(const float *__res
                                                                const float *__restrict__ z, size_t n, float a,
                                                                float b, float c, float d, float th,
                                                                uint64_t *__restrict__ status,
-                                                               uint32_t *__restrict__ ticket, uint32_t nb,
-                                                               uint32_t epoch, float *__restrict__ in_aos,
+                                                               uint64_t *__restrict__ ctl, uint32_t nb,
+                                                               float *__restrict__ in_aos,
                                                                float *__restrict__ out_aos,
                                                                uint32_t *__restrict__ counts) {
   __shared__ uint32_t sm[PRIM_THREADS / WAVE];
-  __shared__ uint32_t s_tile, s_prefix;
+  __shared__ uint32_t s_tile, s_prefix, s_epoch;
   const int t = threadIdx.x;
-  const uint32_t tile = take_tile(ticket, nb, &s_tile);
+  const uint32_t tile = take_tile(ctl, nb, &s_tile, &s_epoch);
+  const uint32_t epoch = s_epoch;
   if (tile >= nb) return;
   const size_t base = (size_t)tile * SCAN_TILE;
   float px[SCAN_ITEMS], py[SCAN_ITEMS], pz[SCAN_ITEMS];
@@ -292,10 +312,10 @@ __global__ __launch_bounds__(PRIM_THREADS) void compact_flagged_k(const float *_
                                                                    const uint32_t *__restrict__ n_dev,
                                                                    float *__restrict__ out_aos,
                                                                    uint64_t *__restrict__ status,
-                                                                   uint32_t *__restrict__ ticket, uint32_t epoch,
+                                                                   uint64_t *__restrict__ ctl,
                                                                    uint32_t *__restrict__ count) {
   __shared__ uint32_t sm[PRIM_THREADS / WAVE];
-  __shared__ uint32_t s_tile, s_prefix;
+  __shared__ uint32_t s_tile, s_prefix, s_epoch;
   const size_t nn = n_dev ? min((size_t)*n_dev, n) : n;
   const uint32_t nb = (uint32_t)((nn + SCAN_TILE - 1) / SCAN_TILE);
   if (nb == 0) {
@@ -304,7 +324,8 @@ __global__ __launch_bounds__(PRIM_THREADS) void compact_flagged_k(const float *_
   }
   if (blockIdx.x >= nb) return;
   const int t = threadIdx.x;
-  const uint32_t tile = take_tile(ticket, nb, &s_tile);
+  const uint32_t tile = take_tile(ctl, nb, &s_tile, &s_epoch);
+  const uint32_t epoch = s_epoch;
   if (tile >= nb) return;
   const size_t base = (size_t)tile * SCAN_TILE;
   uint32_t f[SCAN_ITEMS];
@@ -352,12 +373,13 @@ __global__ __launch_bounds__(PRIM_THREADS) void run_heads_k(const K *__restrict_
                                                              const uint32_t *__restrict__ n_dev,
                                                              uint32_t *__restrict__ vid, uint32_t *__restrict__ start,
                                                              uint64_t *__restrict__ status,
-                                                             uint32_t *__restrict__ ticket, uint32_t nb,
-                                                             uint32_t epoch, uint32_t *__restrict__ total) {
+                                                             uint64_t *__restrict__ ctl, uint32_t nb,
+                                                             uint32_t *__restrict__ total) {
   __shared__ uint32_t sm[PRIM_THREADS / WAVE];
-  __shared__ uint32_t s_tile, s_prefix;
+  __shared__ uint32_t s_tile, s_prefix, s_epoch;
   const int t = threadIdx.x;
-  const uint32_t tile = take_tile(ticket, nb, &s_tile);
+  const uint32_t tile = take_tile(ctl, nb, &s_tile, &s_epoch);
+  const uint32_t epoch = s_epoch;
   if (tile >= nb) return;
   const size_t nn = n_dev ? min((size_t)*n_dev, n) : n;
   const size_t base = (size_t)tile * SCAN_TILE;
@@ -393,16 +415,15 @@ __global__ __launch_bounds__(PRIM_THREADS) void run_heads_k(const K *__restrict_
   }
 }
 
-// Per-(device, stream) look-back state: status words, ticket counter, epoch.  Launches on one stream run in
-// stream order whichever host thread enqueued them, and each accepts only status words of its own epoch, so
-// several threads may drive one stream.  A status array that has been handed out is never freed while the
-// process runs: a larger one replaces it and the old one is kept (retired), because a launch enqueued by
-// another thread may still read it.
+// Per-(device, stream) look-back state: status words and the control word (generation, ticket; see take_tile).
+// Launches on one stream run in stream order whichever host thread enqueued them, and each accepts only status
+// words of its own epoch, so several threads may drive one stream.  A status array that has been handed out is
+// never freed while the process runs: a larger one replaces it and the old one is kept (retired), because a
+// launch enqueued by another thread may still read it.
 struct ScanState {
   uint64_t *status = nullptr;
-  uint32_t *ticket = nullptr;
+  uint64_t *ctl = nullptr;
   size_t cap = 0;
-  uint32_t epoch = 0;
   std::vector<uint64_t *> retired;
 };
 static std::mutex g_scan_mu;
@@ -413,21 +434,21 @@ static std::map<std::pair<int, hipStream_t>, ScanState> g_scan;
 // stream could then read the allocation's previous contents (a nonzero ticket skips a tile; a stale status
 // word of a matching epoch hands out a wrong prefix), and the sort scatters or gathers built on those prefixes
 // would leave their arrays.
-__global__ void state_clear_k(uint64_t *__restrict__ status, size_t words, uint32_t *__restrict__ ticket) {
+__global__ void state_clear_k(uint64_t *__restrict__ status, size_t words, uint64_t *__restrict__ ctl) {
   const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ticket && i0 == 0) *ticket = 0u;
+  if (ctl && i0 == 0) *ctl = 0ull;
   for (size_t i = i0; i < words; i += (size_t)gridDim.x * blockDim.x) status[i] = 0ull;
 }
 
-static int scan_state(hipStream_t s, size_t tiles, uint64_t **status, uint32_t **ticket, uint32_t *epoch) {
+static int scan_state(hipStream_t s, size_t tiles, uint64_t **status, uint64_t **ctl) {
   int dev = 0;
   RVP_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_scan_mu);
   ScanState &st = g_scan[{dev, s}];
-  uint32_t *clear_ticket = nullptr;
-  if (!st.ticket) {
-    RVP_HIP(hipMalloc(&st.ticket, sizeof(uint32_t)));
-    clear_ticket = st.ticket;
+  uint64_t *clear_ctl = nullptr;
+  if (!st.ctl) {
+    RVP_HIP(hipMalloc(&st.ctl, sizeof(uint64_t)));
+    clear_ctl = st.ctl;
   }
   uint64_t *fresh = nullptr;
   size_t cap = st.cap;
@@ -435,37 +456,34 @@ static int scan_state(hipStream_t s, size_t tiles, uint64_t **status, uint32_t *
     cap = std::max<size_t>(std::max<size_t>(tiles, 4096), 2 * st.cap);
     RVP_HIP(hipMalloc(&fresh, cap * sizeof(uint64_t)));
   }
-  if (fresh || clear_ticket) {
+  if (fresh || clear_ctl) {
     const size_t words = fresh ? cap : 0;
     const unsigned nb = (unsigned)std::min<size_t>(1024, std::max<size_t>(1, ceil_div(words, PRIM_THREADS)));
-    RVP_LAUNCH("state_clear_k", s, state_clear_k<<<nb, PRIM_THREADS, 0, s>>>(fresh, words, clear_ticket));
+    RVP_LAUNCH("state_clear_k", s, state_clear_k<<<nb, PRIM_THREADS, 0, s>>>(fresh, words, clear_ctl));
   }
   if (fresh) {
     if (st.status) st.retired.push_back(st.status);
     st.status = fresh;
     st.cap = cap;
   }
-  st.epoch = (st.epoch + 1) & 0x3fffffffu;
-  if (st.epoch == 0) st.epoch = 1;  // 0 is the cleared state
   *status = st.status;
-  *ticket = st.ticket;
-  *epoch = st.epoch;
+  *ctl = st.ctl;
   return RVP_OK;
 }
 
 int exclusive_scan_u32_bounded(const uint32_t *in, uint32_t *out, size_t n, const uint32_t *n_dev, uint32_t add,
                                uint32_t *total_dev, hipStream_t s) {
   if (n == 0) {
-    if (total_dev) RVP_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), s));
+    if (total_dev) RVP_TRY(zero_words(total_dev, 1, s));
     return RVP_OK;
   }
   const size_t nb = ceil_div(n, SCAN_TILE);
   RVP_CHECK(nb < 0xffffffffull && n < 0xffffffffull, RVP_E_UNSUPPORTED, "scan too large");
   uint64_t *status = nullptr;
-  uint32_t *ticket = nullptr, epoch = 0;
-  RVP_TRY(scan_state(s, nb, &status, &ticket, &epoch));
+  uint64_t *ctl = nullptr;
+  RVP_TRY(scan_state(s, nb, &status, &ctl));
   RVP_LAUNCH("scan_1pass_k", s, scan_1pass_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, out, n, n_dev, add, status,
-                                                                                  ticket, epoch, total_dev));
+                                                                                  ctl, total_dev));
   return RVP_OK;
 }
 
@@ -475,16 +493,16 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *to
 int plane_split_aos(const float *x, const float *y, const float *z, size_t n, const float *m, float t, float *in_aos,
                     float *out_aos, uint32_t *counts_dev, hipStream_t s) {
   if (n == 0) {
-    RVP_HIP(hipMemsetAsync(counts_dev, 0, 2 * sizeof(uint32_t), s));
+    RVP_TRY(zero_words(counts_dev, 2, s));
     return RVP_OK;
   }
   const size_t nb = ceil_div(n, SCAN_TILE);
   RVP_CHECK(n < 0xffffffffull, RVP_E_UNSUPPORTED, "plane split: more than 2^32-1 points");
   uint64_t *status = nullptr;
-  uint32_t *ticket = nullptr, epoch = 0;
-  RVP_TRY(scan_state(s, nb, &status, &ticket, &epoch));
+  uint64_t *ctl = nullptr;
+  RVP_TRY(scan_state(s, nb, &status, &ctl));
   RVP_LAUNCH("plane_split_k", s, plane_split_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(
-                                     x, y, z, n, m[0], m[1], m[2], m[3], t, status, ticket, (uint32_t)nb, epoch,
+                                     x, y, z, n, m[0], m[1], m[2], m[3], t, status, ctl, (uint32_t)nb,
                                      in_aos, out_aos, counts_dev));
   return RVP_OK;
 }
@@ -493,16 +511,16 @@ template <class K>
 static int run_heads_impl(const K *keys, size_t n, const uint32_t *n_dev, uint32_t *vid, uint32_t *start,
                           uint32_t *total_dev, hipStream_t s) {
   if (n == 0) {
-    RVP_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), s));
+    RVP_TRY(zero_words(total_dev, 1, s));
     return RVP_OK;
   }
   RVP_CHECK(n < 0xffffffffull, RVP_E_UNSUPPORTED, "run heads: more than 2^32-1 keys");
   const size_t nb = ceil_div(n, SCAN_TILE);
   uint64_t *status = nullptr;
-  uint32_t *ticket = nullptr, epoch = 0;
-  RVP_TRY(scan_state(s, nb, &status, &ticket, &epoch));
+  uint64_t *ctl = nullptr;
+  RVP_TRY(scan_state(s, nb, &status, &ctl));
   RVP_LAUNCH("run_heads_k", s, run_heads_k<K><<<(unsigned)nb, PRIM_THREADS, 0, s>>>(keys, n, n_dev, vid, start, status,
-                                                                                  ticket, (uint32_t)nb, epoch,
+                                                                                  ctl, (uint32_t)nb,
                                                                                   total_dev));
   return RVP_OK;
 }
@@ -616,11 +634,11 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict
                                                               size_t n, int pass, int passes,
                                                               const uint32_t *__restrict__ ghist,
                                                               uint64_t *__restrict__ status,
-                                                              uint32_t *__restrict__ ticket, uint32_t nb,
-                                                              uint32_t epoch, const uint32_t *__restrict__ n_dev,
+                                                              uint64_t *__restrict__ ctl, uint32_t nb,
+                                                              const uint32_t *__restrict__ n_dev,
                                                               const int *__restrict__ bits_dev) {
   constexpr int TILE = PRIM_THREADS * ITEMS;
-  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_tile, s_epoch;
   __shared__ uint32_t wcnt[PRIM_THREADS / WAVE][256];  // per-wave digit counters, then per-wave offsets
   __shared__ uint32_t lstart[256];                     // first slot of digit d in the staged tile
   __shared__ uint32_t gbase[256];                      // global position of staged slot 0 of digit d's run
@@ -641,14 +659,15 @@ __global__ __launch_bounds__(PRIM_THREADS) void radix_pass_k(const K *__restrict
     }
     return;
   }
-  // only the tiles the device-side count needs take tickets (the ticket counter wraps after exactly that
+  // only the tiles the device-side count needs take tickets (the ticket returns to zero after exactly that
   // many); the launch's other blocks leave at once instead of queueing on the counter
   const uint32_t nb_act = (uint32_t)((n + TILE - 1) / TILE);
   if (blockIdx.x >= nb_act) return;
   const uint32_t gcount = ghist[pass * 256 + t];  // issued before the ticket round trip
 #pragma unroll
   for (int q = 0; q < 4; ++q) wcnt[w][l + 64 * q] = 0;
-  const uint32_t tile = take_tile(ticket, nb_act, &s_tile);
+  const uint32_t tile = take_tile(ctl, nb_act, &s_tile, &s_epoch);
+  const uint32_t epoch = s_epoch;
   if (tile >= nb_act) return;
   const size_t base = (size_t)tile * TILE;
   const int shift = 8 * pass;
@@ -789,16 +808,16 @@ static int radix_sort_impl(K **keys, uint32_t **vals, K **kt, uint32_t **vt, siz
                                         part.as<uint32_t>(), hb, n, passes, gh.as<uint32_t>(), n_dev, bits_dev));
   for (int p = 0; p < passes; ++p) {
     uint64_t *status = nullptr;
-    uint32_t *ticket = nullptr, epoch = 0;
-    RVP_TRY(scan_state(s, (size_t)nb * 256, &status, &ticket, &epoch));
+    uint64_t *ctl = nullptr;
+    RVP_TRY(scan_state(s, (size_t)nb * 256, &status, &ctl));
     if (large_tiles)
       RVP_LAUNCH("radix_pass_k", s, radix_pass_k<K, 2 * RS_ITEMS><<<nb, PRIM_THREADS, 0, s>>>(
-                                        *keys, *vals, *kt, *vt, n, p, passes, gh.as<uint32_t>(), status, ticket, nb,
-                                        epoch, n_dev, bits_dev));
+                                        *keys, *vals, *kt, *vt, n, p, passes, gh.as<uint32_t>(), status, ctl, nb,
+                                        n_dev, bits_dev));
     else
       RVP_LAUNCH("radix_pass_k", s, radix_pass_k<K, RS_ITEMS><<<nb, PRIM_THREADS, 0, s>>>(
-                                        *keys, *vals, *kt, *vt, n, p, passes, gh.as<uint32_t>(), status, ticket, nb,
-                                        epoch, n_dev, bits_dev));
+                                        *keys, *vals, *kt, *vt, n, p, passes, gh.as<uint32_t>(), status, ctl, nb,
+                                        n_dev, bits_dev));
     std::swap(*keys, *kt);
     std::swap(*vals, *vt);
   }
@@ -837,23 +856,23 @@ __global__ void scatter_aos_k(const float *__restrict__ x, const float *__restri
 int compact_flagged(const float *x, const float *y, const float *z, uint32_t *pos, size_t n, const uint32_t *n_dev,
                     float *out_aos, uint32_t *count_dev, hipStream_t s) {
   if (n == 0) {
-    RVP_HIP(hipMemsetAsync(count_dev, 0, sizeof(uint32_t), s));
+    RVP_TRY(zero_words(count_dev, 1, s));
     return RVP_OK;
   }
   RVP_CHECK(n < 0xffffffffull, RVP_E_UNSUPPORTED, "compaction: more than 2^32-1 points");
   const size_t nb = ceil_div(n, SCAN_TILE);
   uint64_t *status = nullptr;
-  uint32_t *ticket = nullptr, epoch = 0;
-  RVP_TRY(scan_state(s, nb, &status, &ticket, &epoch));
+  uint64_t *ctl = nullptr;
+  RVP_TRY(scan_state(s, nb, &status, &ctl));
   RVP_LAUNCH("compact_flagged_k", s, compact_flagged_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(
-                                         x, y, z, pos, n, n_dev, out_aos, status, ticket, epoch, count_dev));
+                                         x, y, z, pos, n, n_dev, out_aos, status, ctl, count_dev));
   return RVP_OK;
 }
 
 int compact_xyz_aos_flagged(const float *x, const float *y, const float *z, const uint8_t *flags, size_t n,
                             float *out_aos, uint32_t *pos, uint32_t *count_dev, hipStream_t s) {
   if (n == 0) {
-    RVP_HIP(hipMemsetAsync(count_dev, 0, sizeof(uint32_t), s));
+    RVP_TRY(zero_words(count_dev, 1, s));
     return RVP_OK;
   }
   RVP_TRY(exclusive_scan_u32(pos, pos, n, count_dev, s));
@@ -865,7 +884,7 @@ int compact_xyz_aos_flagged(const float *x, const float *y, const float *z, cons
 int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_t *flags, uint8_t want, size_t n,
                     float *out_aos, uint32_t *count_dev, hipStream_t s) {
   if (n == 0) {
-    RVP_HIP(hipMemsetAsync(count_dev, 0, sizeof(uint32_t), s));
+    RVP_TRY(zero_words(count_dev, 1, s));
     return RVP_OK;
   }
   Tmp f;
@@ -884,46 +903,41 @@ int compact_xyz_aos(const float *x, const float *y, const float *z, const uint8_
 
 // ------------------------------------------------------------------------------------------------
 // Allocation-free plane split: the X1 split of rvp_plane_extract with its look-back state (status words,
-// ticket counter) and its two counts in a fixed per-device array instead of stream-ordered allocations, for
+// control word) and its two counts in a fixed per-device array instead of stream-ordered allocations, for
 // callers that cannot allocate device memory (code under graph capture, serving loops).
 // ------------------------------------------------------------------------------------------------
 namespace rvp {
 
 constexpr size_t NOALLOC_TILES = size_t(1) << 16;                  // status words: one per tile
 constexpr size_t NOALLOC_MAX_POINTS = NOALLOC_TILES * SCAN_TILE;   // 134,217,728 points
-// [0, NOALLOC_TILES): status words; NOALLOC_TILES: ticket counter (low half); NOALLOC_TILES + 1: the counts
+// [0, NOALLOC_TILES): status words; NOALLOC_TILES: the control word (generation, ticket); NOALLOC_TILES + 1: the
+// two counts
 static __device__ __align__(256) uint64_t scratch[NOALLOC_TILES + 2];
 
-struct NoAllocDevice {
-  uint64_t *ptr = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
-  uint32_t epoch = 0;
-};
 static std::mutex g_noalloc_mu;
-static NoAllocDevice g_noalloc[64];
+static uint64_t *g_noalloc[64];  // each device's copy of `scratch` (hipGetSymbolAddress)
 
-__global__ void noalloc_ticket_clear_k(uint64_t *sc) {
-  if (threadIdx.x == 0) reinterpret_cast<uint32_t *>(sc + NOALLOC_TILES)[0] = 0u;
+// The control word's ticket must be zero when the split starts; a complete launch leaves it there, a failed one
+// may not.  Moving to the next generation with a zero ticket also gives the call an epoch its predecessors did
+// not use, so their status words are never taken for this call's.
+__global__ void noalloc_ctl_reset_k(uint64_t *sc) {
+  if (threadIdx.x == 0) sc[NOALLOC_TILES] = ((sc[NOALLOC_TILES] >> 32) + 1ull) << 32;
 }
 
-// this device's scratch address (looked up once per device, with the device current) and the call's epoch
-static int noalloc_scratch(uint64_t **ptr, uint32_t *epoch) {
+// this device's scratch address, looked up once per device with the device current
+static int noalloc_scratch(uint64_t **ptr) {
   int dev = 0;
   RVP_HIP(hipGetDevice(&dev));
   RVP_CHECK(dev >= 0 && dev < 64, RVP_E_UNSUPPORTED, "device index");
   std::lock_guard<std::mutex> lk(g_noalloc_mu);
-  NoAllocDevice &d = g_noalloc[dev];
-  if (!d.ptr) {
+  if (!g_noalloc[dev]) {
     void *p = nullptr;
     const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
     RVP_CHECK(e == hipSuccess && p, RVP_E_DEVICE,
               std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
-    d.ptr = static_cast<uint64_t *>(p);
+    g_noalloc[dev] = static_cast<uint64_t *>(p);
   }
-  // the array starts zeroed (epoch 0 = cleared); status words carry the call's epoch
-  d.epoch = (d.epoch + 1) & 0x3fffffffu;
-  if (d.epoch == 0) d.epoch = 1;
-  *ptr = d.ptr;
-  *epoch = d.epoch;
+  *ptr = g_noalloc[dev];
   return RVP_OK;
 }
 
@@ -941,17 +955,14 @@ extern "C" int rvp_plane_extract_noalloc(const float *x, const float *y, const f
             "rvp_plane_extract_noalloc: more points than its fixed scratch holds (134,217,728)");
   hipStream_t s = S(st);
   uint64_t *sc = nullptr;
-  uint32_t epoch = 0;
-  RVP_TRY(noalloc_scratch(&sc, &epoch));
+  RVP_TRY(noalloc_scratch(&sc));
   const size_t nb = ceil_div(n, SCAN_TILE);
-  uint32_t *ticket = reinterpret_cast<uint32_t *>(sc + NOALLOC_TILES);
   uint32_t *counts = reinterpret_cast<uint32_t *>(sc + NOALLOC_TILES + 1);
-  // the ticket must start at zero (a launch leaves it there, but a failed one may not): cleared in stream order
-  noalloc_ticket_clear_k<<<1, WAVE, 0, s>>>(sc);
+  noalloc_ctl_reset_k<<<1, WAVE, 0, s>>>(sc);
   RVP_HIP(hipGetLastError());
   plane_split_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(x, y, z, n, model_host[0], model_host[1], model_host[2],
-                                                        model_host[3], dist_thresh, sc, ticket, (uint32_t)nb, epoch,
-                                                        inliers_aos, outliers_aos, counts);
+                                                        model_host[3], dist_thresh, sc, sc + NOALLOC_TILES,
+                                                        (uint32_t)nb, inliers_aos, outliers_aos, counts);
   RVP_HIP(hipGetLastError());
   uint32_t h[2] = {0, 0};
   RVP_HIP(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, s));

@@ -9,6 +9,9 @@ constexpr int PRIM_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;                       // items per thread per scan tile
 constexpr int SCAN_TILE = PRIM_THREADS * SCAN_ITEMS; // 2048
 
+// zeroes `words` (<= 256) 32-bit words with a kernel in stream order (capture-safe, unlike hipMemsetAsync)
+int zero_words(void *p, uint32_t words, hipStream_t s);
+
 // Exclusive scan out[i] = sum(in[0..i)), i < n; if total_dev != nullptr it receives sum(in[0..n)).
 // In/Out may alias.  Out is uint32_t or uint64_t (int64_t via reinterpretation).
 int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev, hipStream_t s);

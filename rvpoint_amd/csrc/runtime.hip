@@ -65,19 +65,58 @@ static void cache_release_all(int dev, bool only_stream, hipStream_t s) {
   for (void *p : drop) (void)hipFree(p);
 }
 
+// ---- capture arenas ---------------------------------------------------------------------------------
+// While a stream is being captured into a graph, its temporaries come from an arena that the graph keeps: the
+// graph's kernels hold their addresses for every replay, so those blocks must not go back to the shared cache
+// (where eager work could take them) until the graph is destroyed.  Inside the capture a freed block returns to
+// the arena and may serve a later stage of the same capture (the captured stream is one chain, so stages never
+// overlap).  An arena takes blocks from the stream's cache first, so a capture preceded by an eager warm-up
+// step allocates nothing.
+struct CaptureArena {
+  int dev = 0;
+  hipStream_t s = nullptr;
+  std::map<size_t, std::vector<void *>> free;   // by size class
+  std::vector<std::pair<void *, size_t>> owned;  // every block, with its class
+};
+static std::map<std::pair<int, hipStream_t>, CaptureArena *> g_arenas;  // active captures (under g_cache_mu)
+
 void *cache_alloc(size_t bytes, hipStream_t s, size_t *cap) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   const size_t cls = size_class(bytes);
+  CaptureArena *arena = nullptr;
   {
     std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto ai = g_arenas.find({dev, s});
+    if (ai != g_arenas.end()) {
+      arena = ai->second;
+      auto &fl = arena->free[cls];
+      if (!fl.empty()) {
+        void *p = fl.back();
+        fl.pop_back();
+        *cap = cls;
+        return p;
+      }
+    }
     auto it = g_cache.find(CacheKey{dev, s, cls});
     if (it != g_cache.end() && !it->second.empty()) {
       void *p = it->second.back();
       it->second.pop_back();
       *cap = cls;
+      if (arena) arena->owned.emplace_back(p, cls);
       return p;
     }
+  }
+  if (arena) {
+    void *p = nullptr;
+    if (hipMalloc(&p, cls) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    arena->owned.emplace_back(p, cls);
+    *cap = cls;
+    return p;
   }
   void *p = nullptr;
   if (hipMalloc(&p, cls) != hipSuccess) {
@@ -96,7 +135,53 @@ void cache_free(void *p, size_t cap, hipStream_t s) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return;
   std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto ai = g_arenas.find({dev, s});
+  if (ai != g_arenas.end()) {
+    CaptureArena *a = ai->second;
+    bool mine = false;
+    for (auto &b : a->owned) mine = mine || b.first == p;
+    if (!mine) a->owned.emplace_back(p, cap);  // allocated before the capture began: the graph keeps it too
+    a->free[cap].push_back(p);
+    return;
+  }
   g_cache[CacheKey{dev, s, cap}].push_back(p);
+}
+
+static thread_local bool g_capturing = false;  // this thread is capturing a stream (no profiling events)
+
+int arena_begin(hipStream_t s) {
+  int dev = 0;
+  RVP_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  RVP_CHECK(!g_arenas.count({dev, s}), RVP_E_STATE, "this stream is already being captured");
+  CaptureArena *a = new CaptureArena();
+  a->dev = dev;
+  a->s = s;
+  g_arenas[{dev, s}] = a;
+  g_capturing = true;
+  return RVP_OK;
+}
+
+CaptureArena *arena_end(hipStream_t s) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  g_capturing = false;
+  auto ai = g_arenas.find({dev, s});
+  if (ai == g_arenas.end()) return nullptr;
+  CaptureArena *a = ai->second;
+  g_arenas.erase(ai);
+  return a;
+}
+
+// the graph is gone (its stream drained): the arena's blocks go back to that stream's cache
+void arena_release(CaptureArena *a) {
+  if (!a) return;
+  {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (auto &b : a->owned) g_cache[CacheKey{a->dev, a->s, b.second}].push_back(b.first);
+  }
+  delete a;
 }
 
 // ---- kernel timer ---------------------------------------------------------------------------------
@@ -110,7 +195,7 @@ static std::vector<hipEvent_t> g_ev_pool;  // events reused across collects
 static std::atomic<bool> g_prof_on{false};
 static std::string g_prof_filter;          // empty: every kernel
 
-bool prof_enabled() { return g_prof_on.load(std::memory_order_relaxed); }
+bool prof_enabled() { return !g_capturing && g_prof_on.load(std::memory_order_relaxed); }
 
 bool prof_want(const char *name) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
@@ -523,5 +608,74 @@ extern "C" int rvp_aos_to_soa_bounded(const float *aos, size_t n_bound, const ui
   // a quarter million threads: one pass for a frame's kept points, strided beyond
   const unsigned nb = (unsigned)std::min<size_t>(1024, ceil_div(n_bound, 256));
   RVP_LAUNCH("aos_to_soa_bounded", S(st), aos_to_soa_bounded_k<<<nb, 256, 0, S(st)>>>(aos, n_bound, n_dev, x, y, z));
+  return RVP_OK;
+}
+
+// ---- graphs ------------------------------------------------------------------------------------------
+struct rvp_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  rvp::CaptureArena *arena = nullptr;
+  hipStream_t s = nullptr;
+};
+
+extern "C" int rvp_graph_capture_begin(rvp_stream_t st) {
+  using namespace rvp;
+  RVP_CHECK(st, RVP_E_ARG, "graph capture needs a created stream (the null stream cannot be captured)");
+  hipStream_t s = S(st);
+  RVP_TRY(arena_begin(s));
+  const hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    arena_release(arena_end(s));
+    return fail(RVP_E_DEVICE, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  }
+  return RVP_OK;
+}
+
+extern "C" int rvp_graph_capture_end(rvp_stream_t st, rvp_graph **out) {
+  using namespace rvp;
+  RVP_CHECK(st && out, RVP_E_ARG, "null argument");
+  *out = nullptr;
+  hipStream_t s = S(st);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  CaptureArena *a = arena_end(s);
+  if (e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    arena_release(a);
+    return fail(RVP_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e2 != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    arena_release(a);
+    return fail(RVP_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+  }
+  rvp_graph *r = new rvp_graph();
+  r->graph = g;
+  r->exec = x;
+  r->arena = a;
+  r->s = s;
+  *out = r;
+  return RVP_OK;
+}
+
+extern "C" int rvp_graph_launch(rvp_graph *g, rvp_stream_t st) {
+  RVP_CHECK(g && g->exec, RVP_E_ARG, "null graph");
+  RVP_CHECK(rvp::S(st) == g->s, RVP_E_ARG, "a graph replays on the stream it was captured on");
+  RVP_HIP(hipGraphLaunch(g->exec, rvp::S(st)));
+  return RVP_OK;
+}
+
+extern "C" int rvp_graph_destroy(rvp_graph *g) {
+  if (!g) return RVP_OK;
+  // the graph's last replay may still run: its blocks go back to the cache only after the stream drains
+  const hipError_t e = hipStreamSynchronize(g->s);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  rvp::arena_release(g->arena);
+  delete g;
+  RVP_HIP(e);
   return RVP_OK;
 }

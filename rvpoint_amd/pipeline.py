@@ -35,17 +35,29 @@ class FrameStages:
 class DeviceFramePipeline:
     """One frame resident in HBM through the four stages, stage outputs kept on the device.
 
-    keep_means: also keep SOR's per-point mean k-NN distances (voxel order) for inspection."""
+    keep_means: also keep SOR's per-point mean k-NN distances (voxel order) for inspection.
+    graph: replay voxel -> SOR -> normals as one captured HIP graph (the kernels only; RANSAC's adaptive loop
+    stays on the host); needs a created stream, which the pipeline makes when none is given.  The first step
+    runs eagerly and captures the graph; later steps replay it.  n_max: device capacity in points (default the
+    frame's); load() puts a new frame of at most n_max points into the same buffers."""
 
-    def __init__(self, frame_xyz, stages: FrameStages | None = None, stream=None, keep_means=False):
+    def __init__(self, frame_xyz, stages: FrameStages | None = None, stream=None, keep_means=False, graph=False,
+                 n_max=None):
+        from .device import Stream
         self.st = stages or FrameStages()
+        self._own_stream = None
+        if graph and stream is None:
+            self._own_stream = Stream()
+            stream = self._own_stream.handle
         self.s = stream
-        self.n = n = len(frame_xyz)
+        self.use_graph = graph
+        self.graph = None
+        self.n_max = n = int(n_max or len(frame_xyz))
         D = DeviceArray
-        self.frame = DeviceCloud(frame_xyz)
-        self.off = D.from_numpy(np.array([0, n], np.uint32))
-        self.leaf = D.from_numpy(np.array([self.st.leaf], np.float32))
-        self.radius = D.from_numpy(np.array([self.st.radius], np.float32))
+        self.frame = DeviceCloud(np.zeros((n, 3), np.float32), stream)
+        self.off = D(2, np.uint32)
+        self.leaf = D.from_numpy(np.array([self.st.leaf], np.float32), stream)
+        self.radius = D.from_numpy(np.array([self.st.radius], np.float32), stream)
         self.vox, self.vox_off, self.status = D((n, 3)), D(2, np.uint32), D(1, np.uint32)
         self.vx, self.vy, self.vz = D(n), D(n), D(n)
         self.sor, self.sor_off = D((n, 3)), D(2, np.uint32)
@@ -55,9 +67,23 @@ class DeviceFramePipeline:
         self.model = np.zeros(4, np.float32)
         self.inl = np.zeros(1, np.int32)
         self.its = np.zeros(1, np.int32)
+        self.load(frame_xyz)
 
-    def step(self):
-        st, s, n = self.st, self.s, self.n
+    def load(self, frame_xyz):
+        """A new frame (at most n_max points) into the pipeline's input buffers; a captured graph reads it."""
+        xyz = np.ascontiguousarray(frame_xyz, np.float32).reshape(-1, 3)
+        if len(xyz) > self.n_max:
+            raise ValueError(f"frame of {len(xyz)} points exceeds the pipeline's {self.n_max}")
+        self.n = len(xyz)
+        for dst, i in ((self.frame.x, 0), (self.frame.y, 1), (self.frame.z, 2)):
+            col = np.ascontiguousarray(xyz[:, i])
+            if len(col):
+                call("rvp_memcpy_h2d", dst.ptr, col.ctypes.data_as(C.c_void_p), col.nbytes, self.s)
+        off = np.array([0, self.n], np.uint32)
+        call("rvp_memcpy_h2d", self.off.ptr, off.ctypes.data_as(C.c_void_p), off.nbytes, self.s)
+
+    def _device_stages(self):
+        st, s, n = self.st, self.s, self.n_max
         call("rvp_batch_voxel", *self.frame.ptrs(), self.off.ptr, 1, n, self.leaf.ptr, self.vox.ptr,
              self.vox_off.ptr, self.status.ptr, s)
         # what the stage kept, counted on the device (the second entry of its output offsets)
@@ -65,27 +91,69 @@ class DeviceFramePipeline:
         call("rvp_batch_sor_exact", self.vx.ptr, self.vy.ptr, self.vz.ptr, self.vox_off.ptr, 1, n, st.sor_k,
              st.sor_alpha, self.sor.ptr, self.sor_off.ptr, None, self.means.ptr if self.means else None, s)
         call("rvp_aos_to_soa_bounded", self.sor.ptr, n, self.sor_off.ptr + 4, self.sx.ptr, self.sy.ptr, self.sz.ptr, s)
-        kept = (self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, 1, n)
-        call("rvp_batch_normals_radius", *kept, self.radius.ptr, None, st.max_nn, *st.vp, self.nx.ptr,
-             self.ny.ptr, self.nz.ptr, s)
+        call("rvp_batch_normals_radius", self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, 1, n,
+             self.radius.ptr, None, st.max_nn, *st.vp, self.nx.ptr, self.ny.ptr, self.nz.ptr, s)
+
+    def _ransac(self):
+        st, s = self.st, self.s
         P = C.POINTER
-        call("rvp_batch_ransac_plane", *kept, st.thr, st.iters, st.collinear, st.probability,
-             self.model.ctypes.data_as(P(C.c_float)), self.inl.ctypes.data_as(P(C.c_int)),
-             self.its.ctypes.data_as(P(C.c_int)), s)
+        call("rvp_batch_ransac_plane", self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, 1, self.n_max,
+             st.thr, st.iters, st.collinear, st.probability, self.model.ctypes.data_as(P(C.c_float)),
+             self.inl.ctypes.data_as(P(C.c_int)), self.its.ctypes.data_as(P(C.c_int)), s)
+
+    def _capture(self):
+        g = C.c_void_p()
+        call("rvp_graph_capture_begin", self.s)
+        try:
+            self._device_stages()
+        finally:
+            call("rvp_graph_capture_end", self.s, C.byref(g))
+        self.graph = g.value
+
+    def step(self):
+        if self.use_graph and self.graph is None:
+            self._device_stages()  # the eager warm-up: first-use allocations happen outside the capture
+            self._ransac()
+            self._capture()
+            call("rvp_stream_sync", self.s)
+        if self.graph is not None:
+            call("rvp_graph_launch", self.graph, self.s)
+        else:
+            self._device_stages()
+        self._ransac()
+
+    def sync(self):
+        call("rvp_stream_sync", self.s)
 
     def info(self):
-        return {"n_in": self.n, "n_voxel": int(self.vox_off.numpy()[1]), "n_sor": int(self.sor_off.numpy()[1]),
-                "inliers": int(self.inl[0]), "ransac_iters": int(self.its[0]),
-                "model": [float(v) for v in self.model]}
+        return {"n_in": self.n, "n_voxel": int(self.vox_off.numpy(stream=self.s)[1]),
+                "n_sor": int(self.sor_off.numpy(stream=self.s)[1]), "inliers": int(self.inl[0]),
+                "ransac_iters": int(self.its[0]), "model": [float(v) for v in self.model]}
 
     def results(self):
         """Host copies of every stage's output: voxel centroids, SOR means (when kept) and kept points, normals
         of the kept points, the plane."""
-        nv = int(self.vox_off.numpy()[1])
-        ns = int(self.sor_off.numpy()[1])
-        out = {"voxels": self.vox.numpy()[:nv], "kept": self.sor.numpy()[:ns],
-               "normals": np.stack([self.nx.numpy()[:ns], self.ny.numpy()[:ns], self.nz.numpy()[:ns]], 1),
+        s = self.s
+        nv = int(self.vox_off.numpy(stream=s)[1])
+        ns = int(self.sor_off.numpy(stream=s)[1])
+        out = {"voxels": self.vox.numpy(stream=s)[:nv], "kept": self.sor.numpy(stream=s)[:ns],
+               "normals": np.stack([a.numpy(stream=s)[:ns] for a in (self.nx, self.ny, self.nz)], 1),
                "model": self.model.copy(), "inliers": int(self.inl[0]), "iterations": int(self.its[0])}
         if self.means is not None:
-            out["means"] = self.means.numpy()[:nv]
+            out["means"] = self.means.numpy(stream=s)[:nv]
         return out
+
+    def close(self):
+        if self.graph is not None:
+            from ._lib import lib
+            lib().rvp_graph_destroy(self.graph)
+            self.graph = None
+        if self._own_stream is not None:
+            self._own_stream.close()
+            self._own_stream = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,53 @@
+"""The device frame pipeline replayed as a captured HIP graph: every replay equals the eager pipeline's results
+(voxel centroids, SOR means and kept points, normals, RANSAC plane), with eager work and synchronisations of the
+program between replays, and a new frame loaded into the same buffers between replays is processed from its own
+contents (the graph reads device-side counts, and its look-back epochs advance on the device)."""
+import numpy as np
+import pytest
+
+from rvpoint_amd import scenes
+from rvpoint_amd.pipeline import DeviceFramePipeline
+
+pytestmark = pytest.mark.gpu
+
+R = pytest.importorskip("rvpoint_amd")
+
+
+def same(a, b):
+    return all(np.array_equal(a[k], b[k]) for k in ("voxels", "means", "kept", "normals", "model")) and \
+        a["inliers"] == b["inliers"] and a["iterations"] == b["iterations"]
+
+
+def eager(frame):
+    p = DeviceFramePipeline(frame, keep_means=True)
+    p.step()
+    return p.results()
+
+
+def test_frame_graph_replays_equal_eager():
+    f1 = scenes.lidar_frame(rings=24, azimuths=6000, seed=11, nbox=14, max_range=40.0)
+    f2 = scenes.lidar_frame(rings=12, azimuths=5000, seed=12, nbox=10, max_range=40.0)
+    assert len(f2) < len(f1)
+    e1, e2 = eager(f1), eager(f2)
+    g = DeviceFramePipeline(f1, keep_means=True, graph=True)
+    try:
+        for rep in range(3):
+            g.step()
+            assert g.graph is not None
+            assert same(g.results(), e1), rep
+            # the program's own eager work between replays, on the null stream, then a full synchronisation
+            R.sor_sc(f2[:3000], 20, 1.0)
+            R.synchronize()
+        g.load(f2)
+        g.step()
+        assert same(g.results(), e2)
+        g.load(f1)
+        g.step()
+        g.step()
+        assert same(g.results(), e1)
+    finally:
+        g.close()
+    import ctypes as C
+    errs = C.c_ulonglong()
+    R._lib.call("rvp_scan_wait_errors", C.byref(errs))
+    assert errs.value == 0
