@@ -72,6 +72,8 @@ def parse():
                     help="one device-to-device copy of this many MiB before warmup (HBM counter calibration)")
     ap.add_argument("--only", choices=["frame", "batched", "indoor", "serving"], default=None,
                     help="run one leg alone (profiling runs); the JSON line then carries that leg only")
+    ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
+                    help="set a library tuning knob before running (repeatable; rvp_tune_set names)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the all-core CPU leg (0: all this "
                     "process may use)")
     return ap.parse_args()
@@ -118,51 +120,11 @@ class FramePipeline:
                 "ransac_iters": its.value, "model": [float(v) for v in model]}
 
 
-class BatchPipeline:
-    """256 plane+sphere clouds back to back in HBM, one launch per stage and no host sync between stages:
-    exact-kNN SOR k=32 -> radius lists r=0.02 s_i capped at 64 on the kept points -> kNN normals k=24."""
-
-    K_SOR, K_NRM, CAP = 32, 24, 64
-
-    def __init__(self, clouds, scales, stream=None):
-        import rvpoint_amd as R
-        from rvpoint_amd._lib import call
-        self.call, self.s = call, stream
-        self.sizes = np.array([len(c) for c in clouds], np.int64)
-        self.ncl = len(clouds)
-        self.off_host = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.uint32)
-        self.n = int(self.off_host[-1])
-        n, D = self.n, R.DeviceArray
-        self.cloud = R.DeviceCloud(np.concatenate(clouds))
-        self.off = D.from_numpy(self.off_host)
-        self.radius_host = (0.02 * np.asarray(scales)).astype(np.float32)
-        self.radius = D.from_numpy(self.radius_host)
-        self.sor, self.sor_off = D((n, 3)), D(self.ncl + 1, np.uint32)
-        self.sx, self.sy, self.sz = D(n), D(n), D(n)
-        self.idx, self.d2, self.cnt = D((n, self.CAP), np.int32), D((n, self.CAP)), D(n, np.int32)
-        self.nx, self.ny, self.nz = D(n), D(n), D(n)
-
-    def step(self):
-        call, s, n = self.call, self.s, self.n
-        call("rvp_batch_sor_exact", *self.cloud.ptrs(), self.off.ptr, self.ncl, n, self.K_SOR, 1.0, self.sor.ptr,
-             self.sor_off.ptr, None, None, s)
-        # the kept total is the last entry of the batch's output offsets, read on the device
-        call("rvp_aos_to_soa_bounded", self.sor.ptr, n, self.sor_off.ptr + 4 * self.ncl, self.sx.ptr, self.sy.ptr,
-             self.sz.ptr, s)
-        kept = (self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, self.ncl, n)
-        call("rvp_batch_radius_search", *kept, self.radius.ptr, None, self.CAP, self.idx.ptr, self.d2.ptr,
-             self.cnt.ptr, s)
-        call("rvp_batch_normals_knn", *kept, self.K_NRM, 0.0, 0.0, 0.0, self.nx.ptr, self.ny.ptr, self.nz.ptr, s)
-
-    def kept_total(self):
-        return int(self.sor_off.numpy()[-1])
-
-
 class PerCloudPipeline:
     """The same three stages one cloud per launch (the plain GPU path): SOR, host read of the kept count,
     hash build and radius lists, kNN normals, cloud after cloud on one stream."""
 
-    def __init__(self, bp: BatchPipeline):
+    def __init__(self, bp):
         from rvpoint_amd._lib import call
         self.bp, self.call = bp, call
         h = C.c_void_p()
@@ -189,6 +151,7 @@ class PerCloudPipeline:
 def batch_leg(R, args):
     """Batched (one launch per stage) vs per-cloud on the 256-cloud workload: Mpts/s of input points."""
     from rvpoint_amd import scenes
+    from rvpoint_amd.pipeline import DeviceBatchPipeline as BatchPipeline
     print("bench: generating 256-cloud batch", file=sys.stderr, flush=True)
     clouds, scales = scenes.batch_clouds(with_scales=True)
     bp = BatchPipeline(clouds, scales)
@@ -653,6 +616,16 @@ def main():
     if not R.gpu_available():
         print("no GPU visible to librvp_hip.so", file=sys.stderr)
         sys.exit(1)
+    from rvpoint_amd import autotune
+    knobs_used = {}
+
+    def tune_for(cls):
+        """The autotuner's knobs for this workload class (tuning.json), then any --tune overrides."""
+        autotune.apply(cls)
+        for kv in args.tune:
+            name, value = kv.split("=", 1)
+            _lib.tune(name, float(value))
+        knobs_used[cls] = {name: _lib.tune(name) for name in autotune.DEFAULTS}
     if args.copy_mib > 0:
         nb = args.copy_mib << 20
         a, b = R.DeviceArray(nb // 4), R.DeviceArray(nb // 4)
@@ -661,20 +634,25 @@ def main():
         del a, b
     line = {"metric": "Mpts/s hash-grid radius/kNN+SOR+normals; batched vs per-cloud; p99 ms/frame", "value": None,
             "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "tuning": knobs_used}
     legs = [args.only] if args.only else ["frame", "batched", "indoor", "serving"]
     frame = pipe = info = None
     if "frame" in legs:
         print("bench: generating LiDAR frame (64 rings x 18,750 azimuths)", file=sys.stderr, flush=True)
         frame = scenes.lidar_frame()
+        tune_for("lidar_frame")
         part, pipe, info = frame_leg(R, args, frame)
         line.update(part)
         line["hbm_copy_gbs"] = copy_bandwidth(R)
     if "batched" in legs:
+        tune_for("batch")
         line["batched"] = None if args.no_batch else batch_leg(R, args)
     if "indoor" in legs:
+        tune_for("indoor")
         line["indoor"] = None if args.no_indoor else indoor_leg(R, args)
     if "serving" in legs:
+        tune_for("lidar_frame")
         line["serving"] = None if args.no_serve else serving_leg(R, args)
     if "frame" in legs and not args.no_cpu_baseline:
         print("bench: frame precision against the FP64 oracle", file=sys.stderr, flush=True)

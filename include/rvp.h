@@ -94,6 +94,9 @@ int rvp_profile_collect(char *buf, size_t buflen, size_t *needed);
  *                          fewer below; SOR means 2 up to a quarter million, at most 16; [0, 64])
  *   "seg_scan_table_min"   batches of at least this many points (their bound) build the cell table from run
  *                          lengths and one scan; smaller ones fill the gaps between runs (default 4194304)
+ *   "work_steal"           how the batched k-NN SOR means hand out runs of queries: 0 strided over the launch's
+ *                          waves; 1 persistent waves taking runs from per-XCD ranges by atomic counters and
+ *                          stealing when theirs is drained; 2 per-XCD ranges strided without atomics
  * set() returns RVP_E_ARG for an unknown name or a value out of range. */
 int rvp_tune_set(const char *name, double value);
 int rvp_tune_get(const char *name, double *value);

@@ -566,36 +566,58 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
 // fixes it); within a cloud, the previous query's k-th distance bounds the next one's (knn_bound_from).
 // Only that k-th d2 is carried between queries (the previous position is re-read), which keeps the kernel at
 // 79 VGPRs: six waves per SIMD and no scratch.
-__global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, int forced, float *__restrict__ means) {
+__device__ __forceinline__ void sor_run(const SegArrays &A, int k, uint32_t run, uint32_t pw, uint32_t total,
+                                        WaveLds &L, float *__restrict__ means) {
+  const uint32_t q1 = min(run * pw + pw, total);
+  float pd2 = -1.0f;  // k-th d2 of query q - 1 when it is in the run (< 0: none)
+  for (uint32_t q = run * pw; q < q1; ++q) {
+    const int c = cloud_of(A.off, A.ncl, q);
+    const GridView g = seg_view(A, c);
+    const float x = uniform(g.xs[q]), y = uniform(g.ys[q]), z = uniform(g.zs[q]);
+    TopKT<uint32_t> t;
+    t.init(k + 1);
+    // the previous query of the run is q - 1, re-read from the cache; it bounds q only within one cloud
+    knn_search<false>(g, x, y, z, t, L,
+                      (pd2 >= 0.0f && q > A.off[c])
+                          ? knn_bound_from(x, y, z, uniform(g.xs[q - 1]), uniform(g.ys[q - 1]), uniform(g.zs[q - 1]),
+                                           pd2)
+                          : __builtin_inff());
+    const int l = lane();
+    const int m = min(t.cnt, k + 1);
+    const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
+    float s = 0.0f;
+    for (int j = 1; j <= k; ++j) s = s + readlanef(sq, j);
+    if (l == 0) means[g.perm[q]] = s / __int2float_rn(k);
+    pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
+  }
+}
+
+// queue != nullptr: persistent waves take runs from per-XCD ranges and steal (XcdWork), `chunk` runs at a time;
+// else chunk == 0: runs strided within per-XCD ranges (XcdStride); else runs strided over the launch's waves.
+__global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, int forced, float *__restrict__ means,
+                                                      uint32_t *__restrict__ queue, uint32_t chunk) {
   __shared__ WaveLds lds[QW];
   const int w = threadIdx.x >> 6;
   const uint32_t total = A.off[A.ncl];
   const uint32_t pw = (uint32_t)knn_run_length(total, forced);
   const uint32_t nruns = (total + pw - 1) / pw;
-  for (uint32_t run = wave_id(); run < nruns; run += wave_count()) {
-    const uint32_t q1 = min(run * pw + pw, total);
-    float pd2 = -1.0f;  // k-th d2 of query q - 1 when it is in the run (< 0: none)
-    for (uint32_t q = run * pw; q < q1; ++q) {
-      const int c = cloud_of(A.off, A.ncl, q);
-      const GridView g = seg_view(A, c);
-      const float x = uniform(g.xs[q]), y = uniform(g.ys[q]), z = uniform(g.zs[q]);
-      TopKT<uint32_t> t;
-      t.init(k + 1);
-      // the previous query of the run is q - 1, re-read from the cache; it bounds q only within one cloud
-      knn_search<false>(g, x, y, z, t, lds[w],
-                        (pd2 >= 0.0f && q > A.off[c])
-                            ? knn_bound_from(x, y, z, uniform(g.xs[q - 1]), uniform(g.ys[q - 1]), uniform(g.zs[q - 1]),
-                                             pd2)
-                            : __builtin_inff());
-      const int l = lane();
-      const int m = min(t.cnt, k + 1);
-      const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
-      float s = 0.0f;
-      for (int j = 1; j <= k; ++j) s = s + readlanef(sq, j);
-      if (l == 0) means[g.perm[q]] = s / __int2float_rn(k);
-      pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
-    }
+  if (queue) {
+    XcdWork work;
+    work.init(queue, nruns, chunk);
+    uint32_t r0 = 0, r1 = 0;
+    while (work.next(r0, r1))
+      for (uint32_t run = r0; run < r1; ++run) sor_run(A, k, run, pw, total, lds[w], means);
+    work.finish();
+    return;
   }
+  if (chunk == 0) {  // XCD-contiguous ranges without atomics
+    XcdStride it;
+    it.init(nruns);
+    uint32_t run = 0;
+    while (it.next(run)) sor_run(A, k, run, pw, total, lds[w], means);
+    return;
+  }
+  for (uint32_t run = wave_id(); run < nruns; run += wave_count()) sor_run(A, k, run, pw, total, lds[w], means);
 }
 
 // keep flag (u8, and u32 for the compaction scan) of every point: mean_i <= gmean + alpha * std of its cloud,
@@ -700,8 +722,24 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
     RVP_TRY(mt.alloc(n_bound * sizeof(float), s));
     mp = mt.as<float>();
   }
-  RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<<<query_blocks(n_bound), QT, 0, s>>>(G.arrays(), k,
-                                                                                      tune_queries_per_wave(), mp));
+  {
+    const int forced = tune_queries_per_wave();
+    unsigned blocks = query_blocks(n_bound);
+    uint32_t *queue = nullptr, chunk = 1;
+    const int mode = tune_work_steal();
+    if (mode == 2) chunk = 0;
+    if (mode == 1) {
+      // persistent: as many blocks as the device holds at once (fewer for small batches), runs in chunks of
+      // about half a wave's share
+      RVP_TRY(stream_work_queue(s, &queue));
+      unsigned res = 1;
+      RVP_TRY(resident_blocks(reinterpret_cast<const void *>(seg_sor_means_k), QT, &res));
+      const size_t runs = ceil_div(n_bound, (size_t)knn_run_length(n_bound, forced));
+      blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(res, ceil_div(runs, QW)));
+      chunk = (uint32_t)std::max<size_t>(1, runs / ((size_t)blocks * QW * 2));
+    }
+    RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<<<blocks, QT, 0, s>>>(G.arrays(), k, forced, mp, queue, chunk));
+  }
   RVP_TRY(sums.alloc(nclouds * sizeof(float), s));
   RVP_TRY(vsum.alloc(nclouds * sizeof(float), s));
   // statistical_outlier_removal.cpp:44-51: the sum of the means, then of the squared deviations from their

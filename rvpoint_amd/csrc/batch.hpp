@@ -79,6 +79,83 @@ template <int NB, class K> int batch_blocks(K kernel, size_t n_bound, unsigned *
 __device__ __forceinline__ uint32_t wave_id() { return blockIdx.x * QW + (threadIdx.x >> 6); }
 __device__ __forceinline__ uint32_t wave_count() { return gridDim.x * QW; }
 
+// Work of the query kernels kept on one XCD.  The units [0, units) are split into NXCD contiguous ranges, one
+// per XCD (workgroups are dispatched round-robin over the XCDs, so block b runs on XCD b mod NXCD); neighbouring
+// units (queries in cell order) then stay on one XCD, whose L2 holds only its share of the cloud.
+constexpr int NXCD = 8;
+constexpr int XCD_PAD = 32;  // 32-bit words between the counters: one 128-byte line each, no shared line
+
+__device__ __forceinline__ uint32_t xcd_lo(uint32_t units, int x) {
+  return (uint32_t)((uint64_t)units * (uint64_t)x / NXCD);
+}
+
+// Mode 2: without atomics.  Wave i of XCD x's waves takes units lo_x + i, lo_x + i + G_x, ... of that range
+// (G_x = the launch's waves on XCD x).
+struct XcdStride {
+  uint32_t u, hi, step;
+  __device__ __forceinline__ void init(uint32_t units) {
+    const int x = (int)(blockIdx.x % NXCD);
+    const uint32_t blocks_x = (gridDim.x - (uint32_t)x + NXCD - 1) / NXCD;  // blocks of this launch on XCD x
+    const uint32_t wpb = blockDim.x / WAVE;
+    u = xcd_lo(units, x) + (blockIdx.x / NXCD) * wpb + (threadIdx.x / WAVE);
+    hi = xcd_lo(units, x + 1);
+    step = blocks_x * wpb;
+  }
+  __device__ __forceinline__ bool next(uint32_t &unit) {
+    if (u >= hi) return false;
+    unit = u;
+    u += step;
+    return true;
+  }
+};
+
+// Mode 1: persistent waves that steal.  A wave starts on its own XCD's range, takes `chunk` units at a time
+// from that range's counter, and when the range is drained moves on to the next XCD's range, until every range
+// is drained.  q: the stream's work counters (NXCD take counters XCD_PAD words apart, then the count of
+// finished waves); the last wave of the launch to finish clears them, so every launch starts from zero (also
+// when replayed from a captured graph).
+struct XcdWork {
+  uint32_t *q;
+  uint32_t units, chunk;
+  int home, probe;
+  __device__ __forceinline__ void init(uint32_t *q_, uint32_t units_, uint32_t chunk_) {
+    q = q_;
+    units = units_;
+    chunk = chunk_;
+    home = (int)(blockIdx.x % NXCD);
+    probe = 0;
+  }
+  // the wave's next units [u0, u1) (wave-uniform); false once every range is drained
+  __device__ __forceinline__ bool next(uint32_t &u0, uint32_t &u1) {
+    while (probe < NXCD) {
+      const int x = (home + probe) % NXCD;
+      const uint32_t lo = xcd_lo(units, x), hi = xcd_lo(units, x + 1);
+      uint32_t t = 0;
+      if (lane() == 0) t = atomicAdd(&q[x * XCD_PAD], chunk);
+      t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+      const uint64_t start = (uint64_t)lo + t;
+      if (start < hi) {
+        u0 = (uint32_t)start;
+        u1 = (uint32_t)min<uint64_t>(start + chunk, hi);
+        return true;
+      }
+      ++probe;
+    }
+    return false;
+  }
+  // every wave of the launch calls this once, after its last next(); the last one clears the counters (all other
+  // waves have finished their takes by then)
+  __device__ __forceinline__ void finish() {
+    if (lane() == 0) {
+      const uint32_t waves = gridDim.x * (blockDim.x / WAVE);
+      if (atomicAdd(&q[NXCD * XCD_PAD], 1u) == waves - 1) {
+        for (int x = 0; x <= NXCD; ++x)
+          __hip_atomic_store(&q[x * XCD_PAD], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+};
+
 // Batch k (up to NB queries) of a wave that solves a batch at a time; false when the wave is done.
 // STRIDED (a grid of two rounds of resident waves, for counts that fill only a few rounds of batches): batch
 // k holds the wave's queries g + (k*NB + j)*G, so every wave gets the same count to within one and no

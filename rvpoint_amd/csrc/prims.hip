@@ -415,7 +415,8 @@ __global__ __launch_bounds__(PRIM_THREADS) void run_heads_k(const K *__restrict_
   }
 }
 
-// Per-(device, stream) look-back state: status words and the control word (generation, ticket; see take_tile).
+// Per-(device, stream) look-back state: status words and the control words: word 0 is (generation, ticket; see
+// take_tile), the rest the work counters of the persistent query kernels (stream_work_queue).
 // Launches on one stream run in stream order whichever host thread enqueued them, and each accepts only status
 // words of its own epoch, so several threads may drive one stream.  A status array that has been handed out is
 // never freed while the process runs: a larger one replaces it and the old one is kept (retired), because a
@@ -436,7 +437,7 @@ static std::map<std::pair<int, hipStream_t>, ScanState> g_scan;
 // would leave their arrays.
 __global__ void state_clear_k(uint64_t *__restrict__ status, size_t words, uint64_t *__restrict__ ctl) {
   const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ctl && i0 == 0) *ctl = 0ull;
+  if (ctl && i0 < CTL_WORDS) ctl[i0] = 0ull;
   for (size_t i = i0; i < words; i += (size_t)gridDim.x * blockDim.x) status[i] = 0ull;
 }
 
@@ -447,7 +448,7 @@ static int scan_state(hipStream_t s, size_t tiles, uint64_t **status, uint64_t *
   ScanState &st = g_scan[{dev, s}];
   uint64_t *clear_ctl = nullptr;
   if (!st.ctl) {
-    RVP_HIP(hipMalloc(&st.ctl, sizeof(uint64_t)));
+    RVP_HIP(hipMalloc(&st.ctl, CTL_WORDS * sizeof(uint64_t)));
     clear_ctl = st.ctl;
   }
   uint64_t *fresh = nullptr;
@@ -468,6 +469,13 @@ static int scan_state(hipStream_t s, size_t tiles, uint64_t **status, uint64_t *
   }
   *status = st.status;
   *ctl = st.ctl;
+  return RVP_OK;
+}
+
+int stream_work_queue(hipStream_t s, uint32_t **queue) {
+  uint64_t *status = nullptr, *ctl = nullptr;
+  RVP_TRY(scan_state(s, 0, &status, &ctl));
+  *queue = reinterpret_cast<uint32_t *>(ctl + 1);
   return RVP_OK;
 }
 

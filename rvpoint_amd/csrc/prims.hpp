@@ -9,6 +9,13 @@ constexpr int PRIM_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;                       // items per thread per scan tile
 constexpr int SCAN_TILE = PRIM_THREADS * SCAN_ITEMS; // 2048
 
+// control words of a stream's look-back state (64-bit): the ticket word, then WORK_QUEUE_WORDS 32-bit counters
+constexpr int CTL_WORDS = 192;
+constexpr int WORK_QUEUE_WORDS = 2 * (CTL_WORDS - 1);
+// The work counters of the persistent query kernels on stream s (zeroed when created; each launch leaves them
+// zeroed), in stream order like the look-back state.
+int stream_work_queue(hipStream_t s, uint32_t **queue);
+
 // zeroes `words` (<= 256) 32-bit words with a kernel in stream order (capture-safe, unlike hipMemsetAsync)
 int zero_words(void *p, uint32_t words, hipStream_t s);
 

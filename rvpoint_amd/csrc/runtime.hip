@@ -246,6 +246,8 @@ static std::atomic<double> g_knn_cell_div{2.4};
 static std::atomic<double> g_seg_cells_per_point{32.0};
 static std::atomic<int> g_queries_per_wave{0};
 static std::atomic<double> g_seg_scan_table_min{4194304.0};
+static std::atomic<int> g_work_steal{0};
+int tune_work_steal() { return g_work_steal.load(); }
 double tune_knn_cell_div() { return g_knn_cell_div.load(); }
 double tune_seg_cells_per_point() { return g_seg_cells_per_point.load(); }
 int tune_queries_per_wave() { return g_queries_per_wave.load(); }
@@ -497,6 +499,11 @@ int rvp_tune_set(const char *name, double value) {
     rvp::g_seg_scan_table_min.store(value);
     return RVP_OK;
   }
+  if (std::string(name) == "work_steal") {
+    RVP_CHECK(value == 0.0 || value == 1.0 || value == 2.0, RVP_E_ARG, "work_steal must be 0, 1 or 2");
+    rvp::g_work_steal.store((int)value);
+    return RVP_OK;
+  }
   return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);
 }
 
@@ -516,6 +523,10 @@ int rvp_tune_get(const char *name, double *value) {
   }
   if (std::string(name) == "seg_scan_table_min") {
     *value = rvp::g_seg_scan_table_min.load();
+    return RVP_OK;
+  }
+  if (std::string(name) == "work_steal") {
+    *value = (double)rvp::g_work_steal.load();
     return RVP_OK;
   }
   return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);

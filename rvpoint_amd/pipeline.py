@@ -1,9 +1,14 @@
-"""The LiDAR-frame stage chain on the device: voxel grid -> exact-kNN SOR -> radius normals -> RANSAC plane.
+"""Stage chains on the device.
+
+DeviceFramePipeline: the LiDAR-frame chain, voxel grid -> exact-kNN SOR -> radius normals -> RANSAC plane.
 
 The chain follows the reference's pipeline driver (src/tools/pipeline_export.cpp:356-480: voxel_grid_downsamp,
 sor, normal_estimation with a radius search, ransac_plane), with a cap on the radius neighbours.  The frame runs
 as a batch of one cloud: every stage reads its point count from the previous stage's device offsets, so nothing
 waits on the host until RANSAC's adaptive stop is replayed at the end.
+
+DeviceBatchPipeline: many clouds back to back in HBM, one launch per stage: exact-kNN SOR -> capped radius lists
+of the kept points -> k-NN normals of the kept points.
 """
 from __future__ import annotations
 
@@ -157,3 +162,43 @@ class DeviceFramePipeline:
             self.close()
         except Exception:
             pass
+
+
+class DeviceBatchPipeline:
+    """Clouds back to back in HBM, one launch per stage and no host sync between stages: exact-kNN SOR (k_sor)
+    -> radius lists (radius per cloud, capped at cap) of the kept points -> k-NN normals (k_nrm) of the kept
+    points.  The defaults are the 256-cloud workload's: k=32, radius 0.02 * the cloud's scale, cap 64, k=24."""
+
+    K_SOR, K_NRM, CAP = 32, 24, 64
+
+    def __init__(self, clouds, scales, stream=None):
+        self.s = stream
+        self.sizes = np.array([len(c) for c in clouds], np.int64)
+        self.ncl = len(clouds)
+        self.off_host = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.uint32)
+        self.n = int(self.off_host[-1])
+        n, D = self.n, DeviceArray
+        self.cloud = DeviceCloud(np.concatenate(clouds))
+        self.off = D.from_numpy(self.off_host)
+        self.radius_host = (0.02 * np.asarray(scales)).astype(np.float32)
+        self.radius = D.from_numpy(self.radius_host)
+        self.means = D(n)
+        self.sor, self.sor_off = D((n, 3)), D(self.ncl + 1, np.uint32)
+        self.sx, self.sy, self.sz = D(n), D(n), D(n)
+        self.idx, self.d2, self.cnt = D((n, self.CAP), np.int32), D((n, self.CAP)), D(n, np.int32)
+        self.nx, self.ny, self.nz = D(n), D(n), D(n)
+
+    def step(self):
+        s, n = self.s, self.n
+        call("rvp_batch_sor_exact", *self.cloud.ptrs(), self.off.ptr, self.ncl, n, self.K_SOR, 1.0, self.sor.ptr,
+             self.sor_off.ptr, None, self.means.ptr, s)
+        # the kept total is the last entry of the batch's output offsets, read on the device
+        call("rvp_aos_to_soa_bounded", self.sor.ptr, n, self.sor_off.ptr + 4 * self.ncl, self.sx.ptr, self.sy.ptr,
+             self.sz.ptr, s)
+        kept = (self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, self.ncl, n)
+        call("rvp_batch_radius_search", *kept, self.radius.ptr, None, self.CAP, self.idx.ptr, self.d2.ptr,
+             self.cnt.ptr, s)
+        call("rvp_batch_normals_knn", *kept, self.K_NRM, 0.0, 0.0, 0.0, self.nx.ptr, self.ny.ptr, self.nz.ptr, s)
+
+    def kept_total(self):
+        return int(self.sor_off.numpy(stream=self.s)[-1])

@@ -84,25 +84,30 @@ def c3_subset():
     return clouds, scales
 
 
-@pytest.mark.parametrize("per_wave", [0, 16, 64])
-def test_batch_subset_stages_vs_oracle(c3_subset, per_wave):
+@pytest.mark.parametrize("per_wave,steal", [(0, 0), (0, 1), (0, 2), (16, 1), (64, 2)])
+def test_batch_subset_stages_vs_oracle(c3_subset, per_wave, steal):
     """SOR k=32 -> radius r=0.02*scale capped at 64 on the kept points -> kNN normals k=24, as the batched leg
-    of the benchmark; per_wave 0 = the run length chosen from the count, else forced."""
+    of the benchmark; per_wave 0 = the run length chosen from the count, else forced; steal: persistent waves with
+    XCD-local runs and stealing (1), runs strided within per-XCD ranges (2), or over the launch (0)."""
     clouds, scales = c3_subset
     b = R.Batch(clouds)
-    old = R._lib.tune("queries_per_wave")
+    old = R._lib.tune("queries_per_wave"), R._lib.tune("work_steal")
     try:
         R._lib.tune("queries_per_wave", per_wave)
+        R._lib.tune("work_steal", steal)
         kept, means = R.batch_sor_exact(b, 32, 1.0)
+        kept2, means2 = R.batch_sor_exact(b, 32, 1.0)  # the work counters start from zero again
     finally:
-        R._lib.tune("queries_per_wave", old)
+        R._lib.tune("queries_per_wave", old[0])
+        R._lib.tune("work_steal", old[1])
+    assert (means2 == means).all() and all((a == c).all() for a, c in zip(kept, kept2))
     okept = []
     for c, p in enumerate(clouds):
         oout, omeans = O.sor_sc_mt(p, 32, 1.0)
         assert (b.split(means)[c] == omeans).all(), c
         assert kept[c].shape == oout.shape and (kept[c] == oout).all(), c
         okept.append(oout)
-    if per_wave:
+    if per_wave or steal:
         return
     kb = R.Batch(okept)
     radius = np.array([0.02 * s for s in scales], np.float32)
