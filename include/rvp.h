@@ -337,7 +337,8 @@ typedef struct {
 typedef struct {
   uint64_t ticket;
   size_t n_in, n_voxel, n_sor;          /* points in, voxels, points kept by SOR */
-  float *points;                        /* kept points, AoS float[3 * n_sor] (owned by the result) */
+  float *points;                        /* kept points, AoS float[3 * n_sor] (pinned host memory owned by the
+                                           result: release with rvp_result_free) */
   float *normals;                       /* their normals, float[3 * n_sor] */
   float model[4];                       /* RANSAC plane */
   int inliers, iterations;
