@@ -592,32 +592,32 @@ __device__ __forceinline__ void sor_run(const SegArrays &A, int k, uint32_t run,
   }
 }
 
-// queue != nullptr: persistent waves take runs from per-XCD ranges and steal (XcdWork), `chunk` runs at a time;
-// else chunk == 0: runs strided within per-XCD ranges (XcdStride); else runs strided over the launch's waves.
+// WS (the work_steal knob) 1: persistent waves take runs from per-XCD ranges and steal (XcdWork), `chunk` runs
+// at a time; 2: runs strided within per-XCD ranges (XcdStride); 0: runs strided over the launch's waves.  One
+// instantiation per distribution, so each carries only its own loop's registers.
+template <int WS>
 __global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, int forced, float *__restrict__ means,
                                                       uint32_t *__restrict__ queue, uint32_t chunk) {
   __shared__ WaveLds lds[QW];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_in_block();
   const uint32_t total = A.off[A.ncl];
   const uint32_t pw = (uint32_t)knn_run_length(total, forced);
   const uint32_t nruns = (total + pw - 1) / pw;
-  if (queue) {
+  if constexpr (WS == 1) {
     XcdWork work;
     work.init(queue, nruns, chunk);
     uint32_t r0 = 0, r1 = 0;
     while (work.next(r0, r1))
       for (uint32_t run = r0; run < r1; ++run) sor_run(A, k, run, pw, total, lds[w], means);
     work.finish();
-    return;
-  }
-  if (chunk == 0) {  // XCD-contiguous ranges without atomics
+  } else if constexpr (WS == 2) {  // XCD-contiguous ranges without atomics
     XcdStride it;
     it.init(nruns);
     uint32_t run = 0;
     while (it.next(run)) sor_run(A, k, run, pw, total, lds[w], means);
-    return;
+  } else {
+    for (uint32_t run = wave_id(); run < nruns; run += wave_count()) sor_run(A, k, run, pw, total, lds[w], means);
   }
-  for (uint32_t run = wave_id(); run < nruns; run += wave_count()) sor_run(A, k, run, pw, total, lds[w], means);
 }
 
 // keep flag (u8, and u32 for the compaction scan) of every point: mean_i <= gmean + alpha * std of its cloud,
@@ -655,7 +655,7 @@ __global__ __launch_bounds__(QT) void seg_radius_topk_k(SegArrays A, const float
                                                          float *__restrict__ d2_out, int32_t *__restrict__ counts) {
   __shared__ WaveLds lds[QW];
   __shared__ StageLds stage[QW];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_in_block();
   WaveLds &L = lds[w];
   StageLds &S = stage[w];
   const uint32_t nq = A.off[A.ncl];
@@ -727,18 +727,24 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
     unsigned blocks = query_blocks(n_bound);
     uint32_t *queue = nullptr, chunk = 1;
     const int mode = tune_work_steal();
-    if (mode == 2) chunk = 0;
     if (mode == 1) {
       // persistent: as many blocks as the device holds at once (fewer for small batches), runs in chunks of
       // about half a wave's share
       RVP_TRY(stream_work_queue(s, &queue));
       unsigned res = 1;
-      RVP_TRY(resident_blocks(reinterpret_cast<const void *>(seg_sor_means_k), QT, &res));
+      RVP_TRY(resident_blocks(reinterpret_cast<const void *>(seg_sor_means_k<1>), QT, &res));
       const size_t runs = ceil_div(n_bound, (size_t)knn_run_length(n_bound, forced));
       blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(res, ceil_div(runs, QW)));
       chunk = (uint32_t)std::max<size_t>(1, runs / ((size_t)blocks * QW * 2));
+      RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<1><<<blocks, QT, 0, s>>>(G.arrays(), k, forced, mp, queue,
+                                                                               chunk));
+    } else if (mode == 2) {
+      RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<2><<<blocks, QT, 0, s>>>(G.arrays(), k, forced, mp, queue,
+                                                                               chunk));
+    } else {
+      RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<0><<<blocks, QT, 0, s>>>(G.arrays(), k, forced, mp, queue,
+                                                                               chunk));
     }
-    RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<<<blocks, QT, 0, s>>>(G.arrays(), k, forced, mp, queue, chunk));
   }
   RVP_TRY(sums.alloc(nclouds * sizeof(float), s));
   RVP_TRY(vsum.alloc(nclouds * sizeof(float), s));

@@ -76,7 +76,7 @@ template <int NB, class K> int batch_blocks(K kernel, size_t n_bound, unsigned *
 }
 
 // this wave's index g and the wave count G of the launch
-__device__ __forceinline__ uint32_t wave_id() { return blockIdx.x * QW + (threadIdx.x >> 6); }
+__device__ __forceinline__ uint32_t wave_id() { return blockIdx.x * QW + (uint32_t)wave_in_block(); }
 __device__ __forceinline__ uint32_t wave_count() { return gridDim.x * QW; }
 
 // Work of the query kernels kept on one XCD.  The units [0, units) are split into NXCD contiguous ranges, one
@@ -97,7 +97,7 @@ struct XcdStride {
     const int x = (int)(blockIdx.x % NXCD);
     const uint32_t blocks_x = (gridDim.x - (uint32_t)x + NXCD - 1) / NXCD;  // blocks of this launch on XCD x
     const uint32_t wpb = blockDim.x / WAVE;
-    u = xcd_lo(units, x) + (blockIdx.x / NXCD) * wpb + (threadIdx.x / WAVE);
+    u = xcd_lo(units, x) + (blockIdx.x / NXCD) * wpb + (uint32_t)wave_in_block();
     hi = xcd_lo(units, x + 1);
     step = blocks_x * wpb;
   }

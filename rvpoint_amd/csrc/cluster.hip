@@ -69,7 +69,7 @@ struct UniteSink {
 // one wave per point (cell order): unite with every neighbour of smaller index within the tolerance
 __global__ __launch_bounds__(QT) void cluster_unite_k(GridView g, float tol, uint32_t *__restrict__ parent) {
   __shared__ WaveLds lds[QW];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_in_block();
   const float r2 = __fmul_rn(tol, tol);
   for (uint32_t q = blockIdx.x * QW + w; q < g.n; q += gridDim.x * QW) {
     const float x = g.xs[q], y = g.ys[q], z = g.zs[q];

@@ -2,6 +2,11 @@
 // covariance re-centred at the query point, closed-form FP32 3x3 eigen solve with an FP64 Jacobi
 // fallback for near-degenerate spectra, flip toward the viewpoint.
 //
+// Two kernels per call: the neighbourhood kernels (one wave per query) write each query's moments to a
+// record array, and normals_solve_k solves the eigen problems one lane per query.  Keeping the solver out of
+// the search kernels keeps them near 50-60 VGPRs (seven waves per SIMD instead of four or five), and the
+// solver's instructions serve 64 queries per wave instead of the handful a search wave holds.
+//
 // Reference: normal_estimation_sc (src/features/normal_estimation.cpp:72-112), normal_estimation_rvv with
 // a SpatialHash (:189-214), flipNormalTowardsViewpoint (:55-67).  The reference's own eigen solver
 // (simple_eigen3x3_smallest, :16-52: four non-converged Jacobi steps with a sign error) is deliberately not
@@ -50,16 +55,14 @@ static int fallback_counter(hipStream_t s, unsigned long long **out) {
   return RVP_OK;
 }
 
-// Each wave handles NB consecutive queries (cell order) and then solves their NB eigen problems in NB
-// lanes at once; NB is small so that a launch has enough waves to hide memory latency.
+// Each wave handles NB consecutive queries (cell order): from the second on, a k-NN query is bounded by the
+// previous one's K-th distance, and radius windows stage once in LDS for the queries that share them.
 constexpr int NB = 8;
 
-struct NormLds {
-  float c[9][NB];  // covariance of query j (6), then the neighbourhood mean relative to the query (3)
-  float q[3][NB];  // query point
-  uint32_t row[NB];
-  int m[NB];
-};
+// Per-query record of the neighbourhood kernels, structure of arrays with stride n (cell-order position q):
+// rec[e * n + q] for e = 0..5 the covariance (xx, xy, xz, yy, yz, zz), 6..8 the neighbourhood mean relative
+// to the query point, 9 the neighbour count (int bits).
+constexpr int REC_WORDS = 10;
 
 // cyclic Jacobi in double precision (symmetric 3x3), eigenvector of the smallest eigenvalue
 __device__ void jacobi_smallest_f64(double a00, double a01, double a02, double a11, double a12, double a22,
@@ -213,64 +216,63 @@ __device__ __forceinline__ void wave_cov(const GridView &g, bool valid, uint32_t
   c[8] = mz;
 }
 
-// lane j solves query j of the wave's batch
-// (cx may be null; else cx/cy/cz receive the neighbourhood centroid q + mean)
-__device__ __forceinline__ void solve_batch(NormLds &N, int nbatch, float vpx, float vpy, float vpz,
-                                            float *__restrict__ nx, float *__restrict__ ny, float *__restrict__ nz,
-                                            unsigned long long *fallbacks, float *__restrict__ cx = nullptr,
-                                            float *__restrict__ cy = nullptr, float *__restrict__ cz = nullptr) {
-  wave_lds_sync();
+// the wave's moments of query q (uniform across the wave after wave_cov) into the record: lane e stores word e
+__device__ __forceinline__ void write_rec(float *__restrict__ rec, size_t n, uint32_t q, const float c[9], int m) {
   const int l = lane();
+  float v = __int_as_float(m);
+#pragma unroll
+  for (int e = 0; e < 9; ++e)
+    if (l == e) v = c[e];
+  if (l < REC_WORDS) rec[(size_t)l * n + q] = v;
+}
+
+// One lane per query: eigen solve of the recorded covariance (m >= 3 neighbours, else a zero normal), flip
+// toward the viewpoint, scatter to the query's row (perm: cell order -> caller's order).  count_dev: the
+// device-side query count (batched stages), else n.  cx may be null; else cx/cy/cz receive the neighbourhood
+// centroid q + mean.
+__global__ __launch_bounds__(256) void normals_solve_k(const float *__restrict__ rec, size_t n,
+                                                        const uint32_t *__restrict__ count_dev,
+                                                        const float *__restrict__ xs, const float *__restrict__ ys,
+                                                        const float *__restrict__ zs, const uint32_t *__restrict__ perm,
+                                                        float vpx, float vpy, float vpz, float *__restrict__ nx,
+                                                        float *__restrict__ ny, float *__restrict__ nz,
+                                                        unsigned long long *fallbacks, float *__restrict__ cx,
+                                                        float *__restrict__ cy, float *__restrict__ cz) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t count = count_dev ? min((size_t)*count_dev, n) : n;
   bool fb = false;
-  if (l < nbatch) {
-    float n[3] = {0.f, 0.f, 0.f};
-    if (N.m[l] >= 3) {
-      const float c[6] = {N.c[0][l], N.c[1][l], N.c[2][l], N.c[3][l], N.c[4][l], N.c[5][l]};
-      fb = smallest_eigvec(c, n);
-      flip_toward(N.q[0][l], N.q[1][l], N.q[2][l], vpx, vpy, vpz, n);
+  if (q < count) {
+    const int m = __float_as_int(rec[9 * n + q]);
+    const float px = xs[q], py = ys[q], pz = zs[q];
+    float nrm[3] = {0.f, 0.f, 0.f};
+    if (m >= 3) {
+      const float c[6] = {rec[q], rec[n + q], rec[2 * n + q], rec[3 * n + q], rec[4 * n + q], rec[5 * n + q]};
+      fb = smallest_eigvec(c, nrm);
+      flip_toward(px, py, pz, vpx, vpy, vpz, nrm);
     }
-    const uint32_t r = N.row[l];
-    nx[r] = n[0];
-    ny[r] = n[1];
-    nz[r] = n[2];
+    const uint32_t r = perm[q];
+    nx[r] = nrm[0];
+    ny[r] = nrm[1];
+    nz[r] = nrm[2];
     if (cx) {
-      cx[r] = __fadd_rn(N.q[0][l], N.c[6][l]);
-      cy[r] = __fadd_rn(N.q[1][l], N.c[7][l]);
-      cz[r] = __fadd_rn(N.q[2][l], N.c[8][l]);
+      cx[r] = __fadd_rn(px, rec[6 * n + q]);
+      cy[r] = __fadd_rn(py, rec[7 * n + q]);
+      cz[r] = __fadd_rn(pz, rec[8 * n + q]);
     }
   }
   const int nf = popc(ballot(fb));
-  if (l == 0 && nf) atomicAdd(fallbacks, (unsigned long long)nf);
-  wave_lds_sync();
-}
-
-__device__ __forceinline__ void stash(NormLds &N, int j, const float c[9], float x, float y, float z, uint32_t row,
-                                      int m) {
-  if (lane() == 0) {
-    for (int e = 0; e < 9; ++e) N.c[e][j] = c[e];
-    N.q[0][j] = x;
-    N.q[1][j] = y;
-    N.q[2][j] = z;
-    N.row[j] = row;
-    N.m[j] = m;
-  }
+  if (lane() == 0 && nf) atomicAdd(fallbacks, (unsigned long long)nf);
 }
 
 // MODE 0: k nearest (N1, K = k+1 incl. self); MODE 1: radius list capped at K (N2)
 template <int MODE, bool STRIDED>
-__global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, float vpx, float vpy, float vpz,
-                                                      float *__restrict__ nx, float *__restrict__ ny,
-                                                      float *__restrict__ nz, unsigned long long *fallbacks,
-                                                      float *__restrict__ cx, float *__restrict__ cy,
-                                                      float *__restrict__ cz) {
+__global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, float *__restrict__ rec) {
   __shared__ WaveLds lds[QW];
-  __shared__ NormLds nl[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
   constexpr bool STAGE = MODE == 1 && !STRIDED;
   __shared__ typename std::conditional<STAGE, StageLds, char>::type stage[QW];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_in_block();
   WaveLds &L = lds[w];
-  NormLds &N = nl[w];
   const float r2 = __fmul_rn(radius, radius);
   uint32_t first = 0, step = 1;
   int nbatch = 0;
@@ -306,22 +308,17 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
       float c[9];
       // the list holds (d2, index) keys only: the neighbour's cell-order position is rank[index]
       wave_cov(g, lane() < m, lane() < m ? g.rank[key_idx(t.key)] : 0u, m, x, y, z, c);
-      stash(N, j, c, x, y, z, g.perm[q], m);
+      write_rec(rec, g.n, q, c, m);
     }
-    solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks, cx, cy, cz);
   }
 }
 
 // N2 without a cap: neighbour positions in CSR rows (row = original index), covariance in chunks of 64
 __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *__restrict__ offsets, int cap,
-                                                     const uint32_t *__restrict__ pos, float vpx, float vpy, float vpz,
-                                                     float *__restrict__ nx, float *__restrict__ ny,
-                                                     float *__restrict__ nz, unsigned long long *fallbacks) {
-  __shared__ NormLds nl[QW];
-  const int w = threadIdx.x >> 6;
+                                                     const uint32_t *__restrict__ pos, float *__restrict__ rec) {
+  const int w = wave_in_block();
   const uint32_t base = (blockIdx.x * QW + w) * NB;
   if (base >= g.n) return;
-  NormLds &N = nl[w];
   const int l = lane();
   const int nbatch = (int)min<uint32_t>(NB, g.n - base);
   for (int j = 0; j < nbatch; ++j) {
@@ -371,25 +368,20 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
       wave_sum_comp6(cs, ce, c);
       for (int a = 0; a < 3; ++a) c[6 + a] = mu[a];
     }
-    stash(N, j, c, x, y, z, row, m);
+    write_rec(rec, g.n, q, c, m);
   }
-  solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
 }
 
 // N1 / N2 over a batch: every query's neighbourhood within its own cloud (MODE 0: K nearest incl. self;
 // MODE 1: radius[c] list capped at K)
 template <int MODE, bool STRIDED>
-__global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius, float vpx,
-                                                     float vpy, float vpz, float *__restrict__ nx,
-                                                     float *__restrict__ ny, float *__restrict__ nz,
-                                                     unsigned long long *fallbacks) {
+__global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius,
+                                                     float *__restrict__ rec, size_t n_bound) {
   __shared__ WaveLds lds[QW];
-  __shared__ NormLds nl[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
   constexpr bool STAGE = MODE == 1 && !STRIDED;
   __shared__ typename std::conditional<STAGE, StageLds, char>::type stage[QW];
-  const int w = threadIdx.x >> 6;
-  NormLds &N = nl[w];
+  const int w = wave_in_block();
   const uint32_t total = A.off[A.ncl];
   uint32_t first = 0, step = 1;
   int nbatch = 0;
@@ -431,10 +423,18 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
       const int m = min(t.cnt, K);
       float c6[9];
       wave_cov(g, lane() < m, lane() < m ? g.rank[key_idx(t.key)] : 0u, m, x, y, z, c6);
-      stash(N, j, c6, x, y, z, g.perm[q], m);
+      write_rec(rec, n_bound, q, c6, m);
     }
-    solve_batch(N, nbatch, vpx, vpy, vpz, nx, ny, nz, fallbacks);
   }
+}
+
+// the eigen solves of n recorded queries (count_dev: their device-side count, or null for n)
+static int solve_normals(const float *rec, size_t n, const uint32_t *count_dev, const float *xs, const float *ys,
+                         const float *zs, const uint32_t *perm, float vpx, float vpy, float vpz, float *nx, float *ny,
+                         float *nz, unsigned long long *fb, float *cx, float *cy, float *cz, hipStream_t s) {
+  RVP_LAUNCH("normals_solve_k", s, normals_solve_k<<<ceil_div(n, 256), 256, 0, s>>>(
+                                       rec, n, count_dev, xs, ys, zs, perm, vpx, vpy, vpz, nx, ny, nz, fb, cx, cy, cz));
+  return RVP_OK;
 }
 
 }  // namespace rvp
@@ -457,17 +457,18 @@ int rvp_normals_knn_centroids(const float *x, const float *y, const float *z, si
   RVP_TRY(G.build_for_knn(x, y, z, n, k + 1, s));
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
+  Tmp rec;
+  RVP_TRY(rec.alloc((size_t)REC_WORDS * n * sizeof(float), s));
   unsigned qb = 1;
   bool strided = false;
   RVP_TRY(batch_blocks<NB>(normals_topk_k<0, true>, n, &qb, &strided));
+  const GridView g = G.view();
   if (strided)
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, true><<<qb, QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz,
-                                                                              nx, ny, nz, fb, cx, cy, cz));
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, true><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, rec.as<float>()));
   else
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, false><<<qb, QT, 0, s>>>(G.view(), k + 1, 0.0f, vpx, vpy, vpz,
-                                                                               nx, ny, nz, fb, cx, cy, cz));
-  RVP_LAUNCHED();
-  return RVP_OK;
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, false><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, rec.as<float>()));
+  return solve_normals(rec.as<float>(), n, nullptr, g.xs, g.ys, g.zs, g.perm, vpx, vpy, vpz, nx, ny, nz, fb, cx, cy,
+                       cz, s);
 }
 
 int rvp_normals_knn(const float *x, const float *y, const float *z, size_t n, int k, float vpx, float vpy, float vpz,
@@ -486,12 +487,12 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
   const GridView g = h->g.view();
+  Tmp rec;
+  RVP_TRY(rec.alloc((size_t)REC_WORDS * n * sizeof(float), s));
   if (max_nn > 0 && max_nn <= MAX_TOPK) {
     // batches of NB consecutive queries with staged windows (as the batched radius normals)
     const unsigned qb = ceil_div(ceil_div(n, NB), QW);
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, false><<<qb, QT, 0, s>>>(g, max_nn, radius, vpx, vpy, vpz, nx,
-                                                                               ny, nz, fb, nullptr, nullptr, nullptr));
-    RVP_LAUNCHED();
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, false><<<qb, QT, 0, s>>>(g, max_nn, radius, rec.as<float>()));
   } else {
     // unlimited (or a cap above 64): full lists, sorted by (d2, idx), truncated by the row offsets
     Tmp cnt, off, keys, pos;
@@ -506,11 +507,12 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
                        pos.as<uint32_t>(), s));
     // a cap keeps the first min(len, max_nn) entries of each (d2, idx)-sorted row
     const int cap = max_nn > 0 ? max_nn : 0x7fffffff;
-    RVP_LAUNCH("normals_csr_k", s, normals_csr_k<<<ceil_div(n, QW * NB), QT, 0, s>>>(g, off.as<int64_t>(), cap, pos.as<uint32_t>(), vpx, vpy,
-                                                        vpz, nx, ny, nz, fb));
-    RVP_LAUNCHED();
+    RVP_LAUNCH("normals_csr_k", s, normals_csr_k<<<ceil_div(n, QW * NB), QT, 0, s>>>(g, off.as<int64_t>(), cap,
+                                                                                     pos.as<uint32_t>(),
+                                                                                     rec.as<float>()));
   }
-  return RVP_OK;
+  return solve_normals(rec.as<float>(), n, nullptr, g.xs, g.ys, g.zs, g.perm, vpx, vpy, vpz, nx, ny, nz, fb, nullptr,
+                       nullptr, nullptr, s);
 }
 
 int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
@@ -525,16 +527,20 @@ int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const 
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, true, k + 1, nullptr, s));
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
+  Tmp rec;
+  RVP_TRY(rec.alloc((size_t)REC_WORDS * n_bound * sizeof(float), s));
   unsigned qb = 1;
   bool strided = false;
   RVP_TRY(batch_blocks<NB>(seg_normals_k<0, true>, n_bound, &qb, &strided));
+  const SegArrays A = G.arrays();
   if (strided)
-    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true><<<qb, QT, 0, s>>>(G.arrays(), k + 1, nullptr, vpx, vpy,
-                                                                            vpz, nx, ny, nz, fb));
+    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true><<<qb, QT, 0, s>>>(A, k + 1, nullptr, rec.as<float>(),
+                                                                            n_bound));
   else
-    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false><<<qb, QT, 0, s>>>(G.arrays(), k + 1, nullptr, vpx, vpy,
-                                                                             vpz, nx, ny, nz, fb));
-  return RVP_OK;
+    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false><<<qb, QT, 0, s>>>(A, k + 1, nullptr, rec.as<float>(),
+                                                                             n_bound));
+  return solve_normals(rec.as<float>(), n_bound, G.off + nclouds, A.xs, A.ys, A.zs, A.perm, vpx, vpy, vpz, nx, ny,
+                       nz, fb, nullptr, nullptr, nullptr, s);
 }
 
 int rvp_batch_normals_radius(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
@@ -549,12 +555,16 @@ int rvp_batch_normals_radius(const float *x, const float *y, const float *z, con
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, false, 0, cell ? cell : radius, s));
   unsigned long long *fb = nullptr;
   RVP_TRY(fallback_counter(s, &fb));
+  Tmp rec;
+  RVP_TRY(rec.alloc((size_t)REC_WORDS * n_bound * sizeof(float), s));
   // radius lists: batches of NB consecutive queries, whose shared windows stage once in LDS (faster than the
   // strided batches that balance k-NN queries, also for small counts)
   const unsigned qb = ceil_div(ceil_div(n_bound, NB), QW);
-  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(G.arrays(), max_nn, radius, vpx, vpy,
-                                                                           vpz, nx, ny, nz, fb));
-  return RVP_OK;
+  const SegArrays A = G.arrays();
+  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(A, max_nn, radius, rec.as<float>(),
+                                                                           n_bound));
+  return solve_normals(rec.as<float>(), n_bound, G.off + nclouds, A.xs, A.ys, A.zs, A.perm, vpx, vpy, vpz, nx, ny,
+                       nz, fb, nullptr, nullptr, nullptr, s);
 }
 
 int rvp_normals_fallback_count(size_t *count) {

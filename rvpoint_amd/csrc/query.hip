@@ -18,7 +18,7 @@ __global__ __launch_bounds__(QT) void radius_topk_k(GridView g, const float *__r
                                                      int32_t *__restrict__ counts, int32_t *__restrict__ total) {
   __shared__ WaveLds lds[QW];
   __shared__ StageLds stage[QW];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_in_block();
   WaveLds &L = lds[w];
   StageLds &S = stage[w];
   // this wave's queries [q0, q0 + nb): lane j loads query q0 + j and computes its window once
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(QT) void radius_count_k(GridView g, const float *__
                                                       const float *__restrict__ qy, const float *__restrict__ qz,
                                                       uint32_t nq, float r, int32_t *__restrict__ counts) {
   __shared__ WaveLds lds[QW];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_in_block();
   const uint32_t q = blockIdx.x * QW + w;
   if (q >= nq) return;
   float x, y, z;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(QT) void radius_fill_k(GridView g, const float *__r
                                                      uint32_t nq, float r, const int64_t *__restrict__ offsets,
                                                      uint64_t *__restrict__ keys, uint32_t *__restrict__ pos) {
   __shared__ WaveLds lds[QW];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_in_block();
   const uint32_t q = blockIdx.x * QW + w;
   if (q >= nq) return;
   float x, y, z;
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(QT) void radius_fill_k(GridView g, const float *__r
 // rows of <= 64 keys: one wave each, registers only
 __global__ __launch_bounds__(QT) void segsort_small_k(const int64_t *__restrict__ offsets, uint32_t nrows,
                                                        uint64_t *__restrict__ keys, uint32_t *__restrict__ pos) {
-  const uint32_t r = blockIdx.x * QW + (threadIdx.x >> 6);
+  const uint32_t r = blockIdx.x * QW + (uint32_t)wave_in_block();
   if (r >= nrows) return;
   const int64_t b = offsets[r];
   const int m = (int)(offsets[r + 1] - b);
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(QT) void bruteforce_k(const float *__restrict__ x, 
                                                     const float *__restrict__ qz, uint32_t nq, float r, int max_nn,
                                                     int32_t *__restrict__ idx_out, float *__restrict__ d2_out,
                                                     int32_t *__restrict__ counts) {
-  const uint32_t q = blockIdx.x * QW + (threadIdx.x >> 6);
+  const uint32_t q = blockIdx.x * QW + (uint32_t)wave_in_block();
   if (q >= nq) return;
   const int l = lane();
   const float a = qx[q], b = qy[q], c = qz[q];
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(QT) void knn_k(GridView g, const float *__restrict_
                                              const float *__restrict__ qz, uint32_t nq, int k,
                                              int32_t *__restrict__ idx_out, float *__restrict__ d2_out) {
   __shared__ WaveLds lds[QW];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_in_block();
   const uint32_t q = blockIdx.x * QW + w;
   if (q >= nq) return;
   float x, y, z;
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(QT) void knn_k(GridView g, const float *__restrict_
 // one and only hits within the bound are staged.
 __global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, int per_wave, float *__restrict__ means) {
   __shared__ WaveLds lds[QW];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_in_block();
   const uint32_t q0 = (blockIdx.x * QW + w) * (uint32_t)per_wave;
   if (q0 >= g.n) return;
   const uint32_t q1 = min(q0 + (uint32_t)per_wave, g.n);
@@ -349,7 +349,7 @@ __global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, int per
 __global__ __launch_bounds__(QT) void sor_radius_means_k(GridView g, int k, float radius,
                                                           float *__restrict__ means) {
   __shared__ WaveLds lds[QW];
-  const int w = threadIdx.x >> 6;
+  const int w = wave_in_block();
   const uint32_t q = blockIdx.x * QW + w;
   if (q >= g.n) return;
   WaveLds &L = lds[w];
