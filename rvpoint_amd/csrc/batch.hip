@@ -562,12 +562,13 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
 // batched stages
 // ------------------------------------------------------------------------------------------------
 // S1 per cloud: k+1 nearest within the query's cloud, mean of sqrt of entries 1..k (sequential)
-// Waves take runs of consecutive queries (cell order, run length from the device-side count; `forced` > 0
+// Waves take runs of consecutive queries (cell order; run length from the device-side count, `forced` > 0
 // fixes it); within a cloud, the previous query's k-th distance bounds the next one's (knn_bound_from).
-// Only that k-th d2 is carried between queries (the previous position is re-read), which keeps the kernel at
-// 79 VGPRs: six waves per SIMD and no scratch.
+// Only that k-th d2 is carried between queries (the previous position is re-read).  A wave's sums are
+// deferred over its runs (DeferredMeans).  The launch has a wave per run or two: waves that finish early are
+// replaced by new ones, which balances uneven query costs better than long-lived waves striding over runs.
 __device__ __forceinline__ void sor_run(const SegArrays &A, int k, uint32_t run, uint32_t pw, uint32_t total,
-                                        WaveLds &L, float *__restrict__ means) {
+                                        WaveLds &L, SumLds &S, DeferredMeans &dm, float *__restrict__ means) {
   const uint32_t q1 = min(run * pw + pw, total);
   float pd2 = -1.0f;  // k-th d2 of query q - 1 when it is in the run (< 0: none)
   for (uint32_t q = run * pw; q < q1; ++q) {
@@ -584,10 +585,8 @@ __device__ __forceinline__ void sor_run(const SegArrays &A, int k, uint32_t run,
                           : __builtin_inff());
     const int l = lane();
     const int m = min(t.cnt, k + 1);
-    const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
-    float s = 0.0f;
-    for (int j = 1; j <= k; ++j) s = s + readlanef(sq, j);
-    if (l == 0) means[g.perm[q]] = s / __int2float_rn(k);
+    // entries beyond m add +0, which leaves the sum unchanged
+    dm.add(S, (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f, g.perm[q], k, means);
     pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
   }
 }
@@ -599,25 +598,29 @@ template <int WS>
 __global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, int forced, float *__restrict__ means,
                                                       uint32_t *__restrict__ queue, uint32_t chunk) {
   __shared__ WaveLds lds[QW];
+  __shared__ SumLds sums[QW];
   const int w = wave_in_block();
   const uint32_t total = A.off[A.ncl];
   const uint32_t pw = (uint32_t)knn_run_length(total, forced);
   const uint32_t nruns = (total + pw - 1) / pw;
+  DeferredMeans dm;
   if constexpr (WS == 1) {
     XcdWork work;
     work.init(queue, nruns, chunk);
     uint32_t r0 = 0, r1 = 0;
     while (work.next(r0, r1))
-      for (uint32_t run = r0; run < r1; ++run) sor_run(A, k, run, pw, total, lds[w], means);
+      for (uint32_t run = r0; run < r1; ++run) sor_run(A, k, run, pw, total, lds[w], sums[w], dm, means);
     work.finish();
   } else if constexpr (WS == 2) {  // XCD-contiguous ranges without atomics
     XcdStride it;
     it.init(nruns);
     uint32_t run = 0;
-    while (it.next(run)) sor_run(A, k, run, pw, total, lds[w], means);
+    while (it.next(run)) sor_run(A, k, run, pw, total, lds[w], sums[w], dm, means);
   } else {
-    for (uint32_t run = wave_id(); run < nruns; run += wave_count()) sor_run(A, k, run, pw, total, lds[w], means);
+    for (uint32_t run = wave_id(); run < nruns; run += wave_count())
+      sor_run(A, k, run, pw, total, lds[w], sums[w], dm, means);
   }
+  dm.flush(sums[w], k, means);
 }
 
 // keep flag (u8, and u32 for the compaction scan) of every point: mean_i <= gmean + alpha * std of its cloud,

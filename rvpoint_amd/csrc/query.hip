@@ -316,32 +316,35 @@ __global__ __launch_bounds__(QT) void knn_k(GridView g, const float *__restrict_
 // ------------------------------------------------------------------------------------------------
 // S1 (statistical_outlier_removal.cpp:27-41): k+1 smallest d2 incl. self; sum of sqrt of entries 1..k in
 // ascending order, sequentially; divide by k.
-// Each wave takes a run of per_wave consecutive queries (cell order): from the second on, the previous
-// query's k-th distance bounds the current one (knn_bound_from), so the first box is already the certified
-// one and only hits within the bound are staged.
+// Waves take runs of per_wave consecutive queries (cell order), strided over the launch: from the second
+// query of a run on, the previous query's k-th distance bounds the current one (knn_bound_from), so the first
+// box is already the certified one and only hits within the bound are staged.  The sums of a wave's queries
+// are deferred (DeferredMeans); the launch has one wave per run.
 __global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, int per_wave, float *__restrict__ means) {
   __shared__ WaveLds lds[QW];
+  __shared__ SumLds sums[QW];
   const int w = wave_in_block();
-  const uint32_t q0 = (blockIdx.x * QW + w) * (uint32_t)per_wave;
-  if (q0 >= g.n) return;
-  const uint32_t q1 = min(q0 + (uint32_t)per_wave, g.n);
-  float px = 0.f, py = 0.f, pz = 0.f, pd2 = -1.0f;  // previous query and its k-th d2 (< 0: none)
-  for (uint32_t q = q0; q < q1; ++q) {
-    const float x = uniform(g.xs[q]), y = uniform(g.ys[q]), z = uniform(g.zs[q]);
-    TopKT<uint32_t> t;  // distances only: the means read the multiset of the k+1 smallest
-    t.init(k + 1);
-    knn_search<false>(g, x, y, z, t, lds[w], knn_bound_from(x, y, z, px, py, pz, pd2));
-    const int l = lane();
-    const int m = min(t.cnt, k + 1);
-    const float sq = (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f;
-    float s = 0.0f;
-    for (int j = 1; j <= k; ++j) s = __fadd_rn(s, readlanef(sq, j));  // +0 beyond m leaves s unchanged
-    if (l == 0) means[g.perm[q]] = __fdiv_rn(s, __int2float_rn(k));
-    px = x;
-    py = y;
-    pz = z;
-    pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
+  const uint32_t nruns = (g.n + (uint32_t)per_wave - 1) / (uint32_t)per_wave;
+  DeferredMeans dm;
+  for (uint32_t run = blockIdx.x * QW + (uint32_t)w; run < nruns; run += gridDim.x * QW) {
+    const uint32_t q0 = run * (uint32_t)per_wave, q1 = min(q0 + (uint32_t)per_wave, g.n);
+    float px = 0.f, py = 0.f, pz = 0.f, pd2 = -1.0f;  // previous query and its k-th d2 (< 0: none)
+    for (uint32_t q = q0; q < q1; ++q) {
+      const float x = uniform(g.xs[q]), y = uniform(g.ys[q]), z = uniform(g.zs[q]);
+      TopKT<uint32_t> t;  // distances only: the means read the multiset of the k+1 smallest
+      t.init(k + 1);
+      knn_search<false>(g, x, y, z, t, lds[w], knn_bound_from(x, y, z, px, py, pz, pd2));
+      const int l = lane();
+      const int m = min(t.cnt, k + 1);
+      // entries beyond m add +0, which leaves the sum unchanged
+      dm.add(sums[w], (l >= 1 && l <= k && l < m) ? sqrtf(kd2(t.key)) : 0.0f, g.perm[q], k, means);
+      px = x;
+      py = y;
+      pz = z;
+      pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
+    }
   }
+  dm.flush(sums[w], k, means);
 }
 
 // S2 (statistical_outlier_removal.cpp:246-269): radius list capped at k+1; mean over min(k, m-1) entries

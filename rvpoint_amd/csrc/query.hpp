@@ -525,6 +525,39 @@ __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy
   }
 }
 
+// The SOR means' sequential sums, deferred: each query's sorted distances (entry j on lane j) go to a row of
+// LDS, and once SUM_SLOTS rows are queued lane i sums row i, entries 1..k in order from 0.  The k dependent
+// adds then serve SUM_SLOTS queries instead of one (and need no lane-to-scalar reads); the sums are the same
+// sequence of fp32 adds, so the means are bit for bit those of a per-query sum.  Rows are WAVE + 1 words
+// apart so that the lanes of one read hit distinct banks.
+constexpr int SUM_SLOTS = 4;
+struct SumLds {
+  float v[SUM_SLOTS][WAVE + 1];
+  uint32_t dst[SUM_SLOTS];
+};
+
+struct DeferredMeans {
+  int n = 0;  // rows queued (wave-uniform)
+  // queue one query: `sq` is lane j's entry j (lanes beyond k are never read), `dst` its output index
+  __device__ __forceinline__ void add(SumLds &S, float sq, uint32_t dst, int k, float *__restrict__ means) {
+    S.v[n][lane()] = sq;
+    if (lane() == 0) S.dst[n] = dst;
+    if (++n == SUM_SLOTS) flush(S, k, means);
+  }
+  __device__ __forceinline__ void flush(SumLds &S, int k, float *__restrict__ means) {
+    if (n == 0) return;
+    wave_lds_sync();
+    const int l = lane();
+    if (l < n) {
+      float s = 0.0f;
+      for (int j = 1; j <= k; ++j) s = __fadd_rn(s, S.v[l][j]);
+      means[S.dst[l]] = __fdiv_rn(s, __int2float_rn(k));
+    }
+    wave_lds_sync();
+    n = 0;
+  }
+};
+
 // error-free transformation a + b = s + e
 __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
   s = __fadd_rn(a, b);
