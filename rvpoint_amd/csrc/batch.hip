@@ -567,8 +567,9 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
 // Only that k-th d2 is carried between queries (the previous position is re-read).  A wave's sums are
 // deferred over its runs (DeferredMeans).  The launch has a wave per run or two: waves that finish early are
 // replaced by new ones, which balances uneven query costs better than long-lived waves striding over runs.
-__device__ __forceinline__ void sor_run(const SegArrays &A, int k, uint32_t run, uint32_t pw, uint32_t total,
-                                        WaveLds &L, SumLds &S, DeferredMeans &dm, float *__restrict__ means) {
+__device__ __forceinline__ void sor_run(const SegArrays &A, int k, int h0, uint32_t run, uint32_t pw,
+                                        uint32_t total, WaveLds &L, SumLds &S, DeferredMeans &dm,
+                                        float *__restrict__ means) {
   const uint32_t q1 = min(run * pw + pw, total);
   float pd2 = -1.0f;  // k-th d2 of query q - 1 when it is in the run (< 0: none)
   for (uint32_t q = run * pw; q < q1; ++q) {
@@ -582,7 +583,8 @@ __device__ __forceinline__ void sor_run(const SegArrays &A, int k, uint32_t run,
                       (pd2 >= 0.0f && q > A.off[c])
                           ? knn_bound_from(x, y, z, uniform(g.xs[q - 1]), uniform(g.ys[q - 1]), uniform(g.zs[q - 1]),
                                            pd2)
-                          : __builtin_inff());
+                          : __builtin_inff(),
+                      h0);
     const int l = lane();
     const int m = min(t.cnt, k + 1);
     // entries beyond m add +0, which leaves the sum unchanged
@@ -595,8 +597,9 @@ __device__ __forceinline__ void sor_run(const SegArrays &A, int k, uint32_t run,
 // at a time; 2: runs strided within per-XCD ranges (XcdStride); 0: runs strided over the launch's waves.  One
 // instantiation per distribution, so each carries only its own loop's registers.
 template <int WS>
-__global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, int forced, float *__restrict__ means,
-                                                      uint32_t *__restrict__ queue, uint32_t chunk) {
+__global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, int forced, int h0,
+                                                      float *__restrict__ means, uint32_t *__restrict__ queue,
+                                                      uint32_t chunk) {
   __shared__ WaveLds lds[QW];
   __shared__ SumLds sums[QW];
   const int w = wave_in_block();
@@ -609,16 +612,16 @@ __global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, int fo
     work.init(queue, nruns, chunk);
     uint32_t r0 = 0, r1 = 0;
     while (work.next(r0, r1))
-      for (uint32_t run = r0; run < r1; ++run) sor_run(A, k, run, pw, total, lds[w], sums[w], dm, means);
+      for (uint32_t run = r0; run < r1; ++run) sor_run(A, k, h0, run, pw, total, lds[w], sums[w], dm, means);
     work.finish();
   } else if constexpr (WS == 2) {  // XCD-contiguous ranges without atomics
     XcdStride it;
     it.init(nruns);
     uint32_t run = 0;
-    while (it.next(run)) sor_run(A, k, run, pw, total, lds[w], sums[w], dm, means);
+    while (it.next(run)) sor_run(A, k, h0, run, pw, total, lds[w], sums[w], dm, means);
   } else {
     for (uint32_t run = wave_id(); run < nruns; run += wave_count())
-      sor_run(A, k, run, pw, total, lds[w], sums[w], dm, means);
+      sor_run(A, k, h0, run, pw, total, lds[w], sums[w], dm, means);
   }
   dm.flush(sums[w], k, means);
 }
@@ -739,13 +742,13 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
       const size_t runs = ceil_div(n_bound, (size_t)knn_run_length(n_bound, forced));
       blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(res, ceil_div(runs, QW)));
       chunk = (uint32_t)std::max<size_t>(1, runs / ((size_t)blocks * QW * 2));
-      RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<1><<<blocks, QT, 0, s>>>(G.arrays(), k, forced, mp, queue,
+      RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<1><<<blocks, QT, 0, s>>>(G.arrays(), k, forced, tune_knn_start_cells(), mp, queue,
                                                                                chunk));
     } else if (mode == 2) {
-      RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<2><<<blocks, QT, 0, s>>>(G.arrays(), k, forced, mp, queue,
+      RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<2><<<blocks, QT, 0, s>>>(G.arrays(), k, forced, tune_knn_start_cells(), mp, queue,
                                                                                chunk));
     } else {
-      RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<0><<<blocks, QT, 0, s>>>(G.arrays(), k, forced, mp, queue,
+      RVP_LAUNCH("seg_sor_means_k", s, seg_sor_means_k<0><<<blocks, QT, 0, s>>>(G.arrays(), k, forced, tune_knn_start_cells(), mp, queue,
                                                                                chunk));
     }
   }

@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void normals_solve_k(const float *__restrict__
 
 // MODE 0: k nearest (N1, K = k+1 incl. self); MODE 1: radius list capped at K (N2)
 template <int MODE, bool STRIDED>
-__global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, float *__restrict__ rec) {
+__global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, int h0, float *__restrict__ rec) {
   __shared__ WaveLds lds[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
   constexpr bool STAGE = MODE == 1 && !STRIDED;
@@ -286,7 +286,8 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
       t.init(K);
       if (MODE == 0) {
         // consecutive queries: the previous one's K-th distance bounds this one's (knn_bound_from)
-        knn_search<false>(g, x, y, z, t, L, STRIDED ? __builtin_inff() : knn_bound_from(x, y, z, px, py, pz, pd2));
+        knn_search<false>(g, x, y, z, t, L, STRIDED ? __builtin_inff() : knn_bound_from(x, y, z, px, py, pz, pd2),
+                          h0);
         px = x;
         py = y;
         pz = z;
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
 // N1 / N2 over a batch: every query's neighbourhood within its own cloud (MODE 0: K nearest incl. self;
 // MODE 1: radius[c] list capped at K)
 template <int MODE, bool STRIDED>
-__global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius,
+__global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius, int h0,
                                                      float *__restrict__ rec, size_t n_bound) {
   __shared__ WaveLds lds[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
       if (MODE == 0) {
         // consecutive queries of one cloud: the previous one's K-th distance bounds this one's
         knn_search<false>(g, x, y, z, t, lds[w],
-                          (!STRIDED && c == pc) ? knn_bound_from(x, y, z, px, py, pz, pd2) : __builtin_inff());
+                          (!STRIDED && c == pc) ? knn_bound_from(x, y, z, px, py, pz, pd2) : __builtin_inff(), h0);
         pc = c;
         px = x;
         py = y;
@@ -463,10 +464,11 @@ int rvp_normals_knn_centroids(const float *x, const float *y, const float *z, si
   bool strided = false;
   RVP_TRY(batch_blocks<NB>(normals_topk_k<0, true>, n, &qb, &strided));
   const GridView g = G.view();
+  const int h0 = tune_knn_start_cells();
   if (strided)
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, true><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, rec.as<float>()));
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, true><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, h0, rec.as<float>()));
   else
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, false><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, rec.as<float>()));
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, false><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, h0, rec.as<float>()));
   return solve_normals(rec.as<float>(), n, nullptr, g.xs, g.ys, g.zs, g.perm, vpx, vpy, vpz, nx, ny, nz, fb, cx, cy,
                        cz, s);
 }
@@ -492,7 +494,7 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
   if (max_nn > 0 && max_nn <= MAX_TOPK) {
     // batches of NB consecutive queries with staged windows (as the batched radius normals)
     const unsigned qb = ceil_div(ceil_div(n, NB), QW);
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, false><<<qb, QT, 0, s>>>(g, max_nn, radius, rec.as<float>()));
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, false><<<qb, QT, 0, s>>>(g, max_nn, radius, 1, rec.as<float>()));
   } else {
     // unlimited (or a cap above 64): full lists, sorted by (d2, idx), truncated by the row offsets
     Tmp cnt, off, keys, pos;
@@ -533,11 +535,12 @@ int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const 
   bool strided = false;
   RVP_TRY(batch_blocks<NB>(seg_normals_k<0, true>, n_bound, &qb, &strided));
   const SegArrays A = G.arrays();
+  const int h0 = tune_knn_start_cells();
   if (strided)
-    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true><<<qb, QT, 0, s>>>(A, k + 1, nullptr, rec.as<float>(),
+    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, rec.as<float>(),
                                                                             n_bound));
   else
-    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false><<<qb, QT, 0, s>>>(A, k + 1, nullptr, rec.as<float>(),
+    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, rec.as<float>(),
                                                                              n_bound));
   return solve_normals(rec.as<float>(), n_bound, G.off + nclouds, A.xs, A.ys, A.zs, A.perm, vpx, vpy, vpz, nx, ny,
                        nz, fb, nullptr, nullptr, nullptr, s);
@@ -561,7 +564,7 @@ int rvp_batch_normals_radius(const float *x, const float *y, const float *z, con
   // strided batches that balance k-NN queries, also for small counts)
   const unsigned qb = ceil_div(ceil_div(n_bound, NB), QW);
   const SegArrays A = G.arrays();
-  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(A, max_nn, radius, rec.as<float>(),
+  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(A, max_nn, radius, 1, rec.as<float>(),
                                                                            n_bound));
   return solve_normals(rec.as<float>(), n_bound, G.off + nclouds, A.xs, A.ys, A.zs, A.perm, vpx, vpy, vpz, nx, ny,
                        nz, fb, nullptr, nullptr, nullptr, s);

@@ -320,7 +320,8 @@ __global__ __launch_bounds__(QT) void knn_k(GridView g, const float *__restrict_
 // query of a run on, the previous query's k-th distance bounds the current one (knn_bound_from), so the first
 // box is already the certified one and only hits within the bound are staged.  The sums of a wave's queries
 // are deferred (DeferredMeans); the launch has one wave per run.
-__global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, int per_wave, float *__restrict__ means) {
+__global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, int per_wave, int h0,
+                                                      float *__restrict__ means) {
   __shared__ WaveLds lds[QW];
   __shared__ SumLds sums[QW];
   const int w = wave_in_block();
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, int per
       const float x = uniform(g.xs[q]), y = uniform(g.ys[q]), z = uniform(g.zs[q]);
       TopKT<uint32_t> t;  // distances only: the means read the multiset of the k+1 smallest
       t.init(k + 1);
-      knn_search<false>(g, x, y, z, t, lds[w], knn_bound_from(x, y, z, px, py, pz, pd2));
+      knn_search<false>(g, x, y, z, t, lds[w], knn_bound_from(x, y, z, px, py, pz, pd2), h0);
       const int l = lane();
       const int m = min(t.cnt, k + 1);
       // entries beyond m add +0, which leaves the sum unchanged
@@ -580,7 +581,8 @@ int rvp_sor_exact(const float *x, const float *y, const float *z, size_t n, int 
     mp = mt.as<float>();
   }
   const int pw = knn_run_length(n, tune_queries_per_wave());
-  RVP_LAUNCH("sor_knn_means_k", s, sor_knn_means_k<<<ceil_div(ceil_div(n, pw), QW), QT, 0, s>>>(G.view(), k, pw, mp));
+  RVP_LAUNCH("sor_knn_means_k", s, sor_knn_means_k<<<ceil_div(ceil_div(n, pw), QW), QT, 0, s>>>(G.view(), k, pw,
+                                                                                          tune_knn_start_cells(), mp));
   return sor_tail(G, x, y, z, n, alpha, mp, out_aos, n_out, keep, s);
 }
 

@@ -439,14 +439,14 @@ __device__ __forceinline__ float knn_bound_from(float qx, float qy, float qz, fl
   return __fmul_rn(__fmul_rn(sm, sm), 1.0f + 0x1p-16f);
 }
 
-// Exact k nearest by (d2, idx): 3x3x3 cells around the query, doubling growth until k candidates are
-// known, then the box certified by window3(q, R) with fl(R*R) > d2_k: every point outside it has a
-// larger d2, so the k best are final.  With a finite bound_d2 (at least the K-th smallest d2), the first
+// Exact k nearest by (d2, idx): the cells within h0 of the query's cell (3x3x3 for h0 = 1), doubling growth
+// until k candidates are known, then the box certified by window3(q, R) with fl(R*R) > d2_k: every point
+// outside it has a larger d2, so the k best are final.  With a finite bound_d2 (at least the K-th smallest d2), the first
 // box is the one certified for bound_d2 and only hits within it are staged: once it yields K entries the
 // certification holds at once; if it yields fewer, the search starts over without the bound.
 template <bool PAY, class KT>
 __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy, float qz, TopKT<KT> &t,
-                                           WaveLds &L, float bound_d2 = __builtin_inff()) {
+                                           WaveLds &L, float bound_d2 = __builtin_inff(), int h0 = 1) {
   const int gd[3] = {g.gx, g.gy, g.gz};
   int lo[3], hi[3];
   TopKSink<PAY, KT> sink{t, L};
@@ -463,8 +463,8 @@ __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy
                       floor_clamped(cell_coord(qz, g.mnz, g.icell), g.gz)};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      lo[a] = max(c[a] - 1, 0);
-      hi[a] = min(c[a] + 1, gd[a] - 1);
+      lo[a] = max(c[a] - h0, 0);
+      hi[a] = min(c[a] + h0, gd[a] - 1);
     }
   }
   scan_box(g, qx, qy, qz, lo, hi, false, lo, hi, 0.0f, false, L, sink);

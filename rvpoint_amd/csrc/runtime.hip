@@ -247,7 +247,9 @@ static std::atomic<double> g_seg_cells_per_point{32.0};
 static std::atomic<int> g_queries_per_wave{0};
 static std::atomic<double> g_seg_scan_table_min{4194304.0};
 static std::atomic<int> g_work_steal{0};
+static std::atomic<int> g_knn_start_cells{1};
 int tune_work_steal() { return g_work_steal.load(); }
+int tune_knn_start_cells() { return g_knn_start_cells.load(); }
 double tune_knn_cell_div() { return g_knn_cell_div.load(); }
 double tune_seg_cells_per_point() { return g_seg_cells_per_point.load(); }
 int tune_queries_per_wave() { return g_queries_per_wave.load(); }
@@ -504,6 +506,12 @@ int rvp_tune_set(const char *name, double value) {
     rvp::g_work_steal.store((int)value);
     return RVP_OK;
   }
+  if (std::string(name) == "knn_start_cells") {
+    RVP_CHECK(value >= 1.0 && value <= 8.0 && value == (double)(int)value, RVP_E_ARG,
+              "knn_start_cells must be an integer in [1, 8]");
+    rvp::g_knn_start_cells.store((int)value);
+    return RVP_OK;
+  }
   return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);
 }
 
@@ -527,6 +535,10 @@ int rvp_tune_get(const char *name, double *value) {
   }
   if (std::string(name) == "work_steal") {
     *value = (double)rvp::g_work_steal.load();
+    return RVP_OK;
+  }
+  if (std::string(name) == "knn_start_cells") {
+    *value = (double)rvp::g_knn_start_cells.load();
     return RVP_OK;
   }
   return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);

@@ -72,7 +72,8 @@ def test_tuning_knobs_set_get_and_ranges(built):
     from rvpoint_amd import _lib
     L = _lib.lib()
     for name, good, bad in (("knn_cell_div", 1.7, 20.0), ("seg_cells_per_point", 8.0, 0.5),
-                            ("queries_per_wave", 7, 65), ("seg_scan_table_min", 1000.0, -1.0)):
+                            ("queries_per_wave", 7, 65), ("seg_scan_table_min", 1000.0, -1.0),
+                            ("knn_start_cells", 3, 9)):
         old = _lib.tune(name)
         try:
             assert _lib.tune(name, good) == good
@@ -84,6 +85,8 @@ def test_tuning_knobs_set_get_and_ranges(built):
     assert _lib.tune("knn_cell_div") == 2.4 and _lib.tune("queries_per_wave") == 0
     with pytest.raises(_lib.RvpError):
         _lib.tune("queries_per_wave", 2.5)
+    with pytest.raises(_lib.RvpError):
+        _lib.tune("knn_start_cells", 1.5)
     with pytest.raises(_lib.RvpError):
         _lib.tune("no_such_knob")
     assert L.rvp_tune_get(None, None) == _lib.RVP_E_ARG

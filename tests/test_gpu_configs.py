@@ -84,22 +84,26 @@ def c3_subset():
     return clouds, scales
 
 
-@pytest.mark.parametrize("per_wave,steal", [(0, 0), (0, 1), (0, 2), (16, 1), (64, 2)])
-def test_batch_subset_stages_vs_oracle(c3_subset, per_wave, steal):
+@pytest.mark.parametrize("per_wave,steal,start", [(0, 0, 1), (0, 1, 1), (0, 2, 1), (16, 1, 1), (64, 2, 1),
+                                                  (0, 0, 3), (4, 2, 2)])
+def test_batch_subset_stages_vs_oracle(c3_subset, per_wave, steal, start):
     """SOR k=32 -> radius r=0.02*scale capped at 64 on the kept points -> kNN normals k=24, as the batched leg
     of the benchmark; per_wave 0 = the run length chosen from the count, else forced; steal: persistent waves with
-    XCD-local runs and stealing (1), runs strided within per-XCD ranges (2), or over the launch (0)."""
+    XCD-local runs and stealing (1), runs strided within per-XCD ranges (2), or over the launch (0); start: the
+    half-width in cells of an unbounded k-NN search's first box."""
     clouds, scales = c3_subset
     b = R.Batch(clouds)
-    old = R._lib.tune("queries_per_wave"), R._lib.tune("work_steal")
+    knobs = ("queries_per_wave", "work_steal", "knn_start_cells")
+    old = [R._lib.tune(n) for n in knobs]
     try:
-        R._lib.tune("queries_per_wave", per_wave)
-        R._lib.tune("work_steal", steal)
+        for n, v in zip(knobs, (per_wave, steal, start)):
+            R._lib.tune(n, v)
         kept, means = R.batch_sor_exact(b, 32, 1.0)
         kept2, means2 = R.batch_sor_exact(b, 32, 1.0)  # the work counters start from zero again
+        nrm_start = R.batch_normals_knn(R.Batch(kept), 24) if start != 1 else None
     finally:
-        R._lib.tune("queries_per_wave", old[0])
-        R._lib.tune("work_steal", old[1])
+        for n, v in zip(knobs, old):
+            R._lib.tune(n, v)
     assert (means2 == means).all() and all((a == c).all() for a, c in zip(kept, kept2))
     okept = []
     for c, p in enumerate(clouds):
@@ -107,7 +111,11 @@ def test_batch_subset_stages_vs_oracle(c3_subset, per_wave, steal):
         assert (b.split(means)[c] == omeans).all(), c
         assert kept[c].shape == oout.shape and (kept[c] == oout).all(), c
         okept.append(oout)
-    if per_wave or steal:
+    if start != 1:
+        # the k-NN normals of the same kept points with the default first box: the same neighbourhoods
+        assert all((a == c).all() for a, c in zip(R.Batch(kept).split(nrm_start),
+                                                  R.Batch(kept).split(R.batch_normals_knn(R.Batch(kept), 24))))
+    if per_wave or steal or start != 1:
         return
     kb = R.Batch(okept)
     radius = np.array([0.02 * s for s in scales], np.float32)
@@ -127,16 +135,19 @@ def test_batch_subset_stages_vs_oracle(c3_subset, per_wave, steal):
     assert rep["angle_deg_median"] <= 0.01 and rep["angle_deg_p99"] <= 1.0
 
 
-@pytest.mark.parametrize("per_wave", [16, 64])
-def test_single_cloud_sor_runs_of_queries(per_wave):
-    """Single-cloud exact SOR with forced runs of consecutive queries per wave."""
+@pytest.mark.parametrize("per_wave,start", [(16, 1), (64, 1), (2, 3), (0, 8)])
+def test_single_cloud_sor_runs_of_queries(per_wave, start):
+    """Single-cloud exact SOR with forced runs of consecutive queries per wave and first k-NN boxes of
+    half-width `start` cells."""
     p = scenes.plane_sphere_cloud(6000, seed=77)
-    old = R._lib.tune("queries_per_wave")
+    old = R._lib.tune("queries_per_wave"), R._lib.tune("knn_start_cells")
     try:
         R._lib.tune("queries_per_wave", per_wave)
+        R._lib.tune("knn_start_cells", start)
         out, means = R.sor_sc(p, 30, 1.0, with_means=True)
     finally:
-        R._lib.tune("queries_per_wave", old)
+        R._lib.tune("queries_per_wave", old[0])
+        R._lib.tune("knn_start_cells", old[1])
     oout, omeans = O.sor_sc(p, 30, 1.0)
     assert (means == omeans).all()
     assert out.shape == oout.shape and (out == oout).all()
