@@ -92,8 +92,6 @@ int rvp_profile_collect(char *buf, size_t buflen, size_t *needed);
  *   "queries_per_wave"     consecutive queries one wave serves in the capped radius searches and the k-NN
  *                          SOR means (0 = from the query count: radius 64 for a few million queries and more,
  *                          fewer below; SOR means 2 up to a quarter million, at most 16; [0, 64])
- *   "seg_scan_table_min"   batches of at least this many points (their bound) build the cell table from run
- *                          lengths and one scan; smaller ones fill the gaps between runs (default 4194304)
  *   "work_steal"           how the batched k-NN SOR means hand out runs of queries: 0 strided over the launch's
  *                          waves; 1 persistent waves taking runs from per-XCD ranges by atomic counters and
  *                          stealing when theirs is drained; 2 per-XCD ranges strided without atomics

@@ -47,12 +47,10 @@ __device__ __forceinline__ float ord2f(uint32_t u) {
 }
 
 // The first launch of a grid build: the caller's offsets made a valid partition (wave 0 of block 0, as
-// seg_offsets_k), the integer bbox at its identity, `words` 32-bit words of `zero` and the word zero1 cleared.
+// seg_offsets_k), the integer bbox at its identity and `words` 32-bit words of `zero` cleared.
 __global__ __launch_bounds__(256) void seg_prep_k(const uint32_t *__restrict__ in, int ncl, uint32_t n_bound,
                                                   uint32_t *__restrict__ off, uint32_t *__restrict__ mmu,
-                                                  uint32_t *__restrict__ zero, size_t words,
-                                                  uint32_t *__restrict__ zero1) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *zero1 = 0u;
+                                                  uint32_t *__restrict__ zero, size_t words) {
   if (blockIdx.x == 0 && threadIdx.x < WAVE) {
     const int l = lane();
     uint32_t carry = 0;
@@ -269,7 +267,7 @@ __global__ __launch_bounds__(LAYOUT_MAX) void seg_layout_k(const uint32_t *__res
                                                            const uint32_t *__restrict__ off, int ncl,
                                                            const float *__restrict__ cell, double per_point,
                                                            SegDesc *__restrict__ desc, uint32_t *__restrict__ ncells,
-                                                           uint32_t *__restrict__ toff, int *__restrict__ key_bits) {
+                                                           uint32_t *__restrict__ toff) {
   const int c = threadIdx.x;
   SegDesc d{};
   uint32_t nc = 0;
@@ -299,154 +297,69 @@ __global__ __launch_bounds__(LAYOUT_MAX) void seg_layout_k(const uint32_t *__res
     desc[c] = d;
     toff[c] = d.toff;
   }
-  if (c == 0) {
-    toff[ncl] = total;
-    *key_bits = total > 1 ? 32 - __clz((int)(total - 1)) : 0;
-  }
+  if (c == 0) toff[ncl] = total;
 }
 
-// also the width of the largest cell key, total - 1, for the key sort
-__global__ void seg_toff_k(const uint32_t *__restrict__ toff, int ncl, SegDesc *__restrict__ desc,
-                           int *__restrict__ key_bits) {
+__global__ void seg_toff_k(const uint32_t *__restrict__ toff, int ncl, SegDesc *__restrict__ desc) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < ncl) desc[c].toff = toff[c];
-  if (c == 0) {
-    const uint32_t total = toff[ncl];
-    *key_bits = total > 1 ? 32 - __clz((int)(total - 1)) : 0;
-  }
 }
 
-// also each point as one 16-byte record in input order, so that the gather after the sort reads one sector
-// per point instead of three
-__global__ void seg_keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-                           const uint32_t *__restrict__ off, int ncl, const SegDesc *__restrict__ desc, uint32_t n_bound,
-                           uint32_t pad_key, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                           float4 *__restrict__ xyz) {
+// Counting sort by cell (cell order within a cloud, clouds in order): each point's cell key (its cloud's table
+// offset plus the dense cell id) and its slot among the cell's points, taken with an atomic on the cell's count.
+// The order of points within a cell is that of the atomics; no result depends on it (every neighbour list is
+// ordered by (d2, index), every sum over a list follows that order), only which wave serves which query.
+// Spatially ordered input puts runs of one cell on consecutive lanes: each run takes its slots with one atomic
+// from its first lane (same-address atomics of a wave would serialise in the L2).
+__global__ void seg_count_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+                            const uint32_t *__restrict__ off, int ncl, const SegDesc *__restrict__ desc,
+                            uint32_t *__restrict__ keys, uint32_t *__restrict__ slot, uint32_t *__restrict__ cnt) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_bound) return;
-  vals[i] = i;
-  if (i >= off[ncl]) {  // slots past the batch sort to the end
-    keys[i] = pad_key;
-    return;
-  }
+  // the valid lanes of a wave are a prefix of it (i grows with the lane)
+  if (i >= off[ncl]) return;
   const int c = cloud_of(off, ncl, i);
   const SegDesc d = desc[c];
-  const float a = x[i], b = y[i], e = z[i];
-  keys[i] = d.toff + (uint32_t)cell_clamped(a, d.mn[0], d.cell, d.g[0]) +
-            (uint32_t)d.g[0] * ((uint32_t)cell_clamped(b, d.mn[1], d.cell, d.g[1]) +
-                                (uint32_t)d.g[1] * (uint32_t)cell_clamped(e, d.mn[2], d.cell, d.g[2]));
-  xyz[i] = make_float4(a, b, e, 0.0f);
+  const uint32_t k = d.toff + (uint32_t)cell_clamped(x[i], d.mn[0], d.cell, d.g[0]) +
+                     (uint32_t)d.g[0] * ((uint32_t)cell_clamped(y[i], d.mn[1], d.cell, d.g[1]) +
+                                         (uint32_t)d.g[1] * (uint32_t)cell_clamped(z[i], d.mn[2], d.cell, d.g[2]));
+  keys[i] = k;
+  const int l = lane();
+  // every wave-wide value is taken here, outside any branch: a ballot under a condition sees only the lanes
+  // that take it
+  const int active = popc(ballot(true));
+  const uint32_t kprev = bperm(k, up_addr(1));
+  const uint64_t heads = ballot(l == 0 || kprev != k);
+  const uint64_t upto = (2ull << l) - 1ull;  // lanes 0..l (all 64 for l = 63)
+  const int h = 63 - __clzll((long long)(heads & upto));  // first lane of this lane's run
+  const uint64_t later = heads & ~upto;
+  const int end = later ? __ffsll((unsigned long long)later) - 1 : active;
+  uint32_t base = 0;
+  if (l == h) base = atomicAdd(&cnt[k], (uint32_t)(end - h));
+  slot[i] = bperm(base, h << 2) + (uint32_t)(l - h);
 }
 
-// cell-order SoA, and the inverse permutation rank[perm[i]] = i
-__global__ void seg_gather_k(const float4 *__restrict__ xyz, const uint32_t *__restrict__ perm,
-                             const uint32_t *__restrict__ off, int ncl, float *__restrict__ xs,
-                             float *__restrict__ ys, float *__restrict__ zs, uint32_t *__restrict__ rank) {
+// after the scan cs[k] is the first cell-order position of cell k: point i goes to cs[key] + slot (cell-order
+// SoA, perm = cell order -> batch position, rank = its inverse)
+__global__ void seg_scatter_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+                              const uint32_t *__restrict__ off, int ncl, const uint32_t *__restrict__ keys,
+                              const uint32_t *__restrict__ slot, const uint32_t *__restrict__ cs,
+                              float *__restrict__ xs, float *__restrict__ ys, float *__restrict__ zs,
+                              uint32_t *__restrict__ perm, uint32_t *__restrict__ rank) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= off[ncl]) return;
-  const uint32_t p = perm[i];
-  const float4 v = xyz[p];
-  xs[i] = v.x;
-  ys[i] = v.y;
-  zs[i] = v.z;
-  rank[p] = i;
+  const uint32_t p = cs[keys[i]] + slot[i];
+  xs[p] = x[i];
+  ys[p] = y[i];
+  zs[p] = z[i];
+  perm[p] = i;
+  rank[i] = p;
 }
 
-struct SegGap {
-  uint32_t lo, hi, val;
-};
-constexpr uint32_t SEG_SHORT_GAP = 32;
-constexpr uint32_t SEG_GAP_CHUNK = 8192;  // cells per fill record, so that long empty runs spread over blocks
-
-// records [lo, hi] -> val in pieces of at most SEG_GAP_CHUNK cells
-__device__ __forceinline__ void push_gap(SegGap *gaps, uint32_t *ngaps, uint32_t lo, uint32_t hi, uint32_t val) {
-  const uint32_t pieces = (hi - lo) / SEG_GAP_CHUNK + 1;
-  const uint32_t g = atomicAdd(ngaps, pieces);
-  for (uint32_t p = 0; p < pieces; ++p) {
-    const uint32_t a = lo + p * SEG_GAP_CHUNK;
-    gaps[g + p] = SegGap{a, min(hi, a + SEG_GAP_CHUNK - 1), val};
-  }
-}
-
-// cstart[g] = first cell-order position whose key >= g, for g in [0, table_size]
-// (queries read cs up to the last cloud's end, total_cells = toff[ncl]; the table beyond stays unwritten)
-__global__ void seg_cell_start_k(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ off, int ncl,
-                                 const uint32_t *__restrict__ total_cells, uint32_t *__restrict__ cs,
-                                 SegGap *__restrict__ gaps, uint32_t *__restrict__ ngaps) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t n = off[ncl];
-  const uint32_t table_size = *total_cells;
-  if (n == 0) {
-    if (i == 0) push_gap(gaps, ngaps, 0, table_size, 0);
-    return;
-  }
-  if (i >= n) return;
-  const uint32_t k = keys[i];
-  const uint32_t lo = i == 0 ? 0u : keys[i - 1] + 1;
-  const bool start = i == 0 || k != keys[i - 1];
-  const bool long_gap = start && k - lo + 1 > SEG_SHORT_GAP;
-  // the wave's long gaps take their queue slots with one atomic (a counter shared by the whole launch
-  // serialises per-thread atomics)
-  const uint32_t pieces = long_gap ? (k - lo) / SEG_GAP_CHUNK + 1 : 0u;
-  const uint32_t inc = wave_incl_scan(pieces);
-  const uint64_t active = ballot(true);
-  const int leader = __ffsll((unsigned long long)active) - 1, last = 63 - __clzll((long long)active);
-  const uint32_t wave_pieces = readlaneu(inc, last);
-  uint32_t gbase = 0;
-  if (lane() == leader && wave_pieces) gbase = atomicAdd(ngaps, wave_pieces);
-  gbase = readlaneu(gbase, leader) + inc - pieces;
-  if (start) {
-    if (!long_gap) {
-      for (uint32_t c = lo; c <= k; ++c) cs[c] = i;
-    } else {
-      for (uint32_t p = 0; p < pieces; ++p) {
-        const uint32_t a = lo + p * SEG_GAP_CHUNK;
-        gaps[gbase + p] = SegGap{a, min(k, a + SEG_GAP_CHUNK - 1), i};
-      }
-    }
-  }
-  if (i == n - 1) push_gap(gaps, ngaps, k + 1, table_size, n);
-}
-
-__global__ void seg_fill_gaps_k(const SegGap *__restrict__ gaps, const uint32_t *__restrict__ ngaps,
-                                uint32_t *__restrict__ cs) {
-  const uint32_t ng = *ngaps;
-  for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x) {
-    const SegGap gp = gaps[g];
-    for (uint64_t c = (uint64_t)gp.lo + threadIdx.x; c <= gp.hi; c += blockDim.x) cs[c] = gp.val;
-  }
-}
-
-// The cell-start table of the batch: cstart[g] = first cell-order position whose key >= g, for g in
-// [0, table_size] (table_size = toff[ncl]; queries read no further).  The table is zeroed, each run of equal
-// keys writes its length at its key, and one exclusive scan turns the counts into starts.
+// the cell-count table cleared up to the batch's table size (toff[ncl]) plus the end entry
 __global__ void seg_zero_table_k(uint32_t *__restrict__ cs, size_t bound, const uint32_t *__restrict__ total_cells) {
   const size_t m = min((size_t)*total_cells + 1, bound);
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
     cs[i] = 0u;
-}
-
-__global__ void seg_run_lengths_k(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ off, int ncl,
-                                  uint32_t *__restrict__ cs) {
-  const uint32_t n = off[ncl];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = keys[i];
-  if (i > 0 && keys[i - 1] == k) return;
-  uint32_t lo = i, hi, step = 1;  // keys[lo] == k; hi: first position past the run
-  for (;;) {
-    hi = lo + step;
-    if (hi >= n || keys[hi] != k) break;
-    lo = hi;
-    step <<= 1;
-  }
-  if (hi > n) hi = n;
-  while (hi - lo > 1) {
-    const uint32_t m = lo + (hi - lo) / 2;
-    if (keys[m] == k) lo = m;
-    else hi = m;
-  }
-  cs[k] = hi - i;
 }
 
 int SegGrid::build(const float *x, const float *y, const float *z, const uint32_t *off_, int ncl_, size_t n_bound_,
@@ -464,17 +377,15 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   RVP_TRY(cellv.alloc(ncl * sizeof(float), s));
   RVP_TRY(ncells.alloc((ncl + 1) * sizeof(uint32_t), s));
   RVP_TRY(toff.alloc((ncl + 1) * sizeof(uint32_t), s));
-  Tmp mmu, bits, ng, key_bits;
+  Tmp mmu, bits;
   RVP_TRY(mmu.alloc(ncl * 6 * sizeof(uint32_t), s));
-  RVP_TRY(ng.alloc(sizeof(uint32_t), s));
-  RVP_TRY(key_bits.alloc(sizeof(int), s));
   const size_t bit_words = mode_knn ? (size_t)ncl * OCC_BITS / 4 : 0;
   if (mode_knn) RVP_TRY(bits.alloc((size_t)ncl * OCC_BITS, s));
-  // offsets clamp, bbox identity, occupancy bytes and the gap counter cleared: one launch
-  const unsigned pb = (unsigned)std::min<size_t>(1024, std::max<size_t>(1, ceil_div(std::max<size_t>(6 * (size_t)ncl, bit_words), 256)));
+  // offsets clamp, bbox identity and occupancy bytes cleared: one launch
+  const unsigned pb = (unsigned)std::min<size_t>(
+      1024, std::max<size_t>(1, ceil_div(std::max<size_t>(6 * (size_t)ncl, bit_words), 256)));
   RVP_LAUNCH("seg_prep_k", s, seg_prep_k<<<pb, 256, 0, s>>>(off_, ncl, (uint32_t)n_bound, offs.as<uint32_t>(),
-                                                            mmu.as<uint32_t>(), bits.as<uint32_t>(), bit_words,
-                                                            ng.as<uint32_t>()));
+                                                            mmu.as<uint32_t>(), bits.as<uint32_t>(), bit_words));
   const unsigned per = (unsigned)std::min<size_t>(64, std::max<size_t>(1, n_bound / ((size_t)ncl * 4096)));
   RVP_LAUNCH("seg_minmax_k", s, seg_minmax_k<<<dim3(per, ncl), 256, 0, s>>>(x, y, z, off, mmu.as<uint32_t>()));
   const float *cells = cell_dev;
@@ -495,8 +406,7 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
     // the k-NN path has the float bbox already (seg_knn_c0_k)
     RVP_LAUNCH("seg_layout_k", s, seg_layout_k<<<1, LAYOUT_MAX, 0, s>>>(
                                       mode_knn ? nullptr : mmu.as<uint32_t>(), mm.as<float>(), off, ncl, cells,
-                                      per_point, desc.as<SegDesc>(), ncells.as<uint32_t>(), toff.as<uint32_t>(),
-                                      key_bits.as<int>()));
+                                      per_point, desc.as<SegDesc>(), ncells.as<uint32_t>(), toff.as<uint32_t>()));
   } else {
     if (!mode_knn)
       RVP_LAUNCH("seg_minmax_final_k", s, seg_minmax_final_k<<<ceil_div(6 * ncl, 256), 256, 0, s>>>(
@@ -504,57 +414,35 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
     RVP_LAUNCH("seg_geom_k", s, seg_geom_k<<<cb, 64, 0, s>>>(mm.as<float>(), off, ncl, cells, per_point,
                                                              desc.as<SegDesc>(), ncells.as<uint32_t>()));
     RVP_TRY(exclusive_scan_u32(ncells.as<uint32_t>(), toff.as<uint32_t>(), ncl, toff.as<uint32_t>() + ncl, s));
-    RVP_LAUNCH("seg_toff_k", s, seg_toff_k<<<cb, 64, 0, s>>>(toff.as<uint32_t>(), ncl, desc.as<SegDesc>(),
-                                                             key_bits.as<int>()));
+    RVP_LAUNCH("seg_toff_k", s, seg_toff_k<<<cb, 64, 0, s>>>(toff.as<uint32_t>(), ncl, desc.as<SegDesc>()));
   }
   // the table holds at most sum(max(SEG_MIN_CELLS, per_point n_c)) cells
   const uint64_t table = (uint64_t)std::ceil(per_point * (double)n_bound) + (uint64_t)SEG_MIN_CELLS * (uint64_t)ncl;
   RVP_CHECK(table < 0xfff00000ull, RVP_E_UNSUPPORTED, "batch cell table too large");
-  const uint32_t n32 = (uint32_t)n_bound;
-  Tmp kt, vt;
-  RVP_TRY(keys.alloc(n_bound * sizeof(uint32_t), s));
-  RVP_TRY(perm.alloc(n_bound * sizeof(uint32_t), s));
-  RVP_TRY(kt.alloc(n_bound * sizeof(uint32_t), s));
-  RVP_TRY(vt.alloc(n_bound * sizeof(uint32_t), s));
+  // counting sort by cell: counts (and each point's slot) in the table, one scan of the table turns the counts
+  // into the cell starts (cstart[k] = first cell-order position of cell k, for k in [0, toff[ncl]]), scatter
+  const uint32_t *total_cells = toff.as<uint32_t>() + ncl;
   const unsigned nb = ceil_div(n_bound, 256);
-  Tmp xyz;
-  RVP_TRY(xyz.alloc(n_bound * sizeof(float4), s));
-  RVP_LAUNCH("seg_keys_k", s, seg_keys_k<<<nb, 256, 0, s>>>(x, y, z, off, ncl, desc.as<SegDesc>(), n32,
-                                                            (uint32_t)table, keys.as<uint32_t>(), perm.as<uint32_t>(),
-                                                            xyz.as<float4>()));
-  uint32_t *kp = keys.as<uint32_t>(), *vp = perm.as<uint32_t>(), *ktp = kt.as<uint32_t>(), *vtp = vt.as<uint32_t>();
-  RVP_TRY(radix_sort_pairs_u32(&kp, &vp, &ktp, &vtp, n_bound, bits_for(table), s, off + ncl, key_bits.as<int>()));
-  if (kp != keys.as<uint32_t>()) {
-    keys.swap(kt);
-    perm.swap(vt);
-  }
+  Tmp keys, slot;
+  RVP_TRY(cstart.alloc((table + 1) * sizeof(uint32_t), s));
+  RVP_TRY(keys.alloc(n_bound * sizeof(uint32_t), s));
+  RVP_TRY(slot.alloc(n_bound * sizeof(uint32_t), s));
+  RVP_LAUNCH("seg_zero_table_k", s, seg_zero_table_k<<<(unsigned)std::min<size_t>(4096, ceil_div(table + 1, 256)), 256, 0, s>>>(
+                                        cstart.as<uint32_t>(), table + 1, total_cells));
+  RVP_LAUNCH("seg_count_k", s, seg_count_k<<<nb, 256, 0, s>>>(x, y, z, off, ncl, desc.as<SegDesc>(),
+                                                              keys.as<uint32_t>(), slot.as<uint32_t>(),
+                                                              cstart.as<uint32_t>()));
+  RVP_TRY(exclusive_scan_u32_bounded(cstart.as<uint32_t>(), cstart.as<uint32_t>(), table + 1, total_cells, 1, nullptr,
+                                     s));
   RVP_TRY(xs.alloc(n_bound * sizeof(float), s));
   RVP_TRY(ys.alloc(n_bound * sizeof(float), s));
   RVP_TRY(zs.alloc(n_bound * sizeof(float), s));
+  RVP_TRY(perm.alloc(n_bound * sizeof(uint32_t), s));
   RVP_TRY(rank.alloc(n_bound * sizeof(uint32_t), s));
-  RVP_LAUNCH("seg_gather_k", s, seg_gather_k<<<nb, 256, 0, s>>>(xyz.as<float4>(), perm.as<uint32_t>(), off, ncl,
-                                                                xs.as<float>(), ys.as<float>(), zs.as<float>(),
-                                                                rank.as<uint32_t>()));
-  RVP_TRY(cstart.alloc((table + 1) * sizeof(uint32_t), s));
-  const uint32_t *total_cells = toff.as<uint32_t>() + ncl;
-  if ((double)n_bound >= tune_seg_scan_table_min()) {
-    // run lengths and one streaming scan of the table
-    RVP_LAUNCH("seg_zero_table_k", s, seg_zero_table_k<<<(unsigned)std::min<size_t>(4096, ceil_div(table + 1, 256)), 256, 0, s>>>(
-                                          cstart.as<uint32_t>(), table + 1, total_cells));
-    RVP_LAUNCH("seg_run_lengths_k", s, seg_run_lengths_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), off, ncl,
-                                                                            cstart.as<uint32_t>()));
-    RVP_TRY(exclusive_scan_u32_bounded(cstart.as<uint32_t>(), cstart.as<uint32_t>(), table + 1, total_cells, 1,
-                                       nullptr, s));
-  } else {
-    // small batches: gaps filled by the run that ends them (long ones queued), no pass over the whole table
-    Tmp gaps;
-    RVP_TRY(gaps.alloc((n_bound + 2 + table / SEG_GAP_CHUNK + 2) * sizeof(SegGap), s));
-    RVP_LAUNCH("seg_cell_start_k", s, seg_cell_start_k<<<nb, 256, 0, s>>>(keys.as<uint32_t>(), off, ncl, total_cells,
-                                                                          cstart.as<uint32_t>(), gaps.as<SegGap>(),
-                                                                          ng.as<uint32_t>()));
-    RVP_LAUNCH("seg_fill_gaps_k", s, seg_fill_gaps_k<<<1024, 256, 0, s>>>(gaps.as<SegGap>(), ng.as<uint32_t>(),
-                                                                          cstart.as<uint32_t>()));
-  }
+  RVP_LAUNCH("seg_scatter_k", s, seg_scatter_k<<<nb, 256, 0, s>>>(x, y, z, off, ncl, keys.as<uint32_t>(),
+                                                                  slot.as<uint32_t>(), cstart.as<uint32_t>(),
+                                                                  xs.as<float>(), ys.as<float>(), zs.as<float>(),
+                                                                  perm.as<uint32_t>(), rank.as<uint32_t>()));
   return RVP_OK;
 }
 

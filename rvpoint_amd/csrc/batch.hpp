@@ -185,7 +185,7 @@ int seg_clamp_offsets(const uint32_t *in, int ncl, size_t n_bound, uint32_t *out
 
 // Host owner of a segmented grid (stream-ordered temporaries).
 struct SegGrid {
-  Tmp desc, mm, cellv, ncells, toff, xs, ys, zs, perm, rank, keys, cstart, offs;
+  Tmp desc, mm, cellv, ncells, toff, xs, ys, zs, perm, rank, cstart, offs;
   int ncl = 0;
   const uint32_t *off = nullptr;  // the caller's offsets made non-decreasing, off[0] = 0, off[ncl] <= n_bound
   size_t n_bound = 0;

@@ -115,7 +115,6 @@ double tune_knn_cell_div();
 // dense-table cells per point a batched grid may use (rvp_tune_set "seg_cells_per_point")
 double tune_seg_cells_per_point();
 int tune_queries_per_wave();  // 0: chosen from the query count
-double tune_seg_scan_table_min();  // batch bound (points) from which a batched grid's cell table is scanned
 int tune_work_steal();  // batched SOR runs: 0 strided, 1 per-XCD ranges with stealing, 2 per-XCD ranges strided
 int tune_knn_start_cells();  // half-width in cells of an unbounded k-NN search's first box
 // blocks of `threads` threads of `kernel` the current device holds at once (occupancy x CUs), cached

@@ -245,7 +245,6 @@ void prof_stop(const char *name, hipStream_t s, hipEvent_t a) {
 static std::atomic<double> g_knn_cell_div{2.4};
 static std::atomic<double> g_seg_cells_per_point{32.0};
 static std::atomic<int> g_queries_per_wave{0};
-static std::atomic<double> g_seg_scan_table_min{4194304.0};
 static std::atomic<int> g_work_steal{0};
 static std::atomic<int> g_knn_start_cells{1};
 int tune_work_steal() { return g_work_steal.load(); }
@@ -253,7 +252,6 @@ int tune_knn_start_cells() { return g_knn_start_cells.load(); }
 double tune_knn_cell_div() { return g_knn_cell_div.load(); }
 double tune_seg_cells_per_point() { return g_seg_cells_per_point.load(); }
 int tune_queries_per_wave() { return g_queries_per_wave.load(); }
-double tune_seg_scan_table_min() { return g_seg_scan_table_min.load(); }
 
 struct PinnedStage::Slot {
   std::mutex mu;
@@ -496,11 +494,6 @@ int rvp_tune_set(const char *name, double value) {
     rvp::g_queries_per_wave.store((int)value);
     return RVP_OK;
   }
-  if (std::string(name) == "seg_scan_table_min") {
-    RVP_CHECK(value >= 0.0 && value <= 4294967295.0, RVP_E_ARG, "seg_scan_table_min must be in [0, 2^32)");
-    rvp::g_seg_scan_table_min.store(value);
-    return RVP_OK;
-  }
   if (std::string(name) == "work_steal") {
     RVP_CHECK(value == 0.0 || value == 1.0 || value == 2.0, RVP_E_ARG, "work_steal must be 0, 1 or 2");
     rvp::g_work_steal.store((int)value);
@@ -527,10 +520,6 @@ int rvp_tune_get(const char *name, double *value) {
   }
   if (std::string(name) == "queries_per_wave") {
     *value = (double)rvp::g_queries_per_wave.load();
-    return RVP_OK;
-  }
-  if (std::string(name) == "seg_scan_table_min") {
-    *value = rvp::g_seg_scan_table_min.load();
     return RVP_OK;
   }
   if (std::string(name) == "work_steal") {

@@ -101,18 +101,43 @@ def test_batch_many_small_clouds_layout_fallback():
         assert kept[c].shape == oout.shape and (kept[c] == oout).all(), c
 
 
-@pytest.mark.parametrize("scan_min", [0, 1 << 40])
-def test_batch_cell_table_both_builds(clouds, scan_min):
-    """The batched cell table built from run lengths + a scan, and by gap filling: same radius lists."""
+def test_batch_hot_cells_and_repeats():
+    """The batched grid is a counting sort whose order of points within a cell comes from atomics: cells
+    holding 1,500 exact duplicates each, in shuffled input order, give the oracle's radius lists and SOR
+    output, and repeated calls give identical results."""
+    rng = np.random.default_rng(5)
+    base = rng.random((3000, 3)).astype(np.float32)
+    hot = np.concatenate([base, np.repeat(base[:3], 1500, axis=0)])
+    hot = hot[rng.permutation(len(hot))]
+    other = rng.random((2000, 3)).astype(np.float32) * np.float32(4.0)
+    cl = [hot, other]
+    b = R.Batch(cl)
+    radius = np.array([0.05, 0.2], np.float32)
+    idx, d2, cnt = R.batch_radius_search(b, radius, 64)
+    idx2, d22, cnt2 = R.batch_radius_search(b, radius, 64)
+    assert (idx == idx2).all() and (d2 == d22).all() and (cnt == cnt2).all()
+    kept, means = R.batch_sor_exact(b, 8, 1.0)
+    kept2, means2 = R.batch_sor_exact(b, 8, 1.0)
+    assert (means == means2).all() and all((a == c).all() for a, c in zip(kept, kept2))
+    for c, p in enumerate(cl):
+        x, y, z = soa(p)
+        oi, od, oc, _ = O.hash_radius(x, y, z, float(radius[c]), x, y, z, float(radius[c]), 64)
+        ci, cd, cc = b.split(idx)[c], b.split(d2)[c], b.split(cnt)[c]
+        assert (cc == oc).all(), c
+        for i in range(len(p)):
+            m = oc[i]
+            assert (ci[i, :m] == oi[i, :m]).all() and (cd[i, :m] == od[i, :m]).all(), (c, i)
+        oout, omeans = O.sor_sc(p, 8, 1.0)
+        assert (b.split(means)[c] == omeans).all(), c
+        assert kept[c].shape == oout.shape and (kept[c] == oout).all(), c
+
+
+def test_batch_radius_lists_again(clouds):
+    """The batch workload's radius lists against the oracle (a second grid build over the same clouds)."""
     cl, scales = clouds
     b = R.Batch(cl)
     radius = np.array([0.02 * s for s in scales], np.float32)
-    old = R._lib.tune("seg_scan_table_min")
-    try:
-        R._lib.tune("seg_scan_table_min", min(scan_min, 4294967295))
-        idx, d2, cnt = R.batch_radius_search(b, radius, 64)
-    finally:
-        R._lib.tune("seg_scan_table_min", old)
+    idx, d2, cnt = R.batch_radius_search(b, radius, 64)
     for c, p in enumerate(cl):
         if len(p) == 0:
             continue
