@@ -97,6 +97,8 @@ int rvp_profile_collect(char *buf, size_t buflen, size_t *needed);
  *                          stealing when theirs is drained; 2 per-XCD ranges strided without atomics
  *   "knn_start_cells"      half-width in cells of the first box of a k-NN search that has no bound from a
  *                          previous query (default 1: 3x3x3 cells; [1, 8]); the box then grows until certified
+ *   "normals_per_wave"     consecutive queries one wave serves in the radius normals, which share staged
+ *                          windows (0 = 8; [0, 64])
  * set() returns RVP_E_ARG for an unknown name or a value out of range. */
 int rvp_tune_set(const char *name, double value);
 int rvp_tune_get(const char *name, double *value);

@@ -3,7 +3,8 @@
 The knobs (rvp_tune_set) change only how fast the kernels run, never their results: the k-NN grid edge
 (knn_cell_div: the estimated k-th neighbour distance over this), the dense cells a batched grid may give a point
 (seg_cells_per_point), the query tile (queries_per_wave: consecutive queries one wave serves), how the batched
-SOR means hand out work (work_steal) and the first box of a k-NN search without a bound (knn_start_cells).  One wave serves one
+SOR means hand out work (work_steal), the first box of a k-NN search without a bound (knn_start_cells) and the
+queries per wave of the radius normals (normals_per_wave).  One wave serves one
 query at a time (64 lanes per query); that is fixed by the kernels' design.
 
 tune() measures a workload class on the GPU by coordinate descent: each knob in turn over its candidate values,
@@ -30,12 +31,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 
 DEFAULTS = {"knn_cell_div": 2.4, "seg_cells_per_point": 32.0, "queries_per_wave": 0.0,
-            "work_steal": 0.0, "knn_start_cells": 1.0}
+            "work_steal": 0.0, "knn_start_cells": 1.0, "normals_per_wave": 0.0}
 
 # candidate values per class, in the order the descent visits the knobs
 SPACE = {
     "lidar_frame": {"knn_cell_div": [1.6, 2.0, 2.4, 2.8, 3.2], "knn_start_cells": [1, 2, 3],
-                    "queries_per_wave": [0, 1, 2, 4],
+                    "queries_per_wave": [0, 1, 2, 4], "normals_per_wave": [0, 2, 3, 4, 6],
                     "work_steal": [0, 2], "seg_cells_per_point": [8, 16, 32, 64]},
     "batch": {"knn_cell_div": [1.6, 2.0, 2.4, 2.8, 3.2], "knn_start_cells": [1, 2, 3],
               "queries_per_wave": [0, 4, 8, 16, 32],

@@ -158,25 +158,25 @@ struct XcdWork {
 
 // Batch k (up to NB queries) of a wave that solves a batch at a time; false when the wave is done.
 // STRIDED (a grid of two rounds of resident waves, for counts that fill only a few rounds of batches): batch
-// k holds the wave's queries g + (k*NB + j)*G, so every wave gets the same count to within one and no
+// k holds the wave's queries g + (k*nb + j)*G, so every wave gets the same count to within one and no
 // partial last round of whole batches is left.  Otherwise (a grid of one batch per wave, which the
-// dispatcher balances): the one batch of NB neighbouring queries NB*g + j.
-template <bool STRIDED, int NB>
-__device__ __forceinline__ bool wave_batch(uint32_t k, uint32_t total, uint32_t &first, uint32_t &step,
+// dispatcher balances): the one batch of nb neighbouring queries nb*g + j.
+template <bool STRIDED>
+__device__ __forceinline__ bool wave_batch(uint32_t k, uint32_t total, int nb, uint32_t &first, uint32_t &step,
                                            int &nbatch) {
   if (STRIDED) {
-    const uint64_t G = wave_count(), f = (uint64_t)wave_id() + (uint64_t)k * NB * G;
+    const uint64_t G = wave_count(), f = (uint64_t)wave_id() + (uint64_t)k * (uint64_t)nb * G;
     if (f >= total) return false;
     first = (uint32_t)f;
     step = (uint32_t)G;
-    nbatch = (int)min<uint64_t>(NB, ((uint64_t)total - f + G - 1) / G);
+    nbatch = (int)min<uint64_t>((uint64_t)nb, ((uint64_t)total - f + G - 1) / G);
     return true;
   }
-  const uint64_t f = (uint64_t)wave_id() * NB;
+  const uint64_t f = (uint64_t)wave_id() * (uint64_t)nb;
   if (k > 0 || f >= total) return false;
   first = (uint32_t)f;
   step = 1;
-  nbatch = (int)min<uint64_t>(NB, (uint64_t)total - f);
+  nbatch = (int)min<uint64_t>((uint64_t)nb, (uint64_t)total - f);
   return true;
 }
 

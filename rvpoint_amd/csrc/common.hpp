@@ -117,6 +117,7 @@ double tune_seg_cells_per_point();
 int tune_queries_per_wave();  // 0: chosen from the query count
 int tune_work_steal();  // batched SOR runs: 0 strided, 1 per-XCD ranges with stealing, 2 per-XCD ranges strided
 int tune_knn_start_cells();  // half-width in cells of an unbounded k-NN search's first box
+int tune_normals_per_wave();  // consecutive queries per wave of the radius normals (0: the default)
 // blocks of `threads` threads of `kernel` the current device holds at once (occupancy x CUs), cached
 int resident_blocks(const void *kernel, int threads, unsigned *blocks);
 

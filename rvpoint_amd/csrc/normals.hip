@@ -1,5 +1,5 @@
-// normals.hip -- N1 / N2 normal estimation: neighbourhood (exact kNN or radius list), FP32 compensated
-// covariance re-centred at the query point, closed-form FP32 3x3 eigen solve with an FP64 Jacobi
+// normals.hip -- N1 / N2 normal estimation: neighbourhood (exact kNN or radius list), covariance re-centred
+// at the query point with fp64 terms and sums, closed-form FP32 3x3 eigen solve with an FP64 Jacobi
 // fallback for near-degenerate spectra, flip toward the viewpoint.
 //
 // Two kernels per call: the neighbourhood kernels (one wave per query) write each query's moments to a
@@ -190,30 +190,29 @@ __device__ __forceinline__ void flip_toward(float px, float py, float pz, float 
 }
 
 // covariance of the m <= 64 neighbours held one per lane (payload = cell-order position), coordinates
-// re-centred at the query point, compensated butterfly sums; result uniform across the wave.  c[6..8]
-// receives the neighbourhood mean relative to the query.
+// re-centred at the query point, two passes (mean, then centred products) with fp64 terms and sums; result
+// uniform across the wave, rounded to fp32.  c[6..8] receives the neighbourhood mean relative to the query.
 __device__ __forceinline__ void wave_cov(const GridView &g, bool valid, uint32_t pos, int m, float qx, float qy,
                                          float qz, float c[9]) {
-  float px = 0.f, py = 0.f, pz = 0.f;
+  double p[3] = {0.0, 0.0, 0.0};
   if (valid) {
-    px = __fsub_rn(g.xs[pos], qx);
-    py = __fsub_rn(g.ys[pos], qy);
-    pz = __fsub_rn(g.zs[pos], qz);
+    p[0] = (double)g.xs[pos] - (double)qx;
+    p[1] = (double)g.ys[pos] - (double)qy;
+    p[2] = (double)g.zs[pos] - (double)qz;
   }
-  const float fm = __int2float_rn(m);
-  const float ps[3] = {px, py, pz}, zero[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  float sum[3];
-  wave_sum_comp3(ps, zero, sum);
-  const float mx = __fdiv_rn(sum[0], fm), my = __fdiv_rn(sum[1], fm), mz = __fdiv_rn(sum[2], fm);
-  const float dx = valid ? __fsub_rn(px, mx) : 0.f;
-  const float dy = valid ? __fsub_rn(py, my) : 0.f;
-  const float dz = valid ? __fsub_rn(pz, mz) : 0.f;
-  const float pr[6] = {__fmul_rn(dx, dx), __fmul_rn(dx, dy), __fmul_rn(dx, dz),
-                       __fmul_rn(dy, dy), __fmul_rn(dy, dz), __fmul_rn(dz, dz)};
-  wave_sum_comp6(pr, zero, c);
-  c[6] = mx;
-  c[7] = my;
-  c[8] = mz;
+  double mu[3];
+  wave_sum3_f64(p, mu);
+  const double inv = m > 0 ? 1.0 / (double)m : 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) mu[a] *= inv;
+  const double dx = valid ? p[0] - mu[0] : 0.0, dy = valid ? p[1] - mu[1] : 0.0, dz = valid ? p[2] - mu[2] : 0.0;
+  const double pr[6] = {dx * dx, dx * dy, dx * dz, dy * dy, dy * dz, dz * dz};
+  double cv[6];
+  wave_sum6_f64(pr, cv);
+#pragma unroll
+  for (int e = 0; e < 6; ++e) c[e] = (float)cv[e];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) c[6 + a] = (float)mu[a];
 }
 
 // the wave's moments of query q (uniform across the wave after wave_cov) into the record: lane e stores word e
@@ -266,7 +265,8 @@ __global__ __launch_bounds__(256) void normals_solve_k(const float *__restrict__
 
 // MODE 0: k nearest (N1, K = k+1 incl. self); MODE 1: radius list capped at K (N2)
 template <int MODE, bool STRIDED>
-__global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, int h0, float *__restrict__ rec) {
+__global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, int h0, int nb,
+                                                      float *__restrict__ rec) {
   __shared__ WaveLds lds[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
   constexpr bool STAGE = MODE == 1 && !STRIDED;
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
   const float r2 = __fmul_rn(radius, radius);
   uint32_t first = 0, step = 1;
   int nbatch = 0;
-  for (uint32_t k = 0; wave_batch<STRIDED, NB>(k, g.n, first, step, nbatch); ++k) {
+  for (uint32_t k = 0; wave_batch<STRIDED>(k, g.n, nb, first, step, nbatch); ++k) {
     StagedWindow win;
     float px = 0.f, py = 0.f, pz = 0.f, pd2 = -1.0f;  // previous query of the batch, its K-th d2 (< 0: none)
     for (int j = 0; j < nbatch; ++j) {
@@ -332,42 +332,41 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
     if (m <= WAVE) {
       wave_cov(g, l < m, l < m ? pos[b + l] : 0u, m, x, y, z, c);
     } else {
-      // two passes over the row; each lane keeps a compensated running sum across chunks
-      float s[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+      // two passes over the row; each lane keeps fp64 running sums across chunks
+      double sp[3] = {0.0, 0.0, 0.0};
       for (int c0 = 0; c0 < m; c0 += WAVE) {
         if (c0 + l < m) {
           const uint32_t p = pos[b + c0 + l];
-          const float v[3] = {__fsub_rn(g.xs[p], x), __fsub_rn(g.ys[p], y), __fsub_rn(g.zs[p], z)};
-          for (int a = 0; a < 3; ++a) {
-            float t, err;
-            two_sum(s[a], v[a], t, err);
-            s[a] = t;
-            e[a] = __fadd_rn(e[a], err);
-          }
+          sp[0] += (double)g.xs[p] - (double)x;
+          sp[1] += (double)g.ys[p] - (double)y;
+          sp[2] += (double)g.zs[p] - (double)z;
         }
       }
-      const float fm = __int2float_rn(m);
-      float mu[3];
-      wave_sum_comp3(s, e, mu);
-      for (int a = 0; a < 3; ++a) mu[a] = __fdiv_rn(mu[a], fm);
-      float cs[6] = {0, 0, 0, 0, 0, 0}, ce[6] = {0, 0, 0, 0, 0, 0};
+      double mu[3];
+      wave_sum3_f64(sp, mu);
+      const double inv = 1.0 / (double)m;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) mu[a] *= inv;
+      double cs[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
       for (int c0 = 0; c0 < m; c0 += WAVE) {
         if (c0 + l < m) {
           const uint32_t p = pos[b + c0 + l];
-          const float d[3] = {__fsub_rn(__fsub_rn(g.xs[p], x), mu[0]), __fsub_rn(__fsub_rn(g.ys[p], y), mu[1]),
-                              __fsub_rn(__fsub_rn(g.zs[p], z), mu[2])};
-          const float pr[6] = {__fmul_rn(d[0], d[0]), __fmul_rn(d[0], d[1]), __fmul_rn(d[0], d[2]),
-                               __fmul_rn(d[1], d[1]), __fmul_rn(d[1], d[2]), __fmul_rn(d[2], d[2])};
-          for (int a = 0; a < 6; ++a) {
-            float t, err;
-            two_sum(cs[a], pr[a], t, err);
-            cs[a] = t;
-            ce[a] = __fadd_rn(ce[a], err);
-          }
+          const double d[3] = {(double)g.xs[p] - (double)x - mu[0], (double)g.ys[p] - (double)y - mu[1],
+                               (double)g.zs[p] - (double)z - mu[2]};
+          cs[0] += d[0] * d[0];
+          cs[1] += d[0] * d[1];
+          cs[2] += d[0] * d[2];
+          cs[3] += d[1] * d[1];
+          cs[4] += d[1] * d[2];
+          cs[5] += d[2] * d[2];
         }
       }
-      wave_sum_comp6(cs, ce, c);
-      for (int a = 0; a < 3; ++a) c[6 + a] = mu[a];
+      double cv[6];
+      wave_sum6_f64(cs, cv);
+#pragma unroll
+      for (int e = 0; e < 6; ++e) c[e] = (float)cv[e];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) c[6 + a] = (float)mu[a];
     }
     write_rec(rec, g.n, q, c, m);
   }
@@ -377,7 +376,7 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
 // MODE 1: radius[c] list capped at K)
 template <int MODE, bool STRIDED>
 __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius, int h0,
-                                                     float *__restrict__ rec, size_t n_bound) {
+                                                     int nb, float *__restrict__ rec, size_t n_bound) {
   __shared__ WaveLds lds[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
   constexpr bool STAGE = MODE == 1 && !STRIDED;
@@ -386,7 +385,7 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
   const uint32_t total = A.off[A.ncl];
   uint32_t first = 0, step = 1;
   int nbatch = 0;
-  for (uint32_t k = 0; wave_batch<STRIDED, NB>(k, total, first, step, nbatch); ++k) {
+  for (uint32_t k = 0; wave_batch<STRIDED>(k, total, nb, first, step, nbatch); ++k) {
     StagedWindow win;
     int pc = -1;  // cloud of the previous query of the batch, whose K-th d2 is pd2
     float px = 0.f, py = 0.f, pz = 0.f, pd2 = -1.0f;
@@ -429,6 +428,13 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
   }
 }
 
+// consecutive queries per wave of the radius normals (a batch shares its staged windows): the normals_per_wave
+// knob, else NB.  Fewer per wave balance small counts better (the launch's last round of waves runs part full).
+static int radius_normals_batch() {
+  const int forced = tune_normals_per_wave();
+  return forced > 0 ? std::min(forced, WAVE) : NB;
+}
+
 // the eigen solves of n recorded queries (count_dev: their device-side count, or null for n)
 static int solve_normals(const float *rec, size_t n, const uint32_t *count_dev, const float *xs, const float *ys,
                          const float *zs, const uint32_t *perm, float vpx, float vpy, float vpz, float *nx, float *ny,
@@ -466,9 +472,9 @@ int rvp_normals_knn_centroids(const float *x, const float *y, const float *z, si
   const GridView g = G.view();
   const int h0 = tune_knn_start_cells();
   if (strided)
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, true><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, h0, rec.as<float>()));
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, true><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, h0, NB, rec.as<float>()));
   else
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, false><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, h0, rec.as<float>()));
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, false><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, h0, NB, rec.as<float>()));
   return solve_normals(rec.as<float>(), n, nullptr, g.xs, g.ys, g.zs, g.perm, vpx, vpy, vpz, nx, ny, nz, fb, cx, cy,
                        cz, s);
 }
@@ -492,9 +498,11 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
   Tmp rec;
   RVP_TRY(rec.alloc((size_t)REC_WORDS * n * sizeof(float), s));
   if (max_nn > 0 && max_nn <= MAX_TOPK) {
-    // batches of NB consecutive queries with staged windows (as the batched radius normals)
-    const unsigned qb = ceil_div(ceil_div(n, NB), QW);
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, false><<<qb, QT, 0, s>>>(g, max_nn, radius, 1, rec.as<float>()));
+    // batches of consecutive queries with staged windows (as the batched radius normals)
+    const int nb = radius_normals_batch();
+    const unsigned qb = ceil_div(ceil_div(n, (size_t)nb), QW);
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, false><<<qb, QT, 0, s>>>(g, max_nn, radius, 1, nb,
+                                                                               rec.as<float>()));
   } else {
     // unlimited (or a cap above 64): full lists, sorted by (d2, idx), truncated by the row offsets
     Tmp cnt, off, keys, pos;
@@ -537,10 +545,10 @@ int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const 
   const SegArrays A = G.arrays();
   const int h0 = tune_knn_start_cells();
   if (strided)
-    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, rec.as<float>(),
+    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, NB, rec.as<float>(),
                                                                             n_bound));
   else
-    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, rec.as<float>(),
+    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, NB, rec.as<float>(),
                                                                              n_bound));
   return solve_normals(rec.as<float>(), n_bound, G.off + nclouds, A.xs, A.ys, A.zs, A.perm, vpx, vpy, vpz, nx, ny,
                        nz, fb, nullptr, nullptr, nullptr, s);
@@ -560,12 +568,13 @@ int rvp_batch_normals_radius(const float *x, const float *y, const float *z, con
   RVP_TRY(fallback_counter(s, &fb));
   Tmp rec;
   RVP_TRY(rec.alloc((size_t)REC_WORDS * n_bound * sizeof(float), s));
-  // radius lists: batches of NB consecutive queries, whose shared windows stage once in LDS (faster than the
+  // radius lists: batches of consecutive queries, whose shared windows stage once in LDS (faster than the
   // strided batches that balance k-NN queries, also for small counts)
-  const unsigned qb = ceil_div(ceil_div(n_bound, NB), QW);
+  const int nb = radius_normals_batch();
+  const unsigned qb = ceil_div(ceil_div(n_bound, (size_t)nb), QW);
   const SegArrays A = G.arrays();
-  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(A, max_nn, radius, 1, rec.as<float>(),
-                                                                           n_bound));
+  RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(A, max_nn, radius, 1, nb,
+                                                                           rec.as<float>(), n_bound));
   return solve_normals(rec.as<float>(), n_bound, G.off + nclouds, A.xs, A.ys, A.zs, A.perm, vpx, vpy, vpz, nx, ny,
                        nz, fb, nullptr, nullptr, nullptr, s);
 }

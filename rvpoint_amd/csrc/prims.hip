@@ -177,9 +177,38 @@ __device__ __forceinline__ uint32_t take_tile(uint64_t *ctl, uint32_t nb, uint32
   return *s_tile;
 }
 
-// (in and out may be the same array: a tile reads all its inputs before it writes)
+// One tile of a single-pass scan whose thread t holds the tile's values t*SCAN_ITEMS .. t*SCAN_ITEMS + 7 in v:
+// v becomes the exclusive scan over the whole launch (the tile's look-back prefix included); returns the tile's
+// aggregate.  *s_prefix receives the prefix (shared memory of the block).  Every thread of the block calls it.
+__device__ __forceinline__ uint32_t tile_scan_items(uint32_t (&v)[SCAN_ITEMS], uint64_t *__restrict__ status,
+                                                    uint32_t tile, uint32_t epoch, uint32_t *sm, uint32_t *s_prefix) {
+  uint32_t run = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const uint32_t x = v[j];
+    v[j] = run;
+    run += x;
+  }
+  uint32_t agg;
+  const uint32_t texcl = block_excl_scan_256(run, agg, sm);
+  if (threadIdx.x < WAVE) {
+    const uint32_t prefix = tile_lookback(status, tile, epoch, agg);
+    if (threadIdx.x == 0) *s_prefix = prefix;
+  }
+  __syncthreads();
+  const uint32_t base = *s_prefix + texcl;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) v[j] += base;
+  return agg;
+}
+
+// Single-pass exclusive scan.  Thread t of a tile owns its elements t*SCAN_ITEMS .. t*SCAN_ITEMS + 7
+// (consecutive, 16-byte vector accesses when VEC): a local scan, ONE block-wide scan of the 256 thread totals,
+// the look-back, and the writes.  (Strided items would need a block scan per item, each with its barriers.)
+// in and out may be the same array: a tile reads all its inputs before it writes.
 // n_dev (may be null) bounds the element count on the device: nn = min(*n_dev + add, n); only the tiles nn
 // needs take tickets, the rest of a launch sized for n leaves at once
+template <bool VEC>
 __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in, uint32_t *out, size_t n_bound,
                                                               const uint32_t *__restrict__ n_dev, uint32_t add,
                                                               uint64_t *__restrict__ status,
@@ -198,35 +227,26 @@ __global__ __launch_bounds__(PRIM_THREADS) void scan_1pass_k(const uint32_t *in,
   const uint32_t tile = take_tile(ctl, nb, &s_tile, &s_epoch);
   const uint32_t epoch = s_epoch;
   if (tile >= nb) return;
-  const size_t base = (size_t)tile * SCAN_TILE;
+  const size_t i0 = (size_t)tile * SCAN_TILE + (size_t)t * SCAN_ITEMS;
+  static_assert(SCAN_ITEMS == 8, "two 16-byte vectors per thread");
   uint32_t v[SCAN_ITEMS];
-  uint32_t acc = 0;
+  if (VEC && i0 + SCAN_ITEMS <= n) {
+    const uint4 a = *reinterpret_cast<const uint4 *>(in + i0), b = *reinterpret_cast<const uint4 *>(in + i0 + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; ++j) {
-    const size_t i = base + (size_t)j * PRIM_THREADS + t;
-    v[j] = i < n ? in[i] : 0u;
-    acc += v[j];
+    for (int j = 0; j < SCAN_ITEMS; ++j) v[j] = i0 + j < n ? in[i0 + j] : 0u;
   }
-  acc = wave_sum(acc);
-  if (lane() == 0) sm[t >> 6] = acc;
-  __syncthreads();
-  const uint32_t agg = sm[0] + sm[1] + sm[2] + sm[3];
-  if (t < WAVE) {
-    const uint32_t prefix = tile_lookback(status, tile, epoch, agg);
-    if (t == 0) {
-      s_prefix = prefix;
-      if (total && tile == nb - 1) *total = prefix + agg;
-    }
-  }
-  __syncthreads();
-  uint32_t carry = s_prefix;
+  const uint32_t agg = tile_scan_items(v, status, tile, epoch, sm, &s_prefix);
+  if (total && tile == nb - 1 && t == 0) *total = s_prefix + agg;
+  if (VEC && i0 + SCAN_ITEMS <= n) {
+    *reinterpret_cast<uint4 *>(out + i0) = make_uint4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<uint4 *>(out + i0 + 4) = make_uint4(v[4], v[5], v[6], v[7]);
+  } else {
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; ++j) {
-    const size_t i = base + (size_t)j * PRIM_THREADS + t;
-    uint32_t bt;
-    const uint32_t ex = block_excl_scan_256(v[j], bt, sm);
-    if (i < n) out[i] = carry + ex;
-    carry += bt;
+    for (int j = 0; j < SCAN_ITEMS; ++j)
+      if (i0 + j < n) out[i0 + j] = v[j];
   }
 }
 
@@ -248,13 +268,12 @@ __global__ __launch_bounds__(PRIM_THREADS) void plane_split_k(const float *__res
   const uint32_t tile = take_tile(ctl, nb, &s_tile, &s_epoch);
   const uint32_t epoch = s_epoch;
   if (tile >= nb) return;
-  const size_t base = (size_t)tile * SCAN_TILE;
+  const size_t i0 = (size_t)tile * SCAN_TILE + (size_t)t * SCAN_ITEMS;
   float px[SCAN_ITEMS], py[SCAN_ITEMS], pz[SCAN_ITEMS];
-  uint32_t f[SCAN_ITEMS];
-  uint32_t acc = 0;
+  uint32_t f[SCAN_ITEMS], r[SCAN_ITEMS];
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    const size_t i = base + (size_t)j * PRIM_THREADS + t;
+    const size_t i = i0 + j;
     f[j] = 0u;
     px[j] = py[j] = pz[j] = 0.0f;
     if (i < n) {
@@ -265,39 +284,25 @@ __global__ __launch_bounds__(PRIM_THREADS) void plane_split_k(const float *__res
           __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(a, px[j]), __fmul_rn(b, py[j])), __fmul_rn(c, pz[j])), d);
       f[j] = (dist >= -th && dist <= th) ? 1u : 0u;
     }
-    acc += f[j];
+    r[j] = f[j];
   }
-  acc = wave_sum(acc);
-  if (lane() == 0) sm[t >> 6] = acc;
-  __syncthreads();
-  const uint32_t agg = sm[0] + sm[1] + sm[2] + sm[3];
-  if (t < WAVE) {
-    const uint32_t prefix = tile_lookback(status, tile, epoch, agg);
-    if (t == 0) {
-      s_prefix = prefix;
-      if (tile == nb - 1) {
-        counts[0] = prefix + agg;
-        counts[1] = (uint32_t)n - (prefix + agg);
-      }
-    }
+  const uint32_t agg = tile_scan_items(r, status, tile, epoch, sm, &s_prefix);
+  if (tile == nb - 1 && t == 0) {
+    counts[0] = s_prefix + agg;
+    counts[1] = (uint32_t)n - (s_prefix + agg);
   }
-  __syncthreads();
-  uint32_t carry = s_prefix;
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    const size_t i = base + (size_t)j * PRIM_THREADS + t;
-    uint32_t bt;
-    const uint32_t r = carry + block_excl_scan_256(f[j], bt, sm);
+    const size_t i = i0 + j;
     if (i < n) {
       float *o = f[j] ? in_aos : out_aos;
-      const size_t q = f[j] ? (size_t)r : i - (size_t)r;
+      const size_t q = f[j] ? (size_t)r[j] : i - (size_t)r[j];
       if (o) {
         o[3 * q] = px[j];
         o[3 * q + 1] = py[j];
         o[3 * q + 2] = pz[j];
       }
     }
-    carry += bt;
   }
 }
 
@@ -327,42 +332,23 @@ __global__ __launch_bounds__(PRIM_THREADS) void compact_flagged_k(const float *_
   const uint32_t tile = take_tile(ctl, nb, &s_tile, &s_epoch);
   const uint32_t epoch = s_epoch;
   if (tile >= nb) return;
-  const size_t base = (size_t)tile * SCAN_TILE;
-  uint32_t f[SCAN_ITEMS];
-  uint32_t acc = 0;
+  const size_t i0 = (size_t)tile * SCAN_TILE + (size_t)t * SCAN_ITEMS;
+  uint32_t f[SCAN_ITEMS], r[SCAN_ITEMS];
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) r[j] = f[j] = i0 + j < nn ? pos[i0 + j] : 0u;
+  const uint32_t agg = tile_scan_items(r, status, tile, epoch, sm, &s_prefix);
+  if (tile == nb - 1 && t == 0) *count = s_prefix + agg;
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    const size_t i = base + (size_t)j * PRIM_THREADS + t;
-    f[j] = i < nn ? pos[i] : 0u;
-    acc += f[j];
-  }
-  acc = wave_sum(acc);
-  if (lane() == 0) sm[t >> 6] = acc;
-  __syncthreads();
-  const uint32_t agg = sm[0] + sm[1] + sm[2] + sm[3];
-  if (t < WAVE) {
-    const uint32_t prefix = tile_lookback(status, tile, epoch, agg);
-    if (t == 0) {
-      s_prefix = prefix;
-      if (tile == nb - 1) *count = prefix + agg;
-    }
-  }
-  __syncthreads();
-  uint32_t carry = s_prefix;
-#pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; ++j) {
-    const size_t i = base + (size_t)j * PRIM_THREADS + t;
-    uint32_t bt;
-    const uint32_t r = carry + block_excl_scan_256(f[j], bt, sm);
+    const size_t i = i0 + j;
     if (i < nn) {
-      pos[i] = r;
+      pos[i] = r[j];
       if (f[j] && out_aos) {
-        out_aos[3 * (size_t)r] = x[i];
-        out_aos[3 * (size_t)r + 1] = y[i];
-        out_aos[3 * (size_t)r + 2] = z[i];
+        out_aos[3 * (size_t)r[j]] = x[i];
+        out_aos[3 * (size_t)r[j] + 1] = y[i];
+        out_aos[3 * (size_t)r[j] + 2] = z[i];
       }
     }
-    carry += bt;
   }
 }
 
@@ -382,36 +368,23 @@ __global__ __launch_bounds__(PRIM_THREADS) void run_heads_k(const K *__restrict_
   const uint32_t epoch = s_epoch;
   if (tile >= nb) return;
   const size_t nn = n_dev ? min((size_t)*n_dev, n) : n;
-  const size_t base = (size_t)tile * SCAN_TILE;
-  uint32_t f[SCAN_ITEMS];
-  uint32_t acc = 0;
+  const size_t i0 = (size_t)tile * SCAN_TILE + (size_t)t * SCAN_ITEMS;
+  uint32_t f[SCAN_ITEMS], ex[SCAN_ITEMS];
+  K prev = (i0 > 0 && i0 - 1 < nn) ? keys[i0 - 1] : (K)0;
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    const size_t i = base + (size_t)j * PRIM_THREADS + t;
-    f[j] = (i < nn && (i == 0 || keys[i] != keys[i - 1])) ? 1u : 0u;
-    acc += f[j];
+    const size_t i = i0 + j;
+    const K k = i < nn ? keys[i] : (K)0;
+    ex[j] = f[j] = (i < nn && (i == 0 || k != prev)) ? 1u : 0u;
+    prev = k;
   }
-  acc = wave_sum(acc);
-  if (lane() == 0) sm[t >> 6] = acc;
-  __syncthreads();
-  const uint32_t agg = sm[0] + sm[1] + sm[2] + sm[3];
-  if (t < WAVE) {
-    const uint32_t prefix = tile_lookback(status, tile, epoch, agg);
-    if (t == 0) {
-      s_prefix = prefix;
-      if (tile == nb - 1) *total = prefix + agg;
-    }
-  }
-  __syncthreads();
-  uint32_t carry = s_prefix;
+  const uint32_t agg = tile_scan_items(ex, status, tile, epoch, sm, &s_prefix);
+  if (tile == nb - 1 && t == 0) *total = s_prefix + agg;
 #pragma unroll
   for (int j = 0; j < SCAN_ITEMS; ++j) {
-    const size_t i = base + (size_t)j * PRIM_THREADS + t;
-    uint32_t bt;
-    const uint32_t ex = carry + block_excl_scan_256(f[j], bt, sm);
-    if (i < n) vid[i] = ex;
-    if (f[j]) start[ex] = (uint32_t)i;
-    carry += bt;
+    const size_t i = i0 + j;
+    if (i < n) vid[i] = ex[j];
+    if (f[j]) start[ex[j]] = (uint32_t)i;
   }
 }
 
@@ -490,8 +463,12 @@ int exclusive_scan_u32_bounded(const uint32_t *in, uint32_t *out, size_t n, cons
   uint64_t *status = nullptr;
   uint64_t *ctl = nullptr;
   RVP_TRY(scan_state(s, nb, &status, &ctl));
-  RVP_LAUNCH("scan_1pass_k", s, scan_1pass_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, out, n, n_dev, add, status,
-                                                                                  ctl, total_dev));
+  if ((((uintptr_t)in | (uintptr_t)out) & 15u) == 0)
+    RVP_LAUNCH("scan_1pass_k", s, scan_1pass_k<true><<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, out, n, n_dev, add,
+                                                                                          status, ctl, total_dev));
+  else
+    RVP_LAUNCH("scan_1pass_k", s, scan_1pass_k<false><<<(unsigned)nb, PRIM_THREADS, 0, s>>>(in, out, n, n_dev, add,
+                                                                                           status, ctl, total_dev));
   return RVP_OK;
 }
 

@@ -558,106 +558,57 @@ struct DeferredMeans {
   }
 };
 
-// error-free transformation a + b = s + e
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
-  s = __fadd_rn(a, b);
-  const float bb = __fsub_rn(s, a);
-  e = __fadd_rn(__fsub_rn(a, __fsub_rn(s, bb)), __fsub_rn(b, bb));
-}
-
-// compensated butterfly sum across the wave (identical result on every lane)
-__device__ __forceinline__ float wave_sum_comp(float v, float e_in = 0.0f) {
-  float s = v, e = e_in;
-#pragma unroll
-  for (int d = WAVE / 2; d > 0; d >>= 1) {
-    const int a = xor_addr(d);
-    const float so = bperm(s, a), eo = bperm(e, a);
-    float t, err;
-    two_sum(s, so, t, err);
-    e = __fadd_rn(__fadd_rn(e, eo), err);
-    s = t;
-  }
-  return __fadd_rn(s, e);
-}
-
-// Several compensated sums over the wave at once, reduce-scatter style: at each butterfly distance the lanes
-// split the values between the two halves of each pair, so a value is combined on half as many lanes as a
-// separate butterfly would use, and the partner's copy is always combined in the same order (own, partner).
-// Each value meets exactly the partial sums it would meet in wave_sum_comp, and because two_sum is exact
-// (the error is the unique (a + b) - s) every step is symmetric in its two operands, so each result equals
-// wave_sum_comp of the same inputs, bit for bit.
-
-// (s, e) += the partner's (so, eo), as one step of wave_sum_comp
-__device__ __forceinline__ void comp_add(float &s, float &e, float so, float eo) {
-  float t, err;
-  two_sum(s, so, t, err);
-  e = __fadd_rn(__fadd_rn(e, eo), err);
-  s = t;
+// Several fp64 sums over the wave at once, reduce-scatter style: at each butterfly distance the lanes split
+// the values between the two halves of each pair (each keeps one and adds its partner's copy), so a value is
+// combined on half as many lanes as a separate butterfly would use; the last steps are a full butterfly within
+// the lane group that holds the value.  fp64 adds issue at the rate of unpacked fp32 ones on this GPU, so the
+// sums cost a third of compensated fp32 sums and are more accurate.  Both lanes of a pair add the same two
+// numbers, so every lane of a group ends with the same result.
+__device__ __forceinline__ double readlaned(double v, int l) {
+  return __longlong_as_double((long long)readlane64((uint64_t)__double_as_longlong(v), l));
 }
 
 // lanes with `upper` clear keep value lo, the others value hi; each adds the partner's copy of what it keeps
-__device__ __forceinline__ void rs_step(float &ks, float &ke, float los, float loe, float his, float hie, bool upper,
-                                        int addr) {
-  const float ss = upper ? los : his, se = upper ? loe : hie;  // the value the partner keeps
-  ks = upper ? his : los;
-  ke = upper ? hie : loe;
-  comp_add(ks, ke, bperm(ss, addr), bperm(se, addr));
+__device__ __forceinline__ double rs_step64(double lo, double hi, bool upper, int addr) {
+  const double give = upper ? lo : hi;  // the value the partner keeps
+  return (upper ? hi : lo) + bperm(give, addr);
 }
 
-// the remaining full-butterfly steps d = D/2 .. 1 and the final fl(s + e)
-template <int D> __device__ __forceinline__ float comp_tail(float s, float e) {
-#pragma unroll
-  for (int d = D / 2; d > 0; d >>= 1) {
-    const int a = xor_addr(d);
-    comp_add(s, e, bperm(s, a), bperm(e, a));
-  }
-  return __fadd_rn(s, e);
-}
-
-// out[i] = wave_sum_comp(s[i], e[i]) for three values (wave-uniform results)
-__device__ __forceinline__ void wave_sum_comp3(const float s[3], const float e[3], float out[3]) {
+// out[i] = the sum over the wave of in[i], i < 3 (wave-uniform results)
+__device__ __forceinline__ void wave_sum3_f64(const double in[3], double out[3]) {
   const int l = lane();
   const int a32 = xor_addr(32), a16 = xor_addr(16);
-  float w0s, w0e;  // lanes < 32: value 0; lanes >= 32: value 2
-  rs_step(w0s, w0e, s[0], e[0], s[2], e[2], (l & 32) != 0, a32);
-  float w1s = s[1], w1e = e[1];  // lanes < 32: value 1
-  comp_add(w1s, w1e, bperm(s[1], a32), bperm(e[1], a32));
-  if (l & 32) {
-    w1s = w0s;
-    w1e = w0e;
-  }
-  float ys, ye;  // lanes 0-15: value 0, 16-31: value 1, 32-63: value 2
-  rs_step(ys, ye, w0s, w0e, w1s, w1e, (l & 16) != 0, a16);
-  const float r = comp_tail<16>(ys, ye);
-  out[0] = readlanef(r, 0);
-  out[1] = readlanef(r, 16);
-  out[2] = readlanef(r, 32);
+  const double w0 = rs_step64(in[0], in[2], (l & 32) != 0, a32);  // lanes < 32: value 0; >= 32: value 2
+  double w1 = in[1] + bperm(in[1], a32);                          // lanes < 32: value 1
+  if (l & 32) w1 = w0;
+  double y = rs_step64(w0, w1, (l & 16) != 0, a16);  // lanes 0-15: value 0, 16-31: value 1, 32-63: value 2
+#pragma unroll
+  for (int d = 8; d > 0; d >>= 1) y += bperm(y, xor_addr(d));
+  out[0] = readlaned(y, 0);
+  out[1] = readlaned(y, 16);
+  out[2] = readlaned(y, 32);
 }
 
-// out[i] = wave_sum_comp(s[i], e[i]) for six values (wave-uniform results)
-__device__ __forceinline__ void wave_sum_comp6(const float s[6], const float e[6], float out[6]) {
+// out[i] = the sum over the wave of in[i], i < 6 (wave-uniform results)
+__device__ __forceinline__ void wave_sum6_f64(const double in[6], double out[6]) {
   const int l = lane();
   const int a32 = xor_addr(32), a16 = xor_addr(16), a8 = xor_addr(8);
-  float ws[3], we[3];  // lanes < 32: values 0, 1, 2; lanes >= 32: values 3, 4, 5
+  double w[3];  // lanes < 32: values 0, 1, 2; lanes >= 32: values 3, 4, 5
 #pragma unroll
-  for (int i = 0; i < 3; ++i) rs_step(ws[i], we[i], s[i], e[i], s[i + 3], e[i + 3], (l & 32) != 0, a32);
-  float x0s, x0e;  // bit 4 clear: value 0 (3); set: value 2 (5)
-  rs_step(x0s, x0e, ws[0], we[0], ws[2], we[2], (l & 16) != 0, a16);
-  float x1s = ws[1], x1e = we[1];  // bit 4 clear: value 1 (4)
-  comp_add(x1s, x1e, bperm(ws[1], a16), bperm(we[1], a16));
-  if (l & 16) {
-    x1s = x0s;
-    x1e = x0e;
-  }
-  float ys, ye;  // lanes 0-7 value 0, 8-15 value 1, 16-31 value 2, 32-39 value 3, 40-47 value 4, 48-63 value 5
-  rs_step(ys, ye, x0s, x0e, x1s, x1e, (l & 8) != 0, a8);
-  const float r = comp_tail<8>(ys, ye);
-  out[0] = readlanef(r, 0);
-  out[1] = readlanef(r, 8);
-  out[2] = readlanef(r, 16);
-  out[3] = readlanef(r, 32);
-  out[4] = readlanef(r, 40);
-  out[5] = readlanef(r, 48);
+  for (int i = 0; i < 3; ++i) w[i] = rs_step64(in[i], in[i + 3], (l & 32) != 0, a32);
+  const double x0 = rs_step64(w[0], w[2], (l & 16) != 0, a16);  // bit 4 clear: value 0 (3); set: value 2 (5)
+  double x1 = w[1] + bperm(w[1], a16);                          // bit 4 clear: value 1 (4)
+  if (l & 16) x1 = x0;
+  // lanes 0-7 value 0, 8-15 value 1, 16-31 value 2, 32-39 value 3, 40-47 value 4, 48-63 value 5
+  double y = rs_step64(x0, x1, (l & 8) != 0, a8);
+#pragma unroll
+  for (int d = 4; d > 0; d >>= 1) y += bperm(y, xor_addr(d));
+  out[0] = readlaned(y, 0);
+  out[1] = readlaned(y, 8);
+  out[2] = readlaned(y, 16);
+  out[3] = readlaned(y, 32);
+  out[4] = readlaned(y, 40);
+  out[5] = readlaned(y, 48);
 }
 
 // CSR fill of radius lists (rows = original query index; qx == nullptr: the grid's own points), each row

@@ -247,8 +247,10 @@ static std::atomic<double> g_seg_cells_per_point{32.0};
 static std::atomic<int> g_queries_per_wave{0};
 static std::atomic<int> g_work_steal{0};
 static std::atomic<int> g_knn_start_cells{1};
+static std::atomic<int> g_normals_per_wave{0};
 int tune_work_steal() { return g_work_steal.load(); }
 int tune_knn_start_cells() { return g_knn_start_cells.load(); }
+int tune_normals_per_wave() { return g_normals_per_wave.load(); }
 double tune_knn_cell_div() { return g_knn_cell_div.load(); }
 double tune_seg_cells_per_point() { return g_seg_cells_per_point.load(); }
 int tune_queries_per_wave() { return g_queries_per_wave.load(); }
@@ -505,6 +507,12 @@ int rvp_tune_set(const char *name, double value) {
     rvp::g_knn_start_cells.store((int)value);
     return RVP_OK;
   }
+  if (std::string(name) == "normals_per_wave") {
+    RVP_CHECK(value >= 0.0 && value <= 64.0 && value == (double)(int)value, RVP_E_ARG,
+              "normals_per_wave must be an integer in [0, 64]");
+    rvp::g_normals_per_wave.store((int)value);
+    return RVP_OK;
+  }
   return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);
 }
 
@@ -528,6 +536,10 @@ int rvp_tune_get(const char *name, double *value) {
   }
   if (std::string(name) == "knn_start_cells") {
     *value = (double)rvp::g_knn_start_cells.load();
+    return RVP_OK;
+  }
+  if (std::string(name) == "normals_per_wave") {
+    *value = (double)rvp::g_normals_per_wave.load();
     return RVP_OK;
   }
   return fail(RVP_E_ARG, std::string("unknown tuning knob: ") + name);

@@ -72,7 +72,8 @@ def test_tuning_knobs_set_get_and_ranges(built):
     from rvpoint_amd import _lib
     L = _lib.lib()
     for name, good, bad in (("knn_cell_div", 1.7, 20.0), ("seg_cells_per_point", 8.0, 0.5),
-                            ("queries_per_wave", 7, 65), ("knn_start_cells", 3, 9)):
+                            ("queries_per_wave", 7, 65), ("knn_start_cells", 3, 9),
+                            ("normals_per_wave", 4, 65)):
         old = _lib.tune(name)
         try:
             assert _lib.tune(name, good) == good
