@@ -514,35 +514,6 @@ __global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, int fo
   dm.flush(sums[w], k, means);
 }
 
-// keep flag (u8, and u32 for the compaction scan) of every point: mean_i <= gmean + alpha * std of its cloud,
-// gmean = fl(sum / n), std = sqrtf(fl(sq / n)) (statistical_outlier_removal.cpp:53-62)
-__global__ void seg_keep_k(const float *__restrict__ means, const uint32_t *__restrict__ off, int ncl,
-                           const float *__restrict__ sums, const float *__restrict__ sqs, float alpha,
-                           uint32_t n_bound, uint8_t *__restrict__ keep, uint32_t *__restrict__ keep32) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_bound) return;
-  if (i >= off[ncl]) {
-    keep[i] = 0;
-    keep32[i] = 0;
-    return;
-  }
-  const int c = cloud_of(off, ncl, i);
-  const float len = __uint2float_rn(off[c + 1] - off[c]);
-  const float thr = __fadd_rn(__fdiv_rn(sums[c], len), __fmul_rn(alpha, sqrtf(__fdiv_rn(sqs[c], len))));
-  const bool k = means[i] <= thr;
-  keep[i] = k ? 1 : 0;
-  keep32[i] = k ? 1u : 0u;
-}
-
-__global__ void seg_out_offsets_k(const uint32_t *__restrict__ off, int ncl, uint32_t n_bound,
-                                  const uint32_t *__restrict__ pos, const uint32_t *__restrict__ kept_total,
-                                  uint32_t *__restrict__ out_off) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c > ncl) return;
-  const uint32_t o = off[c];
-  out_off[c] = (o < off[ncl] && o < n_bound) ? pos[o] : *kept_total;
-}
-
 // H3 per cloud: capped radius lists (cloud-local indices), radius per cloud
 __global__ __launch_bounds__(QT) void seg_radius_topk_k(SegArrays A, const float *__restrict__ radius, int K,
                                                          int per_wave, int32_t *__restrict__ idx_out,
@@ -610,7 +581,7 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
   hipStream_t s = S(st);
   SegGrid G;
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, true, k + 1, nullptr, s));
-  Tmp mt, kt, sums, vsum, pos, kept;
+  Tmp mt, sums, vsum;
   float *mp = means;
   if (!mp) {
     RVP_TRY(mt.alloc(n_bound * sizeof(float), s));
@@ -645,23 +616,9 @@ int rvp_batch_sor_exact(const float *x, const float *y, const float *z, const ui
   // statistical_outlier_removal.cpp:44-51: the sum of the means, then of the squared deviations from their
   // mean, both sequential
   RVP_TRY(exact_seq_mean_var_seg(mp, G.off, nclouds, n_bound, sums.as<float>(), vsum.as<float>(), s));
-  uint8_t *kp = keep;
-  if (!kp) {
-    RVP_TRY(kt.alloc(n_bound, s));
-    kp = kt.as<uint8_t>();
-  }
-  RVP_TRY(pos.alloc(n_bound * sizeof(uint32_t), s));
-  RVP_TRY(kept.alloc(sizeof(uint32_t), s));
-  const unsigned nb = ceil_div(n_bound, 256);
-  RVP_LAUNCH("seg_keep_k", s, seg_keep_k<<<nb, 256, 0, s>>>(mp, G.off, nclouds, sums.as<float>(), vsum.as<float>(),
-                                                            alpha, (uint32_t)n_bound, kp, pos.as<uint32_t>()));
-  // stable compaction over the whole batch keeps the clouds in order; a cloud's output starts where the
-  // compaction positions its first point
-  RVP_TRY(compact_flagged(x, y, z, pos.as<uint32_t>(), n_bound, G.off + nclouds, out_aos, kept.as<uint32_t>(), s));
-  RVP_LAUNCH("seg_out_offsets_k", s, seg_out_offsets_k<<<ceil_div(nclouds + 1, 64), 64, 0, s>>>(
-                                         G.off, nclouds, (uint32_t)n_bound, pos.as<uint32_t>(), kept.as<uint32_t>(),
-                                         out_offsets));
-  return RVP_OK;
+  // keep flags, stable compaction over the whole batch (clouds stay in order) and each cloud's first output row
+  return seg_keep_compact(x, y, z, mp, G.off, nclouds, n_bound, sums.as<float>(), vsum.as<float>(), alpha, keep,
+                          out_aos, out_offsets, s);
 }
 
 int rvp_batch_radius_search(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,

@@ -22,9 +22,9 @@
 namespace rvp {
 
 // FP64-fallback counters: one device word per (device, stream), allocated once under a lock and kept until the
-// process ends (a stream handle the driver reuses gets the same word back).  A normals call clears its stream's
-// word with a kernel in stream order (no memset node, so the call can be captured into a graph) and its solve
-// kernels add to it; calls on other streams never touch it.  Each host thread remembers the stream of its last
+// process ends (a stream handle the driver reuses gets the same word back).  A normals call's neighbourhood kernel
+// clears its stream's word (block 0, in stream order: no memset node, so the call can be captured into a graph)
+// and its solve kernel adds to it; calls on other streams never touch it.  Each host thread remembers the stream of its last
 // normals call, which rvp_normals_fallback_count reads.
 static std::mutex g_fallback_mu;
 static std::map<std::pair<int, hipStream_t>, unsigned long long *> g_fallback;
@@ -34,10 +34,6 @@ struct LastNormalsCall {
   unsigned long long *counter = nullptr;
 };
 static thread_local LastNormalsCall g_last_normals;
-
-__global__ void zero_u64_k(unsigned long long *p) {
-  if (threadIdx.x == 0) *p = 0ull;
-}
 
 static int fallback_counter(hipStream_t s, unsigned long long **out) {
   int dev = 0;
@@ -49,7 +45,6 @@ static int fallback_counter(hipStream_t s, unsigned long long **out) {
     if (!slot) RVP_HIP(hipMalloc(&slot, sizeof(unsigned long long)));
     p = slot;
   }
-  RVP_LAUNCH("zero_u64_k", s, zero_u64_k<<<1, WAVE, 0, s>>>(p));
   g_last_normals = LastNormalsCall{dev, s, p};
   *out = p;
   return RVP_OK;
@@ -266,7 +261,8 @@ __global__ __launch_bounds__(256) void normals_solve_k(const float *__restrict__
 // MODE 0: k nearest (N1, K = k+1 incl. self); MODE 1: radius list capped at K (N2)
 template <int MODE, bool STRIDED>
 __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, int h0, int nb,
-                                                      float *__restrict__ rec) {
+                                                      float *__restrict__ rec, unsigned long long *clear) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *clear = 0ull;  // the call's fallback counter
   __shared__ WaveLds lds[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
   constexpr bool STAGE = MODE == 1 && !STRIDED;
@@ -316,7 +312,9 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
 
 // N2 without a cap: neighbour positions in CSR rows (row = original index), covariance in chunks of 64
 __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *__restrict__ offsets, int cap,
-                                                     const uint32_t *__restrict__ pos, float *__restrict__ rec) {
+                                                     const uint32_t *__restrict__ pos, float *__restrict__ rec,
+                                                     unsigned long long *clear) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *clear = 0ull;  // the call's fallback counter
   const int w = wave_in_block();
   const uint32_t base = (blockIdx.x * QW + w) * NB;
   if (base >= g.n) return;
@@ -376,7 +374,9 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
 // MODE 1: radius[c] list capped at K)
 template <int MODE, bool STRIDED>
 __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius, int h0,
-                                                     int nb, float *__restrict__ rec, size_t n_bound) {
+                                                     int nb, float *__restrict__ rec, size_t n_bound,
+                                                     unsigned long long *clear) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *clear = 0ull;  // the call's fallback counter
   __shared__ WaveLds lds[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
   constexpr bool STAGE = MODE == 1 && !STRIDED;
@@ -472,9 +472,11 @@ int rvp_normals_knn_centroids(const float *x, const float *y, const float *z, si
   const GridView g = G.view();
   const int h0 = tune_knn_start_cells();
   if (strided)
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, true><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, h0, NB, rec.as<float>()));
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, true><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, h0, NB,
+                                                                              rec.as<float>(), fb));
   else
-    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, false><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, h0, NB, rec.as<float>()));
+    RVP_LAUNCH("normals_topk_k", s, normals_topk_k<0, false><<<qb, QT, 0, s>>>(g, k + 1, 0.0f, h0, NB,
+                                                                               rec.as<float>(), fb));
   return solve_normals(rec.as<float>(), n, nullptr, g.xs, g.ys, g.zs, g.perm, vpx, vpy, vpz, nx, ny, nz, fb, cx, cy,
                        cz, s);
 }
@@ -502,7 +504,7 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
     const int nb = radius_normals_batch();
     const unsigned qb = ceil_div(ceil_div(n, (size_t)nb), QW);
     RVP_LAUNCH("normals_topk_k", s, normals_topk_k<1, false><<<qb, QT, 0, s>>>(g, max_nn, radius, 1, nb,
-                                                                               rec.as<float>()));
+                                                                               rec.as<float>(), fb));
   } else {
     // unlimited (or a cap above 64): full lists, sorted by (d2, idx), truncated by the row offsets
     Tmp cnt, off, keys, pos;
@@ -519,7 +521,7 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
     const int cap = max_nn > 0 ? max_nn : 0x7fffffff;
     RVP_LAUNCH("normals_csr_k", s, normals_csr_k<<<ceil_div(n, QW * NB), QT, 0, s>>>(g, off.as<int64_t>(), cap,
                                                                                      pos.as<uint32_t>(),
-                                                                                     rec.as<float>()));
+                                                                                     rec.as<float>(), fb));
   }
   return solve_normals(rec.as<float>(), n, nullptr, g.xs, g.ys, g.zs, g.perm, vpx, vpy, vpz, nx, ny, nz, fb, nullptr,
                        nullptr, nullptr, s);
@@ -546,10 +548,10 @@ int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const 
   const int h0 = tune_knn_start_cells();
   if (strided)
     RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, NB, rec.as<float>(),
-                                                                            n_bound));
+                                                                            n_bound, fb));
   else
     RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, NB, rec.as<float>(),
-                                                                             n_bound));
+                                                                             n_bound, fb));
   return solve_normals(rec.as<float>(), n_bound, G.off + nclouds, A.xs, A.ys, A.zs, A.perm, vpx, vpy, vpz, nx, ny,
                        nz, fb, nullptr, nullptr, nullptr, s);
 }
@@ -574,7 +576,7 @@ int rvp_batch_normals_radius(const float *x, const float *y, const float *z, con
   const unsigned qb = ceil_div(ceil_div(n_bound, (size_t)nb), QW);
   const SegArrays A = G.arrays();
   RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(A, max_nn, radius, 1, nb,
-                                                                           rec.as<float>(), n_bound));
+                                                                           rec.as<float>(), n_bound, fb));
   return solve_normals(rec.as<float>(), n_bound, G.off + nclouds, A.xs, A.ys, A.zs, A.perm, vpx, vpy, vpz, nx, ny,
                        nz, fb, nullptr, nullptr, nullptr, s);
 }

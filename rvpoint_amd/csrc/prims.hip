@@ -838,6 +838,101 @@ __global__ void scatter_aos_k(const float *__restrict__ x, const float *__restri
   }
 }
 
+// SOR's keep rule and the stable compaction of a batch in one pass (statistical_outlier_removal.cpp:53-62): point
+// i of cloud c is kept iff means[i] <= fl(sums[c] / n_c) + fl(alpha * sqrtf(fl(sqs[c] / n_c))).  keep[i] (may be
+// null) receives the flag for every i < n_bound (0 past the batch), kept points go to out_aos in index order,
+// out_off[c] = the first output row of cloud c and out_off[ncl] = the kept total.  The launch is sized for
+// n_bound; only the tiles of off[ncl] points take tickets, the other blocks clear their share of keep.
+__global__ __launch_bounds__(PRIM_THREADS) void seg_keep_compact_k(
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+    const float *__restrict__ means, const uint32_t *__restrict__ off, int ncl, size_t n_bound,
+    const float *__restrict__ sums, const float *__restrict__ sqs, float alpha, uint8_t *__restrict__ keep,
+    float *__restrict__ out_aos, uint32_t *__restrict__ out_off, uint64_t *__restrict__ status,
+    uint64_t *__restrict__ ctl) {
+  __shared__ uint32_t sm[PRIM_THREADS / WAVE];
+  __shared__ uint32_t s_tile, s_prefix, s_epoch;
+  const int t = threadIdx.x;
+  const uint32_t nn = off[ncl];  // <= n_bound (the offsets are a clamped partition)
+  const uint32_t nb = (nn + SCAN_TILE - 1) / SCAN_TILE;
+  if (blockIdx.x >= nb) {
+    if (keep)
+      for (size_t i = (size_t)blockIdx.x * SCAN_TILE + t; i < min((size_t)(blockIdx.x + 1) * SCAN_TILE, n_bound);
+           i += PRIM_THREADS)
+        keep[i] = 0;
+    if (nb == 0 && blockIdx.x == 0)
+      for (int c = t; c <= ncl; c += PRIM_THREADS) out_off[c] = 0u;
+    return;
+  }
+  const uint32_t tile = take_tile(ctl, nb, &s_tile, &s_epoch);
+  const uint32_t epoch = s_epoch;
+  if (tile >= nb) return;
+  const size_t i0 = (size_t)tile * SCAN_TILE + (size_t)t * SCAN_ITEMS;
+  // the cloud of the thread's first point (largest c with off[c] <= i), then advanced point by point
+  int c = 0;
+  if (i0 < nn) {
+    int lo = 0, hi = ncl - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= i0) lo = mid;
+      else hi = mid - 1;
+    }
+    c = lo;
+  }
+  int cthr = -1;
+  float thr = 0.0f;
+  uint32_t f[SCAN_ITEMS], r[SCAN_ITEMS];
+  int cl[SCAN_ITEMS];
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = i0 + j;
+    f[j] = 0u;
+    cl[j] = -1;
+    if (i < nn) {
+      while (c + 1 < ncl && off[c + 1] <= i) ++c;
+      if (c != cthr) {
+        const float len = __uint2float_rn(off[c + 1] - off[c]);
+        thr = __fadd_rn(__fdiv_rn(sums[c], len), __fmul_rn(alpha, sqrtf(__fdiv_rn(sqs[c], len))));
+        cthr = c;
+      }
+      f[j] = means[i] <= thr ? 1u : 0u;
+      cl[j] = c;
+    }
+    if (keep && i < n_bound) keep[i] = (uint8_t)f[j];
+    r[j] = f[j];
+  }
+  const uint32_t agg = tile_scan_items(r, status, tile, epoch, sm, &s_prefix);
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = i0 + j;
+    if (f[j] && out_aos) {
+      out_aos[3 * (size_t)r[j]] = x[i];
+      out_aos[3 * (size_t)r[j] + 1] = y[i];
+      out_aos[3 * (size_t)r[j] + 2] = z[i];
+    }
+    // a cloud's first point gives the cloud's first output row, also to the empty clouds just before it
+    if (cl[j] >= 0 && off[cl[j]] == i)
+      for (int e = cl[j]; e >= 0 && off[e] == i; --e) out_off[e] = r[j];
+  }
+  if (tile == nb - 1 && t == 0) {
+    const uint32_t total = s_prefix + agg;
+    for (int e = ncl; e >= 0 && (e == ncl || off[e] >= nn); --e) out_off[e] = total;
+  }
+}
+
+int seg_keep_compact(const float *x, const float *y, const float *z, const float *means, const uint32_t *off,
+                     int ncl, size_t n_bound, const float *sums, const float *sqs, float alpha, uint8_t *keep,
+                     float *out_aos, uint32_t *out_off, hipStream_t s) {
+  RVP_CHECK(n_bound < 0xffffffffull, RVP_E_UNSUPPORTED, "compaction: more than 2^32-1 points");
+  const size_t nb = std::max<size_t>(1, ceil_div(n_bound, SCAN_TILE));
+  uint64_t *status = nullptr;
+  uint64_t *ctl = nullptr;
+  RVP_TRY(scan_state(s, nb, &status, &ctl));
+  RVP_LAUNCH("seg_keep_compact_k", s, seg_keep_compact_k<<<(unsigned)nb, PRIM_THREADS, 0, s>>>(
+                                          x, y, z, means, off, ncl, n_bound, sums, sqs, alpha, keep, out_aos,
+                                          out_off, status, ctl));
+  return RVP_OK;
+}
+
 int compact_flagged(const float *x, const float *y, const float *z, uint32_t *pos, size_t n, const uint32_t *n_dev,
                     float *out_aos, uint32_t *count_dev, hipStream_t s) {
   if (n == 0) {

@@ -81,6 +81,14 @@ int compact_flagged(const float *x, const float *y, const float *z, uint32_t *po
 int compact_xyz_aos_flagged(const float *x, const float *y, const float *z, const uint8_t *flags, size_t n,
                             float *out_aos, uint32_t *pos, uint32_t *count_dev, hipStream_t s);
 
+// SOR's keep rule and stable compaction over a batch of ncl clouds [off[c], off[c+1]) in one launch: point i is
+// kept iff means[i] <= fl(sums[c]/n_c) + fl(alpha * sqrtf(fl(sqs[c]/n_c))); keep (may be null) gets the u8 flags
+// of all n_bound slots, out_aos the kept points in index order, out_off[0..ncl] each cloud's first output row and
+// the kept total.  off must be a clamped partition (off[ncl] <= n_bound); nothing is read on the host.
+int seg_keep_compact(const float *x, const float *y, const float *z, const float *means, const uint32_t *off,
+                     int ncl, size_t n_bound, const float *sums, const float *sqs, float alpha, uint8_t *keep,
+                     float *out_aos, uint32_t *out_off, hipStream_t s);
+
 inline int bits_for(uint64_t max_value) {
   int b = 0;
   while (b < 64 && (max_value >> b) != 0) ++b;

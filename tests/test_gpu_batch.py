@@ -188,11 +188,19 @@ def test_batch_voxel_wide_and_negative_keys():
         assert (vox[c] == ref).all()
 
 
-def test_batch_normals_radius_equal_single_cloud(clouds):
+@pytest.mark.parametrize("per_wave", [0, 3, 64])
+def test_batch_normals_radius_equal_single_cloud(clouds, per_wave):
+    """Batched radius normals with 8 (default), 3 or 64 consecutive queries per wave (batches that share staged
+    windows, cloud boundaries inside them) equal the single-cloud call at default settings."""
     cl, scales = clouds
     b = R.Batch(cl)
     radius = np.array([0.03 * s for s in scales], np.float32)
-    nrm = R.batch_normals_radius(b, radius, 32)
+    old = R._lib.tune("normals_per_wave")
+    try:
+        R._lib.tune("normals_per_wave", per_wave)
+        nrm = R.batch_normals_radius(b, radius, 32)
+    finally:
+        R._lib.tune("normals_per_wave", old)
     for c, p in enumerate(cl):
         if len(p) == 0:
             continue
