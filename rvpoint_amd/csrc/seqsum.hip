@@ -190,7 +190,8 @@ __global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, 
                                                     const uint32_t *__restrict__ coff, int nseg, int mode,
                                                     const float *__restrict__ mean_dev,
                                                     const double *__restrict__ csum,
-                                                    const double *__restrict__ csq, SSChunk *__restrict__ out) {
+                                                    const double *__restrict__ csq, SSChunk *__restrict__ out,
+                                                    int4 *__restrict__ lanes) {
   const uint32_t c = blockIdx.x;
   ChunkRef r;
   if (!chunk_ref(seg, coff, nseg, c, r)) return;
@@ -218,17 +219,44 @@ __global__ __launch_bounds__(64) void ss_compose_k(const float *__restrict__ v, 
   }
   p = wave_sum_f64(p);
   const double lo = p, hi = p + ss_chunk_pred(csum, csq, c, md, (double)(r.b - r.a));
-  int e = 0;
-  bool clean = false;
+  int e = 0, kind = 0;  // 1: one binade (a chunk map); 2: two adjacent binades (lane maps for the crossing)
   if (c > r.c0 && lo > 0.0 && hi < 1e38) {
     const int elo = ilogb(lo * (1.0 - 1e-3)), ehi = ilogb(hi * (1.0 + 1e-3));
     if (elo == ehi && elo >= -125 && elo <= 126) {
-      clean = true;
+      kind = 1;
+      e = elo;
+    } else if (ehi == elo + 1 && elo >= -125 && ehi <= 126 && lanes) {
+      kind = 2;
       e = elo;
     }
   }
-  if (!clean) {
+  if (kind == 0) {
     if (l == 0) out[c] = SSChunk{0, 0, 0, 0};
+    return;
+  }
+  if (kind == 2) {
+    // A chunk predicted to carry the sum from binade e into e + 1: per lane, the inclusive prefix of the lane
+    // maps in binade e (lanes 0..l) and the composition of the later lanes' maps in binade e + 1 (lanes
+    // l+1..63), so that the walk finds the crossing lane and finishes the chunk without building maps.
+    int fe = 0, fo = 0, ge = 0, go = 0;
+#pragma unroll
+    for (int q = 0; q < SS_LANE; ++q) {
+      int ae, ao;
+      ss_term_map(x[q], 23 - e, ae, ao);
+      ss_compose(fe, fo, ae, ao);
+      ss_term_map(x[q], 22 - e, ae, ao);
+      ss_compose(ge, go, ae, ao);
+    }
+    ss_scan_maps(fe, fo);
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {  // inclusive suffix: lanes l .. 63
+      const int pe = __shfl_down(ge, d), po = __shfl_down(go, d);
+      if (l + d < WAVE) ss_compose(ge, go, pe, po);
+    }
+    int se = __shfl_down(ge, 1), so = __shfl_down(go, 1);  // lanes l+1 .. 63
+    if (l == WAVE - 1) se = so = 0;
+    lanes[(size_t)c * WAVE + l] = make_int4(fe, fo, se, so);
+    if (l == 0) out[c] = SSChunk{0, 0, e, 2};
     return;
   }
   int fe = 0, fo = 0;
@@ -269,13 +297,13 @@ __device__ __forceinline__ SSTerms ss_load_terms(const float *__restrict__ v, ui
   return t;
 }
 
-__device__ __forceinline__ float ss_replay(float acc, const SSTerms &t) {
+__device__ __forceinline__ float ss_replay(float acc, const SSTerms &t, int start = 0) {
   const int l = lane();
   const float *x = t.x;
   bool nz = false;
 #pragma unroll
   for (int q = 0; q < SS_LANE; ++q) nz = nz || x[q] != 0.0f;
-  int pos = 0;
+  int pos = start;
   while (pos < WAVE) {
     if (acc >= 0x1p-126f && acc < 0x1p127f) {
       const int e = ilogbf(acc);
@@ -309,6 +337,30 @@ __device__ __forceinline__ float ss_replay(float acc, const SSTerms &t) {
   return acc;
 }
 
+// A chunk predicted to cross from binade e into e + 1 (lane maps from ss_compose_k, m = this lane's (prefix
+// map in e, map of the later lanes in e + 1)): the states after every lane in binade e in one step, the first
+// lane whose state leaves the binade replayed term by term by the fp32 adder, and the later lanes as one map
+// in binade e + 1, checked against 2^24 as every applied map is.  Anything else goes to ss_replay.
+__device__ __forceinline__ float ss_cross(float acc, const SSTerms &t, int4 m, int e) {
+  if (!(acc >= 0x1p-126f && acc < 0x1p127f) || ilogbf(acc) != e) return ss_replay(acc, t);
+  const int M0 = (int)ldexpf(acc, 23 - e);
+  const int Ml = M0 + ((M0 & 1) ? m.y : m.x);
+  const uint64_t bad = ballot(Ml >= (1 << 24));
+  if (!bad) return ldexpf((float)__builtin_amdgcn_readlane(Ml, WAVE - 1), e - 23);
+  const int c = __ffsll((unsigned long long)bad) - 1;  // lanes [0, c) stay in binade e
+  if (c > 0) acc = ldexpf((float)__builtin_amdgcn_readlane(Ml, c - 1), e - 23);
+#pragma unroll
+  for (int r = 0; r < SS_LANE; ++r) acc = acc + readlanef(t.x[r], c);
+  if (c == WAVE - 1) return acc;
+  if (acc >= 0x1p-126f && acc < 0x1p127f && ilogbf(acc) == e + 1) {
+    const int M1 = (int)ldexpf(acc, 22 - e);
+    const int se = __builtin_amdgcn_readlane(m.z, c), so = __builtin_amdgcn_readlane(m.w, c);
+    const int M2 = M1 + ((M1 & 1) ? so : se);
+    if (M2 < (1 << 24)) return ldexpf((float)M2, e - 22);
+  }
+  return ss_replay(acc, t, c + 1);
+}
+
 // One wave per segment walks its chunks in order, 64 at a time (lane j holds chunk j's map).  A run of
 // consecutive clean chunks predicted in the running sum's binade is applied at once: an inclusive scan
 // composes their maps across the lanes, every lane evaluates the state after its chunk from the run's
@@ -318,7 +370,8 @@ __device__ __forceinline__ float ss_replay(float acc, const SSTerms &t) {
 __global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, const uint32_t *__restrict__ seg,
                                                   const uint32_t *__restrict__ coff, int nseg, int mode,
                                                   const float *__restrict__ mean_dev,
-                                                  const SSChunk *__restrict__ ch, float *__restrict__ out) {
+                                                  const SSChunk *__restrict__ ch, const int4 *__restrict__ lanes,
+                                                  float *__restrict__ out) {
   const int s = blockIdx.x;
   if (s >= nseg) return;
   const int l = lane();
@@ -332,16 +385,20 @@ __global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, co
   for (uint32_t c0 = c_begin; c0 < c_end; c0 += WAVE) {
     const SSChunk nxt = (c0 + WAVE + l < c_end) ? ch[c0 + WAVE + l] : SSChunk{0, 0, 0, 0};
     const int cnt = (int)min<uint32_t>(WAVE, c_end - c0);
-    const uint64_t unclean = ballot(l < cnt && !mine.clean);
+    const uint64_t unclean = ballot(l < cnt && mine.clean != 1);
     int pf = unclean ? __ffsll((unsigned long long)unclean) - 1 : -1;
     SSTerms pft;
-    if (pf >= 0) pft = ss_load_terms(v, e_begin + (c0 + pf - c_begin) * SS_C, e_end, mode, mean);
+    int4 pfm = make_int4(0, 0, 0, 0);  // the crossing lane maps of chunk pf, when it has them
+    if (pf >= 0) {
+      pft = ss_load_terms(v, e_begin + (c0 + pf - c_begin) * SS_C, e_end, mode, mean);
+      if (__builtin_amdgcn_readlane(mine.clean, pf) == 2) pfm = lanes[(size_t)(c0 + pf) * WAVE + l];
+    }
     int pos = 0;
     while (pos < cnt) {
       // binade of the running sum (normal, finite, positive), else no map applies
       int e_acc = -100000;
       if (acc >= 0x1p-126f && acc < 0x1p127f) e_acc = ilogbf(acc);
-      const uint64_t elig = ballot(l >= pos && l < cnt && mine.clean && mine.e == e_acc);
+      const uint64_t elig = ballot(l >= pos && l < cnt && mine.clean == 1 && mine.e == e_acc);
       const uint64_t stop = ~elig & (~0ull << pos);
       const int run_end = stop ? min(cnt, __ffsll((unsigned long long)stop) - 1) : cnt;
       if (run_end > pos) {
@@ -361,13 +418,18 @@ __global__ __launch_bounds__(64) void ss_drive_k(const float *__restrict__ v, co
           continue;
         }
       }
+      const int kind = __builtin_amdgcn_readlane(mine.clean, pos), ek = __builtin_amdgcn_readlane(mine.e, pos);
       if (pos == pf) {
-        acc = ss_replay(acc, pft);
+        acc = kind == 2 ? ss_cross(acc, pft, pfm, ek) : ss_replay(acc, pft);
         const uint64_t later = unclean & (~0ull << pos) & ~(1ull << pos);
         pf = later ? __ffsll((unsigned long long)later) - 1 : -1;
-        if (pf >= 0) pft = ss_load_terms(v, e_begin + (c0 + pf - c_begin) * SS_C, e_end, mode, mean);
+        if (pf >= 0) {
+          pft = ss_load_terms(v, e_begin + (c0 + pf - c_begin) * SS_C, e_end, mode, mean);
+          if (__builtin_amdgcn_readlane(mine.clean, pf) == 2) pfm = lanes[(size_t)(c0 + pf) * WAVE + l];
+        }
       } else {
-        acc = ss_replay(acc, ss_load_terms(v, e_begin + (c0 + pos - c_begin) * SS_C, e_end, mode, mean));
+        const SSTerms tt = ss_load_terms(v, e_begin + (c0 + pos - c_begin) * SS_C, e_end, mode, mean);
+        acc = kind == 2 ? ss_cross(acc, tt, lanes[(size_t)(c0 + pos) * WAVE + l], ek) : ss_replay(acc, tt);
       }
       ++pos;
     }
@@ -394,21 +456,22 @@ static int ss_chunk_offsets(const uint32_t *seg_off, int nseg, size_t n_bound, T
 // chunk-sum pass.
 static int ss_run(const float *v, const uint32_t *seg_off, const uint32_t *coff, int nseg, size_t bound, int mode,
                   const float *mean_dev, float *out, double *moments, hipStream_t s) {
-  Tmp cs, ch;
+  Tmp cs, ch, lanes;
   double *csum = moments, *csq = moments ? moments + bound : nullptr;
   if (!moments) {
     RVP_TRY(cs.alloc(bound * sizeof(double), s));
     csum = cs.as<double>();
   }
   RVP_TRY(ch.alloc(bound * sizeof(SSChunk), s));
+  RVP_TRY(lanes.alloc(bound * WAVE * sizeof(int4), s));
   if (mode != 2 || !moments)
     RVP_LAUNCH("ss_chunksum_k", s, ss_chunksum_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff, nseg, mode,
                                                                                 mean_dev, csum, csq));
   RVP_LAUNCH("ss_compose_k", s, ss_compose_k<<<(unsigned)bound, 64, 0, s>>>(v, seg_off, coff, nseg, mode, mean_dev,
                                                                             csum, mode == 2 ? csq : nullptr,
-                                                                            ch.as<SSChunk>()));
+                                                                            ch.as<SSChunk>(), lanes.as<int4>()));
   RVP_LAUNCH("ss_drive_k", s, ss_drive_k<<<nseg, 64, 0, s>>>(v, seg_off, coff, nseg, mode, mean_dev,
-                                                             ch.as<SSChunk>(), out));
+                                                             ch.as<SSChunk>(), lanes.as<int4>(), out));
   return RVP_OK;
 }
 
