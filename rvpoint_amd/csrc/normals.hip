@@ -184,16 +184,70 @@ __device__ __forceinline__ void flip_toward(float px, float py, float pz, float 
   }
 }
 
-// covariance of the m <= 64 neighbours held one per lane (payload = cell-order position), coordinates
-// re-centred at the query point, two passes (mean, then centred products) with fp64 terms and sums; result
-// uniform across the wave, rounded to fp32.  c[6..8] receives the neighbourhood mean relative to the query.
-__device__ __forceinline__ void wave_cov(const GridView &g, bool valid, uint32_t pos, int m, float qx, float qy,
-                                         float qz, float c[9]) {
+// Covariance of the m <= 64 neighbours held one per lane, coordinates re-centred at the query point, with
+// sums that do not depend on which lane holds which neighbour (a neighbourhood found without sorting arrives
+// in an order that can vary between runs).  Every lane rounds its offsets p, and its products p_a p_b, to
+// integer multiples of 2^(E - 46), resp. 2^(2E - 46), where 2^E bounds every |p| of the wave: at most 64
+// terms below 2^46 units each sum exactly in fp64, so the totals are the same for any lane order.  The
+// rounding is 2^-47 of the bound, far below fp32.  Then sum((p - mu)(p - mu)^T) = sum(p p^T) - m mu mu^T.
+// Result uniform across the wave, rounded to fp32; c[6..8] receives the neighbourhood mean relative to the
+// query.
+__device__ __forceinline__ void wave_cov_unordered(bool valid, float px, float py, float pz, int m, float qx,
+                                                   float qy, float qz, float c[9]) {
   double p[3] = {0.0, 0.0, 0.0};
   if (valid) {
-    p[0] = (double)g.xs[pos] - (double)qx;
-    p[1] = (double)g.ys[pos] - (double)qy;
-    p[2] = (double)g.zs[pos] - (double)qz;
+    p[0] = (double)px - (double)qx;
+    p[1] = (double)py - (double)qy;
+    p[2] = (double)pz - (double)qz;
+  }
+  // E: the wave's largest binary exponent of |p| (|p| < 2^E)
+  int e = -1074;
+  const double am = fmax(fmax(fabs(p[0]), fabs(p[1])), fabs(p[2]));
+  if (am > 0.0) e = __builtin_amdgcn_frexp_exp(am);
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) e = max(e, shfl_xor(e, d));
+  e = max(e, -460);  // all-zero offsets (or nearly): any scale leaves them exact
+  double s1[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) s1[a] = rint(ldexp(p[a], 46 - e));
+  const double pr[6] = {rint(ldexp(p[0] * p[0], 46 - 2 * e)), rint(ldexp(p[0] * p[1], 46 - 2 * e)),
+                        rint(ldexp(p[0] * p[2], 46 - 2 * e)), rint(ldexp(p[1] * p[1], 46 - 2 * e)),
+                        rint(ldexp(p[1] * p[2], 46 - 2 * e)), rint(ldexp(p[2] * p[2], 46 - 2 * e))};
+  double mu[3], s2[6];
+  wave_sum3_f64(s1, mu);
+  wave_sum6_f64(pr, s2);
+  const double inv = m > 0 ? 1.0 / (double)m : 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) mu[a] = ldexp(mu[a], e - 46) * inv;
+  const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) c[k] = (float)(ldexp(s2[k], 2 * e - 46) - (double)m * mu[ia[k]] * mu[ib[k]]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) c[6 + a] = (float)mu[a];
+}
+
+// The same for neighbours held in a canonical lane order (sorted by (d2, idx)), whose fp64 sums are therefore
+// reproducible as they stand: two passes (mean, then centred products) with fp64 terms and sums.  UNORDERED
+// selects wave_cov_unordered (the same results for any lane order).  The neighbour is at cell-order position
+// pos of the grid.
+template <bool UNORDERED = false>
+__device__ __forceinline__ void wave_cov(const GridView &g, bool valid, uint32_t pos, int m, float qx, float qy,
+                                         float qz, float c[9]) {
+  float px = 0.0f, py = 0.0f, pz = 0.0f;
+  if (valid) {
+    px = g.xs[pos];
+    py = g.ys[pos];
+    pz = g.zs[pos];
+  }
+  if (UNORDERED) {
+    wave_cov_unordered(valid, px, py, pz, m, qx, qy, qz, c);
+    return;
+  }
+  double p[3] = {0.0, 0.0, 0.0};
+  if (valid) {
+    p[0] = (double)px - (double)qx;
+    p[1] = (double)py - (double)qy;
+    p[2] = (double)pz - (double)qz;
   }
   double mu[3];
   wave_sum3_f64(p, mu);
@@ -294,8 +348,20 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
         TopKSink<false> sink{t, L};
         if constexpr (STAGE) {
           win.ensure(g, lo, hi, 0, L, stage[w]);
-          if (win.total >= 0) scan_staged(stage[w], win.total, x, y, z, r2, sink);
-          else scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
+          if (win.total >= 0) {
+            // at most K hits: they are the capped list, covariance straight from the staged coordinates
+            float hx, hy, hz;
+            const int h = staged_hits_upto(stage[w], win.total, x, y, z, r2, K, L, hx, hy, hz);
+            if (h >= 0) {
+              float c[9];
+              wave_cov_unordered(lane() < h, hx, hy, hz, h, x, y, z, c);
+              write_rec(rec, g.n, q, c, h);
+              continue;
+            }
+            scan_staged(stage[w], win.total, x, y, z, r2, sink);
+          } else {
+            scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
+          }
         } else {
           scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, L, sink);
         }
@@ -303,8 +369,9 @@ __global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float ra
       }
       const int m = min(t.cnt, K);
       float c[9];
-      // the list holds (d2, index) keys only: the neighbour's cell-order position is rank[index]
-      wave_cov(g, lane() < m, lane() < m ? g.rank[key_idx(t.key)] : 0u, m, x, y, z, c);
+      // the list holds (d2, index) keys only: the neighbour's cell-order position is rank[index]; radius lists
+      // use the formula of their unordered fast path
+      wave_cov<MODE == 1>(g, lane() < m, lane() < m ? g.rank[key_idx(t.key)] : 0u, m, x, y, z, c);
       write_rec(rec, g.n, q, c, m);
     }
   }
@@ -413,8 +480,20 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
         TopKSink<false> sink{t, lds[w]};
         if constexpr (STAGE) {
           win.ensure(g, lo, hi, c, lds[w], stage[w]);
-          if (win.total >= 0) scan_staged(stage[w], win.total, x, y, z, r2, sink);
-          else scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, lds[w], sink);
+          if (win.total >= 0) {
+            // at most K hits: they are the capped list, covariance straight from the staged coordinates
+            float hx, hy, hz;
+            const int h = staged_hits_upto(stage[w], win.total, x, y, z, r2, K, lds[w], hx, hy, hz);
+            if (h >= 0) {
+              float c6[9];
+              wave_cov_unordered(lane() < h, hx, hy, hz, h, x, y, z, c6);
+              write_rec(rec, n_bound, q, c6, h);
+              continue;
+            }
+            scan_staged(stage[w], win.total, x, y, z, r2, sink);
+          } else {
+            scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, lds[w], sink);
+          }
         } else {
           scan_box(g, x, y, z, lo, hi, false, lo, hi, r2, true, lds[w], sink);
         }
@@ -422,7 +501,7 @@ __global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const fl
       }
       const int m = min(t.cnt, K);
       float c6[9];
-      wave_cov(g, lane() < m, lane() < m ? g.rank[key_idx(t.key)] : 0u, m, x, y, z, c6);
+      wave_cov<MODE == 1>(g, lane() < m, lane() < m ? g.rank[key_idx(t.key)] : 0u, m, x, y, z, c6);
       write_rec(rec, n_bound, q, c6, m);
     }
   }

@@ -45,9 +45,11 @@ __global__ __launch_bounds__(QT) void radius_topk_k(GridView g, const float *__r
     topk_flush<false>(t, L);
     const int m = min(t.cnt, K);
     const int l = lane();
-    if (l < m) {
-      if (idx_out) idx_out[(size_t)qj.row * K + l] = (int32_t)key_idx(t.key);
-      if (d2_out) d2_out[(size_t)qj.row * K + l] = key_d2(t.key);
+    // whole rows: a row of max_nn words is written by one store per array, so the caches take full lines
+    // without first reading them (entries past the count hold index -1 and d2 +inf)
+    if (l < K) {
+      if (idx_out) idx_out[(size_t)qj.row * K + l] = l < m ? (int32_t)key_idx(t.key) : -1;
+      if (d2_out) d2_out[(size_t)qj.row * K + l] = l < m ? key_d2(t.key) : __builtin_inff();
     }
     if (l == j) {
       my_count = m;

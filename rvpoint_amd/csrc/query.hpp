@@ -21,7 +21,6 @@ constexpr int MAX_TOPK = 64;    // entries one wave keeps in registers
 
 struct WaveLds {
   uint64_t bk[WAVE];   // staged hit keys
-  uint32_t bp[WAVE];   // staged payloads (cell-order position)
   uint32_t excl[WAVE]; // row-slice prefix offsets
   uint32_t s0[WAVE], l0[WAVE], s1[WAVE];
 };
@@ -148,11 +147,12 @@ __device__ __forceinline__ int pow2_at_least(int v) {
 // W = pow2 >= K lanes: the staged batch is sorted in groups of pow2 >= nb lanes, its W smallest are
 // reversed onto lanes [0, W) and min'ed with the list (a bitonic sequence), and W lanes are cleaned.
 template <bool PAY, class KT> __device__ __forceinline__ void topk_flush(TopKT<KT> &t, WaveLds &L) {
+  static_assert(!PAY, "staged hits carry keys only");
   if (t.nb == 0) return;
   const int l = lane();
   wave_lds_sync();
   KT nk = l < t.nb ? (KT)L.bk[l] : ~(KT)0;
-  uint32_t np = (PAY && l < t.nb) ? L.bp[l] : 0u;
+  uint32_t np = 0u;
   wave_lds_sync();
   const int sb = pow2_at_least(max(t.nb, 2));
   bitonic_sort_groups<PAY>(nk, np, sb);
@@ -180,7 +180,8 @@ template <bool PAY, class KT> __device__ __forceinline__ void topk_flush(TopKT<K
 }
 
 template <bool PAY, class KT>
-__device__ __forceinline__ void topk_offer(TopKT<KT> &t, WaveLds &L, bool hit, KT key, uint32_t pay) {
+__device__ __forceinline__ void topk_offer(TopKT<KT> &t, WaveLds &L, bool hit, KT key, uint32_t) {
+  static_assert(!PAY, "staged hits carry keys only");
   t.hits += (uint32_t)popc(ballot(hit));
   bool pass = hit && key < t.kth;
   uint64_t m = ballot(pass);
@@ -196,7 +197,6 @@ __device__ __forceinline__ void topk_offer(TopKT<KT> &t, WaveLds &L, bool hit, K
   if (pass) {
     const int pos = t.nb + popc(m & lanemask_lt());
     L.bk[pos] = key;
-    if (PAY) L.bp[pos] = pay;
   }
   t.nb += np;
 }
@@ -251,24 +251,54 @@ __device__ __forceinline__ void scan_box(const GridView &g, float qx, float qy, 
     // rows of this pass: the search below needs only as many halvings as its row count
     const int pass_rows = min(WAVE, nrows - r0);
     const int top = pass_rows > 16 ? WAVE / 2 : pass_rows > 4 ? 8 : 2;
-    for (uint32_t c0 = 0; c0 < total; c0 += WAVE) {
-      const uint32_t v = c0 + (uint32_t)l;
-      const bool valid = v < total;
+    // candidate v of the pass -> its cell-order position
+    auto position = [&](uint32_t v) {
       int o = 0;
 #pragma unroll
       for (int st = WAVE / 2; st > 0; st >>= 1)
         if (st <= top && L.excl[o + st] <= v) o += st;
       const uint32_t off = v - L.excl[o];
-      const uint32_t pos = off < L.l0[o] ? L.s0[o] + off : L.s1[o] + (off - L.l0[o]);
-      float d2 = 0.0f;
-      uint32_t idx = 0;
-      bool hit = false;
-      if (valid) {
-        d2 = dist2(g.xs[pos], g.ys[pos], g.zs[pos], qx, qy, qz);
-        idx = g.perm[pos];
-        hit = radius_mode ? (d2 <= r2) : (d2 == d2);
+      return off < L.l0[o] ? L.s0[o] + off : L.s1[o] + (off - L.l0[o]);
+    };
+    // two groups of 64 per step: the second group's loads are in flight while the first is offered
+    for (uint32_t c0 = 0; c0 < total; c0 += 2 * WAVE) {
+      const uint32_t v0 = c0 + (uint32_t)l, v1 = v0 + WAVE;
+      const bool two = c0 + WAVE < total;
+      const bool ok0 = v0 < total, ok1 = v1 < total;
+      float x0 = 0.f, y0 = 0.f, z0 = 0.f, x1 = 0.f, y1 = 0.f, z1 = 0.f;
+      uint32_t i0 = 0, i1 = 0, p0 = 0, p1 = 0;
+      if (ok0) {
+        p0 = position(v0);
+        x0 = g.xs[p0];
+        y0 = g.ys[p0];
+        z0 = g.zs[p0];
+        i0 = g.perm[p0];
       }
-      sink(valid, hit, pack_key(d2, idx), pos);
+      if (two && ok1) {
+        p1 = position(v1);
+        x1 = g.xs[p1];
+        y1 = g.ys[p1];
+        z1 = g.zs[p1];
+        i1 = g.perm[p1];
+      }
+      {
+        float d2 = 0.0f;
+        bool hit = false;
+        if (ok0) {
+          d2 = dist2(x0, y0, z0, qx, qy, qz);
+          hit = radius_mode ? (d2 <= r2) : (d2 == d2);
+        }
+        sink(ok0, hit, pack_key(d2, i0), p0);
+      }
+      if (two) {
+        float d2 = 0.0f;
+        bool hit = false;
+        if (ok1) {
+          d2 = dist2(x1, y1, z1, qx, qy, qz);
+          hit = radius_mode ? (d2 <= r2) : (d2 == d2);
+        }
+        sink(ok1, hit, pack_key(d2, i1), p1);
+      }
     }
     wave_lds_sync();
   }
@@ -354,16 +384,27 @@ __device__ __forceinline__ int stage_box(const GridView &g, const int lo[3], con
   L.s0[l] = s0;
   wave_lds_sync();
   const int top = nrows > 16 ? WAVE / 2 : nrows > 4 ? 8 : 2;
-  for (uint32_t c0 = 0; c0 < total; c0 += WAVE) {
-    const uint32_t v = c0 + (uint32_t)l;
-    if (v < total) {
+  // every group's gathers are issued before the first is stored, so the staging waits for one round trip to
+  // memory instead of one per group
+  constexpr int NG = STAGE_CAP / WAVE;
+  float4 c[NG];
+#pragma unroll
+  for (int gi = 0; gi < NG; ++gi) {
+    const uint32_t v = (uint32_t)(gi * WAVE + l);
+    c[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((uint32_t)(gi * WAVE) < total && v < total) {
       int o = 0;
 #pragma unroll
       for (int st = WAVE / 2; st > 0; st >>= 1)
         if (st <= top && L.excl[o + st] <= v) o += st;
       const uint32_t pos = L.s0[o] + (v - L.excl[o]);
-      S.c[v] = make_float4(g.xs[pos], g.ys[pos], g.zs[pos], __uint_as_float(g.perm[pos]));
+      c[gi] = make_float4(g.xs[pos], g.ys[pos], g.zs[pos], __uint_as_float(g.perm[pos]));
     }
+  }
+#pragma unroll
+  for (int gi = 0; gi < NG; ++gi) {
+    const uint32_t v = (uint32_t)(gi * WAVE + l);
+    if (v < total) S.c[v] = c[gi];
   }
   wave_lds_sync();
   return (int)total;
@@ -387,6 +428,50 @@ __device__ __forceinline__ void scan_staged(const StageLds &S, int total, float 
     }
     sink(valid, hit, pack_key(d2, idx), 0u);
   }
+}
+
+// The radius hits of a staged window for callers that read the neighbour set and not its order (covariances):
+// when at most K staged points lie within r2 of the query they ARE the list capped at K, so no selection or sort
+// is needed.  Returns the hit count H <= K, lane j < H holding hit j's coordinates (staging order); -1 when more
+// than K points are hits (the caller then needs the K nearest by (d2, idx)).  Uses L.excl as scratch.
+__device__ __forceinline__ int staged_hits_upto(const StageLds &S, int total, float qx, float qy, float qz,
+                                                float r2, int K, WaveLds &L, float &hx, float &hy, float &hz) {
+  constexpr int NG = STAGE_CAP / WAVE;
+  const int l = lane();
+  uint64_t m[NG];
+  int H = 0;
+#pragma unroll
+  for (int gi = 0; gi < NG; ++gi) {
+    m[gi] = 0;
+    if (gi * WAVE < total) {
+      const int v = gi * WAVE + l;
+      bool hit = false;
+      if (v < total) {
+        const float4 p = S.c[v];
+        hit = dist2(p.x, p.y, p.z, qx, qy, qz) <= r2;
+      }
+      m[gi] = ballot(hit);
+      H += popc(m[gi]);
+    }
+  }
+  if (H > K) return -1;
+  int base = 0;
+  wave_lds_sync();
+#pragma unroll
+  for (int gi = 0; gi < NG; ++gi) {
+    if ((m[gi] >> l) & 1ull) L.excl[base + popc(m[gi] & lanemask_lt())] = (uint32_t)(gi * WAVE + l);
+    base += popc(m[gi]);
+  }
+  wave_lds_sync();
+  hx = hy = hz = 0.0f;
+  if (l < H) {
+    const float4 p = S.c[L.excl[l]];
+    hx = p.x;
+    hy = p.y;
+    hz = p.z;
+  }
+  wave_lds_sync();
+  return H;
 }
 
 // Window cache of a run of queries: restages only when a query's window differs from the staged one.
