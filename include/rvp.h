@@ -57,6 +57,9 @@ int rvp_event_create(void **ev);
 int rvp_event_destroy(void *ev);
 int rvp_event_record(void *ev, rvp_stream_t s);
 int rvp_event_elapsed(void *start, void *stop, float *ms);
+/* later work on stream s waits for the last record of event ev (no counterpart in the reference, which has no
+ * device): lets one stream consume what another produced without a host wait */
+int rvp_stream_wait_event(rvp_stream_t s, void *ev);
 
 /* AoS float[3n] -> SoA x/y/z (the copy the reference's drivers make between stages,
  * src/tools/pipeline_export.cpp:336-342) */

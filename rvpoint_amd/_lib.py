@@ -50,6 +50,7 @@ _SIGS = {
     "rvp_event_destroy": [vp],
     "rvp_event_record": [vp, vp],
     "rvp_event_elapsed": [vp, vp, pfl],
+    "rvp_stream_wait_event": [vp, vp],
     "rvp_aos_to_soa": [vp, sz, vp, vp, vp, vp],
     "rvp_aos_to_soa_bounded": [vp, sz, vp, vp, vp, vp, vp],
     "rvp_graph_capture_begin": [vp],

@@ -422,6 +422,12 @@ int rvp_event_record(void *ev, rvp_stream_t s) {
   return RVP_OK;
 }
 
+int rvp_stream_wait_event(rvp_stream_t s, void *ev) {
+  RVP_CHECK(ev, RVP_E_ARG, "null event");
+  RVP_HIP(hipStreamWaitEvent(S(s), static_cast<hipEvent_t>(ev), 0));
+  return RVP_OK;
+}
+
 int rvp_event_elapsed(void *a, void *b, float *ms) {
   RVP_CHECK(ms, RVP_E_ARG, "null argument");
   RVP_HIP(hipEventSynchronize(static_cast<hipEvent_t>(b)));

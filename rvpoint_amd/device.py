@@ -29,6 +29,29 @@ class Stream:
             pass
 
 
+class Event:
+    """A HIP event: recorded on one stream, waited for by another (rvp_stream_wait_event) or timed."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        call("rvp_event_create", C.byref(h))
+        self.handle = h.value
+
+    def record(self, stream):
+        call("rvp_event_record", self.handle, stream)
+
+    def close(self):
+        if self.handle:
+            lib().rvp_event_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceArray:
     """A contiguous device allocation with a numpy dtype and shape."""
 

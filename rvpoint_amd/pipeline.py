@@ -99,8 +99,8 @@ class DeviceFramePipeline:
         call("rvp_batch_normals_radius", self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, 1, n,
              self.radius.ptr, None, st.max_nn, *st.vp, self.nx.ptr, self.ny.ptr, self.nz.ptr, s)
 
-    def _ransac(self):
-        st, s = self.st, self.s
+    def _ransac(self, stream=None):
+        st, s = self.st, (self.s if stream is None else stream)
         P = C.POINTER
         call("rvp_batch_ransac_plane", self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, 1, self.n_max,
              st.thr, st.iters, st.collinear, st.probability, self.model.ctypes.data_as(P(C.c_float)),
@@ -115,16 +115,25 @@ class DeviceFramePipeline:
             call("rvp_graph_capture_end", self.s, C.byref(g))
         self.graph = g.value
 
-    def step(self):
+    def prepare(self):
+        """Graph mode: the eager first run (first-use allocations happen outside the capture) and the capture;
+        nothing otherwise, or once done."""
         if self.use_graph and self.graph is None:
-            self._device_stages()  # the eager warm-up: first-use allocations happen outside the capture
+            self._device_stages()
             self._ransac()
             self._capture()
             call("rvp_stream_sync", self.s)
+
+    def launch(self):
+        """Enqueue voxel -> SOR -> normals on the pipeline's stream (the captured graph when there is one)."""
+        self.prepare()
         if self.graph is not None:
             call("rvp_graph_launch", self.graph, self.s)
         else:
             self._device_stages()
+
+    def step(self):
+        self.launch()
         self._ransac()
 
     def sync(self):
@@ -162,6 +171,67 @@ class DeviceFramePipeline:
             self.close()
         except Exception:
             pass
+
+
+class PipelinedFrames:
+    """Frames through two DeviceFramePipelines in turn (double buffering), so that the device never waits for the
+    host: each step enqueues its frame's voxel -> SOR -> normals (a captured graph) on the main stream, records an
+    event behind it, and only then finishes the previous frame, whose RANSAC runs on a second stream after that
+    frame's event while the host replays the adaptive stop (ransac_plane.cpp:95-105).  The frame of step i
+    therefore overlaps frame i - 1's RANSAC and host replay.  A frame's buffers are reused two steps later, after
+    its RANSAC has returned.  drain() finishes the last frame; result(i % 2) / info are those of the pipelines.
+    Every frame's results are those of DeviceFramePipeline.step() on the same input."""
+
+    def __init__(self, frame_xyz, stages: FrameStages | None = None, graph=True, n_max=None, keep_means=False):
+        from .device import Event, Stream
+        self.stream, self.rstream = Stream(), Stream()
+        self.pipes = [DeviceFramePipeline(frame_xyz, stages, stream=self.stream.handle, graph=graph, n_max=n_max,
+                                          keep_means=keep_means) for _ in range(2)]
+        self.events = [Event(), Event()]
+        for p in self.pipes:
+            p.prepare()
+        self.i = 0
+        self.pending = None  # pipeline index whose frame awaits RANSAC
+        self.finished = []   # (frame number, pipeline index) in finishing order, for callers that track frames
+
+    def load(self, frame_xyz):
+        """The next step's frame (into the pipeline that step uses; its previous frame is finished already)."""
+        self.pipes[self.i % 2].load(frame_xyz)
+
+    def step(self):
+        k = self.i % 2
+        self.pipes[k].launch()
+        self.events[k].record(self.stream.handle)
+        self._finish()
+        self.pending = (self.i, k)
+        self.i += 1
+
+    def _finish(self):
+        if self.pending is None:
+            return
+        n, k = self.pending
+        call("rvp_stream_wait_event", self.rstream.handle, self.events[k].handle)
+        self.pipes[k]._ransac(self.rstream.handle)
+        self.pending = None
+        self.finished.append((n, k))
+
+    def drain(self):
+        self._finish()
+        self.stream.sync()
+        self.rstream.sync()
+
+    def last(self):
+        """The pipeline holding the most recently finished frame."""
+        return self.pipes[self.finished[-1][1]] if self.finished else None
+
+    def close(self):
+        self.drain()
+        for p in self.pipes:
+            p.close()
+        for e in self.events:
+            e.close()
+        self.stream.close()
+        self.rstream.close()
 
 
 class DeviceBatchPipeline:

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from rvpoint_amd import scenes
-from rvpoint_amd.pipeline import DeviceFramePipeline
+from rvpoint_amd.pipeline import DeviceFramePipeline, PipelinedFrames
 
 pytestmark = pytest.mark.gpu
 
@@ -51,3 +51,30 @@ def test_frame_graph_replays_equal_eager():
     errs = C.c_ulonglong()
     R._lib.call("rvp_scan_wait_errors", C.byref(errs))
     assert errs.value == 0
+
+
+def test_pipelined_frames_equal_eager():
+    """Double-buffered frames: frame i + 1's graph is enqueued before frame i's RANSAC (on a second stream) and
+    its host replay; every frame's results equal the eager pipeline's, with frames of different sizes in turn."""
+    f1 = scenes.lidar_frame(rings=24, azimuths=6000, seed=11, nbox=14, max_range=40.0)
+    f2 = scenes.lidar_frame(rings=12, azimuths=5000, seed=12, nbox=10, max_range=40.0)
+    f3 = scenes.lidar_frame(rings=16, azimuths=4000, seed=13, nbox=8, max_range=30.0)
+    frames = [f1, f2, f3, f2, f1, f3, f3]
+    ref = [eager(f) for f in (f1, f2, f3)]
+    which = [0, 1, 2, 1, 0, 2, 2]
+    pp = PipelinedFrames(f1, keep_means=True, n_max=max(len(f) for f in (f1, f2, f3)))
+    got = []
+    try:
+        for i, f in enumerate(frames):
+            pp.load(f)
+            pp.step()
+            if i >= 1:  # step i finished frame i - 1, held by pipeline (i - 1) % 2 until step i + 1
+                assert pp.finished[-1] == (i - 1, (i - 1) % 2)
+                got.append(pp.pipes[(i - 1) % 2].results())
+        pp.drain()
+        got.append(pp.pipes[(len(frames) - 1) % 2].results())
+    finally:
+        pp.close()
+    assert len(got) == len(frames)
+    for i, (w, r) in enumerate(zip(which, got)):
+        assert same(r, ref[w]), i
