@@ -503,6 +503,12 @@ template <bool PAY, class KT = uint64_t> struct TopKSink {
   }
 };
 
+// Radius whose window3 holds every point p with fl(d2(p, q)) <= b: the hardware square root (within 2 ulp)
+// widened by 2^-20 and one ulp, so fl(R * R) >= b.
+__device__ __forceinline__ float cert_radius(float b) {
+  return next_up(__fmul_rn(__builtin_amdgcn_sqrtf(b), 1.0f + 0x1p-20f));
+}
+
 // the smallest key above every key whose d2 is at most b (b finite, >= 0)
 template <class KT> __device__ __forceinline__ KT bound_key(float b) {
   return as_key<KT>((uint64_t)(__float_as_uint(b) + 1u) << 32);
@@ -511,14 +517,15 @@ template <class KT> __device__ __forceinline__ KT bound_key(float b) {
 // An upper bound on the K-th smallest d2 at query q from the previous query p of the same cloud, whose K-th
 // smallest d2 is pd2: p's K nearest lie within sqrt(pd2) + |q - p| of q (triangle inequality).  Computed
 // d2 values are within a relative 5 * 2^-24 of the true squared distances, and the evaluation below loses
-// at most 6 * 2^-24 more, so the 2^-16 margin covers both.  +inf (no bound) when the step exceeds half the
+// at most 12 * 2^-24 more (two hardware square roots among them), so the 2^-16 margin covers both.  +inf (no bound) when the step exceeds half the
 // previous K-th distance (the bound would admit too many candidates to pay), for tiny or non-finite pd2.
 __device__ __forceinline__ float knn_bound_from(float qx, float qy, float qz, float px, float py, float pz,
                                                 float pd2) {
   if (!(pd2 >= 0x1p-60f && pd2 < 0x1p100f)) return __builtin_inff();
   const float dx = __fsub_rn(qx, px), dy = __fsub_rn(qy, py), dz = __fsub_rn(qz, pz);
-  const float dd = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz)));
-  const float dp = sqrtf(pd2);
+  const float dd =
+      __builtin_amdgcn_sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz)));
+  const float dp = __builtin_amdgcn_sqrtf(pd2);
   if (!(dd <= 0.5f * dp)) return __builtin_inff();
   const float sm = __fadd_rn(dp, dd);
   return __fmul_rn(__fmul_rn(sm, sm), 1.0f + 0x1p-16f);
@@ -537,7 +544,7 @@ __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy
   TopKSink<PAY, KT> sink{t, L};
   const bool bounded = bound_d2 < FLT_MAX;
   if (bounded) {
-    const float R = next_up(__fmul_rn(sqrtf(bound_d2), 1.0f + 0x1p-22f));
+    const float R = cert_radius(bound_d2);
     window3(g, qx, qy, qz, R, lo, hi);
     t.bound = t.kth = bound_key<KT>(bound_d2);
   } else {
@@ -573,7 +580,7 @@ __device__ __forceinline__ void knn_search(const GridView &g, float qx, float qy
     int nlo[3], nhi[3];
     if (t.cnt >= t.K) {
       const float dk = kd2(t.kth);
-      const float R = next_up(__fmul_rn(sqrtf(dk), 1.0f + 0x1p-22f));
+      const float R = cert_radius(dk);
       if (!(R < FLT_MAX)) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
