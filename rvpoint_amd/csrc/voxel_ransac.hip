@@ -430,13 +430,38 @@ __global__ void seg_vox_offsets_k(const uint32_t *__restrict__ off, int ncl, con
   out_off[c] = o < off[ncl] ? vid[o] : *nvox;
 }
 
+// Tiles of RC_TILE points that never straddle clouds: tile_off[c] = first tile of cloud c; also copies the
+// offsets into the read-back buffer and marks every cloud active for the first round of counts.  One wave.
+__device__ __forceinline__ void seg_tiles_wave(const uint32_t *__restrict__ off, int ncl, uint32_t *__restrict__ tile_off,
+                                               uint32_t *__restrict__ off_copy, uint8_t *__restrict__ active) {
+  const int l = lane();
+  uint32_t carry = 0;
+  for (int b0 = 0; b0 < ncl; b0 += 64) {
+    const int c = b0 + l;
+    const uint32_t t = c < ncl ? (off[c + 1] - off[c] + RC_TILE - 1) / RC_TILE : 0u;
+    const uint32_t inc = wave_incl_scan(t);
+    if (c < ncl) {
+      tile_off[c] = carry + inc - t;
+      off_copy[c] = off[c];
+      active[c] = 1;
+    }
+    carry += __shfl(inc, 63);
+  }
+  if (l == 0) {
+    tile_off[ncl] = carry;
+    off_copy[ncl] = off[ncl];
+  }
+}
+
 // hypotheses of every cloud from one rand() stream (each cloud's call starts with srand(0)):
 // draws r[3h..3h+2] % n_c (ransac_plane.cpp:69-71), model as compute_plane_coefficients
 __global__ void seg_plane_models_k(const float *__restrict__ x, const float *__restrict__ y,
                                    const float *__restrict__ z, const uint32_t *__restrict__ off, int ncl,
                                    const int32_t *__restrict__ draws, int H, float collinear,
                                    float *__restrict__ models, int32_t *__restrict__ valid,
-                                   int32_t *__restrict__ counts) {
+                                   int32_t *__restrict__ counts, uint32_t *__restrict__ tile_off,
+                                   uint32_t *__restrict__ off_copy, uint8_t *__restrict__ active) {
+  if (blockIdx.x == 0 && threadIdx.x < WAVE) seg_tiles_wave(off, ncl, tile_off, off_copy, active);
   const size_t gh = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gh >= (size_t)ncl * H) return;
   const int c = (int)(gh / H), h = (int)(gh % H);
@@ -465,28 +490,6 @@ __global__ void seg_plane_models_k(const float *__restrict__ x, const float *__r
   models[4 * gh + 2] = cc;
   models[4 * gh + 3] = d;
   valid[gh] = 1;
-}
-
-// tiles of RC_TILE points that never straddle clouds: tile_off[c] = first tile of cloud c (one wave)
-// (also copies the offsets into the read-back buffer)
-__global__ __launch_bounds__(64) void seg_tiles_k(const uint32_t *__restrict__ off, int ncl,
-                                                  uint32_t *__restrict__ tile_off, uint32_t *__restrict__ off_copy) {
-  const int l = lane();
-  uint32_t carry = 0;
-  for (int b0 = 0; b0 < ncl; b0 += 64) {
-    const int c = b0 + l;
-    const uint32_t t = c < ncl ? (off[c + 1] - off[c] + RC_TILE - 1) / RC_TILE : 0u;
-    const uint32_t inc = wave_incl_scan(t);
-    if (c < ncl) {
-      tile_off[c] = carry + inc - t;
-      off_copy[c] = off[c];
-    }
-    carry += __shfl(inc, 63);
-  }
-  if (l == 0) {
-    tile_off[ncl] = carry;
-    off_copy[ncl] = off[ncl];
-  }
 }
 
 // hypotheses [h_begin, h_end) of every cloud still active (active[c] != 0)
@@ -794,21 +797,20 @@ int rvp_batch_ransac_plane(const float *x, const float *y, const float *z, const
   int32_t *valid = reinterpret_cast<int32_t *>(models + NH * 4), *counts = valid + NH;
   uint32_t *offs_copy = reinterpret_cast<uint32_t *>(counts + NH);
   RVP_TRY(tiles.alloc((nclouds + 1) * sizeof(uint32_t), s));
-  RVP_LAUNCH("seg_plane_models_k", s, seg_plane_models_k<<<ceil_div(NH, 256), 256, 0, s>>>(
-                                          x, y, z, off, nclouds, draws, H, collinear_thresh, models, valid, counts));
-  RVP_LAUNCH("seg_tiles_k", s, seg_tiles_k<<<1, 64, 0, s>>>(off, nclouds, tiles.as<uint32_t>(), offs_copy));
   // Hypotheses are counted in rounds of 64, 128, ... and replayed after each: the reference's loop usually
   // stops long before max_iters (adaptively, ransac_plane.cpp:95-105), and a cloud whose loop has stopped
-  // takes no part in later rounds.  The first read-back brings the models, flags and offsets too.
+  // takes no part in later rounds.  The first read-back brings the models, flags and offsets too.  The models
+  // launch also lays out the count tiles and marks every cloud active for the first round.
   Tmp act;
   RVP_TRY(act.alloc((size_t)nclouds, s));
+  RVP_LAUNCH("seg_plane_models_k", s, seg_plane_models_k<<<ceil_div(NH, 256), 256, 0, s>>>(
+                                          x, y, z, off, nclouds, draws, H, collinear_thresh, models, valid, counts,
+                                          tiles.as<uint32_t>(), offs_copy, act.as<uint8_t>()));
   PinnedStage stage;
   uint8_t *host = nullptr;
   const size_t act_at = (words * sizeof(uint32_t) + 255) & ~(size_t)255;
   RVP_TRY(stage.get(s, act_at + (size_t)nclouds, reinterpret_cast<void **>(&host)));
   uint8_t *hact = host + act_at;
-  std::memset(hact, 1, (size_t)nclouds);
-  RVP_HIP(hipMemcpyAsync(act.p, hact, (size_t)nclouds, hipMemcpyHostToDevice, s));
   const float *hm = reinterpret_cast<const float *>(host);
   const int32_t *hv = reinterpret_cast<const int32_t *>(hm + NH * 4), *hc = hv + NH;
   const uint32_t *ho = reinterpret_cast<const uint32_t *>(hc + NH);
