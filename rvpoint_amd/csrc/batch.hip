@@ -192,15 +192,15 @@ __global__ void seg_occ_k(const float *__restrict__ x, const float *__restrict__
   }
 }
 
-// k-NN edge, step 3 (block per cloud): occupied cells = set bits; surface density m / c0^2 from m points
-// per occupied cell gives the k-th neighbour distance R = c0 sqrt(K / (pi m)); the edge is R / div
-__global__ __launch_bounds__(256) void seg_knn_cell_k(const uint32_t *__restrict__ off, int K,
-                                                       const float *__restrict__ c0, const uint32_t *__restrict__ words,
-                                                       double div, float *__restrict__ cell) {
-  const int c = blockIdx.x;
+// k-NN edge, step 3: occupied cells = set bits; surface density m / c0^2 from m points per occupied cell gives
+// the k-th neighbour distance R = c0 sqrt(K / (pi m)); the edge is R / div.  Every thread of the block takes part
+// and gets the result; sm holds a word per wave.
+__device__ __forceinline__ float knn_cell_of(const uint32_t *__restrict__ off, int K, const float *__restrict__ c0,
+                                             const uint32_t *__restrict__ words, double div, int c,
+                                             uint32_t *sm) {
   uint32_t cnt = 0;
 #pragma unroll 16
-  for (int j = threadIdx.x; j < OCC_BITS / 4; j += 256) {
+  for (int j = threadIdx.x; j < OCC_BITS / 4; j += blockDim.x) {
     uint32_t w = words[(size_t)c * (OCC_BITS / 4) + j];  // four cells: count the nonzero bytes
     w |= w >> 1;
     w |= w >> 2;
@@ -208,16 +208,25 @@ __global__ __launch_bounds__(256) void seg_knn_cell_k(const uint32_t *__restrict
     cnt += __popc(w & 0x01010101u);
   }
   cnt = wave_sum(cnt);
-  __shared__ uint32_t sm[4];
   if (lane() == 0) sm[threadIdx.x >> 6] = cnt;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const uint32_t occ = sm[0] + sm[1] + sm[2] + sm[3];
+  uint32_t occ = 0;
+  for (int i = 0; i < (int)(blockDim.x / WAVE); ++i) occ += sm[i];
+  __syncthreads();  // sm is reused by the next call
   const double n = (double)(off[c + 1] - off[c]);
   const double m = n / (double)max(occ, 1u);
   const double e0 = (double)c0[c];
   const double R = e0 * sqrt((double)K / (3.14159265358979 * fmax(m, 1e-30)));
-  cell[c] = (float)fmin(fmax(R / div, e0 / 64.0), e0 * 2.0);
+  return (float)fmin(fmax(R / div, e0 / 64.0), e0 * 2.0);
+}
+
+// step 3 with a block per cloud
+__global__ __launch_bounds__(256) void seg_knn_cell_k(const uint32_t *__restrict__ off, int K,
+                                                       const float *__restrict__ c0, const uint32_t *__restrict__ words,
+                                                       double div, float *__restrict__ cell) {
+  __shared__ uint32_t sm[4];
+  const float e = knn_cell_of(off, K, c0, words, div, blockIdx.x, sm);
+  if (threadIdx.x == 0) cell[blockIdx.x] = e;
 }
 
 // per cloud: origin (reference margin), dims, coarsened until the cloud's dense table fits its budget
@@ -263,11 +272,28 @@ __global__ void seg_geom_k(const float *__restrict__ mm, const uint32_t *__restr
 // exclusive scan of the clouds' cell counts into table offsets, and the key width; the work of
 // seg_minmax_final_k, seg_geom_k, the scan and seg_toff_k in one launch.
 constexpr int LAYOUT_MAX = 1024;
+// With knn_words (at most KNN_FUSE_MAX clouds), the k-NN edges of step 3 are computed first by the whole block
+// (into cell), instead of by seg_knn_cell_k.
+constexpr int KNN_FUSE_MAX = 4;
 __global__ __launch_bounds__(LAYOUT_MAX) void seg_layout_k(const uint32_t *__restrict__ mmu, float *__restrict__ mm,
                                                            const uint32_t *__restrict__ off, int ncl,
-                                                           const float *__restrict__ cell, double per_point,
+                                                           float *__restrict__ cell, double per_point,
                                                            SegDesc *__restrict__ desc, uint32_t *__restrict__ ncells,
-                                                           uint32_t *__restrict__ toff) {
+                                                           uint32_t *__restrict__ toff,
+                                                           const uint32_t *__restrict__ knn_words, int K,
+                                                           const float *__restrict__ c0, double div) {
+  __shared__ float cell_s[KNN_FUSE_MAX];
+  if (knn_words) {
+    __shared__ uint32_t sm[LAYOUT_MAX / WAVE];
+    for (int cc = 0; cc < ncl; ++cc) {
+      const float e = knn_cell_of(off, K, c0, knn_words, div, cc, sm);
+      if (threadIdx.x == 0) {
+        cell_s[cc] = e;
+        cell[cc] = e;
+      }
+    }
+    __syncthreads();
+  }
   const int c = threadIdx.x;
   SegDesc d{};
   uint32_t nc = 0;
@@ -277,7 +303,7 @@ __global__ __launch_bounds__(LAYOUT_MAX) void seg_layout_k(const uint32_t *__res
       m[j] = mmu ? ord2f(mmu[6 * c + j]) : mm[6 * c + j];
       if (mmu) mm[6 * c + j] = m[j];
     }
-    d = seg_geom(m, off[c + 1] - off[c], cell[c], per_point);
+    d = seg_geom(m, off[c + 1] - off[c], knn_words ? cell_s[c] : cell[c], per_point);
     nc = (uint32_t)d.g[0] * (uint32_t)d.g[1] * (uint32_t)d.g[2];
     ncells[c] = nc;
   }
@@ -389,24 +415,32 @@ int SegGrid::build(const float *x, const float *y, const float *z, const uint32_
   const unsigned per = (unsigned)std::min<size_t>(64, std::max<size_t>(1, n_bound / ((size_t)ncl * 4096)));
   RVP_LAUNCH("seg_minmax_k", s, seg_minmax_k<<<dim3(per, ncl), 256, 0, s>>>(x, y, z, off, mmu.as<uint32_t>()));
   const float *cells = cell_dev;
+  // k-NN edges of a few clouds computed by the layout launch (its inputs live until it is enqueued)
+  Tmp c0, d0;
+  const uint32_t *fuse_words = nullptr;
+  const float *fuse_c0 = nullptr;
   if (mode_knn) {
-    Tmp c0, d0;
     RVP_TRY(c0.alloc(ncl * sizeof(float), s));
     RVP_TRY(d0.alloc(ncl * sizeof(SegDesc), s));
     RVP_LAUNCH("seg_knn_c0_k", s, seg_knn_c0_k<<<cb, 64, 0, s>>>(mmu.as<uint32_t>(), mm.as<float>(), off, ncl, K,
                                                                  c0.as<float>(), d0.as<SegDesc>()));
     RVP_LAUNCH("seg_occ_k", s, seg_occ_k<<<ceil_div(n_bound, 256), 256, 0, s>>>(
                                    x, y, z, off, ncl, d0.as<SegDesc>(), bits.as<uint8_t>()));
-    RVP_LAUNCH("seg_knn_cell_k", s, seg_knn_cell_k<<<ncl, 256, 0, s>>>(off, K, c0.as<float>(), bits.as<uint32_t>(),
-                                                                       tune_knn_cell_div(), cellv.as<float>()));
     cells = cellv.as<float>();
+    if (ncl > KNN_FUSE_MAX)
+      RVP_LAUNCH("seg_knn_cell_k", s, seg_knn_cell_k<<<ncl, 256, 0, s>>>(off, K, c0.as<float>(), bits.as<uint32_t>(),
+                                                                         tune_knn_cell_div(), cellv.as<float>()));
+    else
+      fuse_words = bits.as<uint32_t>(), fuse_c0 = c0.as<float>();
   }
   const double per_point = tune_seg_cells_per_point();
   if (ncl <= LAYOUT_MAX) {
-    // the k-NN path has the float bbox already (seg_knn_c0_k)
+    // the k-NN path has the float bbox already (seg_knn_c0_k); a few clouds' k-NN edges are computed here too
     RVP_LAUNCH("seg_layout_k", s, seg_layout_k<<<1, LAYOUT_MAX, 0, s>>>(
-                                      mode_knn ? nullptr : mmu.as<uint32_t>(), mm.as<float>(), off, ncl, cells,
-                                      per_point, desc.as<SegDesc>(), ncells.as<uint32_t>(), toff.as<uint32_t>()));
+                                      mode_knn ? nullptr : mmu.as<uint32_t>(), mm.as<float>(), off, ncl,
+                                      const_cast<float *>(cells), per_point, desc.as<SegDesc>(),
+                                      ncells.as<uint32_t>(), toff.as<uint32_t>(), fuse_words, K, fuse_c0,
+                                      tune_knn_cell_div()));
   } else {
     if (!mode_knn)
       RVP_LAUNCH("seg_minmax_final_k", s, seg_minmax_final_k<<<ceil_div(6 * ncl, 256), 256, 0, s>>>(

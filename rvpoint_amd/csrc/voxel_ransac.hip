@@ -334,8 +334,8 @@ __global__ __launch_bounds__(256) void seg_vox_minmax_k(const float *__restrict_
 }
 
 // the per-block partials of cloud c (blockIdx.x) -> vmm[c][6]
-__global__ __launch_bounds__(64) void seg_vox_reduce_k(const int *__restrict__ pmm, int per, int *__restrict__ vmm) {
-  const int c = blockIdx.x, l = lane();
+__device__ __forceinline__ void seg_vox_reduce_cloud(const int *__restrict__ pmm, int per, int c, int *__restrict__ vmm) {
+  const int l = lane();
   int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
   for (int b = l; b < per; b += WAVE) {
     const int *p = pmm + ((size_t)c * per + b) * 6;
@@ -354,9 +354,13 @@ __global__ __launch_bounds__(64) void seg_vox_reduce_k(const int *__restrict__ p
     for (int j = 0; j < 6; ++j) vmm[6 * c + j] = v[j];
 }
 
+__global__ __launch_bounds__(64) void seg_vox_reduce_k(const int *__restrict__ pmm, int per, int *__restrict__ vmm) {
+  seg_vox_reduce_cloud(pmm, per, blockIdx.x, vmm);
+}
+
 // key space of each cloud: rx * ry * rz voxels (1 for an empty cloud or one whose range overflows 2^48,
 // which is then flagged in status[c]); base[c] = exclusive prefix (one wave); *bits = key width
-__global__ __launch_bounds__(64) void seg_vox_base_k(int ncl, const uint32_t *__restrict__ off,
+__device__ __forceinline__ void seg_vox_base_wave(int ncl, const uint32_t *__restrict__ off,
                                                      const int *__restrict__ vmm, uint64_t *__restrict__ ryz,
                                                      uint64_t *__restrict__ base, int *__restrict__ bits,
                                                      uint32_t *__restrict__ status) {
@@ -398,6 +402,26 @@ __global__ __launch_bounds__(64) void seg_vox_base_k(int ncl, const uint32_t *__
     while (b < 64 && (mx >> b) != 0) ++b;
     *bits = b;
   }
+}
+
+__global__ __launch_bounds__(64) void seg_vox_base_k(int ncl, const uint32_t *__restrict__ off,
+                                                     const int *__restrict__ vmm, uint64_t *__restrict__ ryz,
+                                                     uint64_t *__restrict__ base, int *__restrict__ bits,
+                                                     uint32_t *__restrict__ status) {
+  seg_vox_base_wave(ncl, off, vmm, ryz, base, bits, status);
+}
+
+// Both steps in one block, for batches of up to VOX_BOUNDS_MAX clouds: each wave reduces the partials of every
+// 16th cloud, then wave 0 lays out the key space.
+constexpr int VOX_BOUNDS_MAX = 4096;
+__global__ __launch_bounds__(1024) void seg_vox_bounds_k(const int *__restrict__ pmm, int per, int ncl,
+                                                         const uint32_t *__restrict__ off, int *__restrict__ vmm,
+                                                         uint64_t *__restrict__ ryz, uint64_t *__restrict__ base,
+                                                         int *__restrict__ bits, uint32_t *__restrict__ status) {
+  const int w = wave_in_block(), nw = (int)(blockDim.x / WAVE);
+  for (int c = w; c < ncl; c += nw) seg_vox_reduce_cloud(pmm, per, c, vmm);
+  __syncthreads();
+  if (w == 0) seg_vox_base_wave(ncl, off, vmm, ryz, base, bits, status);
 }
 
 __global__ void seg_vox_keys_k(const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
@@ -737,10 +761,16 @@ int rvp_batch_voxel(const float *x, const float *y, const float *z, const uint32
   RVP_TRY(bits.alloc(sizeof(int), s));
   RVP_LAUNCH("seg_vox_minmax_k", s, seg_vox_minmax_k<<<dim3(per, nclouds), 256, 0, s>>>(x, y, z, off, leaf,
                                                                                      pmm.as<int>()));
-  RVP_LAUNCH("seg_vox_reduce_k", s, seg_vox_reduce_k<<<nclouds, 64, 0, s>>>(pmm.as<int>(), (int)per,
-                                                                            vmm.as<int>()));
-  RVP_LAUNCH("seg_vox_base_k", s, seg_vox_base_k<<<1, 64, 0, s>>>(nclouds, off, vmm.as<int>(), ryz.as<uint64_t>(),
-                                                                  base.as<uint64_t>(), bits.as<int>(), status));
+  if (nclouds <= VOX_BOUNDS_MAX) {
+    RVP_LAUNCH("seg_vox_bounds_k", s, seg_vox_bounds_k<<<1, 1024, 0, s>>>(pmm.as<int>(), (int)per, nclouds, off,
+                                                                          vmm.as<int>(), ryz.as<uint64_t>(),
+                                                                          base.as<uint64_t>(), bits.as<int>(), status));
+  } else {
+    RVP_LAUNCH("seg_vox_reduce_k", s, seg_vox_reduce_k<<<nclouds, 64, 0, s>>>(pmm.as<int>(), (int)per,
+                                                                              vmm.as<int>()));
+    RVP_LAUNCH("seg_vox_base_k", s, seg_vox_base_k<<<1, 64, 0, s>>>(nclouds, off, vmm.as<int>(), ryz.as<uint64_t>(),
+                                                                    base.as<uint64_t>(), bits.as<int>(), status));
+  }
   RVP_TRY(keys.alloc(n_bound * sizeof(uint64_t), s));
   RVP_TRY(vals.alloc(n_bound * sizeof(uint32_t), s));
   RVP_TRY(kt.alloc(n_bound * sizeof(uint64_t), s));
