@@ -432,13 +432,17 @@ def batch_normals_knn(batch: Batch, k, vp=(0.0, 0.0, 0.0), stream=None):
     return np.stack([a.numpy(stream=stream)[:batch.n] for a in (nx, ny, nz)], 1)
 
 
-def batch_normals_radius(batch: Batch, radius, max_nn, vp=(0.0, 0.0, 0.0), stream=None):
-    """N2 per cloud with a cap (radius per cloud or scalar): (n, 3) normals by batch position."""
+def batch_normals_radius(batch: Batch, radius, max_nn, vp=(0.0, 0.0, 0.0), stream=None, cell=None):
+    """N2 per cloud with a cap (radius per cloud or scalar): (n, 3) normals by batch position.  cell (per cloud or
+    scalar, optional): the grid's cell edge instead of the radius."""
     nb = max(batch.n, 1)
     r = DeviceArray.from_numpy(np.broadcast_to(np.asarray(radius, np.float32), (batch.ncl,)).copy(), stream)
+    ce = None if cell is None else DeviceArray.from_numpy(
+        np.broadcast_to(np.asarray(cell, np.float32), (batch.ncl,)).copy(), stream)
     nx, ny, nz = (DeviceArray(nb, np.float32) for _ in range(3))
-    call("rvp_batch_normals_radius", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, r.ptr, None, int(max_nn),
-         float(vp[0]), float(vp[1]), float(vp[2]), nx.ptr, ny.ptr, nz.ptr, stream)
+    call("rvp_batch_normals_radius", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, r.ptr,
+         None if ce is None else ce.ptr, int(max_nn), float(vp[0]), float(vp[1]), float(vp[2]), nx.ptr, ny.ptr,
+         nz.ptr, stream)
     return np.stack([a.numpy(stream=stream)[:batch.n] for a in (nx, ny, nz)], 1)
 
 

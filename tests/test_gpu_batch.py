@@ -211,6 +211,50 @@ def test_batch_normals_radius_equal_single_cloud(clouds, per_wave):
         assert (b.split(nrm)[c] == single).all(), c
 
 
+def test_radius_normals_independent_of_path_and_order(clouds):
+    """Radius normals read the neighbour set, not its order: a query with at most max_nn hits takes the unsorted
+    staged path (its neighbours in the batched grid's within-cell order, which comes from atomics and varies
+    between calls), one with more takes the sorted top-K; both sum exactly (terms rounded to a common quantum), so
+    repeated calls agree bit for bit, and a query whose hit count is within both caps gets the same normal from a
+    call capped at 24 and one capped at 64 (one path or the other, the same neighbour set)."""
+    cl, scales = clouds
+    b = R.Batch(cl)
+    radius = np.array([0.06 * s for s in scales], np.float32)
+    first = R.batch_normals_radius(b, radius, 64)
+    for _ in range(3):
+        assert (R.batch_normals_radius(b, radius, 64) == first).all()
+    capped = R.batch_normals_radius(b, radius, 24)
+    n_small = n_large = 0
+    for c, p in enumerate(cl):
+        if len(p) < 2:
+            continue
+        x, y, z = soa(p)
+        _, _, cnt, _ = O.hash_radius(x, y, z, float(radius[c]), x, y, z, float(radius[c]), 64)
+        small = cnt <= 24
+        n_small += int(small.sum())
+        n_large += int((cnt > 24).sum())
+        assert (b.split(capped)[c][small] == b.split(first)[c][small]).all(), c
+    assert n_small > 100 and n_large > 100  # both paths were taken
+
+
+def test_radius_normals_unsorted_path_equals_sorted_path():
+    """The same neighbour sets through both paths of the radius normals: with the grid's cell equal to the radius,
+    a 3 x 3 x 3 window of this uniform volume holds about 320 points, too many to stage, so every query walks it
+    and keeps the sorted top-K; with half that cell the window (about 190 points) is staged and the queries with
+    at most 64 hits (about 50 on average) take the unsorted path.  The covariances sum exactly either way, so
+    the normals are equal bit for bit."""
+    rng = np.random.default_rng(7)
+    p = rng.random((20000, 3)).astype(np.float32)
+    r = float(np.cbrt(12.0 / 20000.0))
+    b = R.Batch([p])
+    x, y, z = soa(p)
+    _, _, cnt, _ = O.hash_radius(x, y, z, r, x, y, z, r, 64)
+    assert np.mean(cnt < 64) > 0.8
+    walked = R.batch_normals_radius(b, r, 64, cell=r)
+    staged = R.batch_normals_radius(b, r, 64, cell=0.5 * r)
+    assert (walked == staged).all()
+
+
 def test_batch_ransac_matches_single_and_oracle(clouds):
     cl, scales = clouds
     b = R.Batch(cl)
