@@ -234,6 +234,64 @@ class PipelinedFrames:
         self.rstream.close()
 
 
+class ConcurrentFrames:
+    """Frames on n DeviceFramePipelines at once, each with its own buffers, stream and captured graph and driven
+    by its own host thread: n frames in flight, whose kernels the device interleaves.  A frame alone leaves much
+    of the GPU idle (look-back chains, one-wave sums, launch gaps, RANSAC's host round trip); the other frames'
+    kernels fill it.  Every frame's results are those of DeviceFramePipeline.step() on the same input (the
+    pipelines share no buffers; the library's per-stream state keeps their launches apart).
+
+    run(frames_per_pipe) -> (wall seconds, per-frame latencies in ms: start of a frame's step to its RANSAC result
+    on the host).  load(i, xyz) puts a frame into pipeline i."""
+
+    def __init__(self, frame_xyz, n=2, stages: FrameStages | None = None, n_max=None, keep_means=False):
+        self.pipes = [DeviceFramePipeline(frame_xyz, stages, graph=True, n_max=n_max, keep_means=keep_means)
+                      for _ in range(n)]
+        for p in self.pipes:
+            p.prepare()
+
+    def load(self, i, frame_xyz):
+        self.pipes[i].load(frame_xyz)
+
+    def run(self, frames_per_pipe, on_frame=None):
+        """Each pipeline runs frames_per_pipe steps in its own thread (the ctypes calls release the GIL);
+        on_frame(i, j), when given, is called by pipeline i's thread after its frame j."""
+        import threading
+        import time
+        lat = [[] for _ in self.pipes]
+        errors = []
+
+        def work(i):
+            p = self.pipes[i]
+            try:
+                for j in range(frames_per_pipe):
+                    t0 = time.perf_counter()
+                    p.step()
+                    lat[i].append((time.perf_counter() - t0) * 1e3)
+                    if on_frame is not None:
+                        on_frame(i, j)
+            except Exception as e:  # noqa: BLE001 -- re-raised in the caller's thread
+                errors.append(e)
+
+        threads = [threading.Thread(target=work, args=(i,)) for i in range(len(self.pipes))]
+        t0 = time.perf_counter()
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        for p in self.pipes:
+            p.sync()
+        wall = time.perf_counter() - t0
+        if errors:
+            raise errors[0]
+        return wall, [x for l in lat for x in l]
+
+    def close(self):
+        for p in self.pipes:
+            p.sync()
+            p.close()
+
+
 class DeviceBatchPipeline:
     """Clouds back to back in HBM, one launch per stage and no host sync between stages: exact-kNN SOR (k_sor)
     -> radius lists (radius per cloud, capped at cap) of the kept points -> k-NN normals (k_nrm) of the kept
