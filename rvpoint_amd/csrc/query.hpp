@@ -137,11 +137,8 @@ template <bool PAY, class KT> __device__ __forceinline__ void bitonic_clean64(KT
   bitonic_clean_n<PAY, WAVE>(k, p);
 }
 
-__device__ __forceinline__ int pow2_at_least(int v) {
-  int s = 1;
-  while (s < v) s <<= 1;
-  return s;
-}
+// the smallest power of two >= v, for 2 <= v <= 2^30 (a count of leading zeros, not a loop)
+__device__ __forceinline__ int pow2_at_least(int v) { return 1 << (32 - __clz(v - 1)); }
 
 // Merge the staged hits into the best list.  Only the K best are ever read, so the list spans
 // W = pow2 >= K lanes: the staged batch is sorted in groups of pow2 >= nb lanes, its W smallest are
@@ -432,29 +429,77 @@ __device__ __forceinline__ void scan_staged(const StageLds &S, int total, float 
 
 // The radius hits of a staged window for callers that read the neighbour set and not its order (covariances):
 // when at most K staged points lie within r2 of the query they ARE the list capped at K, so no selection or sort
-// is needed.  Returns the hit count H <= K, lane j < H holding hit j's coordinates (staging order); -1 when more
-// than K points are hits (the caller then needs the K nearest by (d2, idx)).  Uses L.excl as scratch.
+// is needed.  With more hits, the K nearest are those at or below a d2 threshold t with exactly K hits at or
+// below it, found by interpolating the counts of the hits' d2 (every other step bisecting the bit patterns).
+// Returns the neighbour count (min(H, K)), lane j holding neighbour j's coordinates (staging order); -1 when no
+// threshold separates exactly K (the K-th and (K+1)-th distances are equal: the caller then orders the list by
+// (d2, idx)).  Uses L.excl as scratch.
 __device__ __forceinline__ int staged_hits_upto(const StageLds &S, int total, float qx, float qy, float qz,
                                                 float r2, int K, WaveLds &L, float &hx, float &hy, float &hz) {
   constexpr int NG = STAGE_CAP / WAVE;
   const int l = lane();
   uint64_t m[NG];
+  uint32_t d[NG];  // hits' d2 bits, all-ones for the rest
   int H = 0;
 #pragma unroll
   for (int gi = 0; gi < NG; ++gi) {
     m[gi] = 0;
+    d[gi] = ~0u;
     if (gi * WAVE < total) {
       const int v = gi * WAVE + l;
       bool hit = false;
       if (v < total) {
         const float4 p = S.c[v];
-        hit = dist2(p.x, p.y, p.z, qx, qy, qz) <= r2;
+        const float d2 = dist2(p.x, p.y, p.z, qx, qy, qz);
+        hit = d2 <= r2;
+        if (hit) d[gi] = __float_as_uint(d2);
       }
       m[gi] = ballot(hit);
       H += popc(m[gi]);
     }
   }
-  if (H > K) return -1;
+  if (H > K) {
+    auto count = [&](uint32_t t) {
+      int c = 0;
+#pragma unroll
+      for (int gi = 0; gi < NG; ++gi)
+        if (gi * WAVE < total) c += popc(ballot(d[gi] <= t));
+      return c;
+    };
+    // d2 >= +0, so the bit patterns order as the values; c(lo) < K < c(hi) throughout
+    uint32_t lo = 0, hi = __float_as_uint(r2), tau = 0;
+    int clo = count(0), chi = H;
+    bool found = clo == K;
+    if (clo < K) {
+      for (int it = 0; it < 64 && hi - lo > 1; ++it) {
+        uint32_t t = lo + (hi - lo) / 2;
+        if ((it & 1) == 0) {
+          const float flo = __uint_as_float(lo), fhi = __uint_as_float(hi);
+          const float f = flo + (fhi - flo) * (((float)K - 0.5f - (float)clo) / (float)(chi - clo));
+          const uint32_t ti = __float_as_uint(f);
+          if (f == f && ti > lo && ti < hi) t = ti;
+        }
+        const int ct = count(t);
+        if (ct == K) {
+          tau = t;
+          found = true;
+          break;
+        }
+        if (ct < K) {
+          lo = t;
+          clo = ct;
+        } else {
+          hi = t;
+          chi = ct;
+        }
+      }
+    }
+    if (!found) return -1;
+#pragma unroll
+    for (int gi = 0; gi < NG; ++gi)
+      if (gi * WAVE < total) m[gi] = ballot(d[gi] <= tau);
+    H = K;
+  }
   int base = 0;
   wave_lds_sync();
 #pragma unroll
