@@ -352,6 +352,7 @@ __host__ __device__ inline int knn_run_length(size_t nq, int forced) {
 // of the box, in slice order.  Consecutive queries of a run are mostly in one cell and then have the same
 // window, so one staging serves them all and each query reads its candidates from LDS, 64 per instruction.
 constexpr int STAGE_CAP = 256;
+constexpr int STAGE_GROUPS = (STAGE_CAP + WAVE - 1) / WAVE;  // groups of 64 staged points
 struct StageLds {
   float4 c[STAGE_CAP];
 };
@@ -383,7 +384,7 @@ __device__ __forceinline__ int stage_box(const GridView &g, const int lo[3], con
   const int top = nrows > 16 ? WAVE / 2 : nrows > 4 ? 8 : 2;
   // every group's gathers are issued before the first is stored, so the staging waits for one round trip to
   // memory instead of one per group
-  constexpr int NG = STAGE_CAP / WAVE;
+  constexpr int NG = STAGE_GROUPS;
   float4 c[NG];
 #pragma unroll
   for (int gi = 0; gi < NG; ++gi) {
@@ -436,7 +437,7 @@ __device__ __forceinline__ void scan_staged(const StageLds &S, int total, float 
 // (d2, idx)).  Uses L.excl as scratch.
 __device__ __forceinline__ int staged_hits_upto(const StageLds &S, int total, float qx, float qy, float qz,
                                                 float r2, int K, WaveLds &L, float &hx, float &hy, float &hz) {
-  constexpr int NG = STAGE_CAP / WAVE;
+  constexpr int NG = STAGE_GROUPS;
   const int l = lane();
   uint64_t m[NG];
   uint32_t d[NG];  // hits' d2 bits, all-ones for the rest
