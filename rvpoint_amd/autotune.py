@@ -11,8 +11,8 @@ tune() measures a workload class on the GPU by coordinate descent: each knob in 
 the others held at their best so far, keeping the fastest setting (median of a few timed runs) whose outputs are
 identical to the default setting's.  The result is written to tuning.json beside this module; apply() sets a
 class's knobs for the process.  Classes: "lidar_frame" (voxel -> SOR k=50 -> radius normals -> RANSAC on a
-1.04M-point frame), "batch" (256 clouds: SOR k=32 -> radius lists -> k-NN normals k=24) and "indoor" (20M points:
-radius lists r=0.02 cap 64 and k-NN normals k=30).
+1.04M-point frame, two frames in flight as the benchmark runs it; ms per frame), "batch" (256 clouds: SOR k=32 ->
+radius lists -> k-NN normals k=24) and "indoor" (20M points: radius lists r=0.02 cap 64 and k-NN normals k=30).
 
 Usage (on a GPU): python -m rvpoint_amd.autotune [--out PATH] [class ...]   (PATH defaults to tuning.json here)
 """
@@ -68,15 +68,28 @@ def apply(cls, table=None):
 # workloads: each returns (run, outputs) closures; run() does one timed pass, outputs() the results to compare
 # ---------------------------------------------------------------------------------------------------------
 def _frame_workload():
+    """The frame as the bench's headline runs it: two frames in flight (ConcurrentFrames), 20 frames per pass.
+    The pipelines' graphs fix the knobs at capture, so a pass under other knobs captures anew (in the untimed
+    first pass of each setting)."""
     from . import scenes
-    from .pipeline import DeviceFramePipeline
-    pipe = DeviceFramePipeline(scenes.lidar_frame(), keep_means=True)
+    from .pipeline import ConcurrentFrames
+    frame = scenes.lidar_frame()
+    state = {"knobs": None, "cf": None}
+
+    def current():
+        if state["knobs"] != {k: _lib.tune(k) for k in DEFAULTS}:
+            if state["cf"] is not None:
+                state["cf"].close()
+            state["cf"] = ConcurrentFrames(frame, n=2, keep_means=True)
+            state["knobs"] = {k: _lib.tune(k) for k in DEFAULTS}
+        return state["cf"]
 
     def run():
-        pipe.step()
+        current().run(10)
+    run.frames = 20  # timed per frame
 
     def outputs():
-        r = pipe.results()
+        r = current().pipes[0].results()
         return [r["voxels"], r["means"], r["kept"], r["normals"], r["model"], np.array([r["inliers"], r["iterations"]])]
     return run, outputs
 
@@ -138,7 +151,7 @@ def _time(run, reps):
         t0 = time.perf_counter()
         run()
         synchronize()
-        ts.append((time.perf_counter() - t0) * 1e3)
+        ts.append((time.perf_counter() - t0) * 1e3 / getattr(run, "frames", 1))
     return float(np.median(ts))
 
 
@@ -148,7 +161,7 @@ def _same(a, b):
 
 def tune(cls, reps=None, log=print):
     """Coordinate descent over SPACE[cls]; returns {"knobs", "ms", "default_ms", "sweep"}."""
-    reps = reps or (20 if cls == "lidar_frame" else 3)
+    reps = reps or (7 if cls == "lidar_frame" else 3)
     run, outputs = WORKLOADS[cls]()
     apply_knobs = lambda kn: [_lib.tune(k, v) for k, v in kn.items()]  # noqa: E731
     best = dict(DEFAULTS)
