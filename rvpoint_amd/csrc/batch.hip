@@ -519,9 +519,9 @@ __device__ __forceinline__ void sor_run(const SegArrays &A, int k, int h0, uint3
 // at a time; 2: runs strided within per-XCD ranges (XcdStride); 0: runs strided over the launch's waves.  One
 // instantiation per distribution, so each carries only its own loop's registers.
 template <int WS>
-__global__ __launch_bounds__(QT) void seg_sor_means_k(SegArrays A, int k, int forced, int h0,
-                                                      float *__restrict__ means, uint32_t *__restrict__ queue,
-                                                      uint32_t chunk) {
+__global__ __launch_bounds__(QT) RVP_WAVES_PER_SIMD(8)
+void seg_sor_means_k(SegArrays A, int k, int forced, int h0, float *__restrict__ means, uint32_t *__restrict__ queue,
+                     uint32_t chunk) {
   __shared__ WaveLds lds[QW];
   __shared__ SumLds sums[QW];
   const int w = wave_in_block();

@@ -314,8 +314,9 @@ __global__ __launch_bounds__(256) void normals_solve_k(const float *__restrict__
 
 // MODE 0: k nearest (N1, K = k+1 incl. self); MODE 1: radius list capped at K (N2)
 template <int MODE, bool STRIDED>
-__global__ __launch_bounds__(QT) void normals_topk_k(GridView g, int K, float radius, int h0, int nb,
-                                                      float *__restrict__ rec, unsigned long long *clear) {
+__global__ __launch_bounds__(QT) RVP_WAVES_PER_SIMD(MODE == 1 ? 7 : 8)
+void normals_topk_k(GridView g, int K, float radius, int h0, int nb, float *__restrict__ rec,
+                    unsigned long long *clear) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *clear = 0ull;  // the call's fallback counter
   __shared__ WaveLds lds[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
@@ -440,9 +441,9 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
 // N1 / N2 over a batch: every query's neighbourhood within its own cloud (MODE 0: K nearest incl. self;
 // MODE 1: radius[c] list capped at K)
 template <int MODE, bool STRIDED>
-__global__ __launch_bounds__(QT) void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius, int h0,
-                                                     int nb, float *__restrict__ rec, size_t n_bound,
-                                                     unsigned long long *clear) {
+__global__ __launch_bounds__(QT) RVP_WAVES_PER_SIMD(MODE == 1 ? 7 : 8)
+void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius, int h0, int nb, float *__restrict__ rec,
+                   size_t n_bound, unsigned long long *clear) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *clear = 0ull;  // the call's fallback counter
   __shared__ WaveLds lds[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS

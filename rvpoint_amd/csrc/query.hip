@@ -286,9 +286,9 @@ __global__ __launch_bounds__(QT) void bruteforce_k(const float *__restrict__ x, 
 // k nearest (rows of stride k, padded with -1 / +inf)
 // ------------------------------------------------------------------------------------------------
 template <bool SELF>
-__global__ __launch_bounds__(QT) void knn_k(GridView g, const float *__restrict__ qx, const float *__restrict__ qy,
-                                             const float *__restrict__ qz, uint32_t nq, int k,
-                                             int32_t *__restrict__ idx_out, float *__restrict__ d2_out) {
+__global__ __launch_bounds__(QT) RVP_WAVES_PER_SIMD(8)
+void knn_k(GridView g, const float *__restrict__ qx, const float *__restrict__ qy, const float *__restrict__ qz,
+           uint32_t nq, int k, int32_t *__restrict__ idx_out, float *__restrict__ d2_out) {
   __shared__ WaveLds lds[QW];
   const int w = wave_in_block();
   const uint32_t q = blockIdx.x * QW + w;
@@ -322,8 +322,8 @@ __global__ __launch_bounds__(QT) void knn_k(GridView g, const float *__restrict_
 // query of a run on, the previous query's k-th distance bounds the current one (knn_bound_from), so the first
 // box is already the certified one and only hits within the bound are staged.  The sums of a wave's queries
 // are deferred (DeferredMeans); the launch has one wave per run.
-__global__ __launch_bounds__(QT) void sor_knn_means_k(GridView g, int k, int per_wave, int h0,
-                                                      float *__restrict__ means) {
+__global__ __launch_bounds__(QT) RVP_WAVES_PER_SIMD(8)
+void sor_knn_means_k(GridView g, int k, int per_wave, int h0, float *__restrict__ means) {
   __shared__ WaveLds lds[QW];
   __shared__ SumLds sums[QW];
   const int w = wave_in_block();

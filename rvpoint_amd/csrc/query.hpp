@@ -15,6 +15,13 @@
 
 namespace rvp {
 
+// Occupancy floor for a query kernel: the compiler keeps each wave within the registers that let n waves share
+// a SIMD, spilling scalar registers beyond that into lanes of vector registers.  The search kernels are bound by
+// instruction issue, and the extra resident waves hide more stalls than the spill moves cost (k-NN SOR means at
+// 8 waves instead of 7: 0.182 -> 0.160 ms on the LiDAR frame).  A kernel whose vector registers cannot fit is left
+// as it is by the compiler, hence 7 for the radius-normals variants (72 VGPRs).
+#define RVP_WAVES_PER_SIMD(n) __attribute__((amdgpu_waves_per_eu(n)))
+
 constexpr int QT = 256;         // threads per query block
 constexpr int QW = QT / WAVE;   // waves (queries in flight) per block
 constexpr int MAX_TOPK = 64;    // entries one wave keeps in registers
