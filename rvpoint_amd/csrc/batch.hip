@@ -549,14 +549,14 @@ void seg_sor_means_k(SegArrays A, int k, int forced, int h0, float *__restrict__
 }
 
 // H3 per cloud: capped radius lists (cloud-local indices), radius per cloud
-__global__ __launch_bounds__(QT) void seg_radius_topk_k(SegArrays A, const float *__restrict__ radius, int K,
+__global__ __launch_bounds__(QT) RVP_WAVES_PER_SIMD(8) void seg_radius_topk_k(SegArrays A, const float *__restrict__ radius, int K,
                                                          int per_wave, int32_t *__restrict__ idx_out,
                                                          float *__restrict__ d2_out, int32_t *__restrict__ counts) {
   __shared__ WaveLds lds[QW];
-  __shared__ StageLds stage[QW];
+  __shared__ StageLdsSmall stage[QW];
   const int w = wave_in_block();
   WaveLds &L = lds[w];
-  StageLds &S = stage[w];
+  StageLdsSmall &S = stage[w];
   const uint32_t nq = A.off[A.ncl];
   const uint64_t runs = ((uint64_t)nq + (uint64_t)per_wave - 1) / (uint64_t)per_wave;
   const int l = lane();

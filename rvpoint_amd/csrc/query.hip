@@ -11,16 +11,16 @@ namespace rvp {
 // (spatial_hashing.cpp:165-223 with max_nn > 0)
 // ------------------------------------------------------------------------------------------------
 template <bool SELF>
-__global__ __launch_bounds__(QT) void radius_topk_k(GridView g, const float *__restrict__ qx,
+__global__ __launch_bounds__(QT) RVP_WAVES_PER_SIMD(8) void radius_topk_k(GridView g, const float *__restrict__ qx,
                                                      const float *__restrict__ qy, const float *__restrict__ qz,
                                                      uint32_t nq, int per_wave, float r, int K,
                                                      int32_t *__restrict__ idx_out, float *__restrict__ d2_out,
                                                      int32_t *__restrict__ counts, int32_t *__restrict__ total) {
   __shared__ WaveLds lds[QW];
-  __shared__ StageLds stage[QW];
+  __shared__ StageLdsSmall stage[QW];
   const int w = wave_in_block();
   WaveLds &L = lds[w];
-  StageLds &S = stage[w];
+  StageLdsSmall &S = stage[w];
   // this wave's queries [q0, q0 + nb): lane j loads query q0 + j and computes its window once
   const uint64_t q0 = (uint64_t)(blockIdx.x * QW + w) * (uint64_t)per_wave;
   if (q0 >= nq) return;

@@ -358,16 +358,20 @@ __host__ __device__ inline int knn_run_length(size_t nq, int forced) {
 // A wave's copy of the candidates of one window: x, y, z and the original index (as float bits) of every point
 // of the box, in slice order.  Consecutive queries of a run are mostly in one cell and then have the same
 // window, so one staging serves them all and each query reads its candidates from LDS, 64 per instruction.
-constexpr int STAGE_CAP = 256;
-constexpr int STAGE_GROUPS = (STAGE_CAP + WAVE - 1) / WAVE;  // groups of 64 staged points
-struct StageLds {
-  float4 c[STAGE_CAP];
+// CAP points at most: 256 (4 KB per wave), or 224 for kernels that fit eight waves per SIMD only with at most
+// 20 KB of LDS per block of four waves.
+template <int CAP> struct StageLdsT {
+  static constexpr int CAPACITY = CAP;
+  static constexpr int GROUPS = (CAP + WAVE - 1) / WAVE;  // groups of 64 staged points
+  float4 c[CAP];
 };
+using StageLds = StageLdsT<256>;
+using StageLdsSmall = StageLdsT<224>;
 
 // Stage the points of box [lo, hi] into S; returns their count, or -1 when the box has more than 64 (y, z)
-// rows or more than STAGE_CAP points (the caller then walks the box with scan_box).
-__device__ __forceinline__ int stage_box(const GridView &g, const int lo[3], const int hi[3], WaveLds &L,
-                                         StageLds &S) {
+// rows or more points than S holds (the caller then walks the box with scan_box).
+template <class ST>
+__device__ __forceinline__ int stage_box(const GridView &g, const int lo[3], const int hi[3], WaveLds &L, ST &S) {
   const int l = lane();
   const int ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
   if (ny <= 0 || nz <= 0 || hi[0] < lo[0]) return 0;
@@ -383,7 +387,7 @@ __device__ __forceinline__ int stage_box(const GridView &g, const int lo[3], con
   }
   const uint32_t inc = wave_incl_scan(l0);
   const uint32_t total = readlaneu(inc, WAVE - 1);
-  if (total > (uint32_t)STAGE_CAP) return -1;
+  if (total > (uint32_t)ST::CAPACITY) return -1;
   wave_lds_sync();
   L.excl[l] = inc - l0;  // rows past nrows hold `total`, so the search below never selects them
   L.s0[l] = s0;
@@ -391,7 +395,7 @@ __device__ __forceinline__ int stage_box(const GridView &g, const int lo[3], con
   const int top = nrows > 16 ? WAVE / 2 : nrows > 4 ? 8 : 2;
   // every group's gathers are issued before the first is stored, so the staging waits for one round trip to
   // memory instead of one per group
-  constexpr int NG = STAGE_GROUPS;
+  constexpr int NG = ST::GROUPS;
   float4 c[NG];
 #pragma unroll
   for (int gi = 0; gi < NG; ++gi) {
@@ -416,8 +420,8 @@ __device__ __forceinline__ int stage_box(const GridView &g, const int lo[3], con
 }
 
 // The staged counterpart of scan_box in radius mode: sink(valid, hit, key, 0) for each group of 64 candidates.
-template <class Sink>
-__device__ __forceinline__ void scan_staged(const StageLds &S, int total, float qx, float qy, float qz, float r2,
+template <class Sink, class ST>
+__device__ __forceinline__ void scan_staged(const ST &S, int total, float qx, float qy, float qz, float r2,
                                             Sink &sink) {
   for (int c0 = 0; c0 < total; c0 += WAVE) {
     const int v = c0 + lane();
@@ -442,9 +446,10 @@ __device__ __forceinline__ void scan_staged(const StageLds &S, int total, float 
 // Returns the neighbour count (min(H, K)), lane j holding neighbour j's coordinates (staging order); -1 when no
 // threshold separates exactly K (the K-th and (K+1)-th distances are equal: the caller then orders the list by
 // (d2, idx)).  Uses L.excl as scratch.
-__device__ __forceinline__ int staged_hits_upto(const StageLds &S, int total, float qx, float qy, float qz,
-                                                float r2, int K, WaveLds &L, float &hx, float &hy, float &hz) {
-  constexpr int NG = STAGE_GROUPS;
+template <class ST>
+__device__ __forceinline__ int staged_hits_upto(const ST &S, int total, float qx, float qy, float qz, float r2,
+                                                int K, WaveLds &L, float &hx, float &hy, float &hz) {
+  constexpr int NG = ST::GROUPS;
   const int l = lane();
   uint64_t m[NG];
   uint32_t d[NG];  // hits' d2 bits, all-ones for the rest
@@ -533,8 +538,9 @@ struct StagedWindow {
   int lo[3] = {1, 1, 1}, hi[3] = {0, 0, 0};
   int tag = -1;
   int total = -2;  // -2: nothing staged; -1: window too large to stage
+  template <class ST>
   __device__ __forceinline__ void ensure(const GridView &g, const int qlo[3], const int qhi[3], int qtag,
-                                         WaveLds &L, StageLds &S) {
+                                         WaveLds &L, ST &S) {
     if (total != -2 && qtag == tag && qlo[0] == lo[0] && qlo[1] == lo[1] && qlo[2] == lo[2] &&
         qhi[0] == hi[0] && qhi[1] == hi[1] && qhi[2] == hi[2])
       return;
