@@ -166,6 +166,9 @@ def batch_leg(R, args):
         R.synchronize()
         bt.append((time.perf_counter() - t0) * 1e3)
     kept = bp.kept_total()
+    # the batch keeps cloud c's kept points at kept offset c, the per-cloud leg at input offset c
+    kept_off = bp.sor_off.numpy().astype(np.int64)
+    batched_out = [a.numpy().view(np.uint32) for a in (bp.nx, bp.ny, bp.nz, bp.cnt)]
     print(f"bench: batched {np.median(bt):.2f} ms/batch; per-cloud leg", file=sys.stderr, flush=True)
     pc = PerCloudPipeline(bp)
     for c in range(bp.ncl):  # warm every cloud size once
@@ -180,6 +183,11 @@ def batch_leg(R, args):
             R.synchronize()
             per_cloud_ms.append((time.perf_counter() - t1) * 1e3)
         pt.append((time.perf_counter() - t0) * 1e3)
+    # radius counts and normals of the per-cloud leg (same buffers) and of the batch agree bit for bit
+    per_out = [a.numpy().view(np.uint32) for a in (bp.nx, bp.ny, bp.nz, bp.cnt)]
+    same = all(np.array_equal(b[kept_off[c]:kept_off[c + 1]],
+                              p[bp.off_host[c]:bp.off_host[c] + kept_off[c + 1] - kept_off[c]])
+               for b, p in zip(batched_out, per_out) for c in range(bp.ncl))
     b_ms, p_ms = float(np.median(bt)), float(np.median(pt))
     return {
         "workload": "256 plane+sphere clouds, log-uniform 2e3..5e5 points, scale U[1,20]; SOR k=32 -> "
@@ -189,7 +197,7 @@ def batch_leg(R, args):
         "batched_mpts_s": round(bp.n / (b_ms * 1e-3) / 1e6, 3),
         "per_cloud_ms": round(p_ms, 3), "per_cloud_mpts_s": round(bp.n / (p_ms * 1e-3) / 1e6, 3),
         "per_cloud_p99_ms_per_cloud": round(float(np.percentile(per_cloud_ms, 99)), 3),
-        "speedup": round(p_ms / b_ms, 3),
+        "speedup": round(p_ms / b_ms, 3), "same_results": same,
     }
 
 

@@ -276,6 +276,16 @@ int rvp_batch_radius_search(const float *x, const float *y, const float *z, cons
 int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
                           size_t n_bound, int k, float vpx, float vpy, float vpz, float *nx, float *ny, float *nz,
                           rvp_stream_t s);
+/* rvp_batch_normals_knn (same outputs, bit for bit) reading each query's k + 1 nearest from the capped radius
+ * lists of the same clouds (rvp_batch_radius_search's idx / d2 / counts, rows of stride list_cap, d2 may be
+ * NULL) when its list holds at least k + 1 entries: the K nearest by (d2, idx) are then the list's first K.
+ * Queries with shorter lists are searched.  No single reference function: it fuses the reference's
+ * SpatialHash::radiusSearch (spatial_hashing.cpp:110-168) and normal_estimation_sc's k-NN
+ * (normal_estimation.cpp:72-112) for pipelines that run both on one cloud. */
+int rvp_batch_normals_knn_lists(const float *x, const float *y, const float *z, const uint32_t *offsets,
+                                int nclouds, size_t n_bound, int k, const int32_t *list_idx, const float *list_d2,
+                                const int32_t *list_counts, int list_cap, float vpx, float vpy, float vpz, float *nx,
+                                float *ny, float *nz, rvp_stream_t s);
 /* N2 per cloud with a cap (normal_estimation.cpp:189-214 neighbourhood, as rvp_normals_radius with
  * max_nn in [1, 64]): radius[c] / cell[c] device arrays (cell NULL: radius). */
 int rvp_batch_normals_radius(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,

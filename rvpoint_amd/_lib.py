@@ -97,6 +97,7 @@ _SIGS = {
     "rvp_batch_sor_exact": [vp, vp, vp, vp, i32, sz, i32, fl, vp, vp, vp, vp, vp],
     "rvp_batch_radius_search": [vp, vp, vp, vp, i32, sz, vp, vp, i32, vp, vp, vp, vp],
     "rvp_batch_normals_knn": [vp, vp, vp, vp, i32, sz, i32, fl, fl, fl, vp, vp, vp, vp],
+    "rvp_batch_normals_knn_lists": [vp, vp, vp, vp, i32, sz, i32, vp, vp, vp, i32, fl, fl, fl, vp, vp, vp, vp],
     "rvp_batch_normals_radius": [vp, vp, vp, vp, i32, sz, vp, vp, i32, fl, fl, fl, vp, vp, vp, vp],
     "rvp_batch_voxel": [vp, vp, vp, vp, i32, sz, vp, vp, vp, vp, vp],
     "rvp_batch_ransac_plane": [vp, vp, vp, vp, i32, sz, fl, i32, fl, fl, pfl, pi32, pi32, vp],

@@ -432,6 +432,24 @@ def batch_normals_knn(batch: Batch, k, vp=(0.0, 0.0, 0.0), stream=None):
     return np.stack([a.numpy(stream=stream)[:batch.n] for a in (nx, ny, nz)], 1)
 
 
+def batch_normals_knn_lists(batch: Batch, k, idx, d2, cnt, vp=(0.0, 0.0, 0.0), stream=None):
+    """N1 per cloud as batch_normals_knn, each query's k + 1 nearest read from its capped radius list of the same
+    clouds (idx (n, cap) and d2 (n, cap) or None, cnt (n,), as batch_radius_search returns them) when the list
+    holds at least k + 1 entries: (n, 3) normals by batch position."""
+    nb = max(batch.n, 1)
+    idx = np.ascontiguousarray(idx, np.int32)
+    cap = int(idx.shape[1])
+    pad = lambda a, dt: np.concatenate([np.asarray(a, dt), np.zeros((nb - len(a),) + np.shape(a)[1:], dt)])
+    di = DeviceArray.from_numpy(pad(idx, np.int32), stream)
+    dd = DeviceArray.from_numpy(pad(d2, np.float32), stream) if d2 is not None else None
+    dc = DeviceArray.from_numpy(pad(cnt, np.int32), stream)
+    nx, ny, nz = (DeviceArray(nb, np.float32) for _ in range(3))
+    call("rvp_batch_normals_knn_lists", *batch.dev.ptrs(), batch.offsets.ptr, batch.ncl, nb, int(k), di.ptr,
+         dd.ptr if dd is not None else None, dc.ptr, cap, float(vp[0]), float(vp[1]), float(vp[2]), nx.ptr, ny.ptr,
+         nz.ptr, stream)
+    return np.stack([a.numpy(stream=stream)[:batch.n] for a in (nx, ny, nz)], 1)
+
+
 def batch_normals_radius(batch: Batch, radius, max_nn, vp=(0.0, 0.0, 0.0), stream=None, cell=None):
     """N2 per cloud with a cap (radius per cloud or scalar): (n, 3) normals by batch position.  cell (per cloud or
     scalar, optional): the grid's cell edge instead of the radius."""

@@ -438,12 +438,24 @@ __global__ __launch_bounds__(QT) void normals_csr_k(GridView g, const int64_t *_
   }
 }
 
+// Capped radius lists of the same clouds (rvp_batch_radius_search's rows: stride cap, indexed by batch position,
+// neighbour indices local to the cloud, (d2, idx) order), read by the k-NN normals in place of a search.
+struct NbrLists {
+  const int32_t *idx = nullptr;
+  const float *d2 = nullptr;  // may be null
+  const int32_t *cnt = nullptr;
+  int cap = 0;
+};
+
 // N1 / N2 over a batch: every query's neighbourhood within its own cloud (MODE 0: K nearest incl. self;
-// MODE 1: radius[c] list capped at K)
-template <int MODE, bool STRIDED>
+// MODE 1: radius[c] list capped at K).  LISTS (MODE 0, K <= lists.cap): a query whose radius list holds at
+// least K entries takes them as its K nearest: every point outside the radius has a larger d2 than every
+// point inside, and both orders are (d2, idx) on the same coordinates, so the first K entries of the list
+// are the K nearest.  Lane j then holds entry j, as after a search, and the moments are the searched ones.
+template <int MODE, bool STRIDED, bool LISTS = false>
 __global__ __launch_bounds__(QT) RVP_WAVES_PER_SIMD(MODE == 1 ? 7 : 8)
 void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius, int h0, int nb, float *__restrict__ rec,
-                   size_t n_bound, unsigned long long *clear) {
+                   size_t n_bound, unsigned long long *clear, NbrLists lists) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *clear = 0ull;  // the call's fallback counter
   __shared__ WaveLds lds[QW];
   // consecutive queries of a batch (not strided) mostly share their window: stage it once in LDS
@@ -465,14 +477,33 @@ void seg_normals_k(SegArrays A, int K, const float *__restrict__ radius, int h0,
       TopK t;
       t.init(K);
       if (MODE == 0) {
-        // consecutive queries of one cloud: the previous one's K-th distance bounds this one's
-        knn_search<false>(g, x, y, z, t, lds[w],
-                          (!STRIDED && c == pc) ? knn_bound_from(x, y, z, px, py, pz, pd2) : __builtin_inff(), h0);
+        bool listed = false;
+        if constexpr (LISTS) {
+          const uint32_t row = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.perm[q]);
+          if (lists.cnt[row] >= K) {
+            const uint32_t base = A.off[c], size = A.off[c + 1] - base;
+            const size_t at = (size_t)row * (uint32_t)lists.cap;
+            const int32_t li = lane() < K ? lists.idx[at + lane()] : 0;
+            // an index outside the cloud (a list not made for these clouds) falls back to the search
+            if (!ballot(lane() < K && (li < 0 || (uint32_t)li >= size))) {
+              t.key = lane() < K ? pack_key(0.0f, base + (uint32_t)li) : ~0ull;
+              t.cnt = K;
+              listed = true;
+              pd2 = lists.d2 ? uniform(lists.d2[at + K - 1]) : -1.0f;
+            }
+          }
+        }
+        if (!listed) {
+          // consecutive queries of one cloud: the previous one's K-th distance bounds this one's
+          knn_search<false>(g, x, y, z, t, lds[w],
+                            (!STRIDED && c == pc) ? knn_bound_from(x, y, z, px, py, pz, pd2) : __builtin_inff(),
+                            h0);
+          pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
+        }
         pc = c;
         px = x;
         py = y;
         pz = z;
-        pd2 = t.cnt >= t.K ? kd2(t.kth) : -1.0f;
       } else {
         const float r = radius[c];
         const float r2 = __fmul_rn(r, r);
@@ -607,14 +638,10 @@ int rvp_normals_radius(const rvp_hash *h, float radius, int max_nn, float vpx, f
                        nullptr, nullptr, s);
 }
 
-int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
-                          size_t n_bound, int k, float vpx, float vpy, float vpz, float *nx, float *ny, float *nz,
-                          rvp_stream_t st) {
-  RVP_CHECK(x && y && z && offsets && nx && ny && nz, RVP_E_ARG, "null pointer");
-  RVP_CHECK(nclouds > 0, RVP_E_ARG, "nclouds must be positive");
-  RVP_CHECK(k >= 1 && k + 1 <= MAX_TOPK, RVP_E_UNSUPPORTED, "k must be in [1, 63]");
+static int batch_normals_knn_impl(const float *x, const float *y, const float *z, const uint32_t *offsets,
+                                  int nclouds, size_t n_bound, int k, const NbrLists &lists, float vpx, float vpy,
+                                  float vpz, float *nx, float *ny, float *nz, hipStream_t s) {
   RVP_TRY(pool_init());
-  hipStream_t s = S(st);
   SegGrid G;
   RVP_TRY(G.build(x, y, z, offsets, nclouds, n_bound, true, k + 1, nullptr, s));
   unsigned long long *fb = nullptr;
@@ -623,17 +650,56 @@ int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const 
   RVP_TRY(rec.alloc((size_t)REC_WORDS * n_bound * sizeof(float), s));
   unsigned qb = 1;
   bool strided = false;
-  RVP_TRY(batch_blocks<NB>(seg_normals_k<0, true>, n_bound, &qb, &strided));
   const SegArrays A = G.arrays();
   const int h0 = tune_knn_start_cells();
-  if (strided)
-    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, NB, rec.as<float>(),
-                                                                            n_bound, fb));
-  else
-    RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, NB, rec.as<float>(),
-                                                                             n_bound, fb));
+  if (lists.idx) {
+    RVP_TRY(batch_blocks<NB>(seg_normals_k<0, true, true>, n_bound, &qb, &strided));
+    if (strided)
+      RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true, true><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, NB,
+                                                                                  rec.as<float>(), n_bound, fb, lists));
+    else
+      RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false, true><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, NB,
+                                                                                   rec.as<float>(), n_bound, fb, lists));
+  } else {
+    RVP_TRY(batch_blocks<NB>(seg_normals_k<0, true>, n_bound, &qb, &strided));
+    if (strided)
+      RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, true><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, NB,
+                                                                              rec.as<float>(), n_bound, fb, lists));
+    else
+      RVP_LAUNCH("seg_normals_k", s, seg_normals_k<0, false><<<qb, QT, 0, s>>>(A, k + 1, nullptr, h0, NB,
+                                                                               rec.as<float>(), n_bound, fb, lists));
+  }
   return solve_normals(rec.as<float>(), n_bound, G.off + nclouds, A.xs, A.ys, A.zs, A.perm, vpx, vpy, vpz, nx, ny,
                        nz, fb, nullptr, nullptr, nullptr, s);
+}
+
+int rvp_batch_normals_knn(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
+                          size_t n_bound, int k, float vpx, float vpy, float vpz, float *nx, float *ny, float *nz,
+                          rvp_stream_t st) {
+  RVP_CHECK(x && y && z && offsets && nx && ny && nz, RVP_E_ARG, "null pointer");
+  RVP_CHECK(nclouds > 0, RVP_E_ARG, "nclouds must be positive");
+  RVP_CHECK(k >= 1 && k + 1 <= MAX_TOPK, RVP_E_UNSUPPORTED, "k must be in [1, 63]");
+  return batch_normals_knn_impl(x, y, z, offsets, nclouds, n_bound, k, NbrLists{}, vpx, vpy, vpz, nx, ny, nz, S(st));
+}
+
+int rvp_batch_normals_knn_lists(const float *x, const float *y, const float *z, const uint32_t *offsets,
+                                int nclouds, size_t n_bound, int k, const int32_t *list_idx, const float *list_d2,
+                                const int32_t *list_counts, int list_cap, float vpx, float vpy, float vpz, float *nx,
+                                float *ny, float *nz, rvp_stream_t st) {
+  RVP_CHECK(x && y && z && offsets && nx && ny && nz, RVP_E_ARG, "null pointer");
+  RVP_CHECK(list_idx && list_counts, RVP_E_ARG, "null radius list");
+  RVP_CHECK(nclouds > 0, RVP_E_ARG, "nclouds must be positive");
+  RVP_CHECK(k >= 1 && k + 1 <= MAX_TOPK, RVP_E_UNSUPPORTED, "k must be in [1, 63]");
+  RVP_CHECK(list_cap >= 1 && list_cap <= MAX_TOPK, RVP_E_ARG, "list_cap must be in [1, 64]");
+  NbrLists lists;
+  // rows shorter than k + 1 entries can serve no query: every query is searched
+  if (k + 1 <= list_cap) {
+    lists.idx = list_idx;
+    lists.d2 = list_d2;
+    lists.cnt = list_counts;
+    lists.cap = list_cap;
+  }
+  return batch_normals_knn_impl(x, y, z, offsets, nclouds, n_bound, k, lists, vpx, vpy, vpz, nx, ny, nz, S(st));
 }
 
 int rvp_batch_normals_radius(const float *x, const float *y, const float *z, const uint32_t *offsets, int nclouds,
@@ -656,7 +722,8 @@ int rvp_batch_normals_radius(const float *x, const float *y, const float *z, con
   const unsigned qb = ceil_div(ceil_div(n_bound, (size_t)nb), QW);
   const SegArrays A = G.arrays();
   RVP_LAUNCH("seg_normals_k", s, seg_normals_k<1, false><<<qb, QT, 0, s>>>(A, max_nn, radius, 1, nb,
-                                                                           rec.as<float>(), n_bound, fb));
+                                                                           rec.as<float>(), n_bound, fb,
+                                                                           NbrLists{}));
   return solve_normals(rec.as<float>(), n_bound, G.off + nclouds, A.xs, A.ys, A.zs, A.perm, vpx, vpy, vpz, nx, ny,
                        nz, fb, nullptr, nullptr, nullptr, s);
 }

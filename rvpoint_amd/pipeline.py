@@ -295,7 +295,7 @@ class ConcurrentFrames:
 class DeviceBatchPipeline:
     """Clouds back to back in HBM, one launch per stage and no host sync between stages: exact-kNN SOR (k_sor)
     -> radius lists (radius per cloud, capped at cap) of the kept points -> k-NN normals (k_nrm) of the kept
-    points.  The defaults are the 256-cloud workload's: k=32, radius 0.02 * the cloud's scale, cap 64, k=24."""
+    points, taken from the radius lists where a list holds the k_nrm + 1 nearest (the same normals as a search).  The defaults are the 256-cloud workload's: k=32, radius 0.02 * the cloud's scale, cap 64, k=24."""
 
     K_SOR, K_NRM, CAP = 32, 24, 64
 
@@ -326,7 +326,9 @@ class DeviceBatchPipeline:
         kept = (self.sx.ptr, self.sy.ptr, self.sz.ptr, self.sor_off.ptr, self.ncl, n)
         call("rvp_batch_radius_search", *kept, self.radius.ptr, None, self.CAP, self.idx.ptr, self.d2.ptr,
              self.cnt.ptr, s)
-        call("rvp_batch_normals_knn", *kept, self.K_NRM, 0.0, 0.0, 0.0, self.nx.ptr, self.ny.ptr, self.nz.ptr, s)
+        # the k-NN normals read each query's K_NRM + 1 nearest from its radius list where that holds enough entries
+        call("rvp_batch_normals_knn_lists", *kept, self.K_NRM, self.idx.ptr, self.d2.ptr, self.cnt.ptr, self.CAP,
+             0.0, 0.0, 0.0, self.nx.ptr, self.ny.ptr, self.nz.ptr, s)
 
     def kept_total(self):
         return int(self.sor_off.numpy(stream=self.s)[-1])

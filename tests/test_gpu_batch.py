@@ -165,6 +165,41 @@ def test_batch_normals_equal_single_cloud(clouds):
         assert np.median(err) <= 0.01
 
 
+def same_bits(a, b):
+    a, b = np.asarray(a, np.float32), np.asarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("rscale", [0.15, 0.3])
+def test_batch_normals_from_radius_lists_equal_search(clouds, rscale):
+    """k-NN normals whose k + 1 nearest are read from the capped radius lists equal the searched ones bit for bit.
+    At 0.15 x scale some queries hold fewer than 25 hits (searched) and others more; at 0.3 most exceed the cap."""
+    cl, scales = clouds
+    b = R.Batch(cl)
+    radius = np.array([rscale * s for s in scales], np.float32)
+    idx, d2, cnt = R.batch_radius_search(b, radius, 64)
+    served = int((cnt >= 25).sum())
+    assert 0 < served < b.n
+    ref = R.batch_normals_knn(b, 24)
+    assert same_bits(R.batch_normals_knn_lists(b, 24, idx, d2, cnt), ref)
+    assert same_bits(R.batch_normals_knn_lists(b, 24, idx, None, cnt), ref)
+
+
+def test_batch_normals_lists_with_foreign_indices_fall_back(clouds):
+    """Rows holding an index outside their cloud, and rows narrower than k + 1, are searched instead."""
+    cl, scales = clouds
+    b = R.Batch(cl)
+    radius = np.array([0.3 * s for s in scales], np.float32)
+    idx, d2, cnt = R.batch_radius_search(b, radius, 64)
+    ref = R.batch_normals_knn(b, 24)
+    rows = np.flatnonzero(cnt >= 25)[:200]
+    bad = idx.copy()
+    bad[rows[::2], 3] = -7
+    bad[rows[1::2], 24] = 1 << 30
+    assert same_bits(R.batch_normals_knn_lists(b, 24, bad, d2, cnt), ref)
+    assert same_bits(R.batch_normals_knn_lists(b, 24, idx[:, :16], None, np.minimum(cnt, 16)), ref)
+
+
 def test_batch_voxel_matches_oracle(clouds):
     cl, scales = clouds
     b = R.Batch(cl)
