@@ -239,11 +239,21 @@ def indoor_leg(R, args):
     def normals():
         call("rvp_normals_knn", *dev.ptrs(), n, 30, 0.0, 0.0, 0.0, nx.ptr, ny.ptr, nz.ptr, None)
 
+    # the same k=30 normals with each point's 31 nearest read from its radius list when it holds 31 entries
+    off = D.from_numpy(np.array([0, n], np.uint32))
+    lx, ly, lz = D(n), D(n), D(n)
+
+    def normals_lists():
+        call("rvp_batch_normals_knn_lists", *dev.ptrs(), off.ptr, 1, n, 30, idx.ptr, d2.ptr, cnt.ptr, cap, 0.0, 0.0,
+             0.0, lx.ptr, ly.ptr, lz.ptr, None)
+
     out = {"workload": "indoor scan 20M points (10x8x3 m room, 60 boxes, 20 cylinders, sigma 3 mm): hash build "
-                       "cell 0.02 -> radius lists r=0.02 cap 64 (idx + d2, every point) ; normals k=30",
+                       "cell 0.02 -> radius lists r=0.02 cap 64 (idx + d2, every point) ; normals k=30 searched, and read "
+                       "from the radius lists where a list holds the 31 nearest",
            "points": n}
     for name, fn, kern in (("hash_build", build, None), ("radius", radius, "radius_topk_k"),
-                           ("normals_k30", normals, "normals_topk_k")):
+                           ("normals_k30", normals, "normals_topk_k"),
+                           ("normals_k30_lists", normals_lists, "seg_normals_k")):
         fn()
         R.synchronize()
         if kern:
@@ -260,6 +270,9 @@ def indoor_leg(R, args):
             build()  # the hash the queries use
     m = cnt.numpy().astype(np.int64)
     mean_nn = float(m.mean())
+    out["normals_k30_lists_served"] = round(float((m >= 31).mean()), 4)
+    out["normals_k30_lists_same"] = all(np.array_equal(a.numpy().view(np.uint32), b.numpy().view(np.uint32))
+                                        for a, b in ((nx, lx), (ny, ly), (nz, lz)))
     bytes_per_launch = n * (12 + 12 + 4) + 8 * int(m.sum())
     kms = out["radius_kernel_ms"]
     achieved = bytes_per_launch / (kms * 1e-3) / 1e9
