@@ -61,6 +61,16 @@ def test_errors_without_device(built):
     assert L.rvp_batch_radius_search(p, p, p, p, 1, 8, None, None, 8, None, None, p, None) == _lib.RVP_E_ARG
     assert L.rvp_batch_normals_knn(p, p, p, p, 1, 8, 0, C.c_float(0), C.c_float(0), C.c_float(0), p, p, p,
                                    None) == _lib.RVP_E_UNSUPPORTED
+    # normals from radius lists: lists required, k and the row stride in range
+    f0 = C.c_float(0)
+    assert L.rvp_batch_normals_knn_lists(p, p, p, p, 1, 8, 24, None, None, p, 64, f0, f0, f0, p, p, p,
+                                         None) == _lib.RVP_E_ARG
+    assert L.rvp_batch_normals_knn_lists(p, p, p, p, 1, 8, 64, p, None, p, 64, f0, f0, f0, p, p, p,
+                                         None) == _lib.RVP_E_UNSUPPORTED
+    assert L.rvp_batch_normals_knn_lists(p, p, p, p, 1, 8, 24, p, None, p, 65, f0, f0, f0, p, p, p,
+                                         None) == _lib.RVP_E_ARG
+    assert L.rvp_batch_normals_knn_lists(p, p, p, p, 0, 8, 24, p, p, p, 64, f0, f0, f0, p, p, p,
+                                         None) == _lib.RVP_E_ARG
     # server: slots and sizes are checked first
     from rvpoint_amd.serve import FrameParams
     h = C.c_void_p()
